@@ -1,0 +1,119 @@
+"""ctypes binding of liblvm_gpu.so (the C ABI declared in include/lvm_gpu.h).
+
+The library is loaded from this package directory. There is no fallback: if
+the shared object is missing, or no GPU is visible when a compute entry point
+is called, the call raises instead of computing anything on the CPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+LIB_PATH = os.path.join(PKG, "liblvm_gpu.so")
+HEADER = os.path.join(ROOT, "include", "lvm_gpu.h")
+
+u64p = C.POINTER(C.c_uint64)
+u8p = C.POINTER(C.c_uint8)
+szp = C.POINTER(C.c_size_t)
+vpp = C.POINTER(C.c_void_p)
+
+LVM_OK = 0
+
+
+class LvmError(RuntimeError):
+    """A C ABI call returned a non-zero status (the caller's CPU-arm signal)."""
+
+    def __init__(self, name: str, code: int, msg: str):
+        super().__init__(f"{name} failed with status {code}: {msg}")
+        self.code = code
+
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "lvm_init": (C.c_int, [C.c_int]),
+    "lvm_device_count": (C.c_int, [C.POINTER(C.c_int)]),
+    "lvm_last_error": (C.c_char_p, []),
+    "lvm_sync": (C.c_int, []),
+    "lvm_inject_fault_after": (None, [C.c_long]),
+    "lvm_dev_alloc": (C.c_int, [vpp, C.c_size_t]),
+    "lvm_dev_free": (C.c_int, [C.c_void_p]),
+    "lvm_memcpy_htod": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "lvm_memcpy_dtoh": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "lvm_gl_op": (C.c_int, [C.c_int, u64p, u64p, u64p, C.c_size_t]),
+    "lvm_fp3_op": (C.c_int, [C.c_int, u64p, u64p, u64p, C.c_size_t]),
+    "lvm_ntt_forward": (C.c_int, [u64p, u64p, C.c_size_t, C.c_size_t]),
+    "lvm_ntt_inverse": (C.c_int, [u64p, u64p, C.c_size_t, C.c_size_t]),
+    "lvm_coset_lde_row_major_with_merkle_tree_keep": (
+        C.c_int, [u64p, C.c_size_t, C.c_size_t, C.c_size_t, u64p, u64p, u8p, vpp]),
+    "lvm_coset_lde_dev": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_void_p, u8p, vpp]),
+    "lvm_coset_lde_ext3_row_major_with_merkle_tree_keep": (
+        C.c_int, [u64p, C.c_size_t, C.c_size_t, C.c_size_t, u64p, u64p, u8p, vpp]),
+    "lvm_lde_release": (C.c_int, [C.c_void_p]),
+    "lvm_lde_root": (C.c_int, [C.c_void_p, u8p]),
+    "lvm_lde_shape": (C.c_int, [C.c_void_p, szp, szp, szp]),
+    "lvm_lde_download": (C.c_int, [C.c_void_p, u64p]),
+    "lvm_lde_device_ptr": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    "lvm_lde_tree_nodes": (C.c_int, [C.c_void_p, u8p, szp]),
+    "lvm_lde_open": (C.c_int, [C.c_void_p, u64p, C.c_size_t, u8p, u64p]),
+    "lvm_build_merkle_tree": (C.c_int, [u8p, C.c_size_t, u8p]),
+    "lvm_prof_enable": (C.c_int, [C.c_int]),
+    "lvm_prof_reset": (C.c_int, []),
+    "lvm_prof_query": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
+}
+
+
+def header_symbols() -> list[str]:
+    """Every function the public header declares."""
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lvm_[a-z0-9_]+)\s*\(", text)))
+
+
+_lib = None
+
+
+def lib():
+    """Load liblvm_gpu.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m lambda_vm_amd.build` "
+                "(there is no CPU fallback for the GPU path)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(name: str, rc: int) -> None:
+    if rc != LVM_OK:
+        msg = lib().lvm_last_error()
+        raise LvmError(name, rc, msg.decode(errors="replace") if msg else "")
+
+
+def call(name: str, *args) -> None:
+    check(name, getattr(lib(), name)(*args))
+
+
+_initialised = False
+
+
+def ensure_device(device: int = 0) -> None:
+    """Initialise the library on `device`; raises when no GPU is visible."""
+    global _initialised
+    if not _initialised:
+        call("lvm_init", device)
+        _initialised = True
+
+
+def device_count() -> int:
+    c = C.c_int(0)
+    lib().lvm_device_count(C.byref(c))
+    return c.value

@@ -1,0 +1,242 @@
+"""Host-side mirror of the reference GPU backend surface (crypto/math-cuda)
+over the C ABI of liblvm_gpu.so.
+
+Names, argument meaning and error behaviour follow the reference:
+  math_cuda::lde::coset_lde_row_major_with_merkle_tree_keep
+      (crypto/math-cuda/src/lde.rs:674)          -> coset_lde_row_major_with_merkle_tree_keep
+  math_cuda::lde::coset_lde_ext3_row_major_with_merkle_tree_keep
+      (crypto/math-cuda/src/lde.rs:855)          -> coset_lde_ext3_row_major_with_merkle_tree_keep
+  math_cuda::ntt::{forward, inverse} (ntt.rs:50, :57) -> ntt_forward, ntt_inverse
+  math_cuda::merkle::build_merkle_tree_on_device (merkle.rs:316) -> build_merkle_tree
+  math_cuda::merkle::gather_merkle_paths_dev (merkle.rs:358) -> GpuLde.open
+  math_cuda::{gl_add, gl_mul, ext3_mul, ...} (lib.rs:34-197) -> gl_op, fp3_op
+A failing call raises LvmError, the counterpart of the reference's
+`Err(DriverError)` that its `try_*` callers turn into `None` (CPU arm).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+
+OPS = {"add": 0, "sub": 1, "mul": 2, "neg": 3, "inv": 4, "sqr": 5}
+
+
+def _u64(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.uint64)
+
+
+def _p64(a: np.ndarray):
+    return a.ctypes.data_as(N.u64p)
+
+
+def _p8(a: np.ndarray):
+    return a.ctypes.data_as(N.u8p)
+
+
+def gl_op(op: str, a, b=None) -> np.ndarray:
+    N.ensure_device()
+    a = _u64(a)
+    bb = _u64(b) if b is not None else None
+    out = np.empty_like(a)
+    N.call("lvm_gl_op", OPS[op], _p64(a), _p64(bb) if bb is not None else None, _p64(out), a.size)
+    return out
+
+
+def fp3_op(op: str, a, b=None) -> np.ndarray:
+    N.ensure_device()
+    a = _u64(a).reshape(-1, 3)
+    bb = _u64(b).reshape(-1, 3) if b is not None else None
+    out = np.empty_like(a)
+    N.call("lvm_fp3_op", OPS[op], _p64(a), _p64(bb) if bb is not None else None, _p64(out), a.shape[0])
+    return out
+
+
+def ntt_forward(x) -> np.ndarray:
+    """Evaluations of each column at w_n^i (natural order); x: (n,) or (n, m)."""
+    N.ensure_device()
+    a = _u64(x)
+    shape = a.shape
+    a2 = a.reshape(shape[0], -1)
+    out = np.empty_like(a2)
+    N.call("lvm_ntt_forward", _p64(a2), _p64(out), a2.shape[0], a2.shape[1])
+    return out.reshape(shape)
+
+
+def ntt_inverse(x) -> np.ndarray:
+    """Unnormalised inverse (no 1/n), natural order in and out."""
+    N.ensure_device()
+    a = _u64(x)
+    shape = a.shape
+    a2 = a.reshape(shape[0], -1)
+    out = np.empty_like(a2)
+    N.call("lvm_ntt_inverse", _p64(a2), _p64(out), a2.shape[0], a2.shape[1])
+    return out.reshape(shape)
+
+
+class GpuLde:
+    """Device-resident LDE table + its Keccak Merkle tree (GpuLdeBase/GpuLdeExt3
+    + GpuMerkleTree, crypto/math-cuda/src/lde.rs:923-988)."""
+
+    def __init__(self, handle: int):
+        self.h = C.c_void_p(handle)
+        rows, w, n = C.c_size_t(), C.c_size_t(), C.c_size_t()
+        N.call("lvm_lde_shape", self.h, C.byref(rows), C.byref(w), C.byref(n))
+        self.lde_rows, self.words_per_row, self.trace_rows = rows.value, w.value, n.value
+        r = np.zeros(32, dtype=np.uint8)
+        N.call("lvm_lde_root", self.h, _p8(r))
+        self.root = r.tobytes()
+
+    def close(self) -> None:
+        if self.h is not None and self.h.value:
+            N.lib().lvm_lde_release(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def download(self) -> np.ndarray:
+        """Natural-order row-major host copy, (lde_rows, words_per_row)."""
+        out = np.empty((self.lde_rows, self.words_per_row), dtype=np.uint64)
+        N.call("lvm_lde_download", self.h, _p64(out))
+        return out
+
+    def device_ptr(self) -> int:
+        p = C.c_void_p()
+        N.call("lvm_lde_device_ptr", self.h, C.byref(p))
+        return p.value
+
+    def nodes(self) -> np.ndarray:
+        cnt = C.c_size_t()
+        N.call("lvm_lde_tree_nodes", self.h, None, C.byref(cnt))
+        out = np.zeros((cnt.value, 32), dtype=np.uint8)
+        N.call("lvm_lde_tree_nodes", self.h, _p8(out), C.byref(cnt))
+        return out
+
+    def open(self, positions):
+        """Auth paths (count, depth, 32) and the two opened rows per position
+        (count, 2, words_per_row): rows br(2p), br(2p+1) of the natural LDE."""
+        pos = _u64(positions)
+        L = self.lde_rows // 2
+        depth = L.bit_length() - 1
+        paths = np.zeros((pos.size, max(depth, 1), 32), dtype=np.uint8)
+        rows = np.zeros((pos.size, 2, self.words_per_row), dtype=np.uint64)
+        N.call("lvm_lde_open", self.h, _p64(pos), pos.size, _p8(paths), _p64(rows))
+        return paths[:, :depth], rows
+
+
+def coset_lde_row_major_with_merkle_tree_keep(row_major, n: int, m: int, blowup_factor: int, weights,
+                                              retain_host_lde: bool = True):
+    """Row-major n x m trace -> (GpuLde handle, natural-order row-major LDE or None)."""
+    N.ensure_device()
+    tr = _u64(row_major).reshape(-1)
+    if tr.size != n * m:
+        raise ValueError("row_major must hold n*m elements")
+    w = _u64(weights)
+    if w.size != n:
+        raise ValueError("weights length must match n")
+    root = np.zeros(32, dtype=np.uint8)
+    host = np.empty((n * blowup_factor, m), dtype=np.uint64) if retain_host_lde else None
+    h = C.c_void_p()
+    N.call("lvm_coset_lde_row_major_with_merkle_tree_keep", _p64(tr), n, m, blowup_factor, _p64(w),
+           _p64(host) if host is not None else None, _p8(root), C.byref(h))
+    return GpuLde(h.value), host
+
+
+def coset_lde_ext3_row_major_with_merkle_tree_keep(row_major, n: int, m: int, blowup_factor: int, weights,
+                                                   retain_host_lde: bool = True):
+    """Row-major n x m Fp3 trace (3m u64 per row) -> (GpuLde, host LDE (N, 3m) or None)."""
+    N.ensure_device()
+    tr = _u64(row_major).reshape(-1)
+    if tr.size != n * m * 3:
+        raise ValueError("row_major must hold n*m Fp3 elements")
+    w = _u64(weights)
+    root = np.zeros(32, dtype=np.uint8)
+    host = np.empty((n * blowup_factor, 3 * m), dtype=np.uint64) if retain_host_lde else None
+    h = C.c_void_p()
+    N.call("lvm_coset_lde_ext3_row_major_with_merkle_tree_keep", _p64(tr), n, m, blowup_factor, _p64(w),
+           _p64(host) if host is not None else None, _p8(root), C.byref(h))
+    return GpuLde(h.value), host
+
+
+class DeviceBuffer:
+    """Raw device allocation for resident pipelines (the reference's `predev`)."""
+
+    def __init__(self, nbytes: int):
+        N.ensure_device()
+        p = C.c_void_p()
+        N.call("lvm_dev_alloc", C.byref(p), nbytes)
+        self.ptr = p.value
+        self.nbytes = nbytes
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        N.call("lvm_memcpy_htod", C.c_void_p(b.ptr), a.ctypes.data_as(C.c_void_p), a.nbytes)
+        return b
+
+    def to_array(self, dtype, shape) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        N.call("lvm_memcpy_dtoh", out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), out.nbytes)
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            N.lib().lvm_dev_free(C.c_void_p(self.ptr))
+            self.ptr = 0
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def coset_lde_dev(trace: DeviceBuffer, n: int, m: int, blowup_factor: int, weights: DeviceBuffer) -> GpuLde:
+    """Device-resident row-major trace + weights -> GpuLde (no host copies)."""
+    N.ensure_device()
+    root = np.zeros(32, dtype=np.uint8)
+    h = C.c_void_p()
+    N.call("lvm_coset_lde_dev", C.c_void_p(trace.ptr), n, m, blowup_factor, C.c_void_p(weights.ptr), _p8(root),
+           C.byref(h))
+    return GpuLde(h.value)
+
+
+def build_merkle_tree(leaves) -> np.ndarray:
+    """(num_leaves, 32) digests -> (2L-1, 32) nodes, root first."""
+    N.ensure_device()
+    lv = np.ascontiguousarray(leaves, dtype=np.uint8).reshape(-1, 32)
+    L = 1
+    while L < lv.shape[0]:
+        L <<= 1
+    out = np.zeros((2 * L - 1, 32), dtype=np.uint8)
+    N.call("lvm_build_merkle_tree", _p8(lv), lv.shape[0], _p8(out))
+    return out
+
+
+def inject_fault_after(calls: int) -> None:
+    N.lib().lvm_inject_fault_after(calls)
+
+
+def prof_enable(on: bool = True) -> None:
+    N.call("lvm_prof_enable", int(on))
+
+
+def prof_reset() -> None:
+    N.call("lvm_prof_reset")
+
+
+def prof_query(name: str | None = None):
+    cnt, ms = C.c_uint64(), C.c_double()
+    N.call("lvm_prof_query", name.encode() if name else None, C.byref(cnt), C.byref(ms))
+    return cnt.value, ms.value
+
+
+def sync() -> None:
+    N.call("lvm_sync")

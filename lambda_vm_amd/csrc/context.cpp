@@ -1,0 +1,163 @@
+// Library state: per-device stream and twiddle tables, thread-local error
+// strings, test-only fault injection, and the kernel event profiler.
+#include <atomic>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "lvm_internal.hpp"
+#include "prof.hpp"
+
+namespace lvm {
+
+static thread_local std::string g_last_error;
+static std::atomic<long> g_fault_countdown{-1};
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+const char *last_error() { return g_last_error.c_str(); }
+
+int fail(int code, const std::string &msg) {
+    set_error(msg);
+    return code;
+}
+
+bool take_injected_fault() {
+    long v = g_fault_countdown.load();
+    while (v >= 0) {
+        if (g_fault_countdown.compare_exchange_weak(v, v - 1)) return v == 0;
+    }
+    return false;
+}
+
+void inject_fault_after(long calls) { g_fault_countdown.store(calls); }
+
+static constexpr uint64_t ROOT_2_32 = 1753635133440165772ULL;  // goldilocks.rs:540
+
+static std::mutex g_dev_mu;
+static std::vector<Device *> g_devices;
+
+static int build_tables(Device *d) {
+    const size_t T = 1u << 16;
+    std::vector<uint64_t> lo(T), hi(T), ilo(T), ihi(T), r(128), ir(128);
+    const uint64_t w = ROOT_2_32, wi = inv(ROOT_2_32);
+    uint64_t w16 = pow(w, T), wi16 = pow(wi, T);
+    uint64_t a = 1, b = 1, c = 1, e = 1;
+    for (size_t x = 0; x < T; x++) {
+        lo[x] = a; a = mul(a, w);
+        hi[x] = b; b = mul(b, w16);
+        ilo[x] = c; c = mul(c, wi);
+        ihi[x] = e; e = mul(e, wi16);
+    }
+    uint64_t w256 = pow(w, 1ull << 24), wi256 = pow(wi, 1ull << 24);
+    uint64_t p = 1, q = 1;
+    for (int k = 0; k < 128; k++) {
+        r[k] = p; p = mul(p, w256);
+        ir[k] = q; q = mul(q, wi256);
+    }
+    uint64_t *buf = nullptr;
+    const size_t total = 4 * T + 256;
+    LVM_HIP(hipMalloc(&buf, total * sizeof(uint64_t)));
+    d->tb.fwd_lo = buf;
+    d->tb.fwd_hi = buf + T;
+    d->tb.inv_lo = buf + 2 * T;
+    d->tb.inv_hi = buf + 3 * T;
+    d->tb.fwd_r = buf + 4 * T;
+    d->tb.inv_r = buf + 4 * T + 128;
+    LVM_HIP(hipMemcpy(d->tb.fwd_lo, lo.data(), T * 8, hipMemcpyHostToDevice));
+    LVM_HIP(hipMemcpy(d->tb.fwd_hi, hi.data(), T * 8, hipMemcpyHostToDevice));
+    LVM_HIP(hipMemcpy(d->tb.inv_lo, ilo.data(), T * 8, hipMemcpyHostToDevice));
+    LVM_HIP(hipMemcpy(d->tb.inv_hi, ihi.data(), T * 8, hipMemcpyHostToDevice));
+    LVM_HIP(hipMemcpy(d->tb.fwd_r, r.data(), 128 * 8, hipMemcpyHostToDevice));
+    LVM_HIP(hipMemcpy(d->tb.inv_r, ir.data(), 128 * 8, hipMemcpyHostToDevice));
+    return LVM_OK;
+}
+
+Device *device_state() {
+    int id = 0;
+    if (hipGetDevice(&id) != hipSuccess) {
+        set_error("hipGetDevice failed (no GPU visible?)");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    if ((int)g_devices.size() <= id) g_devices.resize(id + 1, nullptr);
+    if (g_devices[id]) return g_devices[id];
+    Device *d = new Device();
+    d->id = id;
+    if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) {
+        set_error("hipStreamCreateWithFlags failed");
+        delete d;
+        return nullptr;
+    }
+    if (build_tables(d) != LVM_OK) {
+        hipStreamDestroy(d->stream);
+        delete d;
+        return nullptr;
+    }
+    g_devices[id] = d;
+    return d;
+}
+
+// ---------------- profiler ----------------
+struct ProfRec {
+    const char *name;
+    hipEvent_t a, b;
+};
+static std::mutex g_prof_mu;
+static bool g_prof_on = false;
+static std::vector<ProfRec> g_prof;
+
+bool prof_enabled() { return g_prof_on; }
+
+void prof_set(bool on) {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof_on = on;
+}
+
+void *prof_begin(const char *name, hipStream_t st) {
+    if (!g_prof_on) return nullptr;
+    ProfRec *r = new ProfRec{name, nullptr, nullptr};
+    hipEventCreate(&r->a);
+    hipEventCreate(&r->b);
+    hipEventRecord(r->a, st);
+    return r;
+}
+
+void prof_end(void *h, hipStream_t st) {
+    if (!h) return;
+    ProfRec *r = (ProfRec *)h;
+    hipEventRecord(r->b, st);
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    g_prof.push_back(*r);
+    delete r;
+}
+
+// Sum of event-measured durations for records whose name matches; clears
+// nothing. Synchronises on the recorded events.
+int prof_query(const char *name, uint64_t *count, double *total_ms) {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    uint64_t c = 0;
+    double t = 0;
+    for (auto &r : g_prof) {
+        if (name && std::string(name) != r.name) continue;
+        float ms = 0;
+        if (hipEventSynchronize(r.b) != hipSuccess) return LVM_ERR_DEVICE;
+        if (hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) return LVM_ERR_DEVICE;
+        c++;
+        t += ms;
+    }
+    *count = c;
+    *total_ms = t;
+    return LVM_OK;
+}
+
+void prof_reset() {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    for (auto &r : g_prof) {
+        hipEventSynchronize(r.b);
+        hipEventDestroy(r.a);
+        hipEventDestroy(r.b);
+    }
+    g_prof.clear();
+}
+
+}  // namespace lvm

@@ -1,0 +1,198 @@
+// Goldilocks (p = 2^64 - 2^32 + 1) and Fp3 = Fp[w]/(w^3 - 2) arithmetic for
+// gfx950 device code.
+//
+// Semantics follow the reference CPU field (crypto/math/src/field/goldilocks.rs
+// :79-121 add/sub/mul, :197-216 reduce128, :506-512 canonical;
+// extensions_goldilocks.rs:305-388 Fp3): values are u64 limbs in [0, 2^64),
+// not necessarily canonical, and every operation returns the same residue
+// class as the reference. Results are compared canonically.
+//
+// The 64x64->128 product is built from four 32x32->64 partial products
+// (v_mad_u64_u32 on CDNA) and reduced with 2^64 = 2^32 - 1, 2^96 = -1.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define LVM_HD __host__ __device__ __forceinline__
+
+namespace lvm {
+
+constexpr uint64_t P = 0xFFFFFFFF00000001ULL;
+constexpr uint64_t EPS = 0xFFFFFFFFULL;
+
+LVM_HD uint64_t canon(uint64_t a) { return a >= P ? a - P : a; }
+
+LVM_HD uint64_t add(uint64_t a, uint64_t b) {
+    uint64_t s = a + b;
+    uint64_t s2 = s + (s < a ? EPS : 0);
+    return s2 + (s2 < s ? EPS : 0);
+}
+
+LVM_HD uint64_t sub(uint64_t a, uint64_t b) {
+    uint64_t d = a - b;
+    uint64_t corr = a < b ? EPS : 0;
+    uint64_t d2 = d - corr;
+    return d2 - (d < corr ? EPS : 0);
+}
+
+// x + y with x + y < 2^64 + p (goldilocks.rs:226-248)
+LVM_HD uint64_t add_nc(uint64_t x, uint64_t y) {
+    uint64_t r = x + y;
+    return r + (r < x ? EPS : 0);
+}
+
+LVM_HD uint64_t reduce128(uint64_t lo, uint64_t hi) {
+    uint64_t hi_hi = hi >> 32, hi_lo = hi & EPS;
+    uint64_t t0 = lo - hi_hi;
+    t0 -= (lo < hi_hi) ? EPS : 0;
+    uint64_t t1 = (hi_lo << 32) - hi_lo;
+    return add_nc(t0, t1);
+}
+
+LVM_HD void mul_wide(uint64_t a, uint64_t b, uint64_t &lo, uint64_t &hi) {
+    uint32_t a0 = (uint32_t)a, a1 = (uint32_t)(a >> 32);
+    uint32_t b0 = (uint32_t)b, b1 = (uint32_t)(b >> 32);
+    uint64_t p00 = (uint64_t)a0 * b0;
+    uint64_t mid = (uint64_t)a0 * b1 + (p00 >> 32);
+    uint64_t mid2 = (uint64_t)a1 * b0 + (uint32_t)mid;
+    lo = (mid2 << 32) | (uint32_t)p00;
+    hi = (uint64_t)a1 * b1 + (mid >> 32) + (mid2 >> 32);
+}
+
+LVM_HD uint64_t mul(uint64_t a, uint64_t b) {
+    uint64_t lo, hi;
+    mul_wide(a, b, lo, hi);
+    return reduce128(lo, hi);
+}
+
+LVM_HD uint64_t sqr(uint64_t a) { return mul(a, a); }
+LVM_HD uint64_t dbl(uint64_t a) { return add(a, a); }
+
+LVM_HD uint64_t neg(uint64_t a) {
+    uint64_t c = canon(a);
+    return c == 0 ? 0 : P - c;
+}
+
+LVM_HD uint64_t pow(uint64_t a, uint64_t e) {
+    uint64_t r = 1;
+    while (e) {
+        if (e & 1) r = mul(r, a);
+        a = sqr(a);
+        e >>= 1;
+    }
+    return r;
+}
+
+// a^(p-2), the addition chain of goldilocks.rs:335-393
+LVM_HD uint64_t exp_acc(uint64_t base, uint64_t tail, int n) {
+    for (int i = 0; i < n; i++) base = sqr(base);
+    return mul(base, tail);
+}
+
+LVM_HD uint64_t inv(uint64_t a) {
+    uint64_t x = canon(a);
+    uint64_t x2 = sqr(x);
+    uint64_t x3 = mul(x2, x);
+    uint64_t x7 = exp_acc(x3, x, 1);
+    uint64_t x63 = exp_acc(x7, x7, 3);
+    uint64_t x12 = exp_acc(x63, x63, 6);
+    uint64_t x24 = exp_acc(x12, x12, 12);
+    uint64_t x30 = exp_acc(x24, x63, 6);
+    uint64_t x31 = exp_acc(x30, x, 1);
+    uint64_t x32 = exp_acc(x31, x, 1);
+    uint64_t t = x31;
+    for (int i = 0; i < 33; i++) t = sqr(t);
+    return mul(t, x32);
+}
+
+// ---------------- Fp3 ----------------
+struct fp3 {
+    uint64_t c0, c1, c2;
+};
+
+LVM_HD fp3 f3(uint64_t a, uint64_t b, uint64_t c) { return fp3{a, b, c}; }
+LVM_HD fp3 f3_add(fp3 a, fp3 b) { return fp3{add(a.c0, b.c0), add(a.c1, b.c1), add(a.c2, b.c2)}; }
+LVM_HD fp3 f3_sub(fp3 a, fp3 b) { return fp3{sub(a.c0, b.c0), sub(a.c1, b.c1), sub(a.c2, b.c2)}; }
+LVM_HD fp3 f3_neg(fp3 a) { return fp3{neg(a.c0), neg(a.c1), neg(a.c2)}; }
+LVM_HD fp3 f3_mul_base(fp3 a, uint64_t b) { return fp3{mul(b, a.c0), mul(b, a.c1), mul(b, a.c2)}; }
+LVM_HD fp3 f3_canon(fp3 a) { return fp3{canon(a.c0), canon(a.c1), canon(a.c2)}; }
+
+// sum of three 128-bit products reduced once, with 2^128 = EPS^2 for each
+// carry out of 128 bits (goldilocks.rs:289-315)
+LVM_HD uint64_t dot3(uint64_t a0, uint64_t b0, uint64_t a1, uint64_t b1, uint64_t a2, uint64_t b2) {
+    uint64_t l0, h0, l1, h1, l2, h2;
+    mul_wide(a0, b0, l0, h0);
+    mul_wide(a1, b1, l1, h1);
+    mul_wide(a2, b2, l2, h2);
+    uint64_t lo = l0 + l1;
+    uint64_t c = lo < l0;
+    uint64_t hi = h0 + h1;
+    uint32_t over = hi < h0;
+    uint64_t hi2 = hi + c;
+    over += hi2 < hi;
+    uint64_t lo2 = lo + l2;
+    c = lo2 < lo;
+    uint64_t hi3 = hi2 + h2;
+    over += hi3 < hi2;
+    uint64_t hi4 = hi3 + c;
+    over += hi4 < hi3;
+    uint64_t r = reduce128(lo2, hi4);
+    constexpr uint64_t EPS_SQ = EPS * EPS;
+    if (over) r = add_nc(r, EPS_SQ);
+    if (over > 1) r = add_nc(r, EPS_SQ);
+    return r;
+}
+
+LVM_HD fp3 f3_mul(fp3 a, fp3 b) {
+    uint64_t b1_2 = dbl(b.c1), b2_2 = dbl(b.c2);
+    return fp3{dot3(a.c0, b.c0, a.c1, b2_2, a.c2, b1_2), dot3(a.c0, b.c1, a.c1, b.c0, a.c2, b2_2),
+               dot3(a.c0, b.c2, a.c1, b.c1, a.c2, b.c0)};
+}
+
+LVM_HD fp3 f3_sqr(fp3 a) { return f3_mul(a, a); }
+
+LVM_HD fp3 f3_inv(fp3 a) {
+    uint64_t a0s = sqr(a.c0), a1s = sqr(a.c1), a2s = sqr(a.c2);
+    uint64_t a0c = mul(a0s, a.c0), a1c = mul(a1s, a.c1), a2c = mul(a2s, a.c2);
+    uint64_t a012 = mul(mul(a.c0, a.c1), a.c2);
+    uint64_t norm = add(add(a0c, dbl(a1c)), dbl(dbl(a2c)));
+    norm = sub(norm, dbl(add(dbl(a012), a012)));
+    uint64_t ni = inv(norm);
+    uint64_t a1a2 = mul(a.c1, a.c2), a0a1 = mul(a.c0, a.c1), a0a2 = mul(a.c0, a.c2);
+    return fp3{mul(sub(a0s, dbl(a1a2)), ni), mul(sub(dbl(a2s), a0a1), ni), mul(sub(a1s, a0a2), ni)};
+}
+
+LVM_HD fp3 f3_pow(fp3 a, uint64_t e) {
+    fp3 r = fp3{1, 0, 0};
+    while (e) {
+        if (e & 1) r = f3_mul(r, a);
+        a = f3_sqr(a);
+        e >>= 1;
+    }
+    return r;
+}
+
+LVM_HD uint64_t bswap64(uint64_t v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+    return ((uint64_t)__builtin_bswap32(lo) << 32) | __builtin_bswap32(hi);
+#else
+    return __builtin_bswap64(v);
+#endif
+}
+
+LVM_HD uint32_t brev32(uint32_t x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_bitreverse32(x);
+#else
+    x = ((x >> 1) & 0x55555555u) | ((x & 0x55555555u) << 1);
+    x = ((x >> 2) & 0x33333333u) | ((x & 0x33333333u) << 2);
+    x = ((x >> 4) & 0x0F0F0F0Fu) | ((x & 0x0F0F0F0Fu) << 4);
+    return __builtin_bswap32(x);
+#endif
+}
+
+// reverse the low `bits` bits of i (bits <= 32)
+LVM_HD uint32_t bitrev(uint32_t i, unsigned bits) { return bits ? brev32(i) >> (32 - bits) : 0u; }
+
+}  // namespace lvm

@@ -1,0 +1,65 @@
+// Internal declarations shared by the HIP translation units of liblvm_gpu.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <string>
+
+#include "field.hpp"
+#include "lvm_gpu.h"
+
+namespace lvm {
+
+// Device twiddle tables (built once per device by the library).
+//  *_lo[x] = w^x, *_hi[x] = w^(x * 2^16) for w the primitive 2^32-th root of
+//  unity of crypto/math/src/field/goldilocks.rs:540 (fwd) or its inverse (inv);
+//  *_r[e] = w_256^e (e < 128).
+struct Tables {
+    uint64_t *fwd_lo, *fwd_hi, *inv_lo, *inv_hi, *fwd_r, *inv_r;
+};
+
+// status codes of the C ABI (LVM_OK, LVM_ERR_*) come from lvm_gpu.h
+
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+// per-device library state
+struct Device {
+    int id = -1;
+    hipStream_t stream = nullptr;
+    Tables tb{};
+};
+Device *device_state();  // current device's state, initialised on first use; null on failure
+
+// fault injection for the fallback tests (never set in production)
+bool take_injected_fault();
+
+#define LVM_HIP(expr)                                                                                   \
+    do {                                                                                                \
+        hipError_t _e = (expr);                                                                         \
+        if (_e != hipSuccess) return ::lvm::fail(LVM_ERR_DEVICE, std::string(#expr) + ": " +    \
+                                                   hipGetErrorString(_e));                              \
+    } while (0)
+
+// ---- NTT (ntt.hip) ----
+void ntt_inverse_rows(const uint64_t *in, uint32_t in_pitch, uint64_t *out, uint32_t out_pitch, uint32_t m,
+                      unsigned log_n, const Tables &tb, hipStream_t st);
+void ntt_forward_rows(const uint64_t *in, uint32_t in_pitch, uint64_t *out, uint32_t out_pitch, uint32_t m,
+                      unsigned log_n, const Tables &tb, hipStream_t st);
+void ntt_coset_expand(const uint64_t *coef, uint32_t coef_pitch, uint64_t *lde, uint32_t lde_pitch, uint32_t m,
+                      unsigned log_n, unsigned log_blowup, const uint64_t *weights, const Tables &tb,
+                      hipStream_t st);
+
+// ---- Keccak / Merkle (merkle.hip) ----
+// Row-pair leaves over words [w0, w1) of each row of a row-major table whose
+// rows are already stored in bit-reversed order: leaf i hashes rows 2i, 2i+1.
+void keccak_row_pair_leaves(const uint64_t *rows, uint64_t pitch, uint64_t w0, uint64_t w1, uint64_t num_leaves,
+                            uint8_t *leaves_out, hipStream_t st);
+// inner nodes of a (2L-1)*32-byte tree whose leaves sit at [L-1, 2L-1)
+void merkle_build_inner(uint8_t *nodes, uint64_t L, hipStream_t st);
+// Keccak(BE24(e[2j]) || BE24(e[2j+1])) over an Fp3 array of `len` elements
+void keccak_fp3_pair_leaves(const uint64_t *evals, uint64_t len, uint8_t *leaves_out, hipStream_t st);
+
+size_t padded_leaves(size_t n);
+
+}  // namespace lvm

@@ -1,0 +1,174 @@
+// Keccak-256 Merkle commitment kernels for gfx950.
+//
+// Leaf layout (crypto/stark/src/prover.rs:895-961, commitment.rs:42, :81-91):
+// leaf i = Keccak( BE(canon(row[br(2i)])) || BE(canon(row[br(2i+1)])) ) over
+// the committed columns in order; an Fp3 element streams components 0,1,2
+// (extensions_goldilocks.rs:497-503). The LDE tables here are stored with
+// their rows already in bit-reversed order, so leaf i reads physical rows
+// 2i and 2i+1. A big-endian 8-byte element absorbed into a little-endian
+// Keccak lane is bswap64(canonical value); 136-byte blocks are 17 lanes.
+//
+// Tree layout (crypto/crypto/src/merkle_tree/merkle.rs:201-224, utils.rs:
+// 68-96): nodes[0..L-1) inner with the root at 0, leaves at [L-1, 2L-1),
+// parent(i) = Keccak(node[2i+1] || node[2i+2]).
+#include "keccak.hpp"
+#include "lvm_internal.hpp"
+
+namespace lvm {
+
+// One thread per leaf. Words [w0, w1) of rows 2i and 2i+1 (row stride
+// `pitch` words) are absorbed in order, 17 lanes per permutation.
+__global__ __launch_bounds__(256) void keccak_row_pair_leaves_kernel(const uint64_t *__restrict__ rows,
+                                                                    uint64_t pitch, uint64_t w0, uint64_t w1,
+                                                                    uint64_t num_leaves,
+                                                                    uint64_t *__restrict__ leaves) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= num_leaves) return;
+    const uint64_t width = w1 - w0;
+    const uint64_t total = 2 * width;
+    const uint64_t *r0 = rows + (2 * i) * pitch + w0;
+    const uint64_t *r1 = rows + (2 * i + 1) * pitch + w0;
+    uint64_t s[25];
+#pragma unroll
+    for (int k = 0; k < 25; k++) s[k] = 0;
+    // (row, word) cursor advanced lane by lane
+    const uint64_t *cur = r0;
+    uint64_t left_in_row = width;
+    bool second = false;
+    uint64_t e = 0;
+    while (e + 17 <= total) {
+#pragma unroll
+        for (int j = 0; j < 17; j++) {
+            s[j] ^= bswap64(canon(*cur));
+            cur++;
+            if (--left_in_row == 0 && !second) {
+                second = true;
+                cur = r1;
+                left_in_row = width;
+            }
+        }
+        keccak_f1600(s);
+        e += 17;
+    }
+    const uint64_t rem = total - e;
+#pragma unroll
+    for (int j = 0; j < 17; j++) {
+        if ((uint64_t)j < rem) {
+            s[j] ^= bswap64(canon(*cur));
+            cur++;
+            if (--left_in_row == 0 && !second) {
+                second = true;
+                cur = r1;
+                left_in_row = width;
+            }
+        } else if ((uint64_t)j == rem) {
+            s[j] ^= 0x01;
+        }
+    }
+    s[16] ^= 0x8000000000000000ULL;
+    keccak_f1600(s);
+    uint64_t *o = leaves + 4 * i;
+    o[0] = s[0];
+    o[1] = s[1];
+    o[2] = s[2];
+    o[3] = s[3];
+}
+
+// FRI layer leaves: Keccak(BE24(e[2j]) || BE24(e[2j+1])), 6 lanes, one permutation
+// (crypto/crypto/src/merkle_tree/backends/field_element_vector.rs:129-134).
+__global__ __launch_bounds__(256) void keccak_fp3_pair_leaves_kernel(const uint64_t *__restrict__ evals,
+                                                                    uint64_t num_leaves,
+                                                                    uint64_t *__restrict__ leaves) {
+    const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= num_leaves) return;
+    uint64_t s[25];
+#pragma unroll
+    for (int k = 0; k < 25; k++) s[k] = 0;
+    const uint64_t *p = evals + 6 * j;
+#pragma unroll
+    for (int k = 0; k < 6; k++) s[k] = bswap64(canon(p[k]));
+    s[6] = 0x01;
+    s[16] = 0x8000000000000000ULL;
+    keccak_f1600(s);
+    uint64_t *o = leaves + 4 * j;
+    o[0] = s[0];
+    o[1] = s[1];
+    o[2] = s[2];
+    o[3] = s[3];
+}
+
+// One level of parents [begin, begin+count).
+__global__ __launch_bounds__(256) void merkle_level_kernel(uint64_t *nodes, uint64_t begin, uint64_t count) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const uint64_t p = begin + k;
+    const uint64_t *l = nodes + 4 * (2 * p + 1);
+    const uint64_t *r = nodes + 4 * (2 * p + 2);
+    uint64_t lv[4] = {l[0], l[1], l[2], l[3]}, rv[4] = {r[0], r[1], r[2], r[3]}, o[4];
+    keccak_pair_node(lv, rv, o);
+    uint64_t *d = nodes + 4 * p;
+    d[0] = o[0];
+    d[1] = o[1];
+    d[2] = o[2];
+    d[3] = o[3];
+}
+
+// All levels whose parent range starts below `top_begin + 1` in one block:
+// level [b/2, b) is built from [b, 2b] for b = top_begin, top_begin/2, ... 1.
+__global__ __launch_bounds__(1024) void merkle_tail_kernel(uint64_t *nodes, uint64_t level_begin) {
+    while (level_begin > 0) {
+        const uint64_t nb = level_begin / 2;
+        const uint64_t count = level_begin - nb;
+        for (uint64_t k = threadIdx.x; k < count; k += blockDim.x) {
+            const uint64_t p = nb + k;
+            const uint64_t *l = nodes + 4 * (2 * p + 1);
+            const uint64_t *r = nodes + 4 * (2 * p + 2);
+            uint64_t lv[4] = {l[0], l[1], l[2], l[3]}, rv[4] = {r[0], r[1], r[2], r[3]}, o[4];
+            keccak_pair_node(lv, rv, o);
+            uint64_t *d = nodes + 4 * p;
+            d[0] = o[0];
+            d[1] = o[1];
+            d[2] = o[2];
+            d[3] = o[3];
+        }
+        __syncthreads();
+        level_begin = nb;
+    }
+}
+
+size_t padded_leaves(size_t n) {
+    size_t p = 1;
+    while (p < n) p <<= 1;
+    return p;
+}
+
+void keccak_row_pair_leaves(const uint64_t *rows, uint64_t pitch, uint64_t w0, uint64_t w1, uint64_t num_leaves,
+                            uint8_t *leaves_out, hipStream_t st) {
+    if (num_leaves == 0) return;
+    dim3 grid((unsigned)((num_leaves + 255) / 256));
+    hipLaunchKernelGGL(keccak_row_pair_leaves_kernel, grid, dim3(256), 0, st, rows, pitch, w0, w1, num_leaves,
+                       (uint64_t *)leaves_out);
+}
+
+void keccak_fp3_pair_leaves(const uint64_t *evals, uint64_t len, uint8_t *leaves_out, hipStream_t st) {
+    uint64_t nl = len / 2;
+    if (nl == 0) return;
+    dim3 grid((unsigned)((nl + 255) / 256));
+    hipLaunchKernelGGL(keccak_fp3_pair_leaves_kernel, grid, dim3(256), 0, st, evals, nl, (uint64_t *)leaves_out);
+}
+
+void merkle_build_inner(uint8_t *nodes_b, uint64_t L, hipStream_t st) {
+    uint64_t *nodes = (uint64_t *)nodes_b;
+    uint64_t level_begin = L - 1;
+    // big levels: one launch per level
+    while (level_begin > 2048) {
+        uint64_t nb = level_begin / 2;
+        uint64_t count = level_begin - nb;
+        dim3 grid((unsigned)((count + 255) / 256));
+        hipLaunchKernelGGL(merkle_level_kernel, grid, dim3(256), 0, st, nodes, nb, count);
+        level_begin = nb;
+    }
+    if (level_begin > 0) hipLaunchKernelGGL(merkle_tail_kernel, dim3(1), dim3(1024), 0, st, nodes, level_begin);
+}
+
+}  // namespace lvm

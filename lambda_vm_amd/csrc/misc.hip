@@ -1,0 +1,96 @@
+// Small kernels: field-arithmetic test kernels (crypto/math-cuda/kernels/
+// arith.cu semantics), bit-reversed row gathers, Merkle path gathers.
+#include "lvm_internal.hpp"
+
+namespace lvm {
+
+__global__ void gl_op_kernel(int op, const uint64_t *a, const uint64_t *b, uint64_t *out, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t x = a[i], y = b ? b[i] : 0, r;
+    switch (op) {
+    case 0: r = add(x, y); break;
+    case 1: r = sub(x, y); break;
+    case 2: r = mul(x, y); break;
+    case 3: r = neg(x); break;
+    case 4: r = inv(x); break;
+    default: r = sqr(x); break;
+    }
+    out[i] = r;
+}
+
+__global__ void fp3_op_kernel(int op, const uint64_t *a, const uint64_t *b, uint64_t *out, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    fp3 x{a[3 * i], a[3 * i + 1], a[3 * i + 2]};
+    fp3 y{0, 0, 0};
+    if (b) y = fp3{b[3 * i], b[3 * i + 1], b[3 * i + 2]};
+    fp3 r;
+    switch (op) {
+    case 0: r = f3_add(x, y); break;
+    case 1: r = f3_sub(x, y); break;
+    case 2: r = f3_mul(x, y); break;
+    case 3: r = f3_neg(x); break;
+    case 4: r = f3_inv(x); break;
+    default: r = f3_sqr(x); break;
+    }
+    out[3 * i] = r.c0;
+    out[3 * i + 1] = r.c1;
+    out[3 * i + 2] = r.c2;
+}
+
+void launch_gl_op(int op, const uint64_t *a, const uint64_t *b, uint64_t *out, uint64_t n, bool ext3,
+                  hipStream_t st) {
+    dim3 grid((unsigned)((n + 255) / 256));
+    if (ext3)
+        hipLaunchKernelGGL(fp3_op_kernel, grid, dim3(256), 0, st, op, a, b, out, n);
+    else
+        hipLaunchKernelGGL(gl_op_kernel, grid, dim3(256), 0, st, op, a, b, out, n);
+}
+
+// dst natural row r <- src physical row bitrev(r) (rows of `w` words)
+__global__ void gather_bitrev_rows_kernel(const uint64_t *src, uint64_t *dst, uint64_t rows, uint32_t log_rows,
+                                          uint64_t w) {
+    uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= rows * w) return;
+    uint64_t r = idx / w, c = idx - r * w;
+    uint64_t pr = bitrev((uint32_t)r, log_rows);
+    dst[idx] = src[pr * w + c];
+}
+
+void launch_gather_bitrev_rows(const uint64_t *src, uint64_t *dst, uint64_t rows, uint32_t log_rows, uint64_t w,
+                               hipStream_t st) {
+    uint64_t total = rows * w;
+    if (!total) return;
+    hipLaunchKernelGGL(gather_bitrev_rows_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, src,
+                       dst, rows, log_rows, w);
+}
+
+// Merkle authentication paths (sibling of each node from the leaf up,
+// crypto/crypto/src/merkle_tree/merkle.rs:291-308) and the two opened rows
+// per position (physical rows 2*pos and 2*pos+1 of the bit-reversed table).
+__global__ void open_kernel(const uint64_t *nodes, uint64_t L, const uint64_t *lde, uint64_t w,
+                            const uint64_t *pos, uint64_t count, uint32_t depth, uint64_t *paths,
+                            uint64_t *rows) {
+    uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= count) return;
+    uint64_t p = pos[q];
+    uint64_t idx = p + L - 1;
+    for (uint32_t k = 0; k < depth; k++) {
+        uint64_t sib = (idx % 2 == 0) ? idx - 1 : idx + 1;
+        for (int j = 0; j < 4; j++) paths[(q * depth + k) * 4 + j] = nodes[sib * 4 + j];
+        idx = (idx % 2 == 0) ? (idx - 1) / 2 : idx / 2;
+    }
+    if (rows) {
+        for (uint64_t j = 0; j < 2 * w; j++) rows[q * 2 * w + j] = lde[2 * p * w + j];
+    }
+}
+
+void launch_open(const uint64_t *nodes, uint64_t L, const uint64_t *lde, uint64_t w, const uint64_t *pos,
+                 uint64_t count, uint32_t depth, uint64_t *paths, uint64_t *rows, hipStream_t st) {
+    if (!count) return;
+    hipLaunchKernelGGL(open_kernel, dim3((unsigned)((count + 63) / 64)), dim3(64), 0, st, nodes, L, lde, w, pos,
+                       count, depth, paths, rows);
+}
+
+}  // namespace lvm

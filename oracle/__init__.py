@@ -1,0 +1,393 @@
+"""ORACLE — TEST INFRASTRUCTURE ONLY.
+
+ctypes bindings for ``oracle/liblvm_oracle.so``, the C restatement of the
+reference CPU prover's STARK hot path (Goldilocks/Fp3 arithmetic, Keccak-256,
+the Fiat-Shamir transcript, the two-half batched FFT and coset LDE, the
+row-pair Merkle commitment, FRI, barycentric OOD, batch inversion, grinding).
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
+``cpu_baseline`` leg may import this package, as the checker or as the timed
+CPU baseline. The product package ``lambda_vm_amd`` never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblvm_oracle.so")
+
+P = 0xFFFFFFFF00000001
+
+u64p = C.POINTER(C.c_uint64)
+u8p = C.POINTER(C.c_uint8)
+szp = C.POINTER(C.c_size_t)
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (gcc, no GPU needed)."""
+    if force and os.path.exists(LIB_PATH):
+        os.remove(LIB_PATH)
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = _declare(C.CDLL(LIB_PATH))
+    return _lib
+
+
+def _declare(L):
+    sig = {
+        "ora_set_threads": (None, [C.c_int]),
+        "ora_get_threads": (C.c_int, []),
+        "ora_gl_op": (None, [C.c_int, u64p, u64p, u64p, C.c_size_t]),
+        "ora_fp3_op": (None, [C.c_int, u64p, u64p, u64p, C.c_size_t]),
+        "ora_gl_root_of_unity": (C.c_uint64, [C.c_uint]),
+        "ora_gl_pow": (C.c_uint64, [C.c_uint64, C.c_uint64]),
+        "ora_keccak256": (None, [u8p, C.c_size_t, u8p]),
+        "ora_keccak_pad": (None, [u8p, C.c_size_t, C.c_uint8, u8p]),
+        "ora_keccak_f1600": (None, [u64p]),
+        "ora_transcript_new": (C.c_void_p, [u8p, C.c_size_t]),
+        "ora_transcript_clone": (C.c_void_p, [C.c_void_p]),
+        "ora_transcript_free": (None, [C.c_void_p]),
+        "ora_transcript_append_bytes": (None, [C.c_void_p, u8p, C.c_size_t]),
+        "ora_transcript_append_base": (None, [C.c_void_p, C.c_uint64]),
+        "ora_transcript_append_fp3": (None, [C.c_void_p, u64p]),
+        "ora_transcript_sample": (None, [C.c_void_p, u8p]),
+        "ora_transcript_state": (None, [C.c_void_p, u8p]),
+        "ora_transcript_sample_base": (C.c_uint64, [C.c_void_p]),
+        "ora_transcript_sample_fp3": (None, [C.c_void_p, u64p]),
+        "ora_transcript_sample_u64": (C.c_uint64, [C.c_void_p, C.c_uint64]),
+        "ora_bitrev_permute_row_major": (None, [u64p, C.c_size_t, C.c_size_t, C.c_size_t]),
+        "ora_fft_two_half": (C.c_int, [u64p, C.c_size_t, C.c_size_t, C.c_int]),
+        "ora_lde_weights": (None, [C.c_size_t, C.c_uint64, u64p]),
+        "ora_coset_lde_row_major": (C.c_int, [u64p, C.c_size_t, C.c_size_t, C.c_size_t, u64p, u64p]),
+        "ora_commit_rows_bit_reversed": (
+            C.c_int, [u64p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, u8p]),
+        "ora_merkle_padded_leaves": (C.c_size_t, [C.c_size_t]),
+        "ora_merkle_build": (C.c_int, [u8p, C.c_size_t, u8p]),
+        "ora_merkle_path": (C.c_size_t, [u8p, C.c_size_t, C.c_size_t, u8p]),
+        "ora_merkle_verify": (C.c_int, [u8p, u8p, C.c_size_t, u8p, C.c_size_t]),
+        "ora_fri_inv_twiddles": (None, [C.c_uint64, C.c_size_t, u64p]),
+        "ora_fri_fold": (None, [u64p, C.c_size_t, u64p, u64p, u64p]),
+        "ora_fri_update_twiddles": (None, [u64p, C.c_size_t, u64p]),
+        "ora_fri_layout": (None, [C.c_uint, C.c_uint, C.c_uint, C.POINTER(C.c_uint), szp, szp,
+                                  C.POINTER(C.c_uint)]),
+        "ora_fri_layer_leaves": (None, [u64p, C.c_size_t, u8p]),
+        "ora_fri_commit": (C.c_int, [u64p, C.c_size_t, C.c_void_p, C.c_uint64, C.c_uint, C.c_uint,
+                                     u8p, u64p, u64p]),
+        "ora_fri_terminal_coeffs": (None, [u64p, C.c_size_t, C.c_uint64, C.c_uint, u64p]),
+        "ora_barycentric_base": (None, [u64p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_uint64, u64p, u64p]),
+        "ora_frame_points": (None, [u64p, szp, C.c_size_t, C.c_uint64, C.c_size_t, u64p]),
+        "ora_batch_inverse_fp3": (C.c_int, [u64p, C.c_size_t]),
+        "ora_batch_inverse_base": (C.c_int, [u64p, C.c_size_t]),
+        "ora_is_valid_nonce": (C.c_int, [u8p, C.c_uint64, C.c_uint8]),
+        "ora_generate_nonce": (C.c_uint64, [u8p, C.c_uint8, C.c_uint64]),
+        "ora_lde_commit": (C.c_int, [u64p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_uint64, u64p, u8p]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    return L
+
+
+def _p64(a: np.ndarray):
+    assert a.dtype == np.uint64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(u64p)
+
+
+def _p8(a):
+    if isinstance(a, (bytes, bytearray)):
+        a = np.frombuffer(bytes(a), dtype=np.uint8).copy()
+    assert a.dtype == np.uint8 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(u8p), a
+
+
+def set_threads(t: int) -> None:
+    lib().ora_set_threads(int(t))
+
+
+def get_threads() -> int:
+    return lib().ora_get_threads()
+
+
+OPS = {"add": 0, "sub": 1, "mul": 2, "neg": 3, "inv": 4, "sqr": 5, "canon": 6}
+
+
+def gl_op(op: str, a, b=None) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.uint64)
+    b = a if b is None else np.ascontiguousarray(b, dtype=np.uint64)
+    out = np.empty_like(a)
+    lib().ora_gl_op(OPS[op], _p64(a), _p64(b), _p64(out), a.size)
+    return out
+
+
+def fp3_op(op: str, a, b=None) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.uint64).reshape(-1, 3)
+    out = np.empty_like(a)
+    if b is None:
+        lib().ora_fp3_op(OPS[op], _p64(a), None, _p64(out), a.shape[0])
+    else:
+        b = np.ascontiguousarray(b, dtype=np.uint64).reshape(-1, 3)
+        lib().ora_fp3_op(OPS[op], _p64(a), _p64(b), _p64(out), a.shape[0])
+    return out
+
+
+def root_of_unity(order: int) -> int:
+    return lib().ora_gl_root_of_unity(order)
+
+
+def gl_pow(a: int, e: int) -> int:
+    return lib().ora_gl_pow(a, e)
+
+
+def keccak256(data: bytes) -> bytes:
+    out = np.zeros(32, dtype=np.uint8)
+    buf = np.frombuffer(bytes(data) + b"\0", dtype=np.uint8).copy()
+    lib().ora_keccak256(buf.ctypes.data_as(u8p), len(data), out.ctypes.data_as(u8p))
+    return out.tobytes()
+
+
+def keccak_pad(data: bytes, pad: int) -> bytes:
+    out = np.zeros(32, dtype=np.uint8)
+    buf = np.frombuffer(bytes(data) + b"\0", dtype=np.uint8).copy()
+    lib().ora_keccak_pad(buf.ctypes.data_as(u8p), len(data), pad, out.ctypes.data_as(u8p))
+    return out.tobytes()
+
+
+class Transcript:
+    """DefaultTranscript restatement (default_transcript.rs:31-146)."""
+
+    def __init__(self, data: bytes = b"", _handle=None):
+        if _handle is not None:
+            self.h = _handle
+        else:
+            p, keep = _p8(np.frombuffer(bytes(data) + b"\0", dtype=np.uint8).copy())
+            self.h = lib().ora_transcript_new(p, len(data))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ora_transcript_free(self.h)
+            self.h = None
+
+    def clone(self) -> "Transcript":
+        return Transcript(_handle=lib().ora_transcript_clone(self.h))
+
+    def append_bytes(self, data: bytes) -> None:
+        p, keep = _p8(np.frombuffer(bytes(data) + b"\0", dtype=np.uint8).copy())
+        lib().ora_transcript_append_bytes(self.h, p, len(data))
+
+    def append_base(self, v: int) -> None:
+        lib().ora_transcript_append_base(self.h, int(v))
+
+    def append_fp3(self, v) -> None:
+        a = np.ascontiguousarray(v, dtype=np.uint64)
+        lib().ora_transcript_append_fp3(self.h, _p64(a))
+
+    def sample(self) -> bytes:
+        out = np.zeros(32, dtype=np.uint8)
+        lib().ora_transcript_sample(self.h, out.ctypes.data_as(u8p))
+        return out.tobytes()
+
+    def state(self) -> bytes:
+        out = np.zeros(32, dtype=np.uint8)
+        lib().ora_transcript_state(self.h, out.ctypes.data_as(u8p))
+        return out.tobytes()
+
+    def sample_base(self) -> int:
+        return lib().ora_transcript_sample_base(self.h)
+
+    def sample_fp3(self) -> np.ndarray:
+        out = np.zeros(3, dtype=np.uint64)
+        lib().ora_transcript_sample_fp3(self.h, _p64(out))
+        return out
+
+    def sample_u64(self, ub: int) -> int:
+        return lib().ora_transcript_sample_u64(self.h, ub)
+
+
+def fft_two_half(buf: np.ndarray, inverse: bool = False) -> np.ndarray:
+    """buf: (n, m) uint64 row-major; returns a transformed copy."""
+    out = np.ascontiguousarray(buf, dtype=np.uint64).copy()
+    n, m = out.shape
+    rc = lib().ora_fft_two_half(_p64(out), n, m, int(inverse))
+    assert rc == 0
+    return out
+
+
+def lde_weights(n: int, offset: int) -> np.ndarray:
+    out = np.empty(n, dtype=np.uint64)
+    lib().ora_lde_weights(n, offset, _p64(out))
+    return out
+
+
+def coset_lde_row_major(trace: np.ndarray, blowup: int, offset: int, weights=None) -> np.ndarray:
+    trace = np.ascontiguousarray(trace, dtype=np.uint64)
+    n, m = trace.shape
+    w = lde_weights(n, offset) if weights is None else np.ascontiguousarray(weights, dtype=np.uint64)
+    out = np.empty((n * blowup, m), dtype=np.uint64)
+    rc = lib().ora_coset_lde_row_major(_p64(trace), n, m, blowup, _p64(w), _p64(out))
+    assert rc == 0, rc
+    return out
+
+
+def merkle_padded_leaves(n: int) -> int:
+    return lib().ora_merkle_padded_leaves(n)
+
+
+def commit_rows_bit_reversed(data: np.ndarray, elem_words: int = 1, c0: int = 0, c1=None) -> np.ndarray:
+    """data: (rows, cols[*3]) row-major; returns the (2L-1, 32) node array."""
+    data = np.ascontiguousarray(data, dtype=np.uint64)
+    rows = data.shape[0]
+    cols = data.shape[1] // elem_words
+    c1 = cols if c1 is None else c1
+    L = merkle_padded_leaves(rows // 2)
+    nodes = np.zeros((2 * L - 1, 32), dtype=np.uint8)
+    rc = lib().ora_commit_rows_bit_reversed(_p64(data), rows, cols, elem_words, c0, c1,
+                                            nodes.ctypes.data_as(u8p))
+    assert rc == 0, rc
+    return nodes
+
+
+def merkle_build(leaves: np.ndarray) -> np.ndarray:
+    leaves = np.ascontiguousarray(leaves, dtype=np.uint8).reshape(-1, 32)
+    L = merkle_padded_leaves(leaves.shape[0])
+    nodes = np.zeros((2 * L - 1, 32), dtype=np.uint8)
+    rc = lib().ora_merkle_build(leaves.ctypes.data_as(u8p), leaves.shape[0], nodes.ctypes.data_as(u8p))
+    assert rc == 0
+    return nodes
+
+
+def merkle_path(nodes: np.ndarray, pos: int) -> np.ndarray:
+    nodes = np.ascontiguousarray(nodes, dtype=np.uint8)
+    L = (nodes.shape[0] + 1) // 2
+    out = np.zeros((64, 32), dtype=np.uint8)
+    k = lib().ora_merkle_path(nodes.ctypes.data_as(u8p), L, pos, out.ctypes.data_as(u8p))
+    return out[:k].copy()
+
+
+def merkle_verify(root: bytes, leaf: bytes, pos: int, path: np.ndarray) -> bool:
+    r, kr = _p8(root)
+    lf, kl = _p8(leaf)
+    path = np.ascontiguousarray(path, dtype=np.uint8)
+    return bool(lib().ora_merkle_verify(r, lf, pos, path.ctypes.data_as(u8p), path.shape[0]))
+
+
+def fri_inv_twiddles(offset: int, domain_size: int) -> np.ndarray:
+    out = np.empty(domain_size // 2, dtype=np.uint64)
+    lib().ora_fri_inv_twiddles(offset, domain_size, _p64(out))
+    return out
+
+
+def fri_fold(evals: np.ndarray, zeta, inv_tw: np.ndarray) -> np.ndarray:
+    evals = np.ascontiguousarray(evals, dtype=np.uint64).reshape(-1, 3)
+    z = np.ascontiguousarray(zeta, dtype=np.uint64)
+    tw = np.ascontiguousarray(inv_tw, dtype=np.uint64)
+    out = np.empty((evals.shape[0] // 2, 3), dtype=np.uint64)
+    lib().ora_fri_fold(_p64(evals), evals.shape[0], _p64(z), _p64(tw), _p64(out))
+    return out
+
+
+def fri_layout(lde_log: int, blowup_log: int, k: int):
+    tf, ek = C.c_uint(), C.c_uint()
+    nc, tl = C.c_size_t(), C.c_size_t()
+    lib().ora_fri_layout(lde_log, blowup_log, k, C.byref(tf), C.byref(nc), C.byref(tl), C.byref(ek))
+    return tf.value, nc.value, tl.value, ek.value
+
+
+def fri_layer_leaves(evals: np.ndarray) -> np.ndarray:
+    evals = np.ascontiguousarray(evals, dtype=np.uint64).reshape(-1, 3)
+    out = np.zeros((evals.shape[0] // 2, 32), dtype=np.uint8)
+    lib().ora_fri_layer_leaves(_p64(evals), evals.shape[0], out.ctypes.data_as(u8p))
+    return out
+
+
+def fri_commit(evals: np.ndarray, transcript: Transcript, offset: int, blowup_log: int, k: int,
+               keep_layers: bool = False):
+    ev = np.ascontiguousarray(evals, dtype=np.uint64).reshape(-1, 3).copy()
+    n = ev.shape[0]
+    tf, nc, tl, ek = fri_layout(n.bit_length() - 1, blowup_log, k)
+    roots = np.zeros((max(nc, 1), 32), dtype=np.uint8)
+    coeffs = np.zeros((1 << ek, 3), dtype=np.uint64)
+    layers = None
+    if keep_layers:
+        tot = sum(n >> (i + 1) for i in range(nc))
+        layers = np.zeros((max(tot, 1), 3), dtype=np.uint64)
+    rc = lib().ora_fri_commit(_p64(ev), n, transcript.h, offset, blowup_log, k,
+                              roots.ctypes.data_as(u8p), _p64(coeffs),
+                              _p64(layers) if layers is not None else None)
+    assert rc == 0
+    return roots[:nc].copy(), coeffs, layers
+
+
+def fri_terminal_coeffs(codeword_bitrev: np.ndarray, terminal_offset: int, k: int) -> np.ndarray:
+    cw = np.ascontiguousarray(codeword_bitrev, dtype=np.uint64).reshape(-1, 3)
+    out = np.zeros((1 << k, 3), dtype=np.uint64)
+    lib().ora_fri_terminal_coeffs(_p64(cw), cw.shape[0], terminal_offset, k, _p64(out))
+    return out
+
+
+def barycentric_base(lde: np.ndarray, n: int, blowup: int, offset: int, z) -> np.ndarray:
+    lde = np.ascontiguousarray(lde, dtype=np.uint64)
+    m = lde.shape[1]
+    zz = np.ascontiguousarray(z, dtype=np.uint64)
+    out = np.zeros((m, 3), dtype=np.uint64)
+    lib().ora_barycentric_base(_p64(lde), n, m, blowup, offset, _p64(zz), _p64(out))
+    return out
+
+
+def frame_points(z, frame_offsets, trace_root: int, step_size: int) -> np.ndarray:
+    zz = np.ascontiguousarray(z, dtype=np.uint64)
+    fo = (C.c_size_t * len(frame_offsets))(*frame_offsets)
+    out = np.zeros((len(frame_offsets) * step_size, 3), dtype=np.uint64)
+    lib().ora_frame_points(_p64(zz), fo, len(frame_offsets), trace_root, step_size, _p64(out))
+    return out
+
+
+def batch_inverse_fp3(v: np.ndarray) -> np.ndarray:
+    out = np.ascontiguousarray(v, dtype=np.uint64).reshape(-1, 3).copy()
+    rc = lib().ora_batch_inverse_fp3(_p64(out), out.shape[0])
+    assert rc == 0
+    return out
+
+
+def batch_inverse_base(v: np.ndarray) -> np.ndarray:
+    out = np.ascontiguousarray(v, dtype=np.uint64).copy()
+    rc = lib().ora_batch_inverse_base(_p64(out), out.shape[0])
+    assert rc == 0
+    return out
+
+
+def is_valid_nonce(seed: bytes, nonce: int, g: int) -> bool:
+    p, keep = _p8(seed)
+    return bool(lib().ora_is_valid_nonce(p, nonce, g))
+
+
+def generate_nonce(seed: bytes, g: int, limit_search: int = 1 << 40) -> int:
+    p, keep = _p8(seed)
+    return lib().ora_generate_nonce(p, g, limit_search)
+
+
+def lde_commit(trace: np.ndarray, blowup: int, offset: int, want_lde: bool = True):
+    trace = np.ascontiguousarray(trace, dtype=np.uint64)
+    n, m = trace.shape
+    N = n * blowup
+    L = merkle_padded_leaves(N // 2)
+    nodes = np.zeros((2 * L - 1, 32), dtype=np.uint8)
+    lde = np.empty((N, m), dtype=np.uint64) if want_lde else None
+    rc = lib().ora_lde_commit(_p64(trace), n, m, blowup, offset,
+                              _p64(lde) if lde is not None else None, nodes.ctypes.data_as(u8p))
+    assert rc == 0, rc
+    return lde, nodes
