@@ -118,6 +118,10 @@ int lvm_lde_tree_nodes(const lvm_lde *h, uint8_t *nodes_out, size_t *num_nodes);
 int lvm_lde_open(const lvm_lde *h, const uint64_t *positions, size_t count, uint8_t *paths_out,
                  uint64_t *rows_out);
 
+/* Coset weights w_i = n^-1 * offset^i, i < n, written to device memory
+ * (crypto/stark/src/prover.rs:451-467 `LdeTwiddles::new` coset_weights). */
+int lvm_coset_weights(size_t n, uint64_t offset, uint64_t *dev_out);
+
 /* ---------------- Keccak Merkle trees on host data ----------------
  * crypto/math-cuda/src/merkle.rs:316 `build_merkle_tree_on_device`: leaves
  * (num_leaves x 32 bytes) padded to a power of two by repeating the last

@@ -58,6 +58,7 @@ SIGNATURES = {
     "lvm_lde_device_ptr": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     "lvm_lde_tree_nodes": (C.c_int, [C.c_void_p, u8p, szp]),
     "lvm_lde_open": (C.c_int, [C.c_void_p, u64p, C.c_size_t, u8p, u64p]),
+    "lvm_coset_weights": (C.c_int, [C.c_size_t, C.c_uint64, C.c_void_p]),
     "lvm_build_merkle_tree": (C.c_int, [u8p, C.c_size_t, u8p]),
     "lvm_prof_enable": (C.c_int, [C.c_int]),
     "lvm_prof_reset": (C.c_int, []),

@@ -208,6 +208,13 @@ def coset_lde_dev(trace: DeviceBuffer, n: int, m: int, blowup_factor: int, weigh
     return GpuLde(h.value)
 
 
+def coset_weights_dev(n: int, offset: int) -> DeviceBuffer:
+    """Device buffer of w_i = n^-1 * offset^i (prover.rs:451-467)."""
+    b = DeviceBuffer(8 * n)
+    N.call("lvm_coset_weights", n, offset, C.c_void_p(b.ptr))
+    return b
+
+
 def build_merkle_tree(leaves) -> np.ndarray:
     """(num_leaves, 32) digests -> (2L-1, 32) nodes, root first."""
     N.ensure_device()

@@ -14,6 +14,7 @@ void launch_gl_op(int op, const uint64_t *a, const uint64_t *b, uint64_t *out, u
                   hipStream_t st);
 void launch_gather_bitrev_rows(const uint64_t *src, uint64_t *dst, uint64_t rows, uint32_t log_rows, uint64_t w,
                                hipStream_t st);
+void launch_coset_weights(uint64_t n, uint64_t offset, uint64_t scale, uint64_t *out, hipStream_t st);
 void launch_open(const uint64_t *nodes, uint64_t L, const uint64_t *lde, uint64_t w, const uint64_t *pos,
                  uint64_t count, uint32_t depth, uint64_t *paths, uint64_t *rows, hipStream_t st);
 }  // namespace lvm
@@ -377,6 +378,17 @@ int lvm_lde_open(const lvm_lde *h, const uint64_t *positions, size_t count, uint
     if (depth) LVM_HIP(hipMemcpyAsync(paths_out, dpaths.p, count * depth * 32, hipMemcpyDeviceToHost, st));
     if (rows_out) LVM_HIP(hipMemcpyAsync(rows_out, drows.p, count * 2 * h->w * 8, hipMemcpyDeviceToHost, st));
     LVM_HIP(hipStreamSynchronize(st));
+    return LVM_OK;
+}
+
+int lvm_coset_weights(size_t n, uint64_t offset, uint64_t *dev_out) {
+    if (!dev_out || !is_pow2(n)) return fail(LVM_ERR_ARG, "bad weights arguments");
+    if (canon(offset) == 0) return fail(LVM_ERR_ARG, "coset offset must be non-zero");
+    Device *d = dev_or_fail();
+    if (!d) return LVM_ERR_DEVICE;
+    launch_coset_weights(n, offset, inv((uint64_t)n), dev_out, d->stream);
+    LVM_HIP(hipGetLastError());
+    LVM_HIP(hipStreamSynchronize(d->stream));
     return LVM_OK;
 }
 

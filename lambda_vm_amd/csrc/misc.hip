@@ -93,4 +93,18 @@ void launch_open(const uint64_t *nodes, uint64_t L, const uint64_t *lde, uint64_
                        count, depth, paths, rows);
 }
 
+// w_i = scale * offset^i (the coset weights n^-1 * g^i of
+// crypto/stark/src/prover.rs:451-467 with scale = n^-1)
+__global__ void coset_weights_kernel(uint64_t n, uint64_t offset, uint64_t scale, uint64_t *out) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = mul(scale, pow(offset, i));
+}
+
+void launch_coset_weights(uint64_t n, uint64_t offset, uint64_t scale, uint64_t *out, hipStream_t st) {
+    if (!n) return;
+    hipLaunchKernelGGL(coset_weights_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, offset, scale,
+                       out);
+}
+
 }  // namespace lvm

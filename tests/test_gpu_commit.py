@@ -149,3 +149,9 @@ def test_bad_shapes_are_rejected(gpu):
         gpu.coset_lde_row_major_with_merkle_tree_keep(np.zeros(3 * 2, np.uint64), 3, 2, 4, np.zeros(3, np.uint64))
     with pytest.raises(LvmError):
         gpu.coset_lde_row_major_with_merkle_tree_keep(np.zeros(8, np.uint64), 4, 2, 3, np.zeros(4, np.uint64))
+
+
+def test_device_coset_weights(gpu, ora):
+    for n, g in ((1, 7), (8, 3), (1 << 12, 7)):
+        b = gpu.coset_weights_dev(n, g)
+        assert np.array_equal(canon(b.to_array(np.uint64, (n,))), canon(ora.lde_weights(n, g)))
