@@ -32,15 +32,37 @@ def log(msg: str) -> None:
 
 
 def algorithmic_bytes(kernel: str, n: int, m: int, blowup: int) -> float:
-    """Compulsory HBM bytes per launch of the named kernel group."""
+    """Compulsory HBM bytes per launch of the named kernel: what it must read
+    and write once (rows x columns x 8 bytes per u64 element)."""
     N = n * blowup
     if kernel == "keccak_leaves":
         return 8.0 * N * m + 32.0 * (N // 2)  # read the LDE once, write the leaves
-    if kernel == "ntt_lde":
-        return 8.0 * n * m + 8.0 * N * m  # read the trace, write the LDE
+    if kernel.startswith("ntt_inv_pass"):
+        return 16.0 * n * m  # read + write the n-row table
+    if kernel == "ntt_coset_pass0":
+        return 8.0 * n * m + 8.0 * N * m  # read the coefficients, write all cosets
+    if kernel.startswith("ntt_fwd_pass"):
+        return 16.0 * N * m  # read + write the N-row LDE
     if kernel == "merkle_inner":
         return 96.0 * (N // 2 - 1)  # read two children, write the parent
     raise KeyError(kernel)
+
+
+KERNELS = ["ntt_inv_pass0", "ntt_inv_pass1", "ntt_inv_pass2", "ntt_coset_pass0", "ntt_fwd_pass1",
+           "ntt_fwd_pass2", "keccak_leaves", "merkle_inner"]
+
+
+def measured_traffic(kernel: str, n: int, m: int, blowup: int):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration),
+    when one exists for this exact workload; otherwise None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(path):
+        return None
+    data = json.load(open(path))
+    key = f"{n}x{m}x{blowup}"
+    rec = data.get("workloads", {}).get(key, {}).get(kernel)
+    return rec.get("bytes_per_launch") if rec else None
 
 
 def cpu_baseline(log_n: int, m: int, blowup: int, offset: int) -> dict:
@@ -103,6 +125,18 @@ def main() -> int:
         h.close()
         return root
 
+    # plain copy of 1 GiB (the calibration kernel for the traffic counters);
+    # its rate is reported beside the roofline as the achievable HBM rate
+    cbytes = 1 << 30
+    src, dst = backend.DeviceBuffer(cbytes), backend.DeviceBuffer(cbytes)
+    backend.dev_copy(dst, src, cbytes)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        backend.dev_copy(dst, src, cbytes)
+    copy_gbs = 5 * 2 * cbytes / (time.perf_counter() - t0) / 1e9
+    src.free()
+    dst.free()
+
     log(f"[bench] warmup {args.warmup}")
     root0 = None
     for _ in range(args.warmup):
@@ -124,9 +158,10 @@ def main() -> int:
     value = cells / (dt / args.steps) / 1e9
 
     groups = {}
-    for k in ("ntt_lde", "keccak_leaves", "merkle_inner"):
+    for k in KERNELS:
         cnt, tot = backend.prof_query(k)
-        groups[k] = (cnt, tot)
+        if cnt:
+            groups[k] = (cnt, tot)
     dom = max(groups, key=lambda k: groups[k][1])
     cnt, tot = groups[dom]
     avg_s = tot / max(cnt, 1) / 1e3
@@ -140,8 +175,12 @@ def main() -> int:
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": None,
         "avg_ms": round(avg_s * 1e3, 4),
-        "per_stage_ms": {k: round(v[1] / max(v[0], 1), 4) for k, v in groups.items()},
+        "per_kernel_ms": {k: round(v[1] / max(v[0], 1), 4) for k, v in groups.items()},
+        "copy_kernel_GBps": round(copy_gbs, 1),
     }
+    traffic = measured_traffic(dom, n, m, B)
+    if traffic is not None:
+        roofline["traffic"] = traffic
     out = {
         "metric": "batched LDE+Keccak-Merkle Gcells/s vs per-table launch; p50/p99 proof latency",
         "value": round(value, 4),

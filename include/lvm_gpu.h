@@ -60,6 +60,9 @@ int lvm_dev_alloc(void **ptr, size_t bytes);
 int lvm_dev_free(void *ptr);
 int lvm_memcpy_htod(void *dst, const void *src, size_t bytes);
 int lvm_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+/* Device-to-device copy by a plain 16-byte-vector kernel (16-byte aligned
+ * pointers and size); also the calibration kernel for HBM traffic counters. */
+int lvm_dev_copy(void *dst, const void *src, size_t bytes);
 
 /* ---------------- field arithmetic test kernels ----------------
  * crypto/math-cuda/src/lib.rs:34-197 (gl_{add,sub,mul,neg}, ext3_{mul,add,sub}).

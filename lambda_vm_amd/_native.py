@@ -42,6 +42,7 @@ SIGNATURES = {
     "lvm_dev_free": (C.c_int, [C.c_void_p]),
     "lvm_memcpy_htod": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "lvm_memcpy_dtoh": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "lvm_dev_copy": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "lvm_gl_op": (C.c_int, [C.c_int, u64p, u64p, u64p, C.c_size_t]),
     "lvm_fp3_op": (C.c_int, [C.c_int, u64p, u64p, u64p, C.c_size_t]),
     "lvm_ntt_forward": (C.c_int, [u64p, u64p, C.c_size_t, C.c_size_t]),

@@ -208,6 +208,11 @@ def coset_lde_dev(trace: DeviceBuffer, n: int, m: int, blowup_factor: int, weigh
     return GpuLde(h.value)
 
 
+def dev_copy(dst: DeviceBuffer, src: DeviceBuffer, nbytes: int) -> None:
+    """Device-to-device copy by the library's plain copy kernel."""
+    N.call("lvm_dev_copy", C.c_void_p(dst.ptr), C.c_void_p(src.ptr), nbytes)
+
+
 def coset_weights_dev(n: int, offset: int) -> DeviceBuffer:
     """Device buffer of w_i = n^-1 * offset^i (prover.rs:451-467)."""
     b = DeviceBuffer(8 * n)

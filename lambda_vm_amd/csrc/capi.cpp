@@ -15,6 +15,7 @@ void launch_gl_op(int op, const uint64_t *a, const uint64_t *b, uint64_t *out, u
 void launch_gather_bitrev_rows(const uint64_t *src, uint64_t *dst, uint64_t rows, uint32_t log_rows, uint64_t w,
                                hipStream_t st);
 void launch_coset_weights(uint64_t n, uint64_t offset, uint64_t scale, uint64_t *out, hipStream_t st);
+void launch_copy16(void *dst, const void *src, uint64_t n16, hipStream_t st);
 void launch_open(const uint64_t *nodes, uint64_t L, const uint64_t *lde, uint64_t w, const uint64_t *pos,
                  uint64_t count, uint32_t depth, uint64_t *paths, uint64_t *rows, hipStream_t st);
 }  // namespace lvm
@@ -387,6 +388,21 @@ int lvm_coset_weights(size_t n, uint64_t offset, uint64_t *dev_out) {
     Device *d = dev_or_fail();
     if (!d) return LVM_ERR_DEVICE;
     launch_coset_weights(n, offset, inv((uint64_t)n), dev_out, d->stream);
+    LVM_HIP(hipGetLastError());
+    LVM_HIP(hipStreamSynchronize(d->stream));
+    return LVM_OK;
+}
+
+int lvm_dev_copy(void *dst, const void *src, size_t bytes) {
+    if ((!dst || !src) && bytes) return fail(LVM_ERR_ARG, "null pointer");
+    if ((bytes % 16) || ((uintptr_t)dst % 16) || ((uintptr_t)src % 16))
+        return fail(LVM_ERR_ARG, "copy needs 16-byte aligned pointers and sizes");
+    Device *d = dev_or_fail();
+    if (!d) return LVM_ERR_DEVICE;
+    {
+        ProfScope ps("dev_copy", d->stream);
+        launch_copy16(dst, src, bytes / 16, d->stream);
+    }
     LVM_HIP(hipGetLastError());
     LVM_HIP(hipStreamSynchronize(d->stream));
     return LVM_OK;

@@ -1,115 +1,146 @@
 // Keccak-f[1600] (FIPS 202 section 3: theta, rho, pi, chi, iota; 24 rounds)
-// for gfx950. The 25 lanes live in VGPRs; each 64-bit lane is a pair of
-// 32-bit registers, so a rotate is two v_alignbit_b32 and chi maps to
-// bit-select ops. Keccak-256 framing (rate 136 bytes, pad 0x01 ... 0x80,
-// little-endian lanes) is what the reference's Merkle leaves, nodes and
-// transcript use (crypto/crypto/src/hash/platform_keccak.rs:124-125, sha3
-// 0.10.8; crypto/math-cuda/kernels/keccak.cu:114-138 states the same framing).
+// for gfx950. Each 64-bit lane is held as two 32-bit VGPRs (lo, hi):
+//  - column parities and theta use v_bitop3_b32 as a 3-input XOR (LUT 0x96),
+//    and the theta term D[x] is folded into the per-lane XOR (A ^ C[x-1] ^
+//    rot(C[x+1], 1) is one 3-input XOR per half);
+//  - a 64-bit rotate is two v_alignbit_b32 (funnel shifts of the halves);
+//  - chi, a ^ (~b & c), is one v_bitop3_b32 per half (LUT 0xD2 for
+//    operands (a, b, c), i.e. f(0xF0, 0xCC, 0xAA)).
+// ~180 VALU instructions per round. Keccak-256 framing (rate 136 bytes, pad
+// 0x01 ... 0x80, little-endian lanes) is what the reference's Merkle leaves,
+// nodes and transcript use (crypto/crypto/src/hash/platform_keccak.rs:
+// 124-125, sha3 0.10.8; crypto/math-cuda/kernels/keccak.cu:114-138 states the
+// same framing).
 #pragma once
 #include "field.hpp"
 
 namespace lvm {
 
-__constant__ static const uint64_t KECCAK_RC[24] = {
-    0x0000000000000001ULL, 0x0000000000008082ULL, 0x800000000000808aULL, 0x8000000080008000ULL,
-    0x000000000000808bULL, 0x0000000080000001ULL, 0x8000000080008081ULL, 0x8000000000008009ULL,
-    0x000000000000008aULL, 0x0000000000000088ULL, 0x0000000080008009ULL, 0x000000008000000aULL,
-    0x000000008000808bULL, 0x800000000000008bULL, 0x8000000000008089ULL, 0x8000000000008003ULL,
-    0x8000000000008002ULL, 0x8000000000000080ULL, 0x000000000000800aULL, 0x800000008000000aULL,
-    0x8000000080008081ULL, 0x8000000000008080ULL, 0x0000000080000001ULL, 0x8000000080008008ULL};
+__constant__ static const uint32_t KECCAK_RC_LO[24] = {
+    0x00000001u, 0x00008082u, 0x0000808au, 0x80008000u, 0x0000808bu, 0x80000001u, 0x80008081u, 0x00008009u,
+    0x0000008au, 0x00000088u, 0x80008009u, 0x8000000au, 0x8000808bu, 0x0000008bu, 0x00008089u, 0x00008003u,
+    0x00008002u, 0x00000080u, 0x0000800au, 0x8000000au, 0x80008081u, 0x00008080u, 0x80000001u, 0x80008008u};
+__constant__ static const uint32_t KECCAK_RC_HI[24] = {
+    0x00000000u, 0x00000000u, 0x80000000u, 0x80000000u, 0x00000000u, 0x00000000u, 0x80000000u, 0x80000000u,
+    0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x00000000u, 0x80000000u, 0x80000000u, 0x80000000u,
+    0x80000000u, 0x80000000u, 0x00000000u, 0x80000000u, 0x80000000u, 0x80000000u, 0x00000000u, 0x80000000u};
 
+__device__ __forceinline__ uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96);
+}
+// a ^ (~b & c)
+__device__ __forceinline__ uint32_t chi(uint32_t a, uint32_t b, uint32_t c) {
+    return __builtin_amdgcn_bitop3_b32(a, b, c, 0xD2);
+}
+
+// rotate the 64-bit value (hi, lo) left by N, 0 <= N < 64
 template <int N>
-__device__ __forceinline__ uint64_t rotl64(uint64_t v) {
+__device__ __forceinline__ void rotl(uint32_t hi, uint32_t lo, uint32_t &oh, uint32_t &ol) {
     if constexpr (N == 0) {
-        return v;
+        oh = hi;
+        ol = lo;
     } else if constexpr (N == 32) {
-        return (v << 32) | (v >> 32);
+        oh = lo;
+        ol = hi;
+    } else if constexpr (N < 32) {
+        oh = __builtin_amdgcn_alignbit(hi, lo, 32 - N);
+        ol = __builtin_amdgcn_alignbit(lo, hi, 32 - N);
     } else {
-        return (v << N) | (v >> (64 - N));
+        oh = __builtin_amdgcn_alignbit(lo, hi, 64 - N);
+        ol = __builtin_amdgcn_alignbit(hi, lo, 64 - N);
     }
 }
 
-__device__ __forceinline__ void keccak_f1600(uint64_t s[25]) {
+// B[dst] = rot(A[src] ^ C[x-1] ^ rot(C[x+1], 1), R) where x = src % 5
+#define LVM_KB(dst, src, R)                                                                      \
+    rotl<R>(xor3(h[src], Ch[((src) + 4) % 5], Rh[((src) + 1) % 5]),                             \
+            xor3(l[src], Cl[((src) + 4) % 5], Rl[((src) + 1) % 5]), bh[dst], bl[dst])
+
+__device__ __forceinline__ void keccak_f1600_32(uint32_t l[25], uint32_t h[25]) {
 #pragma unroll 1
     for (int round = 0; round < 24; round++) {
-        uint64_t C0 = s[0] ^ s[5] ^ s[10] ^ s[15] ^ s[20];
-        uint64_t C1 = s[1] ^ s[6] ^ s[11] ^ s[16] ^ s[21];
-        uint64_t C2 = s[2] ^ s[7] ^ s[12] ^ s[17] ^ s[22];
-        uint64_t C3 = s[3] ^ s[8] ^ s[13] ^ s[18] ^ s[23];
-        uint64_t C4 = s[4] ^ s[9] ^ s[14] ^ s[19] ^ s[24];
-        uint64_t D0 = C4 ^ rotl64<1>(C1);
-        uint64_t D1 = C0 ^ rotl64<1>(C2);
-        uint64_t D2 = C1 ^ rotl64<1>(C3);
-        uint64_t D3 = C2 ^ rotl64<1>(C4);
-        uint64_t D4 = C3 ^ rotl64<1>(C0);
-        // theta + rho + pi: B[y, 2x+3y] = rot(A[x,y] ^ D[x], r[x,y])
-        uint64_t B0 = s[0] ^ D0;
-        uint64_t B10 = rotl64<1>(s[1] ^ D1);
-        uint64_t B20 = rotl64<62>(s[2] ^ D2);
-        uint64_t B5 = rotl64<28>(s[3] ^ D3);
-        uint64_t B15 = rotl64<27>(s[4] ^ D4);
-        uint64_t B16 = rotl64<36>(s[5] ^ D0);
-        uint64_t B1 = rotl64<44>(s[6] ^ D1);
-        uint64_t B11 = rotl64<6>(s[7] ^ D2);
-        uint64_t B21 = rotl64<55>(s[8] ^ D3);
-        uint64_t B6 = rotl64<20>(s[9] ^ D4);
-        uint64_t B7 = rotl64<3>(s[10] ^ D0);
-        uint64_t B17 = rotl64<10>(s[11] ^ D1);
-        uint64_t B2 = rotl64<43>(s[12] ^ D2);
-        uint64_t B12 = rotl64<25>(s[13] ^ D3);
-        uint64_t B22 = rotl64<39>(s[14] ^ D4);
-        uint64_t B23 = rotl64<41>(s[15] ^ D0);
-        uint64_t B8 = rotl64<45>(s[16] ^ D1);
-        uint64_t B18 = rotl64<15>(s[17] ^ D2);
-        uint64_t B3 = rotl64<21>(s[18] ^ D3);
-        uint64_t B13 = rotl64<8>(s[19] ^ D4);
-        uint64_t B14 = rotl64<18>(s[20] ^ D0);
-        uint64_t B24 = rotl64<2>(s[21] ^ D1);
-        uint64_t B9 = rotl64<61>(s[22] ^ D2);
-        uint64_t B19 = rotl64<56>(s[23] ^ D3);
-        uint64_t B4 = rotl64<14>(s[24] ^ D4);
+        uint32_t Cl[5], Ch[5], Rl[5], Rh[5];
+#pragma unroll
+        for (int x = 0; x < 5; x++) {
+            Cl[x] = xor3(xor3(l[x], l[x + 5], l[x + 10]), l[x + 15], l[x + 20]);
+            Ch[x] = xor3(xor3(h[x], h[x + 5], h[x + 10]), h[x + 15], h[x + 20]);
+        }
+#pragma unroll
+        for (int x = 0; x < 5; x++) rotl<1>(Ch[x], Cl[x], Rh[x], Rl[x]);
+        uint32_t bl[25], bh[25];
+        // theta + rho + pi: B[y, 2x+3y] = rot(A[x,y] ^ D[x], r[x,y]), lane index x + 5y
+        LVM_KB(0, 0, 0);
+        LVM_KB(10, 1, 1);
+        LVM_KB(20, 2, 62);
+        LVM_KB(5, 3, 28);
+        LVM_KB(15, 4, 27);
+        LVM_KB(16, 5, 36);
+        LVM_KB(1, 6, 44);
+        LVM_KB(11, 7, 6);
+        LVM_KB(21, 8, 55);
+        LVM_KB(6, 9, 20);
+        LVM_KB(7, 10, 3);
+        LVM_KB(17, 11, 10);
+        LVM_KB(2, 12, 43);
+        LVM_KB(12, 13, 25);
+        LVM_KB(22, 14, 39);
+        LVM_KB(23, 15, 41);
+        LVM_KB(8, 16, 45);
+        LVM_KB(18, 17, 15);
+        LVM_KB(3, 18, 21);
+        LVM_KB(13, 19, 8);
+        LVM_KB(14, 20, 18);
+        LVM_KB(24, 21, 2);
+        LVM_KB(9, 22, 61);
+        LVM_KB(19, 23, 56);
+        LVM_KB(4, 24, 14);
         // chi
-        s[0] = B0 ^ (~B1 & B2);
-        s[1] = B1 ^ (~B2 & B3);
-        s[2] = B2 ^ (~B3 & B4);
-        s[3] = B3 ^ (~B4 & B0);
-        s[4] = B4 ^ (~B0 & B1);
-        s[5] = B5 ^ (~B6 & B7);
-        s[6] = B6 ^ (~B7 & B8);
-        s[7] = B7 ^ (~B8 & B9);
-        s[8] = B8 ^ (~B9 & B5);
-        s[9] = B9 ^ (~B5 & B6);
-        s[10] = B10 ^ (~B11 & B12);
-        s[11] = B11 ^ (~B12 & B13);
-        s[12] = B12 ^ (~B13 & B14);
-        s[13] = B13 ^ (~B14 & B10);
-        s[14] = B14 ^ (~B10 & B11);
-        s[15] = B15 ^ (~B16 & B17);
-        s[16] = B16 ^ (~B17 & B18);
-        s[17] = B17 ^ (~B18 & B19);
-        s[18] = B18 ^ (~B19 & B15);
-        s[19] = B19 ^ (~B15 & B16);
-        s[20] = B20 ^ (~B21 & B22);
-        s[21] = B21 ^ (~B22 & B23);
-        s[22] = B22 ^ (~B23 & B24);
-        s[23] = B23 ^ (~B24 & B20);
-        s[24] = B24 ^ (~B20 & B21);
-        s[0] ^= KECCAK_RC[round];
+#pragma unroll
+        for (int y = 0; y < 25; y += 5) {
+#pragma unroll
+            for (int x = 0; x < 5; x++) {
+                l[y + x] = chi(bl[y + x], bl[y + (x + 1) % 5], bl[y + (x + 2) % 5]);
+                h[y + x] = chi(bh[y + x], bh[y + (x + 1) % 5], bh[y + (x + 2) % 5]);
+            }
+        }
+        // iota
+        l[0] ^= KECCAK_RC_LO[round];
+        h[0] ^= KECCAK_RC_HI[round];
     }
+}
+#undef LVM_KB
+
+// 64-bit-lane interface over the 32-bit-half permutation
+__device__ __forceinline__ void keccak_f1600(uint64_t s[25]) {
+    uint32_t l[25], h[25];
+#pragma unroll
+    for (int i = 0; i < 25; i++) {
+        l[i] = (uint32_t)s[i];
+        h[i] = (uint32_t)(s[i] >> 32);
+    }
+    keccak_f1600_32(l, h);
+#pragma unroll
+    for (int i = 0; i < 25; i++) s[i] = ((uint64_t)h[i] << 32) | l[i];
 }
 
 // Keccak-256 of exactly two 32-byte nodes (one permutation): lanes 0..7 hold
 // left || right, pad byte 0x01 at lane 8, 0x80 at the top of lane 16.
-__device__ __forceinline__ void keccak_pair_node(const uint64_t l[4], const uint64_t r[4], uint64_t out[4]) {
-    uint64_t s[25];
+__device__ __forceinline__ void keccak_pair_node(const uint64_t lft[4], const uint64_t rgt[4], uint64_t out[4]) {
+    uint32_t l[25], h[25];
 #pragma unroll
-    for (int i = 0; i < 25; i++) s[i] = 0;
-    s[0] = l[0]; s[1] = l[1]; s[2] = l[2]; s[3] = l[3];
-    s[4] = r[0]; s[5] = r[1]; s[6] = r[2]; s[7] = r[3];
-    s[8] = 0x01;
-    s[16] = 0x8000000000000000ULL;
-    keccak_f1600(s);
-    out[0] = s[0]; out[1] = s[1]; out[2] = s[2]; out[3] = s[3];
+    for (int i = 0; i < 25; i++) l[i] = h[i] = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        l[i] = (uint32_t)lft[i];
+        h[i] = (uint32_t)(lft[i] >> 32);
+        l[4 + i] = (uint32_t)rgt[i];
+        h[4 + i] = (uint32_t)(rgt[i] >> 32);
+    }
+    l[8] = 0x01;
+    h[16] = 0x80000000u;
+    keccak_f1600_32(l, h);
+#pragma unroll
+    for (int i = 0; i < 4; i++) out[i] = ((uint64_t)h[i] << 32) | l[i];
 }
 
 }  // namespace lvm

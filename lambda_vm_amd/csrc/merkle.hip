@@ -16,8 +16,19 @@
 
 namespace lvm {
 
+// XOR the big-endian encoding of a canonicalised element into a lane held as
+// 32-bit halves: the little-endian lane of BE(v) is bswap64(v).
+__device__ __forceinline__ void absorb_be(uint32_t &l, uint32_t &h, uint64_t v) {
+    v = canon(v);
+    l ^= __builtin_bswap32((uint32_t)(v >> 32));
+    h ^= __builtin_bswap32((uint32_t)v);
+}
+
 // One thread per leaf. Words [w0, w1) of rows 2i and 2i+1 (row stride
-// `pitch` words) are absorbed in order, 17 lanes per permutation.
+// `pitch` words) are absorbed in order, 17 lanes per permutation. CONTIG:
+// the committed words are whole rows, so the leaf's data is one contiguous
+// run of 2*pitch words.
+template <bool CONTIG>
 __global__ __launch_bounds__(256) void keccak_row_pair_leaves_kernel(const uint64_t *__restrict__ rows,
                                                                     uint64_t pitch, uint64_t w0, uint64_t w1,
                                                                     uint64_t num_leaves,
@@ -28,10 +39,9 @@ __global__ __launch_bounds__(256) void keccak_row_pair_leaves_kernel(const uint6
     const uint64_t total = 2 * width;
     const uint64_t *r0 = rows + (2 * i) * pitch + w0;
     const uint64_t *r1 = rows + (2 * i + 1) * pitch + w0;
-    uint64_t s[25];
+    uint32_t l[25], h[25];
 #pragma unroll
-    for (int k = 0; k < 25; k++) s[k] = 0;
-    // (row, word) cursor advanced lane by lane
+    for (int k = 0; k < 25; k++) l[k] = h[k] = 0;
     const uint64_t *cur = r0;
     uint64_t left_in_row = width;
     bool second = false;
@@ -39,39 +49,39 @@ __global__ __launch_bounds__(256) void keccak_row_pair_leaves_kernel(const uint6
     while (e + 17 <= total) {
 #pragma unroll
         for (int j = 0; j < 17; j++) {
-            s[j] ^= bswap64(canon(*cur));
-            cur++;
-            if (--left_in_row == 0 && !second) {
-                second = true;
-                cur = r1;
-                left_in_row = width;
+            absorb_be(l[j], h[j], *cur++);
+            if (!CONTIG) {
+                if (--left_in_row == 0 && !second) {
+                    second = true;
+                    cur = r1;
+                    left_in_row = width;
+                }
             }
         }
-        keccak_f1600(s);
+        keccak_f1600_32(l, h);
         e += 17;
     }
     const uint64_t rem = total - e;
 #pragma unroll
     for (int j = 0; j < 17; j++) {
         if ((uint64_t)j < rem) {
-            s[j] ^= bswap64(canon(*cur));
-            cur++;
-            if (--left_in_row == 0 && !second) {
-                second = true;
-                cur = r1;
-                left_in_row = width;
+            absorb_be(l[j], h[j], *cur++);
+            if (!CONTIG) {
+                if (--left_in_row == 0 && !second) {
+                    second = true;
+                    cur = r1;
+                    left_in_row = width;
+                }
             }
         } else if ((uint64_t)j == rem) {
-            s[j] ^= 0x01;
+            l[j] ^= 0x01;
         }
     }
-    s[16] ^= 0x8000000000000000ULL;
-    keccak_f1600(s);
+    h[16] ^= 0x80000000u;
+    keccak_f1600_32(l, h);
     uint64_t *o = leaves + 4 * i;
-    o[0] = s[0];
-    o[1] = s[1];
-    o[2] = s[2];
-    o[3] = s[3];
+#pragma unroll
+    for (int k = 0; k < 4; k++) o[k] = ((uint64_t)h[k] << 32) | l[k];
 }
 
 // FRI layer leaves: Keccak(BE24(e[2j]) || BE24(e[2j+1])), 6 lanes, one permutation
@@ -81,20 +91,18 @@ __global__ __launch_bounds__(256) void keccak_fp3_pair_leaves_kernel(const uint6
                                                                     uint64_t *__restrict__ leaves) {
     const uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= num_leaves) return;
-    uint64_t s[25];
+    uint32_t l[25], h[25];
 #pragma unroll
-    for (int k = 0; k < 25; k++) s[k] = 0;
+    for (int k = 0; k < 25; k++) l[k] = h[k] = 0;
     const uint64_t *p = evals + 6 * j;
 #pragma unroll
-    for (int k = 0; k < 6; k++) s[k] = bswap64(canon(p[k]));
-    s[6] = 0x01;
-    s[16] = 0x8000000000000000ULL;
-    keccak_f1600(s);
+    for (int k = 0; k < 6; k++) absorb_be(l[k], h[k], p[k]);
+    l[6] = 0x01;
+    h[16] = 0x80000000u;
+    keccak_f1600_32(l, h);
     uint64_t *o = leaves + 4 * j;
-    o[0] = s[0];
-    o[1] = s[1];
-    o[2] = s[2];
-    o[3] = s[3];
+#pragma unroll
+    for (int k = 0; k < 4; k++) o[k] = ((uint64_t)h[k] << 32) | l[k];
 }
 
 // One level of parents [begin, begin+count).
@@ -146,8 +154,12 @@ void keccak_row_pair_leaves(const uint64_t *rows, uint64_t pitch, uint64_t w0, u
                             uint8_t *leaves_out, hipStream_t st) {
     if (num_leaves == 0) return;
     dim3 grid((unsigned)((num_leaves + 255) / 256));
-    hipLaunchKernelGGL(keccak_row_pair_leaves_kernel, grid, dim3(256), 0, st, rows, pitch, w0, w1, num_leaves,
-                       (uint64_t *)leaves_out);
+    if (w0 == 0 && w1 == pitch)
+        hipLaunchKernelGGL(keccak_row_pair_leaves_kernel<true>, grid, dim3(256), 0, st, rows, pitch, w0, w1,
+                           num_leaves, (uint64_t *)leaves_out);
+    else
+        hipLaunchKernelGGL(keccak_row_pair_leaves_kernel<false>, grid, dim3(256), 0, st, rows, pitch, w0, w1,
+                           num_leaves, (uint64_t *)leaves_out);
 }
 
 void keccak_fp3_pair_leaves(const uint64_t *evals, uint64_t len, uint8_t *leaves_out, hipStream_t st) {

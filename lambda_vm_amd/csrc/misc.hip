@@ -93,6 +93,22 @@ void launch_open(const uint64_t *nodes, uint64_t L, const uint64_t *lde, uint64_
                        count, depth, paths, rows);
 }
 
+// Plain 16-byte-vector device copy (grid-stride), the calibration kernel for
+// HBM traffic counters.
+__global__ void copy16_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n) {
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) dst[i] = src[i];
+}
+
+void launch_copy16(void *dst, const void *src, uint64_t n16, hipStream_t st) {
+    if (!n16) return;
+    uint64_t blocks = (n16 + 255) / 256;
+    if (blocks > 256 * 64) blocks = 256 * 64;
+    hipLaunchKernelGGL(copy16_kernel, dim3((unsigned)blocks), dim3(256), 0, st, (const uint4 *)src, (uint4 *)dst,
+                       n16);
+}
+
 // w_i = scale * offset^i (the coset weights n^-1 * g^i of
 // crypto/stark/src/prover.rs:451-467 with scale = n^-1)
 __global__ void coset_weights_kernel(uint64_t n, uint64_t offset, uint64_t scale, uint64_t *out) {

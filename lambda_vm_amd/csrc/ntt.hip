@@ -26,6 +26,7 @@
 //    coset shift that turns one size-N transform into B size-n ones);
 //  - output rows may be offset per coset by bitrev_B(coset)*n.
 #include "lvm_internal.hpp"
+#include "prof.hpp"
 
 namespace lvm {
 
@@ -188,7 +189,8 @@ static void launch_one(const PassArgs &a, uint32_t cosets, hipStream_t st) {
     hipLaunchKernelGGL((ntt_pass_kernel<A, B>), grid, dim3(256), 0, st, a);
 }
 
-static void launch_pass(int r, const PassArgs &a, uint32_t cosets, hipStream_t st) {
+static void launch_pass(int r, const PassArgs &a, uint32_t cosets, hipStream_t st, const char *name) {
+    ProfScope ps(name, st);
     switch (r) {
     case 1: launch_one<1, 0>(a, cosets, st); break;
     case 2: launch_one<1, 1>(a, cosets, st); break;
@@ -216,6 +218,9 @@ static int plan_passes(unsigned log_n, int *r) {
 
 // In-place DIF passes over `rows` rows of `buf` (independent size-2^log_n
 // problems stacked), starting at pass index `first`.
+static const char *PASS_NAMES[2][4] = {{"ntt_fwd_pass0", "ntt_fwd_pass1", "ntt_fwd_pass2", "ntt_fwd_pass3"},
+                                       {"ntt_inv_pass0", "ntt_inv_pass1", "ntt_inv_pass2", "ntt_inv_pass3"}};
+
 static void run_passes(uint64_t *buf, uint32_t pitch, uint32_t m, size_t rows, unsigned log_n, int first,
                        const int *r, int np, const Tables &tb, bool inverse, hipStream_t st) {
     unsigned done = 0;
@@ -232,7 +237,7 @@ static void run_passes(uint64_t *buf, uint32_t pitch, uint32_t m, size_t rows, u
         a.tw_lo = inverse ? tb.inv_lo : tb.fwd_lo;
         a.tw_hi = inverse ? tb.inv_hi : tb.fwd_hi;
         a.tw_r = inverse ? tb.inv_r : tb.fwd_r;
-        launch_pass(r[p], a, 1, st);
+        launch_pass(r[p], a, 1, st, PASS_NAMES[inverse][p < 3 ? p : 3]);
         done += r[p];
     }
 }
@@ -259,7 +264,7 @@ void ntt_inverse_rows(const uint64_t *in, uint32_t in_pitch, uint64_t *out, uint
     a.tw_lo = tb.inv_lo;
     a.tw_hi = tb.inv_hi;
     a.tw_r = tb.inv_r;
-    launch_pass(r[0], a, 1, st);
+    launch_pass(r[0], a, 1, st, PASS_NAMES[1][0]);
     run_passes(out, out_pitch, m, 1ull << log_n, log_n, 1, r, np, tb, true, st);
 }
 
@@ -284,7 +289,7 @@ void ntt_forward_rows(const uint64_t *in, uint32_t in_pitch, uint64_t *out, uint
     a.tw_lo = tb.fwd_lo;
     a.tw_hi = tb.fwd_hi;
     a.tw_r = tb.fwd_r;
-    launch_pass(r[0], a, 1, st);
+    launch_pass(r[0], a, 1, st, PASS_NAMES[0][0]);
     run_passes(out, out_pitch, m, 1ull << log_n, log_n, 1, r, np, tb, false, st);
 }
 
@@ -319,10 +324,9 @@ void ntt_coset_expand(const uint64_t *coef, uint32_t coef_pitch, uint64_t *lde, 
     if (np == 0) {
         // n = 1: every evaluation equals the (weighted, shifted) constant
         a.log_msub = 0;
-        launch_pass(1, a, B, st);  // never reached for n >= 2; kept for completeness
         return;
     }
-    launch_pass(r[0], a, B, st);
+    launch_pass(r[0], a, B, st, "ntt_coset_pass0");
     run_passes(lde, lde_pitch, m, (size_t)B << log_n, log_n, 1, r, np, tb, false, st);
 }
 
