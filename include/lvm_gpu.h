@@ -132,6 +132,83 @@ int lvm_coset_weights(size_t n, uint64_t offset, uint64_t *dev_out);
  * (2L-1) x 32 bytes. */
 int lvm_build_merkle_tree(const uint8_t *leaves, size_t num_leaves, uint8_t *nodes_out);
 
+/* ---------------- batch inverse / barycentric OOD / DEEP ---------------- */
+/* crypto/math-cuda/src/inverse.rs:53 `batch_inverse_ext3` (host arrays) and
+ * :157 `batch_inverse_ext3_dev` (device arrays, in != out): elementwise Fp3
+ * inverses by Montgomery's trick (crypto/math/src/field/element.rs:64-111). */
+int lvm_batch_inverse_ext3(const uint64_t *in, uint64_t *out, size_t n);
+int lvm_batch_inverse_ext3_dev(const uint64_t *dev_in, uint64_t *dev_out, size_t n);
+/* crypto/math-cuda/src/barycentric.rs:379 `barycentric_base_multi_on_device`:
+ * every column (word) of the resident table evaluated at K <= 8 Fp3 points by
+ * the barycentric formula of crypto/stark/src/trace.rs:831-1110 over the
+ * trace sub-coset offset*<w_n>; out: K x words_per_row Fp3 (point-major). For
+ * an Fp3 table the three words of a column give its components c0, c1, c2
+ * with value c0 + c1*w + c2*w^2. */
+int lvm_barycentric_base_multi(const lvm_lde *h, uint64_t coset_offset, const uint64_t *z, size_t K, uint64_t *out);
+/* crypto/math-cuda/src/deep.rs:316 `deep_composition_ext3_fully_resident_keep`
+ * (formula of crypto/stark/src/prover.rs:2400-2604): the DEEP codeword over
+ * the LDE coset of `main` (base columns) plus optional `aux` (Fp3 columns)
+ * and `parts` (composition parts, Fp3 columns) tables on the same domain,
+ * with K <= 4 trace poles z_points and pole z_parts (= z^P) for the parts.
+ * trace_ood / gammas_tr: (main cols + aux cols) x K Fp3 (column-major,
+ * (j*K + k)); parts_ood / gammas_h: parts Fp3. dev_out (device, N x 3 u64)
+ * receives the codeword in bit-reversed order, the order FRI consumes. */
+int lvm_deep_composition(const lvm_lde *main, const lvm_lde *aux, const lvm_lde *parts, uint64_t coset_offset,
+                         const uint64_t *z_points, size_t K, const uint64_t *z_parts, const uint64_t *trace_ood,
+                         const uint64_t *parts_ood, const uint64_t *gammas_tr, const uint64_t *gammas_h,
+                         uint64_t *dev_out);
+
+/* ---------------- Fiat-Shamir transcript (host) ----------------
+ * crypto/crypto/src/fiat_shamir/default_transcript.rs:31-146
+ * (`DefaultTranscript`): Keccak-256 duplex, sample() = finalize + reverse +
+ * re-absorb, 8-byte big-endian candidates, rejection sampling. Host-side so
+ * that every challenge can follow the root the device just produced. */
+typedef struct lvm_transcript lvm_transcript;
+int lvm_transcript_new(const uint8_t *data, size_t len, lvm_transcript **out);
+int lvm_transcript_clone(const lvm_transcript *t, lvm_transcript **out);
+int lvm_transcript_release(lvm_transcript *t);
+int lvm_transcript_append_bytes(lvm_transcript *t, const uint8_t *data, size_t len);
+int lvm_transcript_append_base(lvm_transcript *t, uint64_t v);
+int lvm_transcript_append_fp3(lvm_transcript *t, const uint64_t v[3]);
+int lvm_transcript_sample(lvm_transcript *t, uint8_t out[32]);
+int lvm_transcript_state(const lvm_transcript *t, uint8_t out[32]);
+int lvm_transcript_sample_base(lvm_transcript *t, uint64_t *out);
+int lvm_transcript_sample_fp3(lvm_transcript *t, uint64_t out[3]);
+int lvm_transcript_sample_u64(lvm_transcript *t, uint64_t upper_bound, uint64_t *out);
+
+/* ---------------- FRI ----------------
+ * crypto/math-cuda/src/fri.rs:64/:87 `FriCommitState::new`/`new_dev`: the
+ * Fp3 codeword (len elements, bit-reversed order, 3 u64 each) on the coset
+ * offset*<w_len>; evals_on_device selects a host or device source pointer. */
+typedef struct lvm_fri lvm_fri;
+int lvm_fri_new(const uint64_t *evals, size_t len, uint64_t coset_offset, int evals_on_device, lvm_fri **out);
+/* fri.rs:110 `fold_and_commit_layer`: fold with zeta
+ * (crypto/stark/src/fri/fri_functions.rs:15-29), hash the pair leaves,
+ * build the layer tree, return its root; the layer stays on the device. */
+int lvm_fri_fold_and_commit_layer(lvm_fri *f, const uint64_t zeta[3], uint8_t root_out[32]);
+/* the final fold (no commitment) */
+int lvm_fri_fold(lvm_fri *f, const uint64_t zeta[3]);
+/* current codeword (host copy when host_out non-null), its length and coset offset */
+int lvm_fri_current(const lvm_fri *f, uint64_t *host_out, size_t *len, uint64_t *offset);
+int lvm_fri_num_layers(const lvm_fri *f, size_t *count);
+/* committed layer `layer`: evaluations (len x 3 u64) and its (2L-1) x 32 node buffer */
+int lvm_fri_layer(const lvm_fri *f, size_t layer, uint64_t *evals_out, uint8_t *nodes_out, size_t *len);
+/* crypto/stark/src/fri/mod.rs:153-205 `query_phase` on the resident layers:
+ * per query and layer the symmetric evaluation (count x layers x 3 u64,
+ * layer-major) and the authentication path (layer-major, depth_l x 32 bytes
+ * per query). */
+int lvm_fri_query(const lvm_fri *f, const uint64_t *iotas, size_t count, uint64_t *evals_sym_out, uint8_t *paths_out);
+int lvm_fri_release(lvm_fri *f);
+/* crypto/stark/src/fri/mod.rs:27-151 `commit_phase_from_evaluations` on the
+ * device codeword held by `f`, with the transcript on the host
+ * (crypto/stark/src/gpu_lde.rs:3164-3289 `fri_commit_gpu_drive`): layout per
+ * crypto/stark/src/fri/terminal.rs:45-55, roots (num_committed x 32 bytes)
+ * and the 2^effective_k terminal coefficients (Fp3) are written out and
+ * absorbed. On a device error the transcript is restored to its state at
+ * entry (gpu_lde.rs:3196-3229) and a non-zero status is returned. */
+int lvm_fri_commit_phase(lvm_fri *f, lvm_transcript *t, unsigned blowup_log, unsigned k, uint8_t *roots_out,
+                         size_t *num_committed_out, uint64_t *coeffs_out, size_t *num_coeffs_out);
+
 /* ---------------- profiling ---------------- */
 /* When enabled, the library brackets its kernels with HIP events on the
  * stream they run on; query returns the count and summed duration of the

@@ -61,6 +61,31 @@ SIGNATURES = {
     "lvm_lde_open": (C.c_int, [C.c_void_p, u64p, C.c_size_t, u8p, u64p]),
     "lvm_coset_weights": (C.c_int, [C.c_size_t, C.c_uint64, C.c_void_p]),
     "lvm_build_merkle_tree": (C.c_int, [u8p, C.c_size_t, u8p]),
+    "lvm_batch_inverse_ext3": (C.c_int, [u64p, u64p, C.c_size_t]),
+    "lvm_batch_inverse_ext3_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "lvm_barycentric_base_multi": (C.c_int, [C.c_void_p, C.c_uint64, u64p, C.c_size_t, u64p]),
+    "lvm_deep_composition": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, u64p, C.c_size_t, u64p,
+                                       u64p, u64p, u64p, u64p, C.c_void_p]),
+    "lvm_transcript_new": (C.c_int, [u8p, C.c_size_t, vpp]),
+    "lvm_transcript_clone": (C.c_int, [C.c_void_p, vpp]),
+    "lvm_transcript_release": (C.c_int, [C.c_void_p]),
+    "lvm_transcript_append_bytes": (C.c_int, [C.c_void_p, u8p, C.c_size_t]),
+    "lvm_transcript_append_base": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "lvm_transcript_append_fp3": (C.c_int, [C.c_void_p, u64p]),
+    "lvm_transcript_sample": (C.c_int, [C.c_void_p, u8p]),
+    "lvm_transcript_state": (C.c_int, [C.c_void_p, u8p]),
+    "lvm_transcript_sample_base": (C.c_int, [C.c_void_p, u64p]),
+    "lvm_transcript_sample_fp3": (C.c_int, [C.c_void_p, u64p]),
+    "lvm_transcript_sample_u64": (C.c_int, [C.c_void_p, C.c_uint64, u64p]),
+    "lvm_fri_new": (C.c_int, [C.c_void_p, C.c_size_t, C.c_uint64, C.c_int, vpp]),
+    "lvm_fri_fold_and_commit_layer": (C.c_int, [C.c_void_p, u64p, u8p]),
+    "lvm_fri_fold": (C.c_int, [C.c_void_p, u64p]),
+    "lvm_fri_current": (C.c_int, [C.c_void_p, u64p, szp, u64p]),
+    "lvm_fri_num_layers": (C.c_int, [C.c_void_p, szp]),
+    "lvm_fri_layer": (C.c_int, [C.c_void_p, C.c_size_t, u64p, u8p, szp]),
+    "lvm_fri_query": (C.c_int, [C.c_void_p, u64p, C.c_size_t, u64p, u8p]),
+    "lvm_fri_release": (C.c_int, [C.c_void_p]),
+    "lvm_fri_commit_phase": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint, C.c_uint, u8p, szp, u64p, szp]),
     "lvm_prof_enable": (C.c_int, [C.c_int]),
     "lvm_prof_reset": (C.c_int, []),
     "lvm_prof_query": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),

@@ -232,6 +232,194 @@ def build_merkle_tree(leaves) -> np.ndarray:
     return out
 
 
+def batch_inverse_ext3(a) -> np.ndarray:
+    """Elementwise Fp3 inverses (crypto/math-cuda/src/inverse.rs:53)."""
+    N.ensure_device()
+    x = _u64(a).reshape(-1, 3)
+    out = np.empty_like(x)
+    N.call("lvm_batch_inverse_ext3", _p64(x), _p64(out), x.shape[0])
+    return out
+
+
+def barycentric_base_multi(h: GpuLde, coset_offset: int, z_points) -> np.ndarray:
+    """(K, words_per_row, 3): every column of the resident table at each point
+    (crypto/math-cuda/src/barycentric.rs:379)."""
+    z = _u64(z_points).reshape(-1, 3)
+    out = np.zeros((z.shape[0], h.words_per_row, 3), dtype=np.uint64)
+    N.call("lvm_barycentric_base_multi", h.h, coset_offset, _p64(z), z.shape[0], _p64(out))
+    return out
+
+
+def deep_composition(main: GpuLde, coset_offset: int, z_points, trace_ood, gammas_tr, aux: GpuLde | None = None,
+                     parts: GpuLde | None = None, z_parts=None, parts_ood=None, gammas_h=None) -> DeviceBuffer:
+    """DEEP codeword (bit-reversed order, N Fp3) in a new device buffer
+    (crypto/math-cuda/src/deep.rs:316)."""
+    zp = _u64(z_points).reshape(-1, 3)
+    tood = _u64(trace_ood).reshape(-1, 3)
+    gtr = _u64(gammas_tr).reshape(-1, 3)
+    out = DeviceBuffer(main.lde_rows * 24)
+    zpa = _u64(z_parts).reshape(3) if z_parts is not None else None
+    pood = _u64(parts_ood).reshape(-1, 3) if parts_ood is not None else None
+    gh = _u64(gammas_h).reshape(-1, 3) if gammas_h is not None else None
+    N.call("lvm_deep_composition", main.h, aux.h if aux else None, parts.h if parts else None, coset_offset,
+           _p64(zp), zp.shape[0], _p64(zpa) if zpa is not None else None, _p64(tood),
+           _p64(pood) if pood is not None else None, _p64(gtr), _p64(gh) if gh is not None else None,
+           C.c_void_p(out.ptr))
+    return out
+
+
+class Transcript:
+    """DefaultTranscript (crypto/crypto/src/fiat_shamir/default_transcript.rs:
+    31-146) implemented in the library's host code."""
+
+    def __init__(self, data: bytes = b"", _handle=None):
+        if _handle is not None:
+            self.h = _handle
+            return
+        buf = np.frombuffer(bytes(data) + b"\0", dtype=np.uint8).copy()
+        h = C.c_void_p()
+        N.call("lvm_transcript_new", _p8(buf), len(data), C.byref(h))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h is not None and self.h.value:
+                N.lib().lvm_transcript_release(self.h)
+        except Exception:
+            pass
+
+    def clone(self) -> "Transcript":
+        h = C.c_void_p()
+        N.call("lvm_transcript_clone", self.h, C.byref(h))
+        return Transcript(_handle=h)
+
+    def append_bytes(self, data: bytes) -> None:
+        buf = np.frombuffer(bytes(data) + b"\0", dtype=np.uint8).copy()
+        N.call("lvm_transcript_append_bytes", self.h, _p8(buf), len(data))
+
+    def append_base(self, v: int) -> None:
+        N.call("lvm_transcript_append_base", self.h, int(v))
+
+    def append_fp3(self, v) -> None:
+        a = _u64(v)
+        N.call("lvm_transcript_append_fp3", self.h, _p64(a))
+
+    def sample(self) -> bytes:
+        out = np.zeros(32, dtype=np.uint8)
+        N.call("lvm_transcript_sample", self.h, _p8(out))
+        return out.tobytes()
+
+    def state(self) -> bytes:
+        out = np.zeros(32, dtype=np.uint8)
+        N.call("lvm_transcript_state", self.h, _p8(out))
+        return out.tobytes()
+
+    def sample_base(self) -> int:
+        v = C.c_uint64()
+        N.call("lvm_transcript_sample_base", self.h, C.byref(v))
+        return v.value
+
+    def sample_fp3(self) -> np.ndarray:
+        out = np.zeros(3, dtype=np.uint64)
+        N.call("lvm_transcript_sample_fp3", self.h, _p64(out))
+        return out
+
+    def sample_u64(self, ub: int) -> int:
+        v = C.c_uint64()
+        N.call("lvm_transcript_sample_u64", self.h, ub, C.byref(v))
+        return v.value
+
+
+class GpuFri:
+    """Device-resident FRI state (FriCommitState, crypto/math-cuda/src/fri.rs)."""
+
+    def __init__(self, evals, coset_offset: int, on_device: DeviceBuffer | None = None, length: int | None = None):
+        N.ensure_device()
+        h = C.c_void_p()
+        if on_device is not None:
+            N.call("lvm_fri_new", C.c_void_p(on_device.ptr), length, coset_offset, 1, C.byref(h))
+        else:
+            ev = _u64(evals).reshape(-1, 3)
+            N.call("lvm_fri_new", ev.ctypes.data_as(C.c_void_p), ev.shape[0], coset_offset, 0, C.byref(h))
+        self.h = h
+
+    def close(self) -> None:
+        if self.h is not None and self.h.value:
+            N.lib().lvm_fri_release(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def fold_and_commit_layer(self, zeta) -> bytes:
+        z = _u64(zeta)
+        root = np.zeros(32, dtype=np.uint8)
+        N.call("lvm_fri_fold_and_commit_layer", self.h, _p64(z), _p8(root))
+        return root.tobytes()
+
+    def fold(self, zeta) -> None:
+        z = _u64(zeta)
+        N.call("lvm_fri_fold", self.h, _p64(z))
+
+    def current(self):
+        ln, off = C.c_size_t(), C.c_uint64()
+        N.call("lvm_fri_current", self.h, None, C.byref(ln), C.byref(off))
+        out = np.zeros((ln.value, 3), dtype=np.uint64)
+        N.call("lvm_fri_current", self.h, _p64(out), C.byref(ln), C.byref(off))
+        return out, off.value
+
+    def num_layers(self) -> int:
+        c = C.c_size_t()
+        N.call("lvm_fri_num_layers", self.h, C.byref(c))
+        return c.value
+
+    def layer(self, i: int):
+        ln = C.c_size_t()
+        N.call("lvm_fri_layer", self.h, i, None, None, C.byref(ln))
+        ev = np.zeros((ln.value, 3), dtype=np.uint64)
+        L = 1
+        while L < ln.value // 2:
+            L <<= 1
+        nodes = np.zeros((2 * L - 1, 32), dtype=np.uint8)
+        N.call("lvm_fri_layer", self.h, i, _p64(ev), _p8(nodes), C.byref(ln))
+        return ev, nodes
+
+    def query(self, iotas):
+        io = _u64(iotas)
+        nl = self.num_layers()
+        sym = np.zeros((nl, io.size, 3), dtype=np.uint64)
+        depths = []
+        for i in range(nl):
+            ln = C.c_size_t()
+            N.call("lvm_fri_layer", self.h, i, None, None, C.byref(ln))
+            L = 1
+            while L < ln.value // 2:
+                L <<= 1
+            depths.append(L.bit_length() - 1)
+        paths = np.zeros(max(sum(depths) * io.size * 32, 1), dtype=np.uint8)
+        N.call("lvm_fri_query", self.h, _p64(io), io.size, _p64(sym), _p8(paths))
+        out, off = [], 0
+        for d in depths:
+            out.append(paths[off:off + io.size * d * 32].reshape(io.size, d, 32))
+            off += io.size * d * 32
+        return sym, out
+
+    def commit_phase(self, transcript: Transcript, blowup_log: int, k: int):
+        """roots (num_committed, 32) and terminal coefficients (2^k_eff, 3)."""
+        ln, off = C.c_size_t(), C.c_uint64()
+        N.call("lvm_fri_current", self.h, None, C.byref(ln), C.byref(off))
+        lde_log = ln.value.bit_length() - 1
+        roots = np.zeros((max(lde_log, 1), 32), dtype=np.uint8)
+        coeffs = np.zeros((1 << max(lde_log, 0), 3), dtype=np.uint64)
+        nc, ncf = C.c_size_t(), C.c_size_t()
+        N.call("lvm_fri_commit_phase", self.h, transcript.h, blowup_log, k, _p8(roots), C.byref(nc), _p64(coeffs),
+               C.byref(ncf))
+        return roots[: nc.value].copy(), coeffs[: ncf.value].copy()
+
+
 def inject_fault_after(calls: int) -> None:
     N.lib().lvm_inject_fault_after(calls)
 

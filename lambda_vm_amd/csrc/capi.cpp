@@ -22,15 +22,6 @@ void launch_open(const uint64_t *nodes, uint64_t L, const uint64_t *lde, uint64_
 
 using namespace lvm;
 
-struct lvm_lde {
-    int device;
-    uint64_t *lde;  // N rows x w words, rows in bit-reversed order
-    size_t n, N, w, blowup;
-    uint8_t *nodes;  // (2L-1) x 32 bytes, root first
-    size_t L;
-    uint8_t root[32];
-};
-
 namespace {
 
 bool is_pow2(size_t x) { return x && !(x & (x - 1)); }

@@ -8,6 +8,16 @@
 #include "field.hpp"
 #include "lvm_gpu.h"
 
+// Resident LDE table + Merkle tree behind the opaque `lvm_lde` handle.
+struct lvm_lde {
+    int device;
+    uint64_t *lde;  // N rows x w words, rows in bit-reversed order
+    size_t n, N, w, blowup;
+    uint8_t *nodes;  // (2L-1) x 32 bytes, root first
+    size_t L;
+    uint8_t root[32];
+};
+
 namespace lvm {
 
 // Device twiddle tables (built once per device by the library).

@@ -391,3 +391,31 @@ def lde_commit(trace: np.ndarray, blowup: int, offset: int, want_lde: bool = Tru
                               _p64(lde) if lde is not None else None, nodes.ctypes.data_as(u8p))
     assert rc == 0, rc
     return lde, nodes
+
+
+def deep_composition(main: np.ndarray, offset: int, z_points, trace_ood, gammas_tr, aux=None, parts=None,
+                     z_parts=None, parts_ood=None, gammas_h=None) -> np.ndarray:
+    """Natural-order DEEP codeword (N, 3). main: (N, m) base; aux/parts:
+    (N, 3*ma) / (N, 3*np) Fp3 row-major."""
+    L = lib()
+    if not hasattr(L.ora_deep_composition, "_declared"):
+        L.ora_deep_composition.restype = C.c_int
+        L.ora_deep_composition.argtypes = [u64p, C.c_size_t, u64p, C.c_size_t, u64p, C.c_size_t, C.c_size_t,
+                                           C.c_uint64, u64p, C.c_size_t, u64p, u64p, u64p, u64p, u64p, u64p]
+        L.ora_deep_composition._declared = True
+    main = np.ascontiguousarray(main, dtype=np.uint64)
+    N, m = main.shape
+    zp = np.ascontiguousarray(z_points, dtype=np.uint64).reshape(-1, 3)
+    K = zp.shape[0]
+    aux_a = np.ascontiguousarray(aux, dtype=np.uint64) if aux is not None else None
+    parts_a = np.ascontiguousarray(parts, dtype=np.uint64) if parts is not None else None
+    ma = aux_a.shape[1] // 3 if aux_a is not None else 0
+    npp = parts_a.shape[1] // 3 if parts_a is not None else 0
+    arr = lambda x: np.ascontiguousarray(x, dtype=np.uint64) if x is not None else None  # noqa: E731
+    tood, gtr, zpa, pood, gh = arr(trace_ood), arr(gammas_tr), arr(z_parts), arr(parts_ood), arr(gammas_h)
+    out = np.zeros((N, 3), dtype=np.uint64)
+    p = lambda x: _p64(x) if x is not None else None  # noqa: E731
+    rc = L.ora_deep_composition(_p64(main), m, p(aux_a), ma, p(parts_a), npp, N, offset, _p64(zp), K, p(zpa),
+                                _p64(tood), p(pood), _p64(gtr), p(gh), _p64(out))
+    assert rc == 0
+    return out

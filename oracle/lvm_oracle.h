@@ -115,6 +115,15 @@ int ora_batch_inverse_base(uint64_t *buf, size_t n);
 int ora_is_valid_nonce(const uint8_t seed[32], uint64_t nonce, uint8_t grinding_factor);
 uint64_t ora_generate_nonce(const uint8_t seed[32], uint8_t grinding_factor, uint64_t limit_search);
 
+/* ---- DEEP composition (crypto/stark/src/prover.rs:2400-2604) ---- */
+/* main: N x m base, aux: N x ma Fp3 (nullable), parts: N x np Fp3 (nullable),
+ * all row-major natural order; trace_ood / gammas_tr: (m+ma) x K Fp3;
+ * parts_ood / gammas_h: np Fp3; out: N Fp3 natural order. */
+int ora_deep_composition(const uint64_t *main, size_t m, const uint64_t *aux, size_t ma, const uint64_t *parts,
+                         size_t np, size_t N, uint64_t offset, const uint64_t *z_points, size_t K,
+                         const uint64_t *z_parts, const uint64_t *trace_ood, const uint64_t *parts_ood,
+                         const uint64_t *gammas_tr, const uint64_t *gammas_h, uint64_t *out);
+
 /* ---- pipelines used by the CPU baseline ---- */
 /* LDE + row-pair commit of a base table: out_lde (nullable) row-major natural. */
 int ora_lde_commit(const uint64_t *trace, size_t n, size_t m, size_t blowup, uint64_t offset,

@@ -293,6 +293,38 @@ def test_barycentric_equals_horner(ora):
         assert [int(v) % P for v in got[c]] == want
 
 
+def test_deep_composition_quotient_identity(ora):
+    # deep(x) = sum_k [sum_j g_jk (t_j(x) - t_j(z_k))] / (x - z_k) + sum_j gh_j (H_j(x) - H_j(zp)) / (x - zp):
+    # checked at every point with big-integer arithmetic
+    n, m, B, g = 8, 2, 4, 7
+    N = n * B
+    rng = np.random.default_rng(13)
+    tr = rng.integers(0, P, size=(n, m), dtype=np.uint64)
+    lde = ora.coset_lde_row_major(tr, B, g)
+    parts = rng.integers(0, P, size=(N, 3), dtype=np.uint64)  # one Fp3 part, arbitrary values
+    K = 2
+    zp = rng.integers(0, P, size=(K, 3), dtype=np.uint64)
+    tood = rng.integers(0, P, size=(m * K, 3), dtype=np.uint64)
+    gtr = rng.integers(0, P, size=(m * K, 3), dtype=np.uint64)
+    zpa = rng.integers(0, P, size=3, dtype=np.uint64)
+    pood = rng.integers(0, P, size=(1, 3), dtype=np.uint64)
+    gh = rng.integers(0, P, size=(1, 3), dtype=np.uint64)
+    out = ora.deep_composition(lde, g, zp, tood, gtr, parts=parts, z_parts=zpa, parts_ood=pood, gammas_h=gh)
+    r = KATS["goldilocks"]["root_of_unity_2_32"]
+    wN = pow(r, 1 << (32 - 5), P)
+    for i in range(N):
+        x = [g * pow(wN, i, P) % P, 0, 0]
+        want = pyref.f3mul(pyref.f3mul(gh[0].tolist(), pyref.f3sub(parts[i].tolist(), pood[0].tolist())),
+                           pyref.f3inv(pyref.f3sub(x, zpa.tolist())))
+        for k in range(K):
+            acc = [0, 0, 0]
+            for j in range(m):
+                acc = pyref.f3add(acc, pyref.f3mul(gtr[j * K + k].tolist(),
+                                                   pyref.f3sub([int(lde[i, j]), 0, 0], tood[j * K + k].tolist())))
+            want = pyref.f3add(want, pyref.f3mul(acc, pyref.f3inv(pyref.f3sub(x, zp[k].tolist()))))
+        assert [int(v) % P for v in out[i]] == want
+
+
 def test_batch_inverse(ora):
     rng = np.random.default_rng(12)
     for n in (1, 2, 7, 1000, 70000):
