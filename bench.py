@@ -50,6 +50,8 @@ def algorithmic_bytes(kernel: str, n: int, m: int, blowup: int) -> float:
 
 KERNELS = ["ntt_inv_pass0", "ntt_inv_pass1", "ntt_inv_pass2", "ntt_coset_pass0", "ntt_fwd_pass1",
            "ntt_fwd_pass2", "keccak_leaves", "merkle_inner"]
+# stage groups reported beside the dominant kernel (not single kernels)
+STAGES = ["bary_denoms", "bary_sum", "deep_denoms", "deep_rows", "fri_fold_leaves", "fri_merkle", "fri_fold"]
 
 
 def measured_traffic(kernel: str, n: int, m: int, blowup: int):
@@ -65,9 +67,10 @@ def measured_traffic(kernel: str, n: int, m: int, blowup: int):
     return rec.get("bytes_per_launch") if rec else None
 
 
-def cpu_baseline(log_n: int, m: int, blowup: int, offset: int) -> dict:
+def cpu_baseline(log_n: int, m: int, blowup: int, offset: int, commit_only: bool, fri_k: int, queries: int) -> dict:
     import oracle
     from lambda_vm_amd import workload
+    from oracle.pipeline import replay_single_table
 
     oracle.set_threads(0)
     threads = oracle.get_threads()
@@ -76,7 +79,12 @@ def cpu_baseline(log_n: int, m: int, blowup: int, offset: int) -> dict:
     tr = workload.trace_table(1, 0, n, m)
     oracle.lde_commit(tr[: 1 << 12], blowup, offset, want_lde=False)  # warm the code paths
     t0 = time.perf_counter()
-    oracle.lde_commit(tr, blowup, offset, want_lde=False)
+    if commit_only:
+        oracle.lde_commit(tr, blowup, offset, want_lde=False)
+        what = "LDE+commit"
+    else:
+        replay_single_table(tr, blowup, offset, fri_k, queries, b"lambda-vm-single-table")
+        what = "LDE+commit, OOD, DEEP, FRI and queries"
     dt = time.perf_counter() - t0
     cells = n * blowup * m
     return {
@@ -84,8 +92,8 @@ def cpu_baseline(log_n: int, m: int, blowup: int, offset: int) -> dict:
         "unit": "Gcells/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"one LDE+commit of 2^{sample_log_n} rows x {m} cols, blowup {blowup} "
-                  f"({cells} cells, {dt:.2f} s) by the oracle on {threads} host threads",
+        "sample": f"one {what} of 2^{sample_log_n} rows x {m} cols, blowup {blowup} "
+                  f"({cells} LDE cells, {dt:.2f} s) by the oracle on {threads} host threads",
     }
 
 
@@ -98,6 +106,9 @@ def main() -> int:
     ap.add_argument("--cols", type=int, default=64)
     ap.add_argument("--blowup", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--commit-only", action="store_true", help="time only the LDE + Merkle commitment")
+    ap.add_argument("--fri-k", type=int, default=6, help="log2 terminal degree (terminal 2^(log B + k))")
+    ap.add_argument("--queries", type=int, default=30)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -119,11 +130,17 @@ def main() -> int:
     d_w = backend.coset_weights_dev(n, offset)
     del tr
 
+    from lambda_vm_amd import pipeline
+
     def step_once():
-        h = backend.coset_lde_dev(d_tr, n, m, B, d_w)
-        root = h.root
-        h.close()
-        return root
+        if args.commit_only:
+            h = backend.coset_lde_dev(d_tr, n, m, B, d_w)
+            root = h.root
+            h.close()
+            return root
+        proof = pipeline.prove_single_table(d_tr, n, m, B, offset, d_w, fri_final_log_degree=args.fri_k,
+                                            num_queries=args.queries)
+        return proof.root + proof.transcript_state
 
     # plain copy of 1 GiB (the calibration kernel for the traffic counters);
     # its rate is reported beside the roofline as the achievable HBM rate
@@ -178,6 +195,13 @@ def main() -> int:
         "per_kernel_ms": {k: round(v[1] / max(v[0], 1), 4) for k, v in groups.items()},
         "copy_kernel_GBps": round(copy_gbs, 1),
     }
+    stage_ms = {}
+    for k in STAGES:
+        cnt, tot = backend.prof_query(k)
+        if cnt:
+            stage_ms[k] = round(tot / args.steps, 4)
+    if stage_ms:
+        roofline["other_stage_ms_per_step"] = stage_ms
     traffic = measured_traffic(dom, n, m, B)
     if traffic is not None:
         roofline["traffic"] = traffic
@@ -195,8 +219,11 @@ def main() -> int:
         "dtype": "u64 (Goldilocks field)",
         "data": "synthetic: SplitMix64-seeded trace uniform in [0,p)",
         "config": {
-            "workload": f"single table 2^{args.log_n} rows x {m} cols, blowup {B} coset LDE "
-                        f"(offset {offset}) + row-pair Keccak-256 Merkle tree, trace resident in HBM",
+            "workload": (f"single table 2^{args.log_n} rows x {m} cols, blowup {B} coset LDE (offset {offset}) "
+                         "+ row-pair Keccak-256 Merkle tree, trace resident in HBM"
+                         + ("" if args.commit_only else
+                            f"; then OOD (barycentric, 2 points), DEEP, FRI fold-2 to a 2^{B.bit_length() - 1 + args.fri_k}"
+                            f" terminal and {args.queries} queries with the Fiat-Shamir transcript")),
             "rows": n, "cols": m, "blowup": B, "lde_cells": cells,
             "root": root0.hex() if root0 else None,
         },
@@ -204,7 +231,7 @@ def main() -> int:
     }
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(args.log_n, m, B, offset)
+        out["cpu_baseline"] = cpu_baseline(args.log_n, m, B, offset, args.commit_only, args.fri_k, args.queries)
     print(json.dumps(out), flush=True)
     return 0
 

@@ -179,7 +179,9 @@ int lvm_transcript_sample_u64(lvm_transcript *t, uint64_t upper_bound, uint64_t 
 /* ---------------- FRI ----------------
  * crypto/math-cuda/src/fri.rs:64/:87 `FriCommitState::new`/`new_dev`: the
  * Fp3 codeword (len elements, bit-reversed order, 3 u64 each) on the coset
- * offset*<w_len>; evals_on_device selects a host or device source pointer. */
+ * offset*<w_len>; evals_on_device: 0 host source, 1 device source (copied),
+ * 2 device buffer from lvm_dev_alloc that the FRI state takes over (no copy;
+ * the caller must not free it). */
 typedef struct lvm_fri lvm_fri;
 int lvm_fri_new(const uint64_t *evals, size_t len, uint64_t coset_offset, int evals_on_device, lvm_fri **out);
 /* fri.rs:110 `fold_and_commit_layer`: fold with zeta

@@ -333,11 +333,14 @@ class Transcript:
 class GpuFri:
     """Device-resident FRI state (FriCommitState, crypto/math-cuda/src/fri.rs)."""
 
-    def __init__(self, evals, coset_offset: int, on_device: DeviceBuffer | None = None, length: int | None = None):
+    def __init__(self, evals, coset_offset: int, on_device: DeviceBuffer | None = None, length: int | None = None,
+                 adopt: bool = False):
         N.ensure_device()
         h = C.c_void_p()
         if on_device is not None:
-            N.call("lvm_fri_new", C.c_void_p(on_device.ptr), length, coset_offset, 1, C.byref(h))
+            N.call("lvm_fri_new", C.c_void_p(on_device.ptr), length, coset_offset, 2 if adopt else 1, C.byref(h))
+            if adopt:
+                on_device.ptr = 0  # owned by the FRI state from now on
         else:
             ev = _u64(evals).reshape(-1, 3)
             N.call("lvm_fri_new", ev.ctypes.data_as(C.c_void_p), ev.shape[0], coset_offset, 0, C.byref(h))

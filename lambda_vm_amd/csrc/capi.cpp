@@ -181,7 +181,8 @@ void lvm_inject_fault_after(long calls) { inject_fault_after(calls); }
 int lvm_dev_alloc(void **ptr, size_t bytes) {
     if (!ptr) return fail(LVM_ERR_ARG, "null ptr");
     if (!dev_or_fail()) return LVM_ERR_DEVICE;
-    hipError_t e = hipMalloc(ptr, bytes ? bytes : 8);
+    hipError_t e = lvm_malloc(ptr, bytes);
+    if (e == hipSuccess) e = hipStreamSynchronize(lib_stream());
     if (e != hipSuccess) return fail(LVM_ERR_NOMEM, hipGetErrorString(e));
     return LVM_OK;
 }
@@ -190,8 +191,7 @@ int lvm_dev_free(void *ptr) {
     if (!ptr) return LVM_OK;
     Device *d = dev_or_fail();
     if (!d) return LVM_ERR_DEVICE;
-    LVM_HIP(hipStreamSynchronize(d->stream));
-    LVM_HIP(hipFree(ptr));
+    LVM_HIP(lvm_free(ptr));
     return LVM_OK;
 }
 
@@ -297,10 +297,8 @@ int lvm_coset_lde_dev(const uint64_t *dev_trace, size_t n, size_t m, size_t blow
 
 int lvm_lde_release(lvm_lde *h) {
     if (!h) return LVM_OK;
-    Device *d = dev_or_fail();
-    if (d) hipStreamSynchronize(d->stream);
-    hipFree(h->lde);
-    hipFree(h->nodes);
+    lvm_free(h->lde);
+    lvm_free(h->nodes);
     delete h;
     return LVM_OK;
 }

@@ -88,6 +88,14 @@ Device *device_state() {
         delete d;
         return nullptr;
     }
+    {
+        // keep freed pool memory cached instead of trimming it at every sync
+        hipMemPool_t pool;
+        if (hipDeviceGetDefaultMemPool(&pool, id) == hipSuccess) {
+            uint64_t threshold = UINT64_MAX;
+            hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
+        }
+    }
     if (build_tables(d) != LVM_OK) {
         hipStreamDestroy(d->stream);
         delete d;
@@ -95,6 +103,11 @@ Device *device_state() {
     }
     g_devices[id] = d;
     return d;
+}
+
+hipStream_t lib_stream() {
+    Device *d = device_state();
+    return d ? d->stream : nullptr;
 }
 
 // ---------------- profiler ----------------

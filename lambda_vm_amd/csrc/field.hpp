@@ -172,6 +172,52 @@ LVM_HD fp3 f3_pow(fp3 a, uint64_t e) {
     return r;
 }
 
+// Delayed-reduction accumulator for dot products (the idea of the
+// reference's dot_product_3, goldilocks.rs:289-315, stretched over a whole
+// row of columns): the four 32x32-bit partial products of every 64x64-bit
+// product go into three 64-bit running sums (weights 2^0, 2^32, 2^64), each
+// with a 32-bit count of its wrap-arounds; one reduction at the end. A
+// product costs four v_mad_u64_u32 plus carry bookkeeping, against a full
+// modular multiply-add per product otherwise.
+struct acc128 {
+    uint64_t s0, s1, s2;
+    uint32_t c0, c1, c2;
+};
+
+LVM_HD void acc_zero(acc128 &a) {
+    a.s0 = a.s1 = a.s2 = 0;
+    a.c0 = a.c1 = a.c2 = 0;
+}
+
+LVM_HD void acc_mad(acc128 &a, uint64_t x, uint64_t y) {
+    const uint32_t a0 = (uint32_t)x, a1 = (uint32_t)(x >> 32), b0 = (uint32_t)y, b1 = (uint32_t)(y >> 32);
+    uint64_t t;
+    t = a.s0 + (uint64_t)a0 * b0;
+    a.c0 += t < a.s0;
+    a.s0 = t;
+    t = a.s1 + (uint64_t)a0 * b1;
+    a.c1 += t < a.s1;
+    a.s1 = t;
+    t = a.s1 + (uint64_t)a1 * b0;
+    a.c1 += t < a.s1;
+    a.s1 = t;
+    t = a.s2 + (uint64_t)a1 * b1;
+    a.c2 += t < a.s2;
+    a.s2 = t;
+}
+
+// value = s0 + c0*2^64 + s1*2^32 + c1*2^96 + s2*2^64 + c2*2^128 (mod p), with
+// 2^64 = EPS, 2^96 = -1, 2^128 = EPS^2
+LVM_HD uint64_t acc_reduce(const acc128 &a) {
+    uint64_t r = a.s0;
+    r = add(r, mul((uint64_t)a.c0, EPS));
+    r = add(r, reduce128(a.s1 << 32, a.s1 >> 32));
+    r = sub(r, (uint64_t)a.c1);
+    r = add(r, reduce128(0, a.s2));
+    r = add(r, mul((uint64_t)a.c2, EPS * EPS));
+    return r;
+}
+
 LVM_HD uint64_t bswap64(uint64_t v) {
 #if defined(__HIP_DEVICE_COMPILE__)
     uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);

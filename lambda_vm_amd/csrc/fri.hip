@@ -122,10 +122,10 @@ bool is_pow2(uint64_t x) { return x && !(x & (x - 1)); }
 unsigned log2u(uint64_t x) { return (unsigned)__builtin_ctzll(x); }
 
 void free_fri(lvm_fri *f) {
-    if (f->cur && f->cur_owned) hipFree(f->cur);
+    if (f->cur && f->cur_owned) lvm_free(f->cur);
     for (auto &l : f->layers) {
-        hipFree(l.evals);
-        hipFree(l.nodes);
+        lvm_free(l.evals);
+        lvm_free(l.nodes);
     }
     delete f;
 }
@@ -144,7 +144,13 @@ int lvm_fri_new(const uint64_t *evals, size_t len, uint64_t coset_offset, int ev
     f->device = d->id;
     f->len = len;
     f->offset = coset_offset;
-    if (hipMalloc(&f->cur, len * 24) != hipSuccess) {
+    if (evals_on_device == 2) {
+        // adopt a library allocation (lvm_dev_alloc) without copying it
+        f->cur = (uint64_t *)evals;
+        *out = f;
+        return LVM_OK;
+    }
+    if (lvm_malloc(&f->cur, len * 24) != hipSuccess) {
         delete f;
         return fail(LVM_ERR_NOMEM, "FRI codeword allocation failed");
     }
@@ -169,12 +175,12 @@ static int fri_fold_impl(lvm_fri *f, const uint64_t zeta[3], bool commit, uint8_
     hipStream_t st = d->stream;
     const uint64_t half = f->len / 2;
     uint64_t *out = nullptr;
-    LVM_HIP(hipMalloc(&out, half * 24));
+    LVM_HIP(lvm_malloc(&out, half * 24));
     FriLayer layer;
     if (commit) {
         layer.L = padded_leaves(half / 2);
-        if (hipMalloc(&layer.nodes, (2 * layer.L - 1) * 32) != hipSuccess) {
-            hipFree(out);
+        if (lvm_malloc(&layer.nodes, (2 * layer.L - 1) * 32) != hipSuccess) {
+            lvm_free(out);
             return fail(LVM_ERR_NOMEM, "FRI tree allocation failed");
         }
     }
@@ -195,11 +201,11 @@ static int fri_fold_impl(lvm_fri *f, const uint64_t zeta[3], bool commit, uint8_
     if (e == hipSuccess && commit) e = hipMemcpyAsync(layer.root, layer.nodes, 32, hipMemcpyDeviceToHost, st);
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) {
-        hipFree(out);
-        if (layer.nodes) hipFree(layer.nodes);
+        lvm_free(out);
+        if (layer.nodes) lvm_free(layer.nodes);
         return fail(LVM_ERR_DEVICE, std::string("FRI fold: ") + hipGetErrorString(e));
     }
-    if (f->cur_owned) hipFree(f->cur);
+    if (f->cur_owned) lvm_free(f->cur);
     f->cur = out;
     f->len = half;
     f->offset = sqr(f->offset);
@@ -264,9 +270,9 @@ int lvm_fri_query(const lvm_fri *f, const uint64_t *iotas, size_t count, uint64_
     size_t path_words = 0;
     for (auto &l : f->layers) path_words += count * log2u(l.L) * 4;
     uint64_t *dio = nullptr, *dsym = nullptr, *dpaths = nullptr;
-    LVM_HIP(hipMalloc(&dio, count * 8));
-    LVM_HIP(hipMalloc(&dsym, f->layers.size() * count * 24));
-    LVM_HIP(hipMalloc(&dpaths, (path_words ? path_words : 1) * 8));
+    LVM_HIP(lvm_malloc(&dio, count * 8));
+    LVM_HIP(lvm_malloc(&dsym, f->layers.size() * count * 24));
+    LVM_HIP(lvm_malloc(&dpaths, (path_words ? path_words : 1) * 8));
     LVM_HIP(hipMemcpyAsync(dio, iotas, count * 8, hipMemcpyHostToDevice, st));
     size_t poff = 0;
     for (size_t li = 0; li < f->layers.size(); li++) {
@@ -281,9 +287,9 @@ int lvm_fri_query(const lvm_fri *f, const uint64_t *iotas, size_t count, uint64_
     LVM_HIP(hipMemcpyAsync(evals_sym_out, dsym, f->layers.size() * count * 24, hipMemcpyDeviceToHost, st));
     if (path_words) LVM_HIP(hipMemcpyAsync(paths_out, dpaths, path_words * 8, hipMemcpyDeviceToHost, st));
     LVM_HIP(hipStreamSynchronize(st));
-    hipFree(dio);
-    hipFree(dsym);
-    hipFree(dpaths);
+    lvm_free(dio);
+    lvm_free(dsym);
+    lvm_free(dpaths);
     return LVM_OK;
 }
 

@@ -44,6 +44,16 @@ Device *device_state();  // current device's state, initialised on first use; nu
 // fault injection for the fallback tests (never set in production)
 bool take_injected_fault();
 
+// Device allocations come from the device's stream-ordered memory pool on the
+// library stream (the pool keeps freed blocks for reuse, so steady-state
+// proving does not return to the driver for its multi-GiB buffers).
+hipStream_t lib_stream();
+template <class T>
+inline hipError_t lvm_malloc(T **p, size_t bytes) {
+    return hipMallocAsync((void **)p, bytes ? bytes : 8, lib_stream());
+}
+inline hipError_t lvm_free(void *p) { return p ? hipFreeAsync(p, lib_stream()) : hipSuccess; }
+
 #define LVM_HIP(expr)                                                                                   \
     do {                                                                                                \
         hipError_t _e = (expr);                                                                         \

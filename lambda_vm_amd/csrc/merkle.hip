@@ -144,6 +144,54 @@ __global__ __launch_bounds__(1024) void merkle_tail_kernel(uint64_t *nodes, uint
     }
 }
 
+// Up to 9 tree levels per launch: block b takes the 2*span children
+// [cstart + 2*span*b, ...) of one level (span <= 256 parents), writes their
+// parents, keeps them in LDS and keeps hashing pairs until one node is left,
+// writing every level it produces to the node array.
+__global__ __launch_bounds__(256) void merkle_subtree_kernel(uint64_t *nodes, uint64_t cstart, uint64_t pstart,
+                                                             uint32_t span) {
+    __shared__ uint64_t dig[256][4];
+    const uint32_t t = threadIdx.x;
+    const uint64_t b = blockIdx.x;
+    uint64_t pbase = pstart + (uint64_t)span * b;  // first parent of this block at the current level
+    if (t < span) {
+        const uint64_t *l = nodes + 4 * (cstart + 2 * (uint64_t)span * b + 2 * t);
+        uint64_t lv[4] = {l[0], l[1], l[2], l[3]}, rv[4] = {l[4], l[5], l[6], l[7]}, o[4];
+        keccak_pair_node(lv, rv, o);
+        uint64_t *d = nodes + 4 * (pbase + t);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            d[k] = o[k];
+            dig[t][k] = o[k];
+        }
+    }
+    // next levels: the block's nodes at level start L0 = pbase; the parent of
+    // node x is (x - 1) / 2, and a block's nodes stay contiguous
+    uint32_t cnt = span;
+    while (cnt > 1) {
+        __syncthreads();
+        const uint32_t half = cnt / 2;
+        const uint64_t next = (pbase - 1) / 2;
+        uint64_t o[4];
+        if (t < half) {
+            uint64_t lv[4] = {dig[2 * t][0], dig[2 * t][1], dig[2 * t][2], dig[2 * t][3]};
+            uint64_t rv[4] = {dig[2 * t + 1][0], dig[2 * t + 1][1], dig[2 * t + 1][2], dig[2 * t + 1][3]};
+            keccak_pair_node(lv, rv, o);
+        }
+        __syncthreads();
+        if (t < half) {
+            uint64_t *d = nodes + 4 * (next + t);
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                d[k] = o[k];
+                dig[t][k] = o[k];
+            }
+        }
+        pbase = next;
+        cnt = half;
+    }
+}
+
 size_t padded_leaves(size_t n) {
     size_t p = 1;
     while (p < n) p <<= 1;
@@ -171,16 +219,23 @@ void keccak_fp3_pair_leaves(const uint64_t *evals, uint64_t len, uint8_t *leaves
 
 void merkle_build_inner(uint8_t *nodes_b, uint64_t L, hipStream_t st) {
     uint64_t *nodes = (uint64_t *)nodes_b;
-    uint64_t level_begin = L - 1;
-    // big levels: one launch per level
-    while (level_begin > 2048) {
-        uint64_t nb = level_begin / 2;
-        uint64_t count = level_begin - nb;
-        dim3 grid((unsigned)((count + 255) / 256));
-        hipLaunchKernelGGL(merkle_level_kernel, grid, dim3(256), 0, st, nodes, nb, count);
-        level_begin = nb;
+    // children of the current level: C nodes starting at C - 1 (C a power of two)
+    uint64_t C = L;
+    // wide levels: one fully parallel launch per level
+    while (C > (1u << 14)) {
+        const uint64_t count = C / 2;
+        hipLaunchKernelGGL(merkle_level_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, nodes,
+                           count - 1, count);
+        C = count;
     }
-    if (level_begin > 0) hipLaunchKernelGGL(merkle_tail_kernel, dim3(1), dim3(1024), 0, st, nodes, level_begin);
+    // narrow top: up to 9 levels per launch through LDS
+    while (C > 1) {
+        const uint32_t span = (uint32_t)(C / 2 < 256 ? C / 2 : 256);
+        const uint64_t blocks = C / (2 * span);
+        hipLaunchKernelGGL(merkle_subtree_kernel, dim3((unsigned)blocks), dim3(256), 0, st, nodes, C - 1, C / 2 - 1,
+                           span);
+        C = blocks;
+    }
 }
 
 }  // namespace lvm

@@ -48,6 +48,7 @@ struct PassArgs {
     const uint64_t *shift_lo;      // forward tables for the coset shift
     const uint64_t *shift_hi;
     const uint64_t *tw_r;          // w_256^(+-e), e < 128
+    uint32_t cosets_per_block;     // cosets produced per block (grid.z = cosets / this)
 };
 
 // w_{2^32}^e via the two-level table
@@ -55,7 +56,7 @@ __device__ __forceinline__ uint64_t tw32(const uint64_t *lo, const uint64_t *hi,
     return mul(lo[e & 0xFFFFu], hi[e >> 16]);
 }
 
-template <int LOG_R1, int LOG_R2>
+template <int LOG_R1, int LOG_R2, bool MULTI>
 __global__ __launch_bounds__(256) void ntt_pass_kernel(PassArgs a) {
     constexpr int R1 = 1 << LOG_R1, R2 = 1 << LOG_R2, R = R1 * R2, LOG_R = LOG_R1 + LOG_R2;
     constexpr int C = 16;                 // columns per tile
@@ -76,7 +77,6 @@ __global__ __launch_bounds__(256) void ntt_pass_kernel(PassArgs a) {
     const bool g_ok = g < a.total_groups;
     const uint32_t col = blockIdx.y * C + c;
     const bool c_ok = col < a.m && g_ok;
-    const uint32_t coset = blockIdx.z;
 
     const uint32_t log_s = a.log_s;
     const uint64_t S = 1ull << log_s;
@@ -86,18 +86,10 @@ __global__ __launch_bounds__(256) void ntt_pass_kernel(PassArgs a) {
 
     // sub-transform twiddles w_R^e = w_256^(e*256/R)
     for (int e = tid; e < R / 2; e += 256) s_tw[e] = a.tw_r[e << (8 - LOG_R)];
-    // per-tile pre-multiply and post-twiddle factors, shared by the C columns
+    // per-tile post-twiddle factors, shared by the C columns
     const int tin = tid % (C * TPC);
     for (int t = tin; t < R; t += C * TPC) {
         if (g_ok) {
-            uint64_t pre = 1;
-            uint64_t lrow = row0 + ((uint64_t)t << log_s);
-            if (a.weights) pre = a.weights[lrow];
-            if (a.shift_log) {
-                uint32_t e = (uint32_t)(((uint64_t)coset * lrow) << (32 - a.shift_log));
-                pre = mul(pre, tw32(a.shift_lo, a.shift_hi, e));
-            }
-            s_pre[tl][t] = pre;
             uint64_t post = 1;
             if (log_s) {
                 // position t holds frequency bitrev_R(t); w_M^(jlo*k1)
@@ -108,26 +100,55 @@ __global__ __launch_bounds__(256) void ntt_pass_kernel(PassArgs a) {
             s_post[tl][t] = post;
         }
     }
-    __syncthreads();
 
     uint64_t x[TPC > R1 ? TPC : R1];
+    uint64_t xin[MULTI ? R1 : 1];
     const bool pre_needed = a.weights != nullptr || a.shift_log != 0;
+    auto load_in = [&](int i) -> uint64_t {
+        if (!c_ok) return 0;
+        uint64_t lrow = row0 + ((uint64_t)(tc + R2 * i) << log_s);
+        uint64_t prow = a.in_bitrev_bits ? (uint64_t)bitrev((uint32_t)lrow, a.in_bitrev_bits) : lrow;
+        return a.in[prow * a.in_pitch + col];
+    };
+    // MULTI: the tile's input is read once and serves every coset this block produces
+    if constexpr (MULTI) {
+        if (tc < R2) {
+#pragma unroll
+            for (int i = 0; i < R1; i++) xin[i] = load_in(i);
+        }
+    }
+
+    const uint32_t ncos = MULTI ? a.cosets_per_block : 1u;
+    for (uint32_t ci = 0; ci < ncos; ci++) {
+    const uint32_t coset = blockIdx.z * a.cosets_per_block + ci;
+    if (pre_needed) {
+        // pre-multiply factors weights[j] * w_N^(coset*j) of this coset
+        for (int t = tin; t < R; t += C * TPC) {
+            if (g_ok) {
+                uint64_t pre = 1;
+                uint64_t lrow = row0 + ((uint64_t)t << log_s);
+                if (a.weights) pre = a.weights[lrow];
+                if (a.shift_log) {
+                    uint32_t e = (uint32_t)(((uint64_t)coset * lrow) << (32 - a.shift_log));
+                    pre = mul(pre, tw32(a.shift_lo, a.shift_hi, e));
+                }
+                s_pre[tl][t] = pre;
+            }
+        }
+    }
+    __syncthreads();
 
     // ---- round 1: stages h = R/2 .. R2 on elements t = t0 + R2*i ----
     if (tc < R2) {
         const int t0 = tc;
-        if (c_ok) {
 #pragma unroll
-            for (int i = 0; i < R1; i++) {
-                const int t = t0 + R2 * i;
-                uint64_t lrow = row0 + ((uint64_t)t << log_s);
-                uint64_t prow = a.in_bitrev_bits ? (uint64_t)bitrev((uint32_t)lrow, a.in_bitrev_bits) : lrow;
-                uint64_t v = a.in[prow * a.in_pitch + col];
-                x[i] = pre_needed ? mul(v, s_pre[tl][t]) : v;
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < R1; i++) x[i] = 0;
+        for (int i = 0; i < R1; i++) {
+            uint64_t v;
+            if constexpr (MULTI)
+                v = xin[i];
+            else
+                v = load_in(i);
+            x[i] = pre_needed ? mul(v, s_pre[tl][t0 + R2 * i]) : v;
         }
 #pragma unroll
         for (int H = R1 / 2; H >= 1; H >>= 1) {
@@ -178,15 +199,23 @@ __global__ __launch_bounds__(256) void ntt_pass_kernel(PassArgs a) {
             }
         }
     }
+    __syncthreads();  // the LDS tile and factors are rewritten by the next coset
+    }
 }
 
 // pass shapes: (LOG_R1, LOG_R2) for r = 1..8
 template <int A, int B>
-static void launch_one(const PassArgs &a, uint32_t cosets, hipStream_t st) {
+static void launch_one(const PassArgs &a0, uint32_t cosets, hipStream_t st) {
     constexpr int TPC = (1 << A) > (1 << B) ? (1 << A) : (1 << B);
     constexpr int TILES = 256 / (16 * TPC);
-    dim3 grid((unsigned)((a.total_groups + TILES - 1) / TILES), (a.m + 15) / 16, cosets);
-    hipLaunchKernelGGL((ntt_pass_kernel<A, B>), grid, dim3(256), 0, st, a);
+    PassArgs a = a0;
+    // all cosets of a tile come from one block, which reads the tile once
+    a.cosets_per_block = cosets;
+    dim3 grid((unsigned)((a.total_groups + TILES - 1) / TILES), (a.m + 15) / 16, 1);
+    if (cosets > 1)
+        hipLaunchKernelGGL((ntt_pass_kernel<A, B, true>), grid, dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL((ntt_pass_kernel<A, B, false>), grid, dim3(256), 0, st, a);
 }
 
 static void launch_pass(int r, const PassArgs &a, uint32_t cosets, hipStream_t st, const char *name) {

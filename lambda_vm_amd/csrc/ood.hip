@@ -83,39 +83,57 @@ constexpr int BARY_MAX_K = 8;
 constexpr int BARY_ROWS_PER_BLOCK = 1024;
 
 // partial[chunk][k][c] = sum over the chunk's rows p of lde[p][c] * scale[k][p]
-// block: 256 threads = 4 row lanes x 64 columns; grid (chunks, column groups)
+// block: 256 threads = 4 row lanes x 64 columns; grid (chunks, column groups).
+// The three components of each point's sum are delayed-reduction dot
+// products (one 128-bit accumulate per product, one reduction per thread).
+template <int K>
 __global__ __launch_bounds__(256) void bary_partial_kernel(const uint64_t *__restrict__ lde, uint64_t pitch,
                                                            uint64_t w, uint64_t n, const uint64_t *__restrict__ scale,
-                                                           uint32_t K, uint64_t *__restrict__ partial) {
-    __shared__ uint64_t red[4][BARY_MAX_K][64][3];
+                                                           uint64_t *__restrict__ partial) {
+    __shared__ uint64_t red[4][K][64][3];
     const int lane = threadIdx.x & 63, rl = threadIdx.x >> 6;
     const uint64_t c = (uint64_t)blockIdx.y * 64 + lane;
     const uint64_t p0 = (uint64_t)blockIdx.x * BARY_ROWS_PER_BLOCK;
-    fp3 acc[BARY_MAX_K];
+    acc128 acc[K][3];
 #pragma unroll
-    for (int k = 0; k < BARY_MAX_K; k++) acc[k] = fp3{0, 0, 0};
+    for (int k = 0; k < K; k++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) acc_zero(acc[k][j]);
     if (c < w) {
-        for (uint64_t p = p0 + rl; p < p0 + BARY_ROWS_PER_BLOCK && p < n; p += 4) {
+        const uint64_t pend = p0 + BARY_ROWS_PER_BLOCK < n ? p0 + BARY_ROWS_PER_BLOCK : n;
+        for (uint64_t p = p0 + rl; p < pend; p += 4) {
             uint64_t f = lde[p * pitch + c];
+            // p is the same for the 64 lanes of a wave: read the scales through
+            // the scalar path once per wave instead of 64 identical vector loads
+            const uint64_t pu = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(p >> 32)) << 32) |
+                                __builtin_amdgcn_readfirstlane((uint32_t)p);
 #pragma unroll
-            for (int k = 0; k < BARY_MAX_K; k++)
-                if (k < (int)K) acc[k] = f3_add(acc[k], f3_mul_base(ld3(scale + 3 * (k * n + p)), f));
+            for (int k = 0; k < K; k++) {
+                const uint64_t *s = scale + 3 * (k * n + pu);
+                acc_mad(acc[k][0], s[0], f);
+                acc_mad(acc[k][1], s[1], f);
+                acc_mad(acc[k][2], s[2], f);
+            }
         }
     }
 #pragma unroll
-    for (int k = 0; k < BARY_MAX_K; k++) {
-        red[rl][k][lane][0] = acc[k].c0;
-        red[rl][k][lane][1] = acc[k].c1;
-        red[rl][k][lane][2] = acc[k].c2;
-    }
+    for (int k = 0; k < K; k++)
+#pragma unroll
+        for (int j = 0; j < 3; j++) red[rl][k][lane][j] = acc_reduce(acc[k][j]);
     __syncthreads();
     if (rl == 0 && c < w) {
-        for (int k = 0; k < (int)K; k++) {
+        for (int k = 0; k < K; k++) {
             fp3 s = fp3{red[0][k][lane][0], red[0][k][lane][1], red[0][k][lane][2]};
             for (int r = 1; r < 4; r++) s = f3_add(s, fp3{red[r][k][lane][0], red[r][k][lane][1], red[r][k][lane][2]});
             st3(partial + 3 * (((uint64_t)blockIdx.x * K + k) * w + c), s);
         }
     }
+}
+
+template <int K>
+static void launch_bary_partial(dim3 grid, hipStream_t st, const uint64_t *lde, uint64_t w, uint64_t n,
+                                const uint64_t *scale, uint64_t *partial) {
+    hipLaunchKernelGGL(bary_partial_kernel<K>, grid, dim3(256), 0, st, lde, w, w, n, scale, partial);
 }
 
 // out[k][c] = vf[k] * sum over chunks of partial[chunk][k][c]
@@ -137,37 +155,67 @@ __global__ void bary_combine_kernel(const uint64_t *partial, uint64_t chunks, ui
 // computed once on the host, main columns are base words, aux and parts are
 // Fp3 (3 words each).
 constexpr int DEEP_MAX_K = 4;
+constexpr int DEEP_CHUNK = 16;  // main columns staged per LDS round
+// block: 256 consecutive physical rows. The main (base) columns of the block
+// are staged through LDS 16 at a time with coalesced row-segment loads; each
+// thread accumulates sum_j gt_{j,k} * t_j[p] as 3K delayed-reduction dot
+// products.
+template <int K>
 __global__ __launch_bounds__(256) void deep_kernel(uint64_t N, const uint64_t *__restrict__ main_lde, uint64_t wm,
                                                    const uint64_t *__restrict__ aux_lde, uint64_t wa,
-                                                   const uint64_t *__restrict__ parts_lde, uint64_t np, uint32_t K,
+                                                   const uint64_t *__restrict__ parts_lde, uint64_t np,
                                                    const uint64_t *__restrict__ gtr, const uint64_t *__restrict__ gh,
                                                    const uint64_t *__restrict__ consts,
                                                    const uint64_t *__restrict__ inv_den, uint64_t *__restrict__ out) {
-    uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= N) return;
-    fp3 tk[DEEP_MAX_K];
+    __shared__ uint64_t tile[256][DEEP_CHUNK + 1];
+    __shared__ uint64_t sg[DEEP_CHUNK][K][3];
+    const int t = threadIdx.x;
+    const uint64_t r0 = (uint64_t)blockIdx.x * 256;
+    const uint64_t p = r0 + t;
+    acc128 acc[K][3];
 #pragma unroll
-    for (int k = 0; k < DEEP_MAX_K; k++) tk[k] = fp3{0, 0, 0};
-    const uint64_t *mrow = main_lde + p * wm;
-    for (uint64_t j = 0; j < wm; j++) {
-        uint64_t v = mrow[j];
+    for (int k = 0; k < K; k++)
 #pragma unroll
-        for (int k = 0; k < DEEP_MAX_K; k++)
-            if (k < (int)K) tk[k] = f3_add(tk[k], f3_mul_base(ld3(gtr + 3 * (j * K + k)), v));
+        for (int j = 0; j < 3; j++) acc_zero(acc[k][j]);
+    for (uint64_t j0 = 0; j0 < wm; j0 += DEEP_CHUNK) {
+        const int cw = (int)(wm - j0 < DEEP_CHUNK ? wm - j0 : DEEP_CHUNK);
+        __syncthreads();
+        for (int e = t; e < 256 * DEEP_CHUNK; e += 256) {
+            const int rr = e / DEEP_CHUNK, cc = e % DEEP_CHUNK;
+            tile[rr][cc] = (cc < cw && r0 + rr < N) ? main_lde[(r0 + rr) * wm + j0 + cc] : 0;
+        }
+        for (int e = t; e < DEEP_CHUNK * K * 3; e += 256) {
+            const int cc = e / (K * 3), rem = e % (K * 3);
+            sg[cc][rem / 3][rem % 3] = cc < cw ? gtr[3 * ((j0 + cc) * K + rem / 3) + rem % 3] : 0;
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int cc = 0; cc < DEEP_CHUNK; cc++) {
+            const uint64_t v = tile[t][cc];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                acc_mad(acc[k][0], sg[cc][k][0], v);
+                acc_mad(acc[k][1], sg[cc][k][1], v);
+                acc_mad(acc[k][2], sg[cc][k][2], v);
+            }
+        }
     }
+    if (p >= N) return;
+    fp3 tk[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) tk[k] = fp3{acc_reduce(acc[k][0]), acc_reduce(acc[k][1]), acc_reduce(acc[k][2])};
     if (aux_lde) {
         const uint64_t *arow = aux_lde + p * 3 * wa;
         for (uint64_t j = 0; j < wa; j++) {
             fp3 v = ld3(arow + 3 * j);
 #pragma unroll
-            for (int k = 0; k < DEEP_MAX_K; k++)
-                if (k < (int)K) tk[k] = f3_add(tk[k], f3_mul(ld3(gtr + 3 * ((wm + j) * K + k)), v));
+            for (int k = 0; k < K; k++) tk[k] = f3_add(tk[k], f3_mul(ld3(gtr + 3 * ((wm + j) * K + k)), v));
         }
     }
     fp3 r = fp3{0, 0, 0};
 #pragma unroll
-    for (int k = 0; k < DEEP_MAX_K; k++)
-        if (k < (int)K) r = f3_add(r, f3_mul(ld3(inv_den + 3 * (k * N + p)), f3_sub(tk[k], ld3(consts + 3 * k))));
+    for (int k = 0; k < K; k++)
+        r = f3_add(r, f3_mul(ld3(inv_den + 3 * (k * N + p)), f3_sub(tk[k], ld3(consts + 3 * k))));
     if (parts_lde && np) {
         fp3 hs = fp3{0, 0, 0};
         const uint64_t *prow = parts_lde + p * 3 * np;
@@ -175,6 +223,14 @@ __global__ __launch_bounds__(256) void deep_kernel(uint64_t N, const uint64_t *_
         r = f3_add(r, f3_mul(ld3(inv_den + 3 * (K * N + p)), f3_sub(hs, ld3(consts + 3 * K))));
     }
     st3(out + 3 * p, r);
+}
+
+template <int K>
+static void launch_deep(hipStream_t st, uint64_t N, const uint64_t *main_lde, uint64_t wm, const uint64_t *aux_lde,
+                        uint64_t wa, const uint64_t *parts_lde, uint64_t np, const uint64_t *gtr, const uint64_t *gh,
+                        const uint64_t *consts, const uint64_t *inv_den, uint64_t *out) {
+    hipLaunchKernelGGL(deep_kernel<K>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, N, main_lde, wm, aux_lde,
+                       wa, parts_lde, np, gtr, gh, consts, inv_den, out);
 }
 
 }  // namespace lvm
@@ -187,12 +243,12 @@ unsigned log2u(uint64_t x) { return (unsigned)__builtin_ctzll(x); }
 struct Scratch {
     std::vector<void *> ptrs;
     ~Scratch() {
-        for (void *p : ptrs) hipFree(p);
+        for (void *p : ptrs) lvm_free(p);
     }
     template <class T>
     int get(T **out, size_t bytes) {
         void *p = nullptr;
-        hipError_t e = hipMalloc(&p, bytes ? bytes : 8);
+        hipError_t e = lvm_malloc(&p, bytes ? bytes : 8);
         if (e != hipSuccess) return fail(LVM_ERR_NOMEM, std::string("device allocation failed: ") + hipGetErrorString(e));
         ptrs.push_back(p);
         *out = (T *)p;
@@ -300,7 +356,16 @@ int lvm_barycentric_base_multi(const lvm_lde *h, uint64_t coset_offset, const ui
     {
         ProfScope ps("bary_sum", st);
         dim3 grid((unsigned)chunks, (unsigned)((w + 63) / 64));
-        hipLaunchKernelGGL(bary_partial_kernel, grid, dim3(256), 0, st, h->lde, w, w, n, scale, (uint32_t)K, partial);
+        switch (K) {
+        case 1: launch_bary_partial<1>(grid, st, h->lde, w, n, scale, partial); break;
+        case 2: launch_bary_partial<2>(grid, st, h->lde, w, n, scale, partial); break;
+        case 3: launch_bary_partial<3>(grid, st, h->lde, w, n, scale, partial); break;
+        case 4: launch_bary_partial<4>(grid, st, h->lde, w, n, scale, partial); break;
+        case 5: launch_bary_partial<5>(grid, st, h->lde, w, n, scale, partial); break;
+        case 6: launch_bary_partial<6>(grid, st, h->lde, w, n, scale, partial); break;
+        case 7: launch_bary_partial<7>(grid, st, h->lde, w, n, scale, partial); break;
+        default: launch_bary_partial<8>(grid, st, h->lde, w, n, scale, partial); break;
+        }
         hipLaunchKernelGGL(bary_combine_kernel, dim3((unsigned)((K * w + 255) / 256)), dim3(256), 0, st, partial,
                            chunks, (uint32_t)K, w, vfd, dout);
     }
@@ -371,9 +436,13 @@ int lvm_deep_composition(const lvm_lde *main, const lvm_lde *aux, const lvm_lde 
     }
     {
         ProfScope ps("deep_rows", st);
-        hipLaunchKernelGGL(deep_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, N, main->lde, wm,
-                           aux ? aux->lde : nullptr, wa, parts ? parts->lde : nullptr, np, (uint32_t)K, dgtr, dgh,
-                           dconsts, inv_den, dev_out);
+        const uint64_t *al = aux ? aux->lde : nullptr, *pl = parts ? parts->lde : nullptr;
+        switch (K) {
+        case 1: launch_deep<1>(st, N, main->lde, wm, al, wa, pl, np, dgtr, dgh, dconsts, inv_den, dev_out); break;
+        case 2: launch_deep<2>(st, N, main->lde, wm, al, wa, pl, np, dgtr, dgh, dconsts, inv_den, dev_out); break;
+        case 3: launch_deep<3>(st, N, main->lde, wm, al, wa, pl, np, dgtr, dgh, dconsts, inv_den, dev_out); break;
+        default: launch_deep<4>(st, N, main->lde, wm, al, wa, pl, np, dgtr, dgh, dconsts, inv_den, dev_out); break;
+        }
     }
     LVM_HIP(hipGetLastError());
     LVM_HIP(hipStreamSynchronize(st));

@@ -1,0 +1,92 @@
+"""Single-table proving rounds on the GPU path, ordered like the reference
+prover (crypto/stark/src/prover.rs): R1 commit, out-of-domain point, OOD
+values, DEEP coefficients, FRI commit phase, query sampling and openings.
+
+The table here has no AIR constraints (the single-table workload commits,
+opens and low-degree-tests the trace columns only), so round 2 is absent:
+  R1   coset LDE + row-pair Keccak Merkle tree, root -> transcript
+       (crypto/stark/src/prover.rs:3426-3471)
+  R3   z <- transcript avoiding the trace domain and the LDE coset
+       (crypto/crypto/src/fiat_shamir/is_transcript.rs:44-75); frame points
+       z, z*w_n (trace.rs:1112-1132); barycentric OOD values of every column
+       at both points appended point-major (prover.rs:4547-4560)
+  R4   gamma <- transcript; trace-term coefficients gamma^(j*K + k)
+       (prover.rs:2159-2182 takes successive powers of gamma); DEEP codeword;
+       FRI commit phase to a 2^(log B + k) terminal (fri/mod.rs:27-151);
+       query indexes sample_u64(N/2) (prover.rs:2324-2333); FRI query phase
+       and trace openings.
+Everything between the rounds stays in HBM; the host sees roots, OOD values,
+terminal coefficients and the opened rows.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import backend as B
+from . import field as F
+
+
+@dataclass
+class TableProof:
+    root: bytes
+    z: list
+    ood: np.ndarray          # (K, m, 3)
+    fri_roots: np.ndarray    # (layers, 32)
+    fri_coeffs: np.ndarray   # (2^k, 3)
+    iotas: np.ndarray
+    fri_sym: np.ndarray      # (layers, queries, 3)
+    fri_paths: list
+    trace_paths: np.ndarray  # (queries, depth, 32)
+    trace_rows: np.ndarray   # (queries, 2, m)
+    transcript_state: bytes
+
+
+def sample_z_ood(t: B.Transcript, n: int, N: int, offset: int):
+    g_pow = pow(offset, N, F.P)
+    blowup = N // n
+    while True:
+        z = [int(v) for v in t.sample_fp3()]
+        zt = F.f3_pow(z, n)
+        if F.f3_is_one(zt):
+            continue
+        if F.f3_pow(zt, blowup) == [g_pow, 0, 0]:
+            continue
+        return z
+
+
+def prove_single_table(d_trace: B.DeviceBuffer, n: int, m: int, blowup: int, offset: int,
+                       d_weights: B.DeviceBuffer, fri_final_log_degree: int = 6, num_queries: int = 30,
+                       label: bytes = b"lambda-vm-single-table", keep: bool = False):
+    N = n * blowup
+    blowup_log = blowup.bit_length() - 1
+    t = B.Transcript(label)
+    # R1
+    h = B.coset_lde_dev(d_trace, n, m, blowup, d_weights)
+    t.append_bytes(h.root)
+    # R3
+    z = sample_z_ood(t, n, N, offset)
+    wn = F.root_of_unity(n.bit_length() - 1)
+    points = [z, F.f3_scale(z, wn)]
+    K = len(points)
+    ood = B.barycentric_base_multi(h, offset, np.array(points, dtype=np.uint64))  # (K, m, 3)
+    for k in range(K):
+        for c in range(m):
+            t.append_fp3(ood[k, c])
+    # R4
+    gamma = [int(v) for v in t.sample_fp3()]
+    gammas = np.array(F.powers(gamma, m * K), dtype=np.uint64)  # index j*K + k
+    trace_ood = np.ascontiguousarray(ood.transpose(1, 0, 2).reshape(m * K, 3))
+    deep = B.deep_composition(h, offset, np.array(points, dtype=np.uint64), trace_ood, gammas)
+    fri = B.GpuFri(None, offset, on_device=deep, length=N, adopt=True)
+    roots, coeffs = fri.commit_phase(t, blowup_log, fri_final_log_degree)
+    iotas = np.array([t.sample_u64(N >> 1) for _ in range(num_queries)], dtype=np.uint64)
+    sym, paths = fri.query(iotas)
+    tpaths, trows = h.open(iotas)
+    proof = TableProof(h.root, z, ood, roots, coeffs, iotas, sym, paths, tpaths, trows, t.state())
+    if keep:
+        return proof, h, fri
+    fri.close()
+    h.close()
+    return proof
