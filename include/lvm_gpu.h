@@ -125,6 +125,45 @@ int lvm_lde_open(const lvm_lde *h, const uint64_t *positions, size_t count, uint
  * (crypto/stark/src/prover.rs:451-467 `LdeTwiddles::new` coset_weights). */
 int lvm_coset_weights(size_t n, uint64_t offset, uint64_t *dev_out);
 
+/* ---------------- batched ragged commitment of many tables ----------------
+ * The R1 commitment of all tables of a proof (crypto/stark/src/prover.rs
+ * :3299-3492 `multi_prove`, which calls the per-table
+ * `coset_lde_row_major_with_merkle_tree_keep`, math-cuda/src/lde.rs:674, once
+ * per table) as one launch per stage over every table: each NTT pass shape,
+ * the Keccak leaves and each Merkle tree step run as a single work-list
+ * launch whose blocks are spread over the tables, and the whole sequence is
+ * captured once as a HIP graph and replayed. Results are those of
+ * lvm_coset_lde_dev on each table. Shapes are fixed when the plan is made;
+ * inputs may be rebound to other buffers of the same shapes. */
+typedef struct lvm_table_desc {
+    const uint64_t *trace;   /* device, n x m row-major */
+    const uint64_t *weights; /* device, the n LDE weights n^-1 * offset^i */
+    size_t n, m, blowup;
+} lvm_table_desc;
+typedef struct lvm_batch lvm_batch;
+/* use_graph: 1 = capture the launch sequence as a HIP graph, 0 = plain launches */
+int lvm_batch_new(const lvm_table_desc *tables, size_t count, int use_graph, lvm_batch **out);
+/* point the plan at other inputs with the same shapes (takes effect at the next run) */
+int lvm_batch_bind(lvm_batch *b, const lvm_table_desc *tables, size_t count);
+/* run every stage for all tables; roots_out (nullable) receives count x 32 bytes */
+int lvm_batch_run(lvm_batch *b, uint8_t *roots_out);
+/* hand table `index`'s LDE and tree over as an lvm_lde handle; the plan gets
+ * fresh output buffers for that table at its next run */
+int lvm_batch_take(lvm_batch *b, size_t index, lvm_lde **out);
+int lvm_batch_release(lvm_batch *b);
+/* the plan's launches in order: profiler name (the name each launch is
+ * recorded under when profiling is on and the plan runs without a graph),
+ * compulsory HBM bytes (inputs read once, outputs written once) and blocks */
+int lvm_batch_launch_count(const lvm_batch *b, size_t *count);
+int lvm_batch_launch_info(const lvm_batch *b, size_t index, char *name_out, size_t name_cap, double *bytes_out,
+                          uint32_t *blocks_out);
+/* one-shot form: plan, run, hand every table over (handles_out: count entries) */
+int lvm_batch_lde_commit(const lvm_table_desc *tables, size_t count, uint8_t *roots_out, lvm_lde **handles_out);
+/* NTT pass split for size 2^log_n (radices 1..8 summing to log_n; count 0
+ * restores the default); used by every later transform of that size. */
+int lvm_set_ntt_plan(unsigned log_n, const int *radices, int count);
+int lvm_get_ntt_plan(unsigned log_n, int *radices_out, int *count_out);
+
 /* ---------------- Keccak Merkle trees on host data ----------------
  * crypto/math-cuda/src/merkle.rs:316 `build_merkle_tree_on_device`: leaves
  * (num_leaves x 32 bytes) padded to a power of two by repeating the last

@@ -60,6 +60,17 @@ SIGNATURES = {
     "lvm_lde_tree_nodes": (C.c_int, [C.c_void_p, u8p, szp]),
     "lvm_lde_open": (C.c_int, [C.c_void_p, u64p, C.c_size_t, u8p, u64p]),
     "lvm_coset_weights": (C.c_int, [C.c_size_t, C.c_uint64, C.c_void_p]),
+    "lvm_batch_new": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, vpp]),
+    "lvm_batch_bind": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    "lvm_batch_run": (C.c_int, [C.c_void_p, u8p]),
+    "lvm_batch_take": (C.c_int, [C.c_void_p, C.c_size_t, vpp]),
+    "lvm_batch_release": (C.c_int, [C.c_void_p]),
+    "lvm_batch_lde_commit": (C.c_int, [C.c_void_p, C.c_size_t, u8p, vpp]),
+    "lvm_batch_launch_count": (C.c_int, [C.c_void_p, szp]),
+    "lvm_batch_launch_info": (C.c_int, [C.c_void_p, C.c_size_t, C.c_char_p, C.c_size_t, C.POINTER(C.c_double),
+                                        C.POINTER(C.c_uint32)]),
+    "lvm_set_ntt_plan": (C.c_int, [C.c_uint, C.POINTER(C.c_int), C.c_int]),
+    "lvm_get_ntt_plan": (C.c_int, [C.c_uint, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "lvm_build_merkle_tree": (C.c_int, [u8p, C.c_size_t, u8p]),
     "lvm_batch_inverse_ext3": (C.c_int, [u64p, u64p, C.c_size_t]),
     "lvm_batch_inverse_ext3_dev": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
@@ -90,6 +101,12 @@ SIGNATURES = {
     "lvm_prof_reset": (C.c_int, []),
     "lvm_prof_query": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
 }
+
+
+class TableDesc(C.Structure):
+    """lvm_table_desc: one table of a batched commitment (device pointers)."""
+    _fields_ = [("trace", C.c_void_p), ("weights", C.c_void_p), ("n", C.c_size_t), ("m", C.c_size_t),
+                ("blowup", C.c_size_t)]
 
 
 def header_symbols() -> list[str]:

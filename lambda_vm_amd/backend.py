@@ -20,6 +20,7 @@ import ctypes as C
 import numpy as np
 
 from . import _native as N
+from ._native import LvmError  # noqa: F401  (raised by every failing call)
 
 OPS = {"add": 0, "sub": 1, "mul": 2, "neg": 3, "inv": 4, "sqr": 5}
 
@@ -218,6 +219,92 @@ def coset_weights_dev(n: int, offset: int) -> DeviceBuffer:
     b = DeviceBuffer(8 * n)
     N.call("lvm_coset_weights", n, offset, C.c_void_p(b.ptr))
     return b
+
+
+def _descs(tables) -> "C.Array":
+    """(trace DeviceBuffer, weights DeviceBuffer, n, m, blowup) tuples -> lvm_table_desc[]."""
+    arr = (N.TableDesc * len(tables))()
+    for i, (tr, w, n, m, b) in enumerate(tables):
+        arr[i] = N.TableDesc(tr.ptr, w.ptr, n, m, b)
+    return arr
+
+
+class BatchPlan:
+    """Batched ragged commitment of many tables (lvm_batch_*): one launch per
+    stage step across all tables, optionally replayed as a captured HIP graph.
+    Each table's result equals coset_lde_dev on it."""
+
+    def __init__(self, tables, use_graph: bool = True):
+        N.ensure_device()
+        self.count = len(tables)
+        self._keep = list(tables)
+        descs = _descs(tables)
+        h = C.c_void_p()
+        N.call("lvm_batch_new", C.cast(descs, C.c_void_p), self.count, int(use_graph), C.byref(h))
+        self.h = h
+
+    def bind(self, tables) -> None:
+        """Point the plan at other inputs of the same shapes."""
+        descs = _descs(tables)
+        N.call("lvm_batch_bind", self.h, C.cast(descs, C.c_void_p), len(tables))
+        self._keep = list(tables)
+
+    def run(self) -> list[bytes]:
+        roots = np.zeros((self.count, 32), dtype=np.uint8)
+        N.call("lvm_batch_run", self.h, _p8(roots))
+        return [bytes(r) for r in roots]
+
+    def launches(self) -> list[tuple[str, float, int]]:
+        """(profiler name, compulsory HBM bytes, blocks) of each launch in order."""
+        cnt = C.c_size_t()
+        N.call("lvm_batch_launch_count", self.h, C.byref(cnt))
+        out = []
+        for i in range(cnt.value):
+            name = C.create_string_buffer(64)
+            by, bl = C.c_double(), C.c_uint32()
+            N.call("lvm_batch_launch_info", self.h, i, name, 64, C.byref(by), C.byref(bl))
+            out.append((name.value.decode(), by.value, bl.value))
+        return out
+
+    def take(self, index: int) -> GpuLde:
+        h = C.c_void_p()
+        N.call("lvm_batch_take", self.h, index, C.byref(h))
+        return GpuLde(h.value)
+
+    def close(self) -> None:
+        if self.h is not None and self.h.value:
+            N.lib().lvm_batch_release(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def batch_lde_commit(tables) -> tuple[list[bytes], list[GpuLde]]:
+    """One-shot batched commitment: roots and the resident handle of each table."""
+    N.ensure_device()
+    descs = _descs(tables)
+    roots = np.zeros((len(tables), 32), dtype=np.uint8)
+    hs = (C.c_void_p * len(tables))()
+    N.call("lvm_batch_lde_commit", C.cast(descs, C.c_void_p), len(tables), _p8(roots),
+           C.cast(hs, N.vpp))
+    return [bytes(r) for r in roots], [GpuLde(h) for h in hs]
+
+
+def set_ntt_plan(log_n: int, radices) -> None:
+    """Install a pass split for size 2^log_n (empty restores the default)."""
+    arr = (C.c_int * max(1, len(radices)))(*radices)
+    N.call("lvm_set_ntt_plan", log_n, arr, len(radices))
+
+
+def get_ntt_plan(log_n: int) -> list[int]:
+    arr = (C.c_int * 8)()
+    cnt = C.c_int()
+    N.call("lvm_get_ntt_plan", log_n, arr, C.byref(cnt))
+    return list(arr[:cnt.value])
 
 
 def build_merkle_tree(leaves) -> np.ndarray:

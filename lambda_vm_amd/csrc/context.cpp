@@ -112,7 +112,7 @@ hipStream_t lib_stream() {
 
 // ---------------- profiler ----------------
 struct ProfRec {
-    const char *name;
+    std::string name;  // copied: launch sites may pass names they later free
     hipEvent_t a, b;
 };
 static std::mutex g_prof_mu;
@@ -151,7 +151,7 @@ int prof_query(const char *name, uint64_t *count, double *total_ms) {
     uint64_t c = 0;
     double t = 0;
     for (auto &r : g_prof) {
-        if (name && std::string(name) != r.name) continue;
+        if (name && r.name != name) continue;
         float ms = 0;
         if (hipEventSynchronize(r.b) != hipSuccess) return LVM_ERR_DEVICE;
         if (hipEventElapsedTime(&ms, r.a, r.b) != hipSuccess) return LVM_ERR_DEVICE;

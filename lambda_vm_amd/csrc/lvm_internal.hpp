@@ -62,6 +62,28 @@ inline hipError_t lvm_free(void *p) { return p ? hipFreeAsync(p, lib_stream()) :
     } while (0)
 
 // ---- NTT (ntt.hip) ----
+// Arguments of one NTT pass over one table (layout and remaps described in ntt.hip).
+struct PassArgs {
+    const uint64_t *in;
+    uint64_t *out;
+    uint32_t in_pitch, out_pitch;  // row strides in u64
+    uint32_t m;                    // number of columns
+    uint32_t log_msub;             // log2 of M (sub-problem rows at this pass)
+    uint32_t log_s;                // log2 of the row stride S
+    uint32_t in_bitrev_bits;       // 0: natural input rows; else bitrev over this many bits
+    uint64_t total_groups;         // rows_logical / R
+    uint64_t out_coset_rows;       // rows per coset block in `out` (n); 0 = single block
+    uint32_t coset_bits;           // log2(blowup) for the per-coset output offset
+    uint32_t shift_log;            // log2 N of the coset shift w_N^(coset*j); 0 = none
+    const uint64_t *weights;       // nullable pre-multiply weights[logical row]
+    const uint64_t *tw_lo;         // w_{2^32}^(+-x), x < 2^16
+    const uint64_t *tw_hi;         // w_{2^32}^(+-x*2^16), x < 2^16
+    const uint64_t *shift_lo;      // forward tables for the coset shift
+    const uint64_t *shift_hi;
+    const uint64_t *tw_r;          // w_256^(+-e), e < 128
+    uint32_t cosets_per_block;     // cosets produced per block (grid.z = cosets / this)
+};
+
 void ntt_inverse_rows(const uint64_t *in, uint32_t in_pitch, uint64_t *out, uint32_t out_pitch, uint32_t m,
                       unsigned log_n, const Tables &tb, hipStream_t st);
 void ntt_forward_rows(const uint64_t *in, uint32_t in_pitch, uint64_t *out, uint32_t out_pitch, uint32_t m,
@@ -81,5 +103,47 @@ void merkle_build_inner(uint8_t *nodes, uint64_t L, hipStream_t st);
 void keccak_fp3_pair_leaves(const uint64_t *evals, uint64_t len, uint8_t *leaves_out, hipStream_t st);
 
 size_t padded_leaves(size_t n);
+
+// ---- work lists over several tables (batched stages) ----
+// Work-list slot of block b: the t with prefix[t] <= b < prefix[t+1]
+// (prefix ascending, prefix[0] = 0), by binary search over `count` slots.
+__device__ __forceinline__ uint32_t find_slot(const uint32_t *__restrict__ prefix, uint32_t count, uint32_t b) {
+    uint32_t lo = 0, hi = count - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) / 2;
+        if (prefix[mid] <= b)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    return lo;
+}
+int plan_passes(unsigned log_n, int *r);              // radices of a size-2^log_n transform
+bool set_ntt_plan(unsigned log_n, const int *r, int np);  // np = 0 restores the default
+uint32_t pass_blocks(int r, const PassArgs &a);       // blocks of one table's pass
+void launch_pass_batched(int r, const PassArgs *d_args, const uint32_t *d_prefix, uint32_t count, uint32_t blocks,
+                         bool multi, hipStream_t st);
+// One table's coset-LDE pass sequence (inverse passes into `coef`, then the
+// coset expansion into `lde`); returns the number of passes.
+int lde_pass_list(const uint64_t *trace, uint32_t m, unsigned log_n, unsigned log_blowup, const uint64_t *weights,
+                  uint64_t *coef, uint64_t *lde, const Tables &tb, int *r_out, PassArgs *a_out, uint32_t *cos_out);
+
+struct LeafJob {
+    const uint64_t *rows;  // bit-reversed row-major LDE, `pitch` words per row
+    uint64_t pitch;
+    uint64_t num_leaves;
+    uint64_t *leaves;      // num_leaves x 32 bytes
+};
+struct TreeStep {
+    uint64_t *nodes;
+    uint64_t cstart, pstart;  // first child / first parent index of the level
+    uint32_t span;            // parents per block at the first level (<= 256)
+    uint32_t levels;          // levels this step builds
+};
+void launch_leaves_batched(const LeafJob *d_jobs, const uint32_t *d_prefix, uint32_t count, uint32_t blocks,
+                           hipStream_t st);
+void launch_tree_steps_batched(const TreeStep *d_steps, const uint32_t *d_prefix, uint32_t count, uint32_t blocks,
+                               hipStream_t st);
+void launch_gather_roots(uint64_t *const *d_trees, uint32_t count, uint64_t *d_roots, hipStream_t st);
 
 }  // namespace lvm

@@ -29,12 +29,8 @@ __device__ __forceinline__ void absorb_be(uint32_t &l, uint32_t &h, uint64_t v) 
 // the committed words are whole rows, so the leaf's data is one contiguous
 // run of 2*pitch words.
 template <bool CONTIG>
-__global__ __launch_bounds__(256) void keccak_row_pair_leaves_kernel(const uint64_t *__restrict__ rows,
-                                                                    uint64_t pitch, uint64_t w0, uint64_t w1,
-                                                                    uint64_t num_leaves,
-                                                                    uint64_t *__restrict__ leaves) {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= num_leaves) return;
+__device__ __forceinline__ void row_pair_leaf(const uint64_t *__restrict__ rows, uint64_t pitch, uint64_t w0,
+                                              uint64_t w1, uint64_t i, uint64_t *__restrict__ leaves) {
     const uint64_t width = w1 - w0;
     const uint64_t total = 2 * width;
     const uint64_t *r0 = rows + (2 * i) * pitch + w0;
@@ -84,6 +80,29 @@ __global__ __launch_bounds__(256) void keccak_row_pair_leaves_kernel(const uint6
     for (int k = 0; k < 4; k++) o[k] = ((uint64_t)h[k] << 32) | l[k];
 }
 
+template <bool CONTIG>
+__global__ __launch_bounds__(256) void keccak_row_pair_leaves_kernel(const uint64_t *__restrict__ rows,
+                                                                    uint64_t pitch, uint64_t w0, uint64_t w1,
+                                                                    uint64_t num_leaves,
+                                                                    uint64_t *__restrict__ leaves) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= num_leaves) return;
+    row_pair_leaf<CONTIG>(rows, pitch, w0, w1, i, leaves);
+}
+
+// Leaves of several tables in one launch: blocks [prefix[t], prefix[t+1])
+// hash table t's leaves, 256 per block.
+__global__ __launch_bounds__(256) void keccak_leaves_batched_kernel(const LeafJob *__restrict__ jobs,
+                                                                   const uint32_t *__restrict__ prefix,
+                                                                   uint32_t count) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t t = find_slot(prefix, count, b);
+    const LeafJob j = jobs[t];
+    const uint64_t i = (uint64_t)(b - prefix[t]) * 256 + threadIdx.x;
+    if (i >= j.num_leaves) return;
+    row_pair_leaf<true>(j.rows, j.pitch, 0, j.pitch, i, j.leaves);
+}
+
 // FRI layer leaves: Keccak(BE24(e[2j]) || BE24(e[2j+1])), 6 lanes, one permutation
 // (crypto/crypto/src/merkle_tree/backends/field_element_vector.rs:129-134).
 __global__ __launch_bounds__(256) void keccak_fp3_pair_leaves_kernel(const uint64_t *__restrict__ evals,
@@ -121,38 +140,14 @@ __global__ __launch_bounds__(256) void merkle_level_kernel(uint64_t *nodes, uint
     d[3] = o[3];
 }
 
-// All levels whose parent range starts below `top_begin + 1` in one block:
-// level [b/2, b) is built from [b, 2b] for b = top_begin, top_begin/2, ... 1.
-__global__ __launch_bounds__(1024) void merkle_tail_kernel(uint64_t *nodes, uint64_t level_begin) {
-    while (level_begin > 0) {
-        const uint64_t nb = level_begin / 2;
-        const uint64_t count = level_begin - nb;
-        for (uint64_t k = threadIdx.x; k < count; k += blockDim.x) {
-            const uint64_t p = nb + k;
-            const uint64_t *l = nodes + 4 * (2 * p + 1);
-            const uint64_t *r = nodes + 4 * (2 * p + 2);
-            uint64_t lv[4] = {l[0], l[1], l[2], l[3]}, rv[4] = {r[0], r[1], r[2], r[3]}, o[4];
-            keccak_pair_node(lv, rv, o);
-            uint64_t *d = nodes + 4 * p;
-            d[0] = o[0];
-            d[1] = o[1];
-            d[2] = o[2];
-            d[3] = o[3];
-        }
-        __syncthreads();
-        level_begin = nb;
-    }
-}
-
 // Up to 9 tree levels per launch: block b takes the 2*span children
 // [cstart + 2*span*b, ...) of one level (span <= 256 parents), writes their
-// parents, keeps them in LDS and keeps hashing pairs until one node is left,
-// writing every level it produces to the node array.
-__global__ __launch_bounds__(256) void merkle_subtree_kernel(uint64_t *nodes, uint64_t cstart, uint64_t pstart,
-                                                             uint32_t span) {
+// parents, keeps them in LDS and keeps hashing pairs, at most `levels`
+// levels in all, writing every level it produces to the node array.
+__device__ __forceinline__ void merkle_subtree_body(uint64_t *nodes, uint64_t cstart, uint64_t pstart, uint32_t span,
+                                                    uint32_t levels, uint64_t b) {
     __shared__ uint64_t dig[256][4];
     const uint32_t t = threadIdx.x;
-    const uint64_t b = blockIdx.x;
     uint64_t pbase = pstart + (uint64_t)span * b;  // first parent of this block at the current level
     if (t < span) {
         const uint64_t *l = nodes + 4 * (cstart + 2 * (uint64_t)span * b + 2 * t);
@@ -168,7 +163,7 @@ __global__ __launch_bounds__(256) void merkle_subtree_kernel(uint64_t *nodes, ui
     // next levels: the block's nodes at level start L0 = pbase; the parent of
     // node x is (x - 1) / 2, and a block's nodes stay contiguous
     uint32_t cnt = span;
-    while (cnt > 1) {
+    for (uint32_t lv_done = 1; cnt > 1 && lv_done < levels; lv_done++) {
         __syncthreads();
         const uint32_t half = cnt / 2;
         const uint64_t next = (pbase - 1) / 2;
@@ -190,6 +185,31 @@ __global__ __launch_bounds__(256) void merkle_subtree_kernel(uint64_t *nodes, ui
         pbase = next;
         cnt = half;
     }
+}
+
+__global__ __launch_bounds__(256) void merkle_subtree_kernel(uint64_t *nodes, uint64_t cstart, uint64_t pstart,
+                                                             uint32_t span) {
+    merkle_subtree_body(nodes, cstart, pstart, span, 32, blockIdx.x);
+}
+
+// One tree step for each of several trees: blocks [prefix[t], prefix[t+1])
+// run step t (a single wide level with levels = 1, or a subtree top).
+__global__ __launch_bounds__(256) void merkle_steps_batched_kernel(const TreeStep *__restrict__ steps,
+                                                                   const uint32_t *__restrict__ prefix,
+                                                                   uint32_t count) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t t = find_slot(prefix, count, b);
+    const TreeStep s = steps[t];
+    merkle_subtree_body(s.nodes, s.cstart, s.pstart, s.span, s.levels, b - prefix[t]);
+}
+
+// Copy each tree's root (node 0) into roots[t].
+__global__ void gather_roots_kernel(uint64_t *const *__restrict__ trees, uint32_t count, uint64_t *__restrict__ roots) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    const uint64_t *r = trees[t];
+#pragma unroll
+    for (int k = 0; k < 4; k++) roots[4 * t + k] = r[k];
 }
 
 size_t padded_leaves(size_t n) {
@@ -236,6 +256,21 @@ void merkle_build_inner(uint8_t *nodes_b, uint64_t L, hipStream_t st) {
                            span);
         C = blocks;
     }
+}
+
+void launch_leaves_batched(const LeafJob *d_jobs, const uint32_t *d_prefix, uint32_t count, uint32_t blocks,
+                           hipStream_t st) {
+    if (blocks) hipLaunchKernelGGL(keccak_leaves_batched_kernel, dim3(blocks), dim3(256), 0, st, d_jobs, d_prefix, count);
+}
+
+void launch_tree_steps_batched(const TreeStep *d_steps, const uint32_t *d_prefix, uint32_t count, uint32_t blocks,
+                               hipStream_t st) {
+    if (blocks)
+        hipLaunchKernelGGL(merkle_steps_batched_kernel, dim3(blocks), dim3(256), 0, st, d_steps, d_prefix, count);
+}
+
+void launch_gather_roots(uint64_t *const *d_trees, uint32_t count, uint64_t *d_roots, hipStream_t st) {
+    if (count) hipLaunchKernelGGL(gather_roots_kernel, dim3((count + 63) / 64), dim3(64), 0, st, d_trees, count, d_roots);
 }
 
 }  // namespace lvm

@@ -30,34 +30,15 @@
 
 namespace lvm {
 
-struct PassArgs {
-    const uint64_t *in;
-    uint64_t *out;
-    uint32_t in_pitch, out_pitch;  // row strides in u64
-    uint32_t m;                    // number of columns
-    uint32_t log_msub;             // log2 of M (sub-problem rows at this pass)
-    uint32_t log_s;                // log2 of the row stride S
-    uint32_t in_bitrev_bits;       // 0: natural input rows; else bitrev over this many bits
-    uint64_t total_groups;         // rows_logical / R
-    uint64_t out_coset_rows;       // rows per coset block in `out` (n); 0 = single block
-    uint32_t coset_bits;           // log2(blowup) for the per-coset output offset
-    uint32_t shift_log;            // log2 N of the coset shift w_N^(coset*j); 0 = none
-    const uint64_t *weights;       // nullable pre-multiply weights[logical row]
-    const uint64_t *tw_lo;         // w_{2^32}^(+-x), x < 2^16
-    const uint64_t *tw_hi;         // w_{2^32}^(+-x*2^16), x < 2^16
-    const uint64_t *shift_lo;      // forward tables for the coset shift
-    const uint64_t *shift_hi;
-    const uint64_t *tw_r;          // w_256^(+-e), e < 128
-    uint32_t cosets_per_block;     // cosets produced per block (grid.z = cosets / this)
-};
-
 // w_{2^32}^e via the two-level table
 __device__ __forceinline__ uint64_t tw32(const uint64_t *lo, const uint64_t *hi, uint32_t e) {
     return mul(lo[e & 0xFFFFu], hi[e >> 16]);
 }
 
+// Body of one pass for block (bx, by, bz) of the single-table grid
+// (bx: row-tile index, by: 16-column group, bz: coset slice).
 template <int LOG_R1, int LOG_R2, bool MULTI>
-__global__ __launch_bounds__(256) void ntt_pass_kernel(PassArgs a) {
+__device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t bx, uint32_t by, uint32_t bz) {
     constexpr int R1 = 1 << LOG_R1, R2 = 1 << LOG_R2, R = R1 * R2, LOG_R = LOG_R1 + LOG_R2;
     constexpr int C = 16;                 // columns per tile
     constexpr int TPC = R1 > R2 ? R1 : R2;  // threads per column
@@ -73,9 +54,9 @@ __global__ __launch_bounds__(256) void ntt_pass_kernel(PassArgs a) {
     const int c = tid % C;
     const int tc = (tid / C) % TPC;
     const int tl = tid / (C * TPC);
-    const uint64_t g = (uint64_t)blockIdx.x * TILES + tl;
+    const uint64_t g = bx * TILES + tl;
     const bool g_ok = g < a.total_groups;
-    const uint32_t col = blockIdx.y * C + c;
+    const uint32_t col = by * C + c;
     const bool c_ok = col < a.m && g_ok;
 
     const uint32_t log_s = a.log_s;
@@ -120,7 +101,7 @@ __global__ __launch_bounds__(256) void ntt_pass_kernel(PassArgs a) {
 
     const uint32_t ncos = MULTI ? a.cosets_per_block : 1u;
     for (uint32_t ci = 0; ci < ncos; ci++) {
-    const uint32_t coset = blockIdx.z * a.cosets_per_block + ci;
+    const uint32_t coset = bz * a.cosets_per_block + ci;
     if (pre_needed) {
         // pre-multiply factors weights[j] * w_N^(coset*j) of this coset
         for (int t = tin; t < R; t += C * TPC) {
@@ -203,6 +184,31 @@ __global__ __launch_bounds__(256) void ntt_pass_kernel(PassArgs a) {
     }
 }
 
+template <int LOG_R1, int LOG_R2, bool MULTI>
+__global__ __launch_bounds__(256) void ntt_pass_kernel(PassArgs a) {
+    ntt_pass_body<LOG_R1, LOG_R2, MULTI>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// One launch for the same pass shape over several tables: `args[t]` is
+// table t's pass, blocks [prefix[t], prefix[t+1]) are its tiles, column
+// groups fastest so that neighbouring blocks cover whole rows.
+template <int LOG_R1, int LOG_R2, bool MULTI>
+__global__ __launch_bounds__(256) void ntt_pass_batched_kernel(const PassArgs *__restrict__ args,
+                                                               const uint32_t *__restrict__ prefix, uint32_t count) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t t = find_slot(prefix, count, b);
+    const PassArgs a = args[t];
+    const uint32_t local = b - prefix[t];
+    const uint32_t groups = (a.m + 15) / 16;
+    ntt_pass_body<LOG_R1, LOG_R2, MULTI>(a, local / groups, local % groups, 0);
+}
+
+template <int A, int B>
+static constexpr uint32_t pass_tiles() {
+    constexpr int TPC = (1 << A) > (1 << B) ? (1 << A) : (1 << B);
+    return 256 / (16 * TPC);
+}
+
 // pass shapes: (LOG_R1, LOG_R2) for r = 1..8
 template <int A, int B>
 static void launch_one(const PassArgs &a0, uint32_t cosets, hipStream_t st) {
@@ -233,8 +239,16 @@ static void launch_pass(int r, const PassArgs &a, uint32_t cosets, hipStream_t s
     }
 }
 
-// Split log n into passes of at most 8 bits, largest first.
-static int plan_passes(unsigned log_n, int *r) {
+// Pass plans per log2 size: default splits log n into passes of at most 8
+// bits, largest first; set_ntt_plan installs a measured split.
+static int g_plan[33][8];
+static int g_plan_len[33];
+
+int plan_passes(unsigned log_n, int *r) {
+    if (log_n <= 32 && g_plan_len[log_n] > 0) {
+        for (int i = 0; i < g_plan_len[log_n]; i++) r[i] = g_plan[log_n][i];
+        return g_plan_len[log_n];
+    }
     int np = 0;
     unsigned left = log_n;
     while (left > 0) {
@@ -243,6 +257,128 @@ static int plan_passes(unsigned log_n, int *r) {
         left -= take;
     }
     return np;
+}
+
+bool set_ntt_plan(unsigned log_n, const int *r, int np) {
+    if (log_n > 32 || np < 0 || np > 8) return false;
+    int sum = 0;
+    for (int i = 0; i < np; i++) {
+        if (r[i] < 1 || r[i] > 8) return false;
+        sum += r[i];
+    }
+    if (np > 0 && sum != (int)log_n) return false;
+    for (int i = 0; i < np; i++) g_plan[log_n][i] = r[i];
+    g_plan_len[log_n] = np;  // 0 restores the default split
+    return true;
+}
+
+// ---- work-list launches over several tables ----
+
+uint32_t pass_blocks(int r, const PassArgs &a) {
+    uint32_t tiles = 1;
+    switch (r) {
+    case 1: tiles = pass_tiles<1, 0>(); break;
+    case 2: tiles = pass_tiles<1, 1>(); break;
+    case 3: tiles = pass_tiles<2, 1>(); break;
+    case 4: tiles = pass_tiles<2, 2>(); break;
+    case 5: tiles = pass_tiles<3, 2>(); break;
+    case 6: tiles = pass_tiles<3, 3>(); break;
+    case 7: tiles = pass_tiles<4, 3>(); break;
+    case 8: tiles = pass_tiles<4, 4>(); break;
+    default: break;
+    }
+    return (uint32_t)((a.total_groups + tiles - 1) / tiles) * ((a.m + 15) / 16);
+}
+
+template <int A, int B>
+static void launch_batched_one(const PassArgs *args, const uint32_t *prefix, uint32_t count, uint32_t blocks,
+                               bool multi, hipStream_t st) {
+    if (multi)
+        hipLaunchKernelGGL((ntt_pass_batched_kernel<A, B, true>), dim3(blocks), dim3(256), 0, st, args, prefix,
+                           count);
+    else
+        hipLaunchKernelGGL((ntt_pass_batched_kernel<A, B, false>), dim3(blocks), dim3(256), 0, st, args, prefix,
+                           count);
+}
+
+void launch_pass_batched(int r, const PassArgs *d_args, const uint32_t *d_prefix, uint32_t count, uint32_t blocks,
+                         bool multi, hipStream_t st) {
+    if (blocks == 0) return;
+    switch (r) {
+    case 1: launch_batched_one<1, 0>(d_args, d_prefix, count, blocks, multi, st); break;
+    case 2: launch_batched_one<1, 1>(d_args, d_prefix, count, blocks, multi, st); break;
+    case 3: launch_batched_one<2, 1>(d_args, d_prefix, count, blocks, multi, st); break;
+    case 4: launch_batched_one<2, 2>(d_args, d_prefix, count, blocks, multi, st); break;
+    case 5: launch_batched_one<3, 2>(d_args, d_prefix, count, blocks, multi, st); break;
+    case 6: launch_batched_one<3, 3>(d_args, d_prefix, count, blocks, multi, st); break;
+    case 7: launch_batched_one<4, 3>(d_args, d_prefix, count, blocks, multi, st); break;
+    case 8: launch_batched_one<4, 4>(d_args, d_prefix, count, blocks, multi, st); break;
+    default: break;
+    }
+}
+
+// The pass sequence of one table's coset LDE (inverse transform of the
+// trace, then the coset expansion), as the argument blocks the single-table
+// path launches: radix in r_out, arguments in a_out, coset count in cos_out.
+int lde_pass_list(const uint64_t *trace, uint32_t m, unsigned log_n, unsigned log_blowup, const uint64_t *weights,
+                  uint64_t *coef, uint64_t *lde, const Tables &tb, int *r_out, PassArgs *a_out, uint32_t *cos_out) {
+    int r[8];
+    const int np = plan_passes(log_n, r);
+    if (np == 0) return -1;
+    int k = 0;
+    unsigned done = 0;
+    for (int p = 0; p < np; p++) {
+        PassArgs a{};
+        a.in = p == 0 ? trace : coef;
+        a.out = coef;
+        a.in_pitch = a.out_pitch = m;
+        a.m = m;
+        a.log_msub = log_n - done;
+        a.log_s = log_n - done - r[p];
+        a.total_groups = (1ull << log_n) >> r[p];
+        a.tw_lo = tb.inv_lo;
+        a.tw_hi = tb.inv_hi;
+        a.tw_r = tb.inv_r;
+        a.cosets_per_block = 1;
+        r_out[k] = r[p];
+        cos_out[k] = 1;
+        a_out[k++] = a;
+        done += r[p];
+    }
+    const uint32_t B = 1u << log_blowup;
+    done = 0;
+    for (int p = 0; p < np; p++) {
+        PassArgs a{};
+        a.in = p == 0 ? coef : lde;
+        a.out = lde;
+        a.in_pitch = a.out_pitch = m;
+        a.m = m;
+        a.log_msub = log_n - done;
+        a.log_s = log_n - done - r[p];
+        a.tw_lo = tb.fwd_lo;
+        a.tw_hi = tb.fwd_hi;
+        a.tw_r = tb.fwd_r;
+        if (p == 0) {
+            a.in_bitrev_bits = log_n;
+            a.total_groups = (1ull << log_n) >> r[0];
+            a.out_coset_rows = 1ull << log_n;
+            a.coset_bits = log_blowup;
+            a.shift_log = log_n + log_blowup;
+            a.weights = weights;
+            a.shift_lo = tb.fwd_lo;
+            a.shift_hi = tb.fwd_hi;
+            a.cosets_per_block = B;
+            cos_out[k] = B;
+        } else {
+            a.total_groups = ((uint64_t)B << log_n) >> r[p];
+            a.cosets_per_block = 1;
+            cos_out[k] = 1;
+        }
+        r_out[k] = r[p];
+        a_out[k++] = a;
+        done += r[p];
+    }
+    return k;
 }
 
 // In-place DIF passes over `rows` rows of `buf` (independent size-2^log_n

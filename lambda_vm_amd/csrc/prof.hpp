@@ -18,7 +18,8 @@ void inject_fault_after(long calls);
 struct ProfScope {
     void *h;
     hipStream_t st;
-    ProfScope(const char *name, hipStream_t s) : h(prof_begin(name, s)), st(s) {}
+    // a null name records nothing
+    ProfScope(const char *name, hipStream_t s) : h(name ? prof_begin(name, s) : nullptr), st(s) {}
     ~ProfScope() { prof_end(h, st); }
 };
 }  // namespace lvm

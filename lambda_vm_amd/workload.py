@@ -64,3 +64,18 @@ WORKLOADS = {
     "single_table_2^22x64": dict(cfg=1, n=1 << 22, m=64, blowup=4, offset=GENERATOR, fri_final_log=8,
                                  ood_points=2),
 }
+
+
+def ragged_shapes(cfg: int = 3, count: int = 14, log_lo: int = 12, log_hi: int = 22, m_lo: int = 8,
+                  m_hi: int = 120, blowups=(2, 4)):
+    """Seeded shapes (n, m, blowup) of a batched ragged proof: log2 n uniform
+    in [log_lo, log_hi], m uniform in [m_lo, m_hi], blowup by a coin flip.
+    Three draws per table from the stream of column_seed(cfg, 0xFFFF, 0)."""
+    d = splitmix64(column_seed(cfg, 0xFFFF, 0), 3 * count)
+    out = []
+    for t in range(count):
+        log_n = log_lo + int(d[3 * t]) % (log_hi - log_lo + 1)
+        m = m_lo + int(d[3 * t + 1]) % (m_hi - m_lo + 1)
+        b = blowups[int(d[3 * t + 2]) & 1]
+        out.append((1 << log_n, m, b))
+    return out
