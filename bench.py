@@ -1,17 +1,25 @@
 """Benchmark of the STARK proving hot path on one MI355X.
 
-Workload (N=1): one table of 2^22 rows x 64 Goldilocks columns, uniform
-seeded values, blowup 4 coset LDE (2^24-row evaluation domain, 8 GiB) and the
-row-pair Keccak-256 Merkle tree over it, with the trace already resident in
-HBM (row-major, the layout the reference hands to its device path). A step is
-one full pass: inverse NTT, coset expansion, leaves, inner tree, root to host.
+Default workload (--workload ragged): the batched ragged commitment of one
+proof's tables: 14 tables with log2 rows uniform in [12, 22], widths uniform
+in [8, 120] columns and blowup 2 or 4 by a coin flip (seeded), traces
+uniform seeded Goldilocks values already resident in HBM. A step is the coset
+LDE of every table, its row-pair Keccak-256 leaves and Merkle tree, and all
+roots to the host. The batched path (one work-list launch per stage step
+across all tables, replayed as a captured HIP graph) is the measured value;
+the plain per-table path (one coset_lde_dev call per table) is timed on the
+same inputs beside it.
+
+--workload single: one table of 2^22 rows x 64 columns, blowup 4, with either
+the LDE + Merkle commitment only (--commit-only) or the full single-table
+rounds (OOD, DEEP, FRI, queries).
 
 Prints one JSON line: value = LDE cells (rows x columns of the evaluation
-domain) per second, roofline of the dominant kernel from HIP events recorded
-on the stream it runs on, and the oracle (C restatement of the reference CPU
-prover) timed on the host cores over a bounded sample.
+domains) per second, roofline of the dominant kernel launch from HIP events
+recorded on the stream it runs on, and the oracle (C restatement of the
+reference CPU prover) timed on the host cores over a bounded sample.
 
-usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--log-n 22] [--cols 64]
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ragged|single]
 """
 from __future__ import annotations
 
@@ -54,15 +62,14 @@ KERNELS = ["ntt_inv_pass0", "ntt_inv_pass1", "ntt_inv_pass2", "ntt_coset_pass0",
 STAGES = ["bary_denoms", "bary_sum", "deep_denoms", "deep_rows", "fri_fold_leaves", "fri_merkle", "fri_fold"]
 
 
-def measured_traffic(kernel: str, n: int, m: int, blowup: int):
+def measured_traffic(kernel: str, key: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE scaled by the copy-kernel calibration),
-    when one exists for this exact workload; otherwise None."""
+    when one exists for this exact workload (key); otherwise None."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(path):
         return None
     data = json.load(open(path))
-    key = f"{n}x{m}x{blowup}"
     rec = data.get("workloads", {}).get(key, {}).get(kernel)
     return rec.get("bytes_per_launch") if rec else None
 
@@ -109,14 +116,188 @@ def main() -> int:
     ap.add_argument("--commit-only", action="store_true", help="time only the LDE + Merkle commitment")
     ap.add_argument("--fri-k", type=int, default=6, help="log2 terminal degree (terminal 2^(log B + k))")
     ap.add_argument("--queries", type=int, default=30)
+    ap.add_argument("--workload", choices=["ragged", "single"], default="ragged")
+    ap.add_argument("--tables", type=int, default=14, help="ragged: number of tables")
     args = ap.parse_args()
 
     if args.gpus != 1:
         print("bench.py measures one GPU: the hot path is sharded by table on a single device "
               "and no multi-GPU figure is produced (run with --gpus 1)", file=sys.stderr)
         return 2
+    out = run_ragged(args) if args.workload == "ragged" else run_single(args)
+    print(json.dumps(out), flush=True)
+    return 0
 
-    import numpy as np
+
+def copy_rate_gbs(backend) -> float:
+    """Plain copy of 1 GiB (the calibration kernel for the traffic counters);
+    its rate is reported beside the roofline as the achievable HBM rate."""
+    cbytes = 1 << 30
+    src, dst = backend.DeviceBuffer(cbytes), backend.DeviceBuffer(cbytes)
+    backend.dev_copy(dst, src, cbytes)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        backend.dev_copy(dst, src, cbytes)
+    rate = 5 * 2 * cbytes / (time.perf_counter() - t0) / 1e9
+    src.free()
+    dst.free()
+    return rate
+
+
+def ragged_cpu_baseline(shapes, offset: int) -> dict:
+    """The oracle's LDE + commit of every table of the workload at half its
+    rows (bounded sample), on all host threads."""
+    import oracle
+    from lambda_vm_amd import workload
+
+    oracle.set_threads(0)
+    threads = oracle.get_threads()
+    sample = [(n // 2, m, b) for n, m, b in shapes]
+    traces = [workload.trace_table(3, t, n, m) for t, (n, m, b) in enumerate(sample)]
+    oracle.lde_commit(traces[0][: 1 << 10], 2, offset, want_lde=False)  # warm the code paths
+    t0 = time.perf_counter()
+    for tr, (n, m, b) in zip(traces, sample):
+        oracle.lde_commit(tr, b, offset, want_lde=False)
+    dt = time.perf_counter() - t0
+    cells = sum(n * m * b for n, m, b in sample)
+    return {
+        "value": round(cells / dt / 1e9, 5),
+        "unit": "Gcells/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"LDE+commit of the {len(sample)} tables at half their rows ({cells} LDE cells, {dt:.2f} s) "
+                  f"by the oracle on {threads} host threads",
+    }
+
+
+def run_ragged(args) -> dict:
+    from lambda_vm_amd import _native, backend, workload
+
+    _native.ensure_device(0)
+    offset = workload.GENERATOR
+    shapes = workload.ragged_shapes(cfg=3, count=args.tables)
+    log(f"[bench] ragged workload: {len(shapes)} tables {shapes}")
+    tables = []
+    for t, (n, m, b) in enumerate(shapes):
+        tr = workload.trace_table(3, t, n, m)
+        tables.append((backend.DeviceBuffer.from_array(tr), backend.coset_weights_dev(n, offset), n, m, b))
+        del tr
+    cells = sum(n * m * b for n, m, b in shapes)
+    copy_gbs = copy_rate_gbs(backend)
+
+    def per_table():
+        roots = []
+        for tr, w, n, m, b in tables:
+            h = backend.coset_lde_dev(tr, n, m, b, w)
+            roots.append(h.root)
+            h.close()
+        return roots
+
+    plan = backend.BatchPlan(tables, use_graph=True)
+    log(f"[bench] warmup {args.warmup}")
+    ref = per_table()
+    roots = None
+    for _ in range(max(1, args.warmup)):
+        roots = plan.run()
+    if roots != ref:
+        raise RuntimeError("batched roots differ from the per-table roots")
+    backend.sync()
+    log(f"[bench] timing {args.steps} batched steps (graph replay)")
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        if plan.run() != ref:
+            raise RuntimeError("root changed between steps")
+    dt = (time.perf_counter() - t0) / args.steps
+    value = cells / dt / 1e9
+
+    log(f"[bench] timing {args.steps} per-table steps")
+    for _ in range(args.warmup):
+        per_table()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        per_table()
+    dt_pt = (time.perf_counter() - t0) / args.steps
+
+    # the same launches without the graph, each bracketed by HIP events on
+    # the library stream, for the per-launch durations of the roofline
+    log(f"[bench] timing {args.steps} batched steps with per-launch events")
+    eager = backend.BatchPlan(tables, use_graph=False)
+    eager.run()
+    backend.sync()
+    backend.prof_reset()
+    backend.prof_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        eager.run()
+    dt_eager = (time.perf_counter() - t0) / args.steps
+    backend.prof_enable(False)
+    per_launch = {}
+    for name, nbytes, blocks in eager.launches():
+        cnt, tot = backend.prof_query(name)
+        if cnt:
+            per_launch[name] = (tot / cnt, nbytes, blocks)
+    eager.close()
+    plan.close()
+    dom = max(per_launch, key=lambda k: per_launch[k][0])
+    avg_ms, nbytes, _ = per_launch[dom]
+    achieved = nbytes / (avg_ms / 1e3) / 1e9
+    stage = {"ntt": 0.0, "leaves": 0.0, "merkle": 0.0, "roots": 0.0}
+    for name, (ms, _, _) in per_launch.items():
+        key = name.split("_")[1]
+        stage["ntt" if key == "ntt" else "leaves" if key == "leaves" else "merkle" if key == "merkle" else "roots"] += ms
+    roofline = {
+        "bound": "hbm",
+        "kernel": dom,
+        "achieved": round(achieved, 2),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": measured_traffic(dom, "ragged14"),
+        "avg_ms": round(avg_ms, 4),
+        "per_launch_ms": {k: round(v[0], 4) for k, v in per_launch.items()},
+        "per_launch_blocks": {k: v[2] for k, v in per_launch.items()},
+        "stage_ms": {k: round(v, 3) for k, v in stage.items()},
+        "copy_kernel_GBps": round(copy_gbs, 1),
+    }
+    out = {
+        "metric": "batched LDE+Keccak-Merkle Gcells/s vs per-table launch; p50/p99 proof latency",
+        "value": round(value, 4),
+        "unit": "Gcells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 (Goldilocks field)",
+        "data": "synthetic: SplitMix64-seeded traces uniform in [0,p), seeded ragged shapes",
+        "config": {
+            "workload": (f"batched ragged commitment of {len(shapes)} tables (log2 rows uniform in [12,22], "
+                         "8..120 columns, blowup 2 or 4), coset LDE (offset 7) + row-pair Keccak-256 Merkle "
+                         "tree per table, traces resident in HBM, one launch per stage step over all tables "
+                         "replayed as a HIP graph"),
+            "tables": [list(s) for s in shapes],
+            "lde_cells": cells,
+            "per_table_value": round(cells / dt_pt / 1e9, 4),
+            "per_table_ms_per_step": round(dt_pt * 1e3, 3),
+            "batched_over_per_table": round(dt_pt / dt, 3),
+            "batched_eager_ms_per_step": round(dt_eager * 1e3, 3),
+            "roots_sha": __import__("hashlib").sha256(b"".join(ref)).hexdigest(),
+        },
+        "roofline": roofline,
+    }
+    for tr, w, *_ in tables:
+        tr.free()
+        w.free()
+    if not args.no_cpu_baseline:
+        log("[bench] cpu baseline")
+        out["cpu_baseline"] = ragged_cpu_baseline(shapes, offset)
+    return out
+
+
+def run_single(args) -> dict:
+    import numpy as np  # noqa: F401
 
     from lambda_vm_amd import _native, backend, workload
 
@@ -142,17 +323,7 @@ def main() -> int:
                                             num_queries=args.queries)
         return proof.root + proof.transcript_state
 
-    # plain copy of 1 GiB (the calibration kernel for the traffic counters);
-    # its rate is reported beside the roofline as the achievable HBM rate
-    cbytes = 1 << 30
-    src, dst = backend.DeviceBuffer(cbytes), backend.DeviceBuffer(cbytes)
-    backend.dev_copy(dst, src, cbytes)
-    t0 = time.perf_counter()
-    for _ in range(5):
-        backend.dev_copy(dst, src, cbytes)
-    copy_gbs = 5 * 2 * cbytes / (time.perf_counter() - t0) / 1e9
-    src.free()
-    dst.free()
+    copy_gbs = copy_rate_gbs(backend)
 
     log(f"[bench] warmup {args.warmup}")
     root0 = None
@@ -202,7 +373,7 @@ def main() -> int:
             stage_ms[k] = round(tot / args.steps, 4)
     if stage_ms:
         roofline["other_stage_ms_per_step"] = stage_ms
-    traffic = measured_traffic(dom, n, m, B)
+    traffic = measured_traffic(dom, f"{n}x{m}x{B}")
     if traffic is not None:
         roofline["traffic"] = traffic
     out = {
@@ -232,8 +403,7 @@ def main() -> int:
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
         out["cpu_baseline"] = cpu_baseline(args.log_n, m, B, offset, args.commit_only, args.fri_k, args.queries)
-    print(json.dumps(out), flush=True)
-    return 0
+    return out
 
 
 if __name__ == "__main__":
