@@ -197,6 +197,51 @@ int lvm_deep_composition(const lvm_lde *main, const lvm_lde *aux, const lvm_lde 
                          const uint64_t *parts_ood, const uint64_t *gammas_tr, const uint64_t *gammas_h,
                          uint64_t *dev_out);
 
+/* ---------------- LogUp auxiliary trace ----------------
+ * Flat CSR descriptor of one table's bus interactions, field for field the
+ * reference's crypto/math-cuda/src/logup.rs:26-44 `LogupDescriptor` (built by
+ * crypto/stark/src/logup_gpu.rs:256-290 `build_fingerprint_descriptor`;
+ * canonical coefficients):
+ *   fp_k(i)   = z - (bus_ids[k] + sum_e alpha[elem_alpha_idx[e]] *
+ *               (elem_const[e] + sum_t term_coef[t] * main(i, term_col[t])))
+ *   m_k(i)    = mult_const[k] + sum_t mult_term_coef[t] * main(i, mult_term_col[t])
+ *   term_c(i) = sum over out_col_interactions of column c of m_k(i) / fp_k(i)
+ * with e over [elem_offsets[k], elem_offsets[k+1]), t over the term CSR. */
+typedef struct lvm_logup_desc {
+    size_t num_interactions;
+    const uint64_t *bus_ids;
+    const uint32_t *elem_offsets;
+    const uint32_t *elem_alpha_idx;
+    const uint64_t *elem_const;
+    const uint32_t *term_offsets;
+    const uint64_t *term_coef;
+    const uint32_t *term_col;
+    size_t num_out_cols; /* committed pairs + 1 (the absorbed column) */
+    const uint32_t *out_col_offsets;
+    const uint32_t *out_col_interactions;
+    const uint64_t *mult_const;
+    const uint32_t *mult_term_offsets;
+    const uint64_t *mult_term_coef;
+    const uint32_t *mult_term_col;
+} lvm_logup_desc;
+/* crypto/math-cuda/src/logup.rs:311 `logup_aux_resident`: the whole aux
+ * build on the device from a resident main trace whose element (row, col)
+ * is dev_main[row*row_stride + col*col_stride] (row-major: stride m, 1;
+ * the reference's column-major: 1, num_rows). dev_aux_out receives the
+ * row-major aux trace, num_out_cols Fp3 per row (committed term columns, then
+ * the accumulated column: acc[0] = 0, acc[i+1] = acc[i] + sum_c term_c(i) - L/N,
+ * crypto/stark/src/lookup.rs:1790-1834); dev_terms_out (nullable) every term
+ * column [(c*num_rows + row)*3 + limb]; L_out the table contribution L.
+ * alpha_powers: num_alpha Fp3 values (host). Descriptor arrays are host
+ * arrays; every index in them is checked against the sizes given. */
+int lvm_logup_aux_dev(const uint64_t *dev_main, size_t row_stride, size_t col_stride, size_t num_rows,
+                      size_t num_cols, const lvm_logup_desc *desc, const uint64_t *alpha_powers, size_t num_alpha,
+                      const uint64_t z[3], uint64_t *dev_aux_out, uint64_t *dev_terms_out, uint64_t L_out[3]);
+/* crypto/math-cuda/src/logup.rs:133 `logup_term_columns`: host column-major
+ * main (num_cols x num_rows) -> host term columns [(c*num_rows + row)*3 + limb]. */
+int lvm_logup_term_columns(const uint64_t *main_cols, size_t num_rows, size_t num_cols, const lvm_logup_desc *desc,
+                           const uint64_t *alpha_powers, size_t num_alpha, const uint64_t z[3], uint64_t *terms_out);
+
 /* ---------------- Fiat-Shamir transcript (host) ----------------
  * crypto/crypto/src/fiat_shamir/default_transcript.rs:31-146
  * (`DefaultTranscript`): Keccak-256 duplex, sample() = finalize + reverse +

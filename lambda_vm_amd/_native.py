@@ -77,6 +77,9 @@ SIGNATURES = {
     "lvm_barycentric_base_multi": (C.c_int, [C.c_void_p, C.c_uint64, u64p, C.c_size_t, u64p]),
     "lvm_deep_composition": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint64, u64p, C.c_size_t, u64p,
                                        u64p, u64p, u64p, u64p, C.c_void_p]),
+    "lvm_logup_aux_dev": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, C.c_void_p, u64p,
+                                    C.c_size_t, u64p, C.c_void_p, C.c_void_p, u64p]),
+    "lvm_logup_term_columns": (C.c_int, [u64p, C.c_size_t, C.c_size_t, C.c_void_p, u64p, C.c_size_t, u64p, u64p]),
     "lvm_transcript_new": (C.c_int, [u8p, C.c_size_t, vpp]),
     "lvm_transcript_clone": (C.c_int, [C.c_void_p, vpp]),
     "lvm_transcript_release": (C.c_int, [C.c_void_p]),
@@ -107,6 +110,15 @@ class TableDesc(C.Structure):
     """lvm_table_desc: one table of a batched commitment (device pointers)."""
     _fields_ = [("trace", C.c_void_p), ("weights", C.c_void_p), ("n", C.c_size_t), ("m", C.c_size_t),
                 ("blowup", C.c_size_t)]
+
+
+class LogupDesc(C.Structure):
+    """lvm_logup_desc: flat CSR descriptor of a table's bus interactions (host arrays)."""
+    _fields_ = [("num_interactions", C.c_size_t), ("bus_ids", C.c_void_p), ("elem_offsets", C.c_void_p),
+                ("elem_alpha_idx", C.c_void_p), ("elem_const", C.c_void_p), ("term_offsets", C.c_void_p),
+                ("term_coef", C.c_void_p), ("term_col", C.c_void_p), ("num_out_cols", C.c_size_t),
+                ("out_col_offsets", C.c_void_p), ("out_col_interactions", C.c_void_p), ("mult_const", C.c_void_p),
+                ("mult_term_offsets", C.c_void_p), ("mult_term_coef", C.c_void_p), ("mult_term_col", C.c_void_p)]
 
 
 def header_symbols() -> list[str]:

@@ -307,6 +307,63 @@ def get_ntt_plan(log_n: int) -> list[int]:
     return list(arr[:cnt.value])
 
 
+_LOGUP_ARRAYS = [("bus_ids", np.uint64), ("elem_offsets", np.uint32), ("elem_alpha_idx", np.uint32),
+                 ("elem_const", np.uint64), ("term_offsets", np.uint32), ("term_coef", np.uint64),
+                 ("term_col", np.uint32), ("out_col_offsets", np.uint32), ("out_col_interactions", np.uint32),
+                 ("mult_const", np.uint64), ("mult_term_offsets", np.uint32), ("mult_term_coef", np.uint64),
+                 ("mult_term_col", np.uint32)]
+
+
+def _logup_desc(d):
+    """ctypes lvm_logup_desc over a FingerprintDescriptor (arrays kept alive in the result)."""
+    s = N.LogupDesc()
+    s.num_interactions = d.num_interactions
+    s.num_out_cols = d.num_out_cols
+    keep = []
+    for name, dt in _LOGUP_ARRAYS:
+        a = np.ascontiguousarray(getattr(d, name), dtype=dt)
+        if a.size == 0:
+            a = np.zeros(1, dtype=dt)
+        keep.append(a)
+        setattr(s, name, a.ctypes.data)
+    return s, keep
+
+
+def logup_aux_dev(main: DeviceBuffer, num_rows: int, num_cols: int, desc, alpha_powers, z,
+                  column_major: bool = False, want_terms: bool = False):
+    """Resident LogUp aux build (crypto/math-cuda/src/logup.rs:311
+    `logup_aux_resident`). main: device trace, row-major (num_rows, num_cols)
+    or column-major. Returns (aux DeviceBuffer of num_rows x num_out_cols Fp3,
+    row-major; L as 3 ints; term columns DeviceBuffer or None)."""
+    N.ensure_device()
+    s, keep = _logup_desc(desc)
+    al = _u64(alpha_powers).reshape(-1)
+    zz = _u64(z).reshape(3)
+    C_ = desc.num_out_cols
+    aux = DeviceBuffer(num_rows * C_ * 24)
+    terms = DeviceBuffer(num_rows * C_ * 24) if want_terms else None
+    L = np.zeros(3, dtype=np.uint64)
+    rs, cs = (1, num_rows) if column_major else (num_cols, 1)
+    N.call("lvm_logup_aux_dev", C.c_void_p(main.ptr), rs, cs, num_rows, num_cols, C.byref(s), _p64(al),
+           al.size // 3, _p64(zz), C.c_void_p(aux.ptr), C.c_void_p(terms.ptr) if terms else None, _p64(L))
+    return aux, [int(x) for x in L], terms
+
+
+def logup_term_columns(main_cols, desc, alpha_powers, z) -> np.ndarray:
+    """Host column-major main (num_cols, num_rows) -> term columns (num_out_cols,
+    num_rows, 3) (crypto/math-cuda/src/logup.rs:133 `logup_term_columns`)."""
+    N.ensure_device()
+    mc = _u64(main_cols)
+    num_cols, num_rows = mc.shape
+    s, keep = _logup_desc(desc)
+    al = _u64(alpha_powers).reshape(-1)
+    zz = _u64(z).reshape(3)
+    out = np.zeros((desc.num_out_cols, num_rows, 3), dtype=np.uint64)
+    N.call("lvm_logup_term_columns", _p64(mc), num_rows, num_cols, C.byref(s), _p64(al), al.size // 3, _p64(zz),
+           _p64(out))
+    return out
+
+
 def build_merkle_tree(leaves) -> np.ndarray:
     """(num_leaves, 32) digests -> (2L-1, 32) nodes, root first."""
     N.ensure_device()

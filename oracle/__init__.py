@@ -419,3 +419,78 @@ def deep_composition(main: np.ndarray, offset: int, z_points, trace_ood, gammas_
                                 _p64(tood), p(pood), _p64(gtr), p(gh), _p64(out))
     assert rc == 0
     return out
+
+
+class _LogupDesc(C.Structure):
+    _fields_ = [("num_interactions", C.c_size_t), ("bus_ids", C.c_void_p), ("elem_offsets", C.c_void_p),
+                ("elem_alpha_idx", C.c_void_p), ("elem_const", C.c_void_p), ("term_offsets", C.c_void_p),
+                ("term_coef", C.c_void_p), ("term_col", C.c_void_p), ("num_out_cols", C.c_size_t),
+                ("out_col_offsets", C.c_void_p), ("out_col_interactions", C.c_void_p), ("mult_const", C.c_void_p),
+                ("mult_term_offsets", C.c_void_p), ("mult_term_coef", C.c_void_p), ("mult_term_col", C.c_void_p)]
+
+
+_DESC_ARRAYS = [("bus_ids", np.uint64), ("elem_offsets", np.uint32), ("elem_alpha_idx", np.uint32),
+                ("elem_const", np.uint64), ("term_offsets", np.uint32), ("term_coef", np.uint64),
+                ("term_col", np.uint32), ("out_col_offsets", np.uint32), ("out_col_interactions", np.uint32),
+                ("mult_const", np.uint64), ("mult_term_offsets", np.uint32), ("mult_term_coef", np.uint64),
+                ("mult_term_col", np.uint32)]
+
+
+def _logup_desc(d):
+    """ctypes view of a descriptor object with the FingerprintDescriptor fields;
+    returns (struct, arrays kept alive)."""
+    keep = {}
+    s = _LogupDesc()
+    s.num_interactions = d.num_interactions
+    s.num_out_cols = d.num_out_cols
+    for name, dt in _DESC_ARRAYS:
+        a = np.ascontiguousarray(getattr(d, name), dtype=dt)
+        if a.size == 0:
+            a = np.zeros(1, dtype=dt)
+        keep[name] = a
+        setattr(s, name, a.ctypes.data)
+    return s, keep
+
+
+def _declare_logup(L):
+    if not hasattr(L.ora_logup_aux, "_declared"):
+        L.ora_logup_aux.restype = C.c_int
+        L.ora_logup_aux.argtypes = [C.c_void_p, u64p, C.c_size_t, C.c_size_t, C.c_size_t, u64p, u64p, u64p, u64p,
+                                    u64p]
+        L.ora_logup_fingerprints.restype = C.c_int
+        L.ora_logup_fingerprints.argtypes = [C.c_void_p, u64p, C.c_size_t, C.c_size_t, C.c_size_t, u64p, u64p, u64p]
+        L.ora_logup_aux._declared = True
+
+
+def logup_fingerprints(desc, main: np.ndarray, alpha_powers, z) -> np.ndarray:
+    """(K, n, 3) fingerprints of a row-major (n, m) main trace."""
+    L = lib()
+    _declare_logup(L)
+    main = np.ascontiguousarray(main, dtype=np.uint64)
+    n, m = main.shape
+    s, keep = _logup_desc(desc)
+    al = np.ascontiguousarray(alpha_powers, dtype=np.uint64).reshape(-1)
+    zz = np.ascontiguousarray(z, dtype=np.uint64)
+    out = np.zeros((desc.num_interactions, n, 3), dtype=np.uint64)
+    rc = L.ora_logup_fingerprints(C.addressof(s), _p64(main), m, 1, n, _p64(al), _p64(zz), _p64(out))
+    assert rc == 0
+    return out
+
+
+def logup_aux(desc, main: np.ndarray, alpha_powers, z):
+    """(aux (n, C, 3), L (3,), terms (C, n, 3)) for a row-major (n, m) main
+    trace: committed term columns then the accumulated column."""
+    L = lib()
+    _declare_logup(L)
+    main = np.ascontiguousarray(main, dtype=np.uint64)
+    n, m = main.shape
+    s, keep = _logup_desc(desc)
+    al = np.ascontiguousarray(alpha_powers, dtype=np.uint64).reshape(-1)
+    zz = np.ascontiguousarray(z, dtype=np.uint64)
+    C_ = desc.num_out_cols
+    aux = np.zeros((n, C_, 3), dtype=np.uint64)
+    terms = np.zeros((C_, n, 3), dtype=np.uint64)
+    Lo = np.zeros(3, dtype=np.uint64)
+    rc = L.ora_logup_aux(C.addressof(s), _p64(main), m, 1, n, _p64(al), _p64(zz), _p64(terms), _p64(aux), _p64(Lo))
+    assert rc == 0, rc
+    return aux, Lo, terms

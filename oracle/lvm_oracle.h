@@ -124,6 +124,36 @@ int ora_deep_composition(const uint64_t *main, size_t m, const uint64_t *aux, si
                          const uint64_t *z_parts, const uint64_t *trace_ood, const uint64_t *parts_ood,
                          const uint64_t *gammas_tr, const uint64_t *gammas_h, uint64_t *out);
 
+/* ---- LogUp (crypto/stark/src/lookup.rs, logup_gpu.rs) ---- */
+/* Flat CSR descriptor of a table's bus interactions (logup_gpu.rs:100-142). */
+typedef struct {
+    size_t num_interactions;
+    const uint64_t *bus_ids;       /* [K] */
+    const uint32_t *elem_offsets;  /* [K+1] */
+    const uint32_t *elem_alpha_idx;
+    const uint64_t *elem_const;
+    const uint32_t *term_offsets;  /* [E+1] */
+    const uint64_t *term_coef;
+    const uint32_t *term_col;
+    size_t num_out_cols;           /* committed pairs + 1 */
+    const uint32_t *out_col_offsets;
+    const uint32_t *out_col_interactions;
+    const uint64_t *mult_const;    /* [K] */
+    const uint32_t *mult_term_offsets; /* [K+1] */
+    const uint64_t *mult_term_coef;
+    const uint32_t *mult_term_col;
+} ora_logup_desc;
+/* main value (row, col) = main[row*row_stride + col*col_stride]; alpha: Fp3 powers.
+ * fp_out: [(k*n + row)*3 + limb] */
+int ora_logup_fingerprints(const ora_logup_desc *d, const uint64_t *main, size_t row_stride, size_t col_stride,
+                           size_t n, const uint64_t *alpha, const uint64_t z[3], uint64_t *fp_out);
+/* terms_out (nullable): [(c*n + row)*3 + limb] for all num_out_cols columns;
+ * aux_out: row-major n x num_out_cols Fp3 (committed columns, then the
+ * accumulated column); L_out: the table contribution. */
+int ora_logup_aux(const ora_logup_desc *d, const uint64_t *main, size_t row_stride, size_t col_stride, size_t n,
+                  const uint64_t *alpha, const uint64_t z[3], uint64_t *terms_out, uint64_t *aux_out,
+                  uint64_t L_out[3]);
+
 /* ---- pipelines used by the CPU baseline ---- */
 /* LDE + row-pair commit of a base table: out_lde (nullable) row-major natural. */
 int ora_lde_commit(const uint64_t *trace, size_t n, size_t m, size_t blowup, uint64_t offset,
