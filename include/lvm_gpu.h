@@ -242,6 +242,62 @@ int lvm_logup_aux_dev(const uint64_t *dev_main, size_t row_stride, size_t col_st
 int lvm_logup_term_columns(const uint64_t *main_cols, size_t num_rows, size_t num_cols, const lvm_logup_desc *desc,
                            const uint64_t *alpha_powers, size_t num_alpha, const uint64_t z[3], uint64_t *terms_out);
 
+/* ---------------- constraint evaluation / composition polynomial ----------------
+ * Program: the flat constraint program of crypto/stark/src/constraint_ir/
+ * device.rs (`DeviceProgram`), packed as in crypto/stark/src/constraint_ir/
+ * gpu_interp.rs:30-38: nodes 2 u64 each (op | a << 32, b | res << 32);
+ * roots one u64 each (slot | ext_bit << 31); ext tables 3 u64 per element.
+ * These are the inputs of crypto/math-cuda/src/constraint_interp.rs:53
+ * `eval_constraints_on_device`. */
+typedef struct lvm_program {
+    const uint64_t *nodes;
+    size_t num_nodes, num_base_slots, num_ext_slots;
+    const uint64_t *base_consts;
+    size_t num_base_consts;
+    const uint64_t *ext_consts;
+    size_t num_ext_consts;
+    const uint64_t *roots;
+    size_t num_roots;
+    const uint64_t *rap_challenges;
+    size_t num_rap;
+    const uint64_t *alpha_powers;
+    size_t num_alpha;
+    const uint64_t *table_offset; /* 3 u64, nullable when unused */
+} lvm_program;
+/* Composition inputs (crypto/math-cuda/src/constraint_interp.rs:157-177
+ * `CompositionAccum`), with the zerofiers described by their parameters
+ * instead of precomputed vectors: constraint c belongs to zerofier group
+ * root_group[c], whose inverse zerofier is prod_{j=1..e}(x - w^(n-j)) / (x^n - 1)
+ * with e = group_exemptions[g] (crypto/stark/src/constraints/zerofier.rs);
+ * boundary constraint b contributes beta_b (t_b(x) - value_b) / (x - w^step_b)
+ * (crypto/stark/src/domain.rs:112-131). x runs over coset_offset * <w_N>. */
+typedef struct lvm_composition {
+    size_t trace_rows;          /* n */
+    uint64_t coset_offset;
+    const uint64_t *beta;       /* num_roots Fp3 */
+    const uint32_t *root_group; /* num_roots */
+    const uint32_t *group_exemptions;
+    size_t num_groups;          /* 1..4 */
+    size_t num_boundary;
+    const uint32_t *b_col;
+    const uint32_t *b_is_aux;   /* 0 main column, 1 aux (Fp3) column */
+    const uint64_t *b_step;     /* trace row of the constraint */
+    const uint64_t *b_value;    /* Fp3 each */
+    const uint64_t *b_beta;     /* Fp3 each */
+} lvm_composition;
+/* crypto/math-cuda/src/constraint_interp.rs:353/:409 `eval_composition_on_device(_keep)`:
+ * H over the LDE of the resident main (base) and aux (Fp3, nullable) tables;
+ * a frame offset k reads the row k * next_step further (next_step = blowup for
+ * one trace step). dev_h_out: N Fp3 in device memory, rows in the tables'
+ * bit-reversed order, or natural order when natural_order != 0. */
+int lvm_eval_composition(const lvm_program *program, const lvm_lde *main, const lvm_lde *aux, size_t next_step,
+                         const lvm_composition *comp, int natural_order, uint64_t *dev_h_out);
+/* crypto/math-cuda/src/constraint_interp.rs:53 `eval_constraints_on_device`:
+ * dev_evals_out [(c*N + row)*3 + limb] in natural row order; base-rooted
+ * constraints carry their value in component 0. */
+int lvm_eval_constraints(const lvm_program *program, const lvm_lde *main, const lvm_lde *aux, size_t next_step,
+                         uint64_t *dev_evals_out);
+
 /* ---------------- Fiat-Shamir transcript (host) ----------------
  * crypto/crypto/src/fiat_shamir/default_transcript.rs:31-146
  * (`DefaultTranscript`): Keccak-256 duplex, sample() = finalize + reverse +

@@ -80,6 +80,9 @@ SIGNATURES = {
     "lvm_logup_aux_dev": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_size_t, C.c_void_p, u64p,
                                     C.c_size_t, u64p, C.c_void_p, C.c_void_p, u64p]),
     "lvm_logup_term_columns": (C.c_int, [u64p, C.c_size_t, C.c_size_t, C.c_void_p, u64p, C.c_size_t, u64p, u64p]),
+    "lvm_eval_composition": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_int,
+                                       C.c_void_p]),
+    "lvm_eval_constraints": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "lvm_transcript_new": (C.c_int, [u8p, C.c_size_t, vpp]),
     "lvm_transcript_clone": (C.c_int, [C.c_void_p, vpp]),
     "lvm_transcript_release": (C.c_int, [C.c_void_p]),
@@ -119,6 +122,23 @@ class LogupDesc(C.Structure):
                 ("term_coef", C.c_void_p), ("term_col", C.c_void_p), ("num_out_cols", C.c_size_t),
                 ("out_col_offsets", C.c_void_p), ("out_col_interactions", C.c_void_p), ("mult_const", C.c_void_p),
                 ("mult_term_offsets", C.c_void_p), ("mult_term_coef", C.c_void_p), ("mult_term_col", C.c_void_p)]
+
+
+class Program(C.Structure):
+    """lvm_program: flat constraint program + per-proof uniforms (host arrays)."""
+    _fields_ = [("nodes", C.c_void_p), ("num_nodes", C.c_size_t), ("num_base_slots", C.c_size_t),
+                ("num_ext_slots", C.c_size_t), ("base_consts", C.c_void_p), ("num_base_consts", C.c_size_t),
+                ("ext_consts", C.c_void_p), ("num_ext_consts", C.c_size_t), ("roots", C.c_void_p),
+                ("num_roots", C.c_size_t), ("rap_challenges", C.c_void_p), ("num_rap", C.c_size_t),
+                ("alpha_powers", C.c_void_p), ("num_alpha", C.c_size_t), ("table_offset", C.c_void_p)]
+
+
+class Composition(C.Structure):
+    """lvm_composition: combination coefficients, zerofier groups, boundary constraints."""
+    _fields_ = [("trace_rows", C.c_size_t), ("coset_offset", C.c_uint64), ("beta", C.c_void_p),
+                ("root_group", C.c_void_p), ("group_exemptions", C.c_void_p), ("num_groups", C.c_size_t),
+                ("num_boundary", C.c_size_t), ("b_col", C.c_void_p), ("b_is_aux", C.c_void_p),
+                ("b_step", C.c_void_p), ("b_value", C.c_void_p), ("b_beta", C.c_void_p)]
 
 
 def header_symbols() -> list[str]:

@@ -364,6 +364,80 @@ def logup_term_columns(main_cols, desc, alpha_powers, z) -> np.ndarray:
     return out
 
 
+def _arr_ptr(x, dt, keep):
+    a = np.ascontiguousarray(x if x is not None and len(x) else [0], dtype=dt).reshape(-1)
+    keep.append(a)
+    return a.ctypes.data
+
+
+def _program_struct(dev, rap, alpha, table_offset, keep):
+    """lvm_program over a lowered DeviceProgram (lambda_vm_amd.constraint_ir)."""
+    s = N.Program()
+    s.nodes = _arr_ptr(dev.packed_nodes(), np.uint64, keep)
+    s.num_nodes = len(dev.nodes)
+    s.num_base_slots = dev.num_base_slots
+    s.num_ext_slots = dev.num_ext_slots
+    s.base_consts = _arr_ptr(dev.base_consts, np.uint64, keep)
+    s.num_base_consts = len(dev.base_consts)
+    ec = np.asarray(dev.ext_consts, dtype=np.uint64).reshape(-1)
+    s.ext_consts = _arr_ptr(ec, np.uint64, keep)
+    s.num_ext_consts = ec.size // 3
+    s.roots = _arr_ptr(np.asarray(dev.roots, dtype=np.uint64), np.uint64, keep)
+    s.num_roots = len(dev.roots)
+    r = np.asarray(rap if rap is not None else [], dtype=np.uint64).reshape(-1)
+    a = np.asarray(alpha if alpha is not None else [], dtype=np.uint64).reshape(-1)
+    s.rap_challenges = _arr_ptr(r, np.uint64, keep)
+    s.num_rap = r.size // 3
+    s.alpha_powers = _arr_ptr(a, np.uint64, keep)
+    s.num_alpha = a.size // 3
+    s.table_offset = _arr_ptr(table_offset, np.uint64, keep) if table_offset is not None else None
+    return s
+
+
+def eval_composition(dev, main: GpuLde, aux, next_step: int, trace_rows: int, coset_offset: int, betas,
+                     root_group, group_exemptions, boundary=(), b_betas=None, rap=None, alpha=None,
+                     table_offset=None, natural_order: bool = False) -> DeviceBuffer:
+    """Composition polynomial H on the LDE (crypto/math-cuda/src/constraint_interp.rs:409
+    `eval_composition_on_device_keep`): a device buffer of N Fp3, rows in the
+    tables' bit-reversed order unless natural_order. boundary: (col, is_aux,
+    step, value3) tuples."""
+    N.ensure_device()
+    keep = []
+    s = _program_struct(dev, rap, alpha, table_offset, keep)
+    c = N.Composition()
+    c.trace_rows = trace_rows
+    c.coset_offset = coset_offset
+    c.beta = _arr_ptr(np.asarray(betas, dtype=np.uint64).reshape(-1), np.uint64, keep)
+    c.root_group = _arr_ptr(root_group, np.uint32, keep)
+    c.group_exemptions = _arr_ptr(group_exemptions, np.uint32, keep)
+    c.num_groups = len(group_exemptions)
+    c.num_boundary = len(boundary)
+    c.b_col = _arr_ptr([b[0] for b in boundary], np.uint32, keep)
+    c.b_is_aux = _arr_ptr([int(b[1]) for b in boundary], np.uint32, keep)
+    c.b_step = _arr_ptr([b[2] for b in boundary], np.uint64, keep)
+    c.b_value = _arr_ptr(np.asarray([b[3] for b in boundary], dtype=np.uint64).reshape(-1) if boundary else None,
+                         np.uint64, keep)
+    c.b_beta = _arr_ptr(np.asarray(b_betas, dtype=np.uint64).reshape(-1) if boundary else None, np.uint64, keep)
+    out = DeviceBuffer(main.lde_rows * 24)
+    N.call("lvm_eval_composition", C.byref(s), main.h, aux.h if aux is not None else None, next_step, C.byref(c),
+           int(natural_order), C.c_void_p(out.ptr))
+    return out
+
+
+def eval_constraints(dev, main: GpuLde, aux, next_step: int, rap=None, alpha=None, table_offset=None) -> np.ndarray:
+    """(num_roots, N, 3) per-constraint values, natural row order
+    (crypto/math-cuda/src/constraint_interp.rs:53 `eval_constraints_on_device`)."""
+    N.ensure_device()
+    keep = []
+    s = _program_struct(dev, rap, alpha, table_offset, keep)
+    out = DeviceBuffer(len(dev.roots) * main.lde_rows * 24)
+    N.call("lvm_eval_constraints", C.byref(s), main.h, aux.h if aux is not None else None, next_step,
+           C.c_void_p(out.ptr))
+    res = out.to_array(np.uint64, (len(dev.roots), main.lde_rows, 3))
+    out.free()
+    return res
+
+
 def build_merkle_tree(leaves) -> np.ndarray:
     """(num_leaves, 32) digests -> (2L-1, 32) nodes, root first."""
     N.ensure_device()

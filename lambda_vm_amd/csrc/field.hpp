@@ -105,6 +105,14 @@ LVM_HD uint64_t inv(uint64_t a) {
     return mul(t, x32);
 }
 
+// primitive 2^k-th root of unity: the 2^32-th root of goldilocks.rs:540
+// squared 32 - k times (crypto/math/src/field/traits.rs:83-95)
+LVM_HD uint64_t root_of_unity(unsigned k) {
+    uint64_t r = 1753635133440165772ULL;
+    for (unsigned i = k; i < 32; i++) r = sqr(r);
+    return r;
+}
+
 // ---------------- Fp3 ----------------
 struct fp3 {
     uint64_t c0, c1, c2;

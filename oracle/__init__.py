@@ -494,3 +494,101 @@ def logup_aux(desc, main: np.ndarray, alpha_powers, z):
     rc = L.ora_logup_aux(C.addressof(s), _p64(main), m, 1, n, _p64(al), _p64(zz), _p64(terms), _p64(aux), _p64(Lo))
     assert rc == 0, rc
     return aux, Lo, terms
+
+
+class _Program(C.Structure):
+    _fields_ = [("nodes", C.c_void_p), ("num_nodes", C.c_size_t), ("num_base_slots", C.c_size_t),
+                ("num_ext_slots", C.c_size_t), ("base_consts", C.c_void_p), ("ext_consts", C.c_void_p),
+                ("roots", C.c_void_p), ("num_roots", C.c_size_t), ("rap", C.c_void_p), ("alpha", C.c_void_p),
+                ("table_offset", C.c_void_p)]
+
+
+class _Composition(C.Structure):
+    _fields_ = [("trace_rows", C.c_size_t), ("coset_offset", C.c_uint64), ("beta", C.c_void_p),
+                ("root_group", C.c_void_p), ("group_exemptions", C.c_void_p), ("num_groups", C.c_size_t),
+                ("num_boundary", C.c_size_t), ("b_col", C.c_void_p), ("b_is_aux", C.c_void_p),
+                ("b_step", C.c_void_p), ("b_value", C.c_void_p), ("b_beta", C.c_void_p)]
+
+
+def _arr(x, dt, keep):
+    a = np.ascontiguousarray(x if x is not None and len(x) else [0], dtype=dt).reshape(-1)
+    keep.append(a)
+    return a.ctypes.data
+
+
+def _program(dev, rap, alpha, table_offset, keep):
+    s = _Program()
+    s.nodes = _arr(dev.packed_nodes(), np.uint64, keep)
+    s.num_nodes = len(dev.nodes)
+    s.num_base_slots = dev.num_base_slots
+    s.num_ext_slots = dev.num_ext_slots
+    s.base_consts = _arr(dev.base_consts, np.uint64, keep)
+    s.ext_consts = _arr(np.asarray(dev.ext_consts).reshape(-1), np.uint64, keep)
+    s.roots = _arr(dev.roots.astype(np.uint64), np.uint64, keep)
+    s.num_roots = len(dev.roots)
+    s.rap = _arr(np.asarray(rap, dtype=np.uint64).reshape(-1) if rap is not None else None, np.uint64, keep)
+    s.alpha = _arr(np.asarray(alpha, dtype=np.uint64).reshape(-1) if alpha is not None else None, np.uint64, keep)
+    s.table_offset = _arr(np.asarray(table_offset if table_offset is not None else (0, 0, 0), dtype=np.uint64),
+                          np.uint64, keep)
+    return s
+
+
+def _declare_constraints(L):
+    if not hasattr(L.ora_eval_composition, "_declared"):
+        L.ora_eval_constraints.restype = C.c_int
+        L.ora_eval_constraints.argtypes = [C.c_void_p, u64p, C.c_size_t, u64p, C.c_size_t, C.c_size_t, C.c_size_t,
+                                           u64p]
+        L.ora_eval_composition.restype = C.c_int
+        L.ora_eval_composition.argtypes = [C.c_void_p, u64p, C.c_size_t, u64p, C.c_size_t, C.c_size_t, C.c_size_t,
+                                           C.c_void_p, u64p]
+        L.ora_eval_composition._declared = True
+
+
+def eval_constraints(dev, main_lde, aux_lde, next_step, rap=None, alpha=None, table_offset=None) -> np.ndarray:
+    """(num_roots, N, 3) constraint values on a natural-order LDE (main (N, m),
+    aux (N, 3*ma) or None)."""
+    L = lib()
+    _declare_constraints(L)
+    keep = []
+    s = _program(dev, rap, alpha, table_offset, keep)
+    main = np.ascontiguousarray(main_lde, dtype=np.uint64)
+    N, m = main.shape
+    aux = np.ascontiguousarray(aux_lde if aux_lde is not None else np.zeros((N, 3)), dtype=np.uint64)
+    ma = aux.shape[1] // 3
+    out = np.zeros((len(dev.roots), N, 3), dtype=np.uint64)
+    rc = L.ora_eval_constraints(C.addressof(s), _p64(main), m, _p64(aux), ma, N, next_step, _p64(out))
+    assert rc == 0
+    return out
+
+
+def eval_composition(dev, main_lde, aux_lde, next_step, trace_rows, coset_offset, betas, root_group,
+                     group_exemptions, boundary=(), b_betas=None, rap=None, alpha=None, table_offset=None):
+    """(N, 3) composition evaluations, natural order. boundary: (col, is_aux,
+    step, value3) tuples."""
+    L = lib()
+    _declare_constraints(L)
+    keep = []
+    s = _program(dev, rap, alpha, table_offset, keep)
+    main = np.ascontiguousarray(main_lde, dtype=np.uint64)
+    N, m = main.shape
+    aux = np.ascontiguousarray(aux_lde if aux_lde is not None else np.zeros((N, 3)), dtype=np.uint64)
+    ma = aux.shape[1] // 3
+    c = _Composition()
+    c.trace_rows = trace_rows
+    c.coset_offset = coset_offset
+    c.beta = _arr(np.asarray(betas, dtype=np.uint64).reshape(-1), np.uint64, keep)
+    c.root_group = _arr(root_group, np.uint32, keep)
+    c.group_exemptions = _arr(group_exemptions, np.uint32, keep)
+    c.num_groups = len(group_exemptions)
+    c.num_boundary = len(boundary)
+    c.b_col = _arr([b[0] for b in boundary], np.uint32, keep)
+    c.b_is_aux = _arr([int(b[1]) for b in boundary], np.uint32, keep)
+    c.b_step = _arr([b[2] for b in boundary], np.uint64, keep)
+    c.b_value = _arr(np.asarray([b[3] for b in boundary], dtype=np.uint64).reshape(-1) if boundary else None,
+                     np.uint64, keep)
+    c.b_beta = _arr(np.asarray(b_betas, dtype=np.uint64).reshape(-1) if boundary else None, np.uint64, keep)
+    out = np.zeros((N, 3), dtype=np.uint64)
+    rc = L.ora_eval_composition(C.addressof(s), _p64(main), m, _p64(aux), ma, N, next_step, C.addressof(c),
+                                _p64(out))
+    assert rc == 0, rc
+    return out

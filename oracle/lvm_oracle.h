@@ -154,6 +154,36 @@ int ora_logup_aux(const ora_logup_desc *d, const uint64_t *main, size_t row_stri
                   const uint64_t *alpha, const uint64_t z[3], uint64_t *terms_out, uint64_t *aux_out,
                   uint64_t L_out[3]);
 
+/* ---- constraint programs and the composition polynomial ----
+ * Flat program (crypto/stark/src/constraint_ir/device.rs): nodes are 2 u64
+ * each (op | a << 32, b | res << 32); ext tables hold 3 u64 per element. */
+typedef struct {
+    const uint64_t *nodes;
+    size_t num_nodes, num_base_slots, num_ext_slots;
+    const uint64_t *base_consts, *ext_consts;
+    const uint64_t *roots;
+    size_t num_roots;
+    const uint64_t *rap, *alpha, *table_offset;
+} ora_program;
+typedef struct {
+    size_t trace_rows;                /* n; N / n is the blowup */
+    uint64_t coset_offset;
+    const uint64_t *beta;             /* num_roots Fp3 */
+    const uint32_t *root_group;       /* zerofier group of each constraint */
+    const uint32_t *group_exemptions; /* end exemptions of each group */
+    size_t num_groups;
+    size_t num_boundary;
+    const uint32_t *b_col, *b_is_aux;
+    const uint64_t *b_step;
+    const uint64_t *b_value, *b_beta; /* Fp3 each */
+} ora_composition;
+/* main: N x m natural row-major; aux: N x ma Fp3 natural row-major.
+ * evals_out: [(c*N + i)*3 + limb]; h_out: N Fp3, natural order. */
+int ora_eval_constraints(const ora_program *p, const uint64_t *main, size_t m, const uint64_t *aux, size_t ma,
+                         size_t N, size_t next_step, uint64_t *evals_out);
+int ora_eval_composition(const ora_program *p, const uint64_t *main, size_t m, const uint64_t *aux, size_t ma,
+                         size_t N, size_t next_step, const ora_composition *c, uint64_t *h_out);
+
 /* ---- pipelines used by the CPU baseline ---- */
 /* LDE + row-pair commit of a base table: out_lde (nullable) row-major natural. */
 int ora_lde_commit(const uint64_t *trace, size_t n, size_t m, size_t blowup, uint64_t offset,

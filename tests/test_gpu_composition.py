@@ -1,0 +1,113 @@
+"""GPU constraint evaluation and composition polynomial (lvm_eval_constraints /
+lvm_eval_composition) against the oracle on the same seeded inputs:
+FibonacciMultiColumn, Quadratic and a 4-bus LogUp table, value slots in LDS
+and in global scratch, bit-reversed and natural output order, a low-degree
+check of the GPU output for valid traces, and program validation."""
+import random
+
+import numpy as np
+import pytest
+
+from lambda_vm_amd import air as A
+from lambda_vm_amd import workload
+from test_oracle_composition import G, T, _high_coeffs, rand3, scale3
+from test_oracle_logup import alpha_powers
+
+pytestmark = pytest.mark.gpu
+P = workload.P
+
+
+def canon(a):
+    a = np.asarray(a, dtype=np.uint64)
+    return np.where(a >= np.uint64(P), a - np.uint64(P), a)
+
+
+def _bitrev_perm(N):
+    bits = N.bit_length() - 1
+    return np.array([int(format(i, f"0{bits}b")[::-1], 2) if bits else 0 for i in range(N)], dtype=np.int64)
+
+
+def _case(gpu, ora, which, n, B, m, rng, valid=True):
+    w = ora.lde_weights(n, G)
+    aux_tr = None
+    rap = alpha = toff = None
+    if which == "fib":
+        air = A.fibonacci_multi_column(m)
+        if valid:
+            seeds = np.array([[rng.randrange(P), rng.randrange(P)] for _ in range(m)], dtype=np.uint64)
+            trace = A.fibonacci_trace(n, m, seeds)
+        else:
+            trace = workload.trace_table(7, 1, n, m)
+        A.with_boundaries_from_trace(air, trace)
+    elif which == "quad":
+        air = A.quadratic_multi(m)
+        trace = A.quadratic_trace(n, m, np.array([rng.randrange(P) for _ in range(m)], dtype=np.uint64))
+        A.with_boundaries_from_trace(air, trace)
+    else:
+        air = A.logup_air(24, A.four_bus_interactions())
+        d = A.descriptor(air)
+        trace = workload.trace_table(8, 2, n, 24)
+        z, al = rand3(rng), rand3(rng)
+        ap = alpha_powers(al, d.alpha_powers_len)
+        aux, L, _ = ora.logup_aux(d, trace, ap, z)
+        aux_tr = aux.reshape(n, -1)
+        rap, alpha = [z, al], ap
+        toff = scale3(T(L), pow(n, P - 2, P))
+    main_h, main_lde = gpu.coset_lde_row_major_with_merkle_tree_keep(trace, n, trace.shape[1], B, w)
+    aux_h = aux_lde = None
+    if aux_tr is not None:
+        aux_h, aux_lde = gpu.coset_lde_ext3_row_major_with_merkle_tree_keep(aux_tr, n, aux_tr.shape[1] // 3, B, w)
+    groups, ex = air.groups()
+    betas = [rand3(rng) for _ in range(air.num_constraints)]
+    bnd = [(b.col, b.is_aux, b.step, b.value) for b in air.boundary]
+    b_betas = [rand3(rng) for _ in bnd]
+    return dict(air=air, main_h=main_h, aux_h=aux_h, main_lde=main_lde, aux_lde=aux_lde, groups=groups, ex=ex,
+                betas=betas, bnd=bnd, b_betas=b_betas, rap=rap, alpha=alpha, toff=toff)
+
+
+@pytest.mark.parametrize("which,n,B,m", [("fib", 64, 4, 5), ("fib", 256, 2, 80), ("quad", 128, 8, 3),
+                                         ("logup", 64, 4, 0), ("logup", 1024, 4, 0)])
+def test_composition_matches_oracle(gpu, ora, which, n, B, m):
+    rng = random.Random(hash((which, n, B, m)) & 0xFFFFFF)
+    c = _case(gpu, ora, which, n, B, m, rng)
+    air = c["air"]
+    args = (B, n, G, c["betas"], c["groups"], c["ex"], c["bnd"], c["b_betas"], c["rap"], c["alpha"], c["toff"])
+    ref = ora.eval_composition(air.device, c["main_lde"], c["aux_lde"], *args)
+    hn = gpu.eval_composition(air.device, c["main_h"], c["aux_h"], *args, natural_order=True)
+    got = hn.to_array(np.uint64, (n * B, 3))
+    assert np.array_equal(canon(got), canon(ref))
+    hb = gpu.eval_composition(air.device, c["main_h"], c["aux_h"], *args)
+    got_b = hb.to_array(np.uint64, (n * B, 3))
+    assert np.array_equal(canon(got_b), canon(ref)[_bitrev_perm(n * B)])
+    if which != "logup" or n <= 64:
+        ev = gpu.eval_constraints(air.device, c["main_h"], c["aux_h"], B, c["rap"], c["alpha"], c["toff"])
+        ev_ref = ora.eval_constraints(air.device, c["main_lde"], c["aux_lde"], B, c["rap"], c["alpha"], c["toff"])
+        assert np.array_equal(canon(ev), canon(ev_ref))
+
+
+@pytest.mark.parametrize("which", ["fib", "quad", "logup"])
+def test_gpu_composition_low_degree(gpu, ora, which):
+    rng = random.Random(5)
+    n, B = 64, 4
+    c = _case(gpu, ora, which, n, B, 4, rng)
+    air = c["air"]
+    args = (B, n, G, c["betas"], c["groups"], c["ex"], c["bnd"], c["b_betas"], c["rap"], c["alpha"], c["toff"])
+    H = gpu.eval_composition(air.device, c["main_h"], c["aux_h"], *args, natural_order=True).to_array(
+        np.uint64, (n * B, 3))
+    deg = max(max(air.max_degree * (n - 1) + e - n for e in c["ex"]), n - 2)
+    assert not _high_coeffs(ora, H, deg).any()
+
+
+def test_composition_rejects_bad_programs(gpu, ora):
+    rng = random.Random(9)
+    c = _case(gpu, ora, "fib", 64, 4, 2, rng)
+    air = c["air"]
+    big = A.fibonacci_multi_column(5)  # reads columns the 2-column table lacks
+    with pytest.raises(gpu.LvmError):
+        gpu.eval_composition(big.device, c["main_h"], None, 4, 64, G, [rand3(rng)] * 5, [0] * 5, [2])
+    with pytest.raises(gpu.LvmError):  # zerofier group out of range
+        gpu.eval_composition(air.device, c["main_h"], None, 4, 64, G, c["betas"], [1, 1], [2])
+    logup = A.logup_air(24, A.four_bus_interactions())
+    with pytest.raises(gpu.LvmError):  # aux program without an aux table
+        gpu.eval_composition(logup.device, c["main_h"], None, 4, 64, G, [rand3(rng)] * logup.num_constraints,
+                             [0] * logup.num_constraints, [0])
