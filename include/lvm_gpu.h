@@ -298,6 +298,25 @@ int lvm_eval_composition(const lvm_program *program, const lvm_lde *main, const 
 int lvm_eval_constraints(const lvm_program *program, const lvm_lde *main, const lvm_lde *aux, size_t next_step,
                          uint64_t *dev_evals_out);
 
+/* Degree-2 decomposition and commitment of the composition polynomial
+ * (crypto/stark/src/prover.rs:1605-1680 decompose_and_extend_d2 +
+ * extend_half_to_lde; crypto/math-cuda/src/constraint_interp.rs:469
+ * `decompose_d2_into_slabs` and merkle.rs:494 `build_comp_poly_tree_from_slabs_dev`):
+ * dev_h holds the N composition values in bit-reversed row order (as
+ * lvm_eval_composition writes them); H0(x^2) = (H(x) + H(-x))/2 and
+ * H1(x^2) = (H(x) - H(-x))/(2x) are extended from the offset^2-coset of size
+ * N/2 to the offset-coset of size N and committed with row-pair leaves over
+ * [H0 | H1] (6 words per row). parts_out: the parts LDE (N rows x 6 words). */
+int lvm_composition_commit_d2(const uint64_t *dev_h, size_t N, uint64_t coset_offset, uint8_t root_out[32],
+                              lvm_lde **parts_out);
+
+/* ---------------- grinding ----------------
+ * crypto/math-cuda/src/grinding.rs:29 `generate_nonce_gpu`: the smallest
+ * nonce < max_nonce with BE u64 of Keccak(Keccak(PREFIX || seed || g) ||
+ * BE(nonce))[0..8] < 2^(64-g) (crypto/stark/src/grinding.rs:21-93); status
+ * LVM_ERR_STATE when there is none below max_nonce. */
+int lvm_generate_nonce(const uint8_t seed[32], uint8_t grinding_factor, uint64_t max_nonce, uint64_t *nonce_out);
+
 /* ---------------- Fiat-Shamir transcript (host) ----------------
  * crypto/crypto/src/fiat_shamir/default_transcript.rs:31-146
  * (`DefaultTranscript`): Keccak-256 duplex, sample() = finalize + reverse +

@@ -438,6 +438,24 @@ def eval_constraints(dev, main: GpuLde, aux, next_step: int, rap=None, alpha=Non
     return res
 
 
+def composition_commit_d2(dev_h: DeviceBuffer, lde_rows: int, coset_offset: int) -> GpuLde:
+    """Degree-2 split of the bit-reversed composition values, extension of both
+    halves to the LDE and their commitment: the parts LDE (lde_rows rows of [H0 | H1])."""
+    root = np.zeros(32, dtype=np.uint8)
+    h = C.c_void_p()
+    N.call("lvm_composition_commit_d2", C.c_void_p(dev_h.ptr), lde_rows, coset_offset, _p8(root), C.byref(h))
+    return GpuLde(h.value)
+
+
+def generate_nonce(seed: bytes, grinding_factor: int, max_nonce: int = 1 << 40) -> int:
+    """Smallest valid proof-of-work nonce (crypto/math-cuda/src/grinding.rs:29)."""
+    N.ensure_device()
+    sd = np.frombuffer(bytes(seed), dtype=np.uint8).copy()
+    out = np.zeros(1, dtype=np.uint64)
+    N.call("lvm_generate_nonce", _p8(sd), grinding_factor, max_nonce, _p64(out))
+    return int(out[0])
+
+
 def build_merkle_tree(leaves) -> np.ndarray:
     """(num_leaves, 32) digests -> (2L-1, 32) nodes, root first."""
     N.ensure_device()

@@ -382,6 +382,31 @@ int lvm_coset_weights(size_t n, uint64_t offset, uint64_t *dev_out) {
     return LVM_OK;
 }
 
+int lvm_composition_commit_d2(const uint64_t *dev_h, size_t N, uint64_t coset_offset, uint8_t root_out[32],
+                              lvm_lde **parts_out) {
+    if (!dev_h || !parts_out) return fail(LVM_ERR_ARG, "null argument");
+    if (!is_pow2(N) || N < 4) return fail(LVM_ERR_ARG, "LDE size must be a power of two >= 4");
+    if (log2u(N) > 32) return fail(LVM_ERR_ARG, "LDE domain exceeds the 2^32 subgroup");
+    if (canon(coset_offset) == 0) return fail(LVM_ERR_ARG, "coset offset must be non-zero");
+    Device *d = dev_or_fail();
+    if (!d) return LVM_ERR_DEVICE;
+    TRY(check_fault());
+    hipStream_t st = d->stream;
+    const size_t half = N / 2;
+    DevBuf halves, w;
+    TRY(halves.alloc(half * 6 * 8, st));
+    TRY(w.alloc(half * 8, st));
+    {
+        ProfScope ps("d2_split", st);
+        launch_d2_split(dev_h, log2u(N), coset_offset, d->tb, halves.as<uint64_t>(), st);
+    }
+    // weights g^-j / (N/2): interpolate on the g^2-coset, evaluate on the g-coset
+    // (crypto/stark/src/prover.rs:408-447 CompositionLdeTwiddles)
+    launch_coset_weights(half, inv(coset_offset), inv((uint64_t)half), w.as<uint64_t>(), st);
+    LVM_HIP(hipGetLastError());
+    return lde_core(d, halves.as<uint64_t>(), half, 6, 2, w.as<uint64_t>(), parts_out, root_out);
+}
+
 int lvm_dev_copy(void *dst, const void *src, size_t bytes) {
     if ((!dst || !src) && bytes) return fail(LVM_ERR_ARG, "null pointer");
     if ((bytes % 16) || ((uintptr_t)dst % 16) || ((uintptr_t)src % 16))

@@ -309,6 +309,40 @@ __global__ __launch_bounds__(COMP_BLOCK) void compose_kernel(CompArgs a) {
     }
 }
 
+// Degree-2 split of the composition polynomial (crypto/stark/src/prover.rs
+// :1605-1654 decompose_and_extend_d2): on the LDE coset, -x_r = x_{r+N/2},
+// so with rows stored bit-reversed H(x) and H(-x) are the physical pair
+// (2q, 2q+1) with r = bitrev_{N/2}(q). Writes natural-order half rows
+// [H0(x_r^2) | H1(x_r^2)] (6 words):
+//   H0 = (H(x) + H(-x)) / 2,  H1 = (H(x) - H(-x)) / (2x),
+// with 1/(2x_r) = (2g)^-1 * w_N^-r from the inverse twiddle tables.
+__global__ __launch_bounds__(256) void d2_split_kernel(const uint64_t *__restrict__ hb, uint32_t log_N,
+                                                        uint64_t inv2, uint64_t inv_2g, const uint64_t *inv_lo,
+                                                        const uint64_t *inv_hi, uint64_t *__restrict__ halves) {
+    const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= (1ull << (log_N - 1))) return;
+    const uint32_t r = bitrev((uint32_t)q, log_N - 1);
+    const fp3 a = ld3(hb + 6 * q), b = ld3(hb + 6 * q + 3);
+    const fp3 h0 = f3_mul_base(f3_add(a, b), inv2);
+    const uint64_t ix = mul(inv_2g, tw32_table(inv_lo, inv_hi, r << (32 - log_N)));
+    const fp3 h1 = f3_mul_base(f3_sub(a, b), ix);
+    uint64_t *o = halves + 6 * (uint64_t)r;
+    o[0] = h0.c0;
+    o[1] = h0.c1;
+    o[2] = h0.c2;
+    o[3] = h1.c0;
+    o[4] = h1.c1;
+    o[5] = h1.c2;
+}
+
+void launch_d2_split(const uint64_t *hb, unsigned log_N, uint64_t coset_offset, const Tables &tb, uint64_t *halves,
+                     hipStream_t st) {
+    const uint64_t half = 1ull << (log_N - 1);
+    const uint64_t inv2 = inv(2), inv_2g = inv(dbl(coset_offset));
+    hipLaunchKernelGGL(d2_split_kernel, dim3((unsigned)((half + 255) / 256)), dim3(256), 0, st, hb, log_N, inv2,
+                       inv_2g, tb.inv_lo, tb.inv_hi, halves);
+}
+
 namespace {
 
 template <class T>

@@ -104,6 +104,12 @@ void keccak_fp3_pair_leaves(const uint64_t *evals, uint64_t len, uint8_t *leaves
 
 size_t padded_leaves(size_t n);
 
+// ---- composition (compose.hip) ----
+// degree-2 split of N bit-reversed Fp3 composition values into N/2
+// natural-order rows [H0 | H1]
+void launch_d2_split(const uint64_t *hb, unsigned log_N, uint64_t coset_offset, const Tables &tb, uint64_t *halves,
+                     hipStream_t st);
+
 // ---- work lists over several tables (batched stages) ----
 // Work-list slot of block b: the t with prefix[t] <= b < prefix[t+1]
 // (prefix ascending, prefix[0] = 0), by binary search over `count` slots.

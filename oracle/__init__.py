@@ -592,3 +592,17 @@ def eval_composition(dev, main_lde, aux_lde, next_step, trace_rows, coset_offset
                                 _p64(out))
     assert rc == 0, rc
     return out
+
+
+def decompose_d2(h: np.ndarray, offset: int) -> np.ndarray:
+    """(N/2, 6) natural rows [H0 | H1] of a natural (N, 3) composition codeword."""
+    L = lib()
+    if not hasattr(L.ora_decompose_d2, "_declared"):
+        L.ora_decompose_d2.restype = C.c_int
+        L.ora_decompose_d2.argtypes = [u64p, C.c_size_t, C.c_uint64, u64p]
+        L.ora_decompose_d2._declared = True
+    h = np.ascontiguousarray(h, dtype=np.uint64)
+    N = h.shape[0]
+    out = np.zeros((N // 2, 6), dtype=np.uint64)
+    assert L.ora_decompose_d2(_p64(h), N, offset, _p64(out)) == 0
+    return out

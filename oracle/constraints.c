@@ -186,3 +186,24 @@ int ora_eval_composition(const ora_program *p, const uint64_t *main, size_t m, c
     free(ve);
     return 0;
 }
+
+/* Degree-2 split (crypto/stark/src/prover.rs:1605-1654): h natural N Fp3 ->
+ * halves natural (N/2) x [H0 | H1], H0 = (h[i] + h[i+N/2])/2,
+ * H1 = (h[i] - h[i+N/2]) / (2 offset w_N^i). */
+int ora_decompose_d2(const uint64_t *h, size_t N, uint64_t offset, uint64_t *halves) {
+    if (N < 2 || (N & (N - 1))) return -1;
+    unsigned log_N = 0;
+    while (((size_t)1 << log_N) < N) log_N++;
+    const uint64_t wN = gl_root_of_unity(log_N), inv2 = gl_inv(2);
+    const size_t half = N / 2;
+    uint64_t x = offset;
+    for (size_t i = 0; i < half; i++) {
+        const fp3 a = ld3(h + 3 * i), b = ld3(h + 3 * (i + half));
+        const fp3 h0 = fp3_mul_base(fp3_add(a, b), inv2);
+        const fp3 h1 = fp3_mul_base(fp3_sub(a, b), gl_inv(gl_double(x)));
+        memcpy(halves + 6 * i, h0.c, 24);
+        memcpy(halves + 6 * i + 3, h1.c, 24);
+        x = gl_mul(x, wN);
+    }
+    return 0;
+}

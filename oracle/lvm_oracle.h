@@ -184,6 +184,10 @@ int ora_eval_constraints(const ora_program *p, const uint64_t *main, size_t m, c
 int ora_eval_composition(const ora_program *p, const uint64_t *main, size_t m, const uint64_t *aux, size_t ma,
                          size_t N, size_t next_step, const ora_composition *c, uint64_t *h_out);
 
+/* degree-2 split of a natural-order composition codeword (prover.rs:1605-1654):
+ * halves_out (N/2) x 6 words [H0 | H1] natural order */
+int ora_decompose_d2(const uint64_t *h, size_t N, uint64_t offset, uint64_t *halves_out);
+
 /* ---- pipelines used by the CPU baseline ---- */
 /* LDE + row-pair commit of a base table: out_lde (nullable) row-major natural. */
 int ora_lde_commit(const uint64_t *trace, size_t n, size_t m, size_t blowup, uint64_t offset,

@@ -111,3 +111,19 @@ def test_composition_rejects_bad_programs(gpu, ora):
     with pytest.raises(gpu.LvmError):  # aux program without an aux table
         gpu.eval_composition(logup.device, c["main_h"], None, 4, 64, G, [rand3(rng)] * logup.num_constraints,
                              [0] * logup.num_constraints, [0])
+
+
+@pytest.mark.parametrize("which,n,B", [("quad", 64, 4), ("logup", 256, 4), ("fib", 128, 2)])
+def test_composition_d2_commit_matches_oracle(gpu, ora, which, n, B):
+    rng = random.Random(hash((which, n)) & 0xFFFF)
+    c = _case(gpu, ora, which, n, B, 3, rng)
+    air = c["air"]
+    args = (B, n, G, c["betas"], c["groups"], c["ex"], c["bnd"], c["b_betas"], c["rap"], c["alpha"], c["toff"])
+    N = n * B
+    hb = gpu.eval_composition(air.device, c["main_h"], c["aux_h"], *args)  # bit-reversed rows
+    parts = gpu.composition_commit_d2(hb, N, G)
+    ref_h = ora.eval_composition(air.device, c["main_lde"], c["aux_lde"], *args)
+    halves = ora.decompose_d2(ref_h, G)
+    ref_lde, ref_nodes = ora.lde_commit(halves, 2, pow(G, P - 2, P))
+    assert parts.root == bytes(ref_nodes[0])
+    assert np.array_equal(canon(parts.download()), canon(ref_lde))

@@ -215,3 +215,41 @@ def test_composition_closed_form(ora):
                 zb = pow((x - pow(wn, step, P)) % P, P - 2, P)
                 h = add3(h, mul3(scale3(bb, zb), sub3(t, val)))
             assert T(H[i]) == h, (which, i)
+
+
+def _coset_coeffs(ora, vals, offset):
+    """Coefficients (mod p, Python ints) of the polynomial taking vals[i] at offset*w_N^i."""
+    N = vals.shape[0]
+    raw = ora.fft_two_half(np.ascontiguousarray(vals, dtype=np.uint64), inverse=True)
+    ninv = pow(N, P - 2, P)
+    ginv = pow(offset, P - 2, P)
+    out = []
+    for j in range(N):
+        s = ninv * pow(ginv, j, P) % P
+        out.append(tuple(int(v) * s % P for v in raw[j]))
+    return out
+
+
+def _horner3(coeffs, x):
+    acc = (0, 0, 0)
+    for c in reversed(coeffs):
+        acc = add3(scale3(acc, x), c)
+    return acc
+
+
+def test_d2_decomposition_reconstructs(ora):
+    rng = random.Random(31)
+    N = 64
+    H = np.array([[rng.randrange(P) for _ in range(3)] for _ in range(N)], dtype=np.uint64)
+    halves = ora.decompose_d2(H, G)
+    ginv = pow(G, P - 2, P)
+    parts = ora.coset_lde_row_major(halves, 2, ginv)  # weights (N/2)^-1 g^-j
+    p0 = _coset_coeffs(ora, parts[:, 0:3], G)
+    p1 = _coset_coeffs(ora, parts[:, 3:6], G)
+    assert all(c == (0, 0, 0) for c in p0[N // 2:]) and all(c == (0, 0, 0) for c in p1[N // 2:])
+    hc = _coset_coeffs(ora, H, G)
+    for _ in range(4):
+        x = rng.randrange(P)
+        want = _horner3(hc, x)
+        got = add3(_horner3(p0[:N // 2], x * x % P), scale3(_horner3(p1[:N // 2], x * x % P), x))
+        assert got == want
