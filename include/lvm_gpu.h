@@ -124,6 +124,11 @@ int lvm_lde_open(const lvm_lde *h, const uint64_t *positions, size_t count, uint
 /* Coset weights w_i = n^-1 * offset^i, i < n, written to device memory
  * (crypto/stark/src/prover.rs:451-467 `LdeTwiddles::new` coset_weights). */
 int lvm_coset_weights(size_t n, uint64_t offset, uint64_t *dev_out);
+/* Synthetic input (no counterpart in the reference; workload generation for
+ * benchmarks and the proving service): row-major n x m table whose column c
+ * is the SplitMix64 stream seeded with 0x1A3BDA7A ^ (cfg << 48) ^ (table << 32) ^ c
+ * (the generator of crypto/stark/src/gpu_lde.rs:3409-3418), each draw reduced mod p. */
+int lvm_fill_splitmix_table(uint64_t *dev_out, size_t n, size_t m, uint64_t cfg, uint64_t table);
 
 /* ---------------- batched ragged commitment of many tables ----------------
  * The R1 commitment of all tables of a proof (crypto/stark/src/prover.rs

@@ -60,6 +60,7 @@ SIGNATURES = {
     "lvm_lde_tree_nodes": (C.c_int, [C.c_void_p, u8p, szp]),
     "lvm_lde_open": (C.c_int, [C.c_void_p, u64p, C.c_size_t, u8p, u64p]),
     "lvm_coset_weights": (C.c_int, [C.c_size_t, C.c_uint64, C.c_void_p]),
+    "lvm_fill_splitmix_table": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, C.c_uint64, C.c_uint64]),
     "lvm_batch_new": (C.c_int, [C.c_void_p, C.c_size_t, C.c_int, vpp]),
     "lvm_batch_bind": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
     "lvm_batch_run": (C.c_int, [C.c_void_p, u8p]),

@@ -456,6 +456,13 @@ def generate_nonce(seed: bytes, grinding_factor: int, max_nonce: int = 1 << 40) 
     return int(out[0])
 
 
+def splitmix_table_dev(n: int, m: int, cfg: int, table: int) -> DeviceBuffer:
+    """Device-generated synthetic table (workload.trace_table_modp on the GPU)."""
+    b = DeviceBuffer(max(8, n * m * 8))
+    N.call("lvm_fill_splitmix_table", C.c_void_p(b.ptr), n, m, cfg, table)
+    return b
+
+
 def build_merkle_tree(leaves) -> np.ndarray:
     """(num_leaves, 32) digests -> (2L-1, 32) nodes, root first."""
     N.ensure_device()

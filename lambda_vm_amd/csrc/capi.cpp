@@ -15,6 +15,7 @@ void launch_gl_op(int op, const uint64_t *a, const uint64_t *b, uint64_t *out, u
 void launch_gather_bitrev_rows(const uint64_t *src, uint64_t *dst, uint64_t rows, uint32_t log_rows, uint64_t w,
                                hipStream_t st);
 void launch_coset_weights(uint64_t n, uint64_t offset, uint64_t scale, uint64_t *out, hipStream_t st);
+void launch_fill_splitmix(uint64_t *out, uint64_t n, uint64_t m, uint64_t cfg, uint64_t table, hipStream_t st);
 void launch_copy16(void *dst, const void *src, uint64_t n16, hipStream_t st);
 void launch_open(const uint64_t *nodes, uint64_t L, const uint64_t *lde, uint64_t w, const uint64_t *pos,
                  uint64_t count, uint32_t depth, uint64_t *paths, uint64_t *rows, hipStream_t st);
@@ -405,6 +406,17 @@ int lvm_composition_commit_d2(const uint64_t *dev_h, size_t N, uint64_t coset_of
     launch_coset_weights(half, inv(coset_offset), inv((uint64_t)half), w.as<uint64_t>(), st);
     LVM_HIP(hipGetLastError());
     return lde_core(d, halves.as<uint64_t>(), half, 6, 2, w.as<uint64_t>(), parts_out, root_out);
+}
+
+int lvm_fill_splitmix_table(uint64_t *dev_out, size_t n, size_t m, uint64_t cfg, uint64_t table) {
+    if (!dev_out && n * m) return fail(LVM_ERR_ARG, "null output");
+    if (m && n > ((size_t)1 << 40) / m) return fail(LVM_ERR_ARG, "table too large");
+    Device *d = dev_or_fail();
+    if (!d) return LVM_ERR_DEVICE;
+    launch_fill_splitmix(dev_out, n, m, cfg, table, d->stream);
+    LVM_HIP(hipGetLastError());
+    LVM_HIP(hipStreamSynchronize(d->stream));
+    return LVM_OK;
 }
 
 int lvm_dev_copy(void *dst, const void *src, size_t bytes) {

@@ -123,4 +123,26 @@ void launch_coset_weights(uint64_t n, uint64_t offset, uint64_t scale, uint64_t 
                        out);
 }
 
+// Synthetic table: element (r, c) = SplitMix64 draw r of the stream seeded
+// with 0x1A3BDA7A ^ (cfg << 48) ^ (table << 32) ^ c, reduced mod p (draw k is
+// mix(seed + (k+1) * 0x9E3779B97F4A7C15)). Row-major n x m.
+__global__ void fill_splitmix_kernel(uint64_t *out, uint64_t n, uint64_t m, uint64_t cfg, uint64_t table) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * m) return;
+    const uint64_t r = i / m, c = i % m;
+    const uint64_t seed = 0x1A3BDA7Aull ^ (cfg << 48) ^ (table << 32) ^ c;
+    uint64_t z = seed + (r + 1) * 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    out[i] = canon(z);
+}
+
+void launch_fill_splitmix(uint64_t *out, uint64_t n, uint64_t m, uint64_t cfg, uint64_t table, hipStream_t st) {
+    const uint64_t total = n * m;
+    if (!total) return;
+    hipLaunchKernelGGL(fill_splitmix_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, out, n, m, cfg,
+                       table);
+}
+
 }  // namespace lvm

@@ -56,15 +56,11 @@ def sample_z_ood(t: B.Transcript, n: int, N: int, offset: int):
         return z
 
 
-def prove_single_table(d_trace: B.DeviceBuffer, n: int, m: int, blowup: int, offset: int,
-                       d_weights: B.DeviceBuffer, fri_final_log_degree: int = 6, num_queries: int = 30,
-                       label: bytes = b"lambda-vm-single-table", keep: bool = False):
+def prove_committed_table(h: B.GpuLde, t: B.Transcript, n: int, m: int, blowup: int, offset: int,
+                          fri_final_log_degree: int = 6, num_queries: int = 30, keep: bool = False):
+    """Rounds R3/R4 of a committed table whose root the transcript has absorbed."""
     N = n * blowup
     blowup_log = blowup.bit_length() - 1
-    t = B.Transcript(label)
-    # R1
-    h = B.coset_lde_dev(d_trace, n, m, blowup, d_weights)
-    t.append_bytes(h.root)
     # R3
     z = sample_z_ood(t, n, N, offset)
     wn = F.root_of_unity(n.bit_length() - 1)
@@ -86,7 +82,21 @@ def prove_single_table(d_trace: B.DeviceBuffer, n: int, m: int, blowup: int, off
     tpaths, trows = h.open(iotas)
     proof = TableProof(h.root, z, ood, roots, coeffs, iotas, sym, paths, tpaths, trows, t.state())
     if keep:
-        return proof, h, fri
+        return proof, fri
     fri.close()
-    h.close()
     return proof
+
+
+def prove_single_table(d_trace: B.DeviceBuffer, n: int, m: int, blowup: int, offset: int,
+                       d_weights: B.DeviceBuffer, fri_final_log_degree: int = 6, num_queries: int = 30,
+                       label: bytes = b"lambda-vm-single-table", keep: bool = False):
+    t = B.Transcript(label)
+    # R1
+    h = B.coset_lde_dev(d_trace, n, m, blowup, d_weights)
+    t.append_bytes(h.root)
+    res = prove_committed_table(h, t, n, m, blowup, offset, fri_final_log_degree, num_queries, keep)
+    if keep:
+        proof, fri = res
+        return proof, h, fri
+    h.close()
+    return res

@@ -1,0 +1,257 @@
+"""Proving service: continuous batching of multi-table proving jobs under an
+HBM budget.
+
+The reference bounds concurrent table proving by device memory
+(crypto/stark/src/prover.rs:682-693 `estimate_table_vram_bytes`, :703-789
+`VramGate` / `run_admitted`, heaviest tables first) and commits all tables of
+a proof before forking per-table transcripts (prover.rs:3426-3538). Here the
+same ideas serve a stream of jobs:
+  - a job (4-20 tables) arrives, waits in FIFO order and is admitted once its
+    estimated HBM footprint fits the budget beside the jobs in flight;
+  - every scheduler tick commits the tables of all jobs admitted since the
+    previous tick in one batched launch sequence (lvm_batch_*), so jobs that
+    arrive while others are on the GPU share launches with each other;
+  - committed jobs then advance one table per tick through the remaining
+    rounds (OOD, DEEP, FRI, queries), interleaved across jobs, and free each
+    table's LDE, tree and FRI layers as soon as its rounds finish;
+  - per job: latency from arrival to its final digest; per run: p50/p99
+    latency, jobs/s and LDE cells/s.
+A job's result depends only on its own inputs and transcript: the digest of
+a job is the same alone and in any mix.
+"""
+from __future__ import annotations
+
+import hashlib
+import math
+import random
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+
+P = 0xFFFFFFFF00000001
+
+
+@dataclass
+class JobSpec:
+    job_id: int
+    tables: list  # (n, m)
+    blowup: int = 4
+    cfg: int = 5  # synthetic data stream family
+
+    @property
+    def lde_cells(self) -> int:
+        return sum(n * m * self.blowup for n, m in self.tables)
+
+
+def table_hbm_bytes(n: int, m: int, blowup: int) -> int:
+    """Device bytes of one table while it is proved: trace, coefficients,
+    LDE, Merkle tree, weights, DEEP codeword and FRI layers with their trees
+    (the shape of prover.rs:682-693 for this pipeline)."""
+    N = n * blowup
+    return 16 * n * m + 8 * n + 8 * N * m + 32 * N + 2 * 24 * N + 2 * 32 * N
+
+
+def job_hbm_bytes(spec: JobSpec) -> int:
+    return sum(table_hbm_bytes(n, m, spec.blowup) for n, m in spec.tables)
+
+
+def synth_jobs(rate: float, count: int, seed: int = 1, min_log_cells: int = 20, max_log_cells: int = 28,
+               blowup: int = 4) -> list:
+    """Poisson arrivals (exponential gaps, mean 1/rate) of jobs with 4-20
+    tables and total LDE cells log-uniform in [2^min, 2^max]."""
+    rng = random.Random(seed)
+    t = 0.0
+    out = []
+    for j in range(count):
+        t += rng.expovariate(rate)
+        k = rng.randint(4, 20)
+        total = 2.0 ** rng.uniform(min_log_cells, max_log_cells)
+        w = [rng.random() + 0.05 for _ in range(k)]
+        s = sum(w)
+        tables = []
+        for x in w:
+            m = rng.randint(8, 64)
+            cells = total * x / s
+            log_n = int(round(math.log2(max(cells / (blowup * m), 1.0))))
+            tables.append((1 << min(max(log_n, 8), 22), m))
+        out.append((t, JobSpec(j, tables, blowup)))
+    return out
+
+
+@dataclass
+class JobState:
+    spec: JobSpec
+    arrival: float
+    admitted: float = 0.0
+    finished: float = 0.0
+    hbm: int = 0
+    handles: list = field(default_factory=list)
+    transcripts: list = field(default_factory=list)
+    next_table: int = 0
+    digests: list = field(default_factory=list)
+    roots: list = field(default_factory=list)
+
+    def digest(self) -> str:
+        h = hashlib.sha256()
+        for d in self.digests:
+            h.update(d)
+        return h.hexdigest()
+
+
+def proof_digest(proof) -> bytes:
+    """Canonical digest of a table proof: every field the verifier reads."""
+    h = hashlib.sha256()
+    h.update(proof.root)
+    for arr in (np.asarray(proof.z, dtype=np.uint64), proof.ood, proof.fri_coeffs, proof.iotas, proof.fri_sym,
+                proof.trace_rows):
+        a = np.asarray(arr, dtype=np.uint64)
+        h.update(np.where(a >= np.uint64(P), a - np.uint64(P), a).tobytes())
+    h.update(np.asarray(proof.fri_roots).tobytes())
+    for p in proof.fri_paths:
+        h.update(np.asarray(p).tobytes())
+    h.update(np.asarray(proof.trace_paths).tobytes())
+    h.update(proof.transcript_state)
+    return h.digest()
+
+
+class GpuExecutor:
+    """Runs the service's work on the GPU through the C ABI."""
+
+    def __init__(self, offset: int = 3, fri_k: int = 6, num_queries: int = 30):
+        from . import _native, backend, pipeline
+        _native.ensure_device(0)
+        self.B, self.pipeline = backend, pipeline
+        self.offset, self.fri_k, self.num_queries = offset, fri_k, num_queries
+
+    def commit(self, jobs: list) -> None:
+        """One batched commitment over every table of the given jobs."""
+        B = self.B
+        tables, owners, inputs = [], [], []
+        for js in jobs:
+            for ti, (n, m) in enumerate(js.spec.tables):
+                tr = B.splitmix_table_dev(n, m, js.spec.cfg, js.spec.job_id * 64 + ti)
+                w = B.coset_weights_dev(n, self.offset)
+                inputs.append((tr, w))
+                tables.append((tr, w, n, m, js.spec.blowup))
+                owners.append((js, ti))
+        plan = B.BatchPlan(tables, use_graph=False)
+        roots = plan.run()
+        for i, (js, ti) in enumerate(owners):
+            js.handles.append(plan.take(i))
+            js.roots.append(roots[i])
+        plan.close()
+        for tr, w in inputs:
+            tr.free()
+            w.free()
+        # shared transcript over all roots, then one fork per table (prover.rs:3457-3538)
+        for js in jobs:
+            t = B.Transcript(b"lambda-vm-job-%d" % js.spec.job_id)
+            for r in js.roots:
+                t.append_bytes(r)
+            for ti in range(len(js.spec.tables)):
+                f = t.clone()
+                if len(js.spec.tables) > 1:
+                    f.append_bytes(int(ti).to_bytes(8, "little"))
+                js.transcripts.append(f)
+
+    def prove_table(self, js: JobState, ti: int) -> bytes:
+        n, m = js.spec.tables[ti]
+        h = js.handles[ti]
+        proof = self.pipeline.prove_committed_table(h, js.transcripts[ti], n, m, js.spec.blowup, self.offset,
+                                                    self.fri_k, self.num_queries)
+        h.close()
+        js.handles[ti] = None
+        return proof_digest(proof)
+
+
+class ProvingService:
+    def __init__(self, hbm_budget: int, executor=None, clock=time.perf_counter):
+        self.budget = hbm_budget
+        self.ex = executor if executor is not None else GpuExecutor()
+        self.clock = clock
+        self.waiting: deque = deque()
+        self.running: list = []
+        self.done: list = []
+        self.in_use = 0
+        self.peak_in_use = 0
+        self.ticks = 0
+        self.batch_sizes: list = []
+
+    def submit(self, spec: JobSpec, arrival: float) -> None:
+        hbm = job_hbm_bytes(spec)
+        if hbm > self.budget:
+            raise ValueError(f"job {spec.job_id} needs {hbm} bytes, more than the whole budget")
+        self.waiting.append(JobState(spec, arrival, hbm=hbm))
+
+    def _admit(self, now: float) -> list:
+        admitted = []
+        while self.waiting and self.in_use + self.waiting[0].hbm <= self.budget:
+            js = self.waiting.popleft()
+            js.admitted = now
+            self.in_use += js.hbm
+            self.peak_in_use = max(self.peak_in_use, self.in_use)
+            admitted.append(js)
+        return admitted
+
+    def tick(self) -> bool:
+        """Admit what fits, commit it in one batch, advance every committed job
+        by one table; returns whether any work was done."""
+        self.ticks += 1
+        admitted = self._admit(self.clock())
+        if admitted:
+            self.ex.commit(admitted)
+            self.batch_sizes.append(len(admitted))
+            self.running.extend(admitted)
+        worked = bool(admitted)
+        for js in list(self.running):
+            if js in admitted:
+                continue
+            js.digests.append(self.ex.prove_table(js, js.next_table))
+            js.next_table += 1
+            worked = True
+            if js.next_table == len(js.spec.tables):
+                js.finished = self.clock()
+                self.running.remove(js)
+                self.in_use -= js.hbm
+                self.done.append(js)
+        return worked
+
+    def run(self, arrivals: list, time_scale: float = 1.0) -> dict:
+        """Serve `arrivals` [(t, spec)] in real time (t in seconds after start,
+        scaled by time_scale); returns the run's statistics."""
+        t0 = self.clock()
+        pending = deque(sorted(arrivals, key=lambda a: a[0]))
+        total = len(pending)
+        while len(self.done) < total:
+            now = self.clock() - t0
+            while pending and pending[0][0] * time_scale <= now:
+                t, spec = pending.popleft()
+                self.submit(spec, t0 + t * time_scale)
+            if not self.tick():
+                if pending:
+                    wait = pending[0][0] * time_scale - (self.clock() - t0)
+                    if wait > 0:
+                        time.sleep(min(wait, 0.05))
+        makespan = max(js.finished for js in self.done) - t0
+        return self.stats(makespan)
+
+    def stats(self, makespan: float) -> dict:
+        lat = np.array([js.finished - js.arrival for js in self.done])
+        cells = sum(js.spec.lde_cells for js in self.done)
+        return {
+            "jobs": len(self.done),
+            "makespan_s": makespan,
+            "jobs_per_s": len(self.done) / makespan,
+            "gcells_per_s": cells / makespan / 1e9,
+            "p50_ms": float(np.percentile(lat, 50) * 1e3),
+            "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "mean_ms": float(lat.mean() * 1e3),
+            "max_ms": float(lat.max() * 1e3),
+            "peak_hbm_bytes": self.peak_in_use,
+            "budget_bytes": self.budget,
+            "ticks": self.ticks,
+            "mean_jobs_per_commit_batch": float(np.mean(self.batch_sizes)) if self.batch_sizes else 0.0,
+            "lde_cells": cells,
+        }

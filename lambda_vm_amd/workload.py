@@ -51,6 +51,16 @@ def trace_table(cfg: int, table: int, n: int, m: int) -> np.ndarray:
     return t
 
 
+def trace_table_modp(cfg: int, table: int, n: int, m: int) -> np.ndarray:
+    """Row-major n x m table, column c the first n draws of its stream reduced
+    mod p (no rejection): the device generator's definition."""
+    t = np.empty((n, m), dtype=np.uint64)
+    for c in range(m):
+        d = splitmix64(column_seed(cfg, table, c), n)
+        t[:, c] = np.where(d >= np.uint64(P), d - np.uint64(P), d)
+    return t
+
+
 def fp3_values(seed: int, count: int) -> np.ndarray:
     return field_stream(seed, 3 * count).reshape(count, 3)
 
