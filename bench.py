@@ -116,7 +116,7 @@ def main() -> int:
     ap.add_argument("--commit-only", action="store_true", help="time only the LDE + Merkle commitment")
     ap.add_argument("--fri-k", type=int, default=6, help="log2 terminal degree (terminal 2^(log B + k))")
     ap.add_argument("--queries", type=int, default=30)
-    ap.add_argument("--workload", choices=["ragged", "single", "serve"], default="ragged")
+    ap.add_argument("--workload", choices=["ragged", "single", "serve", "constraints"], default="ragged")
     ap.add_argument("--tables", type=int, default=14, help="ragged: number of tables")
     ap.add_argument("--rate", type=float, default=20.0, help="serve: job arrivals per second (Poisson)")
     ap.add_argument("--jobs", type=int, default=120, help="serve: number of jobs")
@@ -127,7 +127,8 @@ def main() -> int:
         print("bench.py measures one GPU: the hot path is sharded by table on a single device "
               "and no multi-GPU figure is produced (run with --gpus 1)", file=sys.stderr)
         return 2
-    runner = {"ragged": run_ragged, "single": run_single, "serve": run_serve}[args.workload]
+    runner = {"ragged": run_ragged, "single": run_single, "serve": run_serve,
+              "constraints": run_constraints}[args.workload]
     out = runner(args)
     print(json.dumps(out), flush=True)
     return 0
@@ -342,6 +343,128 @@ def run_serve(args) -> dict:
         },
     }
     return out
+
+
+def run_constraints(args) -> dict:
+    """Constraint + LogUp stage: 2^21 x 80 FibonacciMultiColumn, 2^20 x 48
+    Quadratic, 2^18 x 24 four-bus LogUp table, blowup 4, seeded uniform traces
+    resident in HBM. A step: batched LDE+commit of the three main traces,
+    LogUp aux build + aux LDE+commit, composition polynomial of each table
+    (seeded combination coefficients) and its degree-2 split + commit."""
+    import random
+
+    import numpy as np
+
+    from lambda_vm_amd import _native, air as A, backend, workload
+
+    _native.ensure_device(0)
+    offset, B = workload.PROOF_COSET_OFFSET, 4
+    specs = [("fib", 1 << 21, 80), ("quad", 1 << 20, 48), ("logup", 1 << 18, 24)]
+    rng = random.Random(2)
+
+    def r3():
+        return (rng.randrange(workload.P), rng.randrange(workload.P), rng.randrange(workload.P))
+
+    airs, traces, weights = [], [], []
+    for t, (kind, n, m) in enumerate(specs):
+        a = (A.fibonacci_multi_column(m) if kind == "fib" else A.quadratic_multi(m) if kind == "quad"
+             else A.logup_air(m, A.four_bus_interactions()))
+        airs.append(a)
+        traces.append(backend.splitmix_table_dev(n, m, 2, t))
+        weights.append(backend.coset_weights_dev(n, offset))
+    logup_desc = A.descriptor(airs[2])
+    z, al = r3(), r3()
+    ap = []
+    cur = (1, 0, 0)
+    from lambda_vm_amd import field as Fh
+    for _ in range(logup_desc.alpha_powers_len):
+        ap.append(cur)
+        cur = tuple(Fh.f3_mul(list(cur), list(al)))
+    ap = np.array(ap, dtype=np.uint64)
+    betas = [[r3() for _ in range(a.num_constraints)] for a in airs]
+    cells = sum(n * m * B for _, n, m in specs)
+
+    def step():
+        plan = backend.BatchPlan([(traces[i], weights[i], n, m, B) for i, (_, n, m) in enumerate(specs)],
+                                 use_graph=False)
+        roots = plan.run()
+        mains = [plan.take(i) for i in range(3)]
+        plan.close()
+        n_l = specs[2][1]
+        aux, L, _ = backend.logup_aux_dev(traces[2], n_l, specs[2][2], logup_desc, ap, z)
+        aux_h = backend.coset_lde_dev(aux, n_l, 3 * logup_desc.num_out_cols, B, weights[2])
+        aux.free()
+        ninv = pow(n_l, workload.P - 2, workload.P)
+        toff = np.array([x * ninv % workload.P for x in L], dtype=np.uint64)
+        out = list(roots) + [aux_h.root]
+        for i, a in enumerate(airs):
+            groups, ex = a.groups()
+            bnd = [(b.col, b.is_aux, b.step, b.value) for b in a.boundary]
+            extra = (dict(rap=[z, al], alpha=ap, table_offset=toff) if i == 2 else {})
+            H = backend.eval_composition(a.device, mains[i], aux_h if i == 2 else None, B, specs[i][1], offset,
+                                         betas[i], groups, ex, bnd, [r3() for _ in bnd], **extra)
+            parts = backend.composition_commit_d2(H, specs[i][1] * B, offset)
+            out.append(parts.root)
+            H.free()
+            parts.close()
+        for h in mains:
+            h.close()
+        aux_h.close()
+        return out
+
+    for _ in range(max(1, args.warmup)):
+        step()
+    backend.sync()
+    backend.prof_reset()
+    backend.prof_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    backend.sync()
+    dt = (time.perf_counter() - t0) / args.steps
+    backend.prof_enable(False)
+    names = ["logup_terms", "logup_accumulate", "composition", "d2_split", "ntt_lde", "keccak_leaves",
+             "merkle_inner"]
+    stage = {}
+    for k in names:
+        cnt, tot = backend.prof_query(k)
+        if cnt:
+            stage[k] = round(tot / args.steps, 4)
+    cnt, tot = backend.prof_query("composition")
+    # compulsory bytes of the three composition launches: each LDE row read
+    # once (main base words + aux Fp3 words) and one Fp3 written per row
+    comp_bytes = sum(n * B * (8 * m + 24) for _, n, m in specs) + specs[2][1] * B * 24 * logup_desc.num_out_cols
+    achieved = comp_bytes / (tot / 1e3 / args.steps) / 1e9 if cnt else None
+    return {
+        "metric": "batched LDE+Keccak-Merkle Gcells/s vs per-table launch; p50/p99 proof latency",
+        "value": round(cells / dt / 1e9, 4),
+        "unit": "Gcells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64 (Goldilocks field)",
+        "data": "synthetic: device SplitMix64 traces (draws mod p), seeded challenges",
+        "config": {
+            "workload": ("constraint + LogUp stage: 2^21x80 FibonacciMultiColumn, 2^20x48 Quadratic, 2^18x24 "
+                         "4-bus LogUp, blowup 4: batched main LDE+commit, LogUp aux build + aux LDE+commit, "
+                         "composition of each table, degree-2 split + parts commit"),
+            "lde_cells": cells,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "composition",
+            "achieved": round(achieved, 2) if achieved else None,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
+            "traffic": None,
+            "stage_ms_per_step": stage,
+        },
+    }
 
 
 def run_single(args) -> dict:
