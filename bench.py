@@ -423,14 +423,16 @@ def run_constraints(args) -> dict:
     backend.sync()
     dt = (time.perf_counter() - t0) / args.steps
     backend.prof_enable(False)
-    names = ["logup_terms", "logup_accumulate", "composition", "d2_split", "ntt_lde", "keccak_leaves",
-             "merkle_inner"]
+    names = ["logup_terms", "logup_accumulate", "composition_transpose", "composition_compiled", "composition",
+             "d2_split", "ntt_lde", "keccak_leaves", "merkle_inner"]
     stage = {}
     for k in names:
         cnt, tot = backend.prof_query(k)
         if cnt:
             stage[k] = round(tot / args.steps, 4)
-    cnt, tot = backend.prof_query("composition")
+    # the compiled constraint kernel, or the interpreter where it ran instead
+    comp_kernel = "composition_compiled" if "composition_compiled" in stage else "composition"
+    cnt, tot = backend.prof_query(comp_kernel)
     # compulsory bytes of the three composition launches: each LDE row read
     # once (main base words + aux Fp3 words) and one Fp3 written per row
     comp_bytes = sum(n * B * (8 * m + 24) for _, n, m in specs) + specs[2][1] * B * 24 * logup_desc.num_out_cols
@@ -456,7 +458,7 @@ def run_constraints(args) -> dict:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "composition",
+            "kernel": comp_kernel,
             "achieved": round(achieved, 2) if achieved else None,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

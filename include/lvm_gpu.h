@@ -40,6 +40,7 @@ extern "C" {
 #define LVM_ERR_NOMEM 3
 #define LVM_ERR_INJECTED 4
 #define LVM_ERR_STATE 5
+#define LVM_ERR_UNSUPPORTED 6
 
 /* Select and initialise a device (tables, stream). Replaces the backend
  * singleton `backend()` of crypto/math-cuda/src/device.rs:592-619. */
@@ -268,6 +269,10 @@ typedef struct lvm_program {
     const uint64_t *alpha_powers;
     size_t num_alpha;
     const uint64_t *table_offset; /* 3 u64, nullable when unused */
+    /* nullable: index of the node computing each constraint. When given, root
+     * slots need not be pinned (each value is consumed when its node runs);
+     * when null the roots' slots must be pinned, as the reference lowering does. */
+    const uint32_t *root_nodes;
 } lvm_program;
 /* Composition inputs (crypto/math-cuda/src/constraint_interp.rs:157-177
  * `CompositionAccum`), with the zerofiers described by their parameters
@@ -302,6 +307,19 @@ int lvm_eval_composition(const lvm_program *program, const lvm_lde *main, const 
  * constraints carry their value in component 0. */
 int lvm_eval_constraints(const lvm_program *program, const lvm_lde *main, const lvm_lde *aux, size_t next_step,
                          uint64_t *dev_evals_out);
+
+/* Constraint programs run as kernels compiled at run time (hipRTC): the
+ * program becomes straight-line code, compiled once per distinct program and
+ * cached for the life of the process; programs the generator does not take
+ * run on the interpreter kernel, with the same results. No counterpart in the
+ * reference (its CUDA path interprets the program, constraint_interp.rs).
+ * lvm_prepare_constraints compiles ahead of the first evaluation (comp null:
+ * the per-constraint form of lvm_eval_constraints); it needs no device and
+ * returns LVM_ERR_UNSUPPORTED for a program that is not compiled.
+ * lvm_set_constraint_jit(0) makes every evaluation use the interpreter
+ * (default on; the environment variable LVM_CONSTRAINT_JIT=0 does the same). */
+int lvm_prepare_constraints(const lvm_program *program, const lvm_composition *comp);
+int lvm_set_constraint_jit(int on);
 
 /* Degree-2 decomposition and commitment of the composition polynomial
  * (crypto/stark/src/prover.rs:1605-1680 decompose_and_extend_d2 +

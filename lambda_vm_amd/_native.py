@@ -84,6 +84,8 @@ SIGNATURES = {
     "lvm_eval_composition": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_int,
                                        C.c_void_p]),
     "lvm_eval_constraints": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "lvm_prepare_constraints": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "lvm_set_constraint_jit": (C.c_int, [C.c_int]),
     "lvm_composition_commit_d2": (C.c_int, [C.c_void_p, C.c_size_t, C.c_uint64, u8p, vpp]),
     "lvm_generate_nonce": (C.c_int, [u8p, C.c_uint8, C.c_uint64, u64p]),
     "lvm_transcript_new": (C.c_int, [u8p, C.c_size_t, vpp]),
@@ -133,7 +135,8 @@ class Program(C.Structure):
                 ("num_ext_slots", C.c_size_t), ("base_consts", C.c_void_p), ("num_base_consts", C.c_size_t),
                 ("ext_consts", C.c_void_p), ("num_ext_consts", C.c_size_t), ("roots", C.c_void_p),
                 ("num_roots", C.c_size_t), ("rap_challenges", C.c_void_p), ("num_rap", C.c_size_t),
-                ("alpha_powers", C.c_void_p), ("num_alpha", C.c_size_t), ("table_offset", C.c_void_p)]
+                ("alpha_powers", C.c_void_p), ("num_alpha", C.c_size_t), ("table_offset", C.c_void_p),
+                ("root_nodes", C.c_void_p)]
 
 
 class Composition(C.Structure):

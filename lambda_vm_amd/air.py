@@ -65,7 +65,8 @@ def fibonacci_multi_column(m: int) -> AirProgram:
         a0, a1, a2 = b.main(0, c), b.main(1, c), b.main(2, c)
         b.emit(c, a2 - a1 - a0)
     prog = b.finish(m)
-    return AirProgram(f"fibonacci_multi_column_{m}", m, 0, prog, ir.lower(prog), [2] * m, max_degree=1)
+    return AirProgram(f"fibonacci_multi_column_{m}", m, 0, prog, ir.lower(prog, pin_roots=False), [2] * m,
+                      max_degree=1)
 
 
 def quadratic_multi(m: int) -> AirProgram:
@@ -74,7 +75,7 @@ def quadratic_multi(m: int) -> AirProgram:
         x, x_next = b.main(0, c), b.main(1, c)
         b.emit(c, x_next - x * x)
     prog = b.finish(m)
-    return AirProgram(f"quadratic_{m}", m, 0, prog, ir.lower(prog), [1] * m, max_degree=2)
+    return AirProgram(f"quadratic_{m}", m, 0, prog, ir.lower(prog, pin_roots=False), [1] * m, max_degree=2)
 
 
 # ---- LogUp constraint emission (lookup.rs:2160-2325) ----
@@ -193,8 +194,8 @@ def logup_air(num_main: int, interactions) -> AirProgram:
     pairs, absorbed = split_interactions(len(interactions))
     bnd = [Boundary(pairs, True, 0, (0, 0, 0))]
     deg = max(3 if pairs else 0, 1 + absorbed)
-    return AirProgram(f"logup_{len(interactions)}", num_main, pairs + 1, prog, ir.lower(prog), [0] * n_next,
-                      boundary=bnd, interactions=list(interactions), max_degree=deg)
+    return AirProgram(f"logup_{len(interactions)}", num_main, pairs + 1, prog, ir.lower(prog, pin_roots=False),
+                      [0] * n_next, boundary=bnd, interactions=list(interactions), max_degree=deg)
 
 
 def four_bus_interactions():

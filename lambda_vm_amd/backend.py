@@ -391,6 +391,8 @@ def _program_struct(dev, rap, alpha, table_offset, keep):
     s.alpha_powers = _arr_ptr(a, np.uint64, keep)
     s.num_alpha = a.size // 3
     s.table_offset = _arr_ptr(table_offset, np.uint64, keep) if table_offset is not None else None
+    s.root_nodes = (_arr_ptr(dev.root_nodes, np.uint32, keep) if getattr(dev, "root_nodes", None) is not None
+                    else None)
     return s
 
 
@@ -436,6 +438,30 @@ def eval_constraints(dev, main: GpuLde, aux, next_step: int, rap=None, alpha=Non
     res = out.to_array(np.uint64, (len(dev.roots), main.lde_rows, 3))
     out.free()
     return res
+
+
+def prepare_constraints(dev, root_group=None, num_rap: int = 0, num_alpha: int = 0,
+                        has_table_offset: bool = False) -> None:
+    """Compile the program's kernel ahead of its first evaluation (no device
+    needed): the composition form for the given zerofier group of each
+    constraint, the per-constraint form when root_group is None. Raises
+    LvmError (LVM_ERR_UNSUPPORTED) when the program is not compiled; it then
+    runs on the interpreter kernel."""
+    keep = []
+    zero3 = np.zeros(3, dtype=np.uint64)
+    s = _program_struct(dev, np.zeros((num_rap, 3), dtype=np.uint64), np.zeros((num_alpha, 3), dtype=np.uint64),
+                        zero3 if has_table_offset else None, keep)
+    c = None
+    if root_group is not None:
+        c = N.Composition()
+        c.root_group = _arr_ptr(root_group, np.uint32, keep)
+        c.num_groups = int(max(root_group)) + 1 if len(root_group) else 0
+    N.call("lvm_prepare_constraints", C.byref(s), C.byref(c) if c is not None else None)
+
+
+def set_constraint_jit(on: bool) -> None:
+    """Run constraint programs as compiled kernels (default) or on the interpreter."""
+    N.call("lvm_set_constraint_jit", int(bool(on)))
 
 
 def composition_commit_d2(dev_h: DeviceBuffer, lde_rows: int, coset_offset: int) -> GpuLde:

@@ -15,7 +15,7 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "liblvm_gpu.so")
 
-SOURCES = ["ntt.hip", "merkle.hip", "misc.hip", "fri.hip", "ood.hip", "logup.hip", "compose.hip", "grinding.hip", "context.cpp", "transcript.cpp", "capi.cpp", "batch.cpp"]
+SOURCES = ["ntt.hip", "merkle.hip", "misc.hip", "fri.hip", "ood.hip", "logup.hip", "compose.hip", "grinding.hip", "context.cpp", "compose_jit.cpp", "transcript.cpp", "capi.cpp", "batch.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LVM_OFFLOAD_ARCH", "gfx950")
 
@@ -33,13 +33,32 @@ FLAGS = [
 ]
 
 
+def _embed_field_source(objdir: str) -> str:
+    """C++ file holding the text of field.hpp and compose_common.hpp for the
+    run-time compiled constraint kernels (include lines dropped: the run-time
+    compiler provides the HIP device environment)."""
+    body = ""
+    for name in ("field.hpp", "compose_common.hpp"):
+        text = open(os.path.join(CSRC, name)).read()
+        body += "\n".join(line for line in text.splitlines()
+                          if not line.startswith("#include") and not line.startswith("#pragma once")) + "\n"
+    out = os.path.join(objdir, "jit_field_src.cpp")
+    src = ('// generated from csrc/field.hpp and csrc/compose_common.hpp by build.py\n'
+           'extern const char *const lvm_jit_field_src = R"LVMFIELD(' + body + ')LVMFIELD";\n')
+    if not os.path.exists(out) or open(out).read() != src:
+        with open(out, "w") as f:
+            f.write(src)
+    return out
+
+
 def build(verbose: bool = False) -> str:
     objdir = os.path.join(PKG, "_build")
     os.makedirs(objdir, exist_ok=True)
     objs = []
     procs = []
-    for s in SOURCES:
-        src = os.path.join(CSRC, s)
+    sources = [os.path.join(CSRC, s) for s in SOURCES] + [_embed_field_source(objdir)]
+    for src in sources:
+        s = os.path.basename(src)
         obj = os.path.join(objdir, s + ".o")
         lang = ["-x", "hip"] if s.endswith(".cpp") else []
         cmd = [HIPCC, *FLAGS, *lang, "-c", src, "-o", obj]
@@ -59,7 +78,7 @@ def build(verbose: bool = False) -> str:
             sys.stderr.write(f"--- {s} ---\n{msg}\n")
         raise RuntimeError("hipcc failed for: " + ", ".join(s for s, _ in failed))
     tmp = LIB + ".tmp"
-    link = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    link = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lhiprtc"]
     if verbose:
         print(" ".join(link), flush=True)
     subprocess.run(link, check=True)

@@ -173,6 +173,7 @@ class DeviceProgram:
     num_base_slots: int
     num_ext_slots: int
     max_offset: int          # largest frame offset a Var reads
+    root_nodes: np.ndarray = None  # per constraint: index (in `nodes`) of the node computing it
 
     def packed_nodes(self) -> np.ndarray:
         """2 u64 per node: op | a << 32, b | res << 32 (constraint_ir/gpu_interp.rs:30-38)."""
@@ -183,8 +184,11 @@ class DeviceProgram:
         return out
 
 
-def lower(prog: ConstraintProgram) -> DeviceProgram:
-    """DeviceProgram::lower (device.rs:200-330)."""
+def lower(prog: ConstraintProgram, pin_roots: bool = True) -> DeviceProgram:
+    """DeviceProgram::lower (device.rs:200-330). With pin_roots=False a
+    constraint's slot is reused once its value has been consumed: the consumer
+    must take each constraint value when its node runs (root_nodes), which the
+    GPU composition kernel does; the slot set is then the program's live set."""
     n = len(prog.nodes)
     assert n <= OPK_PAYLOAD_MASK
     used, last_use = [False] * n, [0] * n
@@ -213,6 +217,7 @@ def lower(prog: ConstraintProgram) -> DeviceProgram:
         raise AssertionError(op)
 
     slot_of = [None] * n
+    node_index, root_slot = [None] * n, [None] * n
     free_base, free_ext = [], []
     nb = ne = 0
     rows = []
@@ -253,7 +258,7 @@ def lower(prog: ConstraintProgram) -> DeviceProgram:
             tag, a, b = OP_EMBED, enc(op[1]), 0
         # free operand slots at their last use (before allocating the result)
         for j in _operands(op):
-            if emitted[j] and not is_root[j] and last_use[j] == i and slot_of[j] is not None:
+            if emitted[j] and (not is_root[j] or not pin_roots) and last_use[j] == i and slot_of[j] is not None:
                 (free_base if prog.dims[j] == BASE else free_ext).append(slot_of[j])
                 slot_of[j] = None
         if dim == BASE:
@@ -267,15 +272,19 @@ def lower(prog: ConstraintProgram) -> DeviceProgram:
             else:
                 s, ne = ne, ne + 1
         slot_of[i] = s
+        node_index[i] = len(rows)
         rows.append((tag, a, b, s if dim == BASE else (s | RES_EXT_BIT)))
-    roots = []
-    for r in prog.roots:
-        s = slot_of[r]
-        assert s is not None
-        roots.append(s if prog.dims[r] == BASE else (s | RES_EXT_BIT))
+        root_slot[i] = s if dim == BASE else (s | RES_EXT_BIT)
+        if is_root[i] and not pin_roots and last_use[i] <= i:
+            # consumed at emission and never read again
+            (free_base if dim == BASE else free_ext).append(s)
+            slot_of[i] = None
+    roots = [root_slot[r] for r in prog.roots]
+    root_nodes = [node_index[r] for r in prog.roots]
     return DeviceProgram(
         nodes=np.array(rows, dtype=np.uint32).reshape(-1, 4),
         base_consts=np.array(prog.base_consts, dtype=np.uint64),
         ext_consts=np.array(prog.ext_consts, dtype=np.uint64).reshape(-1, 3),
         roots=np.array(roots, dtype=np.uint32),
-        num_base=prog.num_base, num_base_slots=nb, num_ext_slots=ne, max_offset=max_offset)
+        num_base=prog.num_base, num_base_slots=nb, num_ext_slots=ne, max_offset=max_offset,
+        root_nodes=np.array(root_nodes, dtype=np.uint32))

@@ -30,54 +30,11 @@
 #include <string>
 #include <vector>
 
+#include "compose_common.hpp"
 #include "lvm_internal.hpp"
 #include "prof.hpp"
 
 namespace lvm {
-
-constexpr uint32_t OP_CONST_BASE = 0, OP_CONST_EXT = 1, OP_VAR = 2, OP_RAP = 3, OP_ALPHA = 4, OP_OFFSET = 5,
-                   OP_ADD = 6, OP_SUB = 7, OP_MUL = 8, OP_NEG = 9, OP_EMBED = 10;
-constexpr uint32_t K_BASE_SLOT = 0, K_EXT_SLOT = 1, K_BASE_CONST = 2, K_EXT_CONST = 3, K_RAP = 4, K_ALPHA = 5;
-constexpr int COMP_MAX_GROUPS = 4;
-constexpr int COMP_MAX_EXEMPT = 8;
-constexpr int COMP_MAX_STEPS = 16;
-constexpr int COMP_BLOCK = 128;
-
-struct CompArgs {
-    // program
-    const uint64_t *nodes;
-    uint32_t num_nodes, num_roots;
-    const uint64_t *base_consts, *ext_consts, *rap, *alpha, *toff;
-    const uint32_t *roots;
-    uint32_t nbs, nes;  // base / ext slot counts
-    // tables: element (physical row, word) at lde[row * w + word]
-    const uint64_t *main;
-    uint64_t wm;
-    const uint64_t *aux;
-    uint64_t wa;
-    uint32_t log_N;
-    uint64_t next_step;
-    // composition (mode 0) or per-constraint evals (mode 1)
-    const uint64_t *beta;        // num_roots Fp3
-    const uint32_t *root_group;  // num_roots
-    uint32_t num_groups;
-    uint32_t group_ex[COMP_MAX_GROUPS];
-    uint64_t ex_root[COMP_MAX_EXEMPT];  // w^(n-1), w^(n-2), ...
-    uint32_t max_ex;
-    const uint64_t *zcyc;  // 1/(x^n - 1), period B
-    uint32_t log_B;
-    uint64_t offset;
-    const uint64_t *tw_lo, *tw_hi;
-    uint32_t num_steps;
-    uint64_t step_point[COMP_MAX_STEPS];
-    uint32_t num_boundary;
-    const uint32_t *b_col, *b_is_aux, *b_step_idx;  // boundary constraints, step index into step_point
-    const uint64_t *b_value, *b_beta;
-    uint64_t *out;       // mode 0: N Fp3 (order below); mode 1: [(c*N + r)*3]
-    uint32_t natural_out;
-    uint64_t *scratch;   // global slots when not in LDS
-    uint64_t total_threads;
-};
 
 struct Slots {
     uint64_t *base;  // [s * stride]
@@ -100,13 +57,6 @@ struct Val {
     fp3 e;
 };
 
-__device__ __forceinline__ fp3 ld3(const uint64_t *p) { return fp3{p[0], p[1], p[2]}; }
-// w_{2^32}^e from the two-level twiddle tables
-__device__ __forceinline__ uint64_t tw32_table(const uint64_t *lo, const uint64_t *hi, uint32_t e) {
-    return mul(lo[e & 0xFFFFu], hi[e >> 16]);
-}
-__device__ __forceinline__ fp3 embed(uint64_t b) { return fp3{b, 0, 0}; }
-
 __device__ __forceinline__ Val operand(const CompArgs &a, uint32_t enc, const Slots &sl) {
     const uint32_t kind = enc >> 29, x = enc & 0x1FFFFFFFu;
     Val v;
@@ -125,17 +75,73 @@ __device__ __forceinline__ Val operand(const CompArgs &a, uint32_t enc, const Sl
     return v;
 }
 
-// physical row of natural index (r + off*next_step) mod N
-__device__ __forceinline__ uint64_t frame_row(const CompArgs &a, uint32_t r, uint32_t off) {
-    const uint64_t N = 1ull << a.log_N;
-    const uint64_t nat = ((uint64_t)r + off * a.next_step) & (N - 1);
-    return bitrev((uint32_t)nat, a.log_N);
+// Consumes the value of constraint c produced at row r: the composition adds
+// beta_c * value into its zerofier group's sum, the per-constraint mode
+// stores it.
+template <int MODE>
+__device__ __forceinline__ void emit_value(const CompArgs &a, uint32_t c, bool ext, uint64_t vb, const fp3 &ve,
+                                           uint32_t r, uint64_t *gsum) {
+    if (MODE == 1) {
+        const uint64_t N = 1ull << a.log_N;
+        uint64_t *o = a.out + 3 * ((uint64_t)c * N + r);
+        o[0] = ext ? ve.c0 : vb;
+        o[1] = ext ? ve.c1 : 0;
+        o[2] = ext ? ve.c2 : 0;
+        return;
+    }
+    const fp3 beta = ld3(a.beta + 3 * (uint64_t)c);
+    const fp3 t = ext ? f3_mul(ve, beta) : f3_mul_base(beta, vb);
+    uint64_t *gs = gsum + 3 * a.root_group[c] * COMP_BLOCK;  // this lane's group sum in LDS
+    const fp3 cur = fp3{gs[0], gs[COMP_BLOCK], gs[2 * COMP_BLOCK]};
+    const fp3 nxt = f3_add(cur, t);
+    gs[0] = nxt.c0;
+    gs[COMP_BLOCK] = nxt.c1;
+    gs[2 * COMP_BLOCK] = nxt.c2;
 }
 
-__device__ __forceinline__ void walk(const CompArgs &a, uint32_t r, const Slots &sl) {
+constexpr int VAR_BATCH = 8;
+
+// Forward walk of the program at natural row r. Runs of consecutive trace
+// reads are issued together (up to VAR_BATCH loads in flight per lane)
+// before their slots are written. Every constraint value is handed to
+// emit_value when its node has run.
+template <int MODE>
+__device__ __forceinline__ void walk(const CompArgs &a, uint32_t r, const Slots &sl, uint64_t *gsum) {
     for (uint32_t k = 0; k < a.num_nodes; k++) {
         const uint64_t w0 = a.nodes[2 * k], w1 = a.nodes[2 * k + 1];
-        const uint32_t op = (uint32_t)w0, x = (uint32_t)(w0 >> 32), y = (uint32_t)w1, res = (uint32_t)(w1 >> 32);
+        const uint32_t op = (uint32_t)w0;
+        if (op == OP_VAR && ((uint32_t)a.nodes[2 * k + 1] >> 16 & 1)) {
+            // a run of consecutive main-trace reads, loads issued together
+            uint32_t cnt = 1;
+            while (cnt < VAR_BATCH && k + cnt < a.num_nodes && (uint32_t)a.nodes[2 * (k + cnt)] == OP_VAR &&
+                   ((uint32_t)a.nodes[2 * (k + cnt) + 1] >> 16 & 1))
+                cnt++;
+            uint64_t lv[VAR_BATCH];
+#pragma unroll
+            for (int j = 0; j < VAR_BATCH; j++) {
+                if ((uint32_t)j < cnt) {
+                    const uint64_t u0 = a.nodes[2 * (k + j)], u1 = a.nodes[2 * (k + j) + 1];
+                    const uint32_t col = (uint32_t)(u0 >> 32) & 0xFFFF, off = ((uint32_t)u1 >> 8) & 0xFF;
+                    lv[j] = a.main[((uint64_t)col << a.log_N) + frame_row(a, r, off)];
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < VAR_BATCH; j++) {
+                if ((uint32_t)j < cnt) {
+                    const uint32_t res = (uint32_t)(a.nodes[2 * (k + j) + 1] >> 32);
+                    const uint32_t slot = res & 0x7FFFFFFFu;
+                    if (res >> 31)
+                        sl.set_e(slot, embed(lv[j]));
+                    else
+                        sl.b(slot) = lv[j];
+                    for (uint32_t q = a.emit_off[k + j]; q < a.emit_off[k + j + 1]; q++)
+                        emit_value<MODE>(a, a.emit_idx[q], false, lv[j], embed(lv[j]), r, gsum);
+                }
+            }
+            k += cnt - 1;
+            continue;
+        }
+        const uint32_t x = (uint32_t)(w0 >> 32), y = (uint32_t)w1, res = (uint32_t)(w1 >> 32);
         const bool rext = res >> 31;
         const uint32_t slot = res & 0x7FFFFFFFu;
         Val v;
@@ -145,15 +151,10 @@ __device__ __forceinline__ void walk(const CompArgs &a, uint32_t r, const Slots 
         switch (op) {
         case OP_CONST_BASE: v.b = a.base_consts[x]; break;
         case OP_CONST_EXT: v.ext = true; v.e = ld3(a.ext_consts + 3 * (uint64_t)x); break;
-        case OP_VAR: {
-            const uint32_t col = x & 0xFFFF, is_main = (y >> 16) & 1, off = (y >> 8) & 0xFF;
-            const uint64_t row = frame_row(a, r, off);
-            if (is_main) {
-                v.b = a.main[row * a.wm + col];
-            } else {
-                v.ext = true;
-                v.e = ld3(a.aux + row * a.wa + 3 * (uint64_t)col);
-            }
+        case OP_VAR: {  // aux (Fp3) read
+            const uint32_t col = x & 0xFFFF, off = (y >> 8) & 0xFF;
+            v.ext = true;
+            v.e = ld3(a.aux + 3 * (((uint64_t)col << a.log_N) + frame_row(a, r, off)));
             break;
         }
         case OP_RAP: v.ext = true; v.e = ld3(a.rap + 3 * (uint64_t)x); break;
@@ -194,21 +195,20 @@ __device__ __forceinline__ void walk(const CompArgs &a, uint32_t r, const Slots 
             break;
         }
         }
+        if (v.ext && !rext) v.ext = false;  // a base slot keeps component 0 (not produced by valid programs)
         if (rext)
             sl.set_e(slot, v.ext ? v.e : embed(v.b));
         else
             sl.b(slot) = v.b;
+        for (uint32_t q = a.emit_off[k]; q < a.emit_off[k + 1]; q++)
+            emit_value<MODE>(a, a.emit_idx[q], v.ext, v.b, v.ext ? v.e : embed(v.b), r, gsum);
     }
-}
-
-__device__ __forceinline__ fp3 root_value(const CompArgs &a, uint32_t c, const Slots &sl) {
-    const uint32_t rs = a.roots[c];
-    return (rs >> 31) ? sl.e(rs & 0x7FFFFFFFu) : embed(sl.b(rs));
 }
 
 template <bool IN_LDS, int MODE>
 __global__ __launch_bounds__(COMP_BLOCK) void compose_kernel(CompArgs a) {
     extern __shared__ uint64_t s_slots[];
+    __shared__ uint64_t s_gsum[3 * COMP_MAX_GROUPS][COMP_BLOCK];  // per-lane zerofier-group sums
     Slots sl;
     if (IN_LDS) {
         sl.base = s_slots + threadIdx.x;
@@ -223,90 +223,58 @@ __global__ __launch_bounds__(COMP_BLOCK) void compose_kernel(CompArgs a) {
     const uint64_t N = 1ull << a.log_N;
     for (uint64_t p = (uint64_t)blockIdx.x * COMP_BLOCK + threadIdx.x; p < N; p += a.total_threads) {
         const uint32_t r = bitrev((uint32_t)p, a.log_N);  // natural index of physical row p
-        walk(a, r, sl);
-        if (MODE == 1) {
-            for (uint32_t c = 0; c < a.num_roots; c++) {
-                const fp3 v = root_value(a, c, sl);
-                uint64_t *o = a.out + 3 * ((uint64_t)c * N + r);
-                o[0] = v.c0;
-                o[1] = v.c1;
-                o[2] = v.c2;
-            }
-            continue;
-        }
-        fp3 gsum[COMP_MAX_GROUPS];
+        uint64_t *gsum = &s_gsum[0][threadIdx.x];
+        for (uint32_t k = 0; k < 3 * a.num_groups; k++) gsum[k * COMP_BLOCK] = 0;
+        walk<MODE>(a, r, sl, gsum);
+        if (MODE == 1) continue;
+        fp3 gs[COMP_MAX_GROUPS];
 #pragma unroll
-        for (int g = 0; g < COMP_MAX_GROUPS; g++) gsum[g] = fp3{0, 0, 0};
-        for (uint32_t c = 0; c < a.num_roots; c++) {
-            const uint32_t rs = a.roots[c];
-            const fp3 beta = ld3(a.beta + 3 * (uint64_t)c);
-            const fp3 t = (rs >> 31) ? f3_mul(sl.e(rs & 0x7FFFFFFFu), beta) : f3_mul_base(beta, sl.b(rs));
-            const uint32_t g = a.root_group[c];
-#pragma unroll
-            for (int q = 0; q < COMP_MAX_GROUPS; q++)
-                if ((uint32_t)q == g) gsum[q] = f3_add(gsum[q], t);
-        }
-        // x = offset * w_N^r
-        const uint64_t x = mul(a.offset, tw32_table(a.tw_lo, a.tw_hi, r << (32 - a.log_N)));
-        const uint64_t zc = a.zcyc[r & ((1u << a.log_B) - 1)];
-        uint64_t exf[COMP_MAX_EXEMPT + 1];
-        exf[0] = zc;
-#pragma unroll
-        for (int j = 0; j < COMP_MAX_EXEMPT; j++)
-            exf[j + 1] = (uint32_t)j < a.max_ex ? mul(exf[j], sub(x, a.ex_root[j])) : exf[j];
-        fp3 h = fp3{0, 0, 0};
-#pragma unroll
-        for (int g = 0; g < COMP_MAX_GROUPS; g++) {
-            if ((uint32_t)g < a.num_groups) {
-                uint64_t z = exf[0];
-#pragma unroll
-                for (int j = 1; j <= COMP_MAX_EXEMPT; j++)
-                    if ((uint32_t)j == a.group_ex[g]) z = exf[j];
-                h = f3_add(h, f3_mul_base(gsum[g], z));
-            }
-        }
-        if (a.num_boundary) {
-            // 1/(x - w^s) for every distinct step s by one inversion
-            uint64_t pre[COMP_MAX_STEPS];
-            uint64_t run = 1;
-#pragma unroll
-            for (int s = 0; s < COMP_MAX_STEPS; s++) {
-                if ((uint32_t)s < a.num_steps) run = mul(run, sub(x, a.step_point[s]));
-                pre[s] = run;
-            }
-            uint64_t inv_all = inv(run);
-            uint64_t dinv[COMP_MAX_STEPS];
-#pragma unroll
-            for (int s = COMP_MAX_STEPS - 1; s >= 0; s--) {
-                if ((uint32_t)s < a.num_steps) {
-                    dinv[s] = s ? mul(inv_all, pre[s - 1]) : inv_all;
-                    inv_all = mul(inv_all, sub(x, a.step_point[s]));
-                } else {
-                    dinv[s] = 0;
-                }
-            }
-            for (uint32_t bi = 0; bi < a.num_boundary; bi++) {
-                const uint32_t col = a.b_col[bi];
-                fp3 t;
-                if (a.b_is_aux[bi])
-                    t = ld3(a.aux + p * a.wa + 3 * (uint64_t)col);
-                else
-                    t = embed(a.main[p * a.wm + col]);
-                const fp3 bp = f3_sub(t, ld3(a.b_value + 3 * (uint64_t)bi));
-                const uint32_t si = a.b_step_idx[bi];
-                uint64_t zb = 0;
-#pragma unroll
-                for (int s = 0; s < COMP_MAX_STEPS; s++)
-                    if ((uint32_t)s == si) zb = dinv[s];
-                h = f3_add(h, f3_mul(f3_mul_base(ld3(a.b_beta + 3 * (uint64_t)bi), zb), bp));
-            }
-        }
+        for (int g = 0; g < COMP_MAX_GROUPS; g++)
+            gs[g] = (uint32_t)g < a.num_groups ? fp3{gsum[3 * g * COMP_BLOCK], gsum[(3 * g + 1) * COMP_BLOCK],
+                                                    gsum[(3 * g + 2) * COMP_BLOCK]}
+                                               : fp3{0, 0, 0};
+        const fp3 h = composition_tail(a, r, p, gs);
         const uint64_t orow = a.natural_out ? r : p;
         uint64_t *o = a.out + 3 * orow;
         o[0] = h.c0;
         o[1] = h.c1;
         o[2] = h.c2;
     }
+}
+
+// Column-major copy of a row-major table for the constraint walk: element
+// (row, c) of E words moves to out[(c * N + row) * E + k]. A wave's trace
+// reads then cover consecutive rows of one column (one contiguous segment)
+// instead of one word from each of 64 rows. 32 x 32-element tiles through
+// LDS; loads and stores are both row-contiguous.
+template <int E>
+__global__ __launch_bounds__(256) void transpose_kernel(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
+                                                        uint64_t N, uint32_t ncols) {
+    constexpr int T = 32;
+    __shared__ uint64_t tile[T][T * E + 1];
+    const uint64_t r0 = (uint64_t)blockIdx.x * T;
+    const uint32_t c0 = blockIdx.y * T;
+    const uint64_t wpr = (uint64_t)ncols * E;  // words per input row
+    for (int i = threadIdx.x; i < T * T * E; i += 256) {
+        const int rr = i / (T * E), ww = i % (T * E);
+        const uint64_t r = r0 + rr, w = (uint64_t)c0 * E + ww;
+        if (r < N && w < wpr) tile[rr][ww] = in[r * wpr + w];
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < T * T * E; i += 256) {
+        const int k = i % E, rr = (i / E) % T, cc = i / (E * T);
+        const uint64_t r = r0 + rr;
+        const uint32_t c = c0 + cc;
+        if (r < N && c < ncols) out[((uint64_t)c * N + r) * E + k] = tile[rr][cc * E + k];
+    }
+}
+
+static void launch_transpose(const uint64_t *in, uint64_t *out, uint64_t N, uint32_t ncols, int E, hipStream_t st) {
+    dim3 grid((unsigned)((N + 31) / 32), (ncols + 31) / 32);
+    if (E == 1)
+        hipLaunchKernelGGL(transpose_kernel<1>, grid, dim3(256), 0, st, in, out, N, ncols);
+    else
+        hipLaunchKernelGGL(transpose_kernel<3>, grid, dim3(256), 0, st, in, out, N, ncols);
 }
 
 // Degree-2 split of the composition polynomial (crypto/stark/src/prover.rs
@@ -366,6 +334,7 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
     const size_t N = main->N;
     if (!is_pow2(N) || log2u(N) > 32) return fail(LVM_ERR_ARG, "bad LDE size");
     if (aux && aux->N != N) return fail(LVM_ERR_ARG, "main and aux LDE sizes differ");
+    if (aux && aux->w % 3) return fail(LVM_ERR_ARG, "aux table rows must hold whole Fp3 elements");
     if (next_step >= N) return fail(LVM_ERR_ARG, "frame step must be smaller than the LDE");
     if (pg->num_nodes == 0 || pg->num_roots == 0 || !pg->nodes || !pg->roots)
         return fail(LVM_ERR_ARG, "empty program");
@@ -424,7 +393,7 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
             break;
         }
     }
-    (void)max_off;
+    (void)max_off;  // the frame reads stay inside the table for every offset (rows wrap mod N)
     for (size_t c = 0; c < pg->num_roots; c++) {
         const uint32_t rs = (uint32_t)pg->roots[c];
         if ((rs >> 31) ? (rs & 0x7FFFFFFFu) >= pg->num_ext_slots : rs >= pg->num_base_slots)
@@ -435,11 +404,23 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
     std::vector<uint8_t> blob;
     std::vector<uint32_t> roots32(pg->num_roots);
     for (size_t c = 0; c < pg->num_roots; c++) roots32[c] = (uint32_t)pg->roots[c];
+    // node computing each constraint: given, or (pinned root slots) the last
+    // node writing the root's slot
+    std::vector<uint32_t> root_node;
+    if (!root_nodes_of(pg, root_node)) return fail(LVM_ERR_ARG, "root node index out of range or no node writes a root slot");
+    std::vector<uint32_t> emit_off(pg->num_nodes + 1, 0), emit_idx(pg->num_roots);
+    for (size_t c = 0; c < pg->num_roots; c++) emit_off[root_node[c] + 1]++;
+    for (size_t k = 0; k < pg->num_nodes; k++) emit_off[k + 1] += emit_off[k];
+    {
+        std::vector<uint32_t> fill(emit_off.begin(), emit_off.end() - 1);
+        for (size_t c = 0; c < pg->num_roots; c++) emit_idx[fill[root_node[c]]++] = (uint32_t)c;
+    }
     const size_t o_nodes = put(blob, pg->nodes, 2 * pg->num_nodes), o_bc = put(blob, pg->base_consts, pg->num_base_consts),
                  o_ec = put(blob, pg->ext_consts, 3 * pg->num_ext_consts), o_rap = put(blob, pg->rap_challenges, 3 * pg->num_rap),
                  o_al = put(blob, pg->alpha_powers, 3 * pg->num_alpha),
-                 o_to = put(blob, pg->table_offset, pg->table_offset ? 3 : 0), o_roots = put(blob, roots32.data(), roots32.size());
-    size_t o_beta = 0, o_rg = 0, o_z = 0, o_bcol = 0, o_baux = 0, o_bsi = 0, o_bval = 0, o_bbeta = 0;
+                 o_to = put(blob, pg->table_offset, pg->table_offset ? 3 : 0), o_roots = put(blob, roots32.data(), roots32.size()),
+                 o_eoff = put(blob, emit_off.data(), emit_off.size()), o_eidx = put(blob, emit_idx.data(), emit_idx.size());
+    size_t o_beta = 0, o_rg = 0, o_z = 0, o_bcol = 0, o_baux = 0, o_bval = 0, o_bbeta = 0;
     if (mode == 0) {
         const size_t n = cp->trace_rows;
         if (!is_pow2(n) || N % n || n < 2) return fail(LVM_ERR_ARG, "trace rows must be a power of two dividing the LDE size");
@@ -496,14 +477,33 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
         a.num_steps = (uint32_t)steps.size();
         for (size_t k = 0; k < steps.size(); k++) a.step_point[k] = pow(wn, steps[k]);
         a.num_boundary = (uint32_t)cp->num_boundary;
+        // boundaries regrouped by step (stable), with their columns, values and betas
+        std::vector<uint32_t> order;
+        for (uint32_t k = 0; k <= a.num_steps; k++) a.step_off[k] = 0;
+        for (uint32_t k = 0; k < a.num_steps; k++) {
+            a.step_off[k] = (uint32_t)order.size();
+            for (size_t b = 0; b < cp->num_boundary; b++)
+                if (step_idx[b] == k) order.push_back((uint32_t)b);
+        }
+        a.step_off[a.num_steps] = (uint32_t)order.size();
+        std::vector<uint32_t> bcol(order.size()), baux(order.size());
+        std::vector<uint64_t> bval(3 * order.size()), bbeta(3 * order.size());
+        for (size_t i = 0; i < order.size(); i++) {
+            const uint32_t b = order[i];
+            bcol[i] = cp->b_col[b];
+            baux[i] = cp->b_is_aux[b] ? 1u : 0u;
+            for (int k = 0; k < 3; k++) {
+                bval[3 * i + k] = cp->b_value[3 * b + k];
+                bbeta[3 * i + k] = cp->b_beta[3 * b + k];
+            }
+        }
         o_beta = put(blob, cp->beta, 3 * pg->num_roots);
         o_rg = put(blob, cp->root_group, pg->num_roots);
         o_z = put(blob, zc.data(), B);
-        o_bcol = put(blob, cp->b_col, cp->num_boundary);
-        o_baux = put(blob, cp->b_is_aux, cp->num_boundary);
-        o_bsi = put(blob, step_idx.data(), cp->num_boundary);
-        o_bval = put(blob, cp->b_value, 3 * cp->num_boundary);
-        o_bbeta = put(blob, cp->b_beta, 3 * cp->num_boundary);
+        o_bcol = put(blob, bcol.data(), bcol.size());
+        o_baux = put(blob, baux.data(), baux.size());
+        o_bval = put(blob, bval.data(), bval.size());
+        o_bbeta = put(blob, bbeta.data(), bbeta.size());
     }
     Device *dv = device_state();
     if (!dv) return LVM_ERR_DEVICE;
@@ -515,11 +515,24 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
     const uint64_t blocks = std::min<uint64_t>((N + COMP_BLOCK - 1) / COMP_BLOCK, max_blocks);
     a.total_threads = blocks * COMP_BLOCK;
     uint8_t *db = nullptr;
-    uint64_t *scratch = nullptr;
-    if (lvm_malloc(&db, blob.size()) != hipSuccess ||
-        (!in_lds && lvm_malloc(&scratch, slot_bytes * a.total_threads) != hipSuccess)) {
+    uint64_t *scratch = nullptr, *cm_main = nullptr, *cm_aux = nullptr;
+    auto release = [&]() {
         lvm_free(db);
+        lvm_free(scratch);
+        lvm_free(cm_main);
+        lvm_free(cm_aux);
+    };
+    if (lvm_malloc(&db, blob.size()) != hipSuccess ||
+        (!in_lds && lvm_malloc(&scratch, slot_bytes * a.total_threads) != hipSuccess) ||
+        lvm_malloc(&cm_main, N * main->w * 8) != hipSuccess ||
+        (aux && lvm_malloc(&cm_aux, N * aux->w * 8) != hipSuccess)) {
+        release();
         return fail(LVM_ERR_NOMEM, "device allocation for the constraint program failed");
+    }
+    {
+        ProfScope ps("composition_transpose", st);
+        launch_transpose(main->lde, cm_main, N, (uint32_t)main->w, 1, st);
+        if (aux) launch_transpose(aux->lde, cm_aux, N, (uint32_t)(aux->w / 3), 3, st);
     }
     hipError_t e = hipMemcpyAsync(db, blob.data(), blob.size(), hipMemcpyHostToDevice, st);
     a.nodes = (const uint64_t *)(db + o_nodes);
@@ -531,12 +544,12 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
     a.alpha = (const uint64_t *)(db + o_al);
     a.toff = (const uint64_t *)(db + o_to);
     a.roots = (const uint32_t *)(db + o_roots);
+    a.emit_off = (const uint32_t *)(db + o_eoff);
+    a.emit_idx = (const uint32_t *)(db + o_eidx);
     a.nbs = (uint32_t)pg->num_base_slots;
     a.nes = (uint32_t)pg->num_ext_slots;
-    a.main = main->lde;
-    a.wm = main->w;
-    a.aux = aux ? aux->lde : nullptr;
-    a.wa = aux ? aux->w : 0;
+    a.main = cm_main;
+    a.aux = cm_aux;
     a.log_N = logN;
     a.next_step = next_step;
     a.tw_lo = dv->tb.fwd_lo;
@@ -547,7 +560,6 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
         a.zcyc = (const uint64_t *)(db + o_z);
         a.b_col = (const uint32_t *)(db + o_bcol);
         a.b_is_aux = (const uint32_t *)(db + o_baux);
-        a.b_step_idx = (const uint32_t *)(db + o_bsi);
         a.b_value = (const uint64_t *)(db + o_bval);
         a.b_beta = (const uint64_t *)(db + o_bbeta);
     }
@@ -555,7 +567,13 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
     a.natural_out = natural_out ? 1u : 0u;
     a.scratch = scratch;
     const size_t lds = in_lds ? slot_bytes * COMP_BLOCK : 0;
-    {
+    // the compiled form of the program when there is one (compiled on first use)
+    const bool jit = jit_enabled() && jit_prepare(pg, root_node, mode == 0 ? cp->root_group : nullptr, mode);
+    if (jit) {
+        ProfScope ps(mode == 0 ? "composition_compiled" : "constraint_evals_compiled", st);
+        if (e == hipSuccess && !jit_launch(pg, root_node, mode == 0 ? cp->root_group : nullptr, mode, a, blocks, st))
+            e = hipErrorLaunchFailure;
+    } else {
         ProfScope ps(mode == 0 ? "composition" : "constraint_evals", st);
         if (in_lds) {
             if (mode == 0)
@@ -571,8 +589,7 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
     }
     if (e == hipSuccess) e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(st);
-    lvm_free(db);
-    lvm_free(scratch);
+    release();
     if (e != hipSuccess) return fail(LVM_ERR_DEVICE, std::string("constraint evaluation failed: ") + hipGetErrorString(e));
     return LVM_OK;
 }

@@ -1,0 +1,116 @@
+// Shared by the constraint-program interpreter and the run-time compiled
+// constraint kernels (compose.hip): launch arguments, table access helpers
+// and the per-row composition tail (zerofiers and boundary terms). The
+// build embeds this text for the run-time compiler, so it includes nothing
+// beyond field.hpp and uses only plain C++ and HIP device code.
+#pragma once
+#include "field.hpp"
+
+namespace lvm {
+
+constexpr uint32_t OP_CONST_BASE = 0, OP_CONST_EXT = 1, OP_VAR = 2, OP_RAP = 3, OP_ALPHA = 4, OP_OFFSET = 5,
+                   OP_ADD = 6, OP_SUB = 7, OP_MUL = 8, OP_NEG = 9, OP_EMBED = 10;
+constexpr uint32_t K_BASE_SLOT = 0, K_EXT_SLOT = 1, K_BASE_CONST = 2, K_EXT_CONST = 3, K_RAP = 4, K_ALPHA = 5;
+constexpr int COMP_MAX_GROUPS = 4;
+constexpr int COMP_MAX_EXEMPT = 8;
+constexpr int COMP_MAX_STEPS = 16;
+constexpr int COMP_BLOCK = 128;
+
+struct CompArgs {
+    // program
+    const uint64_t *nodes;
+    uint32_t num_nodes, num_roots;
+    const uint64_t *base_consts, *ext_consts, *rap, *alpha, *toff;
+    const uint32_t *roots;
+    const uint32_t *emit_off, *emit_idx;  // per node: the constraints whose value it computes (CSR)
+    uint32_t nbs, nes;  // base / ext slot counts
+    // column-major copies of the tables (rows in the LDE's bit-reversed order):
+    // main (col, row) at main[col * N + row]; aux Fp3 (col, row) at aux[3 * (col * N + row)]
+    const uint64_t *main;
+    const uint64_t *aux;
+    uint32_t log_N;
+    uint64_t next_step;
+    // composition (mode 0) or per-constraint evals (mode 1)
+    const uint64_t *beta;        // num_roots Fp3
+    const uint32_t *root_group;  // num_roots
+    uint32_t num_groups;
+    uint32_t group_ex[COMP_MAX_GROUPS];
+    uint64_t ex_root[COMP_MAX_EXEMPT];  // w^(n-1), w^(n-2), ...
+    uint32_t max_ex;
+    const uint64_t *zcyc;  // 1/(x^n - 1), period B
+    uint32_t log_B;
+    uint64_t offset;
+    const uint64_t *tw_lo, *tw_hi;
+    uint32_t num_steps;
+    uint64_t step_point[COMP_MAX_STEPS];
+    uint32_t num_boundary;
+    uint32_t step_off[COMP_MAX_STEPS + 1];  // boundaries of step s: [step_off[s], step_off[s+1])
+    const uint32_t *b_col, *b_is_aux;      // boundary constraints, grouped by step
+    const uint64_t *b_value, *b_beta;
+    uint64_t *out;       // mode 0: N Fp3 (order below); mode 1: [(c*N + r)*3]
+    uint32_t natural_out;
+    uint64_t *scratch;   // global slots when not in LDS
+    uint64_t total_threads;
+};
+
+__device__ __forceinline__ fp3 ld3(const uint64_t *p) { return fp3{p[0], p[1], p[2]}; }
+// w_{2^32}^e from the two-level twiddle tables
+__device__ __forceinline__ uint64_t tw32_table(const uint64_t *lo, const uint64_t *hi, uint32_t e) {
+    return mul(lo[e & 0xFFFFu], hi[e >> 16]);
+}
+__device__ __forceinline__ fp3 embed(uint64_t b) { return fp3{b, 0, 0}; }
+
+// physical row of natural index (r + off*next_step) mod N
+__device__ __forceinline__ uint64_t frame_row(const CompArgs &a, uint32_t r, uint32_t off) {
+    const uint64_t N = 1ull << a.log_N;
+    const uint64_t nat = ((uint64_t)r + off * a.next_step) & (N - 1);
+    return bitrev((uint32_t)nat, a.log_N);
+}
+
+// H(x_r) from the per-group transition sums: each group's sum times its
+// inverse zerofier prod_j (x - w^(n-j)) / (x^n - 1), plus the boundary terms
+// beta_b (t_b(x) - v_b) / (x - w^step_b) with all step denominators inverted
+// together (one base inversion per row). Boundaries come grouped by step.
+__device__ __forceinline__ fp3 composition_tail(const CompArgs &a, uint32_t r, uint64_t p,
+                                                const fp3 *gs) {
+    // x = offset * w_N^r
+    const uint64_t x = mul(a.offset, tw32_table(a.tw_lo, a.tw_hi, r << (32 - a.log_N)));
+    const uint64_t zc = a.zcyc[r & ((1u << a.log_B) - 1)];
+    fp3 h = fp3{0, 0, 0};
+    for (uint32_t g = 0; g < a.num_groups; g++) {
+        uint64_t z = zc;
+        for (uint32_t j = 0; j < a.group_ex[g]; j++) z = mul(z, sub(x, a.ex_root[j]));
+        fp3 gv = gs[0];
+#pragma unroll
+        for (int q = 1; q < COMP_MAX_GROUPS; q++)
+            if ((uint32_t)q == g) gv = gs[q];
+        h = f3_add(h, f3_mul_base(gv, z));
+    }
+    if (a.num_boundary) {
+        // one inversion for all steps: 1/(x - w^s) = inv(prod) * prod_{t != s}(x - w^t);
+        // boundaries come grouped by step
+        uint64_t prod = 1;
+        for (uint32_t s = 0; s < a.num_steps; s++) prod = mul(prod, sub(x, a.step_point[s]));
+        const uint64_t inv_all = inv(prod);
+        for (uint32_t s = 0; s < a.num_steps; s++) {
+            uint64_t zb = inv_all;
+            for (uint32_t t = 0; t < a.num_steps; t++)
+                if (t != s) zb = mul(zb, sub(x, a.step_point[t]));
+            fp3 acc = fp3{0, 0, 0};
+            for (uint32_t bi = a.step_off[s]; bi < a.step_off[s + 1]; bi++) {
+                const uint32_t col = a.b_col[bi];
+                fp3 t;
+                if (a.b_is_aux[bi])
+                    t = ld3(a.aux + 3 * (((uint64_t)col << a.log_N) + p));
+                else
+                    t = embed(a.main[((uint64_t)col << a.log_N) + p]);
+                const fp3 bp = f3_sub(t, ld3(a.b_value + 3 * (uint64_t)bi));
+                acc = f3_add(acc, f3_mul(ld3(a.b_beta + 3 * (uint64_t)bi), bp));
+            }
+            h = f3_add(h, f3_mul_base(acc, zb));
+        }
+    }
+    return h;
+}
+
+}  // namespace lvm

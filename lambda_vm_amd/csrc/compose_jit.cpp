@@ -1,0 +1,344 @@
+// Run-time compiled constraint kernels.
+//
+// The interpreter (compose.hip) decodes every node of the constraint program
+// at every LDE row: scalar loads of the node words, a dispatch branch, slot
+// traffic through LDS. Here the program is turned into straight-line HIP
+// source once (one local value per node, operands resolved through the slot
+// assignment at generation time, constants as literals, each constraint's
+// value folded into its zerofier group's sum where its node is computed),
+// compiled for gfx950 with hipRTC and cached per program text for the life of
+// the process. The per-row walk then runs from registers, and the compiler
+// schedules the trace reads ahead of their uses. Arithmetic is the same
+// field.hpp code as the interpreter and performs the same operations per
+// node, so both produce the same values.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "compose_common.hpp"
+#include "lvm_internal.hpp"
+
+extern const char *const lvm_jit_field_src;
+
+namespace lvm {
+
+namespace {
+
+std::string u64lit(uint64_t v) {
+    char b[32];
+    std::snprintf(b, sizeof b, "0x%016llxull", (unsigned long long)v);
+    return b;
+}
+
+std::string fp3lit(const uint64_t *v) { return "fp3{" + u64lit(v[0]) + ", " + u64lit(v[1]) + ", " + u64lit(v[2]) + "}"; }
+
+struct Operand {
+    std::string expr;
+    bool ext;
+};
+
+// Straight-line source for the program; empty on a program this generator
+// does not take (then the interpreter runs it).
+std::string generate(const lvm_program *pg, const std::vector<uint32_t> &root_node, const uint32_t *root_group,
+                     int mode) {
+    const size_t n = pg->num_nodes;
+    std::vector<std::vector<uint32_t>> emits(n);
+    for (size_t c = 0; c < pg->num_roots; c++) {
+        if (root_node[c] >= n || (mode == 0 && root_group[c] >= (uint32_t)COMP_MAX_GROUPS)) return "";
+        emits[root_node[c]].push_back((uint32_t)c);
+    }
+    uint32_t max_off = 0;
+    std::vector<std::string> base_slot(pg->num_base_slots), ext_slot(pg->num_ext_slots);
+    std::string body;
+    body.reserve(n * 96);
+    auto operand = [&](uint32_t enc, Operand &o) -> bool {
+        const uint32_t kind = enc >> 29, x = enc & 0x1FFFFFFFu;
+        switch (kind) {
+        case K_BASE_SLOT:
+            if (x >= base_slot.size() || base_slot[x].empty()) return false;
+            o = {base_slot[x], false};
+            return true;
+        case K_EXT_SLOT:
+            if (x >= ext_slot.size() || ext_slot[x].empty()) return false;
+            o = {ext_slot[x], true};
+            return true;
+        case K_BASE_CONST:
+            if (x >= pg->num_base_consts) return false;
+            o = {u64lit(pg->base_consts[x]), false};
+            return true;
+        case K_EXT_CONST:
+            if (x >= pg->num_ext_consts) return false;
+            o = {fp3lit(pg->ext_consts + 3 * (size_t)x), true};
+            return true;
+        case K_RAP:
+            if (x >= pg->num_rap) return false;
+            o = {"ld3(a.rap + " + std::to_string(3 * (size_t)x) + ")", true};
+            return true;
+        case K_ALPHA:
+            if (x >= pg->num_alpha) return false;
+            o = {"ld3(a.alpha + " + std::to_string(3 * (size_t)x) + ")", true};
+            return true;
+        case 6:
+            if (!pg->table_offset) return false;
+            o = {"ld3(a.toff)", true};
+            return true;
+        default: return false;
+        }
+    };
+    auto as_ext = [](const Operand &o) { return o.ext ? o.expr : "embed(" + o.expr + ")"; };
+    for (size_t k = 0; k < n; k++) {
+        const uint64_t w0 = pg->nodes[2 * k], w1 = pg->nodes[2 * k + 1];
+        const uint32_t op = (uint32_t)w0, x = (uint32_t)(w0 >> 32), y = (uint32_t)w1, res = (uint32_t)(w1 >> 32);
+        const bool rext = res >> 31;
+        const uint32_t slot = res & 0x7FFFFFFFu;
+        if (slot >= (rext ? ext_slot.size() : base_slot.size())) return "";
+        const std::string v = "v" + std::to_string(k);
+        std::string expr;
+        bool ext = false;
+        switch (op) {
+        case OP_CONST_BASE:
+            if (x >= pg->num_base_consts) return "";
+            expr = u64lit(pg->base_consts[x]);
+            break;
+        case OP_CONST_EXT:
+            if (x >= pg->num_ext_consts) return "";
+            expr = fp3lit(pg->ext_consts + 3 * (size_t)x);
+            ext = true;
+            break;
+        case OP_VAR: {
+            const uint32_t col = x & 0xFFFF, is_main = (y >> 16) & 1, off = (y >> 8) & 0xFF;
+            max_off = off > max_off ? off : max_off;
+            const std::string idx = "(((uint64_t)" + std::to_string(col) + " << a.log_N) + f" + std::to_string(off) + ")";
+            if (is_main) {
+                expr = "a.main[" + idx + "]";
+            } else {
+                expr = "ld3(a.aux + 3 * " + idx + ")";
+                ext = true;
+            }
+            break;
+        }
+        case OP_RAP:
+        case OP_ALPHA:
+        case OP_OFFSET: {
+            Operand o;
+            if (x > 0x1FFFFFFFu) return "";
+            if (!operand((op == OP_RAP ? K_RAP : op == OP_ALPHA ? K_ALPHA : 6u) << 29 | x, o)) return "";
+            expr = o.expr;
+            ext = true;
+            break;
+        }
+        case OP_ADD:
+        case OP_SUB:
+        case OP_MUL: {
+            Operand p, q;
+            if (!operand(x, p) || !operand(y, q)) return "";
+            if (!p.ext && !q.ext) {
+                expr = std::string(op == OP_ADD ? "add(" : op == OP_SUB ? "sub(" : "mul(") + p.expr + ", " + q.expr + ")";
+            } else {
+                ext = true;
+                if (op == OP_MUL && !p.ext)
+                    expr = "f3_mul_base(" + q.expr + ", " + p.expr + ")";
+                else if (op == OP_MUL && !q.ext)
+                    expr = "f3_mul_base(" + p.expr + ", " + q.expr + ")";
+                else
+                    expr = std::string(op == OP_ADD ? "f3_add(" : op == OP_SUB ? "f3_sub(" : "f3_mul(") + as_ext(p) +
+                           ", " + as_ext(q) + ")";
+            }
+            break;
+        }
+        case OP_NEG: {
+            Operand p;
+            if (!operand(x, p)) return "";
+            ext = p.ext;
+            expr = (p.ext ? "f3_neg(" : "neg(") + p.expr + ")";
+            break;
+        }
+        case OP_EMBED: {
+            Operand p;
+            if (!operand(x, p)) return "";
+            ext = true;
+            expr = as_ext(p);
+            break;
+        }
+        default: return "";
+        }
+        if (ext != rext) return "";  // result class differs from its slot class: leave it to the interpreter
+        body += std::string("        const ") + (ext ? "fp3 " : "uint64_t ") + v + " = " + expr + ";\n";
+        (ext ? ext_slot : base_slot)[slot] = v;
+        for (uint32_t c : emits[k]) {
+            const std::string beta = "ld3(a.beta + " + std::to_string(3 * (size_t)c) + ")";
+            if (mode == 0) {
+                const std::string g = "g" + std::to_string(root_group[c]);
+                body += "        " + g + " = f3_add(" + g + ", " +
+                        (ext ? "f3_mul(" + v + ", " + beta + ")" : "f3_mul_base(" + beta + ", " + v + ")") + ");\n";
+            } else {
+                body += "        store3(a.out + 3 * ((uint64_t)" + std::to_string(c) + " * N + r), " +
+                        (ext ? v : "embed(" + v + ")") + ");\n";
+            }
+        }
+    }
+    std::string src = "typedef unsigned long long uint64_t;\ntypedef unsigned int uint32_t;\n";
+    src += lvm_jit_field_src;
+    src += "\nnamespace lvm {\n__device__ __forceinline__ void store3(uint64_t *o, const fp3 &v) { o[0] = v.c0; o[1] = v.c1; "
+           "o[2] = v.c2; }\n}\n";
+    src += "extern \"C\" __global__ __launch_bounds__(128) void lvm_constraint_kernel(lvm::CompArgs a) {\n";
+    src += "    using namespace lvm;\n    const uint64_t N = 1ull << a.log_N;\n";
+    src += "    for (uint64_t p = (uint64_t)blockIdx.x * 128 + threadIdx.x; p < N; p += a.total_threads) {\n";
+    src += "        const uint32_t r = bitrev((uint32_t)p, a.log_N);\n";
+    for (uint32_t off = 0; off <= max_off; off++)
+        src += "        const uint64_t f" + std::to_string(off) + " = frame_row(a, r, " + std::to_string(off) + ");\n";
+    if (mode == 0)
+        src += "        fp3 g0 = fp3{0, 0, 0}, g1 = fp3{0, 0, 0}, g2 = fp3{0, 0, 0}, g3 = fp3{0, 0, 0};\n";
+    src += body;
+    if (mode == 0) {
+        src += "        const fp3 gs[COMP_MAX_GROUPS] = {g0, g1, g2, g3};\n";
+        src += "        const fp3 h = composition_tail(a, r, p, gs);\n";
+        src += "        store3(a.out + 3 * (a.natural_out ? (uint64_t)r : p), h);\n";
+    }
+    src += "    }\n}\n";
+    return src;
+}
+
+// hipRTC compile to a gfx950 code object; empty on failure (the log goes to
+// stderr when LVM_JIT_LOG is set).
+std::vector<char> compile(const std::string &src) {
+    std::vector<char> code;
+    hiprtcProgram prog;
+    if (hiprtcCreateProgram(&prog, src.c_str(), "lvm_constraints.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
+        return code;
+    const char *opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature", "-Xclang",
+                          "-packed-fp32-ops"};
+    if (hiprtcCompileProgram(prog, 7, opts) == HIPRTC_SUCCESS) {
+        size_t size = 0;
+        if (hiprtcGetCodeSize(prog, &size) == HIPRTC_SUCCESS && size) {
+            code.resize(size);
+            if (hiprtcGetCode(prog, code.data()) != HIPRTC_SUCCESS) code.clear();
+        }
+    } else if (std::getenv("LVM_JIT_LOG")) {
+        size_t ls = 0;
+        hiprtcGetProgramLogSize(prog, &ls);
+        std::string log(ls, '\0');
+        if (ls) hiprtcGetProgramLog(prog, &log[0]);
+        std::fprintf(stderr, "constraint kernel compile failed:\n%s\n", log.c_str());
+    }
+    hiprtcDestroyProgram(&prog);
+    return code;
+}
+
+// One compiled program: its code object, and the loaded kernel per device
+// (each GPU loads its own module the first time it runs the program).
+struct JitEntry {
+    std::vector<char> code;
+    std::unordered_map<int, hipFunction_t> fn;
+};
+
+std::mutex g_jit_mu;
+std::unordered_map<std::string, std::unique_ptr<JitEntry>> g_jit;  // keyed by generated source
+int g_jit_mode = -1;  // -1: unset (LVM_CONSTRAINT_JIT decides, default on), 0: off, 1: on
+
+// Cached entry for the source, compiling it on first sight; null when it
+// does not compile. Called with g_jit_mu held.
+JitEntry *entry_for(const std::string &src) {
+    auto it = g_jit.find(src);
+    if (it == g_jit.end()) {
+        auto e = std::make_unique<JitEntry>();
+        e->code = compile(src);
+        it = g_jit.emplace(src, std::move(e)).first;
+    }
+    return it->second->code.empty() ? nullptr : it->second.get();
+}
+
+}  // namespace
+
+bool root_nodes_of(const lvm_program *pg, std::vector<uint32_t> &root_node) {
+    root_node.assign(pg->num_roots, 0);
+    for (size_t c = 0; c < pg->num_roots; c++) {
+        if (pg->root_nodes) {
+            root_node[c] = pg->root_nodes[c];
+            if (root_node[c] >= pg->num_nodes) return false;
+        } else {
+            // pinned root slots: the last node writing the root's slot
+            size_t k = pg->num_nodes;
+            while (k > 0 && (pg->nodes[2 * (k - 1) + 1] >> 32) != (uint32_t)pg->roots[c]) k--;
+            if (k == 0) return false;
+            root_node[c] = (uint32_t)(k - 1);
+        }
+    }
+    return true;
+}
+
+bool jit_enabled() {
+    std::lock_guard<std::mutex> lk(g_jit_mu);
+    if (g_jit_mode < 0) {
+        const char *e = std::getenv("LVM_CONSTRAINT_JIT");
+        g_jit_mode = (e && e[0] == '0') ? 0 : 1;
+    }
+    return g_jit_mode == 1;
+}
+
+bool jit_prepare(const lvm_program *pg, const std::vector<uint32_t> &root_node, const uint32_t *root_group,
+                 int mode) {
+    const std::string src = generate(pg, root_node, root_group, mode);
+    if (src.empty()) return false;
+    std::lock_guard<std::mutex> lk(g_jit_mu);
+    return entry_for(src) != nullptr;
+}
+
+bool jit_launch(const lvm_program *pg, const std::vector<uint32_t> &root_node, const uint32_t *root_group, int mode,
+                CompArgs &args, uint64_t blocks, hipStream_t st) {
+    const std::string src = generate(pg, root_node, root_group, mode);
+    if (src.empty()) return false;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return false;
+    hipFunction_t fn = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_jit_mu);
+        JitEntry *e = entry_for(src);
+        if (!e) return false;
+        auto it = e->fn.find(dev);
+        if (it != e->fn.end()) {
+            fn = it->second;
+        } else {
+            hipModule_t mod = nullptr;
+            if (hipModuleLoadData(&mod, e->code.data()) != hipSuccess) return false;
+            if (hipModuleGetFunction(&fn, mod, "lvm_constraint_kernel") != hipSuccess) {
+                hipModuleUnload(mod);
+                return false;
+            }
+            e->fn.emplace(dev, fn);  // the module stays loaded for the life of the process
+        }
+    }
+    void *params[] = {&args};
+    return hipModuleLaunchKernel(fn, (unsigned)blocks, 1, 1, COMP_BLOCK, 1, 1, 0, st, params, nullptr) == hipSuccess;
+}
+
+}  // namespace lvm
+
+extern "C" {
+
+int lvm_set_constraint_jit(int on) {
+    std::lock_guard<std::mutex> lk(lvm::g_jit_mu);
+    lvm::g_jit_mode = on ? 1 : 0;
+    return LVM_OK;
+}
+
+int lvm_prepare_constraints(const lvm_program *program, const lvm_composition *comp) {
+    if (!program || !program->nodes || !program->roots || program->num_roots == 0)
+        return lvm::fail(LVM_ERR_ARG, "empty program");
+    if (comp && !comp->root_group) return lvm::fail(LVM_ERR_ARG, "composition arrays missing");
+    std::vector<uint32_t> root_node;
+    if (!lvm::root_nodes_of(program, root_node)) return lvm::fail(LVM_ERR_ARG, "cannot locate the constraint nodes");
+    if (!lvm::jit_prepare(program, root_node, comp ? comp->root_group : nullptr, comp ? 0 : 1))
+        return lvm::fail(LVM_ERR_UNSUPPORTED, "constraint program could not be compiled");
+    return LVM_OK;
+}
+
+}  // extern "C"

@@ -4,6 +4,7 @@
 #include <stdint.h>
 #include <stddef.h>
 #include <string>
+#include <vector>
 
 #include "field.hpp"
 #include "lvm_gpu.h"
@@ -151,5 +152,17 @@ void launch_leaves_batched(const LeafJob *d_jobs, const uint32_t *d_prefix, uint
 void launch_tree_steps_batched(const TreeStep *d_steps, const uint32_t *d_prefix, uint32_t count, uint32_t blocks,
                                hipStream_t st);
 void launch_gather_roots(uint64_t *const *d_trees, uint32_t count, uint64_t *d_roots, hipStream_t st);
+
+// Constraint programs (compose.hip interpreter, compose_jit.cpp compiled form).
+struct CompArgs;
+// Node computing each constraint (given, or the last writer of a pinned root slot).
+bool root_nodes_of(const lvm_program *pg, std::vector<uint32_t> &root_node);
+bool jit_enabled();
+// Compiles the program's kernel (or finds it compiled); false when the
+// generator does not take the program or the compile fails.
+bool jit_prepare(const lvm_program *pg, const std::vector<uint32_t> &root_node, const uint32_t *root_group, int mode);
+// Launches the compiled kernel on the current device; false on failure.
+bool jit_launch(const lvm_program *pg, const std::vector<uint32_t> &root_node, const uint32_t *root_group, int mode,
+                CompArgs &args, uint64_t blocks, hipStream_t st);
 
 }  // namespace lvm

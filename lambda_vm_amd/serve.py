@@ -11,9 +11,10 @@ same ideas serve a stream of jobs:
   - every scheduler tick commits the tables of all jobs admitted since the
     previous tick in one batched launch sequence (lvm_batch_*), so jobs that
     arrive while others are on the GPU share launches with each other;
-  - committed jobs then advance one table per tick through the remaining
-    rounds (OOD, DEEP, FRI, queries), interleaved across jobs, and free each
-    table's LDE, tree and FRI layers as soon as its rounds finish;
+  - committed jobs then advance through the remaining rounds (OOD, DEEP,
+    FRI, queries) a table at a time, a few tables per tick, oldest job first
+    (or one table of every job per tick), and free each table's LDE, tree and
+    FRI layers as soon as its rounds finish;
   - per job: latency from arrival to its final digest; per run: p50/p99
     latency, jobs/s and LDE cells/s.
 A job's result depends only on its own inputs and transcript: the digest of
@@ -167,7 +168,15 @@ class GpuExecutor:
 
 
 class ProvingService:
-    def __init__(self, hbm_budget: int, executor=None, clock=time.perf_counter):
+    def __init__(self, hbm_budget: int, executor=None, clock=time.perf_counter, policy: str = "oldest",
+                 tables_per_tick: int = 4):
+        """policy "oldest": each tick proves up to tables_per_tick tables, oldest
+        admitted job first (jobs finish in admission order, which keeps the
+        latency of early jobs low); "round_robin": one table of every running
+        job per tick (processor sharing)."""
+        if policy not in ("oldest", "round_robin"):
+            raise ValueError(policy)
+        self.policy, self.tables_per_tick = policy, tables_per_tick
         self.budget = hbm_budget
         self.ex = executor if executor is not None else GpuExecutor()
         self.clock = clock
@@ -205,9 +214,17 @@ class ProvingService:
             self.batch_sizes.append(len(admitted))
             self.running.extend(admitted)
         worked = bool(admitted)
-        for js in list(self.running):
-            if js in admitted:
-                continue
+        if self.policy == "round_robin":
+            work = [js for js in self.running if js not in admitted]
+        else:
+            work, left = [], self.tables_per_tick
+            for js in self.running:
+                if js in admitted or left == 0:
+                    continue
+                k = min(left, len(js.spec.tables) - js.next_table)
+                work += [js] * k
+                left -= k
+        for js in work:
             js.digests.append(self.ex.prove_table(js, js.next_table))
             js.next_table += 1
             worked = True

@@ -500,7 +500,7 @@ class _Program(C.Structure):
     _fields_ = [("nodes", C.c_void_p), ("num_nodes", C.c_size_t), ("num_base_slots", C.c_size_t),
                 ("num_ext_slots", C.c_size_t), ("base_consts", C.c_void_p), ("ext_consts", C.c_void_p),
                 ("roots", C.c_void_p), ("num_roots", C.c_size_t), ("rap", C.c_void_p), ("alpha", C.c_void_p),
-                ("table_offset", C.c_void_p)]
+                ("table_offset", C.c_void_p), ("root_nodes", C.c_void_p)]
 
 
 class _Composition(C.Structure):
@@ -530,6 +530,8 @@ def _program(dev, rap, alpha, table_offset, keep):
     s.alpha = _arr(np.asarray(alpha, dtype=np.uint64).reshape(-1) if alpha is not None else None, np.uint64, keep)
     s.table_offset = _arr(np.asarray(table_offset if table_offset is not None else (0, 0, 0), dtype=np.uint64),
                           np.uint64, keep)
+    rn = getattr(dev, "root_nodes", None)
+    s.root_nodes = _arr(rn, np.uint32, keep) if rn is not None else None
     return s
 
 

@@ -49,9 +49,26 @@ static val operand(const ora_program *p, uint32_t enc, const uint64_t *vb, const
 
 static fp3 as_ext(val v) { return v.ext ? v.e : fp3_from_base(v.b); }
 
-/* one row's walk; slots vb[num_base_slots], ve[num_ext_slots] */
+/* Node computing each constraint: given (root_nodes), or for pinned root
+ * slots the last node writing the slot (device.rs pins roots). */
+static int root_nodes_of(const ora_program *p, uint32_t *out) {
+    for (size_t c = 0; c < p->num_roots; c++) {
+        if (p->root_nodes) {
+            out[c] = p->root_nodes[c];
+            continue;
+        }
+        size_t k = p->num_nodes;
+        while (k > 0 && (uint32_t)(p->nodes[2 * (k - 1) + 1] >> 32) != (uint32_t)p->roots[c]) k--;
+        if (k == 0) return -1;
+        out[c] = (uint32_t)(k - 1);
+    }
+    return 0;
+}
+
+/* one row's walk; slots vb[num_base_slots], ve[num_ext_slots]; the value of
+ * constraint c is captured into rv[c] when node root_node[c] has run */
 static void walk(const ora_program *p, const uint64_t *main, size_t m, const uint64_t *aux, size_t ma, size_t N,
-                 size_t next_step, size_t i, uint64_t *vb, fp3 *ve) {
+                 size_t next_step, size_t i, uint64_t *vb, fp3 *ve, const uint32_t *root_node, fp3 *rv) {
     for (size_t k = 0; k < p->num_nodes; k++) {
         const uint64_t w0 = p->nodes[2 * k], w1 = p->nodes[2 * k + 1];
         const uint32_t op = (uint32_t)w0, a = (uint32_t)(w0 >> 32), b = (uint32_t)w1, res = (uint32_t)(w1 >> 32);
@@ -104,28 +121,26 @@ static void walk(const ora_program *p, const uint64_t *main, size_t m, const uin
         }
         if (rext) ve[slot] = as_ext(r);
         else vb[slot] = r.b;
+        for (size_t c = 0; c < p->num_roots; c++)
+            if (root_node[c] == k) rv[c] = as_ext(r);
     }
-}
-
-static fp3 root_value(const ora_program *p, size_t c, const uint64_t *vb, const fp3 *ve) {
-    const uint32_t r = (uint32_t)p->roots[c];
-    return (r >> 31) ? ve[r & 0x7FFFFFFFu] : fp3_from_base(vb[r]);
 }
 
 int ora_eval_constraints(const ora_program *p, const uint64_t *main, size_t m, const uint64_t *aux, size_t ma,
                          size_t N, size_t next_step, uint64_t *evals_out) {
     uint64_t *vb = calloc(p->num_base_slots + 1, 8);
     fp3 *ve = calloc(p->num_ext_slots + 1, sizeof(fp3));
-    if (!vb || !ve) return -1;
+    uint32_t *rn = calloc(p->num_roots + 1, 4);
+    fp3 *rv = calloc(p->num_roots + 1, sizeof(fp3));
+    if (!vb || !ve || !rn || !rv || root_nodes_of(p, rn)) return -1;
     for (size_t i = 0; i < N; i++) {
-        walk(p, main, m, aux, ma, N, next_step, i, vb, ve);
-        for (size_t c = 0; c < p->num_roots; c++) {
-            const fp3 v = root_value(p, c, vb, ve);
-            memcpy(evals_out + 3 * (c * N + i), v.c, 24);
-        }
+        walk(p, main, m, aux, ma, N, next_step, i, vb, ve, rn, rv);
+        for (size_t c = 0; c < p->num_roots; c++) memcpy(evals_out + 3 * (c * N + i), rv[c].c, 24);
     }
     free(vb);
     free(ve);
+    free(rn);
+    free(rv);
     return 0;
 }
 
@@ -142,7 +157,9 @@ int ora_eval_composition(const ora_program *p, const uint64_t *main, size_t m, c
     uint64_t *zcyc = malloc(8 * B);
     uint64_t *vb = calloc(p->num_base_slots + 1, 8);
     fp3 *ve = calloc(p->num_ext_slots + 1, sizeof(fp3));
-    if (!zcyc || !vb || !ve) return -1;
+    uint32_t *rn = calloc(p->num_roots + 1, 4);
+    fp3 *rv = calloc(p->num_roots + 1, sizeof(fp3));
+    if (!zcyc || !vb || !ve || !rn || !rv || root_nodes_of(p, rn)) return -1;
     {
         const uint64_t step = gl_pow(wN, n);
         uint64_t d = gl_pow(cp->coset_offset, n);
@@ -154,7 +171,7 @@ int ora_eval_composition(const ora_program *p, const uint64_t *main, size_t m, c
     const uint64_t w_last = gl_pow(wn, n - 1); /* w^(n-1) = w^-1 */
     uint64_t x = cp->coset_offset;
     for (size_t i = 0; i < N; i++) {
-        walk(p, main, m, aux, ma, N, next_step, i, vb, ve);
+        walk(p, main, m, aux, ma, N, next_step, i, vb, ve, rn, rv);
         fp3 h = fp3_zero();
         for (size_t c = 0; c < p->num_roots; c++) {
             const uint32_t g = cp->root_group[c];
@@ -165,7 +182,7 @@ int ora_eval_composition(const ora_program *p, const uint64_t *main, size_t m, c
                 r = gl_mul(r, w_last);
             }
             const fp3 beta = ld3(cp->beta + 3 * c);
-            h = fp3_add(h, fp3_mul(fp3_mul_base(root_value(p, c, vb, ve), z), beta));
+            h = fp3_add(h, fp3_mul(fp3_mul_base(rv[c], z), beta));
         }
         for (size_t bi = 0; bi < cp->num_boundary; bi++) {
             const uint64_t point = gl_pow(wn, cp->b_step[bi]);
@@ -184,6 +201,8 @@ int ora_eval_composition(const ora_program *p, const uint64_t *main, size_t m, c
     free(zcyc);
     free(vb);
     free(ve);
+    free(rn);
+    free(rv);
     return 0;
 }
 

@@ -164,6 +164,7 @@ typedef struct {
     const uint64_t *roots;
     size_t num_roots;
     const uint64_t *rap, *alpha, *table_offset;
+    const uint32_t *root_nodes; /* nullable: node computing each constraint (else: pinned root slots) */
 } ora_program;
 typedef struct {
     size_t trace_rows;                /* n; N / n is the blowup */

@@ -1,8 +1,9 @@
 """GPU constraint evaluation and composition polynomial (lvm_eval_constraints /
 lvm_eval_composition) against the oracle on the same seeded inputs:
-FibonacciMultiColumn, Quadratic and a 4-bus LogUp table, value slots in LDS
-and in global scratch, bit-reversed and natural output order, a low-degree
-check of the GPU output for valid traces, and program validation."""
+FibonacciMultiColumn, Quadratic and a 4-bus LogUp table, run both as
+run-time compiled kernels and on the interpreter (value slots in LDS and in
+global scratch), bit-reversed and natural output order, a low-degree check of
+the GPU output for valid traces, and program validation."""
 import random
 
 import numpy as np
@@ -15,6 +16,25 @@ from test_oracle_logup import alpha_powers
 
 pytestmark = pytest.mark.gpu
 P = workload.P
+
+
+@pytest.fixture(params=[True, False], ids=["compiled", "interpreted"])
+def jit(request, gpu):
+    """Constraint programs as compiled kernels or on the interpreter; checks
+    that the chosen kernel is the one that ran."""
+    gpu.set_constraint_jit(request.param)
+    gpu.prof_enable(True)
+    gpu.prof_reset()
+    yield request.param
+    names = ("composition_compiled", "constraint_evals_compiled") if request.param else ("composition",
+                                                                                      "constraint_evals")
+    ran = sum(gpu.prof_query(nm)[0] for nm in names)
+    other = ("composition", "constraint_evals") if request.param else ("composition_compiled",
+                                                                       "constraint_evals_compiled")
+    stray = sum(gpu.prof_query(nm)[0] for nm in other)
+    gpu.prof_enable(False)
+    gpu.set_constraint_jit(True)
+    assert ran > 0 and stray == 0
 
 
 def canon(a):
@@ -67,7 +87,7 @@ def _case(gpu, ora, which, n, B, m, rng, valid=True):
 
 @pytest.mark.parametrize("which,n,B,m", [("fib", 64, 4, 5), ("fib", 256, 2, 80), ("quad", 128, 8, 3),
                                          ("logup", 64, 4, 0), ("logup", 1024, 4, 0)])
-def test_composition_matches_oracle(gpu, ora, which, n, B, m):
+def test_composition_matches_oracle(gpu, ora, jit, which, n, B, m):
     rng = random.Random(hash((which, n, B, m)) & 0xFFFFFF)
     c = _case(gpu, ora, which, n, B, m, rng)
     air = c["air"]
@@ -86,7 +106,7 @@ def test_composition_matches_oracle(gpu, ora, which, n, B, m):
 
 
 @pytest.mark.parametrize("which", ["fib", "quad", "logup"])
-def test_gpu_composition_low_degree(gpu, ora, which):
+def test_gpu_composition_low_degree(gpu, ora, jit, which):
     rng = random.Random(5)
     n, B = 64, 4
     c = _case(gpu, ora, which, n, B, 4, rng)
@@ -114,7 +134,7 @@ def test_composition_rejects_bad_programs(gpu, ora):
 
 
 @pytest.mark.parametrize("which,n,B", [("quad", 64, 4), ("logup", 256, 4), ("fib", 128, 2)])
-def test_composition_d2_commit_matches_oracle(gpu, ora, which, n, B):
+def test_composition_d2_commit_matches_oracle(gpu, ora, jit, which, n, B):
     rng = random.Random(hash((which, n)) & 0xFFFF)
     c = _case(gpu, ora, which, n, B, 3, rng)
     air = c["air"]
@@ -127,3 +147,23 @@ def test_composition_d2_commit_matches_oracle(gpu, ora, which, n, B):
     ref_lde, ref_nodes = ora.lde_commit(halves, 2, pow(G, P - 2, P))
     assert parts.root == bytes(ref_nodes[0])
     assert np.array_equal(canon(parts.download()), canon(ref_lde))
+
+
+@pytest.mark.parametrize("which", ["fib", "logup"])
+def test_pinned_reference_style_program(gpu, ora, jit, which):
+    """Programs lowered with pinned root slots and no root node list (the
+    reference lowering's form): the library finds each constraint's node."""
+    from lambda_vm_amd import constraint_ir as ir
+    rng = random.Random(17)
+    c = _case(gpu, ora, which, 64, 4, 6, rng)
+    air = c["air"]
+    dev = ir.lower(air.program)  # pinned roots
+    dev.root_nodes = None
+    args = (4, 64, G, c["betas"], c["groups"], c["ex"], c["bnd"], c["b_betas"], c["rap"], c["alpha"], c["toff"])
+    ref = ora.eval_composition(air.device, c["main_lde"], c["aux_lde"], *args)
+    got = gpu.eval_composition(dev, c["main_h"], c["aux_h"], *args, natural_order=True).to_array(np.uint64,
+                                                                                                (256, 3))
+    assert np.array_equal(canon(got), canon(ref))
+    ev = gpu.eval_constraints(dev, c["main_h"], c["aux_h"], 4, c["rap"], c["alpha"], c["toff"])
+    ev_ref = ora.eval_constraints(dev, c["main_lde"], c["aux_lde"], 4, c["rap"], c["alpha"], c["toff"])
+    assert np.array_equal(canon(ev), canon(ev_ref))
