@@ -38,7 +38,7 @@ static std::vector<Device *> g_devices;
 
 static int build_tables(Device *d) {
     const size_t T = 1u << 16;
-    std::vector<uint64_t> lo(T), hi(T), ilo(T), ihi(T), r(128), ir(128);
+    std::vector<uint64_t> lo(T), hi(T), ilo(T), ihi(T), r(256), ir(256);
     const uint64_t w = ROOT_2_32, wi = inv(ROOT_2_32);
     uint64_t w16 = pow(w, T), wi16 = pow(wi, T);
     uint64_t a = 1, b = 1, c = 1, e = 1;
@@ -50,25 +50,31 @@ static int build_tables(Device *d) {
     }
     uint64_t w256 = pow(w, 1ull << 24), wi256 = pow(wi, 1ull << 24);
     uint64_t p = 1, q = 1;
-    for (int k = 0; k < 128; k++) {
-        r[k] = p; p = mul(p, w256);
-        ir[k] = q; q = mul(q, wi256);
+    for (int k = 0; k < 256; k++) {
+        r[k] = canon(p); p = mul(p, w256);
+        ir[k] = canon(q); q = mul(q, wi256);
+    }
+    for (size_t x = 0; x < T; x++) {  // canonical tables (the NTT's inputs stay canonical)
+        lo[x] = canon(lo[x]);
+        hi[x] = canon(hi[x]);
+        ilo[x] = canon(ilo[x]);
+        ihi[x] = canon(ihi[x]);
     }
     uint64_t *buf = nullptr;
-    const size_t total = 4 * T + 256;
+    const size_t total = 4 * T + 512;
     LVM_HIP(hipMalloc(&buf, total * sizeof(uint64_t)));
     d->tb.fwd_lo = buf;
     d->tb.fwd_hi = buf + T;
     d->tb.inv_lo = buf + 2 * T;
     d->tb.inv_hi = buf + 3 * T;
     d->tb.fwd_r = buf + 4 * T;
-    d->tb.inv_r = buf + 4 * T + 128;
+    d->tb.inv_r = buf + 4 * T + 256;
     LVM_HIP(hipMemcpy(d->tb.fwd_lo, lo.data(), T * 8, hipMemcpyHostToDevice));
     LVM_HIP(hipMemcpy(d->tb.fwd_hi, hi.data(), T * 8, hipMemcpyHostToDevice));
     LVM_HIP(hipMemcpy(d->tb.inv_lo, ilo.data(), T * 8, hipMemcpyHostToDevice));
     LVM_HIP(hipMemcpy(d->tb.inv_hi, ihi.data(), T * 8, hipMemcpyHostToDevice));
-    LVM_HIP(hipMemcpy(d->tb.fwd_r, r.data(), 128 * 8, hipMemcpyHostToDevice));
-    LVM_HIP(hipMemcpy(d->tb.inv_r, ir.data(), 128 * 8, hipMemcpyHostToDevice));
+    LVM_HIP(hipMemcpy(d->tb.fwd_r, r.data(), 256 * 8, hipMemcpyHostToDevice));
+    LVM_HIP(hipMemcpy(d->tb.inv_r, ir.data(), 256 * 8, hipMemcpyHostToDevice));
     return LVM_OK;
 }
 

@@ -113,6 +113,74 @@ LVM_HD uint64_t root_of_unity(unsigned k) {
     return r;
 }
 
+// ---------------- canonical arithmetic ----------------
+// Operations on canonical values (< p) that return canonical values. Every
+// input range is stated; the NTT passes canonicalise what they load and then
+// stay in this set, which needs one conditional correction per add or sub
+// instead of two.
+
+LVM_HD uint64_t ccanon(uint64_t r) { return r >= P ? r + EPS : r; }  // any r -> [0, p)
+
+// a, b < p. A carry means a + b - p = s + EPS (< p, no second wrap); no
+// carry and s >= p means s - p = s + EPS mod 2^64.
+LVM_HD uint64_t cadd(uint64_t a, uint64_t b) {
+    const uint64_t s = a + b;
+    const bool c = (s < a) | (s >= P);
+    return s + (c ? EPS : 0);
+}
+
+// a, b < p. A borrow means a - b + p = d - EPS with d = a - b + 2^64 > EPS.
+LVM_HD uint64_t csub(uint64_t a, uint64_t b) {
+    const uint64_t d = a - b;
+    return a < b ? d - EPS : d;
+}
+
+// canonical reduction of any 128-bit lo + 2^64 hi (2^64 = EPS, 2^96 = -1):
+// t0 = lo - hi_hi (mod p, in [0, 2^64)), t1 = hi_lo * EPS <= (2^32 - 1)^2,
+// t0 + t1 wraps at most once and its correction cannot wrap again.
+LVM_HD uint64_t cred(uint64_t lo, uint64_t hi) {
+    const uint64_t hi_hi = hi >> 32, hi_lo = hi & EPS;
+    uint64_t t0 = lo - hi_hi;
+    t0 -= (lo < hi_hi) ? EPS : 0;
+    const uint64_t t1 = (hi_lo << 32) - hi_lo;
+    uint64_t r = t0 + t1;
+    r += (r < t0) ? EPS : 0;
+    return ccanon(r);
+}
+
+// any a, b; canonical product
+LVM_HD uint64_t cmul(uint64_t a, uint64_t b) {
+    uint64_t lo, hi;
+    mul_wide(a, b, lo, hi);
+    return cred(lo, hi);
+}
+
+// 2^s mod p for s < 96: 2^s = 2^(s-64) * 2^64 = 2^(s-32) - 2^(s-64) for s >= 64
+constexpr uint64_t pow2_mod(int s) {
+    return s < 64 ? (1ull << s) : (1ull << (s - 32)) - (1ull << (s - 64));
+}
+
+// x * 2^S for canonical x and 0 <= S < 96: the shifted 128-bit value
+// reduced (for S <= 32 the high word is below 2^32, so hi_hi = 0); for
+// S >= 64 a product with the constant.
+template <int S>
+LVM_HD uint64_t cmul_pow2(uint64_t x) {
+    static_assert(S >= 0 && S < 96, "shift out of range");
+    if constexpr (S == 0) {
+        return x;
+    } else if constexpr (S <= 32) {
+        const uint64_t lo = x << S, hi = x >> (64 - S);
+        const uint64_t t1 = (hi << 32) - hi;
+        uint64_t r = lo + t1;
+        r += (r < lo) ? EPS : 0;
+        return ccanon(r);
+    } else if constexpr (S < 64) {
+        return cred(x << S, x >> (64 - S));
+    } else {
+        return cmul(x, pow2_mod(S));
+    }
+}
+
 // ---------------- Fp3 ----------------
 struct fp3 {
     uint64_t c0, c1, c2;

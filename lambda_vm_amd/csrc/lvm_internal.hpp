@@ -81,8 +81,9 @@ struct PassArgs {
     const uint64_t *tw_hi;         // w_{2^32}^(+-x*2^16), x < 2^16
     const uint64_t *shift_lo;      // forward tables for the coset shift
     const uint64_t *shift_hi;
-    const uint64_t *tw_r;          // w_256^(+-e), e < 128
+    const uint64_t *tw_r;          // w_256^(+-e), e < 256
     uint32_t cosets_per_block;     // cosets produced per block (grid.z = cosets / this)
+    uint32_t inverse;              // 1: inverse transform (tw_* hold inverse powers)
 };
 
 void ntt_inverse_rows(const uint64_t *in, uint32_t in_pitch, uint64_t *out, uint32_t out_pitch, uint32_t m,

@@ -18,6 +18,15 @@
 // exchanged through LDS, and the last log2(R2) stages run in registers on
 // R2 contiguous elements.
 //
+// Inside a pass the size-R DFT uses the four-step form: round 1 is a size-R1
+// DFT over t = t0 + R2*i followed by the inner twiddle w_R^(t0*k_a), round 2
+// a size-R2 DFT over t0. Both small DFTs (R1, R2 <= 16) are radix-2 DIF
+// whose twiddles are roots of unity of order <= 16, all of the form +-2^s
+// in this field (forward w_16 = 2^156, inverse w_16 = 2^36), so they are
+// shifts and a reduction, chosen at compile time; a negative sign swaps the
+// butterfly's subtraction. Values are kept canonical from the load on, which
+// makes each add and sub one conditional correction (field.hpp cadd/csub).
+//
 // Optional input remaps make the same kernel serve the coset LDE:
 //  - input rows may be read at bitrev(logical row) (the coefficient buffer
 //    produced by the inverse transform is in bit-reversed order);
@@ -30,9 +39,63 @@
 
 namespace lvm {
 
-// w_{2^32}^e via the two-level table
+// w_{2^32}^e via the two-level table (canonical)
 __device__ __forceinline__ uint64_t tw32(const uint64_t *lo, const uint64_t *hi, uint32_t e) {
-    return mul(lo[e & 0xFFFFu], hi[e >> 16]);
+    return cmul(lo[e & 0xFFFFu], hi[e >> 16]);
+}
+
+// log2 of w_R as a power of two (w_R = w_16^(16/R)), forward and inverse
+constexpr int log2_root(int R, bool inverse) { return ((inverse ? 36 : 156) * (16 / R)) % 192; }
+
+constexpr uint64_t cmod_mul(uint64_t a, uint64_t b) { return (uint64_t)(((unsigned __int128)a * b) % P); }
+constexpr uint64_t cmod_pow(uint64_t a, uint64_t e) {
+    uint64_t r = 1;
+    while (e) {
+        if (e & 1) r = cmod_mul(r, a);
+        a = cmod_mul(a, a);
+        e >>= 1;
+    }
+    return r;
+}
+// w_16 = w_{2^32}^(2^28) of the field's root of unity, and its inverse
+static_assert(cmod_pow(1753635133440165772ull, 1ull << 28) == cmod_pow(2, 156), "forward w_16 is 2^156");
+static_assert(cmod_mul(cmod_pow(2, 156), cmod_pow(2, 36)) == 1, "inverse w_16 is 2^36");
+
+// DIF butterfly (u, v) -> (u + v, (u - v) * 2^T); 2^T = -2^(T-96) for T >= 96
+template <int T>
+__device__ __forceinline__ void bfly_pow2(uint64_t &u, uint64_t &v) {
+    const uint64_t a = u, b = v;
+    u = cadd(a, b);
+    if constexpr (T >= 96)
+        v = cmul_pow2<T - 96>(csub(b, a));
+    else
+        v = cmul_pow2<T>(csub(a, b));
+}
+
+template <int R, int LOGW, int H, int I>
+__device__ __forceinline__ void dif_stage(uint64_t *x) {
+    if constexpr (I < R) {
+        if constexpr ((I & H) == 0) bfly_pow2<(LOGW * ((I & (H - 1)) * (R / (2 * H)))) % 192>(x[I], x[I + H]);
+        dif_stage<R, LOGW, H, I + 1>(x);
+    }
+}
+
+// In-register radix-2 DIF DFT of size R <= 16 with root 2^LOGW: natural
+// input order, bit-reversed output order, canonical values.
+template <int R, int LOGW, int H = R / 2>
+__device__ __forceinline__ void dif_small(uint64_t *x) {
+    if constexpr (H >= 1) {
+        dif_stage<R, LOGW, H, 0>(x);
+        dif_small<R, LOGW, H / 2>(x);
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void dft_small(uint64_t *x, bool inverse) {
+    if (inverse)
+        dif_small<R, log2_root(R, true)>(x);
+    else
+        dif_small<R, log2_root(R, false)>(x);
 }
 
 // Body of one pass for block (bx, by, bz) of the single-table grid
@@ -48,7 +111,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t bx, ui
     __shared__ uint64_t s_data[TILES][R * C];
     __shared__ uint64_t s_pre[TILES][R];
     __shared__ uint64_t s_post[TILES][R];
-    __shared__ uint64_t s_tw[R / 2 > 0 ? R / 2 : 1];
+    __shared__ uint64_t s_tw[R];
 
     const int tid = threadIdx.x;
     const int c = tid % C;
@@ -65,8 +128,8 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t bx, ui
     const uint64_t jlo = g & (S - 1);
     const uint64_t row0 = (blk << a.log_msub) + jlo;  // logical row of t = 0
 
-    // sub-transform twiddles w_R^e = w_256^(e*256/R)
-    for (int e = tid; e < R / 2; e += 256) s_tw[e] = a.tw_r[e << (8 - LOG_R)];
+    // inner twiddles w_R^e = w_256^(e*256/R), e < R
+    for (int e = tid; e < R; e += 256) s_tw[e] = a.tw_r[e << (8 - LOG_R)];
     // per-tile post-twiddle factors, shared by the C columns
     const int tin = tid % (C * TPC);
     for (int t = tin; t < R; t += C * TPC) {
@@ -111,7 +174,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t bx, ui
                 if (a.weights) pre = a.weights[lrow];
                 if (a.shift_log) {
                     uint32_t e = (uint32_t)(((uint64_t)coset * lrow) << (32 - a.shift_log));
-                    pre = mul(pre, tw32(a.shift_lo, a.shift_hi, e));
+                    pre = cmul(pre, tw32(a.shift_lo, a.shift_hi, e));
                 }
                 s_pre[tl][t] = pre;
             }
@@ -119,7 +182,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t bx, ui
     }
     __syncthreads();
 
-    // ---- round 1: stages h = R/2 .. R2 on elements t = t0 + R2*i ----
+    // ---- round 1: size-R1 DFT over t = t0 + R2*i, then w_R^(t0*k_a) ----
     if (tc < R2) {
         const int t0 = tc;
 #pragma unroll
@@ -129,52 +192,32 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t bx, ui
                 v = xin[i];
             else
                 v = load_in(i);
-            x[i] = pre_needed ? mul(v, s_pre[tl][t0 + R2 * i]) : v;
+            x[i] = pre_needed ? cmul(v, s_pre[tl][t0 + R2 * i]) : ccanon(v);
         }
+        dft_small<R1>(x, a.inverse);
+        // position i holds frequency k_a = bitrev_R1(i)
+        if constexpr (R2 > 1) {
 #pragma unroll
-        for (int H = R1 / 2; H >= 1; H >>= 1) {
-#pragma unroll
-            for (int i = 0; i < R1; i++) {
-                if ((i & H) == 0) {
-                    uint64_t u = x[i], v = x[i + H];
-                    x[i] = add(u, v);
-                    uint64_t d = sub(u, v);
-                    // j = t mod h = t0 + R2*(i mod H); exponent j * (R1 / 2H)
-                    const int e = (t0 + R2 * (i & (H - 1))) * (R1 / (2 * H));
-                    x[i + H] = e ? mul(d, s_tw[e]) : d;
-                }
-            }
+            for (int i = 1; i < R1; i++) x[i] = cmul(x[i], s_tw[t0 * (int)bitrev((uint32_t)i, LOG_R1)]);
         }
 #pragma unroll
         for (int i = 0; i < R1; i++) s_data[tl][(t0 + R2 * i) * C + c] = x[i];
     }
     __syncthreads();
 
-    // ---- round 2: stages h = R2/2 .. 1 on t = R2*t1 + i ----
+    // ---- round 2: size-R2 DFT over t0 at t = R2*t1 + t0 ----
     if (tc < R1) {
         const int t1 = tc;
 #pragma unroll
         for (int i = 0; i < R2; i++) x[i] = s_data[tl][(R2 * t1 + i) * C + c];
-#pragma unroll
-        for (int H = R2 / 2; H >= 1; H >>= 1) {
-#pragma unroll
-            for (int i = 0; i < R2; i++) {
-                if ((i & H) == 0) {
-                    uint64_t u = x[i], v = x[i + H];
-                    x[i] = add(u, v);
-                    uint64_t d = sub(u, v);
-                    const int e = (i & (H - 1)) * (R / (2 * H));
-                    x[i + H] = e ? mul(d, s_tw[e]) : d;
-                }
-            }
-        }
+        dft_small<R2>(x, a.inverse);
         if (c_ok) {
             uint64_t obase = 0;
             if (a.out_coset_rows) obase = (uint64_t)bitrev(coset, a.coset_bits) * a.out_coset_rows;
 #pragma unroll
             for (int i = 0; i < R2; i++) {
                 const int q = R2 * t1 + i;
-                uint64_t v = log_s ? mul(x[i], s_post[tl][q]) : x[i];
+                uint64_t v = log_s ? cmul(x[i], s_post[tl][q]) : x[i];
                 uint64_t orow = obase + row0 + ((uint64_t)q << log_s);
                 a.out[orow * a.out_pitch + col] = v;
             }
@@ -339,6 +382,7 @@ int lde_pass_list(const uint64_t *trace, uint32_t m, unsigned log_n, unsigned lo
         a.tw_lo = tb.inv_lo;
         a.tw_hi = tb.inv_hi;
         a.tw_r = tb.inv_r;
+        a.inverse = 1;
         a.cosets_per_block = 1;
         r_out[k] = r[p];
         cos_out[k] = 1;
@@ -402,6 +446,7 @@ static void run_passes(uint64_t *buf, uint32_t pitch, uint32_t m, size_t rows, u
         a.tw_lo = inverse ? tb.inv_lo : tb.fwd_lo;
         a.tw_hi = inverse ? tb.inv_hi : tb.fwd_hi;
         a.tw_r = inverse ? tb.inv_r : tb.fwd_r;
+        a.inverse = inverse ? 1u : 0u;
         launch_pass(r[p], a, 1, st, PASS_NAMES[inverse][p < 3 ? p : 3]);
         done += r[p];
     }
@@ -429,6 +474,7 @@ void ntt_inverse_rows(const uint64_t *in, uint32_t in_pitch, uint64_t *out, uint
     a.tw_lo = tb.inv_lo;
     a.tw_hi = tb.inv_hi;
     a.tw_r = tb.inv_r;
+    a.inverse = 1;
     launch_pass(r[0], a, 1, st, PASS_NAMES[1][0]);
     run_passes(out, out_pitch, m, 1ull << log_n, log_n, 1, r, np, tb, true, st);
 }
