@@ -195,6 +195,12 @@ def ensure_device(device: int = 0) -> None:
         _initialised = True
 
 
+def ensure_device_this_thread(device: int = 0) -> None:
+    """Make `device` current for the calling host thread (HIP's current
+    device is per thread); worker threads call this before their first call."""
+    call("lvm_init", device)
+
+
 def device_count() -> int:
     c = C.c_int(0)
     lib().lvm_device_count(C.byref(c))

@@ -85,7 +85,7 @@ int lde_core(Device *d, const uint64_t *d_trace, size_t n, size_t w, size_t blow
     const size_t N = n * blowup;
     if (log2u(N) > 32) return fail(LVM_ERR_ARG, "LDE domain exceeds the 2^32 two-adic subgroup");
     TRY(check_fault());
-    hipStream_t st = d->stream;
+    hipStream_t st = thread_stream(d);
     const size_t L = N / 2;
     DevBuf coef, lde, nodes;
     TRY(coef.alloc(n * w * 8, st));
@@ -130,7 +130,7 @@ int lde_from_host(const uint64_t *trace, size_t n, size_t w, size_t blowup, cons
     Device *d = dev_or_fail();
     if (!d) return LVM_ERR_DEVICE;
     if (!is_pow2(n) || n < 2) return fail(LVM_ERR_ARG, "trace rows must be a power of two >= 2");
-    hipStream_t st = d->stream;
+    hipStream_t st = thread_stream(d);
     DevBuf dt, dw;
     TRY(dt.alloc(n * w * 8, st));
     TRY(dw.alloc(n * 8, st));
@@ -173,7 +173,7 @@ const char *lvm_last_error(void) { return last_error(); }
 int lvm_sync(void) {
     Device *d = dev_or_fail();
     if (!d) return LVM_ERR_DEVICE;
-    LVM_HIP(hipStreamSynchronize(d->stream));
+    LVM_HIP(hipStreamSynchronize(thread_stream(d)));
     return LVM_OK;
 }
 
@@ -199,16 +199,16 @@ int lvm_dev_free(void *ptr) {
 int lvm_memcpy_htod(void *dst, const void *src, size_t bytes) {
     Device *d = dev_or_fail();
     if (!d) return LVM_ERR_DEVICE;
-    LVM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, d->stream));
-    LVM_HIP(hipStreamSynchronize(d->stream));
+    LVM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, thread_stream(d)));
+    LVM_HIP(hipStreamSynchronize(thread_stream(d)));
     return LVM_OK;
 }
 
 int lvm_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
     Device *d = dev_or_fail();
     if (!d) return LVM_ERR_DEVICE;
-    LVM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, d->stream));
-    LVM_HIP(hipStreamSynchronize(d->stream));
+    LVM_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, thread_stream(d)));
+    LVM_HIP(hipStreamSynchronize(thread_stream(d)));
     return LVM_OK;
 }
 
@@ -218,7 +218,7 @@ static int field_op(int op, const uint64_t *a, const uint64_t *b, uint64_t *out,
     if (n == 0) return LVM_OK;
     Device *d = dev_or_fail();
     if (!d) return LVM_ERR_DEVICE;
-    hipStream_t st = d->stream;
+    hipStream_t st = thread_stream(d);
     size_t words = ext3 ? 3 * n : n;
     DevBuf da, db, dout;
     TRY(da.alloc(words * 8, st));
@@ -249,7 +249,7 @@ static int ntt_host(const uint64_t *in, uint64_t *out, size_t n, size_t m, bool 
     Device *d = dev_or_fail();
     if (!d) return LVM_ERR_DEVICE;
     TRY(check_fault());
-    hipStream_t st = d->stream;
+    hipStream_t st = thread_stream(d);
     DevBuf a, b;
     TRY(a.alloc(n * m * 8, st));
     TRY(b.alloc(n * m * 8, st));
@@ -322,7 +322,7 @@ int lvm_lde_download(const lvm_lde *h, uint64_t *host_out) {
     if (!h || !host_out) return fail(LVM_ERR_ARG, "null argument");
     Device *d = dev_or_fail();
     if (!d) return LVM_ERR_DEVICE;
-    hipStream_t st = d->stream;
+    hipStream_t st = thread_stream(d);
     DevBuf tmp;
     TRY(tmp.alloc(h->N * h->w * 8, st));
     launch_gather_bitrev_rows(h->lde, tmp.as<uint64_t>(), h->N, log2u(h->N), h->w, st);
@@ -344,8 +344,8 @@ int lvm_lde_tree_nodes(const lvm_lde *h, uint8_t *nodes_out, size_t *num_nodes) 
     if (!nodes_out) return LVM_OK;
     Device *d = dev_or_fail();
     if (!d) return LVM_ERR_DEVICE;
-    LVM_HIP(hipMemcpyAsync(nodes_out, h->nodes, (2 * h->L - 1) * 32, hipMemcpyDeviceToHost, d->stream));
-    LVM_HIP(hipStreamSynchronize(d->stream));
+    LVM_HIP(hipMemcpyAsync(nodes_out, h->nodes, (2 * h->L - 1) * 32, hipMemcpyDeviceToHost, thread_stream(d)));
+    LVM_HIP(hipStreamSynchronize(thread_stream(d)));
     return LVM_OK;
 }
 
@@ -356,7 +356,7 @@ int lvm_lde_open(const lvm_lde *h, const uint64_t *positions, size_t count, uint
     if (count == 0) return LVM_OK;
     Device *d = dev_or_fail();
     if (!d) return LVM_ERR_DEVICE;
-    hipStream_t st = d->stream;
+    hipStream_t st = thread_stream(d);
     const uint32_t depth = log2u(h->L);
     DevBuf dp, dpaths, drows;
     TRY(dp.alloc(count * 8, st));
@@ -377,9 +377,9 @@ int lvm_coset_weights(size_t n, uint64_t offset, uint64_t *dev_out) {
     if (canon(offset) == 0) return fail(LVM_ERR_ARG, "coset offset must be non-zero");
     Device *d = dev_or_fail();
     if (!d) return LVM_ERR_DEVICE;
-    launch_coset_weights(n, offset, inv((uint64_t)n), dev_out, d->stream);
+    launch_coset_weights(n, offset, inv((uint64_t)n), dev_out, thread_stream(d));
     LVM_HIP(hipGetLastError());
-    LVM_HIP(hipStreamSynchronize(d->stream));
+    LVM_HIP(hipStreamSynchronize(thread_stream(d)));
     return LVM_OK;
 }
 
@@ -392,7 +392,7 @@ int lvm_composition_commit_d2(const uint64_t *dev_h, size_t N, uint64_t coset_of
     Device *d = dev_or_fail();
     if (!d) return LVM_ERR_DEVICE;
     TRY(check_fault());
-    hipStream_t st = d->stream;
+    hipStream_t st = thread_stream(d);
     const size_t half = N / 2;
     DevBuf halves, w;
     TRY(halves.alloc(half * 6 * 8, st));
@@ -413,9 +413,9 @@ int lvm_fill_splitmix_table(uint64_t *dev_out, size_t n, size_t m, uint64_t cfg,
     if (m && n > ((size_t)1 << 40) / m) return fail(LVM_ERR_ARG, "table too large");
     Device *d = dev_or_fail();
     if (!d) return LVM_ERR_DEVICE;
-    launch_fill_splitmix(dev_out, n, m, cfg, table, d->stream);
+    launch_fill_splitmix(dev_out, n, m, cfg, table, thread_stream(d));
     LVM_HIP(hipGetLastError());
-    LVM_HIP(hipStreamSynchronize(d->stream));
+    LVM_HIP(hipStreamSynchronize(thread_stream(d)));
     return LVM_OK;
 }
 
@@ -426,11 +426,11 @@ int lvm_dev_copy(void *dst, const void *src, size_t bytes) {
     Device *d = dev_or_fail();
     if (!d) return LVM_ERR_DEVICE;
     {
-        ProfScope ps("dev_copy", d->stream);
-        launch_copy16(dst, src, bytes / 16, d->stream);
+        ProfScope ps("dev_copy", thread_stream(d));
+        launch_copy16(dst, src, bytes / 16, thread_stream(d));
     }
     LVM_HIP(hipGetLastError());
-    LVM_HIP(hipStreamSynchronize(d->stream));
+    LVM_HIP(hipStreamSynchronize(thread_stream(d)));
     return LVM_OK;
 }
 
@@ -439,7 +439,7 @@ int lvm_build_merkle_tree(const uint8_t *leaves, size_t num_leaves, uint8_t *nod
     Device *d = dev_or_fail();
     if (!d) return LVM_ERR_DEVICE;
     TRY(check_fault());
-    hipStream_t st = d->stream;
+    hipStream_t st = thread_stream(d);
     const size_t L = padded_leaves(num_leaves);
     std::vector<uint8_t> padded(L * 32);
     std::memcpy(padded.data(), leaves, num_leaves * 32);

@@ -1,4 +1,4 @@
-// Library state: per-device stream and twiddle tables, thread-local error
+// Library state: per-device twiddle tables, per-thread streams, thread-local error
 // strings, test-only fault injection, and the kernel event profiler.
 #include <atomic>
 #include <mutex>
@@ -111,10 +111,44 @@ Device *device_state() {
     return d;
 }
 
-hipStream_t lib_stream() {
-    Device *d = device_state();
-    return d ? d->stream : nullptr;
+namespace {
+// Streams this thread created (one per device); destroyed when the thread ends.
+struct ThreadStreams {
+    std::vector<hipStream_t> s;
+    std::vector<bool> owned;
+    ~ThreadStreams() {
+        for (size_t i = 0; i < s.size(); i++)
+            if (s[i] && owned[i]) {
+                hipStreamSynchronize(s[i]);
+                hipStreamDestroy(s[i]);
+            }
+    }
+};
+thread_local ThreadStreams t_streams;
+}  // namespace
+
+hipStream_t thread_stream(Device *d) {
+    if (!d) return nullptr;
+    auto &ts = t_streams;
+    if ((int)ts.s.size() <= d->id) {
+        ts.s.resize(d->id + 1, nullptr);
+        ts.owned.resize(d->id + 1, false);
+    }
+    if (!ts.s[d->id]) {
+        bool expected = false;
+        if (d->stream_claimed.compare_exchange_strong(expected, true)) {
+            ts.s[d->id] = d->stream;  // the first thread keeps the device's stream
+        } else {
+            hipStream_t s = nullptr;
+            if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return d->stream;
+            ts.s[d->id] = s;
+            ts.owned[d->id] = true;
+        }
+    }
+    return ts.s[d->id];
 }
+
+hipStream_t lib_stream() { return thread_stream(device_state()); }
 
 // ---------------- profiler ----------------
 struct ProfRec {

@@ -155,8 +155,8 @@ int lvm_fri_new(const uint64_t *evals, size_t len, uint64_t coset_offset, int ev
         return fail(LVM_ERR_NOMEM, "FRI codeword allocation failed");
     }
     hipError_t e = hipMemcpyAsync(f->cur, evals, len * 24, evals_on_device ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
-                                  d->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(d->stream);
+                                  thread_stream(d));
+    if (e == hipSuccess) e = hipStreamSynchronize(thread_stream(d));
     if (e != hipSuccess) {
         free_fri(f);
         return fail(LVM_ERR_DEVICE, std::string("FRI upload: ") + hipGetErrorString(e));
@@ -172,7 +172,7 @@ static int fri_fold_impl(lvm_fri *f, const uint64_t zeta[3], bool commit, uint8_
     Device *d = device_state();
     if (!d) return LVM_ERR_DEVICE;
     if (take_injected_fault()) return fail(LVM_ERR_INJECTED, "injected fault");
-    hipStream_t st = d->stream;
+    hipStream_t st = thread_stream(d);
     const uint64_t half = f->len / 2;
     uint64_t *out = nullptr;
     LVM_HIP(lvm_malloc(&out, half * 24));
@@ -234,8 +234,8 @@ int lvm_fri_current(const lvm_fri *f, uint64_t *host_out, size_t *len, uint64_t 
     if (!host_out) return LVM_OK;
     Device *d = device_state();
     if (!d) return LVM_ERR_DEVICE;
-    LVM_HIP(hipMemcpyAsync(host_out, f->cur, f->len * 24, hipMemcpyDeviceToHost, d->stream));
-    LVM_HIP(hipStreamSynchronize(d->stream));
+    LVM_HIP(hipMemcpyAsync(host_out, f->cur, f->len * 24, hipMemcpyDeviceToHost, thread_stream(d)));
+    LVM_HIP(hipStreamSynchronize(thread_stream(d)));
     return LVM_OK;
 }
 
@@ -251,9 +251,9 @@ int lvm_fri_layer(const lvm_fri *f, size_t layer, uint64_t *evals_out, uint8_t *
     if (len) *len = l.len;
     Device *d = device_state();
     if (!d) return LVM_ERR_DEVICE;
-    if (evals_out) LVM_HIP(hipMemcpyAsync(evals_out, l.evals, l.len * 24, hipMemcpyDeviceToHost, d->stream));
-    if (nodes_out) LVM_HIP(hipMemcpyAsync(nodes_out, l.nodes, (2 * l.L - 1) * 32, hipMemcpyDeviceToHost, d->stream));
-    LVM_HIP(hipStreamSynchronize(d->stream));
+    if (evals_out) LVM_HIP(hipMemcpyAsync(evals_out, l.evals, l.len * 24, hipMemcpyDeviceToHost, thread_stream(d)));
+    if (nodes_out) LVM_HIP(hipMemcpyAsync(nodes_out, l.nodes, (2 * l.L - 1) * 32, hipMemcpyDeviceToHost, thread_stream(d)));
+    LVM_HIP(hipStreamSynchronize(thread_stream(d)));
     return LVM_OK;
 }
 
@@ -265,7 +265,7 @@ int lvm_fri_query(const lvm_fri *f, const uint64_t *iotas, size_t count, uint64_
         if (iotas[i] >= len0 / 2) return fail(LVM_ERR_ARG, "query index out of range");
     Device *d = device_state();
     if (!d) return LVM_ERR_DEVICE;
-    hipStream_t st = d->stream;
+    hipStream_t st = thread_stream(d);
     // per layer: count sym evals (3 u64) and count paths (depth_l x 32 bytes)
     size_t path_words = 0;
     for (auto &l : f->layers) path_words += count * log2u(l.L) * 4;
@@ -296,7 +296,7 @@ int lvm_fri_query(const lvm_fri *f, const uint64_t *iotas, size_t count, uint64_
 int lvm_fri_release(lvm_fri *f) {
     if (!f) return LVM_OK;
     Device *d = device_state();
-    if (d) hipStreamSynchronize(d->stream);
+    if (d) hipStreamSynchronize(thread_stream(d));
     free_fri(f);
     return LVM_OK;
 }

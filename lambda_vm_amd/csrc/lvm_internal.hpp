@@ -1,6 +1,7 @@
 // Internal declarations shared by the HIP translation units of liblvm_gpu.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <atomic>
 #include <stdint.h>
 #include <stddef.h>
 #include <string>
@@ -37,10 +38,17 @@ int fail(int code, const std::string &msg);
 // per-device library state
 struct Device {
     int id = -1;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // the first host thread's library stream
+    std::atomic<bool> stream_claimed{false};
     Tables tb{};
 };
 Device *device_state();  // current device's state, initialised on first use; null on failure
+// The calling host thread's library stream on device d: every thread that
+// calls the library gets its own (default priority, non-blocking), so tables
+// proved from several threads overlap on the GPU. Work handed from one
+// thread to another is complete when the producing call returns (every
+// entry point that yields a handle or host data synchronises its stream).
+hipStream_t thread_stream(Device *d);
 
 // fault injection for the fallback tests (never set in production)
 bool take_injected_fault();

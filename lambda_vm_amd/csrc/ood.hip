@@ -290,11 +290,11 @@ int lvm_batch_inverse_ext3_dev(const uint64_t *dev_in, uint64_t *dev_out, size_t
     if (!d) return LVM_ERR_DEVICE;
     if (take_injected_fault()) return fail(LVM_ERR_INJECTED, "injected fault");
     {
-        ProfScope ps("batch_inverse", d->stream);
-        launch_batch_inverse(dev_in, dev_out, n, d->stream);
+        ProfScope ps("batch_inverse", thread_stream(d));
+        launch_batch_inverse(dev_in, dev_out, n, thread_stream(d));
     }
     LVM_HIP(hipGetLastError());
-    LVM_HIP(hipStreamSynchronize(d->stream));
+    LVM_HIP(hipStreamSynchronize(thread_stream(d)));
     return LVM_OK;
 }
 
@@ -307,11 +307,11 @@ int lvm_batch_inverse_ext3(const uint64_t *in, uint64_t *out, size_t n) {
     uint64_t *di, *dout;
     TRYS(s.get(&di, n * 24));
     TRYS(s.get(&dout, n * 24));
-    LVM_HIP(hipMemcpyAsync(di, in, n * 24, hipMemcpyHostToDevice, d->stream));
+    LVM_HIP(hipMemcpyAsync(di, in, n * 24, hipMemcpyHostToDevice, thread_stream(d)));
     int rc = lvm_batch_inverse_ext3_dev(di, dout, n);
     if (rc) return rc;
-    LVM_HIP(hipMemcpyAsync(out, dout, n * 24, hipMemcpyDeviceToHost, d->stream));
-    LVM_HIP(hipStreamSynchronize(d->stream));
+    LVM_HIP(hipMemcpyAsync(out, dout, n * 24, hipMemcpyDeviceToHost, thread_stream(d)));
+    LVM_HIP(hipStreamSynchronize(thread_stream(d)));
     return LVM_OK;
 }
 
@@ -321,7 +321,7 @@ int lvm_barycentric_base_multi(const lvm_lde *h, uint64_t coset_offset, const ui
     Device *d = device_state();
     if (!d) return LVM_ERR_DEVICE;
     if (take_injected_fault()) return fail(LVM_ERR_INJECTED, "injected fault");
-    hipStream_t st = d->stream;
+    hipStream_t st = thread_stream(d);
     const uint64_t n = h->n, w = h->w;
     const unsigned logN = log2u(h->N);
     Scratch s;
@@ -387,7 +387,7 @@ int lvm_deep_composition(const lvm_lde *main, const lvm_lde *aux, const lvm_lde 
     Device *d = device_state();
     if (!d) return LVM_ERR_DEVICE;
     if (take_injected_fault()) return fail(LVM_ERR_INJECTED, "injected fault");
-    hipStream_t st = d->stream;
+    hipStream_t st = thread_stream(d);
     const uint64_t N = main->N, wm = main->w, wa = aux ? aux->w / 3 : 0, np = parts ? parts->w / 3 : 0;
     const uint64_t cols = wm + wa;
     const uint32_t npoles = (uint32_t)K + (parts ? 1 : 0);

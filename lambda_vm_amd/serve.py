@@ -12,9 +12,13 @@ same ideas serve a stream of jobs:
     previous tick in one batched launch sequence (lvm_batch_*), so jobs that
     arrive while others are on the GPU share launches with each other;
   - committed jobs then advance through the remaining rounds (OOD, DEEP,
-    FRI, queries) a table at a time, a few tables per tick, oldest job first
-    (or one table of every job per tick), and free each table's LDE, tree and
-    FRI layers as soon as its rounds finish;
+    FRI, queries) a few tables per tick, oldest job first (or one table of
+    every job per tick), and free each table's LDE, tree and FRI layers as
+    soon as its rounds finish;
+  - the GPU executor proves a tick's tables on a small pool of host threads,
+    each with its own library stream (the reference drives tables from
+    several OS threads with per-table streams, prover.rs:620-638, :749-782),
+    and commits the newly admitted jobs concurrently with them;
   - per job: latency from arrival to its final digest; per run: p50/p99
     latency, jobs/s and LDE cells/s.
 A job's result depends only on its own inputs and transcript: the digest of
@@ -27,6 +31,7 @@ import math
 import random
 import time
 from collections import deque
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -90,9 +95,14 @@ class JobState:
     hbm: int = 0
     handles: list = field(default_factory=list)
     transcripts: list = field(default_factory=list)
-    next_table: int = 0
-    digests: list = field(default_factory=list)
+    next_table: int = 0   # next table to hand out
+    tables_done: int = 0
+    digests: list = field(default_factory=list)  # per table, in table order
     roots: list = field(default_factory=list)
+
+    def __post_init__(self):
+        if not self.digests:
+            self.digests = [None] * len(self.spec.tables)
 
     def digest(self) -> str:
         h = hashlib.sha256()
@@ -120,11 +130,33 @@ def proof_digest(proof) -> bytes:
 class GpuExecutor:
     """Runs the service's work on the GPU through the C ABI."""
 
-    def __init__(self, offset: int = 3, fri_k: int = 6, num_queries: int = 30):
+    def __init__(self, offset: int = 3, fri_k: int = 6, num_queries: int = 30, workers: int = 4, device: int = 0):
         from . import _native, backend, pipeline
-        _native.ensure_device(0)
+        _native.ensure_device(device)
         self.B, self.pipeline = backend, pipeline
         self.offset, self.fri_k, self.num_queries = offset, fri_k, num_queries
+        self.workers = workers
+        self.pool = (ThreadPoolExecutor(max_workers=workers, initializer=_native.ensure_device_this_thread,
+                                        initargs=(device,)) if workers > 1 else None)
+
+    def run_tick(self, admitted: list, tasks: list) -> list:
+        """Commit `admitted` and prove `tasks` [(job, table)] concurrently;
+        the digests of the tasks in order."""
+        if self.pool is None:
+            if admitted:
+                self.commit(admitted)
+            return [self.prove_table(js, ti) for js, ti in tasks]
+        fc = self.pool.submit(self.commit, admitted) if admitted else None
+        fs = [self.pool.submit(self.prove_table, js, ti) for js, ti in tasks]
+        out = [f.result() for f in fs]
+        if fc is not None:
+            fc.result()
+        return out
+
+    def close(self) -> None:
+        if self.pool is not None:
+            self.pool.shutdown()
+            self.pool = None
 
     def commit(self, jobs: list) -> None:
         """One batched commitment over every table of the given jobs."""
@@ -169,7 +201,7 @@ class GpuExecutor:
 
 class ProvingService:
     def __init__(self, hbm_budget: int, executor=None, clock=time.perf_counter, policy: str = "oldest",
-                 tables_per_tick: int = 4):
+                 tables_per_tick: int = 8):
         """policy "oldest": each tick proves up to tables_per_tick tables, oldest
         admitted job first (jobs finish in admission order, which keeps the
         latency of early jobs low); "round_robin": one table of every running
@@ -205,35 +237,43 @@ class ProvingService:
         return admitted
 
     def tick(self) -> bool:
-        """Admit what fits, commit it in one batch, advance every committed job
-        by one table; returns whether any work was done."""
+        """Admit what fits and commit it in one batch while the committed jobs
+        advance by up to tables_per_tick tables (or one table of every job);
+        returns whether any work was done."""
         self.ticks += 1
         admitted = self._admit(self.clock())
+        tasks = []
+        if self.policy == "round_robin":
+            for js in self.running:
+                if js.next_table < len(js.spec.tables):
+                    tasks.append((js, js.next_table))
+                    js.next_table += 1
+        else:
+            left = self.tables_per_tick
+            for js in self.running:
+                while left and js.next_table < len(js.spec.tables):
+                    tasks.append((js, js.next_table))
+                    js.next_table += 1
+                    left -= 1
+        if hasattr(self.ex, "run_tick"):
+            digests = self.ex.run_tick(admitted, tasks)
+        else:
+            if admitted:
+                self.ex.commit(admitted)
+            digests = [self.ex.prove_table(js, ti) for js, ti in tasks]
         if admitted:
-            self.ex.commit(admitted)
             self.batch_sizes.append(len(admitted))
             self.running.extend(admitted)
-        worked = bool(admitted)
-        if self.policy == "round_robin":
-            work = [js for js in self.running if js not in admitted]
-        else:
-            work, left = [], self.tables_per_tick
-            for js in self.running:
-                if js in admitted or left == 0:
-                    continue
-                k = min(left, len(js.spec.tables) - js.next_table)
-                work += [js] * k
-                left -= k
-        for js in work:
-            js.digests.append(self.ex.prove_table(js, js.next_table))
-            js.next_table += 1
-            worked = True
-            if js.next_table == len(js.spec.tables):
-                js.finished = self.clock()
+        now = self.clock()
+        for (js, ti), d in zip(tasks, digests):
+            js.digests[ti] = d
+            js.tables_done += 1
+            if js.tables_done == len(js.spec.tables):
+                js.finished = now
                 self.running.remove(js)
                 self.in_use -= js.hbm
                 self.done.append(js)
-        return worked
+        return bool(admitted) or bool(tasks)
 
     def run(self, arrivals: list, time_scale: float = 1.0) -> dict:
         """Serve `arrivals` [(t, spec)] in real time (t in seconds after start,

@@ -1,6 +1,7 @@
 """Proving service on the GPU: each job's result digest is the same alone and
-in a mix of concurrently admitted jobs, and its committed roots equal the
-oracle's commitment of the same synthetic tables."""
+in a mix of concurrently admitted jobs, the same with tables proved on one
+host thread and on a pool of threads (one library stream each), and its
+committed roots equal the oracle's commitment of the same synthetic tables."""
 import numpy as np
 import pytest
 
@@ -32,6 +33,19 @@ def test_results_same_alone_and_in_mix(gpu):
         js = alone.done[0]
         assert js.digest() == mixed[spec.job_id][0], spec.job_id
         assert js.roots == mixed[spec.job_id][1]
+
+
+def test_thread_pool_matches_single_thread(gpu):
+    jobs = _jobs()
+    digests = {}
+    for workers in (1, 4):
+        ex = serve.GpuExecutor(offset=3, fri_k=3, num_queries=8, workers=workers)
+        svc = serve.ProvingService(1 << 40, ex, tables_per_tick=6)
+        svc.run([(0.0, spec) for spec in jobs])
+        ex.close()
+        digests[workers] = {js.spec.job_id: js.digest() for js in svc.done}
+        assert all(all(d is not None for d in js.digests) for js in svc.done)
+    assert digests[1] == digests[4]
 
 
 def test_service_roots_match_oracle(gpu, ora):
