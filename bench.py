@@ -121,6 +121,8 @@ def main() -> int:
     ap.add_argument("--rate", type=float, default=20.0, help="serve: job arrivals per second (Poisson)")
     ap.add_argument("--jobs", type=int, default=120, help="serve: number of jobs")
     ap.add_argument("--budget-gib", type=float, default=48.0, help="serve: HBM budget for jobs in flight")
+    ap.add_argument("--workers", type=int, default=4, help="serve: host threads proving tables concurrently")
+    ap.add_argument("--tables-per-tick", type=int, default=8, help="serve: tables advanced per scheduler tick")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -308,13 +310,15 @@ def run_serve(args) -> dict:
 
     arrivals = serve.synth_jobs(args.rate, args.jobs, seed=11)
     budget = int(args.budget_gib * (1 << 30))
-    ex = serve.GpuExecutor(offset=3, fri_k=args.fri_k, num_queries=args.queries)
-    # warm-up: one small job through the same path (code objects, pools)
-    warm = serve.ProvingService(budget, ex)
-    warm.run([(0.0, serve.JobSpec(10 ** 6, [(1 << 10, 8)] * 4))])
-    log(f"[bench] serving {args.jobs} jobs at {args.rate} jobs/s, budget {args.budget_gib} GiB")
-    svc = serve.ProvingService(budget, ex)
+    ex = serve.GpuExecutor(offset=3, fri_k=args.fri_k, num_queries=args.queries, workers=args.workers)
+    # warm-up: one small job through the same path (code objects, pools, every worker's stream)
+    warm = serve.ProvingService(budget, ex, tables_per_tick=args.tables_per_tick)
+    warm.run([(0.0, serve.JobSpec(10 ** 6, [(1 << 10, 8)] * max(4, args.workers)))])
+    log(f"[bench] serving {args.jobs} jobs at {args.rate} jobs/s, budget {args.budget_gib} GiB, "
+        f"{args.workers} proving threads, {args.tables_per_tick} tables per tick")
+    svc = serve.ProvingService(budget, ex, tables_per_tick=args.tables_per_tick)
     st = svc.run(arrivals)
+    ex.close()
     out = {
         "metric": "batched LDE+Keccak-Merkle Gcells/s vs per-table launch; p50/p99 proof latency",
         "value": round(st["gcells_per_s"], 4),
@@ -340,6 +344,8 @@ def run_serve(args) -> dict:
             "peak_hbm_gib": round(st["peak_hbm_bytes"] / (1 << 30), 3),
             "mean_jobs_per_commit_batch": round(st["mean_jobs_per_commit_batch"], 3),
             "lde_cells": st["lde_cells"],
+            "workers": args.workers,
+            "tables_per_tick": args.tables_per_tick,
         },
     }
     return out

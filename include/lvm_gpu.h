@@ -49,6 +49,12 @@ int lvm_init(int device);
 int lvm_device_count(int *count);
 /* Thread-local message of the last failing call. */
 const char *lvm_last_error(void);
+/* Device buffers released by the library are kept in size classes and
+ * reused (a freed block is reused on the stream that released it at once,
+ * on other streams once that release has completed); this hands every
+ * cached block of the current device back to the driver pool. No counterpart
+ * in the reference (its buffers are freed on drop, math-cuda device.rs). */
+int lvm_trim(void);
 /* Wait for all work the library queued on the current device. */
 int lvm_sync(void);
 /* Test-only fault injection (crypto/math-cuda/src/faults.rs:33-51): the

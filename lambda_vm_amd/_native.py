@@ -36,6 +36,7 @@ SIGNATURES = {
     "lvm_init": (C.c_int, [C.c_int]),
     "lvm_device_count": (C.c_int, [C.POINTER(C.c_int)]),
     "lvm_last_error": (C.c_char_p, []),
+    "lvm_trim": (C.c_int, []),
     "lvm_sync": (C.c_int, []),
     "lvm_inject_fault_after": (None, [C.c_long]),
     "lvm_dev_alloc": (C.c_int, [vpp, C.c_size_t]),

@@ -712,3 +712,8 @@ def prof_query(name: str | None = None):
 
 def sync() -> None:
     N.call("lvm_sync")
+
+
+def trim() -> None:
+    """Hand the library's cached device buffers back to the driver pool."""
+    N.call("lvm_trim")

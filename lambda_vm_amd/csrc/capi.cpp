@@ -35,12 +35,12 @@ struct DevBuf {
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     ~DevBuf() {
-        if (p) hipFreeAsync(p, st);
+        if (p) lvm_free(p);
     }
     int alloc(size_t bytes, hipStream_t s) {
         st = s;
         if (bytes == 0) bytes = 8;
-        hipError_t e = hipMallocAsync(&p, bytes, s);
+        hipError_t e = lvm_malloc_bytes(&p, bytes);
         if (e != hipSuccess) {
             p = nullptr;
             return fail(LVM_ERR_NOMEM, std::string("device allocation of ") + std::to_string(bytes) +
@@ -169,6 +169,13 @@ int lvm_device_count(int *count) {
 }
 
 const char *lvm_last_error(void) { return last_error(); }
+
+int lvm_trim(void) {
+    Device *d = dev_or_fail();
+    if (!d) return LVM_ERR_DEVICE;
+    lvm_release_cached();
+    return LVM_OK;
+}
 
 int lvm_sync(void) {
     Device *d = dev_or_fail();

@@ -1,8 +1,10 @@
 // Library state: per-device twiddle tables, per-thread streams, thread-local error
 // strings, test-only fault injection, and the kernel event profiler.
 #include <atomic>
+#include <memory>
 #include <mutex>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "lvm_internal.hpp"
@@ -149,6 +151,122 @@ hipStream_t thread_stream(Device *d) {
 }
 
 hipStream_t lib_stream() { return thread_stream(device_state()); }
+
+// ---------------- device allocation cache ----------------
+namespace {
+struct CachedBlock {
+    void *p;
+    hipStream_t st;  // stream that released it
+    hipEvent_t ev;   // recorded at the release on st
+};
+struct AllocCache {
+    std::mutex mu;
+    std::unordered_map<void *, size_t> live;                   // block -> its class size
+    std::unordered_map<size_t, std::vector<CachedBlock>> idle;  // class size -> released blocks
+    std::vector<hipEvent_t> spare_events;
+};
+std::mutex g_cache_mu;
+std::vector<std::unique_ptr<AllocCache>> g_caches;  // per device id
+
+AllocCache *cache_for(int id) {
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    if ((int)g_caches.size() <= id) g_caches.resize(id + 1);
+    if (!g_caches[id]) g_caches[id] = std::make_unique<AllocCache>();
+    return g_caches[id].get();
+}
+
+// size classes: powers of two up to 1 MiB, then four classes per octave
+size_t class_of(size_t b) {
+    if (b < 256) b = 256;
+    size_t c = 256;
+    while (c < b) c <<= 1;
+    if (c <= (size_t(1) << 20)) return c;
+    const size_t step = c / 8;  // c/2 < b <= c: round up to a multiple of c/8
+    return (b + step - 1) / step * step;
+}
+}  // namespace
+
+hipError_t lvm_malloc_bytes(void **p, size_t bytes) {
+    Device *d = device_state();
+    if (!d) return hipErrorNoDevice;
+    hipStream_t st = thread_stream(d);
+    AllocCache *c = cache_for(d->id);
+    const size_t cls = class_of(bytes);
+    {
+        std::lock_guard<std::mutex> lk(c->mu);
+        auto it = c->idle.find(cls);
+        if (it != c->idle.end()) {
+            auto &v = it->second;
+            for (size_t i = v.size(); i-- > 0;) {
+                if (v[i].st == st || hipEventQuery(v[i].ev) == hipSuccess) {
+                    *p = v[i].p;
+                    c->spare_events.push_back(v[i].ev);
+                    v[i] = v.back();
+                    v.pop_back();
+                    c->live[*p] = cls;
+                    return hipSuccess;
+                }
+            }
+        }
+    }
+    hipError_t e = hipMallocAsync(p, cls, st);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        lvm_release_cached();
+        e = hipMallocAsync(p, cls, st);
+    }
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return e;
+    }
+    std::lock_guard<std::mutex> lk(c->mu);
+    c->live[*p] = cls;
+    return hipSuccess;
+}
+
+hipError_t lvm_free(void *p) {
+    if (!p) return hipSuccess;
+    Device *d = device_state();
+    if (!d) return hipErrorNoDevice;
+    hipStream_t st = thread_stream(d);
+    AllocCache *c = cache_for(d->id);
+    std::lock_guard<std::mutex> lk(c->mu);
+    auto it = c->live.find(p);
+    if (it == c->live.end()) return hipFreeAsync(p, st);
+    const size_t cls = it->second;
+    c->live.erase(it);
+    hipEvent_t ev = nullptr;
+    if (!c->spare_events.empty()) {
+        ev = c->spare_events.back();
+        c->spare_events.pop_back();
+    } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        return hipFreeAsync(p, st);
+    }
+    hipError_t e = hipEventRecord(ev, st);
+    if (e != hipSuccess) {
+        hipEventDestroy(ev);
+        return hipFreeAsync(p, st);
+    }
+    c->idle[cls].push_back(CachedBlock{p, st, ev});
+    return hipSuccess;
+}
+
+// Return every cached block of the current device to the pool.
+void lvm_release_cached() {
+    Device *d = device_state();
+    if (!d) return;
+    hipStream_t st = thread_stream(d);
+    AllocCache *c = cache_for(d->id);
+    std::lock_guard<std::mutex> lk(c->mu);
+    for (auto &kv : c->idle)
+        for (auto &b : kv.second) {
+            hipEventSynchronize(b.ev);
+            hipFreeAsync(b.p, st);
+            c->spare_events.push_back(b.ev);
+        }
+    c->idle.clear();
+    hipStreamSynchronize(st);
+}
 
 // ---------------- profiler ----------------
 struct ProfRec {
