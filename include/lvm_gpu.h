@@ -358,6 +358,11 @@ int lvm_transcript_release(lvm_transcript *t);
 int lvm_transcript_append_bytes(lvm_transcript *t, const uint8_t *data, size_t len);
 int lvm_transcript_append_base(lvm_transcript *t, uint64_t v);
 int lvm_transcript_append_fp3(lvm_transcript *t, const uint64_t v[3]);
+/* count Fp3 elements appended in order (the same bytes as count single appends). */
+int lvm_transcript_append_fp3_many(lvm_transcript *t, const uint64_t *v, size_t count);
+/* Host helper: out[i] = base^i for i < count (canonical Fp3), the coefficient
+ * powers of crypto/stark/src/prover.rs:2159-2182 (gamma) and :4453-4474 (beta). */
+int lvm_fp3_powers(const uint64_t base[3], size_t count, uint64_t *out);
 int lvm_transcript_sample(lvm_transcript *t, uint8_t out[32]);
 int lvm_transcript_state(const lvm_transcript *t, uint8_t out[32]);
 int lvm_transcript_sample_base(lvm_transcript *t, uint64_t *out);

@@ -93,6 +93,8 @@ SIGNATURES = {
     "lvm_transcript_clone": (C.c_int, [C.c_void_p, vpp]),
     "lvm_transcript_release": (C.c_int, [C.c_void_p]),
     "lvm_transcript_append_bytes": (C.c_int, [C.c_void_p, u8p, C.c_size_t]),
+    "lvm_transcript_append_fp3_many": (C.c_int, [C.c_void_p, u64p, C.c_size_t]),
+    "lvm_fp3_powers": (C.c_int, [u64p, C.c_size_t, u64p]),
     "lvm_transcript_append_base": (C.c_int, [C.c_void_p, C.c_uint64]),
     "lvm_transcript_append_fp3": (C.c_int, [C.c_void_p, u64p]),
     "lvm_transcript_sample": (C.c_int, [C.c_void_p, u8p]),

@@ -573,6 +573,11 @@ class Transcript:
         a = _u64(v)
         N.call("lvm_transcript_append_fp3", self.h, _p64(a))
 
+    def append_fp3_many(self, vals) -> None:
+        """Append Fp3 elements in order (rows of a (count, 3) array)."""
+        a = np.ascontiguousarray(_u64(vals).reshape(-1, 3))
+        N.call("lvm_transcript_append_fp3_many", self.h, _p64(a), a.shape[0])
+
     def sample(self) -> bytes:
         out = np.zeros(32, dtype=np.uint8)
         N.call("lvm_transcript_sample", self.h, _p8(out))
@@ -712,6 +717,14 @@ def prof_query(name: str | None = None):
 
 def sync() -> None:
     N.call("lvm_sync")
+
+
+def fp3_powers(base, count: int) -> np.ndarray:
+    """(count, 3) canonical powers base^0 .. base^(count-1)."""
+    b = _u64(base).reshape(3).copy()
+    out = np.zeros((count, 3), dtype=np.uint64)
+    N.call("lvm_fp3_powers", _p64(b), count, _p64(out))
+    return out
 
 
 def trim() -> None:

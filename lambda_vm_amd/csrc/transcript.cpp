@@ -152,6 +152,26 @@ int lvm_transcript_append_fp3(lvm_transcript *t, const uint64_t v[3]) {
     return lvm_transcript_append_bytes(t, b, 24);
 }
 
+int lvm_transcript_append_fp3_many(lvm_transcript *t, const uint64_t *v, size_t count) {
+    if (!t || (!v && count)) return fail(LVM_ERR_ARG, "null argument");
+    std::vector<uint8_t> b(24 * count);
+    for (size_t k = 0; k < 3 * count; k++) be64(canon(v[k]), b.data() + 8 * k);
+    return lvm_transcript_append_bytes(t, b.data(), b.size());
+}
+
+int lvm_fp3_powers(const uint64_t base[3], size_t count, uint64_t *out) {
+    if (!base || (!out && count)) return fail(LVM_ERR_ARG, "null argument");
+    const fp3 g{base[0], base[1], base[2]};
+    fp3 cur{1, 0, 0};
+    for (size_t i = 0; i < count; i++) {
+        out[3 * i] = canon(cur.c0);
+        out[3 * i + 1] = canon(cur.c1);
+        out[3 * i + 2] = canon(cur.c2);
+        cur = f3_mul(cur, g);
+    }
+    return LVM_OK;
+}
+
 int lvm_transcript_sample(lvm_transcript *t, uint8_t out[32]) {
     if (!t || !out) return fail(LVM_ERR_ARG, "null argument");
     uint8_t d[32];

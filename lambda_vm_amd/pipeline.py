@@ -67,12 +67,10 @@ def prove_committed_table(h: B.GpuLde, t: B.Transcript, n: int, m: int, blowup: 
     points = [z, F.f3_scale(z, wn)]
     K = len(points)
     ood = B.barycentric_base_multi(h, offset, np.array(points, dtype=np.uint64))  # (K, m, 3)
-    for k in range(K):
-        for c in range(m):
-            t.append_fp3(ood[k, c])
+    t.append_fp3_many(ood.reshape(K * m, 3))  # point-major, columns in order
     # R4
     gamma = [int(v) for v in t.sample_fp3()]
-    gammas = np.array(F.powers(gamma, m * K), dtype=np.uint64)  # index j*K + k
+    gammas = B.fp3_powers(gamma, m * K)  # index j*K + k
     trace_ood = np.ascontiguousarray(ood.transpose(1, 0, 2).reshape(m * K, 3))
     deep = B.deep_composition(h, offset, np.array(points, dtype=np.uint64), trace_ood, gammas)
     fri = B.GpuFri(None, offset, on_device=deep, length=N, adopt=True)

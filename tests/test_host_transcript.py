@@ -49,3 +49,27 @@ def test_matches_oracle_on_mixed_sequences(ora):
             assert a.state() == b.state()
         c = a.clone()
         assert c.sample() == a.sample()
+
+
+def test_append_fp3_many_equals_single_appends():
+    rng = np.random.default_rng(5)
+    P = 0xFFFFFFFF00000001
+    for count in (0, 1, 7, 130):
+        vals = rng.integers(0, 2**63, (count, 3), dtype=np.uint64) * np.uint64(2)  # some at or above p
+        vals[: min(count, 2)] = np.uint64(P)  # non-canonical limbs are canonicalised
+        a, b = backend.Transcript(b"x"), backend.Transcript(b"x")
+        a.append_fp3_many(vals)
+        for v in vals:
+            b.append_fp3(v)
+        assert a.sample() == b.sample()
+
+
+def test_fp3_powers_match_schoolbook():
+    from lambda_vm_amd import field as F
+    P = 0xFFFFFFFF00000001
+    for g in ([0, 0, 0], [1, 0, 0], [5, 7, 11], [P - 1, P - 2, 3]):
+        got = backend.fp3_powers(g, 20)
+        want = [[int(x) % P for x in v] for v in F.powers(g, 20)]
+        assert got.tolist() == want
+    # w^3 = 2: (0, 1, 0)^3 = 2
+    assert backend.fp3_powers([0, 1, 0], 4)[3].tolist() == [2, 0, 0]
