@@ -111,6 +111,15 @@ int lvm_coset_lde_ext3_row_major_with_merkle_tree_keep(const uint64_t *trace, si
                                                         size_t blowup, const uint64_t *weights,
                                                         uint64_t *host_lde_out, uint8_t root_out[32],
                                                         lvm_lde **handle_out);
+/* crypto/math-cuda/src/merkle.rs:546 `build_comp_poly_tree_from_evals_ext3_keep`
+ * (with constraint_interp.rs:514 `comp_h_to_slabs`): commit device rows that
+ * already are an LDE (N rows x w words, bit-reversed row order, e.g. the
+ * composition values of lvm_eval_composition when the composition has one
+ * part) with row-pair leaves; the handle keeps a copy. trace_rows: the degree
+ * bound the rows interpolate (barycentric evaluation reads every
+ * (N/trace_rows)-th row). */
+int lvm_commit_rows_dev(const uint64_t *dev_rows, size_t N, size_t w, size_t trace_rows, uint8_t root_out[32],
+                        lvm_lde **handle_out);
 int lvm_lde_release(lvm_lde *h);
 int lvm_lde_root(const lvm_lde *h, uint8_t root_out[32]);
 /* shape: rows of the LDE, base columns (Fp3 columns count 3), trace rows */

@@ -85,6 +85,7 @@ SIGNATURES = {
     "lvm_eval_composition": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_int,
                                        C.c_void_p]),
     "lvm_eval_constraints": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    "lvm_commit_rows_dev": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, u8p, vpp]),
     "lvm_prepare_constraints": (C.c_int, [C.c_void_p, C.c_void_p]),
     "lvm_set_constraint_jit": (C.c_int, [C.c_int]),
     "lvm_composition_commit_d2": (C.c_int, [C.c_void_p, C.c_size_t, C.c_uint64, u8p, vpp]),

@@ -464,6 +464,16 @@ def set_constraint_jit(on: bool) -> None:
     N.call("lvm_set_constraint_jit", int(bool(on)))
 
 
+def commit_rows_dev(dev_rows: DeviceBuffer, lde_rows: int, words_per_row: int, trace_rows: int) -> GpuLde:
+    """Commit device rows that already are an LDE in bit-reversed row order
+    (the one-part composition), as a handle with its Merkle tree."""
+    root = np.zeros(32, dtype=np.uint8)
+    h = C.c_void_p()
+    N.call("lvm_commit_rows_dev", C.c_void_p(dev_rows.ptr), lde_rows, words_per_row, trace_rows, _p8(root),
+           C.byref(h))
+    return GpuLde(h.value)
+
+
 def composition_commit_d2(dev_h: DeviceBuffer, lde_rows: int, coset_offset: int) -> GpuLde:
     """Degree-2 split of the bit-reversed composition values, extension of both
     halves to the LDE and their commitment: the parts LDE (lde_rows rows of [H0 | H1])."""

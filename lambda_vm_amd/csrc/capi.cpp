@@ -303,6 +303,48 @@ int lvm_coset_lde_dev(const uint64_t *dev_trace, size_t n, size_t m, size_t blow
     return lde_core(d, dev_trace, n, m, blowup, dev_weights, handle_out, root_out);
 }
 
+int lvm_commit_rows_dev(const uint64_t *dev_rows, size_t N, size_t w, size_t trace_rows, uint8_t root_out[32],
+                        lvm_lde **handle_out) {
+    if (!dev_rows || !handle_out) return fail(LVM_ERR_ARG, "null argument");
+    if (!is_pow2(N) || N < 2 || log2u(N) > 32) return fail(LVM_ERR_ARG, "rows must be a power of two in [2, 2^32]");
+    if (!is_pow2(trace_rows) || trace_rows > N) return fail(LVM_ERR_ARG, "trace rows must be a power of two <= rows");
+    if (w == 0 || w > (1u << 30)) return fail(LVM_ERR_ARG, "bad row width");
+    Device *d = dev_or_fail();
+    if (!d) return LVM_ERR_DEVICE;
+    TRY(check_fault());
+    hipStream_t st = thread_stream(d);
+    const size_t L = N / 2;
+    DevBuf rows, nodes;
+    TRY(rows.alloc(N * w * 8, st));
+    TRY(nodes.alloc((2 * L - 1) * 32, st));
+    LVM_HIP(hipMemcpyAsync(rows.p, dev_rows, N * w * 8, hipMemcpyDeviceToDevice, st));
+    {
+        ProfScope ps("keccak_leaves", st);
+        keccak_row_pair_leaves(rows.as<uint64_t>(), w, 0, w, L, nodes.as<uint8_t>() + (L - 1) * 32, st);
+    }
+    {
+        ProfScope ps("merkle_inner", st);
+        merkle_build_inner(nodes.as<uint8_t>(), L, st);
+    }
+    LVM_HIP(hipGetLastError());
+    uint8_t root[32];
+    LVM_HIP(hipMemcpyAsync(root, nodes.as<uint8_t>(), 32, hipMemcpyDeviceToHost, st));
+    LVM_HIP(hipStreamSynchronize(st));
+    lvm_lde *h = new lvm_lde();
+    h->device = d->id;
+    h->lde = (uint64_t *)rows.release();
+    h->nodes = (uint8_t *)nodes.release();
+    h->n = trace_rows;
+    h->N = N;
+    h->w = w;
+    h->blowup = N / trace_rows;
+    h->L = L;
+    std::memcpy(h->root, root, 32);
+    if (root_out) std::memcpy(root_out, root, 32);
+    *handle_out = h;
+    return LVM_OK;
+}
+
 int lvm_lde_release(lvm_lde *h) {
     if (!h) return LVM_OK;
     lvm_free(h->lde);

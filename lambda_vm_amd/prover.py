@@ -1,0 +1,233 @@
+"""Proving AIR tables end to end on the GPU: every round of the reference's
+multi-table prover for the synthetic AIRs, in its transcript order.
+
+Order (crypto/stark/src/prover.rs):
+  R1   every table's main LDE + Merkle root, absorbed in table order
+       (:3426-3471); the shared LogUp challenges (z, alpha) when a table has
+       bus interactions (:3476-3484, lookup.rs:105 LOGUP_NUM_CHALLENGES = 2);
+       one transcript fork per table, domain-separated by the table index as
+       u64 LE when there are several tables (:3529-3538); per table the aux
+       (LogUp) trace, its LDE + root, and its table contribution L
+       (:3867, :3928-3931).
+  R2   beta; transition coefficients beta^0..beta^(T-1), then the boundary
+       coefficients (:4453-4474); the composition polynomial on the LDE, split
+       into parts (one part when the degree bound is the trace length, the
+       degree-2 split otherwise, :1683-1760) and committed (:4482).
+  R3   z out of the trace domain and the LDE coset (is_transcript.rs:44-75);
+       the trace at z * g^k for every frame offset k; the current-row block
+       (all columns) then the next-row block column by column (ood.rs:94-126,
+       prover.rs:4536-4549); the parts at z^P (:4551-4554).
+  R4   gamma; the first W*K powers weight the trace terms in the order of
+       ood.rs:57-92 (current row: column j gets gamma^j; next rows: column j,
+       offset k >= 1 gets gamma^(W + j*(K-1) + k-1)), the next P powers the
+       parts (prover.rs:2156-2178); the DEEP codeword, FRI commit phase,
+       grinding nonce (BE bytes, :2277-2286), queries sample_u64(N/2)
+       (:2324-2333), FRI decommitments and the row-pair openings of the main,
+       aux and parts trees (:2888+).
+Every transition AIR here opens all columns at the next rows (the default
+AIR::trace_ood_next_row_columns, traits.rs:217-220).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import backend as B
+from . import field as F
+from .air import AirProgram
+from .pipeline import sample_z_ood
+
+P = F.P
+
+
+@dataclass
+class ProofOptions:
+    blowup: int = 4
+    coset_offset: int = 3
+    fri_final_log_degree: int = 2
+    num_queries: int = 16
+    grinding_factor: int = 0
+
+
+@dataclass
+class AirTableProof:
+    """One table's proof: the values a verifier reads (canonical limbs)."""
+    trace_rows: int
+    main_root: bytes
+    aux_root: bytes | None
+    table_contribution: tuple | None  # L (Fp3)
+    composition_root: bytes
+    ood_block0: np.ndarray   # (W, 3): every column at z
+    ood_block1: np.ndarray   # (W, K-1, 3): every column at z*g^k, k >= 1
+    parts_ood: np.ndarray    # (P, 3): H_i(z^P)
+    fri_roots: np.ndarray
+    fri_coeffs: np.ndarray
+    nonce: int | None
+    iotas: np.ndarray
+    fri_sym: np.ndarray      # (layers, Q, 3)
+    fri_paths: list          # per layer (Q, depth, 32)
+    main_rows: np.ndarray    # (Q, 2, m)
+    main_paths: np.ndarray
+    aux_rows: np.ndarray | None
+    aux_paths: np.ndarray | None
+    parts_rows: np.ndarray   # (Q, 2, 3P)
+    parts_paths: np.ndarray
+    transcript_state: bytes = b""
+    challenges: dict = field(default_factory=dict)
+
+
+def frame_offsets(air: AirProgram) -> int:
+    """K: rows of the OOD frame (transition offsets 0..max read offset)."""
+    return int(air.device.max_offset) + 1
+
+
+def num_parts(air: AirProgram) -> int:
+    """Composition parts: degree bound / trace length (prover.rs:1705)."""
+    return 1 if air.max_degree <= 1 else 2
+
+
+def fp3_columns(vals: np.ndarray) -> np.ndarray:
+    """(K, 3C, 3) component-column values -> (K, C, 3) Fp3-column values
+    a = a0 + a1*w + a2*w^2 (w^3 = 2)."""
+    K, w3, _ = vals.shape
+    out = np.zeros((K, w3 // 3, 3), dtype=np.uint64)
+    for k in range(K):
+        for c in range(w3 // 3):
+            a0, a1, a2 = ([int(x) for x in vals[k, 3 * c + i]] for i in range(3))
+            out[k, c] = [(a0[0] + 2 * a1[2] + 2 * a2[1]) % P, (a0[1] + a1[0] + 2 * a2[2]) % P,
+                         (a0[2] + a1[1] + a2[0]) % P]
+    return out
+
+
+def trace_term_gamma_index(W: int, K: int, j: int, k: int) -> int:
+    """Exponent of gamma weighting column j at frame row k (ood.rs:57-92 with
+    every column opened at the next rows)."""
+    return j if k == 0 else W + j * (K - 1) + (k - 1)
+
+
+def prove_air_tables(tables, opts: ProofOptions = ProofOptions(), label: bytes = b"lambda-vm-air-tables"):
+    """tables: [(AirProgram, device trace DeviceBuffer (n, m) row-major, n)].
+    Returns [AirTableProof] in table order."""
+    t = B.Transcript(label)
+    Bf, g = opts.blowup, opts.coset_offset
+    # ---- R1: main commitments of every table, one batched plan ----
+    weights = [B.coset_weights_dev(n, g) for _, _, n in tables]
+    plan = B.BatchPlan([(tr, w, n, air.num_main, Bf) for (air, tr, n), w in zip(tables, weights)], use_graph=False)
+    roots = plan.run()
+    mains = [plan.take(i) for i in range(len(tables))]
+    plan.close()
+    for r in roots:
+        t.append_bytes(r)
+    needs_lookup = any(air.interactions for air, _, _ in tables)
+    z_logup = alpha = None
+    if needs_lookup:
+        z_logup = [int(v) for v in t.sample_fp3()]
+        alpha = [int(v) for v in t.sample_fp3()]
+    forks = []
+    for idx in range(len(tables)):
+        f = t.clone()
+        if len(tables) > 1:
+            f.append_bytes(int(idx).to_bytes(8, "little"))
+        forks.append(f)
+    proofs = []
+    for idx, (air, tr, n) in enumerate(tables):
+        proofs.append(_prove_table(air, tr, n, mains[idx], weights[idx], forks[idx], opts, z_logup, alpha))
+        mains[idx].close()
+        weights[idx].free()
+    return proofs
+
+
+def _prove_table(air: AirProgram, tr: B.DeviceBuffer, n: int, main: B.GpuLde, weights: B.DeviceBuffer,
+                 t: B.Transcript, opts: ProofOptions, z_logup, alpha) -> AirTableProof:
+    Bf, g = opts.blowup, opts.coset_offset
+    N = n * Bf
+    m = air.num_main
+    rap = ap = toff = None
+    aux = None
+    L = None
+    # ---- R1 (per table): LogUp aux trace, commitment, contribution ----
+    if air.interactions:
+        desc = air_descriptor(air)
+        ap = B.fp3_powers(alpha, desc.alpha_powers_len)
+        aux_dev, L, _ = B.logup_aux_dev(tr, n, m, desc, ap, z_logup)
+        aux = B.coset_lde_dev(aux_dev, n, 3 * desc.num_out_cols, Bf, weights)
+        aux_dev.free()
+        t.append_bytes(aux.root)
+        L = tuple(int(v) % P for v in L)
+        t.append_fp3(L)
+        rap = [z_logup, alpha]
+        ninv = pow(n, P - 2, P)
+        toff = np.array([x * ninv % P for x in L], dtype=np.uint64)
+    # ---- R2: composition ----
+    beta = [int(v) for v in t.sample_fp3()]
+    T, nb = air.num_constraints, len(air.boundary)
+    coeffs = B.fp3_powers(beta, T + nb)
+    groups, ex = air.groups()
+    bnd = [(b.col, b.is_aux, b.step, b.value) for b in air.boundary]
+    H = B.eval_composition(air.device, main, aux, Bf, n, g, coeffs[:T], groups, ex, bnd, coeffs[T:], rap, ap, toff)
+    Pn = num_parts(air)
+    parts = B.composition_commit_d2(H, N, g) if Pn == 2 else B.commit_rows_dev(H, N, 3, n)
+    H.free()
+    t.append_bytes(parts.root)
+    # ---- R3: out-of-domain evaluations ----
+    z = sample_z_ood(t, n, N, g)
+    K = frame_offsets(air)
+    wn = F.root_of_unity(n.bit_length() - 1)
+    points, cur = [], list(z)
+    for _ in range(K):
+        points.append(cur)
+        cur = F.f3_scale(cur, wn)
+    pts = np.array(points, dtype=np.uint64)
+    ood = B.barycentric_base_multi(main, g, pts)  # (K, m, 3)
+    if aux is not None:
+        ood = np.concatenate([ood, fp3_columns(B.barycentric_base_multi(aux, g, pts))], axis=1)
+    W = ood.shape[1]
+    t.append_fp3_many(ood[0])                                  # current-row block
+    block1 = np.ascontiguousarray(ood[1:].transpose(1, 0, 2))  # (W, K-1, 3): column by column
+    t.append_fp3_many(block1.reshape(-1, 3))
+    zP = F.f3_pow(z, Pn)
+    parts_ood = fp3_columns(B.barycentric_base_multi(parts, g, np.array([zP], dtype=np.uint64)))[0]  # (P, 3)
+    t.append_fp3_many(parts_ood)
+    # ---- R4: DEEP, FRI, grinding, queries ----
+    gamma = [int(v) for v in t.sample_fp3()]
+    gp = B.fp3_powers(gamma, W * K + Pn)
+    gammas_tr = np.zeros((W * K, 3), dtype=np.uint64)
+    trace_ood = np.zeros((W * K, 3), dtype=np.uint64)
+    for j in range(W):
+        for k in range(K):
+            gammas_tr[j * K + k] = gp[trace_term_gamma_index(W, K, j, k)]
+            trace_ood[j * K + k] = ood[k, j]
+    deep = B.deep_composition(main, g, pts, trace_ood, gammas_tr, aux=aux, parts=parts, z_parts=zP,
+                              parts_ood=parts_ood, gammas_h=gp[W * K:])
+    fri = B.GpuFri(None, g, on_device=deep, length=N, adopt=True)
+    fri_roots, fri_coeffs = fri.commit_phase(t, Bf.bit_length() - 1, opts.fri_final_log_degree)
+    nonce = None
+    if opts.grinding_factor > 0:
+        nonce = B.generate_nonce(t.state(), opts.grinding_factor)
+        t.append_bytes(int(nonce).to_bytes(8, "big"))
+    iotas = np.array([t.sample_u64(N >> 1) for _ in range(opts.num_queries)], dtype=np.uint64)
+    fri_sym, fri_paths = fri.query(iotas)
+    fri.close()
+    main_paths, main_rows = main.open(iotas)
+    aux_paths = aux_rows = None
+    if aux is not None:
+        aux_paths, aux_rows = aux.open(iotas)
+        aux.close()
+    parts_paths, parts_rows = parts.open(iotas)
+    parts.close()
+    canon = lambda a: None if a is None else np.where(a >= np.uint64(P), a - np.uint64(P), a)  # noqa: E731
+    return AirTableProof(
+        trace_rows=n, main_root=main.root, aux_root=None if aux is None else bytes(aux.root),
+        table_contribution=L, composition_root=parts.root,
+        ood_block0=canon(ood[0].copy()), ood_block1=canon(block1), parts_ood=canon(parts_ood),
+        fri_roots=fri_roots, fri_coeffs=canon(fri_coeffs), nonce=nonce, iotas=iotas, fri_sym=canon(fri_sym),
+        fri_paths=fri_paths, main_rows=canon(main_rows), main_paths=main_paths, aux_rows=canon(aux_rows),
+        aux_paths=aux_paths, parts_rows=canon(parts_rows), parts_paths=parts_paths,
+        transcript_state=t.state(),
+        challenges=dict(z_logup=z_logup, alpha=alpha, beta=beta, z=z, gamma=gamma))
+
+
+def air_descriptor(air: AirProgram):
+    from .air import descriptor
+    return descriptor(air)
