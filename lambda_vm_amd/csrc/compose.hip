@@ -497,6 +497,15 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
                 bbeta[3 * i + k] = cp->b_beta[3 * b + k];
             }
         }
+        for (uint32_t k = 0; k < a.num_steps; k++) {
+            fp3 c{0, 0, 0};
+            for (uint32_t i = a.step_off[k]; i < a.step_off[k + 1]; i++)
+                c = f3_add(c, f3_mul(fp3{bbeta[3 * i], bbeta[3 * i + 1], bbeta[3 * i + 2]},
+                                     fp3{bval[3 * i], bval[3 * i + 1], bval[3 * i + 2]}));
+            a.step_const[3 * k] = c.c0;
+            a.step_const[3 * k + 1] = c.c1;
+            a.step_const[3 * k + 2] = c.c2;
+        }
         o_beta = put(blob, cp->beta, 3 * pg->num_roots);
         o_rg = put(blob, cp->root_group, pg->num_roots);
         o_z = put(blob, zc.data(), B);

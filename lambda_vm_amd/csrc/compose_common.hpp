@@ -45,6 +45,7 @@ struct CompArgs {
     uint64_t step_point[COMP_MAX_STEPS];
     uint32_t num_boundary;
     uint32_t step_off[COMP_MAX_STEPS + 1];  // boundaries of step s: [step_off[s], step_off[s+1])
+    uint64_t step_const[3 * COMP_MAX_STEPS];  // sum of beta_b * v_b over the boundaries of step s
     const uint32_t *b_col, *b_is_aux;      // boundary constraints, grouped by step
     const uint64_t *b_value, *b_beta;
     uint64_t *out;       // mode 0: N Fp3 (order below); mode 1: [(c*N + r)*3]
@@ -96,16 +97,16 @@ __device__ __forceinline__ fp3 composition_tail(const CompArgs &a, uint32_t r, u
             uint64_t zb = inv_all;
             for (uint32_t t = 0; t < a.num_steps; t++)
                 if (t != s) zb = mul(zb, sub(x, a.step_point[t]));
-            fp3 acc = fp3{0, 0, 0};
+            // sum_b beta_b (t_b - v_b) = sum_b beta_b t_b - step_const[s]; a main
+            // column's t_b is a base value (three products instead of nine)
+            fp3 acc = f3_neg(ld3(a.step_const + 3 * s));
             for (uint32_t bi = a.step_off[s]; bi < a.step_off[s + 1]; bi++) {
                 const uint32_t col = a.b_col[bi];
-                fp3 t;
+                const fp3 bb = ld3(a.b_beta + 3 * (uint64_t)bi);
                 if (a.b_is_aux[bi])
-                    t = ld3(a.aux + 3 * (((uint64_t)col << a.log_N) + p));
+                    acc = f3_add(acc, f3_mul(bb, ld3(a.aux + 3 * (((uint64_t)col << a.log_N) + p))));
                 else
-                    t = embed(a.main[((uint64_t)col << a.log_N) + p]);
-                const fp3 bp = f3_sub(t, ld3(a.b_value + 3 * (uint64_t)bi));
-                acc = f3_add(acc, f3_mul(ld3(a.b_beta + 3 * (uint64_t)bi), bp));
+                    acc = f3_add(acc, f3_mul_base(bb, a.main[((uint64_t)col << a.log_N) + p]));
             }
             h = f3_add(h, f3_mul_base(acc, zb));
         }

@@ -114,19 +114,34 @@ Device *device_state() {
 }
 
 namespace {
-// Streams this thread created (one per device); destroyed when the thread ends.
+// Library streams are never destroyed: events recorded on them (the buffer
+// cache keeps one per released block) stay valid for the life of the process.
+// A thread that ends hands its streams back, drained, for the next thread.
+std::mutex g_stream_mu;
+std::vector<std::vector<hipStream_t>> g_spare_streams;  // per device id
+
 struct ThreadStreams {
     std::vector<hipStream_t> s;
     std::vector<bool> owned;
     ~ThreadStreams() {
+        std::lock_guard<std::mutex> lk(g_stream_mu);
         for (size_t i = 0; i < s.size(); i++)
             if (s[i] && owned[i]) {
                 hipStreamSynchronize(s[i]);
-                hipStreamDestroy(s[i]);
+                if (g_spare_streams.size() <= i) g_spare_streams.resize(i + 1);
+                g_spare_streams[i].push_back(s[i]);
             }
     }
 };
 thread_local ThreadStreams t_streams;
+
+hipStream_t take_spare_stream(int id) {
+    std::lock_guard<std::mutex> lk(g_stream_mu);
+    if ((int)g_spare_streams.size() <= id || g_spare_streams[id].empty()) return nullptr;
+    hipStream_t s = g_spare_streams[id].back();
+    g_spare_streams[id].pop_back();
+    return s;
+}
 }  // namespace
 
 hipStream_t thread_stream(Device *d) {
@@ -141,8 +156,8 @@ hipStream_t thread_stream(Device *d) {
         if (d->stream_claimed.compare_exchange_strong(expected, true)) {
             ts.s[d->id] = d->stream;  // the first thread keeps the device's stream
         } else {
-            hipStream_t s = nullptr;
-            if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return d->stream;
+            hipStream_t s = take_spare_stream(d->id);
+            if (!s && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return d->stream;
             ts.s[d->id] = s;
             ts.owned[d->id] = true;
         }
@@ -198,7 +213,15 @@ hipError_t lvm_malloc_bytes(void **p, size_t bytes) {
         if (it != c->idle.end()) {
             auto &v = it->second;
             for (size_t i = v.size(); i-- > 0;) {
-                if (v[i].st == st || hipEventQuery(v[i].ev) == hipSuccess) {
+                bool ready = v[i].st == st;
+                if (!ready) {
+                    const hipError_t q = hipEventQuery(v[i].ev);
+                    ready = q == hipSuccess;
+                    // a pending event is not an error: clear the thread's last
+                    // error so a later hipGetLastError() check does not see it
+                    if (q == hipErrorNotReady) (void)hipGetLastError();
+                }
+                if (ready) {
                     *p = v[i].p;
                     c->spare_events.push_back(v[i].ev);
                     v[i] = v.back();

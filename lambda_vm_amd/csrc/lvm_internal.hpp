@@ -45,7 +45,8 @@ struct Device {
 Device *device_state();  // current device's state, initialised on first use; null on failure
 // The calling host thread's library stream on device d: every thread that
 // calls the library gets its own (default priority, non-blocking), so tables
-// proved from several threads overlap on the GPU. Work handed from one
+// proved from several threads overlap on the GPU. Streams outlive their
+// threads (an ending thread's stream is drained and reused by a later one). Work handed from one
 // thread to another is complete when the producing call returns (every
 // entry point that yields a handle or host data synchronises its stream).
 hipStream_t thread_stream(Device *d);

@@ -70,7 +70,11 @@ def prove_committed_table(h: B.GpuLde, t: B.Transcript, n: int, m: int, blowup: 
     t.append_fp3_many(ood.reshape(K * m, 3))  # point-major, columns in order
     # R4
     gamma = [int(v) for v in t.sample_fp3()]
-    gammas = B.fp3_powers(gamma, m * K)  # index j*K + k
+    # column j at point k weighted by gamma^j (k = 0) or gamma^(m + j(K-1) + k-1)
+    # (crypto/stark/src/ood.rs:57-92, every column opened at the next row)
+    gp = B.fp3_powers(gamma, m * K)
+    idx = [j if k == 0 else m + j * (K - 1) + k - 1 for j in range(m) for k in range(K)]
+    gammas = gp[idx]  # index j*K + k
     trace_ood = np.ascontiguousarray(ood.transpose(1, 0, 2).reshape(m * K, 3))
     deep = B.deep_composition(h, offset, np.array(points, dtype=np.uint64), trace_ood, gammas)
     fri = B.GpuFri(None, offset, on_device=deep, length=N, adopt=True)

@@ -106,9 +106,13 @@ def trace_term_gamma_index(W: int, K: int, j: int, k: int) -> int:
     return j if k == 0 else W + j * (K - 1) + (k - 1)
 
 
-def prove_air_tables(tables, opts: ProofOptions = ProofOptions(), label: bytes = b"lambda-vm-air-tables"):
+def prove_air_tables(tables, opts: ProofOptions = ProofOptions(), label: bytes = b"lambda-vm-air-tables",
+                     workers: int | None = None):
     """tables: [(AirProgram, device trace DeviceBuffer (n, m) row-major, n)].
-    Returns [AirTableProof] in table order."""
+    Returns [AirTableProof] in table order. After the shared R1 every table's
+    remaining rounds run on its own host thread and library stream (the
+    reference chains aux build, aux commit and rounds 2-4 per table on driver
+    threads, prover.rs:3590-3600); workers=1 proves them one after another."""
     t = B.Transcript(label)
     Bf, g = opts.blowup, opts.coset_offset
     # ---- R1: main commitments of every table, one batched plan ----
@@ -130,11 +134,24 @@ def prove_air_tables(tables, opts: ProofOptions = ProofOptions(), label: bytes =
         if len(tables) > 1:
             f.append_bytes(int(idx).to_bytes(8, "little"))
         forks.append(f)
-    proofs = []
-    for idx, (air, tr, n) in enumerate(tables):
-        proofs.append(_prove_table(air, tr, n, mains[idx], weights[idx], forks[idx], opts, z_logup, alpha))
-        mains[idx].close()
-        weights[idx].free()
+
+    def one(idx):
+        air, tr, n = tables[idx]
+        return _prove_table(air, tr, n, mains[idx], weights[idx], forks[idx], opts, z_logup, alpha)
+
+    w = len(tables) if workers is None else workers
+    if w > 1 and len(tables) > 1:
+        from concurrent.futures import ThreadPoolExecutor
+
+        from . import _native
+        with ThreadPoolExecutor(max_workers=min(w, len(tables)), initializer=_native.ensure_device_this_thread,
+                                initargs=(0,)) as pool:
+            proofs = list(pool.map(one, range(len(tables))))
+    else:
+        proofs = [one(i) for i in range(len(tables))]
+    for h, wt in zip(mains, weights):
+        h.close()
+        wt.free()
     return proofs
 
 

@@ -158,6 +158,12 @@ class GpuExecutor:
             self.pool.shutdown()
             self.pool = None
 
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
     def commit(self, jobs: list) -> None:
         """One batched commitment over every table of the given jobs."""
         B = self.B

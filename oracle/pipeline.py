@@ -6,7 +6,8 @@ DEEP coefficients, FRI commit phase and query sampling, in the order of
 crypto/stark/src/prover.rs (R1 :3426-3471, z :4494 with the rejection rule
 of crypto/crypto/src/fiat_shamir/is_transcript.rs:44-75, OOD appends
 :4547-4560, gamma powers :2159-2182, FRI crypto/stark/src/fri/mod.rs:27-151,
-queries prover.rs:2324-2333). Used as the parity checker of the GPU rounds
+queries prover.rs:2324-2333; trace-term weights in the order of
+crypto/stark/src/ood.rs:57-92). Used as the parity checker of the GPU rounds
 and as the timed CPU baseline.
 """
 from __future__ import annotations
@@ -52,10 +53,14 @@ def replay_single_table(tr: np.ndarray, blowup: int, g: int, k: int, queries: in
         for c in range(m):
             t.append_fp3(ood[kk, c])
     gamma = [int(v) for v in t.sample_fp3()]
-    gam, cur = [], [1, 0, 0]
+    pw, cur = [], [1, 0, 0]
     for _ in range(len(points) * m):
-        gam.append(cur)
+        pw.append(cur)
         cur = _f3mul(cur, gamma)
+    K = len(points)
+    # crypto/stark/src/ood.rs:57-92: column j at row 0 gets gamma^j, at row
+    # k >= 1 gamma^(m + j(K-1) + k-1); stored here at index j*K + k
+    gam = [pw[j if k == 0 else m + j * (K - 1) + k - 1] for j in range(m) for k in range(K)]
     tood = np.ascontiguousarray(ood.transpose(1, 0, 2).reshape(len(points) * m, 3))
     deep = deep_composition(lde, g, np.array(points, dtype=np.uint64), tood, np.array(gam, dtype=np.uint64))
     bits = N.bit_length() - 1

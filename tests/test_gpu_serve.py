@@ -33,6 +33,7 @@ def test_results_same_alone_and_in_mix(gpu):
         js = alone.done[0]
         assert js.digest() == mixed[spec.job_id][0], spec.job_id
         assert js.roots == mixed[spec.job_id][1]
+    ex.close()
 
 
 def test_thread_pool_matches_single_thread(gpu):
@@ -58,6 +59,7 @@ def test_service_roots_match_oracle(gpu, ora):
         tr = workload.trace_table_modp(spec.cfg, spec.job_id * 64 + ti, n, m)
         _, nodes = ora.lde_commit(tr, spec.blowup, 3, want_lde=False)
         assert js.roots[ti] == bytes(nodes[0]), ti
+    ex.close()
 
 
 def test_device_generator_matches_host(gpu):
