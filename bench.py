@@ -74,6 +74,16 @@ def measured_traffic(kernel: str, key: str):
     return rec.get("bytes_per_launch") if rec else None
 
 
+def measured_valu_issue(kernel: str, key: str):
+    """VALU issue fraction of `kernel` (SQ_INSTS_VALU / duration / peak issue
+    rate) from the committed counter summary for this workload, or None."""
+    path = os.path.join(ROOT, "profiles", "valu_issue.json")
+    if not os.path.exists(path):
+        return None
+    rec = json.load(open(path)).get("workloads", {}).get(key, {}).get(kernel)
+    return rec.get("valu_issue_frac") if rec else None
+
+
 def cpu_baseline(log_n: int, m: int, blowup: int, offset: int, commit_only: bool, fri_k: int, queries: int) -> dict:
     import oracle
     from lambda_vm_amd import workload
@@ -260,6 +270,7 @@ def run_ragged(args) -> dict:
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": measured_traffic(dom, "ragged14"),
+        "valu_issue_frac": measured_valu_issue(dom, "ragged14"),
         "avg_ms": round(avg_ms, 4),
         "per_launch_ms": {k: round(v[0], 4) for k, v in per_launch.items()},
         "per_launch_blocks": {k: v[2] for k, v in per_launch.items()},
