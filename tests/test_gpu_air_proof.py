@@ -1,7 +1,9 @@
 """AIR-table proofs on the GPU (lambda_vm_amd.prover) against the oracle's
 replay of the same rounds on the same traces: every committed root, OOD
 value, FRI layer root and coefficient, query index, opened row and path is
-equal (canonical limbs), and the verifier restatement accepts the GPU proofs.
+equal (canonical limbs), the canonical proof bytes are identical, and the
+verifier restatement accepts the GPU proofs (also after a round trip through
+the bytes).
 Tables: FibonacciMultiColumn (one composition part), Quadratic and a 4-bus
 LogUp table (two parts), alone and together in one transcript."""
 import numpy as np
@@ -85,3 +87,10 @@ def test_gpu_proofs_match_oracle_and_verify(gpu, which, n, blowup, k, q, grind):
         assert g.transcript_state == r["transcript_state"], t
     ok, why = verify_air_tables([a for a, _ in tables], got, blowup, 3, k, q, grind, b"gpu-air")
     assert ok, why
+    # the canonical proof bytes of the GPU prover and of the replay are identical
+    from lambda_vm_amd import proof_io
+    from lambda_vm_amd.prover import AirTableProof
+    data = proof_io.serialize_multi_proof(got)
+    assert data == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
+    assert verify_air_tables([a for a, _ in tables], proof_io.deserialize_multi_proof(data), blowup, 3, k, q, grind,
+                             b"gpu-air")[0]
