@@ -4,8 +4,9 @@ value, FRI layer root and coefficient, query index, opened row and path is
 equal (canonical limbs), the canonical proof bytes are identical, and the
 verifier restatement accepts the GPU proofs (also after a round trip through
 the bytes).
-Tables: FibonacciMultiColumn (one composition part), Quadratic and a 4-bus
-LogUp table (two parts), alone and together in one transcript."""
+Tables: FibonacciMultiColumn (one composition part), Quadratic, a 4-bus
+LogUp table and the VM's EQ table (two parts), alone and together in one
+transcript."""
 import numpy as np
 import pytest
 
@@ -34,6 +35,13 @@ def _tables(n, which, seed):
             a = A.quadratic_multi(5)
             tr = A.quadratic_trace(n, 5, rng.integers(0, 2**62, 5).astype(np.uint64))
             A.with_boundaries_from_trace(a, tr)
+        elif kind == "eq":
+            from lambda_vm_amd import vm_tables as V
+            a = V.eq_table_air()
+            a.bus_descriptor = A.descriptor(a)
+            ops = [V.EqOperation(int(x), int(x) if i % 3 == 0 else int(y), bool(i % 2))
+                   for i, (x, y) in enumerate(rng.integers(0, 2**63, (n - 3, 2)))]
+            tr = V.generate_eq_trace(ops)
         else:
             a = A.logup_air(24, A.four_bus_interactions())
             a.bus_descriptor = A.descriptor(a)
@@ -49,6 +57,7 @@ def _f3rows(x):
 
 @pytest.mark.parametrize("which,n,blowup,k,q,grind", [(("fib",), 256, 4, 2, 8, 0), (("quad",), 128, 8, 3, 6, 0),
                                                      (("logup",), 256, 4, 2, 8, 0),
+                                                     (("eq",), 256, 4, 2, 8, 0), (("eq", "fib"), 1024, 8, 2, 12, 2),
                                                      (("fib", "quad", "logup"), 512, 4, 3, 10, 4)])
 def test_gpu_proofs_match_oracle_and_verify(gpu, which, n, blowup, k, q, grind):
     from lambda_vm_amd import prover
