@@ -133,6 +133,8 @@ def main() -> int:
     ap.add_argument("--budget-gib", type=float, default=48.0, help="serve: HBM budget for jobs in flight")
     ap.add_argument("--workers", type=int, default=4, help="serve: host threads proving tables concurrently")
     ap.add_argument("--tables-per-tick", type=int, default=8, help="serve: tables advanced per scheduler tick")
+    ap.add_argument("--prove-workers", type=int, default=0,
+                    help="prove: host threads proving tables concurrently (0: one per table)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -507,15 +509,16 @@ def run_prove(args) -> dict:
                                num_queries=args.queries, grinding_factor=0)
     tables = [(a, tr, n) for a, tr, (_, n, _) in zip(airs, traces, specs)]
     cells = sum(n * m * 4 for _, n, m in specs)
+    workers = args.prove_workers or None
     for _ in range(max(1, args.warmup)):
-        prover.prove_air_tables(tables, opts)
+        prover.prove_air_tables(tables, opts, workers=workers)
     backend.sync()
     backend.prof_reset()
     backend.prof_enable(True)
     times = []
     for _ in range(args.steps):
         t0 = time.perf_counter()
-        prover.prove_air_tables(tables, opts)
+        prover.prove_air_tables(tables, opts, workers=workers)
         backend.sync()
         times.append(time.perf_counter() - t0)
     backend.prof_enable(False)
@@ -547,6 +550,7 @@ def run_prove(args) -> dict:
                          % (args.fri_k, args.queries)),
             "lde_cells": cells,
             "proof_ms": [round(x * 1e3, 3) for x in times],
+            "table_workers": workers or len(tables),
             "stage_ms_per_proof": stage,
         },
     }

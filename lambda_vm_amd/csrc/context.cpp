@@ -116,9 +116,13 @@ Device *device_state() {
 namespace {
 // Library streams are never destroyed: events recorded on them (the buffer
 // cache keeps one per released block) stay valid for the life of the process.
-// A thread that ends hands its streams back, drained, for the next thread.
-std::mutex g_stream_mu;
-std::vector<std::vector<hipStream_t>> g_spare_streams;  // per device id
+// A thread that ends hands its streams back for the next thread. The hand-back
+// makes no HIP call: a thread's destructors may run at process exit, after the
+// runtime (or a profiler attached to it) has shut down. Work still queued on a
+// handed-back stream stays ordered before whatever the next owner queues.
+// The list and its lock are never destroyed, for the same reason.
+std::mutex &g_stream_mu = *new std::mutex;
+std::vector<std::vector<hipStream_t>> &g_spare_streams = *new std::vector<std::vector<hipStream_t>>;  // per device
 
 struct ThreadStreams {
     std::vector<hipStream_t> s;
@@ -127,7 +131,6 @@ struct ThreadStreams {
         std::lock_guard<std::mutex> lk(g_stream_mu);
         for (size_t i = 0; i < s.size(); i++)
             if (s[i] && owned[i]) {
-                hipStreamSynchronize(s[i]);
                 if (g_spare_streams.size() <= i) g_spare_streams.resize(i + 1);
                 g_spare_streams[i].push_back(s[i]);
             }

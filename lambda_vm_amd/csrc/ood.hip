@@ -53,30 +53,115 @@ __global__ __launch_bounds__(256) void batch_inverse_fp3_kernel(const uint64_t *
     }
 }
 
-// den[k * rows + p] = sign ? (z_k - x_p) : (x_p - z_k) for the first `rows`
-// physical rows of an LDE of 2^log_N rows on offset*<w_N>.
-__global__ void coset_denoms_kernel(uint64_t *den, uint64_t rows, uint32_t log_N, uint64_t offset,
-                                    const uint64_t *z, uint32_t K, int z_minus_x, const uint64_t *lo,
-                                    const uint64_t *hi) {
-    uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= rows) return;
-    uint32_t e = log_N ? (bitrev((uint32_t)p, log_N) << (32 - log_N)) : 0u;
-    uint64_t x = mul(offset, mul(lo[e & 0xFFFFu], hi[e >> 16]));
-    for (uint32_t k = 0; k < K; k++) {
-        fp3 zk = fp3{z[3 * k], z[3 * k + 1], z[3 * k + 2]};
-        fp3 d = z_minus_x ? fp3{sub(zk.c0, x), zk.c1, zk.c2} : fp3{sub(x, zk.c0), neg(zk.c1), neg(zk.c2)};
-        st3(den + 3 * (k * rows + p), d);
+// Inverted pole factors over the LDE coset, for K poles z_k in Fp3 and the
+// first `rows` physical rows p of an LDE of 2^log_N rows on offset*<w_N>
+// (x_p = offset * w_N^bitrev_N(p)):
+//   mode 0: out[k * rows + p] = 1 / (x_p - z_k)        (DEEP, XMinusZ)
+//   mode 1: out[k * rows + p] = x_p / (z_k - x_p)      (barycentric weights)
+// x - z has constant w and w^2 parts (-z1, -z2), so with a0 = x - z0 its norm
+// and adjugate in Fp[w]/(w^3 - 2) are cheap base-field polynomials in a0:
+//   Norm = a0^3 - 6 z1 z2 a0 - (2 z1^3 + 4 z2^3)
+//   adj  = (a0^2 - 2 z1 z2, 2 z2^2 + z1 a0, z1^2 + z2 a0),  1/(x - z) = adj / Norm.
+// Each thread owns E rows p = t, t + T, ... (coalesced) and inverts the
+// products D(p) = prod_k Norm_k(p) of its rows with one base-field inversion
+// (Montgomery's trick, prefixes kept in registers); Norm_k^-1 = D^-1 * prod_{j != k} Norm_j.
+// Replaces the Fp3 batch inversion of materialised differences
+// (crypto/math-cuda/src/inverse.rs:241-300, crypto/stark/src/trace.rs:900-940).
+struct PoleConst {
+    uint64_t z0, z1, z2, c1x3, c3, c1, c2, z1s;
+};
+constexpr int POLE_E = 16;  // rows per thread
+
+template <int K>
+__global__ __launch_bounds__(256) void pole_inverse_kernel(uint64_t *__restrict__ out, uint64_t rows, uint32_t log_N,
+                                                           uint64_t offset, const uint64_t *__restrict__ z, int mode,
+                                                           const uint64_t *__restrict__ lo,
+                                                           const uint64_t *__restrict__ hi) {
+    const uint64_t T = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= rows) return;
+    PoleConst pc[K];
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const uint64_t z0 = z[3 * k], z1 = z[3 * k + 1], z2 = z[3 * k + 2];
+        const uint64_t z12 = mul(z1, z2), c1 = dbl(z12);
+        const uint64_t z1s = sqr(z1), z2s = sqr(z2);
+        pc[k] = PoleConst{z0, z1, z2, add(dbl(c1), c1), add(dbl(mul(z1s, z1)), dbl(dbl(mul(z2s, z2)))), c1, dbl(z2s),
+                          z1s};
+    }
+    auto point = [&](uint64_t p) -> uint64_t {
+        const uint32_t e = log_N ? (bitrev((uint32_t)p, log_N) << (32 - log_N)) : 0u;
+        return mul(offset, mul(lo[e & 0xFFFFu], hi[e >> 16]));
+    };
+    auto norm = [&](const PoleConst &c, uint64_t x, uint64_t &a0, uint64_t &sq) -> uint64_t {
+        a0 = sub(x, c.z0);
+        sq = sqr(a0);
+        return sub(sub(mul(sq, a0), mul(c.c1x3, a0)), c.c3);
+    };
+    uint64_t pre[POLE_E];
+    uint64_t acc = 1;
+#pragma unroll
+    for (int j = 0; j < POLE_E; j++) {
+        const uint64_t p = t + (uint64_t)j * T;
+        if (p < rows) {
+            const uint64_t x = point(p);
+            uint64_t d = 1;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                uint64_t a0, sq;
+                d = mul(d, norm(pc[k], x, a0, sq));
+            }
+            acc = mul(acc, d);
+        }
+        pre[j] = acc;
+    }
+    uint64_t inv_acc = inv(acc);
+#pragma unroll
+    for (int j = POLE_E - 1; j >= 0; j--) {
+        const uint64_t p = t + (uint64_t)j * T;
+        if (p >= rows) continue;
+        const uint64_t x = point(p);
+        uint64_t a0[K], sq[K], nk[K];
+#pragma unroll
+        for (int k = 0; k < K; k++) nk[k] = norm(pc[k], x, a0[k], sq[k]);
+        uint64_t d = 1;
+#pragma unroll
+        for (int k = 0; k < K; k++) d = mul(d, nk[k]);
+        uint64_t dinv = j ? mul(inv_acc, pre[j - 1]) : inv_acc;
+        inv_acc = mul(inv_acc, d);
+        if (mode) dinv = mul(dinv, neg(x));
+        // prefix / suffix products of the norms: Norm_k^-1 = D^-1 * pre_k * suf_k
+        uint64_t sfx[K];
+        sfx[K - 1] = 1;
+#pragma unroll
+        for (int k = K - 2; k >= 0; k--) sfx[k] = mul(sfx[k + 1], nk[k + 1]);
+        uint64_t pfx = dinv;
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const uint64_t ni = K > 1 ? mul(pfx, sfx[k]) : pfx;
+            if (k + 1 < K) pfx = mul(pfx, nk[k]);
+            const PoleConst &c = pc[k];
+            const fp3 adj = fp3{sub(sq[k], c.c1), add(c.c2, mul(c.z1, a0[k])), add(c.z1s, mul(c.z2, a0[k]))};
+            st3(out + 3 * ((uint64_t)k * rows + p), f3_mul_base(adj, ni));
+        }
     }
 }
 
-// scale[k * n + p] = x_p * inv_den[k * n + p]
-__global__ void bary_scale_kernel(uint64_t *s, uint64_t n, uint32_t K, uint32_t log_N, uint64_t offset,
-                                  const uint64_t *lo, const uint64_t *hi) {
-    uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= n) return;
-    uint32_t e = log_N ? (bitrev((uint32_t)p, log_N) << (32 - log_N)) : 0u;
-    uint64_t x = mul(offset, mul(lo[e & 0xFFFFu], hi[e >> 16]));
-    for (uint32_t k = 0; k < K; k++) st3(s + 3 * (k * n + p), f3_mul_base(ld3(s + 3 * (k * n + p)), x));
+static void launch_pole_inverse(uint64_t *out, uint64_t rows, uint32_t log_N, uint64_t offset, const uint64_t *z,
+                                uint32_t K, int mode, const uint64_t *lo, const uint64_t *hi, hipStream_t st) {
+    if (!rows) return;
+    const uint64_t threads = (rows + POLE_E - 1) / POLE_E;
+    const dim3 grid((unsigned)((threads + 255) / 256)), blk(256);
+    switch (K) {
+    case 1: hipLaunchKernelGGL(pole_inverse_kernel<1>, grid, blk, 0, st, out, rows, log_N, offset, z, mode, lo, hi); break;
+    case 2: hipLaunchKernelGGL(pole_inverse_kernel<2>, grid, blk, 0, st, out, rows, log_N, offset, z, mode, lo, hi); break;
+    case 3: hipLaunchKernelGGL(pole_inverse_kernel<3>, grid, blk, 0, st, out, rows, log_N, offset, z, mode, lo, hi); break;
+    case 4: hipLaunchKernelGGL(pole_inverse_kernel<4>, grid, blk, 0, st, out, rows, log_N, offset, z, mode, lo, hi); break;
+    case 5: hipLaunchKernelGGL(pole_inverse_kernel<5>, grid, blk, 0, st, out, rows, log_N, offset, z, mode, lo, hi); break;
+    case 6: hipLaunchKernelGGL(pole_inverse_kernel<6>, grid, blk, 0, st, out, rows, log_N, offset, z, mode, lo, hi); break;
+    case 7: hipLaunchKernelGGL(pole_inverse_kernel<7>, grid, blk, 0, st, out, rows, log_N, offset, z, mode, lo, hi); break;
+    default: hipLaunchKernelGGL(pole_inverse_kernel<8>, grid, blk, 0, st, out, rows, log_N, offset, z, mode, lo, hi); break;
+    }
 }
 
 constexpr int BARY_MAX_K = 8;
@@ -136,15 +221,29 @@ static void launch_bary_partial(dim3 grid, hipStream_t st, const uint64_t *lde, 
     hipLaunchKernelGGL(bary_partial_kernel<K>, grid, dim3(256), 0, st, lde, w, w, n, scale, partial);
 }
 
-// out[k][c] = vf[k] * sum over chunks of partial[chunk][k][c]
-__global__ void bary_combine_kernel(const uint64_t *partial, uint64_t chunks, uint32_t K, uint64_t w,
-                                    const uint64_t *vf, uint64_t *out) {
-    uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= (uint64_t)K * w) return;
-    uint64_t k = idx / w, c = idx - k * w;
+// out[k][c] = vf[k] * sum over chunks of partial[chunk][k][c]; one block per
+// output (k, c): its threads sum strided chunks, then a tree in LDS.
+__global__ __launch_bounds__(256) void bary_combine_kernel(const uint64_t *partial, uint64_t chunks, uint32_t K,
+                                                           uint64_t w, const uint64_t *vf, uint64_t *out) {
+    __shared__ uint64_t red[256][3];
+    const uint64_t idx = blockIdx.x;
+    const uint64_t k = idx / w, c = idx - k * w;
+    const uint32_t t = threadIdx.x;
     fp3 s = fp3{0, 0, 0};
-    for (uint64_t ch = 0; ch < chunks; ch++) s = f3_add(s, ld3(partial + 3 * ((ch * K + k) * w + c)));
-    st3(out + 3 * idx, f3_mul(ld3(vf + 3 * k), s));
+    for (uint64_t ch = t; ch < chunks; ch += 256) s = f3_add(s, ld3(partial + 3 * ((ch * K + k) * w + c)));
+    red[t][0] = s.c0;
+    red[t][1] = s.c1;
+    red[t][2] = s.c2;
+    for (uint32_t h = 128; h > 0; h >>= 1) {
+        __syncthreads();
+        if (t < h) {
+            fp3 v = f3_add(fp3{red[t][0], red[t][1], red[t][2]}, fp3{red[t + h][0], red[t + h][1], red[t + h][2]});
+            red[t][0] = v.c0;
+            red[t][1] = v.c1;
+            red[t][2] = v.c2;
+        }
+    }
+    if (t == 0) st3(out + 3 * idx, f3_mul(ld3(vf + 3 * k), fp3{red[0][0], red[0][1], red[0][2]}));
 }
 
 // DEEP composition at every LDE row (crypto/math-cuda/kernels/deep.cu:1-25,
@@ -158,23 +257,28 @@ constexpr int DEEP_MAX_K = 4;
 constexpr int DEEP_CHUNK = 16;  // main columns staged per LDS round
 // block: 256 consecutive physical rows. The main (base) columns of the block
 // are staged through LDS 16 at a time with coalesced row-segment loads; each
-// thread accumulates sum_j gt_{j,k} * t_j[p] as 3K delayed-reduction dot
-// products.
-template <int K>
+// thread accumulates sum_j gt_{j,k} * t_j[p] as 3*KA delayed-reduction dot
+// products. KA < K when the trace-term weights are geometric in k past the
+// first KA - 1 points (gt_{j,k} = gt_{j,KA-1} * r^(k-KA+1) for every column j,
+// which the reference's weights are: gamma^(W + j(K-1) + k - 1) for k >= 1,
+// crypto/stark/src/prover.rs:2400-2460): then sum_j gt_{j,k} t_j = r^(k-KA+1)
+// * sum_j gt_{j,KA-1} t_j, and rpow[k] holds r^(k-KA+1).
+template <int K, int KA>
 __global__ __launch_bounds__(256) void deep_kernel(uint64_t N, const uint64_t *__restrict__ main_lde, uint64_t wm,
                                                    const uint64_t *__restrict__ aux_lde, uint64_t wa,
                                                    const uint64_t *__restrict__ parts_lde, uint64_t np,
                                                    const uint64_t *__restrict__ gtr, const uint64_t *__restrict__ gh,
                                                    const uint64_t *__restrict__ consts,
+                                                   const uint64_t *__restrict__ rpow,
                                                    const uint64_t *__restrict__ inv_den, uint64_t *__restrict__ out) {
     __shared__ uint64_t tile[256][DEEP_CHUNK + 1];
-    __shared__ uint64_t sg[DEEP_CHUNK][K][3];
+    __shared__ uint64_t sg[DEEP_CHUNK][KA][3];
     const int t = threadIdx.x;
     const uint64_t r0 = (uint64_t)blockIdx.x * 256;
     const uint64_t p = r0 + t;
-    acc128 acc[K][3];
+    acc128 acc[KA][3];
 #pragma unroll
-    for (int k = 0; k < K; k++)
+    for (int k = 0; k < KA; k++)
 #pragma unroll
         for (int j = 0; j < 3; j++) acc_zero(acc[k][j]);
     for (uint64_t j0 = 0; j0 < wm; j0 += DEEP_CHUNK) {
@@ -184,8 +288,8 @@ __global__ __launch_bounds__(256) void deep_kernel(uint64_t N, const uint64_t *_
             const int rr = e / DEEP_CHUNK, cc = e % DEEP_CHUNK;
             tile[rr][cc] = (cc < cw && r0 + rr < N) ? main_lde[(r0 + rr) * wm + j0 + cc] : 0;
         }
-        for (int e = t; e < DEEP_CHUNK * K * 3; e += 256) {
-            const int cc = e / (K * 3), rem = e % (K * 3);
+        for (int e = t; e < DEEP_CHUNK * KA * 3; e += 256) {
+            const int cc = e / (KA * 3), rem = e % (KA * 3);
             sg[cc][rem / 3][rem % 3] = cc < cw ? gtr[3 * ((j0 + cc) * K + rem / 3) + rem % 3] : 0;
         }
         __syncthreads();
@@ -193,7 +297,7 @@ __global__ __launch_bounds__(256) void deep_kernel(uint64_t N, const uint64_t *_
         for (int cc = 0; cc < DEEP_CHUNK; cc++) {
             const uint64_t v = tile[t][cc];
 #pragma unroll
-            for (int k = 0; k < K; k++) {
+            for (int k = 0; k < KA; k++) {
                 acc_mad(acc[k][0], sg[cc][k][0], v);
                 acc_mad(acc[k][1], sg[cc][k][1], v);
                 acc_mad(acc[k][2], sg[cc][k][2], v);
@@ -203,15 +307,17 @@ __global__ __launch_bounds__(256) void deep_kernel(uint64_t N, const uint64_t *_
     if (p >= N) return;
     fp3 tk[K];
 #pragma unroll
-    for (int k = 0; k < K; k++) tk[k] = fp3{acc_reduce(acc[k][0]), acc_reduce(acc[k][1]), acc_reduce(acc[k][2])};
+    for (int k = 0; k < KA; k++) tk[k] = fp3{acc_reduce(acc[k][0]), acc_reduce(acc[k][1]), acc_reduce(acc[k][2])};
     if (aux_lde) {
         const uint64_t *arow = aux_lde + p * 3 * wa;
         for (uint64_t j = 0; j < wa; j++) {
             fp3 v = ld3(arow + 3 * j);
 #pragma unroll
-            for (int k = 0; k < K; k++) tk[k] = f3_add(tk[k], f3_mul(ld3(gtr + 3 * ((wm + j) * K + k)), v));
+            for (int k = 0; k < KA; k++) tk[k] = f3_add(tk[k], f3_mul(ld3(gtr + 3 * ((wm + j) * K + k)), v));
         }
     }
+#pragma unroll
+    for (int k = KA; k < K; k++) tk[k] = f3_mul(ld3(rpow + 3 * k), tk[KA - 1]);
     fp3 r = fp3{0, 0, 0};
 #pragma unroll
     for (int k = 0; k < K; k++)
@@ -225,14 +331,13 @@ __global__ __launch_bounds__(256) void deep_kernel(uint64_t N, const uint64_t *_
     st3(out + 3 * p, r);
 }
 
-template <int K>
+template <int K, int KA>
 static void launch_deep(hipStream_t st, uint64_t N, const uint64_t *main_lde, uint64_t wm, const uint64_t *aux_lde,
                         uint64_t wa, const uint64_t *parts_lde, uint64_t np, const uint64_t *gtr, const uint64_t *gh,
-                        const uint64_t *consts, const uint64_t *inv_den, uint64_t *out) {
-    hipLaunchKernelGGL(deep_kernel<K>, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, N, main_lde, wm, aux_lde,
-                       wa, parts_lde, np, gtr, gh, consts, inv_den, out);
+                        const uint64_t *consts, const uint64_t *rpow, const uint64_t *inv_den, uint64_t *out) {
+    hipLaunchKernelGGL((deep_kernel<K, KA>), dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, N, main_lde, wm,
+                       aux_lde, wa, parts_lde, np, gtr, gh, consts, rpow, inv_den, out);
 }
-
 }  // namespace lvm
 
 using namespace lvm;
@@ -325,10 +430,9 @@ int lvm_barycentric_base_multi(const lvm_lde *h, uint64_t coset_offset, const ui
     const uint64_t n = h->n, w = h->w;
     const unsigned logN = log2u(h->N);
     Scratch s;
-    uint64_t *dz, *den, *scale, *partial, *vfd, *dout;
+    uint64_t *dz, *scale, *partial, *vfd, *dout;
     const uint64_t chunks = (n + BARY_ROWS_PER_BLOCK - 1) / BARY_ROWS_PER_BLOCK;
     TRYS(s.get(&dz, K * 24));
-    TRYS(s.get(&den, K * n * 24));
     TRYS(s.get(&scale, K * n * 24));
     TRYS(s.get(&partial, chunks * K * w * 24));
     TRYS(s.get(&vfd, K * 24));
@@ -347,11 +451,8 @@ int lvm_barycentric_base_multi(const lvm_lde *h, uint64_t coset_offset, const ui
     LVM_HIP(hipMemcpyAsync(vfd, vf.data(), K * 24, hipMemcpyHostToDevice, st));
     {
         ProfScope ps("bary_denoms", st);
-        hipLaunchKernelGGL(coset_denoms_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, den, n, logN,
-                           coset_offset, dz, (uint32_t)K, 1, d->tb.fwd_lo, d->tb.fwd_hi);
-        launch_batch_inverse(den, scale, K * n, st);
-        hipLaunchKernelGGL(bary_scale_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, scale, n,
-                           (uint32_t)K, logN, coset_offset, d->tb.fwd_lo, d->tb.fwd_hi);
+        // scale[k][p] = x_p / (z_k - x_p)
+        launch_pole_inverse(scale, n, logN, coset_offset, dz, (uint32_t)K, 1, d->tb.fwd_lo, d->tb.fwd_hi, st);
     }
     {
         ProfScope ps("bary_sum", st);
@@ -366,8 +467,8 @@ int lvm_barycentric_base_multi(const lvm_lde *h, uint64_t coset_offset, const ui
         case 7: launch_bary_partial<7>(grid, st, h->lde, w, n, scale, partial); break;
         default: launch_bary_partial<8>(grid, st, h->lde, w, n, scale, partial); break;
         }
-        hipLaunchKernelGGL(bary_combine_kernel, dim3((unsigned)((K * w + 255) / 256)), dim3(256), 0, st, partial,
-                           chunks, (uint32_t)K, w, vfd, dout);
+        hipLaunchKernelGGL(bary_combine_kernel, dim3((unsigned)(K * w)), dim3(256), 0, st, partial, chunks,
+                           (uint32_t)K, w, vfd, dout);
     }
     LVM_HIP(hipGetLastError());
     LVM_HIP(hipMemcpyAsync(out, dout, K * w * 24, hipMemcpyDeviceToHost, st));
@@ -413,35 +514,60 @@ int lvm_deep_composition(const lvm_lde *main, const lvm_lde *aux, const lvm_lde 
         std::memcpy(&consts[3 * K], acc.c, 24);
         std::memcpy(&poles[3 * K], z_parts, 24);
     }
+    // geometric trace-term weights past k = 1: gt_{j,k} = gt_{j,k-1} * r for all j, k >= 2
+    std::vector<uint64_t> rp(3 * K, 0);
+    bool geo = K > 2;
+    if (geo) {
+        const H3 g01{{gammas_tr[3], gammas_tr[4], gammas_tr[5]}}, g02{{gammas_tr[6], gammas_tr[7], gammas_tr[8]}};
+        if (canon(g01.c[0]) == 0 && canon(g01.c[1]) == 0 && canon(g01.c[2]) == 0) {
+            geo = false;
+        } else {
+            const fp3 ri = f3_inv(fp3{g01.c[0], g01.c[1], g01.c[2]});
+            const H3 r = h3mul(g02, H3{{ri.c0, ri.c1, ri.c2}});
+            H3 pw{{1, 0, 0}};
+            for (size_t k = 2; k < K; k++) {
+                pw = h3mul(pw, r);
+                std::memcpy(&rp[3 * k], pw.c, 24);
+            }
+            for (uint64_t j = 0; j < cols && geo; j++)
+                for (size_t k = 2; k < K && geo; k++) {
+                    const uint64_t *a = gammas_tr + 3 * (j * K + k - 1), *b = gammas_tr + 3 * (j * K + k);
+                    const H3 want = h3mul(H3{{a[0], a[1], a[2]}}, r);
+                    for (int c = 0; c < 3; c++) geo = geo && canon(want.c[c]) == canon(b[c]);
+                }
+        }
+    }
     Scratch s;
-    uint64_t *dpoles, *den, *inv_den, *dconsts, *dgtr, *dgh = nullptr;
+    uint64_t *dpoles, *inv_den, *dconsts, *dgtr, *drp, *dgh = nullptr;
     TRYS(s.get(&dpoles, npoles * 24));
-    TRYS(s.get(&den, (uint64_t)npoles * N * 24));
     TRYS(s.get(&inv_den, (uint64_t)npoles * N * 24));
     TRYS(s.get(&dconsts, (K + 1) * 24));
     TRYS(s.get(&dgtr, cols * K * 24));
+    TRYS(s.get(&drp, K * 24));
     LVM_HIP(hipMemcpyAsync(dpoles, poles.data(), npoles * 24, hipMemcpyHostToDevice, st));
     LVM_HIP(hipMemcpyAsync(dconsts, consts.data(), (K + 1) * 24, hipMemcpyHostToDevice, st));
     LVM_HIP(hipMemcpyAsync(dgtr, gammas_tr, cols * K * 24, hipMemcpyHostToDevice, st));
+    LVM_HIP(hipMemcpyAsync(drp, rp.data(), K * 24, hipMemcpyHostToDevice, st));
     if (parts) {
         TRYS(s.get(&dgh, np * 24));
         LVM_HIP(hipMemcpyAsync(dgh, gammas_h, np * 24, hipMemcpyHostToDevice, st));
     }
     {
         ProfScope ps("deep_denoms", st);
-        // x - z (XMinusZ, crypto/math-cuda/src/inverse.rs:241-300)
-        hipLaunchKernelGGL(coset_denoms_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, den, N,
-                           log2u(N), coset_offset, dpoles, npoles, 0, d->tb.fwd_lo, d->tb.fwd_hi);
-        launch_batch_inverse(den, inv_den, (uint64_t)npoles * N, st);
+        // 1 / (x - z) (XMinusZ, crypto/math-cuda/src/inverse.rs:241-300)
+        launch_pole_inverse(inv_den, N, log2u(N), coset_offset, dpoles, npoles, 0, d->tb.fwd_lo, d->tb.fwd_hi, st);
     }
     {
         ProfScope ps("deep_rows", st);
         const uint64_t *al = aux ? aux->lde : nullptr, *pl = parts ? parts->lde : nullptr;
-        switch (K) {
-        case 1: launch_deep<1>(st, N, main->lde, wm, al, wa, pl, np, dgtr, dgh, dconsts, inv_den, dev_out); break;
-        case 2: launch_deep<2>(st, N, main->lde, wm, al, wa, pl, np, dgtr, dgh, dconsts, inv_den, dev_out); break;
-        case 3: launch_deep<3>(st, N, main->lde, wm, al, wa, pl, np, dgtr, dgh, dconsts, inv_den, dev_out); break;
-        default: launch_deep<4>(st, N, main->lde, wm, al, wa, pl, np, dgtr, dgh, dconsts, inv_den, dev_out); break;
+        const uint64_t *m = main->lde;
+        switch (K * 2 + (geo ? 1 : 0)) {
+        case 2: launch_deep<1, 1>(st, N, m, wm, al, wa, pl, np, dgtr, dgh, dconsts, drp, inv_den, dev_out); break;
+        case 4: launch_deep<2, 2>(st, N, m, wm, al, wa, pl, np, dgtr, dgh, dconsts, drp, inv_den, dev_out); break;
+        case 6: launch_deep<3, 3>(st, N, m, wm, al, wa, pl, np, dgtr, dgh, dconsts, drp, inv_den, dev_out); break;
+        case 7: launch_deep<3, 2>(st, N, m, wm, al, wa, pl, np, dgtr, dgh, dconsts, drp, inv_den, dev_out); break;
+        case 8: launch_deep<4, 4>(st, N, m, wm, al, wa, pl, np, dgtr, dgh, dconsts, drp, inv_den, dev_out); break;
+        default: launch_deep<4, 2>(st, N, m, wm, al, wa, pl, np, dgtr, dgh, dconsts, drp, inv_den, dev_out); break;
         }
     }
     LVM_HIP(hipGetLastError());

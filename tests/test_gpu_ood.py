@@ -58,9 +58,13 @@ def test_barycentric_equals_trace_interpolant(gpu, ora):
         assert [int(v) % workload.P for v in got[0, c]] == pyref.horner3([[cf, 0, 0] for cf in coeffs], z)
 
 
-@pytest.mark.parametrize("with_aux,with_parts,K", [(False, False, 2), (True, False, 1), (True, True, 2),
-                                                   (False, True, 4)])
-def test_deep_composition(gpu, ora, with_aux, with_parts, K):
+@pytest.mark.parametrize("with_aux,with_parts,K,geo", [(False, False, 2, False), (True, False, 1, False),
+                                                       (True, True, 2, False), (False, True, 4, False),
+                                                       (True, True, 3, True), (False, True, 4, True),
+                                                       (True, False, 3, False)])
+def test_deep_composition(gpu, ora, with_aux, with_parts, K, geo):
+    """geo: trace-term weights geometric in k past k = 1 (the reference's
+    gamma layout), which the library folds into two accumulated sums."""
     log_n, m, ma, np_, blowup, g = 9, 6, 2, 2, 4, 3
     n, N = 1 << log_n, (1 << log_n) * blowup
     w = ora.lde_weights(n, g)
@@ -77,6 +81,13 @@ def test_deep_composition(gpu, ora, with_aux, with_parts, K):
     zp = workload.fp3_values(91, K)
     tood = workload.fp3_values(92, cols * K)
     gtr = workload.fp3_values(93, cols * K)
+    if geo:
+        r = [int(v) for v in workload.fp3_values(97, 1)[0]]
+        g3 = gtr.reshape(cols, K, 3)
+        for j in range(cols):
+            for k in range(2, K):
+                g3[j, k] = pyref.f3mul([int(v) for v in g3[j, k - 1]], r)
+        gtr = g3.reshape(-1, 3)
     zpa = workload.fp3_values(94, 1)[0]
     pood = workload.fp3_values(95, np_)
     gh = workload.fp3_values(96, np_)
