@@ -221,8 +221,8 @@ __global__ __launch_bounds__(COMP_BLOCK) void compose_kernel(CompArgs a) {
         sl.stride = a.total_threads;
     }
     const uint64_t N = 1ull << a.log_N;
-    for (uint64_t p = (uint64_t)blockIdx.x * COMP_BLOCK + threadIdx.x; p < N; p += a.total_threads) {
-        const uint32_t r = bitrev((uint32_t)p, a.log_N);  // natural index of physical row p
+    for (uint64_t q = (uint64_t)blockIdx.x * COMP_BLOCK + threadIdx.x; q < N; q += a.total_threads) {
+        const uint32_t r = (uint32_t)q;  // natural index; its LDE row is bitrev(r)
         uint64_t *gsum = &s_gsum[0][threadIdx.x];
         for (uint32_t k = 0; k < 3 * a.num_groups; k++) gsum[k * COMP_BLOCK] = 0;
         walk<MODE>(a, r, sl, gsum);
@@ -233,8 +233,8 @@ __global__ __launch_bounds__(COMP_BLOCK) void compose_kernel(CompArgs a) {
             gs[g] = (uint32_t)g < a.num_groups ? fp3{gsum[3 * g * COMP_BLOCK], gsum[(3 * g + 1) * COMP_BLOCK],
                                                     gsum[(3 * g + 2) * COMP_BLOCK]}
                                                : fp3{0, 0, 0};
-        const fp3 h = composition_tail(a, r, p, gs);
-        const uint64_t orow = a.natural_out ? r : p;
+        const fp3 h = composition_tail(a, r, gs);
+        const uint64_t orow = a.natural_out ? r : bitrev(r, a.log_N);
         uint64_t *o = a.out + 3 * orow;
         o[0] = h.c0;
         o[1] = h.c1;
@@ -242,11 +242,61 @@ __global__ __launch_bounds__(COMP_BLOCK) void compose_kernel(CompArgs a) {
     }
 }
 
-// Column-major copy of a row-major table for the constraint walk: element
-// (row, c) of E words moves to out[(c * N + row) * E + k]. A wave's trace
-// reads then cover consecutive rows of one column (one contiguous segment)
-// instead of one word from each of 64 rows. 32 x 32-element tiles through
-// LDS; loads and stores are both row-contiguous.
+// binv[s * N + r] = 1 / (x_r - w^step_s) at every natural LDE index r
+// (x_r = offset * w_N^r). Each thread takes STEP_INV_E indices r = t,
+// t + T, ... and inverts the product of their D(r) = prod_s (x_r - w^step_s)
+// with one base-field inversion (Montgomery's trick, prefixes in registers);
+// then 1 / (x - c_s) = D^-1 * prod_{u != s} (x - c_u).
+constexpr int STEP_INV_E = 16;
+__global__ __launch_bounds__(256) void step_inverse_kernel(const CompArgs a, uint64_t *__restrict__ binv) {
+    const uint64_t N = 1ull << a.log_N;
+    const uint64_t T = (uint64_t)gridDim.x * blockDim.x;
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= N) return;
+    const uint32_t S = a.num_steps;
+    auto point = [&](uint64_t p) -> uint64_t {
+        return mul(a.offset, tw32_table(a.tw_lo, a.tw_hi, (uint32_t)p << (32 - a.log_N)));
+    };
+    uint64_t pre[STEP_INV_E];
+    uint64_t acc = 1;
+#pragma unroll
+    for (int j = 0; j < STEP_INV_E; j++) {
+        const uint64_t p = t + (uint64_t)j * T;
+        if (p < N) {
+            const uint64_t x = point(p);
+            uint64_t d = 1;
+            for (uint32_t s = 0; s < S; s++) d = mul(d, sub(x, a.step_point[s]));
+            acc = mul(acc, d);
+        }
+        pre[j] = acc;
+    }
+    uint64_t inv_acc = inv(acc);
+#pragma unroll
+    for (int j = STEP_INV_E - 1; j >= 0; j--) {
+        const uint64_t p = t + (uint64_t)j * T;
+        if (p >= N) continue;
+        const uint64_t x = point(p);
+        uint64_t d = 1;
+        for (uint32_t s = 0; s < S; s++) d = mul(d, sub(x, a.step_point[s]));
+        const uint64_t dinv = j ? mul(inv_acc, pre[j - 1]) : inv_acc;
+        inv_acc = mul(inv_acc, d);
+        for (uint32_t s = 0; s < S; s++) {
+            uint64_t v = dinv;
+            for (uint32_t u = 0; u < S; u++)
+                if (u != s) v = mul(v, sub(x, a.step_point[u]));
+            binv[((uint64_t)s << a.log_N) + p] = v;
+        }
+    }
+}
+
+
+// Column-major copy of a row-major LDE table (rows in bit-reversed order) in
+// natural row order for the constraint walk: element (natural row r, c) of E
+// words, stored at input row bitrev(r), moves to out[(c * N + r) * E + k].
+// A wave's reads then cover consecutive natural rows of one column, and the
+// frame rows r + k*blowup of neighbouring lanes overlap. 32 x 32-element tiles
+// through LDS: a tile's 32 natural rows are 32 whole input row segments, its
+// stores 32-row column segments.
 template <int E>
 __global__ __launch_bounds__(256) void transpose_kernel(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
                                                         uint64_t N, uint32_t ncols) {
@@ -255,10 +305,11 @@ __global__ __launch_bounds__(256) void transpose_kernel(const uint64_t *__restri
     const uint64_t r0 = (uint64_t)blockIdx.x * T;
     const uint32_t c0 = blockIdx.y * T;
     const uint64_t wpr = (uint64_t)ncols * E;  // words per input row
+    const unsigned logN = (unsigned)__builtin_ctzll(N);
     for (int i = threadIdx.x; i < T * T * E; i += 256) {
         const int rr = i / (T * E), ww = i % (T * E);
         const uint64_t r = r0 + rr, w = (uint64_t)c0 * E + ww;
-        if (r < N && w < wpr) tile[rr][ww] = in[r * wpr + w];
+        if (r < N && w < wpr) tile[rr][ww] = in[(uint64_t)bitrev((uint32_t)r, logN) * wpr + w];
     }
     __syncthreads();
     for (int i = threadIdx.x; i < T * T * E; i += 256) {
@@ -401,6 +452,7 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
     }
     const unsigned logN = log2u(N);
     CompArgs a{};
+    BoundarySteps jb;
     std::vector<uint8_t> blob;
     std::vector<uint32_t> roots32(pg->num_roots);
     for (size_t c = 0; c < pg->num_roots; c++) roots32[c] = (uint32_t)pg->roots[c];
@@ -420,7 +472,7 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
                  o_al = put(blob, pg->alpha_powers, 3 * pg->num_alpha),
                  o_to = put(blob, pg->table_offset, pg->table_offset ? 3 : 0), o_roots = put(blob, roots32.data(), roots32.size()),
                  o_eoff = put(blob, emit_off.data(), emit_off.size()), o_eidx = put(blob, emit_idx.data(), emit_idx.size());
-    size_t o_beta = 0, o_rg = 0, o_z = 0, o_bcol = 0, o_baux = 0, o_bval = 0, o_bbeta = 0;
+    size_t o_beta = 0, o_rg = 0, o_z = 0, o_bcol = 0, o_baux = 0, o_bval = 0, o_bbeta = 0, o_bpcol = 0, o_bpflag = 0, o_bpbeta = 0;
     if (mode == 0) {
         const size_t n = cp->trace_rows;
         if (!is_pow2(n) || N % n || n < 2) return fail(LVM_ERR_ARG, "trace rows must be a power of two dividing the LDE size");
@@ -460,32 +512,16 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
         a.log_B = logB;
         a.offset = cp->coset_offset;
         // boundary steps
-        std::vector<uint32_t> step_idx(cp->num_boundary);
-        std::vector<uint64_t> steps;
-        for (size_t b = 0; b < cp->num_boundary; b++) {
+        for (size_t b = 0; b < cp->num_boundary; b++)
             if (cp->b_is_aux[b] ? (!aux || cp->b_col[b] >= ma) : cp->b_col[b] >= main->w)
                 return fail(LVM_ERR_ARG, "boundary column out of range");
-            const uint64_t s = cp->b_step[b] % n;
-            size_t k = 0;
-            while (k < steps.size() && steps[k] != s) k++;
-            if (k == steps.size()) {
-                if (steps.size() == COMP_MAX_STEPS) return fail(LVM_ERR_ARG, "at most 16 distinct boundary steps");
-                steps.push_back(s);
-            }
-            step_idx[b] = (uint32_t)k;
-        }
-        a.num_steps = (uint32_t)steps.size();
-        for (size_t k = 0; k < steps.size(); k++) a.step_point[k] = pow(wn, steps[k]);
-        a.num_boundary = (uint32_t)cp->num_boundary;
-        // boundaries regrouped by step (stable), with their columns, values and betas
         std::vector<uint32_t> order;
-        for (uint32_t k = 0; k <= a.num_steps; k++) a.step_off[k] = 0;
-        for (uint32_t k = 0; k < a.num_steps; k++) {
-            a.step_off[k] = (uint32_t)order.size();
-            for (size_t b = 0; b < cp->num_boundary; b++)
-                if (step_idx[b] == k) order.push_back((uint32_t)b);
-        }
-        a.step_off[a.num_steps] = (uint32_t)order.size();
+        if (!boundary_steps_of(cp->b_col, cp->b_is_aux, cp->b_step, cp->num_boundary, n, jb, &order))
+            return fail(LVM_ERR_ARG, "at most 16 distinct boundary steps");
+        a.num_steps = jb.num_steps;
+        for (uint32_t k = 0; k < jb.num_steps; k++) a.step_point[k] = pow(wn, jb.step_row[k]);
+        a.num_boundary = (uint32_t)cp->num_boundary;
+        for (uint32_t k = 0; k <= jb.num_steps; k++) a.step_off[k] = jb.step_off[k];
         std::vector<uint32_t> bcol(order.size()), baux(order.size());
         std::vector<uint64_t> bval(3 * order.size()), bbeta(3 * order.size());
         for (size_t i = 0; i < order.size(); i++) {
@@ -513,6 +549,45 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
         o_baux = put(blob, baux.data(), baux.size());
         o_bval = put(blob, bval.data(), bval.size());
         o_bbeta = put(blob, bbeta.data(), bbeta.size());
+        // main-column boundaries per pair of steps: one entry per column, the
+        // betas of its boundaries at each step of the pair summed
+        std::vector<uint32_t> pcol, pflag;
+        std::vector<uint64_t> pbeta;
+        for (uint32_t k = 0; 2 * k < a.num_steps; k++) {
+            a.bp_off[k] = (uint32_t)pcol.size();
+            const uint32_t s0 = 2 * k, s1 = std::min(a.num_steps, s0 + 2);
+            std::vector<uint32_t> cols;
+            for (uint32_t i = a.step_off[s0]; i < a.step_off[s1]; i++)
+                if (!baux[i]) cols.push_back(bcol[i]);
+            std::sort(cols.begin(), cols.end());
+            cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+            for (uint32_t c : cols) {
+                uint32_t fl = 0;
+                fp3 bsum[2] = {fp3{0, 0, 0}, fp3{0, 0, 0}};
+                for (uint32_t st = s0; st < s1; st++)
+                    for (uint32_t i = a.step_off[st]; i < a.step_off[st + 1]; i++)
+                        if (!baux[i] && bcol[i] == c) {
+                            fl |= 1u << (st - s0);
+                            bsum[st - s0] = f3_add(bsum[st - s0], fp3{bbeta[3 * i], bbeta[3 * i + 1], bbeta[3 * i + 2]});
+                        }
+                pcol.push_back(c);
+                pflag.push_back(fl);
+                for (int j = 0; j < 2; j++) {
+                    pbeta.push_back(bsum[j].c0);
+                    pbeta.push_back(bsum[j].c1);
+                    pbeta.push_back(bsum[j].c2);
+                }
+            }
+            a.bp_off[k + 1] = (uint32_t)pcol.size();
+        }
+        if (pcol.empty()) {
+            pcol.push_back(0);
+            pflag.push_back(0);
+            pbeta.resize(6, 0);
+        }
+        o_bpcol = put(blob, pcol.data(), pcol.size());
+        o_bpflag = put(blob, pflag.data(), pflag.size());
+        o_bpbeta = put(blob, pbeta.data(), pbeta.size());
     }
     Device *dv = device_state();
     if (!dv) return LVM_ERR_DEVICE;
@@ -524,17 +599,19 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
     const uint64_t blocks = std::min<uint64_t>((N + COMP_BLOCK - 1) / COMP_BLOCK, max_blocks);
     a.total_threads = blocks * COMP_BLOCK;
     uint8_t *db = nullptr;
-    uint64_t *scratch = nullptr, *cm_main = nullptr, *cm_aux = nullptr;
+    uint64_t *scratch = nullptr, *cm_main = nullptr, *cm_aux = nullptr, *binv = nullptr;
     auto release = [&]() {
         lvm_free(db);
         lvm_free(scratch);
         lvm_free(cm_main);
         lvm_free(cm_aux);
+        lvm_free(binv);
     };
     if (lvm_malloc(&db, blob.size()) != hipSuccess ||
         (!in_lds && lvm_malloc(&scratch, slot_bytes * a.total_threads) != hipSuccess) ||
         lvm_malloc(&cm_main, N * main->w * 8) != hipSuccess ||
-        (aux && lvm_malloc(&cm_aux, N * aux->w * 8) != hipSuccess)) {
+        (aux && lvm_malloc(&cm_aux, N * aux->w * 8) != hipSuccess) ||
+        (mode == 0 && a.num_steps && lvm_malloc(&binv, N * a.num_steps * 8) != hipSuccess)) {
         release();
         return fail(LVM_ERR_NOMEM, "device allocation for the constraint program failed");
     }
@@ -571,6 +648,15 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
         a.b_is_aux = (const uint32_t *)(db + o_baux);
         a.b_value = (const uint64_t *)(db + o_bval);
         a.b_beta = (const uint64_t *)(db + o_bbeta);
+        a.bp_col = (const uint32_t *)(db + o_bpcol);
+        a.bp_flags = (const uint32_t *)(db + o_bpflag);
+        a.bp_beta = (const uint64_t *)(db + o_bpbeta);
+    }
+    a.binv = binv;
+    if (mode == 0 && a.num_steps) {
+        ProfScope ps("composition_step_inverse", st);
+        const uint64_t threads = (N + STEP_INV_E - 1) / STEP_INV_E;
+        hipLaunchKernelGGL(step_inverse_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, a, binv);
     }
     a.out = out;
     a.natural_out = natural_out ? 1u : 0u;

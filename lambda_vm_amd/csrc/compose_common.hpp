@@ -24,8 +24,8 @@ struct CompArgs {
     const uint32_t *roots;
     const uint32_t *emit_off, *emit_idx;  // per node: the constraints whose value it computes (CSR)
     uint32_t nbs, nes;  // base / ext slot counts
-    // column-major copies of the tables (rows in the LDE's bit-reversed order):
-    // main (col, row) at main[col * N + row]; aux Fp3 (col, row) at aux[3 * (col * N + row)]
+    // column-major copies of the tables in natural row order (natural LDE
+    // index r): main (col, r) at main[col * N + r]; aux Fp3 at aux[3 * (col * N + r)]
     const uint64_t *main;
     const uint64_t *aux;
     uint32_t log_N;
@@ -48,6 +48,14 @@ struct CompArgs {
     uint64_t step_const[3 * COMP_MAX_STEPS];  // sum of beta_b * v_b over the boundaries of step s
     const uint32_t *b_col, *b_is_aux;      // boundary constraints, grouped by step
     const uint64_t *b_value, *b_beta;
+    const uint64_t *binv;  // num_steps x N: 1 / (x_r - w^step_s) at natural index r
+    // main-column boundaries per pair of steps (2k, 2k+1): entries
+    // [bp_off[k], bp_off[k+1]), one per column, with the column, a presence
+    // flag per step (bit j for step 2k+j) and the summed betas of the
+    // column's boundaries at each step (6 words: step 2k, then 2k+1)
+    const uint32_t *bp_col, *bp_flags;
+    const uint64_t *bp_beta;
+    uint32_t bp_off[COMP_MAX_STEPS / 2 + 2];
     uint64_t *out;       // mode 0: N Fp3 (order below); mode 1: [(c*N + r)*3]
     uint32_t natural_out;
     uint64_t *scratch;   // global slots when not in LDS
@@ -61,19 +69,19 @@ __device__ __forceinline__ uint64_t tw32_table(const uint64_t *lo, const uint64_
 }
 __device__ __forceinline__ fp3 embed(uint64_t b) { return fp3{b, 0, 0}; }
 
-// physical row of natural index (r + off*next_step) mod N
+// row of the column-major copies holding natural index (r + off*next_step) mod N
 __device__ __forceinline__ uint64_t frame_row(const CompArgs &a, uint32_t r, uint32_t off) {
     const uint64_t N = 1ull << a.log_N;
-    const uint64_t nat = ((uint64_t)r + off * a.next_step) & (N - 1);
-    return bitrev((uint32_t)nat, a.log_N);
+    return ((uint64_t)r + off * a.next_step) & (N - 1);
 }
 
 // H(x_r) from the per-group transition sums: each group's sum times its
 // inverse zerofier prod_j (x - w^(n-j)) / (x^n - 1), plus the boundary terms
-// beta_b (t_b(x) - v_b) / (x - w^step_b) with all step denominators inverted
-// together (one base inversion per row). Boundaries come grouped by step.
-__device__ __forceinline__ fp3 composition_tail(const CompArgs &a, uint32_t r, uint64_t p,
-                                                const fp3 *gs) {
+// beta_b (t_b(x) - v_b) / (x - w^step_b), boundaries grouped by step, with the
+// step denominators inverted beforehand (binv). A step's main-column terms
+// sum_b beta_b t_b are three delayed-reduction dot products.
+// Transition part: sum over groups of the group's sum times its inverse zerofier.
+__device__ __forceinline__ fp3 composition_groups(const CompArgs &a, uint32_t r, const fp3 *gs) {
     // x = offset * w_N^r
     const uint64_t x = mul(a.offset, tw32_table(a.tw_lo, a.tw_hi, r << (32 - a.log_N)));
     const uint64_t zc = a.zcyc[r & ((1u << a.log_B) - 1)];
@@ -87,28 +95,52 @@ __device__ __forceinline__ fp3 composition_tail(const CompArgs &a, uint32_t r, u
             if ((uint32_t)q == g) gv = gs[q];
         h = f3_add(h, f3_mul_base(gv, z));
     }
-    if (a.num_boundary) {
-        // one inversion for all steps: 1/(x - w^s) = inv(prod) * prod_{t != s}(x - w^t);
-        // boundaries come grouped by step
-        uint64_t prod = 1;
-        for (uint32_t s = 0; s < a.num_steps; s++) prod = mul(prod, sub(x, a.step_point[s]));
-        const uint64_t inv_all = inv(prod);
-        for (uint32_t s = 0; s < a.num_steps; s++) {
-            uint64_t zb = inv_all;
-            for (uint32_t t = 0; t < a.num_steps; t++)
-                if (t != s) zb = mul(zb, sub(x, a.step_point[t]));
-            // sum_b beta_b (t_b - v_b) = sum_b beta_b t_b - step_const[s]; a main
-            // column's t_b is a base value (three products instead of nine)
-            fp3 acc = f3_neg(ld3(a.step_const + 3 * s));
-            for (uint32_t bi = a.step_off[s]; bi < a.step_off[s + 1]; bi++) {
-                const uint32_t col = a.b_col[bi];
-                const fp3 bb = ld3(a.b_beta + 3 * (uint64_t)bi);
-                if (a.b_is_aux[bi])
-                    acc = f3_add(acc, f3_mul(bb, ld3(a.aux + 3 * (((uint64_t)col << a.log_N) + p))));
-                else
-                    acc = f3_add(acc, f3_mul_base(bb, a.main[((uint64_t)col << a.log_N) + p]));
+    return h;
+}
+
+__device__ __forceinline__ fp3 composition_tail(const CompArgs &a, uint32_t r, const fp3 *gs) {
+    fp3 h = composition_groups(a, r, gs);
+    // steps in pairs: each main column of a pair's boundaries is read once and
+    // feeds its boundaries' sums sum_b beta_b t_b, kept as delayed-reduction
+    // dot products (three per step)
+    for (uint32_t s0 = 0, k = 0; s0 < a.num_steps; s0 += 2, k++) {
+        acc128 m00, m01, m02, m10, m11, m12;
+        acc_zero(m00);
+        acc_zero(m01);
+        acc_zero(m02);
+        acc_zero(m10);
+        acc_zero(m11);
+        acc_zero(m12);
+        // unrolled so that several columns' loads are in flight together
+#pragma unroll 8
+        for (uint32_t e = a.bp_off[k]; e < a.bp_off[k + 1]; e++) {
+            const uint32_t fl = a.bp_flags[e];
+            const uint64_t v = a.main[((uint64_t)a.bp_col[e] << a.log_N) + r];
+            const uint64_t *bb = a.bp_beta + 6 * (uint64_t)e;
+            if (fl & 1) {
+                acc_mad(m00, bb[0], v);
+                acc_mad(m01, bb[1], v);
+                acc_mad(m02, bb[2], v);
             }
-            h = f3_add(h, f3_mul_base(acc, zb));
+            if (fl & 2) {
+                acc_mad(m10, bb[3], v);
+                acc_mad(m11, bb[4], v);
+                acc_mad(m12, bb[5], v);
+            }
+        }
+        for (uint32_t j = 0; j < 2 && s0 + j < a.num_steps; j++) {
+            const uint32_t s = s0 + j;
+            // sum_b beta_b (t_b - v_b) = sum_b beta_b t_b - step_const[s]
+            fp3 acc = f3_neg(ld3(a.step_const + 3 * s));
+            if (j == 0)
+                acc = f3_add(acc, fp3{acc_reduce(m00), acc_reduce(m01), acc_reduce(m02)});
+            else
+                acc = f3_add(acc, fp3{acc_reduce(m10), acc_reduce(m11), acc_reduce(m12)});
+            for (uint32_t bi = a.step_off[s]; bi < a.step_off[s + 1]; bi++)
+                if (a.b_is_aux[bi])
+                    acc = f3_add(acc, f3_mul(ld3(a.b_beta + 3 * (uint64_t)bi),
+                                             ld3(a.aux + 3 * (((uint64_t)a.b_col[bi] << a.log_N) + r))));
+            h = f3_add(h, f3_mul_base(acc, a.binv[((uint64_t)s << a.log_N) + r]));
         }
     }
     return h;

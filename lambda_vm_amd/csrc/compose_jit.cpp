@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -45,6 +46,8 @@ struct Operand {
     bool ext;
 };
 
+constexpr size_t JIT_LOAD_WINDOW = 48;
+
 // Straight-line source for the program; empty on a program this generator
 // does not take (then the interpreter runs it).
 std::string generate(const lvm_program *pg, const std::vector<uint32_t> &root_node, const uint32_t *root_group,
@@ -56,6 +59,7 @@ std::string generate(const lvm_program *pg, const std::vector<uint32_t> &root_no
         emits[root_node[c]].push_back((uint32_t)c);
     }
     uint32_t max_off = 0;
+    bool acc_group[COMP_MAX_GROUPS] = {false, false, false, false};
     std::vector<std::string> base_slot(pg->num_base_slots), ext_slot(pg->num_ext_slots);
     std::string body;
     body.reserve(n * 96);
@@ -94,6 +98,11 @@ std::string generate(const lvm_program *pg, const std::vector<uint32_t> &root_no
         }
     };
     auto as_ext = [](const Operand &o) { return o.ext ? o.expr : "embed(" + o.expr + ")"; };
+    // per-node source; trace reads are then moved to the front of their
+    // window of JIT_LOAD_WINDOW nodes so that a window's loads are in flight
+    // together (they depend on nothing, so the order of values is unchanged)
+    std::vector<std::string> node_text(n);
+    std::vector<bool> is_load(n, false);
     for (size_t k = 0; k < n; k++) {
         const uint64_t w0 = pg->nodes[2 * k], w1 = pg->nodes[2 * k + 1];
         const uint32_t op = (uint32_t)w0, x = (uint32_t)(w0 >> 32), y = (uint32_t)w1, res = (uint32_t)(w1 >> 32);
@@ -171,19 +180,35 @@ std::string generate(const lvm_program *pg, const std::vector<uint32_t> &root_no
         default: return "";
         }
         if (ext != rext) return "";  // result class differs from its slot class: leave it to the interpreter
-        body += std::string("        const ") + (ext ? "fp3 " : "uint64_t ") + v + " = " + expr + ";\n";
+        std::string &t = node_text[k];
+        is_load[k] = op == OP_VAR;
+        t += std::string("        const ") + (ext ? "fp3 " : "uint64_t ") + v + " = " + expr + ";\n";
         (ext ? ext_slot : base_slot)[slot] = v;
         for (uint32_t c : emits[k]) {
             const std::string beta = "ld3(a.beta + " + std::to_string(3 * (size_t)c) + ")";
-            if (mode == 0) {
+            if (mode == 0 && ext) {
                 const std::string g = "g" + std::to_string(root_group[c]);
-                body += "        " + g + " = f3_add(" + g + ", " +
-                        (ext ? "f3_mul(" + v + ", " + beta + ")" : "f3_mul_base(" + beta + ", " + v + ")") + ");\n";
+                t += "        " + g + " = f3_add(" + g + ", f3_mul(" + v + ", " + beta + "));\n";
+            } else if (mode == 0) {
+                // a base-valued constraint enters its group's sum through three
+                // delayed-reduction dot products (one reduction per row)
+                const uint32_t g = root_group[c];
+                acc_group[g] = true;
+                for (int q = 0; q < 3; q++)
+                    t += "        acc_mad(A" + std::to_string(g) + "_" + std::to_string(q) + ", a.beta[" +
+                            std::to_string(3 * (size_t)c + q) + "], " + v + ");\n";
             } else {
-                body += "        store3(a.out + 3 * ((uint64_t)" + std::to_string(c) + " * N + r), " +
+                t += "        store3(a.out + 3 * ((uint64_t)" + std::to_string(c) + " * N + r), " +
                         (ext ? v : "embed(" + v + ")") + ");\n";
             }
         }
+    }
+    for (size_t w0 = 0; w0 < n; w0 += JIT_LOAD_WINDOW) {
+        const size_t w1 = std::min(n, w0 + JIT_LOAD_WINDOW);
+        for (size_t k = w0; k < w1; k++)
+            if (is_load[k]) body += node_text[k];
+        for (size_t k = w0; k < w1; k++)
+            if (!is_load[k]) body += node_text[k];
     }
     std::string src = "typedef unsigned long long uint64_t;\ntypedef unsigned int uint32_t;\n";
     src += lvm_jit_field_src;
@@ -191,17 +216,28 @@ std::string generate(const lvm_program *pg, const std::vector<uint32_t> &root_no
            "o[2] = v.c2; }\n}\n";
     src += "extern \"C\" __global__ __launch_bounds__(128) void lvm_constraint_kernel(lvm::CompArgs a) {\n";
     src += "    using namespace lvm;\n    const uint64_t N = 1ull << a.log_N;\n";
-    src += "    for (uint64_t p = (uint64_t)blockIdx.x * 128 + threadIdx.x; p < N; p += a.total_threads) {\n";
-    src += "        const uint32_t r = bitrev((uint32_t)p, a.log_N);\n";
+    src += "    for (uint64_t q = (uint64_t)blockIdx.x * 128 + threadIdx.x; q < N; q += a.total_threads) {\n";
+    src += "        const uint32_t r = (uint32_t)q;\n";
     for (uint32_t off = 0; off <= max_off; off++)
         src += "        const uint64_t f" + std::to_string(off) + " = frame_row(a, r, " + std::to_string(off) + ");\n";
-    if (mode == 0)
+    if (mode == 0) {
         src += "        fp3 g0 = fp3{0, 0, 0}, g1 = fp3{0, 0, 0}, g2 = fp3{0, 0, 0}, g3 = fp3{0, 0, 0};\n";
+        auto decl = [&](const std::string &A) { src += "        acc128 " + A + ";\n        acc_zero(" + A + ");\n"; };
+        for (int g = 0; g < COMP_MAX_GROUPS; g++)
+            if (acc_group[g])
+                for (int q = 0; q < 3; q++) decl("A" + std::to_string(g) + "_" + std::to_string(q));
+    }
     src += body;
     if (mode == 0) {
+        for (int g = 0; g < COMP_MAX_GROUPS; g++)
+            if (acc_group[g]) {
+                const std::string A = "A" + std::to_string(g) + "_";
+                src += "        g" + std::to_string(g) + " = f3_add(g" + std::to_string(g) + ", fp3{acc_reduce(" + A +
+                       "0), acc_reduce(" + A + "1), acc_reduce(" + A + "2)});\n";
+            }
         src += "        const fp3 gs[COMP_MAX_GROUPS] = {g0, g1, g2, g3};\n";
-        src += "        const fp3 h = composition_tail(a, r, p, gs);\n";
-        src += "        store3(a.out + 3 * (a.natural_out ? (uint64_t)r : p), h);\n";
+        src += "        const fp3 h = composition_tail(a, r, gs);\n";
+        src += "        store3(a.out + 3 * (a.natural_out ? (uint64_t)r : (uint64_t)bitrev(r, a.log_N)), h);\n";
     }
     src += "    }\n}\n";
     return src;
@@ -211,6 +247,15 @@ std::string generate(const lvm_program *pg, const std::vector<uint32_t> &root_no
 // stderr when LVM_JIT_LOG is set).
 std::vector<char> compile(const std::string &src) {
     std::vector<char> code;
+    if (const char *dump = std::getenv("LVM_JIT_DUMP")) {
+        // write the generated source for inspection (one file per program)
+        static int seq = 0;
+        const std::string path = std::string(dump) + "." + std::to_string(seq++) + ".hip";
+        if (FILE *f = std::fopen(path.c_str(), "w")) {
+            std::fwrite(src.data(), 1, src.size(), f);
+            std::fclose(f);
+        }
+    }
     hiprtcProgram prog;
     if (hiprtcCreateProgram(&prog, src.c_str(), "lvm_constraints.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS)
         return code;
@@ -272,6 +317,38 @@ bool root_nodes_of(const lvm_program *pg, std::vector<uint32_t> &root_node) {
             root_node[c] = (uint32_t)(k - 1);
         }
     }
+    return true;
+}
+
+bool boundary_steps_of(const uint32_t *b_col, const uint32_t *b_is_aux, const uint64_t *b_step, size_t num_boundary,
+                       uint64_t trace_rows, BoundarySteps &jb, std::vector<uint32_t> *order_out) {
+    jb = BoundarySteps{};
+    if (num_boundary && !trace_rows) return false;
+    std::vector<uint32_t> idx(num_boundary);
+    for (size_t b = 0; b < num_boundary; b++) {
+        const uint64_t s = b_step[b] % trace_rows;
+        size_t k = 0;
+        while (k < jb.step_row.size() && jb.step_row[k] != s) k++;
+        if (k == jb.step_row.size()) {
+            if (jb.step_row.size() == (size_t)COMP_MAX_STEPS) return false;
+            jb.step_row.push_back(s);
+        }
+        idx[b] = (uint32_t)k;
+    }
+    jb.num_steps = (uint32_t)jb.step_row.size();
+    std::vector<uint32_t> order;
+    jb.step_off.assign(jb.num_steps + 1, 0);
+    for (uint32_t k = 0; k < jb.num_steps; k++) {
+        jb.step_off[k] = (uint32_t)order.size();
+        for (size_t b = 0; b < num_boundary; b++)
+            if (idx[b] == k) order.push_back((uint32_t)b);
+    }
+    jb.step_off[jb.num_steps] = (uint32_t)order.size();
+    for (uint32_t b : order) {
+        jb.col.push_back(b_col[b]);
+        jb.is_aux.push_back(b_is_aux[b] ? 1u : 0u);
+    }
+    if (order_out) *order_out = std::move(order);
     return true;
 }
 

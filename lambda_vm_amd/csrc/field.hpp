@@ -267,6 +267,21 @@ LVM_HD void acc_zero(acc128 &a) {
 
 LVM_HD void acc_mad(acc128 &a, uint64_t x, uint64_t y) {
     const uint32_t a0 = (uint32_t)x, a1 = (uint32_t)(x >> 32), b0 = (uint32_t)y, b1 = (uint32_t)(y >> 32);
+#if defined(__HIP_DEVICE_COMPILE__)
+    // each partial product: one v_mad_u64_u32 into its running sum with the
+    // carry-out mask, one v_addc_co_u32 adding that carry to the sum's count
+    uint64_t cy;
+    asm("v_mad_u64_u32 %0, %6, %7, %9, %0\n\t"
+        "v_addc_co_u32 %3, %6, 0, %3, %6\n\t"
+        "v_mad_u64_u32 %1, %6, %7, %10, %1\n\t"
+        "v_addc_co_u32 %4, %6, 0, %4, %6\n\t"
+        "v_mad_u64_u32 %1, %6, %8, %9, %1\n\t"
+        "v_addc_co_u32 %4, %6, 0, %4, %6\n\t"
+        "v_mad_u64_u32 %2, %6, %8, %10, %2\n\t"
+        "v_addc_co_u32 %5, %6, 0, %5, %6"
+        : "+v"(a.s0), "+v"(a.s1), "+v"(a.s2), "+v"(a.c0), "+v"(a.c1), "+v"(a.c2), "=&s"(cy)
+        : "v"(a0), "v"(a1), "v"(b0), "v"(b1));
+#else
     uint64_t t;
     t = a.s0 + (uint64_t)a0 * b0;
     a.c0 += t < a.s0;
@@ -280,6 +295,7 @@ LVM_HD void acc_mad(acc128 &a, uint64_t x, uint64_t y) {
     t = a.s2 + (uint64_t)a1 * b1;
     a.c2 += t < a.s2;
     a.s2 = t;
+#endif
 }
 
 // value = s0 + c0*2^64 + s1*2^32 + c1*2^96 + s2*2^64 + c2*2^128 (mod p), with

@@ -173,11 +173,21 @@ struct CompArgs;
 // Node computing each constraint (given, or the last writer of a pinned root slot).
 bool root_nodes_of(const lvm_program *pg, std::vector<uint32_t> &root_node);
 bool jit_enabled();
+// Boundary constraints of a composition grouped by step, steps in
+// first-seen order (the order of CompArgs::b_col).
+struct BoundarySteps {
+    uint32_t num_steps = 0;
+    std::vector<uint32_t> step_off;  // num_steps + 1
+    std::vector<uint32_t> col, is_aux;
+    std::vector<uint64_t> step_row;  // trace row of each step
+};
+// false when there are more than COMP_MAX_STEPS distinct steps (mod trace_rows)
+bool boundary_steps_of(const uint32_t *b_col, const uint32_t *b_is_aux, const uint64_t *b_step, size_t num_boundary,
+                       uint64_t trace_rows, BoundarySteps &bs, std::vector<uint32_t> *order);
 // Compiles the program's kernel (or finds it compiled); false when the
 // generator does not take the program or the compile fails.
 bool jit_prepare(const lvm_program *pg, const std::vector<uint32_t> &root_node, const uint32_t *root_group, int mode);
 // Launches the compiled kernel on the current device; false on failure.
 bool jit_launch(const lvm_program *pg, const std::vector<uint32_t> &root_node, const uint32_t *root_group, int mode,
                 CompArgs &args, uint64_t blocks, hipStream_t st);
-
 }  // namespace lvm

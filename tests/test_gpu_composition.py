@@ -105,6 +105,27 @@ def test_composition_matches_oracle(gpu, ora, jit, which, n, B, m):
         assert np.array_equal(canon(ev), canon(ev_ref))
 
 
+@pytest.mark.parametrize("which", ["fib", "logup"])
+def test_composition_many_boundary_steps(gpu, ora, jit, which):
+    """Boundaries on seven steps (the compiled kernel keeps four steps' sums
+    live through the program and sums the rest afterwards), several per step,
+    on main and aux columns, with arbitrary values."""
+    rng = random.Random(77)
+    n, B = 64, 4
+    c = _case(gpu, ora, which, n, B, 5, rng)
+    air = c["air"]
+    w = 5 if which == "fib" else 24
+    bnd = [(col, False, s, rand3(rng)) for s in (0, 3, 9, 63, 1, 17, 40) for col in range(0, w, 2)]
+    if which == "logup":
+        bnd += [(0, True, 9, rand3(rng)), (1, True, 40, rand3(rng)), (0, True, 5, rand3(rng))]
+    bb = [rand3(rng) for _ in bnd]
+    args = (B, n, G, c["betas"], c["groups"], c["ex"], bnd, bb, c["rap"], c["alpha"], c["toff"])
+    ref = ora.eval_composition(air.device, c["main_lde"], c["aux_lde"], *args)
+    got = gpu.eval_composition(air.device, c["main_h"], c["aux_h"], *args, natural_order=True).to_array(
+        np.uint64, (n * B, 3))
+    assert np.array_equal(canon(got), canon(ref))
+
+
 @pytest.mark.parametrize("which", ["fib", "quad", "logup"])
 def test_gpu_composition_low_degree(gpu, ora, jit, which):
     rng = random.Random(5)
