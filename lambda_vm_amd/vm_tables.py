@@ -15,6 +15,17 @@ on each diff half and ZERO[sum of the halves] -> eq, sent with multiplicity
 mu; ALU[a (DWordWL), b (DWordWL), EQ + 64 invert, res, 0] received with mu.
 Trace generation follows eq.rs:120-145 (one row per operation, padded with
 zero rows to a power of two, at least 4).
+
+BYTEWISE (prover/src/tables/bytewise.rs): full-word AND/OR/XOR of two 64-bit
+values, byte by byte. Columns (bytewise.rs:29-42): a (8 bytes, DWordBL), b (8
+bytes), op (the alu_op code), res (8 bytes), mu. No polynomial constraints of
+its own; its bus interactions (bytewise.rs:127-203): BYTE_ALU[op, a_i, b_i,
+res_i] sent with multiplicity mu for each byte, ARE_BYTES[mu, 0] sent once per
+row, ALU[a, b, op, res] (DWordBL operands) received with mu. Trace generation
+(bytewise.rs:96-121): duplicate operations merged with their count as mu,
+split at MU_MAX = 255 (types.rs:528-550), padded to a power of two (at least
+4). The reference merges through a hash map, so its row order is arbitrary;
+here rows follow first occurrence.
 """
 from __future__ import annotations
 
@@ -24,14 +35,15 @@ import numpy as np
 
 from . import constraint_ir as ir
 from .air import AirProgram, Boundary, emit_logup_constraints
-from .logup import BusInteraction, BusValue, LinearTerm, Multiplicity, Packing, split_interactions
+from .logup import ONE, BusInteraction, BusValue, LinearTerm, Multiplicity, Packing, split_interactions
 
 P = ir.P
 INV_SHIFT_32 = 18446744065119617026  # 2^-32 mod p (templates.rs:26)
 
 # bus ids (prover/src/tables/types.rs:255-336) and the ALU op code (:562-570)
-BUS_IS_HALFWORD, BUS_ZERO, BUS_ALU = 1, 8, 25
-ALU_OP_EQ = 3
+BUS_ARE_BYTES, BUS_IS_HALFWORD, BUS_ZERO, BUS_BYTE_ALU, BUS_ALU = 0, 1, 8, 24, 25
+ALU_OP_AND, ALU_OP_OR, ALU_OP_XOR, ALU_OP_EQ = 0, 1, 2, 3
+MU_MAX = 255
 
 
 class EqCols:
@@ -114,3 +126,84 @@ def eq_table_air() -> AirProgram:
                      boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
                      max_degree=max(3 if pairs else 0, 1 + absorbed, 2))
     return air
+
+
+class BytewiseCols:
+    A = tuple(range(0, 8))
+    B = tuple(range(8, 16))
+    OP = 16
+    RES = tuple(range(17, 25))
+    MU = 25
+    NUM_COLUMNS = 26
+
+
+@dataclass(frozen=True)
+class BytewiseOperation:
+    a: int
+    b: int
+    op: int
+
+    def compute_res(self) -> int:
+        if self.op == ALU_OP_AND:
+            return self.a & self.b
+        if self.op == ALU_OP_OR:
+            return self.a | self.b
+        if self.op == ALU_OP_XOR:
+            return self.a ^ self.b
+        raise ValueError(f"BYTEWISE only handles AND/OR/XOR, got opcode {self.op}")
+
+
+def dedup_rows(ops):
+    """(op, mu) rows: each distinct op with its count, split at MU_MAX
+    (types.rs:534-550), in order of first occurrence."""
+    counts = {}
+    for op in ops:
+        counts[op] = counts.get(op, 0) + 1
+    rows = []
+    for op, left in counts.items():
+        while left > 0:
+            mu = min(left, MU_MAX)
+            left -= mu
+            rows.append((op, mu))
+    return rows
+
+
+def generate_bytewise_trace(ops) -> np.ndarray:
+    c = BytewiseCols
+    rows = dedup_rows(ops)
+    n = max(4, 1 << max(len(rows) - 1, 0).bit_length())
+    t = np.zeros((n, c.NUM_COLUMNS), dtype=np.uint64)
+    for i, (op, mu) in enumerate(rows):
+        res = op.compute_res()
+        for k in range(8):
+            t[i, c.A[k]] = (op.a >> (8 * k)) & 0xFF
+            t[i, c.B[k]] = (op.b >> (8 * k)) & 0xFF
+            t[i, c.RES[k]] = (res >> (8 * k)) & 0xFF
+        t[i, c.OP] = op.op
+        t[i, c.MU] = mu
+    return t
+
+
+def bytewise_bus_interactions():
+    c = BytewiseCols
+    mu = Multiplicity("Column", (c.MU,))
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    its = [BusInteraction.sender(BUS_BYTE_ALU, mu, [d(c.OP), d(c.A[i]), d(c.B[i]), d(c.RES[i])]) for i in range(8)]
+    its.append(BusInteraction.sender(BUS_ARE_BYTES, ONE, [d(c.MU), BusValue.of_linear([LinearTerm(0)])]))
+    its.append(BusInteraction.receiver(BUS_ALU, mu, [
+        BusValue.packed(Packing.DWORD_BL, c.A[0]), BusValue.packed(Packing.DWORD_BL, c.B[0]), d(c.OP),
+        BusValue.packed(Packing.DWORD_BL, c.RES[0])]))
+    return its
+
+
+def bytewise_table_air() -> AirProgram:
+    """BYTEWISE as an AIR: only the LogUp constraints of its interactions."""
+    c = BytewiseCols
+    b = ir.IrBuilder()
+    its = bytewise_bus_interactions()
+    n_total = emit_logup_constraints(b, its, 0)
+    prog = b.finish(c.NUM_COLUMNS)
+    pairs, absorbed = split_interactions(len(its))
+    return AirProgram("bytewise", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
+                      boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
+                      max_degree=max(3 if pairs else 0, 1 + absorbed))

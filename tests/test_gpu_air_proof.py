@@ -5,8 +5,8 @@ equal (canonical limbs), the canonical proof bytes are identical, and the
 verifier restatement accepts the GPU proofs (also after a round trip through
 the bytes).
 Tables: FibonacciMultiColumn (one composition part), Quadratic, a 4-bus
-LogUp table and the VM's EQ table (two parts), alone and together in one
-transcript."""
+LogUp table and the VM's EQ and BYTEWISE tables (two parts), alone and
+together in one transcript."""
 import numpy as np
 import pytest
 
@@ -42,6 +42,13 @@ def _tables(n, which, seed):
             ops = [V.EqOperation(int(x), int(x) if i % 3 == 0 else int(y), bool(i % 2))
                    for i, (x, y) in enumerate(rng.integers(0, 2**63, (n - 3, 2)))]
             tr = V.generate_eq_trace(ops)
+        elif kind == "bytewise":
+            from lambda_vm_amd import vm_tables as V
+            a = V.bytewise_table_air()
+            a.bus_descriptor = A.descriptor(a)
+            ops = [V.BytewiseOperation(int(x), int(y), int(o)) for x, y, o in
+                   zip(rng.integers(0, 2**63, n - 3), rng.integers(0, 2**63, n - 3), rng.integers(0, 3, n - 3))]
+            tr = V.generate_bytewise_trace(ops)
         else:
             a = A.logup_air(24, A.four_bus_interactions())
             a.bus_descriptor = A.descriptor(a)
@@ -58,6 +65,7 @@ def _f3rows(x):
 @pytest.mark.parametrize("which,n,blowup,k,q,grind", [(("fib",), 256, 4, 2, 8, 0), (("quad",), 128, 8, 3, 6, 0),
                                                      (("logup",), 256, 4, 2, 8, 0),
                                                      (("eq",), 256, 4, 2, 8, 0), (("eq", "fib"), 1024, 8, 2, 12, 2),
+                                                     (("bytewise", "eq"), 512, 4, 3, 10, 0),
                                                      (("fib", "quad", "logup"), 512, 4, 3, 10, 4)])
 def test_gpu_proofs_match_oracle_and_verify(gpu, which, n, blowup, k, q, grind):
     from lambda_vm_amd import prover
