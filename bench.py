@@ -131,8 +131,10 @@ def main() -> int:
     ap.add_argument("--rate", type=float, default=20.0, help="serve: job arrivals per second (Poisson)")
     ap.add_argument("--jobs", type=int, default=120, help="serve: number of jobs")
     ap.add_argument("--budget-gib", type=float, default=48.0, help="serve: HBM budget for jobs in flight")
-    ap.add_argument("--workers", type=int, default=4, help="serve: host threads proving tables concurrently")
+    ap.add_argument("--workers", type=int, default=8, help="serve: host threads proving tables concurrently")
     ap.add_argument("--tables-per-tick", type=int, default=8, help="serve: tables advanced per scheduler tick")
+    ap.add_argument("--serve-mode", choices=["tick", "async"], default="async",
+                    help="serve: tick barriers, or work-conserving dispatch to the worker pool")
     ap.add_argument("--prove-workers", type=int, default=0,
                     help="prove: host threads proving tables concurrently (0: one per table)")
     args = ap.parse_args()
@@ -326,11 +328,12 @@ def run_serve(args) -> dict:
     ex = serve.GpuExecutor(offset=3, fri_k=args.fri_k, num_queries=args.queries, workers=args.workers)
     # warm-up: one small job through the same path (code objects, pools, every worker's stream)
     warm = serve.ProvingService(budget, ex, tables_per_tick=args.tables_per_tick)
-    warm.run([(0.0, serve.JobSpec(10 ** 6, [(1 << 10, 8)] * max(4, args.workers)))])
+    warm_jobs = [(0.0, serve.JobSpec(10 ** 6, [(1 << 10, 8)] * max(4, args.workers)))]
+    warm.run_async(warm_jobs) if args.serve_mode == "async" else warm.run(warm_jobs)
     log(f"[bench] serving {args.jobs} jobs at {args.rate} jobs/s, budget {args.budget_gib} GiB, "
-        f"{args.workers} proving threads, {args.tables_per_tick} tables per tick")
+        f"{args.workers} proving threads, {args.serve_mode} dispatch")
     svc = serve.ProvingService(budget, ex, tables_per_tick=args.tables_per_tick)
-    st = svc.run(arrivals)
+    st = svc.run_async(arrivals) if args.serve_mode == "async" else svc.run(arrivals)
     ex.close()
     out = {
         "metric": "batched LDE+Keccak-Merkle Gcells/s vs per-table launch; p50/p99 proof latency",
@@ -358,7 +361,8 @@ def run_serve(args) -> dict:
             "mean_jobs_per_commit_batch": round(st["mean_jobs_per_commit_batch"], 3),
             "lde_cells": st["lde_cells"],
             "workers": args.workers,
-            "tables_per_tick": args.tables_per_tick,
+            "dispatch": args.serve_mode,
+            "tables_per_tick": args.tables_per_tick if args.serve_mode == "tick" else None,
         },
     }
     return out

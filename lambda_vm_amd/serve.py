@@ -31,7 +31,7 @@ import math
 import random
 import time
 from collections import deque
-from concurrent.futures import ThreadPoolExecutor
+from concurrent.futures import FIRST_COMPLETED, ThreadPoolExecutor, wait
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -297,6 +297,67 @@ class ProvingService:
                     wait = pending[0][0] * time_scale - (self.clock() - t0)
                     if wait > 0:
                         time.sleep(min(wait, 0.05))
+        makespan = max(js.finished for js in self.done) - t0
+        return self.stats(makespan)
+
+    def run_async(self, arrivals: list, time_scale: float = 1.0) -> dict:
+        """Work-conserving form of run(): no tick barrier. The executor's worker
+        pool (`ex.pool`, `ex.workers` threads) receives the commit of each
+        admitted batch as soon as it is admitted and a table's remaining rounds
+        as soon as a worker is free, oldest job's tables first; at most
+        workers - 1 table tasks are in flight so that a commit never queues
+        behind them. A job's tables start when its commit has finished (they
+        fork its transcript). Same admission rule and accounting as run()."""
+        pool, workers = getattr(self.ex, "pool", None), getattr(self.ex, "workers", 1)
+        if pool is None:
+            raise ValueError("run_async needs an executor with a worker pool")
+        t0 = self.clock()
+        pending = deque(sorted(arrivals, key=lambda a: a[0]))
+        total = len(pending)
+        ready: deque = deque()
+        inflight = {}
+        tables_in_flight = 0
+        table_slots = max(1, workers - 1)
+        while len(self.done) < total:
+            now = self.clock() - t0
+            while pending and pending[0][0] * time_scale <= now:
+                t, spec = pending.popleft()
+                self.submit(spec, t0 + t * time_scale)
+            admitted = self._admit(self.clock())
+            if admitted:
+                self.ticks += 1
+                self.batch_sizes.append(len(admitted))
+                inflight[pool.submit(self.ex.commit, admitted)] = ("commit", admitted)
+            while ready and tables_in_flight < table_slots:
+                js, ti = ready.popleft()
+                inflight[pool.submit(self.ex.prove_table, js, ti)] = ("table", (js, ti))
+                tables_in_flight += 1
+            wait_s = 0.05
+            if pending:
+                wait_s = min(wait_s, max(0.0, pending[0][0] * time_scale - (self.clock() - t0)))
+            if not inflight:
+                if pending and wait_s > 0:
+                    time.sleep(wait_s)
+                continue
+            finished, _ = wait(list(inflight), timeout=wait_s, return_when=FIRST_COMPLETED)
+            for f in finished:
+                kind, payload = inflight.pop(f)
+                res = f.result()
+                if kind == "commit":
+                    for js in payload:
+                        self.running.append(js)
+                        ready.extend((js, ti) for ti in range(len(js.spec.tables)))
+                        js.next_table = len(js.spec.tables)
+                    continue
+                tables_in_flight -= 1
+                js, ti = payload
+                js.digests[ti] = res
+                js.tables_done += 1
+                if js.tables_done == len(js.spec.tables):
+                    js.finished = self.clock()
+                    self.running.remove(js)
+                    self.in_use -= js.hbm
+                    self.done.append(js)
         makespan = max(js.finished for js in self.done) - t0
         return self.stats(makespan)
 

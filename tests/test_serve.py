@@ -114,3 +114,49 @@ def test_oldest_first_policy_lowers_median_latency():
         # same work either way: every job proved every table
         assert all(len(js.digests) == len(js.spec.tables) for js in svc.done)
     assert stats["oldest"]["p50_ms"] <= stats["round_robin"]["p50_ms"]
+
+
+class ThreadedFakeExecutor:
+    """A worker pool doing token work (a short sleep) per commit and table."""
+
+    def __init__(self, workers=3):
+        from concurrent.futures import ThreadPoolExecutor
+        self.workers = workers
+        self.pool = ThreadPoolExecutor(max_workers=workers)
+        self.commits = []
+
+    def commit(self, jobs):
+        import time
+        self.commits.append([js.spec.job_id for js in jobs])
+        time.sleep(0.0005)
+        for js in jobs:
+            js.handles = [object() for _ in js.spec.tables]
+            js.roots = [b"r" * 32 for _ in js.spec.tables]
+
+    def prove_table(self, js, ti):
+        import time
+        assert js.handles, "table proved before its job was committed"
+        time.sleep(0.0002)
+        return bytes([js.spec.job_id % 256, ti])
+
+
+def test_async_dispatch_completes_every_job_within_budget():
+    arrivals = serve.synth_jobs(rate=2000.0, count=40, seed=5, max_log_cells=22)
+    budget = 2 * max(serve.job_hbm_bytes(s) for _, s in arrivals)
+    ex = ThreadedFakeExecutor()
+    svc = serve.ProvingService(budget, ex)
+    st = svc.run_async(arrivals)
+    ex.pool.shutdown()
+    assert st["jobs"] == 40 and svc.in_use == 0 and svc.peak_in_use <= budget
+    order = [j for batch in ex.commits for j in batch]
+    assert order == sorted(order)  # FIFO admission
+    for js in svc.done:
+        assert js.finished >= js.admitted >= js.arrival
+        assert js.digests == [bytes([js.spec.job_id % 256, ti]) for ti in range(len(js.spec.tables))]
+    assert st["p99_ms"] >= st["p50_ms"] > 0
+
+
+def test_async_dispatch_needs_a_pool():
+    svc, _ = _service(1 << 40)
+    with pytest.raises(ValueError):
+        svc.run_async([(0.0, serve.JobSpec(0, [(256, 8)]))])
