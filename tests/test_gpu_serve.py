@@ -1,7 +1,8 @@
 """Proving service on the GPU: each job's result digest is the same alone and
 in a mix of concurrently admitted jobs, the same with tables proved on one
-host thread and on a pool of threads (one library stream each), and its
-committed roots equal the oracle's commitment of the same synthetic tables."""
+host thread and on a pool of threads (one library stream each), the same
+under tick and work-conserving dispatch, and its committed roots equal the
+oracle's commitment of the same synthetic tables."""
 import numpy as np
 import pytest
 
@@ -47,6 +48,20 @@ def test_thread_pool_matches_single_thread(gpu):
         digests[workers] = {js.spec.job_id: js.digest() for js in svc.done}
         assert all(all(d is not None for d in js.digests) for js in svc.done)
     assert digests[1] == digests[4]
+
+
+def test_async_dispatch_matches_tick(gpu):
+    """Work-conserving dispatch (commits and tables on the pool as threads free
+    up) gives every job the digest the tick form gives it."""
+    jobs = _jobs()
+    ex = serve.GpuExecutor(offset=3, fri_k=3, num_queries=8, workers=4)
+    tick = serve.ProvingService(1 << 40, ex)
+    tick.run([(0.0, spec) for spec in jobs])
+    asy = serve.ProvingService(1 << 40, ex)
+    st = asy.run_async([(0.002 * j, spec) for j, spec in enumerate(jobs)])
+    ex.close()
+    assert st["jobs"] == len(jobs) and asy.in_use == 0
+    assert {js.spec.job_id: js.digest() for js in asy.done} == {js.spec.job_id: js.digest() for js in tick.done}
 
 
 def test_service_roots_match_oracle(gpu, ora):
