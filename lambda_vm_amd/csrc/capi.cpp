@@ -107,8 +107,11 @@ int lde_core(Device *d, const uint64_t *d_trace, size_t n, size_t w, size_t blow
     }
     LVM_HIP(hipGetLastError());
     uint8_t root[32];
-    LVM_HIP(hipMemcpyAsync(root, nodes.as<uint8_t>(), 32, hipMemcpyDeviceToHost, st));
-    LVM_HIP(hipStreamSynchronize(st));
+    {
+        StageScope ss(st);
+        LVM_HIP(stage_download(root, nodes.as<uint8_t>(), 32, st));
+        LVM_HIP(stage_sync(st));
+    }
     lvm_lde *h = new lvm_lde();
     h->device = d->id;
     h->lde = (uint64_t *)lde.release();
@@ -328,8 +331,11 @@ int lvm_commit_rows_dev(const uint64_t *dev_rows, size_t N, size_t w, size_t tra
     }
     LVM_HIP(hipGetLastError());
     uint8_t root[32];
-    LVM_HIP(hipMemcpyAsync(root, nodes.as<uint8_t>(), 32, hipMemcpyDeviceToHost, st));
-    LVM_HIP(hipStreamSynchronize(st));
+    {
+        StageScope ss(st);
+        LVM_HIP(stage_download(root, nodes.as<uint8_t>(), 32, st));
+        LVM_HIP(stage_sync(st));
+    }
     lvm_lde *h = new lvm_lde();
     h->device = d->id;
     h->lde = (uint64_t *)rows.release();
@@ -411,13 +417,14 @@ int lvm_lde_open(const lvm_lde *h, const uint64_t *positions, size_t count, uint
     TRY(dp.alloc(count * 8, st));
     TRY(dpaths.alloc(count * (depth ? depth : 1) * 32, st));
     if (rows_out) TRY(drows.alloc(count * 2 * h->w * 8, st));
-    LVM_HIP(hipMemcpyAsync(dp.p, positions, count * 8, hipMemcpyHostToDevice, st));
+    StageScope ss(st);
+    LVM_HIP(stage_upload(dp.p, positions, count * 8, st));
     launch_open((const uint64_t *)h->nodes, h->L, h->lde, h->w, dp.as<uint64_t>(), count, depth,
                 dpaths.as<uint64_t>(), rows_out ? drows.as<uint64_t>() : nullptr, st);
     LVM_HIP(hipGetLastError());
-    if (depth) LVM_HIP(hipMemcpyAsync(paths_out, dpaths.p, count * depth * 32, hipMemcpyDeviceToHost, st));
-    if (rows_out) LVM_HIP(hipMemcpyAsync(rows_out, drows.p, count * 2 * h->w * 8, hipMemcpyDeviceToHost, st));
-    LVM_HIP(hipStreamSynchronize(st));
+    if (depth) LVM_HIP(stage_download(paths_out, dpaths.p, count * depth * 32, st));
+    if (rows_out) LVM_HIP(stage_download(rows_out, drows.p, count * 2 * h->w * 8, st));
+    LVM_HIP(stage_sync(st));
     return LVM_OK;
 }
 

@@ -620,7 +620,8 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
         launch_transpose(main->lde, cm_main, N, (uint32_t)main->w, 1, st);
         if (aux) launch_transpose(aux->lde, cm_aux, N, (uint32_t)(aux->w / 3), 3, st);
     }
-    hipError_t e = hipMemcpyAsync(db, blob.data(), blob.size(), hipMemcpyHostToDevice, st);
+    StageScope ss(st);
+    hipError_t e = stage_upload(db, blob.data(), blob.size(), st);
     a.nodes = (const uint64_t *)(db + o_nodes);
     a.num_nodes = (uint32_t)pg->num_nodes;
     a.num_roots = (uint32_t)pg->num_roots;
@@ -683,7 +684,7 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
         }
     }
     if (e == hipSuccess) e = hipGetLastError();
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = stage_sync(st);
     release();
     if (e != hipSuccess) return fail(LVM_ERR_DEVICE, std::string("constraint evaluation failed: ") + hipGetErrorString(e));
     return LVM_OK;

@@ -7,6 +7,8 @@
 #include <unordered_map>
 #include <vector>
 
+#include <cstring>
+
 #include "lvm_internal.hpp"
 #include "prof.hpp"
 
@@ -292,6 +294,109 @@ void lvm_release_cached() {
         }
     c->idle.clear();
     hipStreamSynchronize(st);
+}
+
+// ---------------- pinned staging ----------------
+namespace {
+constexpr size_t STAGE_DIRECT = 4u << 20;  // larger transfers bypass the area
+struct PendingDownload {
+    void *dst;
+    size_t off, bytes;
+};
+struct Stage {
+    uint8_t *buf = nullptr;
+    size_t cap = 0, head = 0;
+    hipStream_t last = nullptr;  // stream of the copies queued since the area was last drained
+    std::vector<PendingDownload> pend;
+};
+// The area is never freed (a thread's destructors may run after the runtime
+// has shut down); a thread that ends leaves it to the process.
+thread_local Stage t_stage;
+
+void deliver(Stage &sg) {
+    for (auto &pd : sg.pend) std::memcpy(pd.dst, sg.buf + pd.off, pd.bytes);
+    sg.pend.clear();
+    sg.head = 0;
+    sg.last = nullptr;
+}
+
+// A region of `bytes` in the area; drains or grows it first when needed.
+hipError_t stage_region(Stage &sg, size_t bytes, hipStream_t st, size_t *off) {
+    const size_t need = (bytes + 255) & ~(size_t)255;
+    if (sg.head && (sg.last != st || sg.head + need > sg.cap)) {
+        // queued copies still use the area: wait for them and deliver
+        hipError_t e = hipStreamSynchronize(sg.last);
+        if (e != hipSuccess) return e;
+        deliver(sg);
+    }
+    if (need > sg.cap) {
+        if (sg.buf) hipHostFree(sg.buf);
+        sg.buf = nullptr;
+        size_t cap = sg.cap ? 2 * sg.cap : (size_t)256 << 10;
+        while (cap < need) cap *= 2;
+        hipError_t e = hipHostMalloc((void **)&sg.buf, cap, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            sg.cap = 0;
+            return e;
+        }
+        sg.cap = cap;
+    }
+    *off = sg.head;
+    sg.head += need;
+    sg.last = st;
+    return hipSuccess;
+}
+}  // namespace
+
+hipError_t stage_upload(void *dst_dev, const void *src_host, size_t bytes, hipStream_t st) {
+    if (!bytes) return hipSuccess;
+    if (bytes > STAGE_DIRECT) return hipMemcpyAsync(dst_dev, src_host, bytes, hipMemcpyHostToDevice, st);
+    Stage &sg = t_stage;
+    size_t off = 0;
+    hipError_t e = stage_region(sg, bytes, st, &off);
+    if (e != hipSuccess) return e;
+    std::memcpy(sg.buf + off, src_host, bytes);
+    return hipMemcpyAsync(dst_dev, sg.buf + off, bytes, hipMemcpyHostToDevice, st);
+}
+
+hipError_t stage_download(void *dst_host, const void *src_dev, size_t bytes, hipStream_t st) {
+    if (!bytes) return hipSuccess;
+    if (bytes > STAGE_DIRECT) return hipMemcpyAsync(dst_host, src_dev, bytes, hipMemcpyDeviceToHost, st);
+    Stage &sg = t_stage;
+    size_t off = 0;
+    hipError_t e = stage_region(sg, bytes, st, &off);
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(sg.buf + off, src_dev, bytes, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) sg.pend.push_back(PendingDownload{dst_host, off, bytes});
+    return e;
+}
+
+hipError_t stage_sync(hipStream_t st) {
+    Stage &sg = t_stage;
+    hipError_t e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    if (sg.head && sg.last && sg.last != st) {
+        e = hipStreamSynchronize(sg.last);
+        if (e != hipSuccess) return e;
+    }
+    deliver(sg);
+    return hipSuccess;
+}
+
+StageScope::StageScope(hipStream_t s) : st(s), mark(t_stage.pend.size()) {}
+
+StageScope::~StageScope() {
+    Stage &sg = t_stage;
+    if (sg.pend.size() > mark) {
+        // an early return left downloads queued: let the copies land in the
+        // area, then forget them (their destinations may no longer exist)
+        if (sg.last) hipStreamSynchronize(sg.last);
+        sg.pend.resize(mark);
+        if (sg.pend.empty()) {
+            sg.head = 0;
+            sg.last = nullptr;
+        }
+    }
 }
 
 // ---------------- profiler ----------------

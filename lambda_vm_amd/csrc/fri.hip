@@ -197,9 +197,10 @@ static int fri_fold_impl(lvm_fri *f, const uint64_t zeta[3], bool commit, uint8_
         ProfScope ps("fri_merkle", st);
         merkle_build_inner(layer.nodes, layer.L, st);
     }
+    StageScope ss(st);
     hipError_t e = hipGetLastError();
-    if (e == hipSuccess && commit) e = hipMemcpyAsync(layer.root, layer.nodes, 32, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess && commit) e = stage_download(layer.root, layer.nodes, 32, st);
+    if (e == hipSuccess) e = stage_sync(st);
     if (e != hipSuccess) {
         lvm_free(out);
         if (layer.nodes) lvm_free(layer.nodes);
@@ -234,8 +235,9 @@ int lvm_fri_current(const lvm_fri *f, uint64_t *host_out, size_t *len, uint64_t 
     if (!host_out) return LVM_OK;
     Device *d = device_state();
     if (!d) return LVM_ERR_DEVICE;
-    LVM_HIP(hipMemcpyAsync(host_out, f->cur, f->len * 24, hipMemcpyDeviceToHost, thread_stream(d)));
-    LVM_HIP(hipStreamSynchronize(thread_stream(d)));
+    StageScope ss(thread_stream(d));
+    LVM_HIP(stage_download(host_out, f->cur, f->len * 24, thread_stream(d)));
+    LVM_HIP(stage_sync(thread_stream(d)));
     return LVM_OK;
 }
 
@@ -273,7 +275,8 @@ int lvm_fri_query(const lvm_fri *f, const uint64_t *iotas, size_t count, uint64_
     LVM_HIP(lvm_malloc(&dio, count * 8));
     LVM_HIP(lvm_malloc(&dsym, f->layers.size() * count * 24));
     LVM_HIP(lvm_malloc(&dpaths, (path_words ? path_words : 1) * 8));
-    LVM_HIP(hipMemcpyAsync(dio, iotas, count * 8, hipMemcpyHostToDevice, st));
+    StageScope ss(st);
+    LVM_HIP(stage_upload(dio, iotas, count * 8, st));
     size_t poff = 0;
     for (size_t li = 0; li < f->layers.size(); li++) {
         const FriLayer &l = f->layers[li];
@@ -284,9 +287,9 @@ int lvm_fri_query(const lvm_fri *f, const uint64_t *iotas, size_t count, uint64_
         poff += count * depth * 4;
     }
     LVM_HIP(hipGetLastError());
-    LVM_HIP(hipMemcpyAsync(evals_sym_out, dsym, f->layers.size() * count * 24, hipMemcpyDeviceToHost, st));
-    if (path_words) LVM_HIP(hipMemcpyAsync(paths_out, dpaths, path_words * 8, hipMemcpyDeviceToHost, st));
-    LVM_HIP(hipStreamSynchronize(st));
+    LVM_HIP(stage_download(evals_sym_out, dsym, f->layers.size() * count * 24, st));
+    if (path_words) LVM_HIP(stage_download(paths_out, dpaths, path_words * 8, st));
+    LVM_HIP(stage_sync(st));
     lvm_free(dio);
     lvm_free(dsym);
     lvm_free(dpaths);

@@ -92,7 +92,8 @@ extern "C" int lvm_generate_nonce(const uint8_t seed[32], uint8_t grinding_facto
         return fail(LVM_ERR_NOMEM, "device allocation for the nonce search failed");
     }
     std::vector<uint64_t> mins(BLOCKS);
-    hipError_t e = hipMemcpyAsync(d_inner, inner, 32, hipMemcpyHostToDevice, st);
+    StageScope ss(st);
+    hipError_t e = stage_upload(d_inner, inner, 32, st);
     uint64_t best = ~0ull;
     for (uint64_t base = 0; e == hipSuccess && base < max_nonce; base += BATCH) {
         const uint64_t count = max_nonce - base < BATCH ? max_nonce - base : BATCH;
@@ -103,8 +104,8 @@ extern "C" int lvm_generate_nonce(const uint8_t seed[32], uint8_t grinding_facto
                                limit, d_min);
         }
         e = hipGetLastError();
-        if (e == hipSuccess) e = hipMemcpyAsync(mins.data(), d_min, blocks * 8, hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e == hipSuccess) e = stage_download(mins.data(), d_min, blocks * 8, st);
+        if (e == hipSuccess) e = stage_sync(st);
         if (e != hipSuccess) break;
         for (uint64_t b = 0; b < blocks; b++) best = mins[b] < best ? mins[b] : best;
         if (best != ~0ull) break;

@@ -302,7 +302,8 @@ int logup_aux_impl(const uint64_t *dev_main, size_t row_stride, size_t col_strid
         cleanup();
         return fail(LVM_ERR_NOMEM, "device allocation for the LogUp build failed");
     }
-    hipError_t e = hipMemcpyAsync(db, blob.data(), blob.size(), hipMemcpyHostToDevice, st);
+    StageScope ss(st);
+    hipError_t e = stage_upload(db, blob.data(), blob.size(), st);
     LogupDev ld;
     ld.bus_ids = (const uint64_t *)(db + o_bus);
     ld.elem_offsets = (const uint32_t *)(db + o_eoff);
@@ -338,8 +339,8 @@ int logup_aux_impl(const uint64_t *dev_main, size_t row_stride, size_t col_strid
     }
     if (e == hipSuccess) e = hipGetLastError();
     uint64_t hl[6];
-    if (e == hipSuccess) e = hipMemcpyAsync(hl, lo, 48, hipMemcpyDeviceToHost, st);
-    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = stage_download(hl, lo, 48, st);
+    if (e == hipSuccess) e = stage_sync(st);
     cleanup();
     if (e != hipSuccess) return fail(LVM_ERR_DEVICE, std::string("LogUp build failed: ") + hipGetErrorString(e));
     L_out[0] = hl[0];

@@ -69,6 +69,24 @@ inline hipError_t lvm_malloc(T **p, size_t bytes) {
     return lvm_malloc_bytes((void **)p, bytes);
 }
 
+// Small host<->device transfers through a per-thread pinned staging area: a
+// copy of pageable memory is staged by the runtime at tens of microseconds
+// each, a copy of pinned memory is a plain DMA. stage_upload copies the host
+// data into the area at the call and queues the DMA; stage_download queues the
+// DMA into the area and the bytes reach dst_host at stage_sync, which
+// synchronises the stream. A StageScope around an entry point drops (after a
+// stream synchronisation) the downloads it queued but never delivered, so an
+// early error return leaves nothing pending. Transfers above 4 MiB go direct.
+hipError_t stage_upload(void *dst_dev, const void *src_host, size_t bytes, hipStream_t st);
+hipError_t stage_download(void *dst_host, const void *src_dev, size_t bytes, hipStream_t st);
+hipError_t stage_sync(hipStream_t st);
+struct StageScope {
+    explicit StageScope(hipStream_t st);
+    ~StageScope();
+    hipStream_t st;
+    size_t mark;
+};
+
 #define LVM_HIP(expr)                                                                                   \
     do {                                                                                                \
         hipError_t _e = (expr);                                                                         \

@@ -165,29 +165,32 @@ static void launch_pole_inverse(uint64_t *out, uint64_t rows, uint32_t log_N, ui
 }
 
 constexpr int BARY_MAX_K = 8;
-constexpr int BARY_ROWS_PER_BLOCK = 1024;
+constexpr uint64_t BARY_MAX_ROWS_PER_BLOCK = 1024;
 
 // partial[chunk][k][c] = sum over the chunk's rows p of lde[p][c] * scale[k][p]
-// block: 256 threads = 4 row lanes x 64 columns; grid (chunks, column groups).
-// The three components of each point's sum are delayed-reduction dot
-// products (one 128-bit accumulate per product, one reduction per thread).
+// block: 256 threads = 4 row lanes x 64 columns; grid (chunks, column groups),
+// a chunk being `rpb` consecutive rows (fewer for small tables, so that the
+// grid still has enough blocks; each thread walks rpb/4 rows, loads of four
+// rows issued together). The three components of each point's sum are
+// delayed-reduction dot products (one 128-bit accumulate per product, one
+// reduction per thread).
 template <int K>
 __global__ __launch_bounds__(256) void bary_partial_kernel(const uint64_t *__restrict__ lde, uint64_t pitch,
-                                                           uint64_t w, uint64_t n, const uint64_t *__restrict__ scale,
+                                                           uint64_t w, uint64_t n, uint64_t rpb,
+                                                           const uint64_t *__restrict__ scale,
                                                            uint64_t *__restrict__ partial) {
     __shared__ uint64_t red[4][K][64][3];
     const int lane = threadIdx.x & 63, rl = threadIdx.x >> 6;
     const uint64_t c = (uint64_t)blockIdx.y * 64 + lane;
-    const uint64_t p0 = (uint64_t)blockIdx.x * BARY_ROWS_PER_BLOCK;
+    const uint64_t p0 = (uint64_t)blockIdx.x * rpb;
     acc128 acc[K][3];
 #pragma unroll
     for (int k = 0; k < K; k++)
 #pragma unroll
         for (int j = 0; j < 3; j++) acc_zero(acc[k][j]);
     if (c < w) {
-        const uint64_t pend = p0 + BARY_ROWS_PER_BLOCK < n ? p0 + BARY_ROWS_PER_BLOCK : n;
-        for (uint64_t p = p0 + rl; p < pend; p += 4) {
-            uint64_t f = lde[p * pitch + c];
+        const uint64_t pend = p0 + rpb < n ? p0 + rpb : n;
+        auto row = [&](uint64_t p, uint64_t f) {
             // p is the same for the 64 lanes of a wave: read the scales through
             // the scalar path once per wave instead of 64 identical vector loads
             const uint64_t pu = ((uint64_t)__builtin_amdgcn_readfirstlane((uint32_t)(p >> 32)) << 32) |
@@ -199,7 +202,17 @@ __global__ __launch_bounds__(256) void bary_partial_kernel(const uint64_t *__res
                 acc_mad(acc[k][1], s[1], f);
                 acc_mad(acc[k][2], s[2], f);
             }
+        };
+        uint64_t p = p0 + rl;
+        for (; p + 12 < pend; p += 16) {
+            const uint64_t f0 = lde[p * pitch + c], f1 = lde[(p + 4) * pitch + c];
+            const uint64_t f2 = lde[(p + 8) * pitch + c], f3 = lde[(p + 12) * pitch + c];
+            row(p, f0);
+            row(p + 4, f1);
+            row(p + 8, f2);
+            row(p + 12, f3);
         }
+        for (; p < pend; p += 4) row(p, lde[p * pitch + c]);
     }
 #pragma unroll
     for (int k = 0; k < K; k++)
@@ -217,8 +230,8 @@ __global__ __launch_bounds__(256) void bary_partial_kernel(const uint64_t *__res
 
 template <int K>
 static void launch_bary_partial(dim3 grid, hipStream_t st, const uint64_t *lde, uint64_t w, uint64_t n,
-                                const uint64_t *scale, uint64_t *partial) {
-    hipLaunchKernelGGL(bary_partial_kernel<K>, grid, dim3(256), 0, st, lde, w, w, n, scale, partial);
+                                uint64_t rpb, const uint64_t *scale, uint64_t *partial) {
+    hipLaunchKernelGGL(bary_partial_kernel<K>, grid, dim3(256), 0, st, lde, w, w, n, rpb, scale, partial);
 }
 
 // out[k][c] = vf[k] * sum over chunks of partial[chunk][k][c]; one block per
@@ -431,7 +444,12 @@ int lvm_barycentric_base_multi(const lvm_lde *h, uint64_t coset_offset, const ui
     const unsigned logN = log2u(h->N);
     Scratch s;
     uint64_t *dz, *scale, *partial, *vfd, *dout;
-    const uint64_t chunks = (n + BARY_ROWS_PER_BLOCK - 1) / BARY_ROWS_PER_BLOCK;
+    // rows per block: 1024, fewer (down to 64) when the grid would have under
+    // ~1024 blocks
+    const uint64_t cgroups = (w + 63) / 64;
+    uint64_t rpb = BARY_MAX_ROWS_PER_BLOCK;
+    while (rpb > 64 && ((n + rpb - 1) / rpb) * cgroups < 1024) rpb /= 2;
+    const uint64_t chunks = (n + rpb - 1) / rpb;
     TRYS(s.get(&dz, K * 24));
     TRYS(s.get(&scale, K * n * 24));
     TRYS(s.get(&partial, chunks * K * w * 24));
@@ -447,8 +465,9 @@ int lvm_barycentric_base_multi(const lvm_lde *h, uint64_t coset_offset, const ui
         vf[3 * k + 1] = v.c1;
         vf[3 * k + 2] = v.c2;
     }
-    LVM_HIP(hipMemcpyAsync(dz, z, K * 24, hipMemcpyHostToDevice, st));
-    LVM_HIP(hipMemcpyAsync(vfd, vf.data(), K * 24, hipMemcpyHostToDevice, st));
+    StageScope ss(st);
+    LVM_HIP(stage_upload(dz, z, K * 24, st));
+    LVM_HIP(stage_upload(vfd, vf.data(), K * 24, st));
     {
         ProfScope ps("bary_denoms", st);
         // scale[k][p] = x_p / (z_k - x_p)
@@ -458,21 +477,21 @@ int lvm_barycentric_base_multi(const lvm_lde *h, uint64_t coset_offset, const ui
         ProfScope ps("bary_sum", st);
         dim3 grid((unsigned)chunks, (unsigned)((w + 63) / 64));
         switch (K) {
-        case 1: launch_bary_partial<1>(grid, st, h->lde, w, n, scale, partial); break;
-        case 2: launch_bary_partial<2>(grid, st, h->lde, w, n, scale, partial); break;
-        case 3: launch_bary_partial<3>(grid, st, h->lde, w, n, scale, partial); break;
-        case 4: launch_bary_partial<4>(grid, st, h->lde, w, n, scale, partial); break;
-        case 5: launch_bary_partial<5>(grid, st, h->lde, w, n, scale, partial); break;
-        case 6: launch_bary_partial<6>(grid, st, h->lde, w, n, scale, partial); break;
-        case 7: launch_bary_partial<7>(grid, st, h->lde, w, n, scale, partial); break;
-        default: launch_bary_partial<8>(grid, st, h->lde, w, n, scale, partial); break;
+        case 1: launch_bary_partial<1>(grid, st, h->lde, w, n, rpb, scale, partial); break;
+        case 2: launch_bary_partial<2>(grid, st, h->lde, w, n, rpb, scale, partial); break;
+        case 3: launch_bary_partial<3>(grid, st, h->lde, w, n, rpb, scale, partial); break;
+        case 4: launch_bary_partial<4>(grid, st, h->lde, w, n, rpb, scale, partial); break;
+        case 5: launch_bary_partial<5>(grid, st, h->lde, w, n, rpb, scale, partial); break;
+        case 6: launch_bary_partial<6>(grid, st, h->lde, w, n, rpb, scale, partial); break;
+        case 7: launch_bary_partial<7>(grid, st, h->lde, w, n, rpb, scale, partial); break;
+        default: launch_bary_partial<8>(grid, st, h->lde, w, n, rpb, scale, partial); break;
         }
         hipLaunchKernelGGL(bary_combine_kernel, dim3((unsigned)(K * w)), dim3(256), 0, st, partial, chunks,
                            (uint32_t)K, w, vfd, dout);
     }
     LVM_HIP(hipGetLastError());
-    LVM_HIP(hipMemcpyAsync(out, dout, K * w * 24, hipMemcpyDeviceToHost, st));
-    LVM_HIP(hipStreamSynchronize(st));
+    LVM_HIP(stage_download(out, dout, K * w * 24, st));
+    LVM_HIP(stage_sync(st));
     return LVM_OK;
 }
 
@@ -544,13 +563,14 @@ int lvm_deep_composition(const lvm_lde *main, const lvm_lde *aux, const lvm_lde 
     TRYS(s.get(&dconsts, (K + 1) * 24));
     TRYS(s.get(&dgtr, cols * K * 24));
     TRYS(s.get(&drp, K * 24));
-    LVM_HIP(hipMemcpyAsync(dpoles, poles.data(), npoles * 24, hipMemcpyHostToDevice, st));
-    LVM_HIP(hipMemcpyAsync(dconsts, consts.data(), (K + 1) * 24, hipMemcpyHostToDevice, st));
-    LVM_HIP(hipMemcpyAsync(dgtr, gammas_tr, cols * K * 24, hipMemcpyHostToDevice, st));
-    LVM_HIP(hipMemcpyAsync(drp, rp.data(), K * 24, hipMemcpyHostToDevice, st));
+    StageScope ss(st);
+    LVM_HIP(stage_upload(dpoles, poles.data(), npoles * 24, st));
+    LVM_HIP(stage_upload(dconsts, consts.data(), (K + 1) * 24, st));
+    LVM_HIP(stage_upload(dgtr, gammas_tr, cols * K * 24, st));
+    LVM_HIP(stage_upload(drp, rp.data(), K * 24, st));
     if (parts) {
         TRYS(s.get(&dgh, np * 24));
-        LVM_HIP(hipMemcpyAsync(dgh, gammas_h, np * 24, hipMemcpyHostToDevice, st));
+        LVM_HIP(stage_upload(dgh, gammas_h, np * 24, st));
     }
     {
         ProfScope ps("deep_denoms", st);
@@ -571,7 +591,7 @@ int lvm_deep_composition(const lvm_lde *main, const lvm_lde *aux, const lvm_lde 
         }
     }
     LVM_HIP(hipGetLastError());
-    LVM_HIP(hipStreamSynchronize(st));
+    LVM_HIP(stage_sync(st));
     return LVM_OK;
 }
 
