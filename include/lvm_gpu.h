@@ -413,6 +413,36 @@ int lvm_fri_release(lvm_fri *f);
 int lvm_fri_commit_phase(lvm_fri *f, lvm_transcript *t, unsigned blowup_log, unsigned k, uint8_t *roots_out,
                          size_t *num_committed_out, uint64_t *coeffs_out, size_t *num_coeffs_out);
 
+/* ---------------- rounds of a committed table ---------------- */
+/* Caller-provided outputs of lvm_prove_committed_table, for a table of m
+ * base columns (the LDE's words per row), N = n * blowup LDE rows, Q queries,
+ * K = 2 frame points; layer l of FRI (l < num_fri_roots) has N / 2^(l+1)
+ * values and a tree of depth log2(N) - l - 2. */
+typedef struct lvm_table_rounds {
+    uint64_t z[3];            /* out-of-domain point */
+    uint64_t *ood;            /* K x m x 3: the columns at z and z * w_n */
+    uint8_t *fri_roots;       /* room for log2(N) x 32 bytes */
+    size_t num_fri_roots;     /* set by the call */
+    uint64_t *fri_coeffs;     /* room for 2^(log2(blowup) + fri_k) x 3 */
+    size_t num_fri_coeffs;    /* set by the call */
+    uint64_t *iotas;          /* Q query indexes */
+    uint64_t *fri_sym;        /* layers x Q x 3 */
+    uint8_t *fri_paths;       /* per layer, Q paths of its depth (layer-major) */
+    uint8_t *trace_paths;     /* Q x (log2(N) - 1) x 32 */
+    uint64_t *trace_rows;     /* Q x 2 x m */
+} lvm_table_rounds;
+/* Rounds 3-4 of a table whose LDE `h` is committed and whose root the
+ * transcript has absorbed: the out-of-domain point and values
+ * (crypto/stark/src/prover.rs:4547-4560), DEEP weights, codeword and FRI
+ * commit phase (prover.rs:2159-2333, fri/mod.rs:27-151), query indexes, FRI
+ * query phase and trace openings, in one call. The same sequence the host
+ * can drive through lvm_barycentric_base_multi, lvm_deep_composition,
+ * lvm_fri_commit_phase, lvm_fri_query and lvm_lde_open, without host round
+ * trips between them. No counterpart in the reference, which drives these
+ * steps from its prover loop (crypto/stark/src/prover.rs:2159-2333). */
+int lvm_prove_committed_table(const lvm_lde *h, lvm_transcript *t, size_t n, unsigned blowup, uint64_t offset,
+                              unsigned fri_k, size_t num_queries, lvm_table_rounds *rounds);
+
 /* ---------------- profiling ---------------- */
 /* When enabled, the library brackets its kernels with HIP events on the
  * stream they run on; query returns the count and summed duration of the

@@ -112,6 +112,8 @@ SIGNATURES = {
     "lvm_fri_query": (C.c_int, [C.c_void_p, u64p, C.c_size_t, u64p, u8p]),
     "lvm_fri_release": (C.c_int, [C.c_void_p]),
     "lvm_fri_commit_phase": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint, C.c_uint, u8p, szp, u64p, szp]),
+    "lvm_prove_committed_table": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_uint, C.c_uint64, C.c_uint,
+                                            C.c_size_t, C.c_void_p]),
     "lvm_prof_enable": (C.c_int, [C.c_int]),
     "lvm_prof_reset": (C.c_int, []),
     "lvm_prof_query": (C.c_int, [C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
@@ -149,6 +151,14 @@ class Composition(C.Structure):
                 ("root_group", C.c_void_p), ("group_exemptions", C.c_void_p), ("num_groups", C.c_size_t),
                 ("num_boundary", C.c_size_t), ("b_col", C.c_void_p), ("b_is_aux", C.c_void_p),
                 ("b_step", C.c_void_p), ("b_value", C.c_void_p), ("b_beta", C.c_void_p)]
+
+
+class TableRounds(C.Structure):
+    """lvm_table_rounds: caller-provided outputs of lvm_prove_committed_table."""
+    _fields_ = [("z", C.c_uint64 * 3), ("ood", C.c_void_p), ("fri_roots", C.c_void_p),
+                ("num_fri_roots", C.c_size_t), ("fri_coeffs", C.c_void_p), ("num_fri_coeffs", C.c_size_t),
+                ("iotas", C.c_void_p), ("fri_sym", C.c_void_p), ("fri_paths", C.c_void_p),
+                ("trace_paths", C.c_void_p), ("trace_rows", C.c_void_p)]
 
 
 def header_symbols() -> list[str]:

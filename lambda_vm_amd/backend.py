@@ -740,3 +740,43 @@ def fp3_powers(base, count: int) -> np.ndarray:
 def trim() -> None:
     """Hand the library's cached device buffers back to the driver pool."""
     N.call("lvm_trim")
+
+
+def prove_committed_table(h: GpuLde, transcript: Transcript, n: int, m: int, blowup: int, offset: int,
+                          fri_k: int = 6, num_queries: int = 30):
+    """Rounds 3-4 of a committed table in one call (lvm_prove_committed_table):
+    a pipeline.TableProof."""
+    from .pipeline import TableProof
+    N.ensure_device()
+    Nl = n * blowup
+    lde_log = Nl.bit_length() - 1
+    blog = blowup.bit_length() - 1
+    terminal_log = min(blog + fri_k, lde_log)
+    total_folds = lde_log - terminal_log
+    layers = total_folds - 1 if total_folds else 0
+    depths = [lde_log - l - 2 for l in range(layers)]
+    Q = num_queries
+    ood = np.zeros((2, m, 3), dtype=np.uint64)
+    roots = np.zeros((max(lde_log, 1), 32), dtype=np.uint8)
+    coeffs = np.zeros((1 << terminal_log, 3), dtype=np.uint64)
+    iotas = np.zeros(max(Q, 1), dtype=np.uint64)
+    sym = np.zeros((max(layers, 1), max(Q, 1), 3), dtype=np.uint64)
+    fpaths = np.zeros(max(sum(depths) * Q * 32, 1), dtype=np.uint8)
+    tdepth = (Nl // 2).bit_length() - 1
+    tpaths = np.zeros((max(Q, 1), max(tdepth, 1), 32), dtype=np.uint8)
+    trows = np.zeros((max(Q, 1), 2, m), dtype=np.uint64)
+    r = N.TableRounds()
+    r.ood, r.fri_roots, r.fri_coeffs = ood.ctypes.data, roots.ctypes.data, coeffs.ctypes.data
+    r.iotas, r.fri_sym, r.fri_paths = iotas.ctypes.data, sym.ctypes.data, fpaths.ctypes.data
+    r.trace_paths, r.trace_rows = tpaths.ctypes.data, trows.ctypes.data
+    N.call("lvm_prove_committed_table", h.h, transcript.h, n, blowup, offset, fri_k, Q, C.byref(r))
+    nl = r.num_fri_roots
+    if nl != layers:
+        raise LvmError("lvm_prove_committed_table", 5, f"FRI committed {nl} layers, expected {layers}")
+    paths, off = [], 0
+    for d in depths:
+        paths.append(fpaths[off:off + Q * d * 32].reshape(Q, d, 32))
+        off += Q * d * 32
+    return TableProof(h.root, [int(v) for v in r.z], ood, roots[:nl].copy(), coeffs[:r.num_fri_coeffs].copy(),
+                      iotas[:Q].copy(), sym[:nl, :Q].copy(), paths, tpaths[:Q, :tdepth], trows[:Q],
+                      transcript.state())

@@ -15,7 +15,7 @@ CSRC = os.path.join(PKG, "csrc")
 INCLUDE = os.path.join(ROOT, "include")
 LIB = os.path.join(PKG, "liblvm_gpu.so")
 
-SOURCES = ["ntt.hip", "merkle.hip", "misc.hip", "fri.hip", "ood.hip", "logup.hip", "compose.hip", "grinding.hip", "context.cpp", "compose_jit.cpp", "transcript.cpp", "capi.cpp", "batch.cpp"]
+SOURCES = ["ntt.hip", "merkle.hip", "misc.hip", "fri.hip", "ood.hip", "logup.hip", "compose.hip", "grinding.hip", "context.cpp", "compose_jit.cpp", "transcript.cpp", "capi.cpp", "batch.cpp", "rounds.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("LVM_OFFLOAD_ARCH", "gfx950")
 

@@ -70,9 +70,10 @@ __global__ __launch_bounds__(256) void batch_inverse_fp3_kernel(const uint64_t *
 struct PoleConst {
     uint64_t z0, z1, z2, c1x3, c3, c1, c2, z1s;
 };
-constexpr int POLE_E = 16;  // rows per thread
+// rows per thread E: 16 on large domains; 2 on small ones, where the chain of
+// E rows (not the grid) sets the kernel's duration
 
-template <int K>
+template <int K, int E>
 __global__ __launch_bounds__(256) void pole_inverse_kernel(uint64_t *__restrict__ out, uint64_t rows, uint32_t log_N,
                                                            uint64_t offset, const uint64_t *__restrict__ z, int mode,
                                                            const uint64_t *__restrict__ lo,
@@ -98,10 +99,10 @@ __global__ __launch_bounds__(256) void pole_inverse_kernel(uint64_t *__restrict_
         sq = sqr(a0);
         return sub(sub(mul(sq, a0), mul(c.c1x3, a0)), c.c3);
     };
-    uint64_t pre[POLE_E];
+    uint64_t pre[E];
     uint64_t acc = 1;
 #pragma unroll
-    for (int j = 0; j < POLE_E; j++) {
+    for (int j = 0; j < E; j++) {
         const uint64_t p = t + (uint64_t)j * T;
         if (p < rows) {
             const uint64_t x = point(p);
@@ -117,7 +118,7 @@ __global__ __launch_bounds__(256) void pole_inverse_kernel(uint64_t *__restrict_
     }
     uint64_t inv_acc = inv(acc);
 #pragma unroll
-    for (int j = POLE_E - 1; j >= 0; j--) {
+    for (int j = E - 1; j >= 0; j--) {
         const uint64_t p = t + (uint64_t)j * T;
         if (p >= rows) continue;
         const uint64_t x = point(p);
@@ -147,21 +148,30 @@ __global__ __launch_bounds__(256) void pole_inverse_kernel(uint64_t *__restrict_
     }
 }
 
+template <int E>
+static void launch_pole_inverse_e(uint64_t *out, uint64_t rows, uint32_t log_N, uint64_t offset, const uint64_t *z,
+                                  uint32_t K, int mode, const uint64_t *lo, const uint64_t *hi, hipStream_t st) {
+    const uint64_t threads = (rows + E - 1) / E;
+    const dim3 grid((unsigned)((threads + 255) / 256)), blk(256);
+    switch (K) {
+    case 1: hipLaunchKernelGGL((pole_inverse_kernel<1, E>), grid, blk, 0, st, out, rows, log_N, offset, z, mode, lo, hi); break;
+    case 2: hipLaunchKernelGGL((pole_inverse_kernel<2, E>), grid, blk, 0, st, out, rows, log_N, offset, z, mode, lo, hi); break;
+    case 3: hipLaunchKernelGGL((pole_inverse_kernel<3, E>), grid, blk, 0, st, out, rows, log_N, offset, z, mode, lo, hi); break;
+    case 4: hipLaunchKernelGGL((pole_inverse_kernel<4, E>), grid, blk, 0, st, out, rows, log_N, offset, z, mode, lo, hi); break;
+    case 5: hipLaunchKernelGGL((pole_inverse_kernel<5, E>), grid, blk, 0, st, out, rows, log_N, offset, z, mode, lo, hi); break;
+    case 6: hipLaunchKernelGGL((pole_inverse_kernel<6, E>), grid, blk, 0, st, out, rows, log_N, offset, z, mode, lo, hi); break;
+    case 7: hipLaunchKernelGGL((pole_inverse_kernel<7, E>), grid, blk, 0, st, out, rows, log_N, offset, z, mode, lo, hi); break;
+    default: hipLaunchKernelGGL((pole_inverse_kernel<8, E>), grid, blk, 0, st, out, rows, log_N, offset, z, mode, lo, hi); break;
+    }
+}
+
 static void launch_pole_inverse(uint64_t *out, uint64_t rows, uint32_t log_N, uint64_t offset, const uint64_t *z,
                                 uint32_t K, int mode, const uint64_t *lo, const uint64_t *hi, hipStream_t st) {
     if (!rows) return;
-    const uint64_t threads = (rows + POLE_E - 1) / POLE_E;
-    const dim3 grid((unsigned)((threads + 255) / 256)), blk(256);
-    switch (K) {
-    case 1: hipLaunchKernelGGL(pole_inverse_kernel<1>, grid, blk, 0, st, out, rows, log_N, offset, z, mode, lo, hi); break;
-    case 2: hipLaunchKernelGGL(pole_inverse_kernel<2>, grid, blk, 0, st, out, rows, log_N, offset, z, mode, lo, hi); break;
-    case 3: hipLaunchKernelGGL(pole_inverse_kernel<3>, grid, blk, 0, st, out, rows, log_N, offset, z, mode, lo, hi); break;
-    case 4: hipLaunchKernelGGL(pole_inverse_kernel<4>, grid, blk, 0, st, out, rows, log_N, offset, z, mode, lo, hi); break;
-    case 5: hipLaunchKernelGGL(pole_inverse_kernel<5>, grid, blk, 0, st, out, rows, log_N, offset, z, mode, lo, hi); break;
-    case 6: hipLaunchKernelGGL(pole_inverse_kernel<6>, grid, blk, 0, st, out, rows, log_N, offset, z, mode, lo, hi); break;
-    case 7: hipLaunchKernelGGL(pole_inverse_kernel<7>, grid, blk, 0, st, out, rows, log_N, offset, z, mode, lo, hi); break;
-    default: hipLaunchKernelGGL(pole_inverse_kernel<8>, grid, blk, 0, st, out, rows, log_N, offset, z, mode, lo, hi); break;
-    }
+    if (rows >= (1u << 20))
+        launch_pole_inverse_e<16>(out, rows, log_N, offset, z, K, mode, lo, hi, st);
+    else
+        launch_pole_inverse_e<2>(out, rows, log_N, offset, z, K, mode, lo, hi, st);
 }
 
 constexpr int BARY_MAX_K = 8;

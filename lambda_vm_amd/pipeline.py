@@ -58,7 +58,17 @@ def sample_z_ood(t: B.Transcript, n: int, N: int, offset: int):
 
 def prove_committed_table(h: B.GpuLde, t: B.Transcript, n: int, m: int, blowup: int, offset: int,
                           fri_final_log_degree: int = 6, num_queries: int = 30, keep: bool = False):
-    """Rounds R3/R4 of a committed table whose root the transcript has absorbed."""
+    """Rounds R3/R4 of a committed table whose root the transcript has absorbed,
+    in one library call (lvm_prove_committed_table); keep=True runs the same
+    steps one call at a time and also returns the FRI state."""
+    if not keep:
+        return B.prove_committed_table(h, t, n, m, blowup, offset, fri_final_log_degree, num_queries)
+    return prove_committed_table_steps(h, t, n, m, blowup, offset, fri_final_log_degree, num_queries, keep=True)
+
+
+def prove_committed_table_steps(h: B.GpuLde, t: B.Transcript, n: int, m: int, blowup: int, offset: int,
+                                fri_final_log_degree: int = 6, num_queries: int = 30, keep: bool = False):
+    """Rounds R3/R4 driven from the host one library call at a time."""
     N = n * blowup
     blowup_log = blowup.bit_length() - 1
     # R3

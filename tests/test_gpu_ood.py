@@ -100,6 +100,21 @@ def test_deep_composition(gpu, ora, with_aux, with_parts, K, geo):
     assert np.array_equal(canon(got), canon(ref[bitrev_perm(N.bit_length() - 1)]))
 
 
+def test_deep_composition_large_domain(gpu, ora):
+    """2^20 LDE points: the pole factors take the sixteen-rows-per-thread path."""
+    log_n, m, blowup, g, K = 18, 3, 4, 3, 2
+    n, N = 1 << log_n, (1 << log_n) * blowup
+    tr = workload.trace_table(11, 0, n, m)
+    hm, lm = gpu.coset_lde_row_major_with_merkle_tree_keep(tr, n, m, blowup, ora.lde_weights(n, g))
+    zp = workload.fp3_values(81, K)
+    tood = workload.fp3_values(82, m * K)
+    gtr = workload.fp3_values(83, m * K)
+    got = gpu.deep_composition(hm, g, zp, tood, gtr).to_array(np.uint64, (N, 3))
+    ref = ora.deep_composition(lm, g, zp, tood, gtr)
+    perm = bitrev_perm(N.bit_length() - 1)
+    assert np.array_equal(canon(got), canon(ref[perm]))
+
+
 def test_deep_quotient_identity(gpu, ora):
     # deep(x) * (x - z) = sum_j gamma_j (t_j(x) - t_j(z)) for one pole: check
     # at a few rows with the exact OOD values, using big-integer arithmetic

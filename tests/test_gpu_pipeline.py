@@ -50,3 +50,36 @@ def test_single_table_rounds_match_oracle(gpu, ora, log_n, m, blowup, g, k):
         off += size
         for q, iota in enumerate(ref["iotas"]):
             assert np.array_equal(canon(proof.fri_sym[li, q]), canon(ev[(iota >> li) ^ 1]))
+
+
+@pytest.mark.parametrize("log_n,m,blowup,k,q", [(8, 8, 4, 2, 10), (11, 13, 2, 3, 7), (6, 4, 4, 6, 5),
+                                              (13, 33, 8, 4, 30)])
+def test_native_rounds_equal_stepwise(gpu, log_n, m, blowup, k, q):
+    """lvm_prove_committed_table gives every field the host-driven sequence
+    of library calls gives, and leaves the transcript in the same state."""
+    from lambda_vm_amd import pipeline
+
+    n = 1 << log_n
+    tr = workload.trace_table(4, log_n, n, m)
+    d_tr = gpu.DeviceBuffer.from_array(tr)
+    d_w = gpu.coset_weights_dev(n, 3)
+    out = []
+    for native in (True, False):
+        h = gpu.coset_lde_dev(d_tr, n, m, blowup, d_w)
+        t = gpu.Transcript(b"rounds")
+        t.append_bytes(h.root)
+        if native:
+            p = pipeline.prove_committed_table(h, t, n, m, blowup, 3, k, q)
+        else:
+            p, fri = pipeline.prove_committed_table_steps(h, t, n, m, blowup, 3, k, q, keep=True)
+            fri.close()
+        h.close()
+        out.append(p)
+    a, b = out
+    assert a.root == b.root and a.z == b.z and a.transcript_state == b.transcript_state
+    for f in ("ood", "fri_roots", "fri_coeffs", "iotas", "fri_sym", "trace_paths", "trace_rows"):
+        assert np.array_equal(canon(getattr(a, f)) if getattr(a, f).dtype == np.uint64 else getattr(a, f),
+                              canon(getattr(b, f)) if getattr(b, f).dtype == np.uint64 else getattr(b, f)), f
+    assert len(a.fri_paths) == len(b.fri_paths)
+    for x, y in zip(a.fri_paths, b.fri_paths):
+        assert np.array_equal(x, y)
