@@ -26,6 +26,15 @@ row, ALU[a, b, op, res] (DWordBL operands) received with mu. Trace generation
 split at MU_MAX = 255 (types.rs:528-550), padded to a power of two (at least
 4). The reference merges through a hash map, so its row order is arbitrary;
 here rows follow first occurrence.
+
+IS_WH (prover/src/tables/is_wh.rs): range check of the top 48 bits of a 64-bit
+value, one row per distinct (word, half) pair. Columns (is_wh.rs:41-54):
+value[0] (the Half: bits 16..32), value[1], value[2] (the Word as two halves),
+mu (a B20). No polynomial constraints; bus interactions (is_wh.rs:158-190):
+IS_B20[mu] sent once per row, IS_HALFWORD[value[i]] sent with mu for i = 0..2,
+IS_WH[value[1] + 2^16 value[2], value[0]] received with mu. Trace generation
+(is_wh.rs:83-127): pairs counted, sorted by (word, half), split at 2^20 - 1,
+padded with zero rows to a power of two (at least 4).
 """
 from __future__ import annotations
 
@@ -41,7 +50,7 @@ P = ir.P
 INV_SHIFT_32 = 18446744065119617026  # 2^-32 mod p (templates.rs:26)
 
 # bus ids (prover/src/tables/types.rs:255-336) and the ALU op code (:562-570)
-BUS_ARE_BYTES, BUS_IS_HALFWORD, BUS_ZERO, BUS_BYTE_ALU, BUS_ALU = 0, 1, 8, 24, 25
+BUS_ARE_BYTES, BUS_IS_HALFWORD, BUS_IS_B20, BUS_ZERO, BUS_BYTE_ALU, BUS_ALU, BUS_IS_WH = 0, 1, 2, 8, 24, 25, 32
 ALU_OP_AND, ALU_OP_OR, ALU_OP_XOR, ALU_OP_EQ = 0, 1, 2, 3
 MU_MAX = 255
 
@@ -205,5 +214,72 @@ def bytewise_table_air() -> AirProgram:
     prog = b.finish(c.NUM_COLUMNS)
     pairs, absorbed = split_interactions(len(its))
     return AirProgram("bytewise", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
+                      boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
+                      max_degree=max(3 if pairs else 0, 1 + absorbed))
+
+
+class IsWhCols:
+    VALUE_0, VALUE_1, VALUE_2, MU = 0, 1, 2, 3
+    NUM_COLUMNS = 4
+
+
+IS_WH_MU_MAX = (1 << 20) - 1
+
+
+def is_wh_key(value: int) -> tuple:
+    """(word, half) of a 64-bit value: bits 32..64 and 16..32 (is_wh.rs:68-76)."""
+    return ((value >> 32) & 0xFFFFFFFF, (value >> 16) & 0xFFFF)
+
+
+def dedup_is_wh_rows(keys):
+    """((word, half), mu) rows sorted by key, split at 2^20 - 1 (is_wh.rs:83-99)."""
+    counts = {}
+    for k in keys:
+        counts[k] = counts.get(k, 0) + 1
+    rows = []
+    for k in sorted(counts):
+        left = counts[k]
+        while left > 0:
+            mu = min(left, IS_WH_MU_MAX)
+            left -= mu
+            rows.append((k, mu))
+    return rows
+
+
+def generate_is_wh_trace(rows) -> np.ndarray:
+    c = IsWhCols
+    n = max(4, 1 << max(len(rows) - 1, 0).bit_length())
+    t = np.zeros((n, c.NUM_COLUMNS), dtype=np.uint64)
+    for i, ((word, half), mu) in enumerate(rows):
+        t[i, c.VALUE_0] = half
+        t[i, c.VALUE_1] = word & 0xFFFF
+        t[i, c.VALUE_2] = word >> 16
+        t[i, c.MU] = mu
+    return t
+
+
+def is_wh_bus_interactions():
+    c = IsWhCols
+    mu = Multiplicity("Column", (c.MU,))
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    return [
+        BusInteraction.sender(BUS_IS_B20, ONE, [d(c.MU)]),
+        BusInteraction.sender(BUS_IS_HALFWORD, mu, [d(c.VALUE_0)]),
+        BusInteraction.sender(BUS_IS_HALFWORD, mu, [d(c.VALUE_1)]),
+        BusInteraction.sender(BUS_IS_HALFWORD, mu, [d(c.VALUE_2)]),
+        BusInteraction.receiver(BUS_IS_WH, mu, [
+            BusValue.of_linear([LinearTerm(1, c.VALUE_1), LinearTerm(1 << 16, c.VALUE_2)]), d(c.VALUE_0)]),
+    ]
+
+
+def is_wh_table_air() -> AirProgram:
+    """IS_WH as an AIR: only the LogUp constraints of its interactions."""
+    c = IsWhCols
+    b = ir.IrBuilder()
+    its = is_wh_bus_interactions()
+    n_total = emit_logup_constraints(b, its, 0)
+    prog = b.finish(c.NUM_COLUMNS)
+    pairs, absorbed = split_interactions(len(its))
+    return AirProgram("is_wh", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
                       boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
                       max_degree=max(3 if pairs else 0, 1 + absorbed))
