@@ -48,6 +48,7 @@ class ProofOptions:
     fri_final_log_degree: int = 2
     num_queries: int = 16
     grinding_factor: int = 0
+    vram_budget: int | None = None  # device bytes for tables in flight (None: no limit)
 
 
 @dataclass
@@ -139,16 +140,16 @@ def prove_air_tables(tables, opts: ProofOptions = ProofOptions(), label: bytes =
         air, tr, n = tables[idx]
         return _prove_table(air, tr, n, mains[idx], weights[idx], forks[idx], opts, z_logup, alpha)
 
+    # rounds 2-4 per table: driver threads take tables heaviest first, each
+    # admitted through the device-byte gate (prover.rs:3590-3600 with
+    # run_admitted / VramGate / heaviest_first, prover.rs:677-789)
+    from . import _native
+    from .scheduler import VramGate, estimate_table_vram_bytes, heaviest_first, run_admitted
+    est = [estimate_table_vram_bytes(air.num_main, air.num_aux, n * Bf) for air, _, n in tables]
+    gate = VramGate(opts.vram_budget if opts.vram_budget is not None else 1 << 62)
     w = len(tables) if workers is None else workers
-    if w > 1 and len(tables) > 1:
-        from concurrent.futures import ThreadPoolExecutor
-
-        from . import _native
-        with ThreadPoolExecutor(max_workers=min(w, len(tables)), initializer=_native.ensure_device_this_thread,
-                                initargs=(0,)) as pool:
-            proofs = list(pool.map(one, range(len(tables))))
-    else:
-        proofs = [one(i) for i in range(len(tables))]
+    proofs = run_admitted(heaviest_first(est), est, gate, w, one,
+                          initializer=lambda: _native.ensure_device_this_thread(0))
     for h, wt in zip(mains, weights):
         h.close()
         wt.free()

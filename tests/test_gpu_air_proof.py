@@ -118,3 +118,20 @@ def test_gpu_proofs_match_oracle_and_verify(gpu, which, n, blowup, k, q, grind):
     assert data == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
     assert verify_air_tables([a for a, _ in tables], proof_io.deserialize_multi_proof(data), blowup, 3, k, q, grind,
                              b"gpu-air")[0]
+
+
+def test_table_scheduling_does_not_change_proofs(gpu):
+    """Heaviest-first driver threads under a device-byte budget (one table at a
+    time when the budget is tiny) give the same proof bytes as unlimited."""
+    from lambda_vm_amd import proof_io, prover
+    tables = _tables(256, ("fib", "quad", "eq", "bytewise"), seed=41)
+    out = []
+    for workers, budget in ((None, None), (2, 1), (1, None)):
+        opts = prover.ProofOptions(blowup=4, coset_offset=3, fri_final_log_degree=2, num_queries=8,
+                                   vram_budget=budget)
+        dev = [(a, gpu.DeviceBuffer.from_array(tr), 256) for a, tr in tables]
+        out.append(proof_io.serialize_multi_proof(prover.prove_air_tables(dev, opts, label=b"sched",
+                                                                          workers=workers)))
+        for _, d, _ in dev:
+            d.free()
+    assert out[0] == out[1] == out[2]
