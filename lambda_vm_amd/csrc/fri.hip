@@ -65,7 +65,7 @@ __global__ __launch_bounds__(256) void fri_fold_kernel(const uint64_t *__restric
         }
         l[6] = 0x01;
         h[16] = 0x80000000u;
-        keccak_f1600_32(l, h);
+        keccak_f1600_32<true>(l, h);  // small layers are latency-bound: rounds unrolled
         uint64_t *o = leaves + 4 * t;
 #pragma unroll
         for (int k = 0; k < 4; k++) o[k] = ((uint64_t)h[k] << 32) | l[k];

@@ -56,56 +56,68 @@ __device__ __forceinline__ void rotl(uint32_t hi, uint32_t lo, uint32_t &oh, uin
     rotl<R>(xor3(h[src], Ch[((src) + 4) % 5], Rh[((src) + 1) % 5]),                             \
             xor3(l[src], Cl[((src) + 4) % 5], Rl[((src) + 1) % 5]), bh[dst], bl[dst])
 
-__device__ __forceinline__ void keccak_f1600_32(uint32_t l[25], uint32_t h[25]) {
-#pragma unroll 1
-    for (int round = 0; round < 24; round++) {
-        uint32_t Cl[5], Ch[5], Rl[5], Rh[5];
+// one round (theta, rho, pi, chi, iota) on the 32-bit halves
+__device__ __forceinline__ void keccak_round_32(uint32_t l[25], uint32_t h[25], int round) {
+    uint32_t Cl[5], Ch[5], Rl[5], Rh[5];
+#pragma unroll
+    for (int x = 0; x < 5; x++) {
+        Cl[x] = xor3(xor3(l[x], l[x + 5], l[x + 10]), l[x + 15], l[x + 20]);
+        Ch[x] = xor3(xor3(h[x], h[x + 5], h[x + 10]), h[x + 15], h[x + 20]);
+    }
+#pragma unroll
+    for (int x = 0; x < 5; x++) rotl<1>(Ch[x], Cl[x], Rh[x], Rl[x]);
+    uint32_t bl[25], bh[25];
+    // theta + rho + pi: B[y, 2x+3y] = rot(A[x,y] ^ D[x], r[x,y]), lane index x + 5y
+    LVM_KB(0, 0, 0);
+    LVM_KB(10, 1, 1);
+    LVM_KB(20, 2, 62);
+    LVM_KB(5, 3, 28);
+    LVM_KB(15, 4, 27);
+    LVM_KB(16, 5, 36);
+    LVM_KB(1, 6, 44);
+    LVM_KB(11, 7, 6);
+    LVM_KB(21, 8, 55);
+    LVM_KB(6, 9, 20);
+    LVM_KB(7, 10, 3);
+    LVM_KB(17, 11, 10);
+    LVM_KB(2, 12, 43);
+    LVM_KB(12, 13, 25);
+    LVM_KB(22, 14, 39);
+    LVM_KB(23, 15, 41);
+    LVM_KB(8, 16, 45);
+    LVM_KB(18, 17, 15);
+    LVM_KB(3, 18, 21);
+    LVM_KB(13, 19, 8);
+    LVM_KB(14, 20, 18);
+    LVM_KB(24, 21, 2);
+    LVM_KB(9, 22, 61);
+    LVM_KB(19, 23, 56);
+    LVM_KB(4, 24, 14);
+    // chi
+#pragma unroll
+    for (int y = 0; y < 25; y += 5) {
 #pragma unroll
         for (int x = 0; x < 5; x++) {
-            Cl[x] = xor3(xor3(l[x], l[x + 5], l[x + 10]), l[x + 15], l[x + 20]);
-            Ch[x] = xor3(xor3(h[x], h[x + 5], h[x + 10]), h[x + 15], h[x + 20]);
+            l[y + x] = chi(bl[y + x], bl[y + (x + 1) % 5], bl[y + (x + 2) % 5]);
+            h[y + x] = chi(bh[y + x], bh[y + (x + 1) % 5], bh[y + (x + 2) % 5]);
         }
+    }
+    // iota
+    l[0] ^= KECCAK_RC_LO[round];
+    h[0] ^= KECCAK_RC_HI[round];
+}
+
+// The 24 rounds as a loop (throughput kernels: one copy of the round body
+// keeps the code small), or fully unrolled (latency-bound tree tops: no loop
+// overhead, round constants as literals).
+template <bool UNROLL = false>
+__device__ __forceinline__ void keccak_f1600_32(uint32_t l[25], uint32_t h[25]) {
+    if constexpr (UNROLL) {
 #pragma unroll
-        for (int x = 0; x < 5; x++) rotl<1>(Ch[x], Cl[x], Rh[x], Rl[x]);
-        uint32_t bl[25], bh[25];
-        // theta + rho + pi: B[y, 2x+3y] = rot(A[x,y] ^ D[x], r[x,y]), lane index x + 5y
-        LVM_KB(0, 0, 0);
-        LVM_KB(10, 1, 1);
-        LVM_KB(20, 2, 62);
-        LVM_KB(5, 3, 28);
-        LVM_KB(15, 4, 27);
-        LVM_KB(16, 5, 36);
-        LVM_KB(1, 6, 44);
-        LVM_KB(11, 7, 6);
-        LVM_KB(21, 8, 55);
-        LVM_KB(6, 9, 20);
-        LVM_KB(7, 10, 3);
-        LVM_KB(17, 11, 10);
-        LVM_KB(2, 12, 43);
-        LVM_KB(12, 13, 25);
-        LVM_KB(22, 14, 39);
-        LVM_KB(23, 15, 41);
-        LVM_KB(8, 16, 45);
-        LVM_KB(18, 17, 15);
-        LVM_KB(3, 18, 21);
-        LVM_KB(13, 19, 8);
-        LVM_KB(14, 20, 18);
-        LVM_KB(24, 21, 2);
-        LVM_KB(9, 22, 61);
-        LVM_KB(19, 23, 56);
-        LVM_KB(4, 24, 14);
-        // chi
-#pragma unroll
-        for (int y = 0; y < 25; y += 5) {
-#pragma unroll
-            for (int x = 0; x < 5; x++) {
-                l[y + x] = chi(bl[y + x], bl[y + (x + 1) % 5], bl[y + (x + 2) % 5]);
-                h[y + x] = chi(bh[y + x], bh[y + (x + 1) % 5], bh[y + (x + 2) % 5]);
-            }
-        }
-        // iota
-        l[0] ^= KECCAK_RC_LO[round];
-        h[0] ^= KECCAK_RC_HI[round];
+        for (int round = 0; round < 24; round++) keccak_round_32(l, h, round);
+    } else {
+#pragma unroll 1
+        for (int round = 0; round < 24; round++) keccak_round_32(l, h, round);
     }
 }
 #undef LVM_KB
@@ -125,6 +137,7 @@ __device__ __forceinline__ void keccak_f1600(uint64_t s[25]) {
 
 // Keccak-256 of exactly two 32-byte nodes (one permutation): lanes 0..7 hold
 // left || right, pad byte 0x01 at lane 8, 0x80 at the top of lane 16.
+template <bool UNROLL = false>
 __device__ __forceinline__ void keccak_pair_node(const uint64_t lft[4], const uint64_t rgt[4], uint64_t out[4]) {
     uint32_t l[25], h[25];
 #pragma unroll
@@ -138,7 +151,7 @@ __device__ __forceinline__ void keccak_pair_node(const uint64_t lft[4], const ui
     }
     l[8] = 0x01;
     h[16] = 0x80000000u;
-    keccak_f1600_32(l, h);
+    keccak_f1600_32<UNROLL>(l, h);
 #pragma unroll
     for (int i = 0; i < 4; i++) out[i] = ((uint64_t)h[i] << 32) | l[i];
 }

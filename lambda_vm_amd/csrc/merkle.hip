@@ -152,7 +152,7 @@ __device__ __forceinline__ void merkle_subtree_body(uint64_t *nodes, uint64_t cs
     if (t < span) {
         const uint64_t *l = nodes + 4 * (cstart + 2 * (uint64_t)span * b + 2 * t);
         uint64_t lv[4] = {l[0], l[1], l[2], l[3]}, rv[4] = {l[4], l[5], l[6], l[7]}, o[4];
-        keccak_pair_node(lv, rv, o);
+        keccak_pair_node<true>(lv, rv, o);
         uint64_t *d = nodes + 4 * (pbase + t);
 #pragma unroll
         for (int k = 0; k < 4; k++) {
@@ -171,7 +171,7 @@ __device__ __forceinline__ void merkle_subtree_body(uint64_t *nodes, uint64_t cs
         if (t < half) {
             uint64_t lv[4] = {dig[2 * t][0], dig[2 * t][1], dig[2 * t][2], dig[2 * t][3]};
             uint64_t rv[4] = {dig[2 * t + 1][0], dig[2 * t + 1][1], dig[2 * t + 1][2], dig[2 * t + 1][3]};
-            keccak_pair_node(lv, rv, o);
+            keccak_pair_node<true>(lv, rv, o);
         }
         __syncthreads();
         if (t < half) {
