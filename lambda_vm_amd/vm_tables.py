@@ -35,6 +35,18 @@ IS_B20[mu] sent once per row, IS_HALFWORD[value[i]] sent with mu for i = 0..2,
 IS_WH[value[1] + 2^16 value[2], value[0]] received with mu. Trace generation
 (is_wh.rs:83-127): pairs counted, sorted by (word, half), split at 2^20 - 1,
 padded with zero rows to a power of two (at least 4).
+
+LT (prover/src/tables/lt.rs): signed or unsigned lhs < rhs for 64-bit values,
+optionally inverted. Columns (lt.rs:41-87): lhs, rhs (DWordHHW: word, half,
+half), signed, lt, lhs - rhs (DWordHL, 4 halves), the two sign bits, invert,
+out, mu. Constraints (lt.rs:400-466): the carry pair of rhs + diff = lhs
+(each carry a bit), lt = signed * signed_lt + (1 - signed) * carry_1 with
+signed_lt = a(1 - b) + a carry_1 + (1 - b) carry_1 for the sign bits a, b,
+out = lt XOR invert, invert and signed bits. Bus interactions (lt.rs:190-330):
+MSB16[lhs[2], lhs_msb] and MSB16[rhs[2], rhs_msb], IS_HALFWORD of the four
+difference halves and of lhs[1], rhs[1] (all with mu), ARE_BYTES[mu, 0] once,
+ALU[lhs, rhs, LT + 32 signed + 64 invert, out, 0] received with mu. Trace
+generation (lt.rs:133-175): deduplicated with counts as mu (split at 255).
 """
 from __future__ import annotations
 
@@ -50,8 +62,9 @@ P = ir.P
 INV_SHIFT_32 = 18446744065119617026  # 2^-32 mod p (templates.rs:26)
 
 # bus ids (prover/src/tables/types.rs:255-336) and the ALU op code (:562-570)
-BUS_ARE_BYTES, BUS_IS_HALFWORD, BUS_IS_B20, BUS_ZERO, BUS_BYTE_ALU, BUS_ALU, BUS_IS_WH = 0, 1, 2, 8, 24, 25, 32
-ALU_OP_AND, ALU_OP_OR, ALU_OP_XOR, ALU_OP_EQ = 0, 1, 2, 3
+BUS_ARE_BYTES, BUS_IS_HALFWORD, BUS_IS_B20, BUS_MSB16, BUS_ZERO, BUS_BYTE_ALU, BUS_ALU, BUS_IS_WH = (
+    0, 1, 2, 7, 8, 24, 25, 32)
+ALU_OP_AND, ALU_OP_OR, ALU_OP_XOR, ALU_OP_EQ, ALU_OP_LT = 0, 1, 2, 3, 4
 MU_MAX = 255
 
 
@@ -283,3 +296,95 @@ def is_wh_table_air() -> AirProgram:
     return AirProgram("is_wh", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
                       boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
                       max_degree=max(3 if pairs else 0, 1 + absorbed))
+
+
+class LtCols:
+    LHS_0, LHS_1, LHS_2, RHS_0, RHS_1, RHS_2, SIGNED, LT = 0, 1, 2, 3, 4, 5, 6, 7
+    SUB_0, SUB_1, SUB_2, SUB_3, LHS_MSB, RHS_MSB, INVERT, OUT, MU = 8, 9, 10, 11, 12, 13, 14, 15, 16
+    NUM_COLUMNS = 17
+
+
+@dataclass(frozen=True)
+class LtOperation:
+    lhs: int
+    rhs: int
+    signed: bool
+    invert: bool = False
+
+    def compute_lt(self) -> bool:
+        if self.signed:
+            s = lambda v: v - (1 << 64) if v >> 63 else v  # noqa: E731
+            return s(self.lhs) < s(self.rhs)
+        return self.lhs < self.rhs
+
+    def compute_out(self) -> bool:
+        return self.compute_lt() ^ self.invert
+
+
+def generate_lt_trace(ops) -> np.ndarray:
+    c = LtCols
+    rows = dedup_rows(ops)
+    n = max(4, 1 << max(len(rows) - 1, 0).bit_length())
+    t = np.zeros((n, c.NUM_COLUMNS), dtype=np.uint64)
+    for i, (op, mu) in enumerate(rows):
+        for base, v in ((c.LHS_0, op.lhs), (c.RHS_0, op.rhs)):  # DWordHHW
+            t[i, base] = v & 0xFFFFFFFF
+            t[i, base + 1] = (v >> 32) & 0xFFFF
+            t[i, base + 2] = (v >> 48) & 0xFFFF
+        t[i, c.SIGNED] = int(op.signed)
+        t[i, c.LT] = int(op.compute_lt())
+        d = (op.lhs - op.rhs) & (2**64 - 1)
+        for k in range(4):
+            t[i, c.SUB_0 + k] = (d >> (16 * k)) & 0xFFFF
+        t[i, c.LHS_MSB] = op.lhs >> 63
+        t[i, c.RHS_MSB] = op.rhs >> 63
+        t[i, c.INVERT] = int(op.invert)
+        t[i, c.OUT] = int(op.compute_out())
+        t[i, c.MU] = mu
+    return t
+
+
+def lt_bus_interactions():
+    c = LtCols
+    mu = Multiplicity("Column", (c.MU,))
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    its = [BusInteraction.sender(BUS_MSB16, mu, [d(c.LHS_2), d(c.LHS_MSB)]),
+           BusInteraction.sender(BUS_MSB16, mu, [d(c.RHS_2), d(c.RHS_MSB)])]
+    its += [BusInteraction.sender(BUS_IS_HALFWORD, mu, [d(col)])
+            for col in (c.SUB_0, c.SUB_1, c.SUB_2, c.SUB_3, c.LHS_1, c.RHS_1)]
+    its.append(BusInteraction.sender(BUS_ARE_BYTES, ONE, [d(c.MU), BusValue.of_linear([LinearTerm(0)])]))
+    its.append(BusInteraction.receiver(BUS_ALU, mu, [
+        BusValue.packed(Packing.DWORD_HHW, c.LHS_0), BusValue.packed(Packing.DWORD_HHW, c.RHS_0),
+        BusValue.of_linear([LinearTerm(ALU_OP_LT), LinearTerm(32, c.SIGNED), LinearTerm(64, c.INVERT)]),
+        d(c.OUT), BusValue.of_linear([LinearTerm(0)])]))
+    return its
+
+
+def lt_table_air() -> AirProgram:
+    c = LtCols
+    b = ir.IrBuilder()
+    one, inv, two16 = b.const_base(1), b.const_base(INV_SHIFT_32), b.const_base(1 << 16)
+    m = lambda col: b.main(0, col)  # noqa: E731
+    carry0 = (m(c.RHS_0) + (m(c.SUB_0) + m(c.SUB_1) * two16) - m(c.LHS_0)) * inv
+    lhs_hi = m(c.LHS_1) + m(c.LHS_2) * two16
+    rhs_hi = m(c.RHS_1) + m(c.RHS_2) * two16
+    sub_hi = m(c.SUB_2) + m(c.SUB_3) * two16
+    carry1 = (rhs_hi + sub_hi + carry0 - lhs_hi) * inv
+    b.emit(0, carry0 * (one - carry0))
+    b.emit(1, carry1 * (one - carry1))
+    a_msb, b_msb, signed = m(c.LHS_MSB), m(c.RHS_MSB), m(c.SIGNED)
+    one_minus_b = one - b_msb
+    signed_lt = a_msb * one_minus_b + a_msb * carry1 + one_minus_b * carry1
+    expected = signed * signed_lt + (one - signed) * carry1
+    b.emit(2, m(c.LT) - expected)
+    lt, invert = m(c.LT), m(c.INVERT)
+    b.emit(3, m(c.OUT) - (lt + invert - b.const_base(2) * lt * invert))
+    b.emit(4, invert * (one - invert))
+    b.emit(5, signed * (one - signed))
+    its = lt_bus_interactions()
+    n_total = emit_logup_constraints(b, its, 6)
+    prog = b.finish(c.NUM_COLUMNS)
+    pairs, absorbed = split_interactions(len(its))
+    return AirProgram("lt", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
+                      boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
+                      max_degree=max(3, 1 + absorbed))

@@ -5,8 +5,8 @@ equal (canonical limbs), the canonical proof bytes are identical, and the
 verifier restatement accepts the GPU proofs (also after a round trip through
 the bytes).
 Tables: FibonacciMultiColumn (one composition part), Quadratic, a 4-bus
-LogUp table and the VM's EQ, BYTEWISE and IS_WH tables (two parts), alone
-and together in one transcript."""
+LogUp table and the VM's EQ, BYTEWISE, IS_WH and LT tables (two parts),
+alone and together in one transcript."""
 import numpy as np
 import pytest
 
@@ -49,6 +49,14 @@ def _tables(n, which, seed):
             ops = [V.BytewiseOperation(int(x), int(y), int(o)) for x, y, o in
                    zip(rng.integers(0, 2**63, n - 3), rng.integers(0, 2**63, n - 3), rng.integers(0, 3, n - 3))]
             tr = V.generate_bytewise_trace(ops)
+        elif kind == "lt":
+            from lambda_vm_amd import vm_tables as V
+            a = V.lt_table_air()
+            a.bus_descriptor = A.descriptor(a)
+            xs = rng.integers(0, 2**63, (n - 3, 2))
+            ops = [V.LtOperation(int(x) * 2 + (i & 1), int(y) * 2 + 1, bool(i % 2), bool(i % 3 == 0))
+                   for i, (x, y) in enumerate(xs)]
+            tr = V.generate_lt_trace(ops)
         elif kind == "is_wh":
             from lambda_vm_amd import vm_tables as V
             a = V.is_wh_table_air()
@@ -73,6 +81,7 @@ def _f3rows(x):
                                                      (("eq",), 256, 4, 2, 8, 0), (("eq", "fib"), 1024, 8, 2, 12, 2),
                                                      (("bytewise", "eq"), 512, 4, 3, 10, 0),
                                                      (("is_wh", "bytewise", "eq"), 256, 4, 2, 8, 0),
+                                                     (("lt", "eq"), 256, 4, 2, 8, 2),
                                                      (("fib", "quad", "logup"), 512, 4, 3, 10, 4)])
 def test_gpu_proofs_match_oracle_and_verify(gpu, which, n, blowup, k, q, grind):
     from lambda_vm_amd import prover
