@@ -309,9 +309,25 @@ struct Stage {
     hipStream_t last = nullptr;  // stream of the copies queued since the area was last drained
     std::vector<PendingDownload> pend;
 };
-// The area is never freed (a thread's destructors may run after the runtime
-// has shut down); a thread that ends leaves it to the process.
-thread_local Stage t_stage;
+// Pinned areas outlive their threads: a thread that ends hands its area to
+// a spare list (no HIP call: a thread's destructors may run after the runtime
+// has shut down) and a new thread takes one from there before allocating, so
+// threads created per proof do not accumulate pinned memory.
+struct SpareArea {
+    uint8_t *buf;
+    size_t cap;
+};
+std::mutex &g_area_mu = *new std::mutex;
+std::vector<SpareArea> &g_spare_areas = *new std::vector<SpareArea>;
+
+struct ThreadStage : Stage {
+    ~ThreadStage() {
+        if (!buf) return;
+        std::lock_guard<std::mutex> lk(g_area_mu);
+        g_spare_areas.push_back(SpareArea{buf, cap});
+    }
+};
+thread_local ThreadStage t_stage;
 
 void deliver(Stage &sg) {
     for (auto &pd : sg.pend) std::memcpy(pd.dst, sg.buf + pd.off, pd.bytes);
@@ -328,6 +344,17 @@ hipError_t stage_region(Stage &sg, size_t bytes, hipStream_t st, size_t *off) {
         hipError_t e = hipStreamSynchronize(sg.last);
         if (e != hipSuccess) return e;
         deliver(sg);
+    }
+    if (!sg.buf) {
+        // a spare area left by an ended thread
+        std::lock_guard<std::mutex> lk(g_area_mu);
+        for (size_t i = 0; i < g_spare_areas.size(); i++)
+            if (g_spare_areas[i].cap >= need) {
+                sg.buf = g_spare_areas[i].buf;
+                sg.cap = g_spare_areas[i].cap;
+                g_spare_areas.erase(g_spare_areas.begin() + (long)i);
+                break;
+            }
     }
     if (need > sg.cap) {
         if (sg.buf) hipHostFree(sg.buf);
