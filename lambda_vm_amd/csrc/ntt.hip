@@ -100,10 +100,11 @@ __device__ __forceinline__ void dft_small(uint64_t *x, bool inverse) {
 
 // Body of one pass for block (bx, by, bz) of the single-table grid
 // (bx: row-tile index, by: 16-column group, bz: coset slice).
-template <int LOG_R1, int LOG_R2, bool MULTI>
+template <int LOG_R1, int LOG_R2, bool MULTI, int C = 16>
 __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t bx, uint32_t by, uint32_t bz) {
     constexpr int R1 = 1 << LOG_R1, R2 = 1 << LOG_R2, R = R1 * R2, LOG_R = LOG_R1 + LOG_R2;
-    constexpr int C = 16;                 // columns per tile
+    // C columns per tile: 16 (128-byte row segments), or 8 for narrow tables
+    // whose last 16-column group would leave most lanes idle
     constexpr int TPC = R1 > R2 ? R1 : R2;  // threads per column
     constexpr int TILES = 256 / (C * TPC);
     static_assert(TILES >= 1, "tile too large");
@@ -227,9 +228,9 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t bx, ui
     }
 }
 
-template <int LOG_R1, int LOG_R2, bool MULTI>
+template <int LOG_R1, int LOG_R2, bool MULTI, int C>
 __global__ __launch_bounds__(256) void ntt_pass_kernel(PassArgs a) {
-    ntt_pass_body<LOG_R1, LOG_R2, MULTI>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+    ntt_pass_body<LOG_R1, LOG_R2, MULTI, C>(a, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // One launch for the same pass shape over several tables: `args[t]` is
@@ -253,18 +254,29 @@ static constexpr uint32_t pass_tiles() {
 }
 
 // pass shapes: (LOG_R1, LOG_R2) for r = 1..8
-template <int A, int B>
-static void launch_one(const PassArgs &a0, uint32_t cosets, hipStream_t st) {
+template <int A, int B, int C>
+static void launch_one_c(const PassArgs &a0, uint32_t cosets, hipStream_t st) {
     constexpr int TPC = (1 << A) > (1 << B) ? (1 << A) : (1 << B);
-    constexpr int TILES = 256 / (16 * TPC);
+    constexpr int TILES = 256 / (C * TPC);
     PassArgs a = a0;
     // all cosets of a tile come from one block, which reads the tile once
     a.cosets_per_block = cosets;
-    dim3 grid((unsigned)((a.total_groups + TILES - 1) / TILES), (a.m + 15) / 16, 1);
+    dim3 grid((unsigned)((a.total_groups + TILES - 1) / TILES), (a.m + C - 1) / C, 1);
     if (cosets > 1)
-        hipLaunchKernelGGL((ntt_pass_kernel<A, B, true>), grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((ntt_pass_kernel<A, B, true, C>), grid, dim3(256), 0, st, a);
     else
-        hipLaunchKernelGGL((ntt_pass_kernel<A, B, false>), grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((ntt_pass_kernel<A, B, false, C>), grid, dim3(256), 0, st, a);
+}
+
+// 8-column tiles when they leave fewer lanes idle than 16-column ones (a
+// table of 6, 17 or 18 columns), 16 otherwise
+template <int A, int B>
+static void launch_one(const PassArgs &a, uint32_t cosets, hipStream_t st) {
+    const uint64_t idle16 = (a.m + 15) / 16 * 16 - a.m, idle8 = (a.m + 7) / 8 * 8 - a.m;
+    if (idle8 < idle16)
+        launch_one_c<A, B, 8>(a, cosets, st);
+    else
+        launch_one_c<A, B, 16>(a, cosets, st);
 }
 
 static void launch_pass(int r, const PassArgs &a, uint32_t cosets, hipStream_t st, const char *name) {
