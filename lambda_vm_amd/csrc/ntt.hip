@@ -236,6 +236,9 @@ __global__ __launch_bounds__(256) void ntt_pass_kernel(PassArgs a) {
 // One launch for the same pass shape over several tables: `args[t]` is
 // table t's pass, blocks [prefix[t], prefix[t+1]) are its tiles, column
 // groups fastest so that neighbouring blocks cover whole rows.
+// columns per tile of the batched passes
+constexpr int BATCH_C = 16;  // 8-column tiles measured slower here (28.4 vs 30.8 Gcells/s)
+
 template <int LOG_R1, int LOG_R2, bool MULTI>
 __global__ __launch_bounds__(256) void ntt_pass_batched_kernel(const PassArgs *__restrict__ args,
                                                                const uint32_t *__restrict__ prefix, uint32_t count) {
@@ -243,14 +246,14 @@ __global__ __launch_bounds__(256) void ntt_pass_batched_kernel(const PassArgs *_
     const uint32_t t = find_slot(prefix, count, b);
     const PassArgs a = args[t];
     const uint32_t local = b - prefix[t];
-    const uint32_t groups = (a.m + 15) / 16;
-    ntt_pass_body<LOG_R1, LOG_R2, MULTI>(a, local / groups, local % groups, 0);
+    const uint32_t groups = (a.m + BATCH_C - 1) / BATCH_C;
+    ntt_pass_body<LOG_R1, LOG_R2, MULTI, BATCH_C>(a, local / groups, local % groups, 0);
 }
 
 template <int A, int B>
 static constexpr uint32_t pass_tiles() {
     constexpr int TPC = (1 << A) > (1 << B) ? (1 << A) : (1 << B);
-    return 256 / (16 * TPC);
+    return 256 / (BATCH_C * TPC);
 }
 
 // pass shapes: (LOG_R1, LOG_R2) for r = 1..8
@@ -268,12 +271,12 @@ static void launch_one_c(const PassArgs &a0, uint32_t cosets, hipStream_t st) {
         hipLaunchKernelGGL((ntt_pass_kernel<A, B, false, C>), grid, dim3(256), 0, st, a);
 }
 
-// 8-column tiles when they leave fewer lanes idle than 16-column ones (a
-// table of 6, 17 or 18 columns), 16 otherwise
+// 8-column tiles for tables of at most 8 columns (a 16-column tile would
+// leave half its lanes or more idle); 16 otherwise: on wide tables 8-column
+// tiles measured slower even where they idle fewer lanes (64-byte row segments)
 template <int A, int B>
 static void launch_one(const PassArgs &a, uint32_t cosets, hipStream_t st) {
-    const uint64_t idle16 = (a.m + 15) / 16 * 16 - a.m, idle8 = (a.m + 7) / 8 * 8 - a.m;
-    if (idle8 < idle16)
+    if (a.m <= 8)
         launch_one_c<A, B, 8>(a, cosets, st);
     else
         launch_one_c<A, B, 16>(a, cosets, st);
@@ -342,7 +345,7 @@ uint32_t pass_blocks(int r, const PassArgs &a) {
     case 8: tiles = pass_tiles<4, 4>(); break;
     default: break;
     }
-    return (uint32_t)((a.total_groups + tiles - 1) / tiles) * ((a.m + 15) / 16);
+    return (uint32_t)((a.total_groups + tiles - 1) / tiles) * ((a.m + BATCH_C - 1) / BATCH_C);
 }
 
 template <int A, int B>
