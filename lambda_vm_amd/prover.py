@@ -135,10 +135,30 @@ def prove_air_tables(tables, opts: ProofOptions = ProofOptions(), label: bytes =
         if len(tables) > 1:
             f.append_bytes(int(idx).to_bytes(8, "little"))
         forks.append(f)
+    # ---- R1 (per table): LogUp aux traces of every table, committed in one
+    # batched plan (each aux root then goes into its own table's fork, so the
+    # transcripts are those of committing the tables one by one) ----
+    auxs = [None] * len(tables)
+    built = []
+    for idx, (air, tr, n) in enumerate(tables):
+        if air.interactions:
+            desc = air_descriptor(air)
+            ap = B.fp3_powers(alpha, desc.alpha_powers_len)
+            aux_dev, L, _ = B.logup_aux_dev(tr, n, air.num_main, desc, ap, z_logup)
+            built.append((idx, aux_dev, desc, ap, L))
+    if built:
+        aplan = B.BatchPlan([(aux_dev, weights[idx], tables[idx][2], 3 * desc.num_out_cols, Bf)
+                             for idx, aux_dev, desc, ap, L in built], use_graph=False)
+        aplan.run()
+        for i, (idx, aux_dev, desc, ap, L) in enumerate(built):
+            auxs[idx] = (aplan.take(i), desc, ap, L)
+        aplan.close()
+        for _, aux_dev, _, _, _ in built:
+            aux_dev.free()
 
     def one(idx):
         air, tr, n = tables[idx]
-        return _prove_table(air, tr, n, mains[idx], weights[idx], forks[idx], opts, z_logup, alpha)
+        return _prove_table(air, tr, n, mains[idx], weights[idx], forks[idx], opts, z_logup, alpha, auxs[idx])
 
     # rounds 2-4 per table: driver threads take tables heaviest first, each
     # admitted through the device-byte gate (prover.rs:3590-3600 with
@@ -157,7 +177,10 @@ def prove_air_tables(tables, opts: ProofOptions = ProofOptions(), label: bytes =
 
 
 def _prove_table(air: AirProgram, tr: B.DeviceBuffer, n: int, main: B.GpuLde, weights: B.DeviceBuffer,
-                 t: B.Transcript, opts: ProofOptions, z_logup, alpha) -> AirTableProof:
+                 t: B.Transcript, opts: ProofOptions, z_logup, alpha, committed_aux=None) -> AirTableProof:
+    """Rounds of one table after R1. committed_aux: (aux LDE handle, LogUp
+    descriptor, alpha powers, L) when the table's aux trace was committed with
+    the other tables' (prove_air_tables), else it is built and committed here."""
     Bf, g = opts.blowup, opts.coset_offset
     N = n * Bf
     m = air.num_main
@@ -166,11 +189,14 @@ def _prove_table(air: AirProgram, tr: B.DeviceBuffer, n: int, main: B.GpuLde, we
     L = None
     # ---- R1 (per table): LogUp aux trace, commitment, contribution ----
     if air.interactions:
-        desc = air_descriptor(air)
-        ap = B.fp3_powers(alpha, desc.alpha_powers_len)
-        aux_dev, L, _ = B.logup_aux_dev(tr, n, m, desc, ap, z_logup)
-        aux = B.coset_lde_dev(aux_dev, n, 3 * desc.num_out_cols, Bf, weights)
-        aux_dev.free()
+        if committed_aux is not None:
+            aux, desc, ap, L = committed_aux
+        else:
+            desc = air_descriptor(air)
+            ap = B.fp3_powers(alpha, desc.alpha_powers_len)
+            aux_dev, L, _ = B.logup_aux_dev(tr, n, m, desc, ap, z_logup)
+            aux = B.coset_lde_dev(aux_dev, n, 3 * desc.num_out_cols, Bf, weights)
+            aux_dev.free()
         t.append_bytes(aux.root)
         L = tuple(int(v) % P for v in L)
         t.append_fp3(L)
