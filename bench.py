@@ -493,6 +493,12 @@ def run_constraints(args) -> dict:
     }
 
 
+# stage names the library's event profiler records in a proof
+STAGE_NAMES = ["ntt_lde", "keccak_leaves", "merkle_inner", "logup_terms", "logup_accumulate", "composition_transpose",
+               "composition_step_inverse", "composition_compiled", "d2_split", "bary_denoms", "bary_sum",
+               "deep_denoms", "deep_rows", "fri_fold_leaves", "fri_merkle", "fri_fold"]
+
+
 def run_vm(args) -> dict:
     """Full proof of four of the VM's own tables in one transcript, at 2^20 rows
     (IS_WH 2^18): EQ (12 columns), BYTEWISE (26), IS_WH (4), LT (17), blowup 4,
@@ -515,12 +521,20 @@ def run_vm(args) -> dict:
     for _ in range(max(1, args.warmup)):
         prover.prove_air_tables(tables, opts)
     backend.sync()
+    backend.prof_reset()
+    backend.prof_enable(True)
     times = []
     for _ in range(args.steps):
         t0 = time.perf_counter()
         prover.prove_air_tables(tables, opts)
         backend.sync()
         times.append(time.perf_counter() - t0)
+    backend.prof_enable(False)
+    stage = {}
+    for k in STAGE_NAMES:
+        cnt, tot = backend.prof_query(k)
+        if cnt:
+            stage[k] = round(tot / args.steps, 3)
     dt = sum(times) / len(times)
     return {
         "metric": "batched LDE+Keccak-Merkle Gcells/s vs per-table launch; p50/p99 proof latency",
@@ -542,6 +556,7 @@ def run_vm(args) -> dict:
             "lde_cells": cells,
             "proof_ms": [round(x * 1e3, 3) for x in times],
             "p50_ms": round(sorted(times)[len(times) // 2] * 1e3, 3),
+            "stage_ms_per_proof": stage,
         },
     }
 
@@ -580,11 +595,8 @@ def run_prove(args) -> dict:
         backend.sync()
         times.append(time.perf_counter() - t0)
     backend.prof_enable(False)
-    names = ["ntt_lde", "keccak_leaves", "merkle_inner", "logup_terms", "logup_accumulate", "composition_transpose",
-             "composition_compiled", "d2_split", "bary_denoms", "bary_sum", "deep_denoms", "deep_rows",
-             "fri_fold_leaves", "fri_merkle", "fri_fold"]
     stage = {}
-    for k in names:
+    for k in STAGE_NAMES:
         cnt, tot = backend.prof_query(k)
         if cnt:
             stage[k] = round(tot / args.steps, 3)
