@@ -320,7 +320,64 @@ __global__ __launch_bounds__(256) void transpose_kernel(const uint64_t *__restri
     }
 }
 
+// The same copy in tiles that follow the bit reversal (N = 2^L, L >= 8): the
+// natural rows r = (h << (L-3)) | (m << 5) | l (h < 8, l < 32, one middle m
+// per block) sit at the input rows (bitrev5(l) << (L-5)) | (bitrev(m) << 3) |
+// bitrev3(h), i.e. for each l a run of 8 consecutive input rows. A block moves
+// one such 256-row tile for 16 words of the row: its reads are runs of 8
+// row segments, its stores 32-row column runs.
+template <int E>
+__global__ __launch_bounds__(256) void transpose_bitrev_tile_kernel(const uint64_t *__restrict__ in,
+                                                                    uint64_t *__restrict__ out, uint64_t N,
+                                                                    uint32_t wpr, unsigned logN) {
+    constexpr int WC = 16;
+    __shared__ uint64_t tile[256][WC + 1];
+    const uint32_t nch = (wpr + WC - 1) / WC;  // a tile's word chunks are adjacent blocks
+    const uint32_t m = blockIdx.x / nch;
+    const uint32_t w0 = (blockIdx.x % nch) * WC;
+    const uint32_t wn = wpr - w0 < (uint32_t)WC ? wpr - w0 : (uint32_t)WC;
+    const unsigned mid = logN - 8;
+    const uint64_t mrev = mid ? (uint64_t)(__brev(m) >> (32 - mid)) : 0;
+    // all 16 loads of a thread in flight before the LDS stores
+    uint64_t v[WC];
+#pragma unroll
+    for (int q = 0; q < WC; q++) {
+        const uint32_t i = threadIdx.x + 256 * q;
+        const uint32_t ww = i % WC, j = (i / WC) % 8, l = i / (WC * 8);
+        const uint64_t s = ((uint64_t)(__brev(l) >> 27) << (logN - 5)) | (mrev << 3) | j;
+        v[q] = ww < wn ? in[s * wpr + w0 + ww] : 0;
+    }
+#pragma unroll
+    for (int q = 0; q < WC; q++) {
+        const uint32_t i = threadIdx.x + 256 * q;
+        const uint32_t ww = i % WC, j = (i / WC) % 8, l = i / (WC * 8);
+        tile[(__brev(j) >> 29) * 32 + l][ww] = v[q];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < WC; q++) {
+        const uint32_t i = threadIdx.x + 256 * q;
+        const uint32_t l = i % 32, ww = (i / 32) % WC, h = i / (32 * WC);
+        if (ww < wn) {
+            const uint32_t w = w0 + ww;
+            const uint64_t r = ((uint64_t)h << (logN - 3)) | ((uint64_t)m << 5) | l;
+            out[((uint64_t)(w / E) * N + r) * E + (w % E)] = tile[h * 32 + l][ww];
+        }
+    }
+}
+
 static void launch_transpose(const uint64_t *in, uint64_t *out, uint64_t N, uint32_t ncols, int E, hipStream_t st) {
+    const unsigned logN = (unsigned)__builtin_ctzll(N);
+    if (ncols == 0) return;
+    if (logN >= 8) {
+        const uint32_t wpr = ncols * (uint32_t)E;
+        dim3 g((unsigned)((N >> 8) * ((wpr + 15) / 16)));
+        if (E == 1)
+            hipLaunchKernelGGL(transpose_bitrev_tile_kernel<1>, g, dim3(256), 0, st, in, out, N, wpr, logN);
+        else
+            hipLaunchKernelGGL(transpose_bitrev_tile_kernel<3>, g, dim3(256), 0, st, in, out, N, wpr, logN);
+        return;
+    }
     dim3 grid((unsigned)((N + 31) / 32), (ncols + 31) / 32);
     if (E == 1)
         hipLaunchKernelGGL(transpose_kernel<1>, grid, dim3(256), 0, st, in, out, N, ncols);
