@@ -47,6 +47,18 @@ MSB16[lhs[2], lhs_msb] and MSB16[rhs[2], rhs_msb], IS_HALFWORD of the four
 difference halves and of lhs[1], rhs[1] (all with mu), ARE_BYTES[mu, 0] once,
 ALU[lhs, rhs, LT + 32 signed + 64 invert, out, 0] received with mu. Trace
 generation (lt.rs:133-175): deduplicated with counts as mu (split at 255).
+
+STORE (prover/src/tables/store.rs): receives the CPU's store MEMORY op and
+sends the MEMW write. Columns (store.rs:35-49): base_address (DWordWL),
+timestamp (DWordWL), write2, write4, write8 (width bits, none set for one
+byte), value (8 bytes, DWordBL), mu. Constraints (store.rs:259-283): IS_BIT
+of write2, write4, write8 and mu; (sum of widths)(1 - sum); (sum of
+widths)(1 - mu). Bus interactions (store.rs:129-248): MEMW[0, base_address,
+value bytes, timestamp, write2, write4, write8] sent with mu (16 elements),
+MEMORY[timestamp, base_address, value (DWordBL), 1 + 4 write2 + 8 write4 +
+16 write8, 0, 0] received with mu, ARE_BYTES[value_i, 0] sent with mu for each
+byte. Trace generation (store.rs:94-116): one row per operation (mu = 1), zero
+rows to a power of two, at least 4.
 """
 from __future__ import annotations
 
@@ -64,6 +76,7 @@ INV_SHIFT_32 = 18446744065119617026  # 2^-32 mod p (templates.rs:26)
 # bus ids (prover/src/tables/types.rs:255-336) and the ALU op code (:562-570)
 BUS_ARE_BYTES, BUS_IS_HALFWORD, BUS_IS_B20, BUS_MSB16, BUS_ZERO, BUS_BYTE_ALU, BUS_ALU, BUS_IS_WH = (
     0, 1, 2, 7, 8, 24, 25, 32)
+BUS_MEMW, BUS_MEMORY_OP = 14, 26
 ALU_OP_AND, ALU_OP_OR, ALU_OP_XOR, ALU_OP_EQ, ALU_OP_LT = 0, 1, 2, 3, 4
 MU_MAX = 255
 
@@ -388,3 +401,77 @@ def lt_table_air() -> AirProgram:
     return AirProgram("lt", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
                       boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
                       max_degree=max(3, 1 + absorbed))
+
+
+class StoreCols:
+    BASE_ADDRESS_0, BASE_ADDRESS_1, TIMESTAMP_0, TIMESTAMP_1, WRITE2, WRITE4, WRITE8 = 0, 1, 2, 3, 4, 5, 6
+    VALUE = tuple(range(7, 15))
+    MU = 15
+    NUM_COLUMNS = 16
+
+
+@dataclass(frozen=True)
+class StoreOperation:
+    base_address: int
+    timestamp: int
+    value: int
+    write2: bool = False
+    write4: bool = False
+    write8: bool = False
+
+    @classmethod
+    def of_width(cls, base_address: int, timestamp: int, value: int, nbytes: int) -> "StoreOperation":
+        """store.rs:69-78: a width of 1 sets no flag."""
+        return cls(base_address, timestamp, value, nbytes == 2, nbytes == 4, nbytes == 8)
+
+
+def generate_store_trace(ops) -> np.ndarray:
+    c = StoreCols
+    n = max(4, 1 << max(len(ops) - 1, 0).bit_length())
+    t = np.zeros((n, c.NUM_COLUMNS), dtype=np.uint64)
+    for i, op in enumerate(ops):
+        a, ts, v = op.base_address & (2**64 - 1), op.timestamp & (2**64 - 1), op.value & (2**64 - 1)
+        t[i, c.BASE_ADDRESS_0], t[i, c.BASE_ADDRESS_1] = a & 0xFFFFFFFF, a >> 32
+        t[i, c.TIMESTAMP_0], t[i, c.TIMESTAMP_1] = ts & 0xFFFFFFFF, ts >> 32
+        t[i, c.WRITE2], t[i, c.WRITE4], t[i, c.WRITE8] = int(op.write2), int(op.write4), int(op.write8)
+        for k in range(8):
+            t[i, c.VALUE[k]] = (v >> (8 * k)) & 0xFF
+        t[i, c.MU] = 1
+    return t
+
+
+def store_bus_interactions():
+    c = StoreCols
+    mu = Multiplicity("Column", (c.MU,))
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    zero = BusValue.of_linear([LinearTerm(0)])
+    its = [BusInteraction.sender(BUS_MEMW, mu, [zero, BusValue.packed(Packing.DWORD_WL, c.BASE_ADDRESS_0)]
+                                 + [d(col) for col in c.VALUE]
+                                 + [BusValue.packed(Packing.DWORD_WL, c.TIMESTAMP_0), d(c.WRITE2), d(c.WRITE4),
+                                    d(c.WRITE8)]),
+           BusInteraction.receiver(BUS_MEMORY_OP, mu, [
+               BusValue.packed(Packing.DWORD_WL, c.TIMESTAMP_0), BusValue.packed(Packing.DWORD_WL, c.BASE_ADDRESS_0),
+               BusValue.packed(Packing.DWORD_BL, c.VALUE[0]),
+               BusValue.of_linear([LinearTerm(1), LinearTerm(4, c.WRITE2), LinearTerm(8, c.WRITE4),
+                                   LinearTerm(16, c.WRITE8)]),
+               zero, zero])]
+    its += [BusInteraction.sender(BUS_ARE_BYTES, mu, [d(col), zero]) for col in c.VALUE]
+    return its
+
+
+def store_table_air() -> AirProgram:
+    c = StoreCols
+    b = ir.IrBuilder()
+    for idx, col in enumerate((c.WRITE2, c.WRITE4, c.WRITE8, c.MU)):
+        _emit_is_bit(b, idx, col)
+    one = b.const_base(1)
+    width = b.main(0, c.WRITE2) + b.main(0, c.WRITE4) + b.main(0, c.WRITE8)
+    b.emit(4, width * (one - width))
+    b.emit(5, width * (one - b.main(0, c.MU)))
+    its = store_bus_interactions()
+    n_total = emit_logup_constraints(b, its, 6)
+    prog = b.finish(c.NUM_COLUMNS)
+    pairs, absorbed = split_interactions(len(its))
+    return AirProgram("store", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
+                      boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
+                      max_degree=max(3 if pairs else 0, 1 + absorbed, 2))

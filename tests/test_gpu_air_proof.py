@@ -5,7 +5,7 @@ equal (canonical limbs), the canonical proof bytes are identical, and the
 verifier restatement accepts the GPU proofs (also after a round trip through
 the bytes).
 Tables: FibonacciMultiColumn (one composition part), Quadratic, a 4-bus
-LogUp table and the VM's EQ, BYTEWISE, IS_WH and LT tables (two parts),
+LogUp table and the VM's EQ, BYTEWISE, IS_WH, LT and STORE tables (two parts),
 alone and together in one transcript."""
 import numpy as np
 import pytest
@@ -57,6 +57,13 @@ def _tables(n, which, seed):
             ops = [V.LtOperation(int(x) * 2 + (i & 1), int(y) * 2 + 1, bool(i % 2), bool(i % 3 == 0))
                    for i, (x, y) in enumerate(xs)]
             tr = V.generate_lt_trace(ops)
+        elif kind == "store":
+            from lambda_vm_amd import vm_tables as V
+            a = V.store_table_air()
+            a.bus_descriptor = A.descriptor(a)
+            ops = [V.StoreOperation.of_width(int(x), int(t), int(v), (1, 2, 4, 8)[i % 4])
+                   for i, (x, t, v) in enumerate(rng.integers(0, 2**63, (n - 3, 3)))]
+            tr = V.generate_store_trace(ops)
         elif kind == "is_wh":
             from lambda_vm_amd import vm_tables as V
             a = V.is_wh_table_air()
@@ -82,6 +89,7 @@ def _f3rows(x):
                                                      (("bytewise", "eq"), 512, 4, 3, 10, 0),
                                                      (("is_wh", "bytewise", "eq"), 256, 4, 2, 8, 0),
                                                      (("lt", "eq"), 256, 4, 2, 8, 2),
+                                                     (("store", "lt"), 512, 4, 2, 8, 0),
                                                      (("fib", "quad", "logup"), 512, 4, 3, 10, 4)])
 def test_gpu_proofs_match_oracle_and_verify(gpu, which, n, blowup, k, q, grind):
     from lambda_vm_amd import prover
