@@ -304,18 +304,33 @@ __global__ __launch_bounds__(256) void deep_kernel(uint64_t N, const uint64_t *_
     for (int k = 0; k < KA; k++)
 #pragma unroll
         for (int j = 0; j < 3; j++) acc_zero(acc[k][j]);
+    // the block's 256 rows x DEEP_CHUNK columns are DEEP_CHUNK values per
+    // thread; the next chunk's loads are issued before the current chunk is
+    // accumulated, so they are in flight during the products
+    uint64_t nv[DEEP_CHUNK];
+    auto fetch = [&](uint64_t j0) {
+        const int cw = (int)(wm - j0 < DEEP_CHUNK ? wm - j0 : DEEP_CHUNK);
+#pragma unroll
+        for (int q = 0; q < DEEP_CHUNK; q++) {
+            const int e = t + 256 * q, rr = e / DEEP_CHUNK, cc = e % DEEP_CHUNK;
+            nv[q] = (cc < cw && r0 + rr < N) ? main_lde[(r0 + rr) * wm + j0 + cc] : 0;
+        }
+    };
+    if (wm) fetch(0);
     for (uint64_t j0 = 0; j0 < wm; j0 += DEEP_CHUNK) {
         const int cw = (int)(wm - j0 < DEEP_CHUNK ? wm - j0 : DEEP_CHUNK);
         __syncthreads();
-        for (int e = t; e < 256 * DEEP_CHUNK; e += 256) {
-            const int rr = e / DEEP_CHUNK, cc = e % DEEP_CHUNK;
-            tile[rr][cc] = (cc < cw && r0 + rr < N) ? main_lde[(r0 + rr) * wm + j0 + cc] : 0;
+#pragma unroll
+        for (int q = 0; q < DEEP_CHUNK; q++) {
+            const int e = t + 256 * q;
+            tile[e / DEEP_CHUNK][e % DEEP_CHUNK] = nv[q];
         }
         for (int e = t; e < DEEP_CHUNK * KA * 3; e += 256) {
             const int cc = e / (KA * 3), rem = e % (KA * 3);
             sg[cc][rem / 3][rem % 3] = cc < cw ? gtr[3 * ((j0 + cc) * K + rem / 3) + rem % 3] : 0;
         }
         __syncthreads();
+        if (j0 + DEEP_CHUNK < wm) fetch(j0 + DEEP_CHUNK);
 #pragma unroll 4
         for (int cc = 0; cc < DEEP_CHUNK; cc++) {
             const uint64_t v = tile[t][cc];
