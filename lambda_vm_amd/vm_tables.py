@@ -59,6 +59,19 @@ MEMORY[timestamp, base_address, value (DWordBL), 1 + 4 write2 + 8 write4 +
 16 write8, 0, 0] received with mu, ARE_BYTES[value_i, 0] sent with mu for each
 byte. Trace generation (store.rs:94-116): one row per operation (mu = 1), zero
 rows to a power of two, at least 4.
+
+BRANCH (prover/src/tables/branch.rs): next pc of a branch or jump, base +
+offset with the low bit cleared (base = register for JALR, else pc). Columns
+(branch.rs:38-88): pc, offset, register (DWordWL each), JALR, next_pc_high
+(3 halves), next_pc_low (2 bytes, bit 0 cleared), unmasked_low_byte, mu.
+Constraints (branch.rs:430-473): the carry pair of base + offset = unmasked
+next pc, each carry a bit, on the pc path weighted by 1 - JALR and on the
+register path by JALR; IS_BIT(JALR). Bus interactions (branch.rs:230-370):
+ARE_BYTES[mu, next_pc_low_1] once per row, BYTE_ALU[AND, unmasked_low_byte,
+254, next_pc_low_0] and IS_HALFWORD of the three high halves with mu,
+BRANCH[next_pc (two words), pc, offset, register, JALR] received with mu.
+Trace generation (branch.rs:165-222): deduplicated with counts as mu (split
+at 255), zero rows to a power of two, at least 4.
 """
 from __future__ import annotations
 
@@ -76,7 +89,7 @@ INV_SHIFT_32 = 18446744065119617026  # 2^-32 mod p (templates.rs:26)
 # bus ids (prover/src/tables/types.rs:255-336) and the ALU op code (:562-570)
 BUS_ARE_BYTES, BUS_IS_HALFWORD, BUS_IS_B20, BUS_MSB16, BUS_ZERO, BUS_BYTE_ALU, BUS_ALU, BUS_IS_WH = (
     0, 1, 2, 7, 8, 24, 25, 32)
-BUS_MEMW, BUS_MEMORY_OP = 14, 26
+BUS_MEMW, BUS_BRANCH, BUS_MEMORY_OP = 14, 17, 26
 ALU_OP_AND, ALU_OP_OR, ALU_OP_XOR, ALU_OP_EQ, ALU_OP_LT = 0, 1, 2, 3, 4
 MU_MAX = 255
 
@@ -475,3 +488,91 @@ def store_table_air() -> AirProgram:
     return AirProgram("store", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
                       boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
                       max_degree=max(3 if pairs else 0, 1 + absorbed, 2))
+
+
+class BranchCols:
+    PC_0, PC_1, OFFSET_0, OFFSET_1, REGISTER_0, REGISTER_1, JALR = 0, 1, 2, 3, 4, 5, 6
+    NEXT_PC_HIGH_0, NEXT_PC_HIGH_1, NEXT_PC_HIGH_2, NEXT_PC_LOW_0, NEXT_PC_LOW_1 = 7, 8, 9, 10, 11
+    UNMASKED_LOW_BYTE, MU = 12, 13
+    NUM_COLUMNS = 14
+
+
+BRANCH_MASK = 254  # clears the low bit of the target (branch.rs:95-99)
+
+
+@dataclass(frozen=True)
+class BranchOperation:
+    pc: int
+    offset: int
+    register: int
+    jalr: bool
+
+    def next_pc_unmasked(self) -> int:
+        return ((self.register if self.jalr else self.pc) + self.offset) & (2**64 - 1)
+
+    def next_pc(self) -> int:
+        return self.next_pc_unmasked() & ~1 & (2**64 - 1)
+
+
+def generate_branch_trace(ops) -> np.ndarray:
+    c = BranchCols
+    rows = dedup_rows(ops)
+    n = max(4, 1 << max(len(rows) - 1, 0).bit_length())
+    t = np.zeros((n, c.NUM_COLUMNS), dtype=np.uint64)
+    for i, (op, mu) in enumerate(rows):
+        for base, v in ((c.PC_0, op.pc), (c.OFFSET_0, op.offset), (c.REGISTER_0, op.register)):
+            v &= 2**64 - 1
+            t[i, base], t[i, base + 1] = v & 0xFFFFFFFF, v >> 32
+        t[i, c.JALR] = int(op.jalr)
+        nxt = op.next_pc()
+        t[i, c.NEXT_PC_HIGH_0] = (nxt >> 16) & 0xFFFF
+        t[i, c.NEXT_PC_HIGH_1] = (nxt >> 32) & 0xFFFF
+        t[i, c.NEXT_PC_HIGH_2] = (nxt >> 48) & 0xFFFF
+        t[i, c.NEXT_PC_LOW_0] = nxt & 0xFF
+        t[i, c.NEXT_PC_LOW_1] = (nxt >> 8) & 0xFF
+        t[i, c.UNMASKED_LOW_BYTE] = op.next_pc_unmasked() & 0xFF
+        t[i, c.MU] = mu
+    return t
+
+
+def branch_bus_interactions():
+    c = BranchCols
+    mu = Multiplicity("Column", (c.MU,))
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    its = [BusInteraction.sender(BUS_ARE_BYTES, ONE, [d(c.MU), d(c.NEXT_PC_LOW_1)]),
+           BusInteraction.sender(BUS_BYTE_ALU, mu, [BusValue.of_linear([LinearTerm(ALU_OP_AND)]),
+                                                    d(c.UNMASKED_LOW_BYTE),
+                                                    BusValue.of_linear([LinearTerm(BRANCH_MASK)]),
+                                                    d(c.NEXT_PC_LOW_0)])]
+    its += [BusInteraction.sender(BUS_IS_HALFWORD, mu, [d(col)])
+            for col in (c.NEXT_PC_HIGH_0, c.NEXT_PC_HIGH_1, c.NEXT_PC_HIGH_2)]
+    its.append(BusInteraction.receiver(BUS_BRANCH, mu, [
+        BusValue.of_linear([LinearTerm(1, c.NEXT_PC_LOW_0), LinearTerm(1 << 8, c.NEXT_PC_LOW_1),
+                            LinearTerm(1 << 16, c.NEXT_PC_HIGH_0)]),
+        BusValue.of_linear([LinearTerm(1, c.NEXT_PC_HIGH_1), LinearTerm(1 << 16, c.NEXT_PC_HIGH_2)]),
+        d(c.PC_0), d(c.PC_1), d(c.OFFSET_0), d(c.OFFSET_1), d(c.REGISTER_0), d(c.REGISTER_1), d(c.JALR)]))
+    return its
+
+
+def branch_table_air() -> AirProgram:
+    c = BranchCols
+    b = ir.IrBuilder()
+    one, inv = b.const_base(1), b.const_base(INV_SHIFT_32)
+    m = lambda col: b.main(0, col)  # noqa: E731
+    unmasked0 = m(c.UNMASKED_LOW_BYTE) + m(c.NEXT_PC_LOW_1) * b.const_base(1 << 8) \
+        + m(c.NEXT_PC_HIGH_0) * b.const_base(1 << 16)
+    unmasked1 = m(c.NEXT_PC_HIGH_1) + m(c.NEXT_PC_HIGH_2) * b.const_base(1 << 16)
+    jalr = m(c.JALR)
+    for idx, (cond, base0) in enumerate(((one - jalr, c.PC_0), (jalr, c.REGISTER_0))):
+        carry0 = (m(base0) + m(c.OFFSET_0) - unmasked0) * inv
+        carry1 = (m(base0 + 1) + m(c.OFFSET_1) + carry0 - unmasked1) * inv
+        b.emit(2 * idx, cond * carry0 * (one - carry0))
+        b.emit(2 * idx + 1, cond * carry1 * (one - carry1))
+    b.emit(4, jalr * (one - jalr))
+    its = branch_bus_interactions()
+    n_total = emit_logup_constraints(b, its, 5)
+    prog = b.finish(c.NUM_COLUMNS)
+    pairs, absorbed = split_interactions(len(its))
+    return AirProgram("branch", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
+                      boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
+                      max_degree=max(3, 1 + absorbed))

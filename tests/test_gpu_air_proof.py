@@ -5,8 +5,8 @@ equal (canonical limbs), the canonical proof bytes are identical, and the
 verifier restatement accepts the GPU proofs (also after a round trip through
 the bytes).
 Tables: FibonacciMultiColumn (one composition part), Quadratic, a 4-bus
-LogUp table and the VM's EQ, BYTEWISE, IS_WH, LT and STORE tables (two parts),
-alone and together in one transcript."""
+LogUp table and the VM's EQ, BYTEWISE, IS_WH, LT, STORE and BRANCH tables (two
+parts), alone and together in one transcript."""
 import numpy as np
 import pytest
 
@@ -64,6 +64,14 @@ def _tables(n, which, seed):
             ops = [V.StoreOperation.of_width(int(x), int(t), int(v), (1, 2, 4, 8)[i % 4])
                    for i, (x, t, v) in enumerate(rng.integers(0, 2**63, (n - 3, 3)))]
             tr = V.generate_store_trace(ops)
+        elif kind == "branch":
+            from lambda_vm_amd import vm_tables as V
+            a = V.branch_table_air()
+            a.bus_descriptor = A.descriptor(a)
+            ops = [V.BranchOperation(int(x) * 4, int(o) & (2**64 - 1), int(r), bool(i % 3 == 0))
+                   for i, (x, o, r) in enumerate(zip(rng.integers(0, 2**61, n - 3), rng.integers(-2**20, 2**20, n - 3),
+                                                     rng.integers(0, 2**63, n - 3)))]
+            tr = V.generate_branch_trace(ops)
         elif kind == "is_wh":
             from lambda_vm_amd import vm_tables as V
             a = V.is_wh_table_air()
@@ -90,6 +98,7 @@ def _f3rows(x):
                                                      (("is_wh", "bytewise", "eq"), 256, 4, 2, 8, 0),
                                                      (("lt", "eq"), 256, 4, 2, 8, 2),
                                                      (("store", "lt"), 512, 4, 2, 8, 0),
+                                                     (("branch", "store", "eq"), 256, 4, 2, 8, 2),
                                                      (("fib", "quad", "logup"), 512, 4, 3, 10, 4)])
 def test_gpu_proofs_match_oracle_and_verify(gpu, which, n, blowup, k, q, grind):
     from lambda_vm_amd import prover
