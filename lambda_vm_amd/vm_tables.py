@@ -72,6 +72,19 @@ ARE_BYTES[mu, next_pc_low_1] once per row, BYTE_ALU[AND, unmasked_low_byte,
 BRANCH[next_pc (two words), pc, offset, register, JALR] received with mu.
 Trace generation (branch.rs:165-222): deduplicated with counts as mu (split
 at 255), zero rows to a power of two, at least 4.
+
+MEMW_REGISTER (prover/src/tables/memw_register.rs): register reads and writes.
+Columns (memw_register.rs:55-82): address (register index), timestamp
+(DWordWL), value (2 words), old value (2 words), old timestamp low word (the
+high word is shared with the timestamp), mu_read, mu_write. Constraints
+(memw_register.rs:497-507): IS_BIT of both multiplicities and of their sum.
+Bus interactions (memw_register.rs:280-480), all but the last two with
+multiplicity mu_read + mu_write: IS_HALFWORD[timestamp_0 - old_timestamp_lo -
+1]; per word the Memory token [1, 2 address + i, 0, old timestamp, old_i]
+sent and [1, 2 address + i, 0, timestamp, value_i] received; the MEMW read
+(24 elements, with the old value) received with mu_read and the MEMW write
+(16 elements) with mu_write. Trace generation (memw_register.rs:210-247): one
+row per access, zero rows to a power of two, at least 4.
 """
 from __future__ import annotations
 
@@ -89,7 +102,7 @@ INV_SHIFT_32 = 18446744065119617026  # 2^-32 mod p (templates.rs:26)
 # bus ids (prover/src/tables/types.rs:255-336) and the ALU op code (:562-570)
 BUS_ARE_BYTES, BUS_IS_HALFWORD, BUS_IS_B20, BUS_MSB16, BUS_ZERO, BUS_BYTE_ALU, BUS_ALU, BUS_IS_WH = (
     0, 1, 2, 7, 8, 24, 25, 32)
-BUS_MEMW, BUS_BRANCH, BUS_MEMORY_OP = 14, 17, 26
+BUS_MEMW, BUS_MEMORY, BUS_BRANCH, BUS_MEMORY_OP = 14, 16, 17, 26
 ALU_OP_AND, ALU_OP_OR, ALU_OP_XOR, ALU_OP_EQ, ALU_OP_LT = 0, 1, 2, 3, 4
 MU_MAX = 255
 
@@ -576,3 +589,76 @@ def branch_table_air() -> AirProgram:
     return AirProgram("branch", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
                       boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
                       max_degree=max(3, 1 + absorbed))
+
+
+class MemwRegisterCols:
+    ADDRESS, TIMESTAMP_0, TIMESTAMP_1, VAL_0, VAL_1, OLD_0, OLD_1, OLD_TIMESTAMP_LO, MU_READ, MU_WRITE = range(10)
+    NUM_COLUMNS = 10
+
+
+@dataclass(frozen=True)
+class RegisterAccess:
+    """One register access (memw_register.rs:84-167): base_address = 2 x the
+    register index; the old timestamp shares the high word of the timestamp."""
+    base_address: int
+    timestamp: int
+    value: int
+    old: int
+    old_timestamp: int
+    is_read: bool
+
+
+def generate_memw_register_trace(accesses) -> np.ndarray:
+    c = MemwRegisterCols
+    n = max(4, 1 << max(len(accesses) - 1, 0).bit_length())
+    t = np.zeros((n, c.NUM_COLUMNS), dtype=np.uint64)
+    for i, a in enumerate(accesses):
+        if a.base_address % 2:
+            raise ValueError("register base address must be even")
+        t[i, c.ADDRESS] = a.base_address // 2
+        t[i, c.TIMESTAMP_0], t[i, c.TIMESTAMP_1] = a.timestamp & 0xFFFFFFFF, a.timestamp >> 32
+        t[i, c.VAL_0], t[i, c.VAL_1] = a.value & 0xFFFFFFFF, a.value >> 32
+        t[i, c.OLD_0], t[i, c.OLD_1] = a.old & 0xFFFFFFFF, a.old >> 32
+        t[i, c.OLD_TIMESTAMP_LO] = a.old_timestamp & 0xFFFFFFFF
+        t[i, c.MU_READ], t[i, c.MU_WRITE] = int(a.is_read), int(not a.is_read)
+    return t
+
+
+def memw_register_bus_interactions():
+    c = MemwRegisterCols
+    mu_sum = Multiplicity("Sum", (c.MU_READ, c.MU_WRITE))
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    k = lambda v: BusValue.of_linear([LinearTerm(v)])  # noqa: E731
+    its = [BusInteraction.sender(BUS_IS_HALFWORD, mu_sum, [BusValue.of_linear(
+        [LinearTerm(1, c.TIMESTAMP_0), LinearTerm(-1, c.OLD_TIMESTAMP_LO), LinearTerm(-1)])])]
+    for i in range(2):
+        its.append(BusInteraction.sender(BUS_MEMORY, mu_sum, [
+            k(1), BusValue.of_linear([LinearTerm(2, c.ADDRESS), LinearTerm(i)]), k(0), d(c.OLD_TIMESTAMP_LO),
+            d(c.TIMESTAMP_1), d(c.OLD_0 + i)]))
+    for i in range(2):
+        its.append(BusInteraction.receiver(BUS_MEMORY, mu_sum, [
+            k(1), BusValue.of_linear([LinearTerm(2, c.ADDRESS), LinearTerm(i)]), k(0), d(c.TIMESTAMP_0),
+            d(c.TIMESTAMP_1), d(c.VAL_0 + i)]))
+    addr = BusValue.of_linear([LinearTerm(2, c.ADDRESS)])
+    write = [k(1), addr, k(0), d(c.VAL_0), d(c.VAL_1)] + [k(0)] * 6 + [d(c.TIMESTAMP_0), d(c.TIMESTAMP_1), k(1), k(0),
+                                                                         k(0)]
+    its.append(BusInteraction.receiver(BUS_MEMW, Multiplicity("Column", (c.MU_READ,)),
+                                       [d(c.OLD_0), d(c.OLD_1)] + [k(0)] * 6 + write))
+    its.append(BusInteraction.receiver(BUS_MEMW, Multiplicity("Column", (c.MU_WRITE,)), list(write)))
+    return its
+
+
+def memw_register_table_air() -> AirProgram:
+    c = MemwRegisterCols
+    b = ir.IrBuilder()
+    _emit_is_bit(b, 0, c.MU_READ)
+    _emit_is_bit(b, 1, c.MU_WRITE)
+    mu_sum = b.main(0, c.MU_READ) + b.main(0, c.MU_WRITE)
+    b.emit(2, mu_sum * (b.const_base(1) - mu_sum))
+    its = memw_register_bus_interactions()
+    n_total = emit_logup_constraints(b, its, 3)
+    prog = b.finish(c.NUM_COLUMNS)
+    pairs, absorbed = split_interactions(len(its))
+    return AirProgram("memw_register", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False),
+                      [0] * n_total, boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
+                      max_degree=max(3 if pairs else 0, 1 + absorbed, 2))

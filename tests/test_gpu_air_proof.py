@@ -5,8 +5,8 @@ equal (canonical limbs), the canonical proof bytes are identical, and the
 verifier restatement accepts the GPU proofs (also after a round trip through
 the bytes).
 Tables: FibonacciMultiColumn (one composition part), Quadratic, a 4-bus
-LogUp table and the VM's EQ, BYTEWISE, IS_WH, LT, STORE and BRANCH tables (two
-parts), alone and together in one transcript."""
+LogUp table and the VM's EQ, BYTEWISE, IS_WH, LT, STORE, BRANCH and
+MEMW_REGISTER tables (two parts), alone and together in one transcript."""
 import numpy as np
 import pytest
 
@@ -72,6 +72,16 @@ def _tables(n, which, seed):
                    for i, (x, o, r) in enumerate(zip(rng.integers(0, 2**61, n - 3), rng.integers(-2**20, 2**20, n - 3),
                                                      rng.integers(0, 2**63, n - 3)))]
             tr = V.generate_branch_trace(ops)
+        elif kind == "memw_register":
+            from lambda_vm_amd import vm_tables as V
+            a = V.memw_register_table_air()
+            a.bus_descriptor = A.descriptor(a)
+            acc, ts = [], 5 << 32
+            for i, (r, dt, v) in enumerate(zip(rng.integers(0, 32, n - 3), rng.integers(1, 1 << 16, n - 3),
+                                               rng.integers(0, 2**63, n - 3))):
+                acc.append(V.RegisterAccess(2 * int(r), ts + int(dt), int(v), int(v) ^ (i & 7), ts, bool(i % 2)))
+                ts += int(dt)
+            tr = V.generate_memw_register_trace(acc)
         elif kind == "is_wh":
             from lambda_vm_amd import vm_tables as V
             a = V.is_wh_table_air()
@@ -99,6 +109,7 @@ def _f3rows(x):
                                                      (("lt", "eq"), 256, 4, 2, 8, 2),
                                                      (("store", "lt"), 512, 4, 2, 8, 0),
                                                      (("branch", "store", "eq"), 256, 4, 2, 8, 2),
+                                                     (("memw_register", "branch"), 512, 4, 3, 8, 0),
                                                      (("fib", "quad", "logup"), 512, 4, 3, 10, 4)])
 def test_gpu_proofs_match_oracle_and_verify(gpu, which, n, blowup, k, q, grind):
     from lambda_vm_amd import prover
