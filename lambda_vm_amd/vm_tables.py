@@ -85,6 +85,20 @@ sent and [1, 2 address + i, 0, timestamp, value_i] received; the MEMW read
 (24 elements, with the old value) received with mu_read and the MEMW write
 (16 elements) with mu_write. Trace generation (memw_register.rs:210-247): one
 row per access, zero rows to a power of two, at least 4.
+
+LOAD (prover/src/tables/load.rs): loads of 1, 2, 4 or 8 bytes with sign or
+zero extension. Columns (load.rs:36-68): base_address, timestamp (DWordWL),
+read2, read4, read8, signed, res (8 bytes, already extended), sign_bit, mu.
+Constraints (load.rs:505-590): IS_BIT of signed and the three width flags;
+width sum a bit; a width implies mu; the bytes above the loaded width equal
+signed * sign_bit * 255 (bytes 4..8 unless read8, 2..4 unless read4/read8,
+byte 1 on one-byte loads); IS_BIT(mu). Bus interactions (load.rs:236-470):
+the MEMW read (24 elements: res as old and as value) sent with mu; MSB8[the
+top loaded byte, sign_bit] sent with mu - read2 - read4 - read8 (byte 0),
+read2 (byte 1) or read4 (byte 3); MEMORY[timestamp, base_address, 0, 0,
+2 signed + 4 read2 + 8 read4 + 16 read8, res (DWordBL)] received with mu.
+Trace generation (load.rs:182-232): one row per operation (mu = 1), zero rows
+to a power of two, at least 4.
 """
 from __future__ import annotations
 
@@ -102,7 +116,7 @@ INV_SHIFT_32 = 18446744065119617026  # 2^-32 mod p (templates.rs:26)
 # bus ids (prover/src/tables/types.rs:255-336) and the ALU op code (:562-570)
 BUS_ARE_BYTES, BUS_IS_HALFWORD, BUS_IS_B20, BUS_MSB16, BUS_ZERO, BUS_BYTE_ALU, BUS_ALU, BUS_IS_WH = (
     0, 1, 2, 7, 8, 24, 25, 32)
-BUS_MEMW, BUS_MEMORY, BUS_BRANCH, BUS_MEMORY_OP = 14, 16, 17, 26
+BUS_MSB8, BUS_MEMW, BUS_MEMORY, BUS_BRANCH, BUS_MEMORY_OP = 6, 14, 16, 17, 26
 ALU_OP_AND, ALU_OP_OR, ALU_OP_XOR, ALU_OP_EQ, ALU_OP_LT = 0, 1, 2, 3, 4
 MU_MAX = 255
 
@@ -662,3 +676,97 @@ def memw_register_table_air() -> AirProgram:
     return AirProgram("memw_register", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False),
                       [0] * n_total, boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
                       max_degree=max(3 if pairs else 0, 1 + absorbed, 2))
+
+
+class LoadCols:
+    BASE_ADDRESS_0, BASE_ADDRESS_1, TIMESTAMP_0, TIMESTAMP_1, READ2, READ4, READ8, SIGNED = range(8)
+    RES = tuple(range(8, 16))
+    SIGN_BIT, MU = 16, 17
+    NUM_COLUMNS = 18
+
+
+@dataclass(frozen=True)
+class LoadOperation:
+    base_address: int
+    timestamp: int
+    width: int
+    signed: bool
+    res: tuple  # 8 bytes, already extended
+
+    @classmethod
+    def of_memory(cls, base_address: int, timestamp: int, width: int, signed: bool, raw: int) -> "LoadOperation":
+        """The extended result of loading `width` bytes of `raw` (RISC-V LB..LD,
+        LBU..LWU)."""
+        v = raw & ((1 << (8 * width)) - 1)
+        if signed and v >> (8 * width - 1):
+            v |= (2**64 - 1) ^ ((1 << (8 * width)) - 1)
+        return cls(base_address, timestamp, width, signed, tuple((v >> (8 * k)) & 0xFF for k in range(8)))
+
+    def sign_bit(self) -> int:
+        """load.rs:113-122: the top bit of the top loaded byte."""
+        return (self.res[{1: 0, 2: 1, 4: 3, 8: 7}.get(self.width, 0)] >> 7) & 1
+
+
+def generate_load_trace(ops) -> np.ndarray:
+    c = LoadCols
+    n = max(4, 1 << max(len(ops) - 1, 0).bit_length())
+    t = np.zeros((n, c.NUM_COLUMNS), dtype=np.uint64)
+    for i, op in enumerate(ops):
+        a, ts = op.base_address & (2**64 - 1), op.timestamp & (2**64 - 1)
+        t[i, c.BASE_ADDRESS_0], t[i, c.BASE_ADDRESS_1] = a & 0xFFFFFFFF, a >> 32
+        t[i, c.TIMESTAMP_0], t[i, c.TIMESTAMP_1] = ts & 0xFFFFFFFF, ts >> 32
+        t[i, c.READ2], t[i, c.READ4], t[i, c.READ8] = int(op.width == 2), int(op.width == 4), int(op.width == 8)
+        t[i, c.SIGNED] = int(op.signed)
+        for k in range(8):
+            t[i, c.RES[k]] = op.res[k]
+        t[i, c.SIGN_BIT] = op.sign_bit()
+        t[i, c.MU] = 1
+    return t
+
+
+def load_bus_interactions():
+    c = LoadCols
+    mu = Multiplicity("Column", (c.MU,))
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    k = lambda v: BusValue.of_linear([LinearTerm(v)])  # noqa: E731
+    res = [d(col) for col in c.RES]
+    its = [BusInteraction.sender(BUS_MEMW, mu, res + [k(0), d(c.BASE_ADDRESS_0), d(c.BASE_ADDRESS_1)] + list(res)
+                                 + [d(c.TIMESTAMP_0), d(c.TIMESTAMP_1), d(c.READ2), d(c.READ4), d(c.READ8)]),
+           BusInteraction.sender(BUS_MSB8, Multiplicity("Linear", linear=(
+               LinearTerm(1, c.MU), LinearTerm(-1, c.READ2), LinearTerm(-1, c.READ4), LinearTerm(-1, c.READ8))), [d(c.RES[0]), d(c.SIGN_BIT)]),
+           BusInteraction.sender(BUS_MSB8, Multiplicity("Column", (c.READ2,)), [d(c.RES[1]), d(c.SIGN_BIT)]),
+           BusInteraction.sender(BUS_MSB8, Multiplicity("Column", (c.READ4,)), [d(c.RES[3]), d(c.SIGN_BIT)]),
+           BusInteraction.receiver(BUS_MEMORY_OP, mu, [
+               BusValue.packed(Packing.DWORD_WL, c.TIMESTAMP_0), BusValue.packed(Packing.DWORD_WL, c.BASE_ADDRESS_0),
+               k(0), k(0),
+               BusValue.of_linear([LinearTerm(2, c.SIGNED), LinearTerm(4, c.READ2), LinearTerm(8, c.READ4),
+                                   LinearTerm(16, c.READ8)]),
+               BusValue.packed(Packing.DWORD_BL, c.RES[0])])]
+    return its
+
+
+def load_table_air() -> AirProgram:
+    c = LoadCols
+    b = ir.IrBuilder()
+    one = b.const_base(1)
+    m = lambda col: b.main(0, col)  # noqa: E731
+    for idx, col in enumerate((c.SIGNED, c.READ2, c.READ4, c.READ8)):
+        _emit_is_bit(b, idx, col)
+    r2, r4, r8 = m(c.READ2), m(c.READ4), m(c.READ8)
+    width = r2 + r4 + r8
+    b.emit(4, width * (one - width))
+    b.emit(5, width * (one - m(c.MU)))
+    ext = m(c.SIGNED) * m(c.SIGN_BIT) * b.const_base(255)
+    for off, i in enumerate(range(4, 8)):
+        b.emit(6 + off, (one - r8) * (m(c.RES[i]) - ext))
+    for off, i in enumerate(range(2, 4)):
+        b.emit(10 + off, (one - r4 - r8) * (m(c.RES[i]) - ext))
+    b.emit(12, (one - r2 - r4 - r8) * (m(c.RES[1]) - ext))
+    _emit_is_bit(b, 13, c.MU)
+    its = load_bus_interactions()
+    n_total = emit_logup_constraints(b, its, 14)
+    prog = b.finish(c.NUM_COLUMNS)
+    pairs, absorbed = split_interactions(len(its))
+    return AirProgram("load", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
+                      boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
+                      max_degree=max(3, 1 + absorbed))

@@ -5,8 +5,8 @@ equal (canonical limbs), the canonical proof bytes are identical, and the
 verifier restatement accepts the GPU proofs (also after a round trip through
 the bytes).
 Tables: FibonacciMultiColumn (one composition part), Quadratic, a 4-bus
-LogUp table and the VM's EQ, BYTEWISE, IS_WH, LT, STORE, BRANCH and
-MEMW_REGISTER tables (two parts), alone and together in one transcript."""
+LogUp table and the VM's EQ, BYTEWISE, IS_WH, LT, STORE, BRANCH, MEMW_REGISTER
+and LOAD tables (two parts), alone and together in one transcript."""
 import numpy as np
 import pytest
 
@@ -82,6 +82,14 @@ def _tables(n, which, seed):
                 acc.append(V.RegisterAccess(2 * int(r), ts + int(dt), int(v), int(v) ^ (i & 7), ts, bool(i % 2)))
                 ts += int(dt)
             tr = V.generate_memw_register_trace(acc)
+        elif kind == "load":
+            from lambda_vm_amd import vm_tables as V
+            a = V.load_table_air()
+            a.bus_descriptor = A.descriptor(a)
+            ops = [V.LoadOperation.of_memory(int(x), 10 + i, (1, 2, 4, 8)[i % 4], bool(i % 3) and i % 4 != 3, int(v))
+                   for i, (x, v) in enumerate(zip(rng.integers(0, 2**40, n - 3),
+                                                  rng.integers(0, 2**64, n - 3, dtype=np.uint64)))]
+            tr = V.generate_load_trace(ops)
         elif kind == "is_wh":
             from lambda_vm_amd import vm_tables as V
             a = V.is_wh_table_air()
@@ -110,6 +118,7 @@ def _f3rows(x):
                                                      (("store", "lt"), 512, 4, 2, 8, 0),
                                                      (("branch", "store", "eq"), 256, 4, 2, 8, 2),
                                                      (("memw_register", "branch"), 512, 4, 3, 8, 0),
+                                                     (("load", "memw_register"), 256, 4, 2, 8, 2),
                                                      (("fib", "quad", "logup"), 512, 4, 3, 10, 4)])
 def test_gpu_proofs_match_oracle_and_verify(gpu, which, n, blowup, k, q, grind):
     from lambda_vm_amd import prover
