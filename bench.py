@@ -500,9 +500,9 @@ STAGE_NAMES = ["ntt_lde", "keccak_leaves", "merkle_inner", "logup_terms", "logup
 
 
 def run_vm(args) -> dict:
-    """Full proof of five of the VM's own tables in one transcript, at 2^20 rows
+    """Full proof of eight of the VM's own tables in one transcript, at 2^20 rows
     (IS_WH 2^18): EQ (12 columns), BYTEWISE (26), IS_WH (4), LT (17), STORE
-    (16), blowup 4,
+    (16), BRANCH (14), MEMW_REGISTER (10), LOAD (18), blowup 4,
     their constraints and bus interactions as captured from the VM's table
     definitions. Traces are uniform draws of the tables' shapes (the prover's
     work does not depend on whether the constraints hold; valid traces are
@@ -510,8 +510,9 @@ def run_vm(args) -> dict:
     from lambda_vm_amd import _native, air as A, backend, prover, vm_tables as V
 
     _native.ensure_device(0)
-    airs = [V.eq_table_air(), V.bytewise_table_air(), V.is_wh_table_air(), V.lt_table_air(), V.store_table_air()]
-    rows = [1 << 20, 1 << 20, 1 << 18, 1 << 20, 1 << 20]
+    airs = [V.eq_table_air(), V.bytewise_table_air(), V.is_wh_table_air(), V.lt_table_air(), V.store_table_air(),
+            V.branch_table_air(), V.memw_register_table_air(), V.load_table_air()]
+    rows = [1 << 20, 1 << 20, 1 << 18, 1 << 20, 1 << 20, 1 << 20, 1 << 20, 1 << 20]
     for a in airs:
         a.bus_descriptor = A.descriptor(a)
     traces = [backend.splitmix_table_dev(n, a.num_main, 2, 40 + t) for t, (a, n) in enumerate(zip(airs, rows))]
@@ -551,8 +552,9 @@ def run_vm(args) -> dict:
         "dtype": "u64 (Goldilocks field)",
         "data": "synthetic: device SplitMix64 traces of the VM tables' shapes (draws mod p)",
         "config": {
-            "workload": ("full proof of five VM tables in one transcript: EQ 2^20x12, BYTEWISE 2^20x26, "
-                         "IS_WH 2^18x4, LT 2^20x17, STORE 2^20x16 (their constraints and bus interactions), blowup 4, "
+            "workload": ("full proof of eight VM tables in one transcript: EQ 2^20x12, BYTEWISE 2^20x26, "
+                         "IS_WH 2^18x4, LT 2^20x17, STORE 2^20x16, BRANCH 2^20x14, MEMW_REGISTER 2^20x10, "
+                         "LOAD 2^20x18 (their constraints and bus interactions), blowup 4, "
                          "FRI terminal 2^(2+%d), %d queries" % (args.fri_k, args.queries)),
             "lde_cells": cells,
             "proof_ms": [round(x * 1e3, 3) for x in times],
