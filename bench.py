@@ -427,7 +427,7 @@ def run_constraints(args) -> dict:
             extra = (dict(rap=[z, al], alpha=ap, table_offset=toff) if i == 2 else {})
             H = backend.eval_composition(a.device, mains[i], aux_h if i == 2 else None, B, specs[i][1], offset,
                                          betas[i], groups, ex, bnd, [r3() for _ in bnd], **extra)
-            parts = backend.composition_commit_d2(H, specs[i][1] * B, offset)
+            parts = backend.composition_commit_d2(H, specs[i][1] * B, specs[i][1], offset)
             out.append(parts.root)
             H.free()
             parts.close()

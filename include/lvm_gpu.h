@@ -344,9 +344,23 @@ int lvm_set_constraint_jit(int on);
  * lvm_eval_composition writes them); H0(x^2) = (H(x) + H(-x))/2 and
  * H1(x^2) = (H(x) - H(-x))/(2x) are extended from the offset^2-coset of size
  * N/2 to the offset-coset of size N and committed with row-pair leaves over
- * [H0 | H1] (6 words per row). parts_out: the parts LDE (N rows x 6 words). */
-int lvm_composition_commit_d2(const uint64_t *dev_h, size_t N, uint64_t coset_offset, uint8_t root_out[32],
-                              lvm_lde **parts_out);
+ * [H0 | H1] (6 words per row). parts_out: the parts LDE (N rows x 6 words)
+ * whose trace rows are `trace_rows` (the table's n: the parts' out-of-domain
+ * values read every (N/n)-th row, prover.rs:2012-2100). */
+int lvm_composition_commit_d2(const uint64_t *dev_h, size_t N, size_t trace_rows, uint64_t coset_offset,
+                              uint8_t root_out[32], lvm_lde **parts_out);
+/* The composition polynomial's parts for any part count P = degree bound / n
+ * (crypto/stark/src/prover.rs:1683-1900), committed with row-pair leaves over
+ * the P Fp3 columns of each row: P = 1 the values themselves
+ * (lvm_commit_rows_dev, constraint_interp.rs:514 `comp_h_to_slabs`); P = 2
+ * lvm_composition_commit_d2; P = 3..8 the interpolant's coefficients split by
+ * stride P (`break_in_parts`, crypto/math/src/polynomial.rs:137-150) and each
+ * part evaluated on the offset-coset of size N
+ * (crypto/math-cuda/src/lde.rs:1651 `evaluate_poly_coset_batch_ext3_into_keep`
+ * via crypto/stark/src/gpu_lde.rs:2262). dev_h: N composition values in
+ * bit-reversed row order; parts_out: N rows x 3P words, trace rows n. */
+int lvm_composition_commit_parts(const uint64_t *dev_h, size_t N, size_t trace_rows, size_t num_parts,
+                                 uint64_t coset_offset, uint8_t root_out[32], lvm_lde **parts_out);
 
 /* ---------------- grinding ----------------
  * crypto/math-cuda/src/grinding.rs:29 `generate_nonce_gpu`: the smallest

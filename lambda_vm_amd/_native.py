@@ -88,7 +88,9 @@ SIGNATURES = {
     "lvm_commit_rows_dev": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, u8p, vpp]),
     "lvm_prepare_constraints": (C.c_int, [C.c_void_p, C.c_void_p]),
     "lvm_set_constraint_jit": (C.c_int, [C.c_int]),
-    "lvm_composition_commit_d2": (C.c_int, [C.c_void_p, C.c_size_t, C.c_uint64, u8p, vpp]),
+    "lvm_composition_commit_d2": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, C.c_uint64, u8p, vpp]),
+    "lvm_composition_commit_parts": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_uint64, u8p,
+                                               vpp]),
     "lvm_generate_nonce": (C.c_int, [u8p, C.c_uint8, C.c_uint64, u64p]),
     "lvm_transcript_new": (C.c_int, [u8p, C.c_size_t, vpp]),
     "lvm_transcript_clone": (C.c_int, [C.c_void_p, vpp]),

@@ -38,7 +38,24 @@ class Boundary:
 
 @dataclass
 class AirProgram:
-    """A captured AIR: lowered program, per-constraint end exemptions, boundaries."""
+    """A captured AIR: lowered program, per-constraint end exemptions, boundaries.
+
+    bus: an AirWithBuses table (crypto/stark/src/lookup.rs:1000-1085; every VM
+    table and the synthetic LogUp table) rather than one of the reference's
+    example AIRs. The two kinds differ in three pieces of public shape
+    metadata that fix the proof layout:
+      - composition parts = composition_poly_degree_bound(n) / n: for bus AIRs
+        (max_degree - 1).max(1) with max_degree the larger of the base
+        constraints' and LogUp's (lookup.rs:1063-1078, logup_max_degree
+        :2290-2301); for an example AIR its own bound (`example_parts`:
+        fibonacci_multi_column.rs:93 gives 1, quadratic_air.rs:149 gives 2);
+      - frame rows: bus AIRs have transition offsets [0, 1] (lookup.rs:950);
+        example AIRs their own (fibonacci_multi_column.rs:188 [0, 1, 2],
+        quadratic_air.rs:88 [0, 1]);
+      - columns opened at the next rows: bus AIRs only the LogUp accumulated
+        column (lookup.rs:1042-1053), none without interactions; example AIRs
+        every column (the trait default, traits.rs:217-220).
+    """
     name: str
     num_main: int
     num_aux: int  # Fp3 columns
@@ -48,10 +65,34 @@ class AirProgram:
     boundary: list = field(default_factory=list)
     interactions: list = field(default_factory=list)
     max_degree: int = 1
+    bus: bool = True
+    example_parts: int | None = None
 
     @property
     def num_constraints(self) -> int:
         return len(self.program.roots)
+
+    @property
+    def num_parts(self) -> int:
+        """Composition parts: degree bound / trace length (prover.rs:1705)."""
+        if self.bus:
+            return max(self.max_degree - 1, 1)
+        if self.example_parts is None:
+            raise ValueError(f"{self.name}: an example AIR states its composition degree bound")
+        return self.example_parts
+
+    @property
+    def frame_rows(self) -> int:
+        """Rows of the OOD frame: the number of transition offsets."""
+        return 2 if self.bus else int(self.device.max_offset) + 1
+
+    def next_row_cols(self) -> list:
+        """Full-width ([main | aux]) indices of the columns opened at the next
+        rows, ascending."""
+        if self.bus:
+            # the accumulated column is the last aux column (after the committed pairs)
+            return [self.num_main + self.num_aux - 1] if self.interactions else []
+        return list(range(self.num_main + self.num_aux))
 
     def groups(self):
         """(group of each constraint, exemption count of each group)."""
@@ -66,7 +107,7 @@ def fibonacci_multi_column(m: int) -> AirProgram:
         b.emit(c, a2 - a1 - a0)
     prog = b.finish(m)
     return AirProgram(f"fibonacci_multi_column_{m}", m, 0, prog, ir.lower(prog, pin_roots=False), [2] * m,
-                      max_degree=1)
+                      max_degree=1, bus=False, example_parts=1)
 
 
 def quadratic_multi(m: int) -> AirProgram:
@@ -75,7 +116,8 @@ def quadratic_multi(m: int) -> AirProgram:
         x, x_next = b.main(0, c), b.main(1, c)
         b.emit(c, x_next - x * x)
     prog = b.finish(m)
-    return AirProgram(f"quadratic_{m}", m, 0, prog, ir.lower(prog, pin_roots=False), [1] * m, max_degree=2)
+    return AirProgram(f"quadratic_{m}", m, 0, prog, ir.lower(prog, pin_roots=False), [1] * m, max_degree=2,
+                      bus=False, example_parts=2)
 
 
 # ---- LogUp constraint emission (lookup.rs:2160-2325) ----
@@ -196,6 +238,35 @@ def logup_air(num_main: int, interactions) -> AirProgram:
     deg = max(3 if pairs else 0, 1 + absorbed)
     return AirProgram(f"logup_{len(interactions)}", num_main, pairs + 1, prog, ir.lower(prog, pin_roots=False),
                       [0] * n_next, boundary=bnd, interactions=list(interactions), max_degree=deg)
+
+
+def product_bus_air(interactions) -> AirProgram:
+    """A bus table of 25 main columns with one degree-4 base constraint
+    (columns 20..23 multiply to column 24 on every row) beside the LogUp
+    constraints of `interactions`: max degree 4, so three composition parts
+    (the break_in_parts path of crypto/stark/src/prover.rs:1858-1900)."""
+    b = ir.IrBuilder()
+    x = [b.main(0, c) for c in range(20, 25)]
+    b.emit(0, x[0] * x[1] * x[2] * x[3] - x[4])
+    n_total = emit_logup_constraints(b, interactions, 1) if interactions else 1
+    prog = b.finish(25)
+    pairs, absorbed = split_interactions(len(interactions)) if interactions else (0, 0)
+    num_aux = pairs + 1 if interactions else 0
+    bnd = [Boundary(pairs, True, 0, (0, 0, 0))] if interactions else []
+    deg = max(4, 3 if pairs else 0, 1 + absorbed if interactions else 0)
+    return AirProgram(f"product_bus_{len(interactions)}", 25, num_aux, prog, ir.lower(prog, pin_roots=False),
+                      [0] * n_total, boundary=bnd, interactions=list(interactions), max_degree=deg)
+
+
+def product_trace(n: int, cfg: int = 9) -> np.ndarray:
+    """A trace of product_bus_air: uniform columns, column 24 the product of 20..23."""
+    from .workload import trace_table
+    t = trace_table(cfg, 0, n, 25)
+    prod = np.ones(n, dtype=object)
+    for c in range(20, 24):
+        prod = prod * t[:, c].astype(object) % P
+    t[:, 24] = prod.astype(np.uint64)
+    return t
 
 
 def four_bus_interactions():

@@ -474,12 +474,25 @@ def commit_rows_dev(dev_rows: DeviceBuffer, lde_rows: int, words_per_row: int, t
     return GpuLde(h.value)
 
 
-def composition_commit_d2(dev_h: DeviceBuffer, lde_rows: int, coset_offset: int) -> GpuLde:
+def composition_commit_d2(dev_h: DeviceBuffer, lde_rows: int, trace_rows: int, coset_offset: int) -> GpuLde:
     """Degree-2 split of the bit-reversed composition values, extension of both
-    halves to the LDE and their commitment: the parts LDE (lde_rows rows of [H0 | H1])."""
+    halves to the LDE and their commitment: the parts LDE (lde_rows rows of
+    [H0 | H1]) over a table of trace_rows rows."""
     root = np.zeros(32, dtype=np.uint8)
     h = C.c_void_p()
-    N.call("lvm_composition_commit_d2", C.c_void_p(dev_h.ptr), lde_rows, coset_offset, _p8(root), C.byref(h))
+    N.call("lvm_composition_commit_d2", C.c_void_p(dev_h.ptr), lde_rows, trace_rows, coset_offset, _p8(root),
+           C.byref(h))
+    return GpuLde(h.value)
+
+
+def composition_commit_parts(dev_h: DeviceBuffer, lde_rows: int, trace_rows: int, num_parts: int,
+                             coset_offset: int) -> GpuLde:
+    """The composition polynomial's num_parts parts on the LDE, committed
+    (crypto/stark/src/prover.rs:1683-1900): lde_rows rows of num_parts Fp3."""
+    root = np.zeros(32, dtype=np.uint8)
+    h = C.c_void_p()
+    N.call("lvm_composition_commit_parts", C.c_void_p(dev_h.ptr), lde_rows, trace_rows, num_parts, coset_offset,
+           _p8(root), C.byref(h))
     return GpuLde(h.value)
 
 

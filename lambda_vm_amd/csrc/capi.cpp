@@ -75,15 +75,22 @@ int check_fault() {
     return LVM_OK;
 }
 
-// LDE + commit from device-resident trace rows (n x w words) and weights.
-int lde_core(Device *d, const uint64_t *d_trace, size_t n, size_t w, size_t blowup, const uint64_t *d_weights,
-             lvm_lde **out, uint8_t root_out[32]) {
+// Shape of a coset LDE: checked before any device work (a table of one row
+// has no transform; the domain must fit the 2^32 two-adic subgroup).
+int check_lde_shape(size_t n, size_t w, size_t blowup) {
     if (!is_pow2(n) || n < 2) return fail(LVM_ERR_ARG, "trace rows must be a power of two >= 2");
     if (!is_pow2(blowup)) return fail(LVM_ERR_ARG, "blowup must be a power of two");
     if (w == 0) return fail(LVM_ERR_ARG, "no columns");
     if (w > (1u << 30)) return fail(LVM_ERR_ARG, "too many columns");
+    if (log2u(n) + log2u(blowup) > 32) return fail(LVM_ERR_ARG, "LDE domain exceeds the 2^32 two-adic subgroup");
+    return LVM_OK;
+}
+
+// LDE + commit from device-resident trace rows (n x w words) and weights.
+int lde_core(Device *d, const uint64_t *d_trace, size_t n, size_t w, size_t blowup, const uint64_t *d_weights,
+             lvm_lde **out, uint8_t root_out[32]) {
+    TRY(check_lde_shape(n, w, blowup));
     const size_t N = n * blowup;
-    if (log2u(N) > 32) return fail(LVM_ERR_ARG, "LDE domain exceeds the 2^32 two-adic subgroup");
     TRY(check_fault());
     hipStream_t st = thread_stream(d);
     const size_t L = N / 2;
@@ -130,9 +137,9 @@ int lde_core(Device *d, const uint64_t *d_trace, size_t n, size_t w, size_t blow
 int lde_from_host(const uint64_t *trace, size_t n, size_t w, size_t blowup, const uint64_t *weights,
                   uint64_t *host_lde_out, uint8_t root_out[32], lvm_lde **handle_out) {
     if (!trace || !weights || !handle_out) return fail(LVM_ERR_ARG, "null argument");
+    TRY(check_lde_shape(n, w, blowup));
     Device *d = dev_or_fail();
     if (!d) return LVM_ERR_DEVICE;
-    if (!is_pow2(n) || n < 2) return fail(LVM_ERR_ARG, "trace rows must be a power of two >= 2");
     hipStream_t st = thread_stream(d);
     DevBuf dt, dw;
     TRY(dt.alloc(n * w * 8, st));
@@ -301,26 +308,20 @@ int lvm_coset_lde_ext3_row_major_with_merkle_tree_keep(const uint64_t *trace, si
 int lvm_coset_lde_dev(const uint64_t *dev_trace, size_t n, size_t m, size_t blowup, const uint64_t *dev_weights,
                       uint8_t root_out[32], lvm_lde **handle_out) {
     if (!dev_trace || !dev_weights || !handle_out) return fail(LVM_ERR_ARG, "null argument");
+    TRY(check_lde_shape(n, m, blowup));
     Device *d = dev_or_fail();
     if (!d) return LVM_ERR_DEVICE;
     return lde_core(d, dev_trace, n, m, blowup, dev_weights, handle_out, root_out);
 }
 
-int lvm_commit_rows_dev(const uint64_t *dev_rows, size_t N, size_t w, size_t trace_rows, uint8_t root_out[32],
-                        lvm_lde **handle_out) {
-    if (!dev_rows || !handle_out) return fail(LVM_ERR_ARG, "null argument");
-    if (!is_pow2(N) || N < 2 || log2u(N) > 32) return fail(LVM_ERR_ARG, "rows must be a power of two in [2, 2^32]");
-    if (!is_pow2(trace_rows) || trace_rows > N) return fail(LVM_ERR_ARG, "trace rows must be a power of two <= rows");
-    if (w == 0 || w > (1u << 30)) return fail(LVM_ERR_ARG, "bad row width");
-    Device *d = dev_or_fail();
-    if (!d) return LVM_ERR_DEVICE;
-    TRY(check_fault());
-    hipStream_t st = thread_stream(d);
+namespace {
+// Row-pair leaves and tree over N bit-reversed rows of w words that the new
+// handle takes over (rows is released into it on success).
+int commit_owned_rows(Device *d, hipStream_t st, DevBuf &rows, size_t N, size_t w, size_t trace_rows,
+                      uint8_t root_out[32], lvm_lde **handle_out) {
     const size_t L = N / 2;
-    DevBuf rows, nodes;
-    TRY(rows.alloc(N * w * 8, st));
+    DevBuf nodes;
     TRY(nodes.alloc((2 * L - 1) * 32, st));
-    LVM_HIP(hipMemcpyAsync(rows.p, dev_rows, N * w * 8, hipMemcpyDeviceToDevice, st));
     {
         ProfScope ps("keccak_leaves", st);
         keccak_row_pair_leaves(rows.as<uint64_t>(), w, 0, w, L, nodes.as<uint8_t>() + (L - 1) * 32, st);
@@ -349,6 +350,23 @@ int lvm_commit_rows_dev(const uint64_t *dev_rows, size_t N, size_t w, size_t tra
     if (root_out) std::memcpy(root_out, root, 32);
     *handle_out = h;
     return LVM_OK;
+}
+}  // namespace
+
+int lvm_commit_rows_dev(const uint64_t *dev_rows, size_t N, size_t w, size_t trace_rows, uint8_t root_out[32],
+                        lvm_lde **handle_out) {
+    if (!dev_rows || !handle_out) return fail(LVM_ERR_ARG, "null argument");
+    if (!is_pow2(N) || N < 2 || log2u(N) > 32) return fail(LVM_ERR_ARG, "rows must be a power of two in [2, 2^32]");
+    if (!is_pow2(trace_rows) || trace_rows > N) return fail(LVM_ERR_ARG, "trace rows must be a power of two <= rows");
+    if (w == 0 || w > (1u << 30)) return fail(LVM_ERR_ARG, "bad row width");
+    Device *d = dev_or_fail();
+    if (!d) return LVM_ERR_DEVICE;
+    TRY(check_fault());
+    hipStream_t st = thread_stream(d);
+    DevBuf rows;
+    TRY(rows.alloc(N * w * 8, st));
+    LVM_HIP(hipMemcpyAsync(rows.p, dev_rows, N * w * 8, hipMemcpyDeviceToDevice, st));
+    return commit_owned_rows(d, st, rows, N, w, trace_rows, root_out, handle_out);
 }
 
 int lvm_lde_release(lvm_lde *h) {
@@ -439,11 +457,13 @@ int lvm_coset_weights(size_t n, uint64_t offset, uint64_t *dev_out) {
     return LVM_OK;
 }
 
-int lvm_composition_commit_d2(const uint64_t *dev_h, size_t N, uint64_t coset_offset, uint8_t root_out[32],
-                              lvm_lde **parts_out) {
+int lvm_composition_commit_d2(const uint64_t *dev_h, size_t N, size_t trace_rows, uint64_t coset_offset,
+                              uint8_t root_out[32], lvm_lde **parts_out) {
     if (!dev_h || !parts_out) return fail(LVM_ERR_ARG, "null argument");
     if (!is_pow2(N) || N < 4) return fail(LVM_ERR_ARG, "LDE size must be a power of two >= 4");
     if (log2u(N) > 32) return fail(LVM_ERR_ARG, "LDE domain exceeds the 2^32 subgroup");
+    if (!is_pow2(trace_rows) || trace_rows < 2 || trace_rows > N / 2)
+        return fail(LVM_ERR_ARG, "trace rows must be a power of two in [2, N/2]");
     if (canon(coset_offset) == 0) return fail(LVM_ERR_ARG, "coset offset must be non-zero");
     Device *d = dev_or_fail();
     if (!d) return LVM_ERR_DEVICE;
@@ -461,7 +481,52 @@ int lvm_composition_commit_d2(const uint64_t *dev_h, size_t N, uint64_t coset_of
     // (crypto/stark/src/prover.rs:408-447 CompositionLdeTwiddles)
     launch_coset_weights(half, inv(coset_offset), inv((uint64_t)half), w.as<uint64_t>(), st);
     LVM_HIP(hipGetLastError());
-    return lde_core(d, halves.as<uint64_t>(), half, 6, 2, w.as<uint64_t>(), parts_out, root_out);
+    TRY(lde_core(d, halves.as<uint64_t>(), half, 6, 2, w.as<uint64_t>(), parts_out, root_out));
+    // the parts interpolate the trace degree bound: their out-of-domain values
+    // read the n trace-coset rows (crypto/stark/src/prover.rs:2012-2100)
+    (*parts_out)->n = trace_rows;
+    (*parts_out)->blowup = N / trace_rows;
+    return LVM_OK;
+}
+
+int lvm_composition_commit_parts(const uint64_t *dev_h, size_t N, size_t trace_rows, size_t num_parts,
+                                 uint64_t coset_offset, uint8_t root_out[32], lvm_lde **parts_out) {
+    if (!dev_h || !parts_out) return fail(LVM_ERR_ARG, "null argument");
+    if (!is_pow2(N) || N < 4 || log2u(N) > 32) return fail(LVM_ERR_ARG, "LDE size must be a power of two in [4, 2^32]");
+    if (!is_pow2(trace_rows) || trace_rows < 2 || trace_rows > N / 2)
+        return fail(LVM_ERR_ARG, "trace rows must be a power of two in [2, N/2]");
+    if (num_parts == 0 || num_parts > 8) return fail(LVM_ERR_ARG, "composition parts must be 1..8");
+    if (canon(coset_offset) == 0) return fail(LVM_ERR_ARG, "coset offset must be non-zero");
+    if (num_parts == 1) return lvm_commit_rows_dev(dev_h, N, 3, trace_rows, root_out, parts_out);
+    if (num_parts == 2) return lvm_composition_commit_d2(dev_h, N, trace_rows, coset_offset, root_out, parts_out);
+    Device *d = dev_or_fail();
+    if (!d) return LVM_ERR_DEVICE;
+    TRY(check_fault());
+    hipStream_t st = thread_stream(d);
+    const unsigned log_N = log2u(N);
+    const uint32_t P = (uint32_t)num_parts;
+    DevBuf nat, coef, wt, scat, lde;
+    TRY(nat.alloc(N * 3 * 8, st));
+    TRY(coef.alloc(N * 3 * 8, st));
+    TRY(wt.alloc(N * 8, st));
+    TRY(scat.alloc(N * 3 * P * 8, st));
+    TRY(lde.alloc(N * 3 * P * 8, st));
+    // offset^-i for the part shifts, and the per-row factor N^-1 offset^(t(1-P))
+    uint64_t gi[8];
+    const uint64_t ginv = inv(coset_offset);
+    gi[0] = 1;
+    for (uint32_t i = 1; i < P; i++) gi[i] = mul(gi[i - 1], ginv);
+    const uint64_t step = pow(ginv, P - 1);
+    {
+        ProfScope ps("composition_parts", st);
+        launch_gather_bitrev_rows(dev_h, nat.as<uint64_t>(), N, log_N, 3, st);
+        ntt_inverse_rows(nat.as<uint64_t>(), 3, coef.as<uint64_t>(), 3, 3, log_N, d->tb, st);
+        launch_coset_weights(N, step, inv((uint64_t)N), wt.as<uint64_t>(), st);
+        launch_parts_scatter(coef.as<uint64_t>(), log_N, P, wt.as<uint64_t>(), gi, scat.as<uint64_t>(), st);
+        ntt_forward_rows(scat.as<uint64_t>(), 3 * P, lde.as<uint64_t>(), 3 * P, 3 * P, log_N, d->tb, st);
+    }
+    LVM_HIP(hipGetLastError());
+    return commit_owned_rows(d, st, lde, N, 3 * P, trace_rows, root_out, parts_out);
 }
 
 int lvm_fill_splitmix_table(uint64_t *dev_out, size_t n, size_t m, uint64_t cfg, uint64_t table) {

@@ -419,6 +419,47 @@ void launch_d2_split(const uint64_t *hb, unsigned log_N, uint64_t coset_offset, 
                        inv_2g, tb.inv_lo, tb.inv_hi, halves);
 }
 
+// Composition parts for three or more parts (crypto/stark/src/prover.rs
+// :1858-1900: interpolate_offset_fft, break_in_parts, each part evaluated on
+// the LDE coset). coef holds the unnormalised inverse DFT of the N natural
+// composition values on offset*<w_N>, rows in bit-reversed order, so row
+// bitrev(k) = N * c_k * offset^k for the interpolant's coefficients c_k.
+// Part i has coefficients c_{i + P t}; its values on the coset are the DFT of
+// c_{i+Pt} * offset^t = coef[bitrev(i+Pt)] * (N^-1 offset^(t(1-P))) * offset^-i,
+// written here as natural row t, words [3i, 3i+3) of a 3P-word row (zero
+// where i + P t >= N). wt[t] = N^-1 * offset^(t(1-P)).
+struct PartShifts {
+    uint64_t v[8];  // offset^-i, i < P
+};
+__global__ __launch_bounds__(256) void parts_scatter_kernel(const uint64_t *__restrict__ coef, uint32_t log_N,
+                                                            uint32_t P, const uint64_t *__restrict__ wt,
+                                                            PartShifts gi, uint64_t *__restrict__ out) {
+    const uint64_t idx = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t N = 1ull << log_N;
+    if (idx >= N * P) return;
+    const uint64_t t = idx / P, i = idx - t * P;
+    const uint64_t k = i + (uint64_t)P * t;
+    uint64_t *o = out + 3 * idx;
+    if (k >= N) {
+        o[0] = o[1] = o[2] = 0;
+        return;
+    }
+    const fp3 c = ld3(coef + 3 * (uint64_t)bitrev((uint32_t)k, log_N));
+    const fp3 v = f3_mul_base(c, mul(wt[t], gi.v[i]));
+    o[0] = v.c0;
+    o[1] = v.c1;
+    o[2] = v.c2;
+}
+
+void launch_parts_scatter(const uint64_t *coef, unsigned log_N, uint32_t P, const uint64_t *wt,
+                          const uint64_t *offset_inv_pows, uint64_t *out, hipStream_t st) {
+    PartShifts gi{};
+    for (uint32_t i = 0; i < P && i < 8; i++) gi.v[i] = offset_inv_pows[i];
+    const uint64_t total = (1ull << log_N) * P;
+    hipLaunchKernelGGL(parts_scatter_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, coef, log_N,
+                       P, wt, gi, out);
+}
+
 namespace {
 
 template <class T>

@@ -271,28 +271,32 @@ int lvm_fri_query(const lvm_fri *f, const uint64_t *iotas, size_t count, uint64_
     // per layer: count sym evals (3 u64) and count paths (depth_l x 32 bytes)
     size_t path_words = 0;
     for (auto &l : f->layers) path_words += count * log2u(l.L) * 4;
-    uint64_t *dio = nullptr, *dsym = nullptr, *dpaths = nullptr;
-    LVM_HIP(lvm_malloc(&dio, count * 8));
-    LVM_HIP(lvm_malloc(&dsym, f->layers.size() * count * 24));
-    LVM_HIP(lvm_malloc(&dpaths, (path_words ? path_words : 1) * 8));
+    // released on every exit (stream-ordered: a block is reused on this stream
+    // only after the work queued before its release)
+    struct Owned {
+        uint64_t *p = nullptr;
+        ~Owned() {
+            if (p) lvm_free(p);
+        }
+    } dio, dsym, dpaths;
+    LVM_HIP(lvm_malloc(&dio.p, count * 8));
+    LVM_HIP(lvm_malloc(&dsym.p, f->layers.size() * count * 24));
+    LVM_HIP(lvm_malloc(&dpaths.p, (path_words ? path_words : 1) * 8));
     StageScope ss(st);
-    LVM_HIP(stage_upload(dio, iotas, count * 8, st));
+    LVM_HIP(stage_upload(dio.p, iotas, count * 8, st));
     size_t poff = 0;
     for (size_t li = 0; li < f->layers.size(); li++) {
         const FriLayer &l = f->layers[li];
         uint32_t depth = log2u(l.L);
         hipLaunchKernelGGL(fri_query_layer_kernel, dim3((unsigned)((count + 63) / 64)), dim3(64), 0, st, l.evals,
-                           (const uint64_t *)l.nodes, l.L, depth, dio, count, (uint32_t)li, dsym + li * count * 3,
-                           dpaths + poff);
+                           (const uint64_t *)l.nodes, l.L, depth, dio.p, count, (uint32_t)li,
+                           dsym.p + li * count * 3, dpaths.p + poff);
         poff += count * depth * 4;
     }
     LVM_HIP(hipGetLastError());
-    LVM_HIP(stage_download(evals_sym_out, dsym, f->layers.size() * count * 24, st));
-    if (path_words) LVM_HIP(stage_download(paths_out, dpaths, path_words * 8, st));
+    LVM_HIP(stage_download(evals_sym_out, dsym.p, f->layers.size() * count * 24, st));
+    if (path_words) LVM_HIP(stage_download(paths_out, dpaths.p, path_words * 8, st));
     LVM_HIP(stage_sync(st));
-    lvm_free(dio);
-    lvm_free(dsym);
-    lvm_free(dpaths);
     return LVM_OK;
 }
 

@@ -143,6 +143,11 @@ size_t padded_leaves(size_t n);
 // natural-order rows [H0 | H1]
 void launch_d2_split(const uint64_t *hb, unsigned log_N, uint64_t coset_offset, const Tables &tb, uint64_t *halves,
                      hipStream_t st);
+// three or more composition parts: natural rows t of the P part coefficient
+// columns (Fp3, scaled for evaluation on the coset) from the bit-reversed
+// unnormalised inverse DFT `coef` of the N composition values; P <= 8
+void launch_parts_scatter(const uint64_t *coef, unsigned log_N, uint32_t P, const uint64_t *wt,
+                          const uint64_t *offset_inv_pows, uint64_t *out, hipStream_t st);
 
 // ---- work lists over several tables (batched stages) ----
 // Work-list slot of block b: the t with prefix[t] <= b < prefix[t+1]

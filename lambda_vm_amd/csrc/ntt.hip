@@ -522,11 +522,13 @@ void ntt_forward_rows(const uint64_t *in, uint32_t in_pitch, uint64_t *out, uint
 // Coset LDE expansion: coefficient table `coef` (n rows, bit-reversed order,
 // unnormalised iDFT of the trace) -> `lde` (n*B rows, bit-reversed order over
 // log2(n*B) bits) of the evaluations on offset*<w_N> (offset folded into weights).
+// log_n >= 1 (every entry point rejects tables of fewer than two rows).
 void ntt_coset_expand(const uint64_t *coef, uint32_t coef_pitch, uint64_t *lde, uint32_t lde_pitch, uint32_t m,
                       unsigned log_n, unsigned log_blowup, const uint64_t *weights, const Tables &tb,
                       hipStream_t st) {
     int r[8];
     int np = plan_passes(log_n, r);
+    if (np == 0) return;  // unreachable: n >= 2
     const uint32_t B = 1u << log_blowup;
     PassArgs a{};
     a.in = coef;
@@ -535,9 +537,9 @@ void ntt_coset_expand(const uint64_t *coef, uint32_t coef_pitch, uint64_t *lde, 
     a.out_pitch = lde_pitch;
     a.m = m;
     a.log_msub = log_n;
-    a.log_s = np ? log_n - r[0] : 0;
+    a.log_s = log_n - r[0];
     a.in_bitrev_bits = log_n;
-    a.total_groups = np ? (1ull << log_n) >> r[0] : 1;
+    a.total_groups = (1ull << log_n) >> r[0];
     a.out_coset_rows = 1ull << log_n;
     a.coset_bits = log_blowup;
     a.shift_log = log_n + log_blowup;
@@ -547,11 +549,6 @@ void ntt_coset_expand(const uint64_t *coef, uint32_t coef_pitch, uint64_t *lde, 
     a.shift_lo = tb.fwd_lo;
     a.shift_hi = tb.fwd_hi;
     a.tw_r = tb.fwd_r;
-    if (np == 0) {
-        // n = 1: every evaluation equals the (weighted, shifted) constant
-        a.log_msub = 0;
-        return;
-    }
     launch_pass(r[0], a, B, st, "ntt_coset_pass0");
     run_passes(lde, lde_pitch, m, (size_t)B << log_n, log_n, 1, r, np, tb, false, st);
 }

@@ -17,7 +17,8 @@ Field order of one proof:
   trace_length; lde_trace_main_merkle_root; lde_trace_aux_merkle_root
   (Option); lde_trace_precomputed_merkle_root (Option, always none here);
   trace_ood_evaluations (Table: 1 row x W); trace_ood_next_evaluations
-  (Table: K-1 rows x W, every column opened at the next rows);
+  (Table: K-1 rows x C, the C columns the AIR opens at the next rows,
+  crypto/stark/src/ood.rs:94-126);
   composition_poly_root; composition_poly_parts_ood_evaluation (Vec<E>);
   fri_layers_merkle_roots (Vec); fri_final_poly_coeffs (Vec<E>); query_list
   (Vec of FriDecommitment: layers_auth_paths (Vec of Vec<Commitment>),
@@ -81,19 +82,39 @@ class _W:
             self.raw(n.tobytes())
 
 
+# Frame offsets are 8-bit in the constraint program encoding, so a proof has
+# fewer than 256 next-row rows.
+MAX_FRAME_ROWS = 255
+
+
 class _R:
     def __init__(self, data: bytes):
         self.d, self.i = memoryview(data), 0
 
     def u64(self):
+        if self.i + 8 > len(self.d):
+            raise ValueError("truncated proof")
         v = struct.unpack_from("<Q", self.d, self.i)[0]
         self.i += 8
         return v
 
     def u8(self):
+        if self.i >= len(self.d):
+            raise ValueError("truncated proof")
         v = self.d[self.i]
         self.i += 1
         return v
+
+    def remaining(self) -> int:
+        return len(self.d) - self.i
+
+    def count(self, item_bytes: int, what: str) -> int:
+        """A length prefix, refused when its items (at least item_bytes each)
+        cannot fit in the bytes left."""
+        k = self.u64()
+        if k * item_bytes > self.remaining():
+            raise ValueError(f"{what}: count {k} exceeds the proof's remaining bytes")
+        return k
 
     def raw(self, n):
         v = bytes(self.d[self.i:self.i + n])
@@ -112,13 +133,13 @@ class _R:
         return [self.fe(), self.fe(), self.fe()]
 
     def vec_fe3(self):
-        return np.array([self.fe3() for _ in range(self.u64())], dtype=np.uint64).reshape(-1, 3)
+        return np.array([self.fe3() for _ in range(self.count(24, "Fp3 vector"))], dtype=np.uint64).reshape(-1, 3)
 
     def vec_fe(self):
-        return np.array([self.fe() for _ in range(self.u64())], dtype=np.uint64)
+        return np.array([self.fe() for _ in range(self.count(8, "field vector"))], dtype=np.uint64)
 
     def path(self):
-        k = self.u64()
+        k = self.count(32, "authentication path")
         return np.frombuffer(self.raw(32 * k), dtype=np.uint8).reshape(k, 32).copy()
 
 
@@ -153,11 +174,13 @@ def serialize_table_proof(p: AirTableProof) -> bytes:
     w.u64(1)
     for v in b0:
         w.fe3(v)
-    b1 = np.asarray(p.ood_block1, dtype=np.uint64).reshape(W, -1, 3)  # (W, K-1)
-    w.u64(W)
-    w.u64(b1.shape[1])
-    for r in range(b1.shape[1]):
-        for j in range(W):
+    b1 = np.asarray(p.ood_block1, dtype=np.uint64)
+    C1 = b1.shape[0] if b1.ndim == 3 else 0  # (C, K-1, 3): next-row columns
+    h1 = b1.shape[1] if C1 else 0  # a width-0 Table has height 0 (crypto/stark/src/table.rs:291-301)
+    w.u64(C1)
+    w.u64(h1)
+    for r in range(h1):
+        for j in range(C1):
             w.fe3(b1[j, r])
     w.raw(p.composition_root)
     w.vec_fe3(p.parts_ood)
@@ -200,32 +223,36 @@ def deserialize_table_proof(data: bytes, reader: _R | None = None) -> AirTablePr
     aux_root = r.raw(32) if r.u8() else None
     if r.u8():
         raise ValueError("precomputed commitments are not produced by this prover")
-    W, h0 = r.u64(), r.u64()
-    if h0 != 1:
+    W = r.count(24, "current-row block width")
+    if r.u64() != 1:
         raise ValueError("current-row block must have one row")
-    b0 = np.array([r.fe3() for _ in range(W)], dtype=np.uint64)
-    W1, h1 = r.u64(), r.u64()
-    if W1 != W:
-        raise ValueError("next-row block width")
-    rows = [[r.fe3() for _ in range(W)] for _ in range(h1)]
-    b1 = np.array([[rows[k][j] for k in range(h1)] for j in range(W)], dtype=np.uint64).reshape(W, h1, 3)
+    b0 = np.array([r.fe3() for _ in range(W)], dtype=np.uint64).reshape(W, 3)
+    # next-row block: only the columns the AIR opens at the next rows (the
+    # verifier checks the width against the AIR)
+    W1 = r.count(24, "next-row block width")
+    h1 = r.u64()
+    if h1 > MAX_FRAME_ROWS or W1 * h1 * 24 > r.remaining():
+        raise ValueError("next-row block height")
+    rows = [[r.fe3() for _ in range(W1)] for _ in range(h1)]
+    b1 = np.array([[rows[k][j] for k in range(h1)] for j in range(W1)], dtype=np.uint64).reshape(W1, h1, 3)
     comp_root = r.raw(32)
     parts_ood = r.vec_fe3()
-    nroots = r.u64()
+    nroots = r.count(32, "FRI layer roots")
     fri_roots = np.frombuffer(r.raw(32 * nroots), dtype=np.uint8).reshape(nroots, 32).copy()
     fri_coeffs = r.vec_fe3()
-    Q = r.u64()
-    fri_paths = [[None] * Q for _ in range(nroots)]
-    fri_sym = np.zeros((nroots, Q, 3), dtype=np.uint64)
+    Q = r.count(16, "FRI decommitments")  # each carries at least its layer count and a vector length
+    fri_paths = [[] for _ in range(nroots)]
+    sym_q = []
     for q in range(Q):
         if r.u64() != nroots:
             raise ValueError("FRI decommitment layer count")
         for li in range(nroots):
-            fri_paths[li][q] = r.path()
+            fri_paths[li].append(r.path())
         s = r.vec_fe3()
         if s.shape[0] != nroots:
             raise ValueError("FRI symmetric values")
-        fri_sym[:, q] = s
+        sym_q.append(s)
+    fri_sym = np.stack(sym_q, axis=1) if Q else np.zeros((nroots, 0, 3), dtype=np.uint64)
     if r.u64() != Q:
         raise ValueError("opening count")
     parts_paths, parts_rows, main_paths, main_rows, aux_paths, aux_rows = [], [], [], [], [], []

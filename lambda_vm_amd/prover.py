@@ -11,21 +11,27 @@ Order (crypto/stark/src/prover.rs):
        (:3867, :3928-3931).
   R2   beta; transition coefficients beta^0..beta^(T-1), then the boundary
        coefficients (:4453-4474); the composition polynomial on the LDE, split
-       into parts (one part when the degree bound is the trace length, the
-       degree-2 split otherwise, :1683-1760) and committed (:4482).
+       into P = degree bound / n parts (:1705; one part: H itself, two: the
+       degree-2 split, three or more: interpolation, break_in_parts and each
+       part evaluated on the LDE coset, :1683-1900) and committed (:4482).
   R3   z out of the trace domain and the LDE coset (is_transcript.rs:44-75);
        the trace at z * g^k for every frame offset k; the current-row block
-       (all columns) then the next-row block column by column (ood.rs:94-126,
-       prover.rs:4536-4549); the parts at z^P (:4551-4554).
-  R4   gamma; the first W*K powers weight the trace terms in the order of
-       ood.rs:57-92 (current row: column j gets gamma^j; next rows: column j,
-       offset k >= 1 gets gamma^(W + j*(K-1) + k-1)), the next P powers the
-       parts (prover.rs:2156-2178); the DEEP codeword, FRI commit phase,
-       grinding nonce (BE bytes, :2277-2286), queries sample_u64(N/2)
-       (:2324-2333), FRI decommitments and the row-pair openings of the main,
-       aux and parts trees (:2888+).
-Every transition AIR here opens all columns at the next rows (the default
-AIR::trace_ood_next_row_columns, traits.rs:217-220).
+       (all columns) then the pruned next-row block, only the columns the
+       AIR reads at the next row, column by column (ood.rs:94-126,
+       prover.rs:4543-4553); the parts at z^P, each by the barycentric formula
+       over the n trace-coset points (:2012-2100).
+  R4   gamma; the first S powers (S = surviving openings: W current-row
+       values plus (K-1) per next-row column) weight the trace terms in the
+       order of ood.rs:57-92 (current row: column j gets gamma^j; then each
+       next-row column in ascending order, offsets 1..K-1), every pruned
+       (column, offset) gets weight zero; the next P powers the parts
+       (prover.rs:2155-2182); the DEEP codeword, FRI commit phase, grinding
+       nonce (BE bytes, :2277-2286), queries sample_u64(N/2) (:2324-2333), FRI
+       decommitments and the row-pair openings of the main, aux and parts
+       trees (:2888+).
+Bus AIRs (every VM table, the LogUp table) open only the accumulated column
+at the next row (lookup.rs:1042-1053); the example AIRs open every column
+(traits.rs:217-220).
 """
 from __future__ import annotations
 
@@ -60,7 +66,7 @@ class AirTableProof:
     table_contribution: tuple | None  # L (Fp3)
     composition_root: bytes
     ood_block0: np.ndarray   # (W, 3): every column at z
-    ood_block1: np.ndarray   # (W, K-1, 3): every column at z*g^k, k >= 1
+    ood_block1: np.ndarray   # (C, K-1, 3): the C next-row columns at z*g^k, k >= 1
     parts_ood: np.ndarray    # (P, 3): H_i(z^P)
     fri_roots: np.ndarray
     fri_coeffs: np.ndarray
@@ -79,13 +85,13 @@ class AirTableProof:
 
 
 def frame_offsets(air: AirProgram) -> int:
-    """K: rows of the OOD frame (transition offsets 0..max read offset)."""
-    return int(air.device.max_offset) + 1
+    """K: rows of the OOD frame (the AIR's transition offsets)."""
+    return air.frame_rows
 
 
 def num_parts(air: AirProgram) -> int:
     """Composition parts: degree bound / trace length (prover.rs:1705)."""
-    return 1 if air.max_degree <= 1 else 2
+    return air.num_parts
 
 
 def fp3_columns(vals: np.ndarray) -> np.ndarray:
@@ -101,10 +107,18 @@ def fp3_columns(vals: np.ndarray) -> np.ndarray:
     return out
 
 
-def trace_term_gamma_index(W: int, K: int, j: int, k: int) -> int:
-    """Exponent of gamma weighting column j at frame row k (ood.rs:57-92 with
-    every column opened at the next rows)."""
-    return j if k == 0 else W + j * (K - 1) + (k - 1)
+def trace_term_gamma_exponents(W: int, K: int, next_cols) -> dict:
+    """{(column j, frame row k): exponent of gamma} for every surviving OOD
+    opening (crypto/stark/src/ood.rs:57-92 build_pruned_trace_term_coeffs with
+    step size 1): the current row of every column first, then each next-row
+    column (ascending) at rows 1..K-1. Pruned openings are absent (weight 0)."""
+    e = {(j, 0): j for j in range(W)}
+    p = W
+    for j in sorted(next_cols):
+        for k in range(1, K):
+            e[(j, k)] = p
+            p += 1
+    return e
 
 
 def prove_air_tables(tables, opts: ProofOptions = ProofOptions(), label: bytes = b"lambda-vm-air-tables",
@@ -209,9 +223,9 @@ def _prove_table(air: AirProgram, tr: B.DeviceBuffer, n: int, main: B.GpuLde, we
     coeffs = B.fp3_powers(beta, T + nb)
     groups, ex = air.groups()
     bnd = [(b.col, b.is_aux, b.step, b.value) for b in air.boundary]
-    H = B.eval_composition(air.device, main, aux, Bf, n, g, coeffs[:T], groups, ex, bnd, coeffs[T:], rap, ap, toff)
     Pn = num_parts(air)
-    parts = B.composition_commit_d2(H, N, g) if Pn == 2 else B.commit_rows_dev(H, N, 3, n)
+    H = B.eval_composition(air.device, main, aux, Bf, n, g, coeffs[:T], groups, ex, bnd, coeffs[T:], rap, ap, toff)
+    parts = B.composition_commit_parts(H, N, n, Pn, g)
     H.free()
     t.append_bytes(parts.root)
     # ---- R3: out-of-domain evaluations ----
@@ -227,23 +241,29 @@ def _prove_table(air: AirProgram, tr: B.DeviceBuffer, n: int, main: B.GpuLde, we
     if aux is not None:
         ood = np.concatenate([ood, fp3_columns(B.barycentric_base_multi(aux, g, pts))], axis=1)
     W = ood.shape[1]
+    next_cols = sorted(air.next_row_cols())
     t.append_fp3_many(ood[0])                                  # current-row block
-    block1 = np.ascontiguousarray(ood[1:].transpose(1, 0, 2))  # (W, K-1, 3): column by column
+    # pruned next-row block, column by column: (C, K-1, 3)
+    block1 = np.ascontiguousarray(ood[1:, next_cols].transpose(1, 0, 2)).reshape(len(next_cols), K - 1, 3)
     t.append_fp3_many(block1.reshape(-1, 3))
     zP = F.f3_pow(z, Pn)
     parts_ood = fp3_columns(B.barycentric_base_multi(parts, g, np.array([zP], dtype=np.uint64)))[0]  # (P, 3)
     t.append_fp3_many(parts_ood)
     # ---- R4: DEEP, FRI, grinding, queries ----
     gamma = [int(v) for v in t.sample_fp3()]
-    gp = B.fp3_powers(gamma, W * K + Pn)
+    expo = trace_term_gamma_exponents(W, K, next_cols)
+    S = len(expo)
+    gp = B.fp3_powers(gamma, S + Pn)
+    # the rectangular (column, frame row) grid; pruned openings weigh zero
     gammas_tr = np.zeros((W * K, 3), dtype=np.uint64)
     trace_ood = np.zeros((W * K, 3), dtype=np.uint64)
     for j in range(W):
         for k in range(K):
-            gammas_tr[j * K + k] = gp[trace_term_gamma_index(W, K, j, k)]
+            if (j, k) in expo:
+                gammas_tr[j * K + k] = gp[expo[(j, k)]]
             trace_ood[j * K + k] = ood[k, j]
     deep = B.deep_composition(main, g, pts, trace_ood, gammas_tr, aux=aux, parts=parts, z_parts=zP,
-                              parts_ood=parts_ood, gammas_h=gp[W * K:])
+                              parts_ood=parts_ood, gammas_h=gp[S:])
     fri = B.GpuFri(None, g, on_device=deep, length=N, adopt=True)
     fri_roots, fri_coeffs = fri.commit_phase(t, Bf.bit_length() - 1, opts.fri_final_log_degree)
     nonce = None

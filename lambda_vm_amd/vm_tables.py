@@ -626,13 +626,24 @@ def generate_memw_register_trace(accesses) -> np.ndarray:
     c = MemwRegisterCols
     n = max(4, 1 << max(len(accesses) - 1, 0).bit_length())
     t = np.zeros((n, c.NUM_COLUMNS), dtype=np.uint64)
+    M64 = (1 << 64) - 1
     for i, a in enumerate(accesses):
         if a.base_address % 2:
             raise ValueError("register base address must be even")
+        if not 0 <= a.base_address // 2 < 1 << 16:
+            raise ValueError("register index must fit in 16 bits")
+        # RegRow::from_memw (prover/src/tables/memw_register.rs:130-195): the old
+        # access shares the timestamp's high word and lies 1..2^16 below it in
+        # the low word (the IS_HALFWORD check of ts_lo - old_ts_lo - 1)
+        if (a.old_timestamp >> 32) != (a.timestamp >> 32):
+            raise ValueError("old timestamp must share the timestamp's high word")
+        if not 1 <= (a.timestamp & 0xFFFFFFFF) - (a.old_timestamp & 0xFFFFFFFF) <= 1 << 16:
+            raise ValueError("timestamp low-word delta must be in [1, 2^16]")
+        value, old = a.value & M64, a.old & M64
         t[i, c.ADDRESS] = a.base_address // 2
-        t[i, c.TIMESTAMP_0], t[i, c.TIMESTAMP_1] = a.timestamp & 0xFFFFFFFF, a.timestamp >> 32
-        t[i, c.VAL_0], t[i, c.VAL_1] = a.value & 0xFFFFFFFF, a.value >> 32
-        t[i, c.OLD_0], t[i, c.OLD_1] = a.old & 0xFFFFFFFF, a.old >> 32
+        t[i, c.TIMESTAMP_0], t[i, c.TIMESTAMP_1] = a.timestamp & 0xFFFFFFFF, (a.timestamp >> 32) & 0xFFFFFFFF
+        t[i, c.VAL_0], t[i, c.VAL_1] = value & 0xFFFFFFFF, value >> 32
+        t[i, c.OLD_0], t[i, c.OLD_1] = old & 0xFFFFFFFF, old >> 32
         t[i, c.OLD_TIMESTAMP_LO] = a.old_timestamp & 0xFFFFFFFF
         t[i, c.MU_READ], t[i, c.MU_WRITE] = int(a.is_read), int(not a.is_read)
     return t

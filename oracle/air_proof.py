@@ -18,21 +18,36 @@ Round order and formulas follow crypto/stark/src/prover.rs (R1 :3426-3538,
     (fri/terminal.rs:113-155);
   - step 4 (:597-704): the main, aux and composition row pairs open against
     their roots.
-The OOD grid opens every column at the next rows (traits.rs:217-220), so the
-trace-term coefficients follow ood.rs:57-92 with all columns masked.
+Shape metadata, restated here from the reference rules (the AIR description
+only says which kind of AIR it is):
+  - composition parts = composition_poly_degree_bound(n) / n: AirWithBuses
+    (lookup.rs:1063-1078) (max_degree - 1).max(1); an example AIR states its
+    own bound (`example_parts`: fibonacci_multi_column.rs:93 -> 1,
+    quadratic_air.rs:149 -> 2). One part is H itself, two the degree-2 split
+    (prover.rs:1605-1680), three or more H's interpolant broken by stride P
+    and each part evaluated on the LDE coset (prover.rs:1858-1900,
+    polynomial.rs:137-150 break_in_parts);
+  - frame rows: AirWithBuses [0, 1] (lookup.rs:950), example AIRs theirs;
+  - next-row openings: AirWithBuses only the accumulated LogUp column
+    (lookup.rs:1042-1053), none without interactions; example AIRs every
+    column (traits.rs:217-220). The trace-term coefficients follow
+    ood.rs:57-92 build_pruned_trace_term_coeffs (pruned openings weigh zero),
+    the proof's next-row block holds the opened columns only (ood.rs:94-126).
 
 Used only by tests. AIR descriptions come in as data: an object with the
 lowered constraint program (`device`), `num_main`, `num_aux`, `max_degree`,
-`num_constraints`, `groups()`, `boundary`, `interactions` and, for tables with
-bus interactions, the flat bus layout `bus_descriptor`.
+`bus` (an AirWithBuses table), `example_parts` (an example AIR's degree
+bound / n), `num_constraints`, `groups()`, `boundary`, `interactions` and, for
+tables with bus interactions, the flat bus layout `bus_descriptor`.
 """
 from __future__ import annotations
 
 import numpy as np
 
-from . import (P, Transcript, barycentric_base, decompose_d2, deep_composition, eval_composition, fri_commit,
-               fri_layout, generate_nonce, is_valid_nonce, keccak256, lde_commit, logup_aux, merkle_build,
-               merkle_path, merkle_verify, root_of_unity)
+from . import (P, Transcript, barycentric_base, commit_rows_bit_reversed, decompose_d2, deep_composition,
+               eval_composition, fft_two_half, fri_commit, fri_layout, generate_nonce, gl_op, is_valid_nonce,
+               keccak256, lde_commit, lde_weights, logup_aux, merkle_build, merkle_path, merkle_verify,
+               root_of_unity)
 
 # ---------------- Fp3 = Fp[w]/(w^3 - 2) on Python ints ----------------
 
@@ -131,15 +146,61 @@ def sample_z_ood(t: Transcript, n: int, N: int, offset: int):
 
 
 def frame_rows(air) -> int:
-    return int(air.device.max_offset) + 1
+    """Transition offsets: [0, 1] for AirWithBuses (lookup.rs:950)."""
+    return 2 if air.bus else int(air.device.max_offset) + 1
 
 
 def num_parts(air) -> int:
-    return 1 if air.max_degree <= 1 else 2
+    """composition_poly_degree_bound(n) / n (prover.rs:1705)."""
+    if air.bus:
+        return max(air.max_degree - 1, 1)  # lookup.rs:1063-1078
+    return int(air.example_parts)
 
 
-def gamma_index(W, K, j, k):
-    return j if k == 0 else W + j * (K - 1) + (k - 1)
+def next_row_cols(air) -> list:
+    """Columns opened at z*g ([main | aux] indices): lookup.rs:1042-1053 for
+    AirWithBuses (the accumulated column sits after the committed term
+    columns, lookup.rs:1946), every column for the example AIRs."""
+    W = air.num_main + air.num_aux
+    if air.bus:
+        return [air.num_main + air.num_aux - 1] if air.interactions else []
+    return list(range(W))
+
+
+def gamma_exponents(W, K, next_cols):
+    """ood.rs:57-92: exponents of gamma of the surviving trace openings
+    {(column, frame row): e}: every column's current row, then the opened
+    columns (ascending) at rows 1..K-1."""
+    e = {(j, 0): j for j in range(W)}
+    p = W
+    for j in sorted(next_cols):
+        for k in range(1, K):
+            e[(j, k)] = p
+            p += 1
+    return e
+
+
+def break_in_parts_lde(H: np.ndarray, num_parts_: int, g: int) -> np.ndarray:
+    """prover.rs:1858-1900 for three or more parts: H (N, 3) natural values on
+    g*<w_N>; interpolate_offset_fft (polynomial.rs:359-372: inverse FFT, 1/N,
+    scale by g^-k), break_in_parts (polynomial.rs:137-150: part i takes
+    coefficients i, i+P, i+2P, ...), then each part's evaluate_offset_fft at
+    the N coset points (polynomial.rs:316-327). Returns (N, 3P) natural rows."""
+    N = H.shape[0]
+    a = fft_two_half(np.ascontiguousarray(H, dtype=np.uint64), inverse=True)  # N * c_k * g^k
+    ginv = pow(g, P - 2, P)
+    sc = lde_weights(N, ginv)  # N^-1 g^-k
+    coef = np.stack([gl_op("mul", a[:, c].copy(), sc) for c in range(3)], axis=1)
+    gpow = gl_op("mul", lde_weights(N, g), np.full(N, N, dtype=np.uint64))  # g^t
+    out = np.zeros((N, 3 * num_parts_), dtype=np.uint64)
+    for i in range(num_parts_):
+        part = coef[i::num_parts_]
+        L = part.shape[0]
+        d = np.zeros((N, 3), dtype=np.uint64)
+        for c in range(3):
+            d[:L, c] = gl_op("mul", part[:, c].copy(), gpow[:L].copy())
+        out[:, 3 * i:3 * i + 3] = fft_two_half(d)
+    return out
 
 
 # ---------------- constraint program at the OOD frame ----------------
@@ -251,10 +312,11 @@ def _replay_table(air, tr, commit, t, Bf, g, fri_k, num_queries, grinding, z_log
     if Pn == 2:
         halves = decompose_d2(H, g)
         parts_lde, parts_nodes = lde_commit(halves, 2, pow(g, P - 2, P))
-        parts_n, parts_b = N // 2, 2
-    else:
+    elif Pn == 1:
         parts_lde, parts_nodes = H, commit_rows(H)
-        parts_n, parts_b = n, Bf
+    else:
+        parts_lde = break_in_parts_lde(H, Pn, g)
+        parts_nodes = commit_rows_bit_reversed(parts_lde)
     t.append_bytes(bytes(parts_nodes[0]))
     z = sample_z_ood(t, n, N, g)
     K = frame_rows(air)
@@ -268,23 +330,29 @@ def _replay_table(air, tr, commit, t, Bf, g, fri_k, num_queries, grinding, z_log
             row += [fp3_of_components(*comp[3 * c:3 * c + 3]) for c in range(len(comp) // 3)]
         ood.append(row)
     W = len(ood[0])
+    ncols = sorted(next_row_cols(air))
     for j in range(W):
         t.append_fp3(np.array(ood[0][j], dtype=np.uint64))
-    for j in range(W):
+    for j in ncols:
         for k in range(1, K):
             t.append_fp3(np.array(ood[k][j], dtype=np.uint64))
     zP = f3pow(z, Pn)
-    comp = [f3(v) for v in barycentric_base(parts_lde, parts_n, parts_b, g, np.array(zP, dtype=np.uint64))]
+    # each part's value at z^P by the barycentric formula over the n
+    # trace-coset points (prover.rs:2012-2100: domain_size n, stride blowup)
+    comp = [f3(v) for v in barycentric_base(parts_lde, n, Bf, g, np.array(zP, dtype=np.uint64))]
     parts_ood = [fp3_of_components(*comp[3 * i:3 * i + 3]) for i in range(Pn)]
     for v in parts_ood:
         t.append_fp3(np.array(v, dtype=np.uint64))
     gamma = f3(t.sample_fp3())
-    gp = f3powers(gamma, W * K + Pn)
+    expo = gamma_exponents(W, K, ncols)
+    S = len(expo)
+    gp = f3powers(gamma, S + Pn)
     trace_ood = np.array([ood[k][j] for j in range(W) for k in range(K)], dtype=np.uint64)
-    gammas_tr = np.array([gp[gamma_index(W, K, j, k)] for j in range(W) for k in range(K)], dtype=np.uint64)
+    gammas_tr = np.array([gp[expo[(j, k)]] if (j, k) in expo else ZERO for j in range(W) for k in range(K)],
+                         dtype=np.uint64)
     deep = deep_composition(main_lde, g, np.array(points, dtype=np.uint64), trace_ood, gammas_tr, aux=aux_lde,
                             parts=parts_lde, z_parts=np.array(zP, dtype=np.uint64),
-                            parts_ood=np.array(parts_ood, dtype=np.uint64), gammas_h=np.array(gp[W * K:],
+                            parts_ood=np.array(parts_ood, dtype=np.uint64), gammas_h=np.array(gp[S:],
                                                                                                dtype=np.uint64))
     perm = np.array([bitrev(i, bits) for i in range(N)], dtype=np.int64)
     roots, fcoeffs, layers = fri_commit(deep[perm], t, g, Bf.bit_length() - 1, fri_k, keep_layers=True)
@@ -314,7 +382,8 @@ def _replay_table(air, tr, commit, t, Bf, g, fri_k, num_queries, grinding, z_log
         fri_paths.append([merkle_path(lnodes, (i >> li) >> 1) for i in iotas])
     return dict(trace_rows=n, main_root=bytes(main_nodes[0]), aux_root=None if aux_nodes is None else bytes(aux_nodes[0]),
                 table_contribution=L, composition_root=bytes(parts_nodes[0]), ood_block0=[ood[0][j] for j in range(W)],
-                ood_block1=[[ood[k][j] for k in range(1, K)] for j in range(W)], parts_ood=parts_ood,
+                ood_block1=np.array([[ood[k][j] for k in range(1, K)] for j in ncols],
+                                    dtype=np.uint64).reshape(len(ncols), K - 1, 3), parts_ood=parts_ood,
                 fri_roots=[bytes(r) for r in roots], fri_coeffs=[f3(c) for c in fcoeffs], nonce=nonce, iotas=iotas,
                 fri_sym=fri_sym, fri_paths=fri_paths, main_rows=main_rows, main_paths=main_paths, aux_rows=aux_rows,
                 aux_paths=aux_paths, parts_rows=parts_rows, parts_paths=parts_paths, transcript_state=t.state(),
@@ -389,22 +458,30 @@ def _verify_table(air, pr, t, Bf, g, fri_k, num_queries, grinding, z_logup, alph
     Pn = num_parts(air)
     ma = air.num_aux
     W = m + ma
+    ncols = sorted(next_row_cols(air))
     b0 = _f3_list(G("ood_block0"))
     b1 = _f3_list(G("ood_block1"))
     pood = _f3_list(G("parts_ood"))
-    if len(b0) != W or len(b1) != W * (K - 1) or len(pood) != Pn:
+    if len(b0) != W or len(b1) != len(ncols) * (K - 1) or len(pood) != Pn:
         return False, "OOD shape"
-    b1 = [[f3(b1[j * (K - 1) + k]) for k in range(K - 1)] for j in range(W)]
+    b1 = [[f3(b1[i * (K - 1) + k]) for k in range(K - 1)] for i in range(len(ncols))]
     for v in b0:
         t.append_fp3(np.array(v, dtype=np.uint64))
-    for j in range(W):
+    for i in range(len(ncols)):
         for k in range(K - 1):
-            t.append_fp3(np.array(b1[j][k], dtype=np.uint64))
+            t.append_fp3(np.array(b1[i][k], dtype=np.uint64))
     for v in pood:
         t.append_fp3(np.array(v, dtype=np.uint64))
-    ood = [b0] + [[b1[j][k] for j in range(W)] for k in range(K - 1)]  # ood[k][j]
+    # the full frame with pruned next-row entries zero (ood.rs reconstruct_ood_full)
+    ood = [b0] + [[ZERO] * W for _ in range(K - 1)]  # ood[k][j]
+    for i, j in enumerate(ncols):
+        for k in range(K - 1):
+            ood[k + 1][j] = b1[i][k]
     gamma = f3(t.sample_fp3())
-    gp = f3powers(gamma, W * K + Pn)
+    expo = gamma_exponents(W, K, ncols)
+    S = len(expo)
+    gp = f3powers(gamma, S + Pn)
+    gw = lambda j, k: gp[expo[(j, k)]] if (j, k) in expo else ZERO  # noqa: E731
     # FRI transcript replay
     tf, nc, tl, ek = fri_layout(bits, Bf.bit_length() - 1, fri_k)
     roots = [bytes(r) for r in G("fri_roots")]
@@ -469,12 +546,12 @@ def _verify_table(air, pr, t, Bf, g, fri_k, num_queries, grinding, z_logup, alph
     for k in range(K):
         s = ZERO
         for j in range(W):
-            s = f3add(s, f3mul(gp[gamma_index(W, K, j, k)], ood[k][j]))
+            s = f3add(s, f3mul(gw(j, k), ood[k][j]))
         row_sum.append(s)
     zP = f3pow(z, Pn)
     h_sum_z = ZERO
     for i in range(Pn):
-        h_sum_z = f3add(h_sum_z, f3mul(gp[W * K + i], pood[i]))
+        h_sum_z = f3add(h_sum_z, f3mul(gp[S + i], pood[i]))
     wN = root_of_unity(bits)
     toff_fri = pow(g, 1 << tf, P)
     wT = root_of_unity(tl.bit_length() - 1)
@@ -501,13 +578,13 @@ def _verify_table(air, pr, t, Bf, g, fri_k, num_queries, grinding, z_logup, alph
             for k in range(K):
                 bs = ZERO
                 for j in range(W):
-                    bs = f3add(bs, f3mul(gp[gamma_index(W, K, j, k)], tvals[j]))
+                    bs = f3add(bs, f3mul(gw(j, k), tvals[j]))
                 den = f3sub((x, 0, 0), f3scale(z, pow(wn, k, P)))
                 val = f3add(val, f3mul(f3sub(bs, row_sum[k]), f3inv(den)))
             pw = [int(v) for v in np.asarray(parts_rows[q, s]).reshape(-1)]
             hs = ZERO
             for pi in range(Pn):
-                hs = f3add(hs, f3mul(gp[W * K + pi], (pw[3 * pi] % P, pw[3 * pi + 1] % P, pw[3 * pi + 2] % P)))
+                hs = f3add(hs, f3mul(gp[S + pi], (pw[3 * pi] % P, pw[3 * pi + 1] % P, pw[3 * pi + 2] % P)))
             val = f3add(val, f3mul(f3sub(hs, h_sum_z), f3inv(f3sub((x, 0, 0), zP))))
             deep_pair.append(val)
         p0, p0s = deep_pair

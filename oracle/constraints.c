@@ -20,6 +20,7 @@
 
 #include "gl.h"
 #include "lvm_oracle.h"
+#include "par.h"
 
 enum { OP_CONST_BASE = 0, OP_CONST_EXT, OP_VAR, OP_RAP, OP_ALPHA, OP_OFFSET, OP_ADD, OP_SUB, OP_MUL, OP_NEG, OP_EMBED };
 enum { K_BASE_SLOT = 0, K_EXT_SLOT, K_BASE_CONST, K_EXT_CONST, K_RAP, K_ALPHA, K_OFFSET };
@@ -65,10 +66,22 @@ static int root_nodes_of(const ora_program *p, uint32_t *out) {
     return 0;
 }
 
+/* Constraints computed by each node, as chains: head[k] is the first
+ * constraint whose value node k computes (or -1), nxt[c] the next one. */
+static void root_chains(const ora_program *p, const uint32_t *root_node, int32_t *head, int32_t *nxt) {
+    for (size_t k = 0; k < p->num_nodes; k++) head[k] = -1;
+    for (size_t c = p->num_roots; c-- > 0;) {
+        nxt[c] = head[root_node[c]];
+        head[root_node[c]] = (int32_t)c;
+    }
+}
+
 /* one row's walk; slots vb[num_base_slots], ve[num_ext_slots]; the value of
- * constraint c is captured into rv[c] when node root_node[c] has run */
+ * constraint c is captured into rv[c] when node root_node[c] has run (the
+ * chains of root_chains list them per node) */
 static void walk(const ora_program *p, const uint64_t *main, size_t m, const uint64_t *aux, size_t ma, size_t N,
-                 size_t next_step, size_t i, uint64_t *vb, fp3 *ve, const uint32_t *root_node, fp3 *rv) {
+                 size_t next_step, size_t i, uint64_t *vb, fp3 *ve, const int32_t *head, const int32_t *nxt,
+                 fp3 *rv) {
     for (size_t k = 0; k < p->num_nodes; k++) {
         const uint64_t w0 = p->nodes[2 * k], w1 = p->nodes[2 * k + 1];
         const uint32_t op = (uint32_t)w0, a = (uint32_t)(w0 >> 32), b = (uint32_t)w1, res = (uint32_t)(w1 >> 32);
@@ -121,8 +134,7 @@ static void walk(const ora_program *p, const uint64_t *main, size_t m, const uin
         }
         if (rext) ve[slot] = as_ext(r);
         else vb[slot] = r.b;
-        for (size_t c = 0; c < p->num_roots; c++)
-            if (root_node[c] == k) rv[c] = as_ext(r);
+        for (int32_t c = head[k]; c >= 0; c = nxt[c]) rv[c] = as_ext(r);
     }
 }
 
@@ -132,16 +144,97 @@ int ora_eval_constraints(const ora_program *p, const uint64_t *main, size_t m, c
     fp3 *ve = calloc(p->num_ext_slots + 1, sizeof(fp3));
     uint32_t *rn = calloc(p->num_roots + 1, 4);
     fp3 *rv = calloc(p->num_roots + 1, sizeof(fp3));
-    if (!vb || !ve || !rn || !rv || root_nodes_of(p, rn)) return -1;
-    for (size_t i = 0; i < N; i++) {
-        walk(p, main, m, aux, ma, N, next_step, i, vb, ve, rn, rv);
-        for (size_t c = 0; c < p->num_roots; c++) memcpy(evals_out + 3 * (c * N + i), rv[c].c, 24);
+    int32_t *head = calloc(p->num_nodes + 1, 4), *nxt = calloc(p->num_roots + 1, 4);
+    int rc = 0;
+    if (!vb || !ve || !rn || !rv || !head || !nxt || root_nodes_of(p, rn)) {
+        rc = -1;
+    } else {
+        root_chains(p, rn, head, nxt);
+        for (size_t i = 0; i < N; i++) {
+            walk(p, main, m, aux, ma, N, next_step, i, vb, ve, head, nxt, rv);
+            for (size_t c = 0; c < p->num_roots; c++) memcpy(evals_out + 3 * (c * N + i), rv[c].c, 24);
+        }
     }
     free(vb);
     free(ve);
     free(rn);
     free(rv);
-    return 0;
+    free(head);
+    free(nxt);
+    return rc;
+}
+
+typedef struct {
+    const ora_program *p;
+    const uint64_t *main, *aux;
+    size_t m, ma, N, B, next_step;
+    const ora_composition *cp;
+    const int32_t *head, *nxt;
+    const uint64_t *zcyc;
+    uint64_t wN, w_last, wn;
+    uint64_t *h_out;
+    int failed;
+} compctx;
+
+/* rows [b, e): the walk and the composition sum per row, rayon-style chunks
+ * (crypto/stark/src/constraints/evaluator.rs:117-137 per-thread buffers) */
+static void comp_body(void *vctx, size_t b, size_t e, int th) {
+    (void)th;
+    compctx *c = (compctx *)vctx;
+    const ora_program *p = c->p;
+    const ora_composition *cp = c->cp;
+    uint64_t *vb = calloc(p->num_base_slots + 1, 8);
+    fp3 *ve = calloc(p->num_ext_slots + 1, sizeof(fp3));
+    fp3 *rv = calloc(p->num_roots + 1, sizeof(fp3));
+    if (!vb || !ve || !rv) {
+        c->failed = 1;
+        free(vb);
+        free(ve);
+        free(rv);
+        return;
+    }
+    uint64_t zg[64];
+    uint64_t x = gl_mul(cp->coset_offset, gl_pow(c->wN, b));
+    for (size_t i = b; i < e; i++) {
+        walk(p, c->main, c->m, c->aux, c->ma, c->N, c->next_step, i, vb, ve, c->head, c->nxt, rv);
+        /* each zerofier group's inverse zerofier at this row, once */
+        for (size_t g = 0; g < cp->num_groups && g < 64; g++) {
+            uint64_t z = c->zcyc[i % c->B];
+            uint64_t r = c->w_last;
+            for (uint32_t j = 0; j < cp->group_exemptions[g]; j++) {
+                z = gl_mul(z, gl_sub(x, r));
+                r = gl_mul(r, c->w_last);
+            }
+            zg[g] = z;
+        }
+        fp3 h = fp3_zero();
+        for (size_t k = 0; k < p->num_roots; k++) {
+            const fp3 beta = ld3(cp->beta + 3 * k);
+            h = fp3_add(h, fp3_mul(fp3_mul_base(rv[k], zg[cp->root_group[k]]), beta));
+        }
+        uint64_t last_step = 0, zb = 0;
+        int have = 0;
+        for (size_t bi = 0; bi < cp->num_boundary; bi++) {
+            /* 1/(x - w^step); consecutive constraints on one step share it */
+            if (!have || cp->b_step[bi] != last_step) {
+                last_step = cp->b_step[bi];
+                zb = gl_inv(gl_sub(x, gl_pow(c->wn, last_step)));
+                have = 1;
+            }
+            fp3 t;
+            if (cp->b_is_aux[bi])
+                t = ld3(c->aux + 3 * (i * c->ma + cp->b_col[bi]));
+            else
+                t = fp3_from_base(c->main[i * c->m + cp->b_col[bi]]);
+            const fp3 bp = fp3_sub(t, ld3(cp->b_value + 3 * bi));
+            h = fp3_add(h, fp3_mul(fp3_mul_base(ld3(cp->b_beta + 3 * bi), zb), bp));
+        }
+        memcpy(c->h_out + 3 * i, h.c, 24);
+        x = gl_mul(x, c->wN);
+    }
+    free(vb);
+    free(ve);
+    free(rv);
 }
 
 int ora_eval_composition(const ora_program *p, const uint64_t *main, size_t m, const uint64_t *aux, size_t ma,
@@ -155,11 +248,16 @@ int ora_eval_composition(const ora_program *p, const uint64_t *main, size_t m, c
     const uint64_t wn = gl_root_of_unity(log_n), wN = gl_root_of_unity(log_N);
     /* 1/(x^n - 1) repeats with period B: x_i^n = offset^n * (w_N^n)^i */
     uint64_t *zcyc = malloc(8 * B);
-    uint64_t *vb = calloc(p->num_base_slots + 1, 8);
-    fp3 *ve = calloc(p->num_ext_slots + 1, sizeof(fp3));
     uint32_t *rn = calloc(p->num_roots + 1, 4);
-    fp3 *rv = calloc(p->num_roots + 1, sizeof(fp3));
-    if (!zcyc || !vb || !ve || !rn || !rv || root_nodes_of(p, rn)) return -1;
+    int32_t *head = calloc(p->num_nodes + 1, 4), *nxt = calloc(p->num_roots + 1, 4);
+    if (!zcyc || !rn || !head || !nxt || root_nodes_of(p, rn)) {
+        free(zcyc);
+        free(rn);
+        free(head);
+        free(nxt);
+        return -1;
+    }
+    root_chains(p, rn, head, nxt);
     {
         const uint64_t step = gl_pow(wN, n);
         uint64_t d = gl_pow(cp->coset_offset, n);
@@ -168,42 +266,16 @@ int ora_eval_composition(const ora_program *p, const uint64_t *main, size_t m, c
             d = gl_mul(d, step);
         }
     }
-    const uint64_t w_last = gl_pow(wn, n - 1); /* w^(n-1) = w^-1 */
-    uint64_t x = cp->coset_offset;
-    for (size_t i = 0; i < N; i++) {
-        walk(p, main, m, aux, ma, N, next_step, i, vb, ve, rn, rv);
-        fp3 h = fp3_zero();
-        for (size_t c = 0; c < p->num_roots; c++) {
-            const uint32_t g = cp->root_group[c];
-            uint64_t z = zcyc[i % B];
-            uint64_t r = w_last;
-            for (uint32_t j = 0; j < cp->group_exemptions[g]; j++) {
-                z = gl_mul(z, gl_sub(x, r));
-                r = gl_mul(r, w_last);
-            }
-            const fp3 beta = ld3(cp->beta + 3 * c);
-            h = fp3_add(h, fp3_mul(fp3_mul_base(rv[c], z), beta));
-        }
-        for (size_t bi = 0; bi < cp->num_boundary; bi++) {
-            const uint64_t point = gl_pow(wn, cp->b_step[bi]);
-            const uint64_t zb = gl_inv(gl_sub(x, point));
-            fp3 t;
-            if (cp->b_is_aux[bi])
-                t = ld3(aux + 3 * (i * ma + cp->b_col[bi]));
-            else
-                t = fp3_from_base(main[i * m + cp->b_col[bi]]);
-            const fp3 bp = fp3_sub(t, ld3(cp->b_value + 3 * bi));
-            h = fp3_add(h, fp3_mul(fp3_mul_base(ld3(cp->b_beta + 3 * bi), zb), bp));
-        }
-        memcpy(h_out + 3 * i, h.c, 24);
-        x = gl_mul(x, wN);
-    }
+    int bad = cp->num_groups > 64;
+    for (size_t k = 0; k < p->num_roots; k++)
+        if (cp->root_group[k] >= cp->num_groups) bad = 1;
+    compctx c = {p, main, aux, m, ma, N, B, next_step, cp, head, nxt, zcyc, wN, gl_pow(wn, n - 1), wn, h_out, 0};
+    if (!bad) par_for(N, 1024, comp_body, &c);
     free(zcyc);
-    free(vb);
-    free(ve);
     free(rn);
-    free(rv);
-    return 0;
+    free(head);
+    free(nxt);
+    return bad ? -3 : c.failed ? -1 : 0;
 }
 
 /* Degree-2 split (crypto/stark/src/prover.rs:1605-1654): h natural N Fp3 ->

@@ -35,11 +35,18 @@ def _f3pow(a, e):
 
 
 def replay_single_table(tr: np.ndarray, blowup: int, g: int, k: int, queries: int, label: bytes):
-    n, m = tr.shape
-    N = n * blowup
     t = Transcript(label)
     lde, nodes = lde_commit(tr, blowup, g)
     t.append_bytes(bytes(nodes[0]))
+    return replay_committed_table(lde, nodes, t, tr.shape[0], blowup, g, k, queries)
+
+
+def replay_committed_table(lde: np.ndarray, nodes: np.ndarray, t: Transcript, n: int, blowup: int, g: int, k: int,
+                           queries: int):
+    """Rounds R3/R4 of a committed table (natural-order LDE and its tree) whose
+    root the transcript `t` has absorbed."""
+    m = lde.shape[1]
+    N = n * blowup
     while True:
         z = [int(v) for v in t.sample_fp3()]
         zt = _f3pow(z, n)

@@ -40,3 +40,24 @@ def test_product_package_never_imports_the_oracle():
                 text = open(os.path.join(root, f), errors="replace").read()
                 assert "import oracle" not in text and "from oracle" not in text, f
                 assert "lvm_oracle" not in text and "liblvm_oracle" not in text, f
+
+
+@pytest.mark.parametrize("n,m,blowup", [(1, 4, 4), (3, 4, 4), (16, 0, 4), (16, 4, 3), (1 << 31, 1, 4)])
+def test_lde_shapes_refused_before_device_work(n, m, blowup):
+    """Shapes the transform cannot take (one row, non-powers of two, no
+    columns, a domain past 2^32) are refused with LVM_ERR_ARG before any
+    device is touched, so the caller's CPU arm runs (gpu_lde.rs:313-335)."""
+    lib = _native.lib()
+    dummy = (ctypes.c_uint64 * 4)()
+    h = ctypes.c_void_p()
+    root = (ctypes.c_uint8 * 32)()
+    for fn in ("lvm_coset_lde_dev", "lvm_coset_lde_row_major_with_merkle_tree_keep"):
+        f = getattr(lib, fn)
+        if fn == "lvm_coset_lde_dev":
+            rc = f(dummy, ctypes.c_size_t(n), ctypes.c_size_t(m), ctypes.c_size_t(blowup), dummy, root,
+                   ctypes.byref(h))
+        else:
+            rc = f(dummy, ctypes.c_size_t(n), ctypes.c_size_t(m), ctypes.c_size_t(blowup), dummy, None, root,
+                   ctypes.byref(h))
+        assert rc == 1, (fn, rc)  # LVM_ERR_ARG
+        assert h.value is None

@@ -6,7 +6,10 @@ verifier restatement accepts the GPU proofs (also after a round trip through
 the bytes).
 Tables: FibonacciMultiColumn (one composition part), Quadratic, a 4-bus
 LogUp table and the VM's EQ, BYTEWISE, IS_WH, LT, STORE, BRANCH, MEMW_REGISTER
-and LOAD tables (two parts), alone and together in one transcript."""
+and LOAD tables (two parts), a one-interaction bus table (degree 2: one
+part) and a degree-4 bus table with and without interactions (three parts),
+alone and together in one transcript. Bus tables open only their
+accumulated column at the next row."""
 import numpy as np
 import pytest
 
@@ -96,6 +99,14 @@ def _tables(n, which, seed):
             a.bus_descriptor = A.descriptor(a)
             keys = [V.is_wh_key(int(v) << 16) for v in rng.integers(0, 2**47, n - 3)]
             tr = V.generate_is_wh_trace(V.dedup_is_wh_rows(keys))
+        elif kind == "bus_deg2":
+            a = A.logup_air(24, A.four_bus_interactions()[:1])
+            a.bus_descriptor = A.descriptor(a)
+            tr = workload.trace_table(8, 3, n, 24)
+        elif kind in ("product", "product_nobus"):
+            a = A.product_bus_air(A.four_bus_interactions() if kind == "product" else [])
+            a.bus_descriptor = A.descriptor(a) if a.interactions else None
+            tr = A.product_trace(n, 4 if kind == "product" else 5)
         else:
             a = A.logup_air(24, A.four_bus_interactions())
             a.bus_descriptor = A.descriptor(a)
@@ -119,7 +130,9 @@ def _f3rows(x):
                                                      (("branch", "store", "eq"), 256, 4, 2, 8, 2),
                                                      (("memw_register", "branch"), 512, 4, 3, 8, 0),
                                                      (("load", "memw_register"), 256, 4, 2, 8, 2),
-                                                     (("fib", "quad", "logup"), 512, 4, 3, 10, 4)])
+                                                     (("fib", "quad", "logup"), 512, 4, 3, 10, 4),
+                                                     (("bus_deg2", "product"), 256, 4, 2, 8, 0),
+                                                     (("product_nobus", "fib"), 512, 8, 2, 8, 2)])
 def test_gpu_proofs_match_oracle_and_verify(gpu, which, n, blowup, k, q, grind):
     from lambda_vm_amd import prover
     tables = _tables(n, which, seed=n + len(which))

@@ -162,7 +162,7 @@ def test_composition_d2_commit_matches_oracle(gpu, ora, jit, which, n, B):
     args = (B, n, G, c["betas"], c["groups"], c["ex"], c["bnd"], c["b_betas"], c["rap"], c["alpha"], c["toff"])
     N = n * B
     hb = gpu.eval_composition(air.device, c["main_h"], c["aux_h"], *args)  # bit-reversed rows
-    parts = gpu.composition_commit_d2(hb, N, G)
+    parts = gpu.composition_commit_d2(hb, N, n, G)
     ref_h = ora.eval_composition(air.device, c["main_lde"], c["aux_lde"], *args)
     halves = ora.decompose_d2(ref_h, G)
     ref_lde, ref_nodes = ora.lde_commit(halves, 2, pow(G, P - 2, P))
@@ -188,3 +188,44 @@ def test_pinned_reference_style_program(gpu, ora, jit, which):
     ev = gpu.eval_constraints(dev, c["main_h"], c["aux_h"], 4, c["rap"], c["alpha"], c["toff"])
     ev_ref = ora.eval_constraints(dev, c["main_lde"], c["aux_lde"], 4, c["rap"], c["alpha"], c["toff"])
     assert np.array_equal(canon(ev), canon(ev_ref))
+
+
+@pytest.mark.parametrize("num_parts,n,B", [(1, 64, 4), (2, 128, 4), (3, 64, 4), (4, 256, 8), (5, 32, 8)])
+def test_composition_commit_parts_matches_oracle(gpu, ora, num_parts, n, B):
+    """lvm_composition_commit_parts for 1..5 parts against the oracle's
+    (H itself, the degree-2 split, break_in_parts on the coset): the parts
+    LDE, its root, and the parts' values at a point by the barycentric
+    formula over the n trace-coset rows."""
+    from oracle.air_proof import break_in_parts_lde
+    N = n * B
+    H = workload.trace_table(13, num_parts, N, 3)  # natural-order values
+    perm = _bitrev_perm(N)
+    hb = gpu.DeviceBuffer.from_array(np.ascontiguousarray(H[perm]))
+    parts = gpu.composition_commit_parts(hb, N, n, num_parts, G)
+    if num_parts == 1:
+        ref = H
+    elif num_parts == 2:
+        ref, _ = ora.lde_commit(ora.decompose_d2(H, G), 2, pow(G, P - 2, P))
+    else:
+        ref = break_in_parts_lde(H, num_parts, G)
+    nodes = ora.commit_rows_bit_reversed(ref)
+    assert parts.root == bytes(nodes[0])
+    assert np.array_equal(canon(parts.download()), canon(ref))
+    z = np.array([5, 7, 11], dtype=np.uint64)
+    got = gpu.barycentric_base_multi(parts, G, z.reshape(1, 3))[0]
+    want = ora.barycentric_base(ref, n, B, G, z)
+    assert np.array_equal(canon(got), canon(want))
+    hb.free()
+    parts.close()
+
+
+def test_composition_commit_parts_rejects_bad_shapes(gpu):
+    hb = gpu.DeviceBuffer(64 * 24)
+    with pytest.raises(gpu.LvmError):
+        gpu.composition_commit_parts(hb, 64, 16, 9, G)   # more than 8 parts
+    with pytest.raises(gpu.LvmError):
+        gpu.composition_commit_parts(hb, 64, 64, 3, G)   # trace rows above N/2
+    with pytest.raises(gpu.LvmError):
+        gpu.composition_commit_parts(hb, 64, 16, 0, G)
+    hb.free()
+

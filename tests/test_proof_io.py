@@ -12,6 +12,7 @@ from oracle.air_proof import verify_air_tables
 from test_oracle_air_proof import OPTS, _prove, _tables
 
 LABEL = b"air-proof-test"
+P = 0xFFFFFFFF00000001
 
 
 @pytest.fixture(scope="module")
@@ -72,3 +73,49 @@ def test_flipped_bytes_caught(proved):
 def struct_error():
     import struct
     return struct.error
+
+
+def test_pruned_next_row_block_round_trip(proved):
+    """Bus tables carry a one-column next-row block (the accumulated column),
+    the example AIRs one column per trace column: both read back as written."""
+    _, proofs = proved
+    widths = [np.asarray(p.ood_block1).shape[0] for p in proofs]
+    assert widths == [4, 3, 1]
+    back = proof_io.deserialize_multi_proof(proof_io.serialize_multi_proof(proofs))
+    for a, b in zip(proofs, back):
+        assert np.array_equal(np.asarray(a.ood_block1, dtype=np.uint64) % np.uint64(P), b.ood_block1)
+
+
+@pytest.mark.parametrize("where", ["block0_width", "next_width", "next_height", "parts", "fri_roots", "queries"])
+def test_oversized_counts_refused(proved, where):
+    """A length prefix larger than the bytes left is refused before anything
+    is allocated for it (a corrupted count of 2^60 would otherwise ask for
+    exabytes or loop for ages)."""
+    _, proofs = proved
+    p = proofs[0]
+    data = bytearray(proof_io.serialize_table_proof(p))
+    W = np.asarray(p.ood_block0).reshape(-1, 3).shape[0]
+    C1, h1 = np.asarray(p.ood_block1).shape[:2]
+    off = 8 + 32 + 1 + 1  # n, main root, aux tag (none), precomputed tag
+    pos = {"block0_width": off}
+    off += 16 + 24 * W
+    pos["next_width"] = off
+    pos["next_height"] = off + 8
+    off += 16 + 24 * C1 * h1 + 32
+    pos["parts"] = off
+    off += 8 + 24 * len(p.parts_ood)
+    pos["fri_roots"] = off
+    off += 8 + 32 * len(p.fri_roots)
+    off += 8 + 24 * len(p.fri_coeffs)
+    pos["queries"] = off
+    data[pos[where]:pos[where] + 8] = (1 << 60).to_bytes(8, "little")
+    with pytest.raises(ValueError):
+        proof_io.deserialize_table_proof(bytes(data))
+
+
+def test_truncated_proof_refused(proved):
+    _, proofs = proved
+    data = proof_io.serialize_table_proof(proofs[2])
+    for cut in (3, 50, len(data) // 2, len(data) - 1):
+        with pytest.raises(ValueError):
+            proof_io.deserialize_table_proof(data[:cut])

@@ -106,3 +106,26 @@ def test_memw_register_proof_verifies():
     proofs = replay_air_tables([(air, V.generate_memw_register_trace(acc))], 4, 3, 2, 6, 0, b"memw-r")
     ok, why = verify_air_tables([air], proofs, 4, 3, 2, 6, 0, b"memw-r")
     assert ok, why
+
+
+def test_generator_refuses_accesses_the_table_cannot_route():
+    """RegRow::from_memw's conditions (prover/src/tables/memw_register.rs:
+    130-195): the old access shares the timestamp's high word and lies 1..2^16
+    below it in the low word; register indexes fit 16 bits. A violating access
+    would give a trace whose Memory/IS_HALFWORD tokens do not balance, so the
+    generator refuses it."""
+    import pytest
+    ok = V.RegisterAccess(2, HI + 100, 7, 5, HI + 99, True)
+    V.generate_memw_register_trace([ok])
+    bad = [V.RegisterAccess(2, HI + 100, 7, 5, (4 << 32) + 99, True),      # other high word
+           V.RegisterAccess(2, HI + 100, 7, 5, HI + 100, True),            # delta 0
+           V.RegisterAccess(2, HI + (1 << 17), 7, 5, HI + 1, True),        # delta > 2^16
+           V.RegisterAccess(2 << 16, HI + 100, 7, 5, HI + 99, True)]       # index >= 2^16
+    for a in bad:
+        with pytest.raises(ValueError):
+            V.generate_memw_register_trace([a])
+    # values wider than 64 bits are taken mod 2^64
+    t = V.generate_memw_register_trace([V.RegisterAccess(2, HI + 100, (1 << 64) + 9, -1, HI + 99, False)])
+    c = V.MemwRegisterCols
+    assert int(t[0, c.VAL_0]) == 9 and int(t[0, c.VAL_1]) == 0
+    assert int(t[0, c.OLD_0]) == 0xFFFFFFFF and int(t[0, c.OLD_1]) == 0xFFFFFFFF
