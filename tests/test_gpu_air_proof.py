@@ -116,8 +116,10 @@ def _tables(n, which, seed):
 
 
 def _f3rows(x):
-    return canon(np.array([[int(c) for c in v] for v in x], dtype=np.uint64)) if not isinstance(x, np.ndarray) \
-        else canon(x).reshape(-1, 3)
+    """Fp3 values as canonical (k, 3) rows (k = 0 for an empty block)."""
+    if not isinstance(x, np.ndarray):
+        x = np.array([[int(c) for c in v] for v in x], dtype=np.uint64)
+    return canon(x).reshape(-1, 3)
 
 
 @pytest.mark.parametrize("which,n,blowup,k,q,grind", [(("fib",), 256, 4, 2, 8, 0), (("quad",), 128, 8, 3, 6, 0),

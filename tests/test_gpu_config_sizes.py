@@ -65,7 +65,8 @@ def test_single_table_2p22x64_full_rounds(gpu, ora):
     assert proof.z == ref["z"]
     assert np.array_equal(canon(proof.ood), canon(ref["ood"]))
     assert np.array_equal(proof.fri_roots, ref["roots"])
-    assert len(ref["roots"]) == 24 - 2 - k
+    # folds from 2^24 down to the 2^(2+k) terminal; the last fold is not committed
+    assert len(ref["roots"]) == 24 - (2 + k) - 1
     assert np.array_equal(canon(proof.fri_coeffs), canon(ref["coeffs"]))
     assert proof.iotas.tolist() == ref["iotas"]
     assert proof.transcript_state == ref["state"]
