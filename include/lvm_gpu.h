@@ -368,6 +368,17 @@ int lvm_composition_commit_parts(const uint64_t *dev_h, size_t N, size_t trace_r
  * BE(nonce))[0..8] < 2^(64-g) (crypto/stark/src/grinding.rs:21-93); status
  * LVM_ERR_STATE when there is none below max_nonce. */
 int lvm_generate_nonce(const uint8_t seed[32], uint8_t grinding_factor, uint64_t max_nonce, uint64_t *nonce_out);
+/* No counterpart in the reference: lvm_generate_nonce for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc). Same arguments, result and status
+ * codes; it calls no hipMalloc/hipFree and creates no stream. Its scratch (the
+ * block minima of one batch of 2^24 candidates, 512 KiB + 8 B) is a fixed
+ * device array each GPU holds until the process ends; searches of any length
+ * run batch by batch through it, so no input size exceeds it. Needs lvm_init
+ * on the current device first (LVM_ERR_STATE otherwise). Calls that overlap
+ * on one device share that array and race. */
+int lvm_generate_nonce_noalloc(const uint8_t seed[32], uint8_t grinding_factor, uint64_t max_nonce,
+                               uint64_t *nonce_out);
 
 /* ---------------- Fiat-Shamir transcript (host) ----------------
  * crypto/crypto/src/fiat_shamir/default_transcript.rs:31-146

@@ -92,6 +92,7 @@ SIGNATURES = {
     "lvm_composition_commit_parts": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_uint64, u8p,
                                                vpp]),
     "lvm_generate_nonce": (C.c_int, [u8p, C.c_uint8, C.c_uint64, u64p]),
+    "lvm_generate_nonce_noalloc": (C.c_int, [u8p, C.c_uint8, C.c_uint64, u64p]),
     "lvm_transcript_new": (C.c_int, [u8p, C.c_size_t, vpp]),
     "lvm_transcript_clone": (C.c_int, [C.c_void_p, vpp]),
     "lvm_transcript_release": (C.c_int, [C.c_void_p]),

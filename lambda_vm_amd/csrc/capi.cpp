@@ -168,7 +168,8 @@ int lvm_init(int device) {
     if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(LVM_ERR_DEVICE, "no HIP device");
     if (device < 0 || device >= count) return fail(LVM_ERR_ARG, "device index out of range");
     LVM_HIP(hipSetDevice(device));
-    return device_state() ? LVM_OK : LVM_ERR_DEVICE;
+    if (!device_state()) return LVM_ERR_DEVICE;
+    return grind_scratch_setup(device);
 }
 
 int lvm_device_count(int *count) {

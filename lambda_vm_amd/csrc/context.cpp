@@ -149,6 +149,13 @@ hipStream_t take_spare_stream(int id) {
 }
 }  // namespace
 
+hipStream_t thread_stream_if_any(Device *d) {
+    if (!d) return nullptr;
+    auto &ts = t_streams;
+    if ((int)ts.s.size() > d->id && ts.s[d->id]) return ts.s[d->id];
+    return d->stream;
+}
+
 hipStream_t thread_stream(Device *d) {
     if (!d) return nullptr;
     auto &ts = t_streams;

@@ -50,6 +50,12 @@ Device *device_state();  // current device's state, initialised on first use; nu
 // thread to another is complete when the producing call returns (every
 // entry point that yields a handle or host data synchronises its stream).
 hipStream_t thread_stream(Device *d);
+// The calling thread's library stream if it already has one, else the
+// device's first stream; creates nothing (for the allocation-free entry point).
+hipStream_t thread_stream_if_any(Device *d);
+// Resolves the per-device address of the nonce search's fixed scratch array
+// (grinding.hip); lvm_init calls it with the device current.
+int grind_scratch_setup(int device);
 
 // fault injection for the fallback tests (never set in production)
 bool take_injected_fault();
