@@ -17,7 +17,11 @@ rounds (OOD, DEEP, FRI, queries).
 Prints one JSON line: value = LDE cells (rows x columns of the evaluation
 domains) per second, roofline of the dominant kernel launch from HIP events
 recorded on the stream it runs on, and the oracle (C restatement of the
-reference CPU prover) timed on the host cores over a bounded sample.
+reference CPU prover) timed on the host cores over a bounded sample. The
+default workload's config also carries the PCIe-inclusive rate (every trace
+uploaded from page-locked host memory before each batched step,
+value_with_h2d) and the p50/p99 latency of a short proving-service run
+(60 multi-table jobs at 20 jobs/s, --serve-jobs / --rate / --no-serve).
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload ragged|single]
 """
@@ -131,6 +135,9 @@ def main() -> int:
     ap.add_argument("--tables", type=int, default=14, help="ragged: number of tables")
     ap.add_argument("--rate", type=float, default=20.0, help="serve: job arrivals per second (Poisson)")
     ap.add_argument("--jobs", type=int, default=120, help="serve: number of jobs")
+    ap.add_argument("--serve-jobs", type=int, default=60,
+                    help="ragged: jobs of the short serving pass that reports p50/p99 latency")
+    ap.add_argument("--no-serve", action="store_true", help="ragged: skip the short serving pass")
     ap.add_argument("--budget-gib", type=float, default=48.0, help="serve: HBM budget for jobs in flight")
     ap.add_argument("--workers", type=int, default=8, help="serve: host threads proving tables concurrently")
     ap.add_argument("--tables-per-tick", type=int, default=8, help="serve: tables advanced per scheduler tick")
@@ -166,6 +173,84 @@ def copy_rate_gbs(backend) -> float:
     return rate
 
 
+class PinnedHost:
+    """Page-locked host memory from the HIP runtime (hipHostMalloc), as the
+    reference stages host traces before upload (crypto/math-cuda/src/device.rs:148-156)."""
+
+    def __init__(self, nbytes: int):
+        import ctypes as C
+        self._hip = C.CDLL("libamdhip64.so")
+        self.ptr = C.c_void_p()
+        if self._hip.hipHostMalloc(C.byref(self.ptr), C.c_size_t(nbytes), C.c_uint(0)) != 0:
+            raise RuntimeError("hipHostMalloc failed")
+        self.nbytes = nbytes
+
+    def array(self, count: int, offset_words: int = 0):
+        import ctypes as C
+
+        import numpy as np
+        buf = (C.c_uint64 * count).from_address(self.ptr.value + 8 * offset_words)
+        return np.ctypeslib.as_array(buf)
+
+    def free(self) -> None:
+        if self.ptr:
+            self._hip.hipHostFree(self.ptr)
+            self.ptr = None
+
+
+def h2d_inclusive(backend, plan, tables, hosts, steps: int) -> dict:
+    """Batched steps that first upload every trace from page-locked host memory
+    into its device buffer (the end-to-end rate with the PCIe transfer)."""
+    import ctypes as C
+    total = sum(h.nbytes for h in hosts)
+    pin = PinnedHost(total)
+    views, off = [], 0
+    for h in hosts:
+        v = pin.array(h.size, off)
+        v[:] = h.reshape(-1)
+        views.append(v)
+        off += h.size
+    ref = plan.run()
+
+    def upload():
+        for (tr, *_), v in zip(tables, views):
+            backend.N.call("lvm_memcpy_htod", C.c_void_p(tr.ptr), C.c_void_p(v.ctypes.data), v.nbytes)
+
+    upload()
+    backend.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        upload()
+    t_h2d = (time.perf_counter() - t0) / steps
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        upload()
+        if plan.run() != ref:
+            raise RuntimeError("roots changed after the upload")
+    t_all = (time.perf_counter() - t0) / steps
+    pin.free()
+    return {"h2d_ms_per_step": round(t_h2d * 1e3, 3), "h2d_GBps": round(total / t_h2d / 1e9, 2),
+            "ms_per_step_with_h2d": round(t_all * 1e3, 3), "trace_bytes": total}
+
+
+def short_serving_pass(jobs: int, rate: float, budget_gib: float, workers: int) -> dict:
+    """A short run of the proving service (Poisson arrivals of multi-table
+    jobs, HBM-budget admission, work-conserving dispatch): p50/p99 latency."""
+    from lambda_vm_amd import serve
+
+    budget = int(budget_gib * (1 << 30))
+    ex = serve.GpuExecutor(offset=3, fri_k=6, num_queries=30, workers=workers)
+    warm = serve.ProvingService(budget, ex)
+    warm.run_async([(0.0, serve.JobSpec(10 ** 6, [(1 << 10, 8)] * max(4, workers)))])
+    svc = serve.ProvingService(budget, ex)
+    st = svc.run_async(serve.synth_jobs(rate, jobs, seed=11))
+    ex.close()
+    return {"p50_ms": round(st["p50_ms"], 3), "p99_ms": round(st["p99_ms"], 3),
+            "jobs_per_s": round(st["jobs_per_s"], 3), "serve_jobs": st["jobs"], "serve_rate_per_s": rate,
+            "serve_gcells_per_s": round(st["gcells_per_s"], 4),
+            "serve_mean_jobs_per_commit_batch": round(st["mean_jobs_per_commit_batch"], 3)}
+
+
 def ragged_cpu_baseline(shapes, offset: int) -> dict:
     """The oracle's LDE + commit of every table of the workload at half its
     rows (bounded sample), on all host threads."""
@@ -199,11 +284,11 @@ def run_ragged(args) -> dict:
     offset = workload.GENERATOR
     shapes = workload.ragged_shapes(cfg=3, count=args.tables)
     log(f"[bench] ragged workload: {len(shapes)} tables {shapes}")
-    tables = []
+    tables, hosts = [], []
     for t, (n, m, b) in enumerate(shapes):
         tr = workload.trace_table(3, t, n, m)
         tables.append((backend.DeviceBuffer.from_array(tr), backend.coset_weights_dev(n, offset), n, m, b))
-        del tr
+        hosts.append(tr)
     cells = sum(n * m * b for n, m, b in shapes)
     copy_gbs = copy_rate_gbs(backend)
 
@@ -239,6 +324,10 @@ def run_ragged(args) -> dict:
     for _ in range(args.steps):
         per_table()
     dt_pt = (time.perf_counter() - t0) / args.steps
+
+    log(f"[bench] timing {args.steps} batched steps with the trace upload from pinned host memory")
+    pcie = h2d_inclusive(backend, plan, tables, hosts, args.steps)
+    del hosts
 
     # the same launches without the graph, each bracketed by HIP events on
     # the library stream, for the per-launch durations of the roofline
@@ -307,12 +396,18 @@ def run_ragged(args) -> dict:
             "batched_over_per_table": round(dt_pt / dt, 3),
             "batched_eager_ms_per_step": round(dt_eager * 1e3, 3),
             "roots_sha": __import__("hashlib").sha256(b"".join(ref)).hexdigest(),
+            # PCIe-inclusive: every trace uploaded from page-locked host memory, then the batched step
+            "value_with_h2d": round(cells / (pcie["ms_per_step_with_h2d"] / 1e3) / 1e9, 4),
+            **pcie,
         },
         "roofline": roofline,
     }
     for tr, w, *_ in tables:
         tr.free()
         w.free()
+    if not args.no_serve:
+        log(f"[bench] serving pass: {args.serve_jobs} jobs at {args.rate} jobs/s")
+        out["config"].update(short_serving_pass(args.serve_jobs, args.rate, args.budget_gib, args.workers))
     if not args.no_cpu_baseline:
         log("[bench] cpu baseline")
         out["cpu_baseline"] = ragged_cpu_baseline(shapes, offset)
