@@ -135,6 +135,8 @@ def main() -> int:
     ap.add_argument("--tables", type=int, default=14, help="ragged: number of tables")
     ap.add_argument("--rate", type=float, default=20.0, help="serve: job arrivals per second (Poisson)")
     ap.add_argument("--jobs", type=int, default=120, help="serve: number of jobs")
+    ap.add_argument("--sweep-rates", default="",
+                    help="serve: comma-separated arrival rates (jobs/s) of a saturation sweep, e.g. 20,60,120,240")
     ap.add_argument("--serve-jobs", type=int, default=60,
                     help="ragged: jobs of the short serving pass that reports p50/p99 latency")
     ap.add_argument("--no-serve", action="store_true", help="ragged: skip the short serving pass")
@@ -430,6 +432,16 @@ def run_serve(args) -> dict:
         f"{args.workers} proving threads, {args.serve_mode} dispatch")
     svc = serve.ProvingService(budget, ex, tables_per_tick=args.tables_per_tick)
     st = svc.run_async(arrivals) if args.serve_mode == "async" else svc.run(arrivals)
+    sweep = []
+    for rate in [float(r) for r in args.sweep_rates.split(",") if r]:
+        log(f"[bench] sweep: {args.jobs} jobs at {rate} jobs/s")
+        sv = serve.ProvingService(budget, ex, tables_per_tick=args.tables_per_tick)
+        sw = sv.run_async(serve.synth_jobs(rate, args.jobs, seed=11)) if args.serve_mode == "async" else \
+            sv.run(serve.synth_jobs(rate, args.jobs, seed=11))
+        sweep.append({"offered_jobs_per_s": rate, "jobs_per_s": round(sw["jobs_per_s"], 3),
+                      "p50_ms": round(sw["p50_ms"], 3), "p99_ms": round(sw["p99_ms"], 3),
+                      "gcells_per_s": round(sw["gcells_per_s"], 4),
+                      "mean_jobs_per_commit_batch": round(sw["mean_jobs_per_commit_batch"], 3)})
     ex.close()
     out = {
         "metric": "batched LDE+Keccak-Merkle Gcells/s vs per-table launch; p50/p99 proof latency",
@@ -459,6 +471,7 @@ def run_serve(args) -> dict:
             "workers": args.workers,
             "dispatch": args.serve_mode,
             "tables_per_tick": args.tables_per_tick if args.serve_mode == "tick" else None,
+            "saturation_sweep": sweep or None,
         },
     }
     return out
@@ -516,12 +529,18 @@ def run_constraints(args) -> dict:
         ninv = pow(n_l, workload.P - 2, workload.P)
         toff = np.array([x * ninv % workload.P for x in L], dtype=np.uint64)
         out = list(roots) + [aux_h.root]
+        # the three compositions in shared launches (one compiled kernel over all tables)
+        comps = []
         for i, a in enumerate(airs):
             groups, ex = a.groups()
             bnd = [(b.col, b.is_aux, b.step, b.value) for b in a.boundary]
-            extra = (dict(rap=[z, al], alpha=ap, table_offset=toff) if i == 2 else {})
-            H = backend.eval_composition(a.device, mains[i], aux_h if i == 2 else None, B, specs[i][1], offset,
-                                         betas[i], groups, ex, bnd, [r3() for _ in bnd], **extra)
+            comps.append(dict(dev=a.device, main=mains[i], aux=aux_h if i == 2 else None, next_step=B,
+                              trace_rows=specs[i][1], coset_offset=offset, betas=betas[i], root_group=groups,
+                              group_exemptions=ex, boundary=bnd, b_betas=[r3() for _ in bnd],
+                              rap=[z, al] if i == 2 else None, alpha=ap if i == 2 else None,
+                              table_offset=toff if i == 2 else None))
+        Hs = backend.eval_composition_batch(comps)
+        for i, H in enumerate(Hs):
             parts = backend.composition_commit_d2(H, specs[i][1] * B, specs[i][1], offset)
             out.append(parts.root)
             H.free()
@@ -572,7 +591,7 @@ def run_constraints(args) -> dict:
         "config": {
             "workload": ("constraint + LogUp stage: 2^21x80 FibonacciMultiColumn, 2^20x48 Quadratic, 2^18x24 "
                          "4-bus LogUp, blowup 4: batched main LDE+commit, LogUp aux build + aux LDE+commit, "
-                         "composition of each table, degree-2 split + parts commit"),
+                         "the three compositions in shared launches, degree-2 split + parts commit"),
             "lde_cells": cells,
         },
         "roofline": {

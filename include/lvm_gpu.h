@@ -104,6 +104,20 @@ int lvm_coset_lde_row_major_with_merkle_tree_keep(const uint64_t *trace, size_t 
 /* Same with the trace and weights already in device memory (row-major n x m). */
 int lvm_coset_lde_dev(const uint64_t *dev_trace, size_t n, size_t m, size_t blowup, const uint64_t *dev_weights,
                       uint8_t root_out[32], lvm_lde **handle_out);
+/* crypto/math-cuda/src/lde.rs:726 `coset_lde_row_major_split_trees` for a
+ * preprocessed table (device trace: the reference's `predev`): one coset LDE
+ * of all m columns and two subset trees over it, the fixed (precomputed)
+ * columns [0, split_col) and the multiplicity columns [split_col, m), leaves =
+ * the row-pair bit-reversed Keccak leaves of the column range
+ * (crypto/stark/src/prover.rs:895-961). The handle holds the LDE (all
+ * columns; lvm_lde_open returns whole rows) and the multiplicity tree, whose
+ * root goes to root_out; pre_root_out receives the precomputed tree's root
+ * and pre_nodes_out (nullable: the reference's build_precomputed = false) its
+ * node buffer, (2L-1) x 32 bytes with L = n*blowup/2, for host-side openings
+ * as in the reference. */
+int lvm_coset_lde_split_trees_dev(const uint64_t *dev_trace, size_t n, size_t m, size_t blowup,
+                                  const uint64_t *dev_weights, size_t split_col, uint8_t *pre_nodes_out,
+                                  uint8_t pre_root_out[32], uint8_t root_out[32], lvm_lde **handle_out);
 /* Fp3 tables: crypto/math-cuda/src/lde.rs:855 `coset_lde_ext3_row_major_with_merkle_tree_keep`.
  * `trace` holds n rows of m Fp3 elements (3m u64 per row); leaves stream the
  * 24-byte BE encoding of each element (extensions_goldilocks.rs:497-503). */
@@ -317,6 +331,20 @@ typedef struct lvm_composition {
  * bit-reversed order, or natural order when natural_order != 0. */
 int lvm_eval_composition(const lvm_program *program, const lvm_lde *main, const lvm_lde *aux, size_t next_step,
                          const lvm_composition *comp, int natural_order, uint64_t *dev_h_out);
+/* lvm_eval_composition for `count` tables (table t: programs[t], mains[t],
+ * auxs[t] (nullable), next_steps[t], comps[t], output dev_h_out[t]) in
+ * shared launches: the natural-order column copies of every table in one
+ * launch per element width, the boundary step inverses of every table in one
+ * launch, and every table's constraint program compiled into one kernel that
+ * runs once over all tables' LDE rows; the per-table composition of
+ * crypto/stark/src/prover.rs:3974-3977 (constraint_interp.rs:353 per table)
+ * as one stage. Each output equals lvm_eval_composition's for that table; if
+ * the run-time compiler is off or declines a program, the tables are
+ * evaluated one by one. */
+int lvm_eval_composition_batch(const lvm_program *const *programs, const lvm_lde *const *mains,
+                               const lvm_lde *const *auxs, const size_t *next_steps,
+                               const lvm_composition *const *comps, size_t count, int natural_order,
+                               uint64_t *const *dev_h_out);
 /* crypto/math-cuda/src/constraint_interp.rs:53 `eval_constraints_on_device`:
  * dev_evals_out [(c*N + row)*3 + limb] in natural row order; base-rooted
  * constraints carry their value in component 0. */
@@ -437,6 +465,20 @@ int lvm_fri_release(lvm_fri *f);
  * entry (gpu_lde.rs:3196-3229) and a non-zero status is returned. */
 int lvm_fri_commit_phase(lvm_fri *f, lvm_transcript *t, unsigned blowup_log, unsigned k, uint8_t *roots_out,
                          size_t *num_committed_out, uint64_t *coeffs_out, size_t *num_coeffs_out);
+/* lvm_fri_commit_phase for `count` tables at once, table t on fs[t] with its
+ * own transcript ts[t] and blowup 2^blowup_log[t] (all with terminal parameter
+ * k): per layer depth, one launch folds (and hashes the leaves of) every table
+ * still committing at that depth, one launch per tree step builds all their
+ * layer trees and one copy brings the roots back, where each table's
+ * transcript absorbs its root and samples its next zeta. Every table's roots,
+ * coefficients and transcript end as lvm_fri_commit_phase leaves them
+ * (roots_out[t]: room for its committed layers x 32 bytes; coeffs_out[t]: for
+ * 2^k Fp3). The multi-table drive of crypto/stark/src/gpu_lde.rs:3164-3289
+ * batched over the tables of crypto/stark/src/prover.rs:3974-3977. On an
+ * error every transcript is restored to its state at entry. */
+int lvm_fri_commit_phase_batch(lvm_fri *const *fs, lvm_transcript *const *ts, size_t count,
+                               const unsigned *blowup_log, unsigned k, uint8_t *const *roots_out,
+                               size_t *num_committed_out, uint64_t *const *coeffs_out, size_t *num_coeffs_out);
 
 /* ---------------- rounds of a committed table ---------------- */
 /* Caller-provided outputs of lvm_prove_committed_table, for a table of m

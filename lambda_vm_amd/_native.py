@@ -51,6 +51,8 @@ SIGNATURES = {
     "lvm_coset_lde_row_major_with_merkle_tree_keep": (
         C.c_int, [u64p, C.c_size_t, C.c_size_t, C.c_size_t, u64p, u64p, u8p, vpp]),
     "lvm_coset_lde_dev": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_void_p, u8p, vpp]),
+    "lvm_coset_lde_split_trees_dev": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, C.c_void_p,
+                                                C.c_size_t, C.c_void_p, u8p, u8p, vpp]),
     "lvm_coset_lde_ext3_row_major_with_merkle_tree_keep": (
         C.c_int, [u64p, C.c_size_t, C.c_size_t, C.c_size_t, u64p, u64p, u8p, vpp]),
     "lvm_lde_release": (C.c_int, [C.c_void_p]),
@@ -84,6 +86,8 @@ SIGNATURES = {
     "lvm_logup_term_columns": (C.c_int, [u64p, C.c_size_t, C.c_size_t, C.c_void_p, u64p, C.c_size_t, u64p, u64p]),
     "lvm_eval_composition": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_int,
                                        C.c_void_p]),
+    "lvm_eval_composition_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                             C.c_size_t, C.c_int, C.c_void_p]),
     "lvm_eval_constraints": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
     "lvm_commit_rows_dev": (C.c_int, [C.c_void_p, C.c_size_t, C.c_size_t, C.c_size_t, u8p, vpp]),
     "lvm_prepare_constraints": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -115,6 +119,8 @@ SIGNATURES = {
     "lvm_fri_query": (C.c_int, [C.c_void_p, u64p, C.c_size_t, u64p, u8p]),
     "lvm_fri_release": (C.c_int, [C.c_void_p]),
     "lvm_fri_commit_phase": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint, C.c_uint, u8p, szp, u64p, szp]),
+    "lvm_fri_commit_phase_batch": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p, C.c_uint, C.c_void_p,
+                                             C.c_void_p, C.c_void_p, C.c_void_p]),
     "lvm_prove_committed_table": (C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_uint, C.c_uint64, C.c_uint,
                                             C.c_size_t, C.c_void_p]),
     "lvm_prof_enable": (C.c_int, [C.c_int]),
@@ -204,11 +210,16 @@ def call(name: str, *args) -> None:
 _initialised = False
 
 
-def ensure_device(device: int = 0) -> None:
-    """Initialise the library on `device`; raises when no GPU is visible."""
+def ensure_device(device: int = 0, tuned: bool = True) -> None:
+    """Initialise the library on `device` and install the tuned NTT pass
+    splits (lambda_vm_amd/tune.py) when a table for this GPU exists; raises
+    when no GPU is visible."""
     global _initialised
     if not _initialised:
         call("lvm_init", device)
+        if tuned:
+            from . import tune
+            tune.install(call)
         _initialised = True
 
 

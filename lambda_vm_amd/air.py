@@ -67,6 +67,11 @@ class AirProgram:
     max_degree: int = 1
     bus: bool = True
     example_parts: int | None = None
+    # preprocessed tables (prover.rs:1194-1290): the first num_precomputed
+    # columns are fixed per table, produced by precomputed() as (n, k), and
+    # commit to a tree of their own; the rest (multiplicities) to the main tree
+    num_precomputed: int = 0
+    precomputed: object = None
 
     @property
     def num_constraints(self) -> int:

@@ -209,6 +209,38 @@ def coset_lde_dev(trace: DeviceBuffer, n: int, m: int, blowup_factor: int, weigh
     return GpuLde(h.value)
 
 
+def coset_lde_split_trees_dev(trace: DeviceBuffer, n: int, m: int, blowup_factor: int, weights: DeviceBuffer,
+                              split_col: int, want_precomputed_nodes: bool = True):
+    """A preprocessed table's commitment (crypto/math-cuda/src/lde.rs:726):
+    (GpuLde with the multiplicity-columns tree, precomputed root, precomputed
+    node array (2L-1, 32) or None)."""
+    N.ensure_device()
+    root = np.zeros(32, dtype=np.uint8)
+    pre_root = np.zeros(32, dtype=np.uint8)
+    L = n * blowup_factor // 2
+    pre_nodes = np.zeros((2 * L - 1, 32), dtype=np.uint8) if want_precomputed_nodes else None
+    h = C.c_void_p()
+    N.call("lvm_coset_lde_split_trees_dev", C.c_void_p(trace.ptr), n, m, blowup_factor, C.c_void_p(weights.ptr),
+           split_col, pre_nodes.ctypes.data_as(C.c_void_p) if pre_nodes is not None else None, _p8(pre_root),
+           _p8(root), C.byref(h))
+    return GpuLde(h.value), pre_root.tobytes(), pre_nodes
+
+
+def merkle_paths_from_nodes(nodes: np.ndarray, positions) -> np.ndarray:
+    """Authentication paths (Q, depth, 32) of leaves `positions` in a host
+    node array (2L-1, 32) (crypto/crypto/src/merkle_tree/utils.rs:7-21)."""
+    L = (nodes.shape[0] + 1) // 2
+    depth = L.bit_length() - 1
+    out = np.zeros((len(positions), depth, 32), dtype=np.uint8)
+    for q, pos in enumerate(positions):
+        idx = int(pos) + L - 1
+        for k in range(depth):
+            sib = idx - 1 if idx % 2 == 0 else idx + 1
+            out[q, k] = nodes[sib]
+            idx = (idx - 1) // 2 if idx % 2 == 0 else idx // 2
+    return out
+
+
 def dev_copy(dst: DeviceBuffer, src: DeviceBuffer, nbytes: int) -> None:
     """Device-to-device copy by the library's plain copy kernel."""
     N.call("lvm_dev_copy", C.c_void_p(dst.ptr), C.c_void_p(src.ptr), nbytes)
@@ -405,6 +437,16 @@ def eval_composition(dev, main: GpuLde, aux, next_step: int, trace_rows: int, co
     step, value3) tuples."""
     N.ensure_device()
     keep = []
+    s, c = _composition_structs(dev, trace_rows, coset_offset, betas, root_group, group_exemptions, boundary,
+                                b_betas, rap, alpha, table_offset, keep)
+    out = DeviceBuffer(main.lde_rows * 24)
+    N.call("lvm_eval_composition", C.byref(s), main.h, aux.h if aux is not None else None, next_step, C.byref(c),
+           int(natural_order), C.c_void_p(out.ptr))
+    return out
+
+
+def _composition_structs(dev, trace_rows, coset_offset, betas, root_group, group_exemptions, boundary, b_betas, rap,
+                         alpha, table_offset, keep):
     s = _program_struct(dev, rap, alpha, table_offset, keep)
     c = N.Composition()
     c.trace_rows = trace_rows
@@ -420,10 +462,37 @@ def eval_composition(dev, main: GpuLde, aux, next_step: int, trace_rows: int, co
     c.b_value = _arr_ptr(np.asarray([b[3] for b in boundary], dtype=np.uint64).reshape(-1) if boundary else None,
                          np.uint64, keep)
     c.b_beta = _arr_ptr(np.asarray(b_betas, dtype=np.uint64).reshape(-1) if boundary else None, np.uint64, keep)
-    out = DeviceBuffer(main.lde_rows * 24)
-    N.call("lvm_eval_composition", C.byref(s), main.h, aux.h if aux is not None else None, next_step, C.byref(c),
-           int(natural_order), C.c_void_p(out.ptr))
-    return out
+    keep.extend([s, c])
+    return s, c
+
+
+def eval_composition_batch(tables, natural_order: bool = False) -> list:
+    """The composition polynomials of several tables in shared launches
+    (lvm_eval_composition_batch). tables: one dict per table with the
+    arguments of eval_composition (dev, main, aux, next_step, trace_rows,
+    coset_offset, betas, root_group, group_exemptions and optionally boundary,
+    b_betas, rap, alpha, table_offset). Returns one device buffer per table,
+    each equal to eval_composition's."""
+    N.ensure_device()
+    T = len(tables)
+    if T == 0:
+        return []
+    keep, progs, comps, mains, auxs, steps, outs = [], [], [], [], [], [], []
+    for t in tables:
+        s, c = _composition_structs(t["dev"], t["trace_rows"], t["coset_offset"], t["betas"], t["root_group"],
+                                    t["group_exemptions"], t.get("boundary", ()), t.get("b_betas"), t.get("rap"),
+                                    t.get("alpha"), t.get("table_offset"), keep)
+        progs.append(C.addressof(s))
+        comps.append(C.addressof(c))
+        mains.append(t["main"].h.value if isinstance(t["main"].h, C.c_void_p) else t["main"].h)
+        aux = t.get("aux")
+        auxs.append(None if aux is None else (aux.h.value if isinstance(aux.h, C.c_void_p) else aux.h))
+        steps.append(t["next_step"])
+        outs.append(DeviceBuffer(t["main"].lde_rows * 24))
+    N.call("lvm_eval_composition_batch", (C.c_void_p * T)(*progs), (C.c_void_p * T)(*mains),
+           (C.c_void_p * T)(*auxs), (C.c_size_t * T)(*steps), (C.c_void_p * T)(*comps), T, int(natural_order),
+           (C.c_void_p * T)(*[o.ptr for o in outs]))
+    return outs
 
 
 def eval_constraints(dev, main: GpuLde, aux, next_step: int, rap=None, alpha=None, table_offset=None) -> np.ndarray:
@@ -718,6 +787,32 @@ class GpuFri:
         N.call("lvm_fri_commit_phase", self.h, transcript.h, blowup_log, k, _p8(roots), C.byref(nc), _p64(coeffs),
                C.byref(ncf))
         return roots[: nc.value].copy(), coeffs[: ncf.value].copy()
+
+
+def fri_commit_phase_batch(fris, transcripts, blowup_logs, k: int):
+    """The FRI commit phase of several tables in shared launches
+    (lvm_fri_commit_phase_batch): [(roots (num_committed, 32), terminal
+    coefficients (2^k_eff, 3))] per table, each transcript advanced as by its
+    own GpuFri.commit_phase."""
+    T = len(fris)
+    if T == 0:
+        return []
+    fs = (C.c_void_p * T)(*[f.h.value for f in fris])
+    ts = (C.c_void_p * T)(*[t.h.value for t in transcripts])
+    bl = (C.c_uint * T)(*blowup_logs)
+    roots, coeffs = [], []
+    for f in fris:
+        ln, off = C.c_size_t(), C.c_uint64()
+        N.call("lvm_fri_current", f.h, None, C.byref(ln), C.byref(off))
+        lde_log = ln.value.bit_length() - 1
+        roots.append(np.zeros((max(lde_log, 1), 32), dtype=np.uint8))
+        coeffs.append(np.zeros((1 << max(lde_log, 0), 3), dtype=np.uint64))
+    rp = (C.c_void_p * T)(*[r.ctypes.data for r in roots])
+    cp = (C.c_void_p * T)(*[c.ctypes.data for c in coeffs])
+    nc = (C.c_size_t * T)()
+    ncf = (C.c_size_t * T)()
+    N.call("lvm_fri_commit_phase_batch", fs, ts, T, bl, k, rp, nc, cp, ncf)
+    return [(roots[t][: nc[t]].copy(), coeffs[t][: ncf[t]].copy()) for t in range(T)]
 
 
 def inject_fault_after(calls: int) -> None:

@@ -89,9 +89,11 @@ struct Writer {
     }
 };
 
+}  // namespace
+
 // Tree steps of one table with L leaves (power of two): wide levels one per
 // step while more than 2^14 children remain, then subtree tops.
-std::vector<std::pair<TreeStep, uint32_t>> tree_steps(uint8_t *nodes_b, uint64_t L) {
+std::vector<std::pair<TreeStep, uint32_t>> lvm::tree_steps(uint8_t *nodes_b, uint64_t L) {
     std::vector<std::pair<TreeStep, uint32_t>> out;  // (step, blocks)
     uint64_t *nodes = (uint64_t *)nodes_b;
     uint64_t C = L;
@@ -109,6 +111,8 @@ std::vector<std::pair<TreeStep, uint32_t>> tree_steps(uint8_t *nodes_b, uint64_t
     }
     return out;
 }
+
+namespace {
 
 // Walks the whole launch sequence, writing descriptors through `w` and, when
 // `record` is set, the launch list of the plan.

@@ -87,17 +87,24 @@ int check_lde_shape(size_t n, size_t w, size_t blowup) {
 }
 
 // LDE + commit from device-resident trace rows (n x w words) and weights.
+// Coset LDE of a device trace and its Merkle tree over the row-pair leaves of
+// columns [split, w) (all columns when split = 0). With split > 0 a second
+// tree over columns [0, split) is built too (a preprocessed table's fixed
+// columns): its root goes to pre_root_out and, when pre_nodes_host is set,
+// its node buffer to the host.
 int lde_core(Device *d, const uint64_t *d_trace, size_t n, size_t w, size_t blowup, const uint64_t *d_weights,
-             lvm_lde **out, uint8_t root_out[32]) {
+             lvm_lde **out, uint8_t root_out[32], size_t split = 0, uint8_t *pre_nodes_host = nullptr,
+             uint8_t *pre_root_out = nullptr) {
     TRY(check_lde_shape(n, w, blowup));
     const size_t N = n * blowup;
     TRY(check_fault());
     hipStream_t st = thread_stream(d);
     const size_t L = N / 2;
-    DevBuf coef, lde, nodes;
+    DevBuf coef, lde, nodes, pre;
     TRY(coef.alloc(n * w * 8, st));
     TRY(lde.alloc(N * w * 8, st));
     TRY(nodes.alloc((2 * L - 1) * 32, st));
+    if (split) TRY(pre.alloc((2 * L - 1) * 32, st));
     {
         ProfScope ps("ntt_lde", st);
         ntt_inverse_rows(d_trace, (uint32_t)w, coef.as<uint64_t>(), (uint32_t)w, (uint32_t)w, log2u(n), d->tb, st);
@@ -106,18 +113,25 @@ int lde_core(Device *d, const uint64_t *d_trace, size_t n, size_t w, size_t blow
     }
     {
         ProfScope ps("keccak_leaves", st);
-        keccak_row_pair_leaves(lde.as<uint64_t>(), w, 0, w, L, nodes.as<uint8_t>() + (L - 1) * 32, st);
+        keccak_row_pair_leaves(lde.as<uint64_t>(), w, split, w, L, nodes.as<uint8_t>() + (L - 1) * 32, st);
+        if (split) keccak_row_pair_leaves(lde.as<uint64_t>(), w, 0, split, L, pre.as<uint8_t>() + (L - 1) * 32, st);
     }
     {
         ProfScope ps("merkle_inner", st);
         merkle_build_inner(nodes.as<uint8_t>(), L, st);
+        if (split) merkle_build_inner(pre.as<uint8_t>(), L, st);
     }
     LVM_HIP(hipGetLastError());
     uint8_t root[32];
     {
         StageScope ss(st);
         LVM_HIP(stage_download(root, nodes.as<uint8_t>(), 32, st));
+        if (split && pre_root_out) LVM_HIP(stage_download(pre_root_out, pre.as<uint8_t>(), 32, st));
         LVM_HIP(stage_sync(st));
+        if (split && pre_nodes_host) {
+            LVM_HIP(hipMemcpyAsync(pre_nodes_host, pre.p, (2 * L - 1) * 32, hipMemcpyDeviceToHost, st));
+            LVM_HIP(hipStreamSynchronize(st));
+        }
     }
     lvm_lde *h = new lvm_lde();
     h->device = d->id;
@@ -313,6 +327,18 @@ int lvm_coset_lde_dev(const uint64_t *dev_trace, size_t n, size_t m, size_t blow
     Device *d = dev_or_fail();
     if (!d) return LVM_ERR_DEVICE;
     return lde_core(d, dev_trace, n, m, blowup, dev_weights, handle_out, root_out);
+}
+
+int lvm_coset_lde_split_trees_dev(const uint64_t *dev_trace, size_t n, size_t m, size_t blowup,
+                                  const uint64_t *dev_weights, size_t split_col, uint8_t *pre_nodes_out,
+                                  uint8_t pre_root_out[32], uint8_t root_out[32], lvm_lde **handle_out) {
+    if (!dev_trace || !dev_weights || !handle_out || !pre_root_out) return fail(LVM_ERR_ARG, "null argument");
+    TRY(check_lde_shape(n, m, blowup));
+    if (split_col == 0 || split_col >= m) return fail(LVM_ERR_ARG, "split column must fall inside the row");
+    Device *d = dev_or_fail();
+    if (!d) return LVM_ERR_DEVICE;
+    return lde_core(d, dev_trace, n, m, blowup, dev_weights, handle_out, root_out, split_col, pre_nodes_out,
+                    pre_root_out);
 }
 
 namespace {

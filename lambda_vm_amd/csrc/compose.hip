@@ -248,10 +248,10 @@ __global__ __launch_bounds__(COMP_BLOCK) void compose_kernel(CompArgs a) {
 // with one base-field inversion (Montgomery's trick, prefixes in registers);
 // then 1 / (x - c_s) = D^-1 * prod_{u != s} (x - c_u).
 constexpr int STEP_INV_E = 16;
-__global__ __launch_bounds__(256) void step_inverse_kernel(const CompArgs a, uint64_t *__restrict__ binv) {
+// thread t of T threads covering one table
+__device__ __forceinline__ void step_inverse_body(const CompArgs &a, uint64_t *__restrict__ binv, uint64_t t,
+                                                  uint64_t T) {
     const uint64_t N = 1ull << a.log_N;
-    const uint64_t T = (uint64_t)gridDim.x * blockDim.x;
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= N) return;
     const uint32_t S = a.num_steps;
     auto point = [&](uint64_t p) -> uint64_t {
@@ -287,6 +287,25 @@ __global__ __launch_bounds__(256) void step_inverse_kernel(const CompArgs a, uin
             binv[((uint64_t)s << a.log_N) + p] = v;
         }
     }
+}
+
+__global__ __launch_bounds__(256) void step_inverse_kernel(const CompArgs a, uint64_t *__restrict__ binv) {
+    step_inverse_body(a, binv, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, (uint64_t)gridDim.x * blockDim.x);
+}
+
+// blocks per table of the step-inverse pass (STEP_INV_E points per thread)
+static uint64_t step_inverse_blocks(uint64_t N) { return ((N + STEP_INV_E - 1) / STEP_INV_E + 255) / 256; }
+
+// The step inverses of several tables: blocks [prefix[t], prefix[t+1]) take
+// table t (args[t].binv is its output).
+__global__ __launch_bounds__(256) void step_inverse_batched_kernel(const CompArgs *__restrict__ args,
+                                                                   const uint32_t *__restrict__ prefix,
+                                                                   uint32_t count) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t t = find_slot(prefix, count, b);
+    const CompArgs &a = args[t];
+    const uint64_t T = (uint64_t)(prefix[t + 1] - prefix[t]) * blockDim.x;
+    step_inverse_body(a, const_cast<uint64_t *>(a.binv), (uint64_t)(b - prefix[t]) * blockDim.x + threadIdx.x, T);
 }
 
 
@@ -327,14 +346,13 @@ __global__ __launch_bounds__(256) void transpose_kernel(const uint64_t *__restri
 // one such 256-row tile for 16 words of the row: its reads are runs of 8
 // row segments, its stores 32-row column runs.
 template <int E>
-__global__ __launch_bounds__(256) void transpose_bitrev_tile_kernel(const uint64_t *__restrict__ in,
-                                                                    uint64_t *__restrict__ out, uint64_t N,
-                                                                    uint32_t wpr, unsigned logN) {
+__device__ __forceinline__ void transpose_bitrev_tile_body(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
+                                                           uint64_t N, uint32_t wpr, unsigned logN, uint32_t blk) {
     constexpr int WC = 16;
     __shared__ uint64_t tile[256][WC + 1];
     const uint32_t nch = (wpr + WC - 1) / WC;  // a tile's word chunks are adjacent blocks
-    const uint32_t m = blockIdx.x / nch;
-    const uint32_t w0 = (blockIdx.x % nch) * WC;
+    const uint32_t m = blk / nch;
+    const uint32_t w0 = (blk % nch) * WC;
     const uint32_t wn = wpr - w0 < (uint32_t)WC ? wpr - w0 : (uint32_t)WC;
     const unsigned mid = logN - 8;
     const uint64_t mrev = mid ? (uint64_t)(__brev(m) >> (32 - mid)) : 0;
@@ -364,6 +382,32 @@ __global__ __launch_bounds__(256) void transpose_bitrev_tile_kernel(const uint64
             out[((uint64_t)(w / E) * N + r) * E + (w % E)] = tile[h * 32 + l][ww];
         }
     }
+}
+
+template <int E>
+__global__ __launch_bounds__(256) void transpose_bitrev_tile_kernel(const uint64_t *__restrict__ in,
+                                                                    uint64_t *__restrict__ out, uint64_t N,
+                                                                    uint32_t wpr, unsigned logN) {
+    transpose_bitrev_tile_body<E>(in, out, N, wpr, logN, blockIdx.x);
+}
+
+// One table's copy for the batched transposes (N >= 256).
+struct TransposeJob {
+    const uint64_t *in;
+    uint64_t *out;
+    uint64_t N;
+    uint32_t wpr, logN;
+};
+static uint32_t transpose_tile_blocks(uint64_t N, uint32_t wpr) { return (uint32_t)((N >> 8) * ((wpr + 15) / 16)); }
+
+template <int E>
+__global__ __launch_bounds__(256) void transpose_bitrev_tile_batched_kernel(const TransposeJob *__restrict__ jobs,
+                                                                            const uint32_t *__restrict__ prefix,
+                                                                            uint32_t count) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t t = find_slot(prefix, count, b);
+    const TransposeJob j = jobs[t];
+    transpose_bitrev_tile_body<E>(j.in, j.out, j.N, j.wpr, j.logN, b - prefix[t]);
 }
 
 static void launch_transpose(const uint64_t *in, uint64_t *out, uint64_t N, uint32_t ncols, int E, hipStream_t st) {
@@ -475,9 +519,29 @@ unsigned log2u(size_t x) { return (unsigned)__builtin_ctzll(x); }
 
 }  // namespace
 
-// Shared by the composition and the per-constraint entry points.
-int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux, size_t next_step,
-                 const lvm_composition *cp, int mode, int natural_out, uint64_t *out) {
+// One table's composition (or per-constraint evaluation) ready to launch:
+// its host-side arguments (device pointers filled by compose_bind), the blob
+// of its small tables and their offsets in it.
+struct ComposeJob {
+    const lvm_program *pg = nullptr;
+    const lvm_lde *main = nullptr, *aux = nullptr;
+    int mode = 0;
+    uint64_t *out = nullptr;
+    size_t N = 0;
+    unsigned logN = 0;
+    CompArgs a{};
+    std::vector<uint8_t> blob;
+    std::vector<uint32_t> root_node;
+    const uint32_t *root_group = nullptr;
+    size_t o_nodes = 0, o_bc = 0, o_ec = 0, o_rap = 0, o_al = 0, o_to = 0, o_roots = 0, o_eoff = 0, o_eidx = 0;
+    size_t o_beta = 0, o_rg = 0, o_z = 0, o_bcol = 0, o_baux = 0, o_bval = 0, o_bbeta = 0, o_bpcol = 0, o_bpflag = 0,
+           o_bpbeta = 0;
+};
+
+// Validation and host-side arguments of one table's composition (mode 0)
+// or per-constraint evaluation (mode 1); no device work.
+int compose_prepare(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux, size_t next_step,
+                    const lvm_composition *cp, int mode, int natural_out, uint64_t *out, ComposeJob &j) {
     if (!pg || !main || !out) return fail(LVM_ERR_ARG, "null argument");
     if (mode == 0 && !cp) return fail(LVM_ERR_ARG, "null composition");
     const size_t N = main->N;
@@ -549,14 +613,23 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
             return fail(LVM_ERR_ARG, "root slot out of range");
     }
     const unsigned logN = log2u(N);
-    CompArgs a{};
+    j = ComposeJob{};
+    j.pg = pg;
+    j.main = main;
+    j.aux = aux;
+    j.mode = mode;
+    j.out = out;
+    j.N = N;
+    j.logN = logN;
+    j.root_group = mode == 0 ? cp->root_group : nullptr;
+    CompArgs &a = j.a;
     BoundarySteps jb;
-    std::vector<uint8_t> blob;
+    std::vector<uint8_t> &blob = j.blob;
     std::vector<uint32_t> roots32(pg->num_roots);
     for (size_t c = 0; c < pg->num_roots; c++) roots32[c] = (uint32_t)pg->roots[c];
     // node computing each constraint: given, or (pinned root slots) the last
     // node writing the root's slot
-    std::vector<uint32_t> root_node;
+    std::vector<uint32_t> &root_node = j.root_node;
     if (!root_nodes_of(pg, root_node)) return fail(LVM_ERR_ARG, "root node index out of range or no node writes a root slot");
     std::vector<uint32_t> emit_off(pg->num_nodes + 1, 0), emit_idx(pg->num_roots);
     for (size_t c = 0; c < pg->num_roots; c++) emit_off[root_node[c] + 1]++;
@@ -565,12 +638,15 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
         std::vector<uint32_t> fill(emit_off.begin(), emit_off.end() - 1);
         for (size_t c = 0; c < pg->num_roots; c++) emit_idx[fill[root_node[c]]++] = (uint32_t)c;
     }
-    const size_t o_nodes = put(blob, pg->nodes, 2 * pg->num_nodes), o_bc = put(blob, pg->base_consts, pg->num_base_consts),
-                 o_ec = put(blob, pg->ext_consts, 3 * pg->num_ext_consts), o_rap = put(blob, pg->rap_challenges, 3 * pg->num_rap),
-                 o_al = put(blob, pg->alpha_powers, 3 * pg->num_alpha),
-                 o_to = put(blob, pg->table_offset, pg->table_offset ? 3 : 0), o_roots = put(blob, roots32.data(), roots32.size()),
-                 o_eoff = put(blob, emit_off.data(), emit_off.size()), o_eidx = put(blob, emit_idx.data(), emit_idx.size());
-    size_t o_beta = 0, o_rg = 0, o_z = 0, o_bcol = 0, o_baux = 0, o_bval = 0, o_bbeta = 0, o_bpcol = 0, o_bpflag = 0, o_bpbeta = 0;
+    j.o_nodes = put(blob, pg->nodes, 2 * pg->num_nodes);
+    j.o_bc = put(blob, pg->base_consts, pg->num_base_consts);
+    j.o_ec = put(blob, pg->ext_consts, 3 * pg->num_ext_consts);
+    j.o_rap = put(blob, pg->rap_challenges, 3 * pg->num_rap);
+    j.o_al = put(blob, pg->alpha_powers, 3 * pg->num_alpha);
+    j.o_to = put(blob, pg->table_offset, pg->table_offset ? 3 : 0);
+    j.o_roots = put(blob, roots32.data(), roots32.size());
+    j.o_eoff = put(blob, emit_off.data(), emit_off.size());
+    j.o_eidx = put(blob, emit_idx.data(), emit_idx.size());
     if (mode == 0) {
         const size_t n = cp->trace_rows;
         if (!is_pow2(n) || N % n || n < 2) return fail(LVM_ERR_ARG, "trace rows must be a power of two dividing the LDE size");
@@ -640,13 +716,13 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
             a.step_const[3 * k + 1] = c.c1;
             a.step_const[3 * k + 2] = c.c2;
         }
-        o_beta = put(blob, cp->beta, 3 * pg->num_roots);
-        o_rg = put(blob, cp->root_group, pg->num_roots);
-        o_z = put(blob, zc.data(), B);
-        o_bcol = put(blob, bcol.data(), bcol.size());
-        o_baux = put(blob, baux.data(), baux.size());
-        o_bval = put(blob, bval.data(), bval.size());
-        o_bbeta = put(blob, bbeta.data(), bbeta.size());
+        j.o_beta = put(blob, cp->beta, 3 * pg->num_roots);
+        j.o_rg = put(blob, cp->root_group, pg->num_roots);
+        j.o_z = put(blob, zc.data(), B);
+        j.o_bcol = put(blob, bcol.data(), bcol.size());
+        j.o_baux = put(blob, baux.data(), baux.size());
+        j.o_bval = put(blob, bval.data(), bval.size());
+        j.o_bbeta = put(blob, bbeta.data(), bbeta.size());
         // main-column boundaries per pair of steps: one entry per column, the
         // betas of its boundaries at each step of the pair summed
         std::vector<uint32_t> pcol, pflag;
@@ -683,10 +759,65 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
             pflag.push_back(0);
             pbeta.resize(6, 0);
         }
-        o_bpcol = put(blob, pcol.data(), pcol.size());
-        o_bpflag = put(blob, pflag.data(), pflag.size());
-        o_bpbeta = put(blob, pbeta.data(), pbeta.size());
+        j.o_bpcol = put(blob, pcol.data(), pcol.size());
+        j.o_bpflag = put(blob, pflag.data(), pflag.size());
+        j.o_bpbeta = put(blob, pbeta.data(), pbeta.size());
     }
+    a.log_N = logN;
+    a.next_step = next_step;
+    a.nbs = (uint32_t)pg->num_base_slots;
+    a.nes = (uint32_t)pg->num_ext_slots;
+    a.num_nodes = (uint32_t)pg->num_nodes;
+    a.num_roots = (uint32_t)pg->num_roots;
+    a.out = out;
+    a.natural_out = natural_out ? 1u : 0u;
+    return LVM_OK;
+}
+
+// Device pointers of a prepared job: its blob uploaded at db, the column-major
+// copies of its tables, its step inverses.
+void compose_bind(ComposeJob &j, const uint8_t *db, const uint64_t *cm_main, const uint64_t *cm_aux,
+                  const uint64_t *binv, const Tables &tb) {
+    CompArgs &a = j.a;
+    a.nodes = (const uint64_t *)(db + j.o_nodes);
+    a.base_consts = (const uint64_t *)(db + j.o_bc);
+    a.ext_consts = (const uint64_t *)(db + j.o_ec);
+    a.rap = (const uint64_t *)(db + j.o_rap);
+    a.alpha = (const uint64_t *)(db + j.o_al);
+    a.toff = (const uint64_t *)(db + j.o_to);
+    a.roots = (const uint32_t *)(db + j.o_roots);
+    a.emit_off = (const uint32_t *)(db + j.o_eoff);
+    a.emit_idx = (const uint32_t *)(db + j.o_eidx);
+    a.main = cm_main;
+    a.aux = cm_aux;
+    a.tw_lo = tb.fwd_lo;
+    a.tw_hi = tb.fwd_hi;
+    if (j.mode == 0) {
+        a.beta = (const uint64_t *)(db + j.o_beta);
+        a.root_group = (const uint32_t *)(db + j.o_rg);
+        a.zcyc = (const uint64_t *)(db + j.o_z);
+        a.b_col = (const uint32_t *)(db + j.o_bcol);
+        a.b_is_aux = (const uint32_t *)(db + j.o_baux);
+        a.b_value = (const uint64_t *)(db + j.o_bval);
+        a.b_beta = (const uint64_t *)(db + j.o_bbeta);
+        a.bp_col = (const uint32_t *)(db + j.o_bpcol);
+        a.bp_flags = (const uint32_t *)(db + j.o_bpflag);
+        a.bp_beta = (const uint64_t *)(db + j.o_bpbeta);
+    }
+    a.binv = const_cast<uint64_t *>(binv);
+}
+
+// Shared by the composition and the per-constraint entry points.
+int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux, size_t next_step,
+                 const lvm_composition *cp, int mode, int natural_out, uint64_t *out) {
+    ComposeJob j;
+    int rc = compose_prepare(pg, main, aux, next_step, cp, mode, natural_out, out, j);
+    if (rc != LVM_OK) return rc;
+    const size_t N = j.N;
+    CompArgs &a = j.a;
+    const std::vector<uint8_t> &blob = j.blob;
+    const std::vector<uint32_t> &root_node = j.root_node;
+
     Device *dv = device_state();
     if (!dv) return LVM_ERR_DEVICE;
     if (take_injected_fault()) return fail(LVM_ERR_INJECTED, "injected fault");
@@ -720,45 +851,12 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
     }
     StageScope ss(st);
     hipError_t e = stage_upload(db, blob.data(), blob.size(), st);
-    a.nodes = (const uint64_t *)(db + o_nodes);
-    a.num_nodes = (uint32_t)pg->num_nodes;
-    a.num_roots = (uint32_t)pg->num_roots;
-    a.base_consts = (const uint64_t *)(db + o_bc);
-    a.ext_consts = (const uint64_t *)(db + o_ec);
-    a.rap = (const uint64_t *)(db + o_rap);
-    a.alpha = (const uint64_t *)(db + o_al);
-    a.toff = (const uint64_t *)(db + o_to);
-    a.roots = (const uint32_t *)(db + o_roots);
-    a.emit_off = (const uint32_t *)(db + o_eoff);
-    a.emit_idx = (const uint32_t *)(db + o_eidx);
-    a.nbs = (uint32_t)pg->num_base_slots;
-    a.nes = (uint32_t)pg->num_ext_slots;
-    a.main = cm_main;
-    a.aux = cm_aux;
-    a.log_N = logN;
-    a.next_step = next_step;
-    a.tw_lo = dv->tb.fwd_lo;
-    a.tw_hi = dv->tb.fwd_hi;
-    if (mode == 0) {
-        a.beta = (const uint64_t *)(db + o_beta);
-        a.root_group = (const uint32_t *)(db + o_rg);
-        a.zcyc = (const uint64_t *)(db + o_z);
-        a.b_col = (const uint32_t *)(db + o_bcol);
-        a.b_is_aux = (const uint32_t *)(db + o_baux);
-        a.b_value = (const uint64_t *)(db + o_bval);
-        a.b_beta = (const uint64_t *)(db + o_bbeta);
-        a.bp_col = (const uint32_t *)(db + o_bpcol);
-        a.bp_flags = (const uint32_t *)(db + o_bpflag);
-        a.bp_beta = (const uint64_t *)(db + o_bpbeta);
-    }
-    a.binv = binv;
+    compose_bind(j, db, cm_main, cm_aux, binv, dv->tb);
     if (mode == 0 && a.num_steps) {
         ProfScope ps("composition_step_inverse", st);
         const uint64_t threads = (N + STEP_INV_E - 1) / STEP_INV_E;
         hipLaunchKernelGGL(step_inverse_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, a, binv);
     }
-    a.out = out;
-    a.natural_out = natural_out ? 1u : 0u;
     a.scratch = scratch;
     const size_t lds = in_lds ? slot_bytes * COMP_BLOCK : 0;
     // the compiled form of the program when there is one (compiled on first use)
@@ -788,9 +886,167 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
     return LVM_OK;
 }
 
+// The compositions of several tables in shared launches: the column-major
+// copies in one launch per element width (tables of at least 256 LDE rows),
+// the boundary step inverses in one launch, and every table's program
+// compiled into one kernel launched once over all tables' rows. Tables are
+// evaluated one by one (compose_impl) when the run-time compiler is off or
+// does not take one of the programs.
+int compose_batch_impl(const lvm_program *const *pgs, const lvm_lde *const *mains, const lvm_lde *const *auxs,
+                       const size_t *next_steps, const lvm_composition *const *cps, size_t count, int natural_out,
+                       uint64_t *const *outs) {
+    if (count == 0) return LVM_OK;
+    if (!pgs || !mains || !auxs || !next_steps || !cps || !outs) return fail(LVM_ERR_ARG, "null argument");
+    std::vector<ComposeJob> jobs(count);
+    for (size_t t = 0; t < count; t++) {
+        if (!cps[t]) return fail(LVM_ERR_ARG, "null composition");
+        int rc = compose_prepare(pgs[t], mains[t], auxs[t], next_steps[t], cps[t], 0, natural_out, outs[t], jobs[t]);
+        if (rc != LVM_OK) return rc;
+    }
+    std::vector<const lvm_program *> vp(count);
+    std::vector<std::vector<uint32_t>> vr(count);
+    std::vector<const uint32_t *> vg(count);
+    for (size_t t = 0; t < count; t++) {
+        vp[t] = jobs[t].pg;
+        vr[t] = jobs[t].root_node;
+        vg[t] = jobs[t].root_group;
+    }
+    if (!jit_enabled() || !jit_prepare_batch(vp, vr, vg)) {
+        for (size_t t = 0; t < count; t++) {
+            int rc = compose_impl(pgs[t], mains[t], auxs[t], next_steps[t], cps[t], 0, natural_out, outs[t]);
+            if (rc != LVM_OK) return rc;
+        }
+        return LVM_OK;
+    }
+    Device *dv = device_state();
+    if (!dv) return LVM_ERR_DEVICE;
+    if (take_injected_fault()) return fail(LVM_ERR_INJECTED, "injected fault");
+    hipStream_t st = lib_stream();
+    std::vector<uint64_t *> cm_main(count, nullptr), cm_aux(count, nullptr), binv(count, nullptr);
+    uint8_t *arena = nullptr;
+    auto release = [&]() {
+        for (size_t t = 0; t < count; t++) {
+            lvm_free(cm_main[t]);
+            lvm_free(cm_aux[t]);
+            lvm_free(binv[t]);
+        }
+        lvm_free(arena);
+    };
+    for (size_t t = 0; t < count; t++) {
+        const ComposeJob &j = jobs[t];
+        if (lvm_malloc(&cm_main[t], j.N * j.main->w * 8) != hipSuccess ||
+            (j.aux && lvm_malloc(&cm_aux[t], j.N * j.aux->w * 8) != hipSuccess) ||
+            (j.a.num_steps && lvm_malloc(&binv[t], j.N * j.a.num_steps * 8) != hipSuccess)) {
+            release();
+            return fail(LVM_ERR_NOMEM, "device allocation for the constraint programs failed");
+        }
+    }
+    // arena: every table's blob, then the launch descriptors
+    std::vector<uint8_t> host;
+    auto place = [&](const void *p, size_t bytes) {
+        const size_t off = (host.size() + 255) & ~(size_t)255;
+        host.resize(off + bytes);
+        if (bytes) std::memcpy(host.data() + off, p, bytes);
+        return off;
+    };
+    std::vector<size_t> blob_off(count);
+    for (size_t t = 0; t < count; t++) blob_off[t] = place(jobs[t].blob.data(), jobs[t].blob.size());
+    std::vector<TransposeJob> tj[2];
+    std::vector<uint32_t> tp[2] = {std::vector<uint32_t>(1, 0), std::vector<uint32_t>(1, 0)};
+    std::vector<uint32_t> sp(1, 0), cpfx(1, 0);
+    uint64_t s_total = 0, c_total = 0;
+    for (size_t t = 0; t < count; t++) {
+        const ComposeJob &j = jobs[t];
+        if (j.logN >= 8) {
+            tj[0].push_back({j.main->lde, cm_main[t], j.N, (uint32_t)j.main->w, j.logN});
+            tp[0].push_back(tp[0].back() + transpose_tile_blocks(j.N, (uint32_t)j.main->w));
+            if (j.aux) {
+                tj[1].push_back({j.aux->lde, cm_aux[t], j.N, (uint32_t)j.aux->w, j.logN});
+                tp[1].push_back(tp[1].back() + transpose_tile_blocks(j.N, (uint32_t)j.aux->w));
+            }
+        }
+        s_total += j.a.num_steps ? step_inverse_blocks(j.N) : 0;
+        c_total += (j.N + COMP_BLOCK - 1) / COMP_BLOCK;
+        if (s_total >= (1ull << 31) || c_total >= (1ull << 31) || tp[0].back() >= (1u << 31) || tp[1].back() >= (1u << 31)) {
+            release();
+            return fail(LVM_ERR_ARG, "batched composition exceeds 2^31 blocks");
+        }
+        sp.push_back((uint32_t)s_total);
+        cpfx.push_back((uint32_t)c_total);
+    }
+    // the CompArgs array needs the arena's address: allocate, bind, then place
+    const size_t args_bytes = count * sizeof(CompArgs);
+    size_t o_args = (host.size() + 255) & ~(size_t)255;
+    const size_t total = o_args + args_bytes + 4 * (sp.size() + cpfx.size() + tp[0].size() + tp[1].size()) +
+                         sizeof(TransposeJob) * (tj[0].size() + tj[1].size()) + 8 * 256;
+    if (lvm_malloc(&arena, total) != hipSuccess) {
+        release();
+        return fail(LVM_ERR_NOMEM, "device allocation for the composition descriptors failed");
+    }
+    std::vector<CompArgs> args(count);
+    for (size_t t = 0; t < count; t++) {
+        compose_bind(jobs[t], arena + blob_off[t], cm_main[t], cm_aux[t], binv[t], dv->tb);
+        args[t] = jobs[t].a;
+    }
+    o_args = place(args.data(), args_bytes);
+    const size_t o_sp = place(sp.data(), 4 * sp.size()), o_cp = place(cpfx.data(), 4 * cpfx.size());
+    size_t o_tj[2], o_tp[2];
+    for (int e = 0; e < 2; e++) {
+        o_tj[e] = place(tj[e].data(), sizeof(TransposeJob) * tj[e].size());
+        o_tp[e] = place(tp[e].data(), 4 * tp[e].size());
+    }
+    if (host.size() > total) {
+        release();
+        return fail(LVM_ERR_STATE, "composition descriptor layout overflow");
+    }
+    StageScope ss(st);
+    hipError_t e = stage_upload(arena, host.data(), host.size(), st);
+    if (e == hipSuccess) {
+        ProfScope ps("composition_transpose", st);
+        if (!tj[0].empty())
+            hipLaunchKernelGGL(transpose_bitrev_tile_batched_kernel<1>, dim3(tp[0].back()), dim3(256), 0, st,
+                               (const TransposeJob *)(arena + o_tj[0]), (const uint32_t *)(arena + o_tp[0]),
+                               (uint32_t)tj[0].size());
+        if (!tj[1].empty())
+            hipLaunchKernelGGL(transpose_bitrev_tile_batched_kernel<3>, dim3(tp[1].back()), dim3(256), 0, st,
+                               (const TransposeJob *)(arena + o_tj[1]), (const uint32_t *)(arena + o_tp[1]),
+                               (uint32_t)tj[1].size());
+        for (size_t t = 0; t < count; t++) {
+            const ComposeJob &j = jobs[t];
+            if (j.logN >= 8) continue;
+            launch_transpose(j.main->lde, cm_main[t], j.N, (uint32_t)j.main->w, 1, st);
+            if (j.aux) launch_transpose(j.aux->lde, cm_aux[t], j.N, (uint32_t)(j.aux->w / 3), 3, st);
+        }
+    }
+    if (e == hipSuccess && s_total) {
+        ProfScope ps("composition_step_inverse", st);
+        hipLaunchKernelGGL(step_inverse_batched_kernel, dim3((unsigned)s_total), dim3(256), 0, st,
+                           (const CompArgs *)(arena + o_args), (const uint32_t *)(arena + o_sp), (uint32_t)count);
+    }
+    if (e == hipSuccess) {
+        ProfScope ps("composition_compiled", st);
+        if (!jit_launch_batch(vp, vr, vg, (const CompArgs *)(arena + o_args), (const uint32_t *)(arena + o_cp),
+                              (uint32_t)count, c_total, st))
+            e = hipErrorLaunchFailure;
+    }
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = stage_sync(st);
+    release();
+    if (e != hipSuccess)
+        return fail(LVM_ERR_DEVICE, std::string("batched composition failed: ") + hipGetErrorString(e));
+    return LVM_OK;
+}
+
 }  // namespace lvm
 
 extern "C" {
+
+int lvm_eval_composition_batch(const lvm_program *const *programs, const lvm_lde *const *mains,
+                               const lvm_lde *const *auxs, const size_t *next_steps,
+                               const lvm_composition *const *comps, size_t count, int natural_order,
+                               uint64_t *const *dev_h_out) {
+    return lvm::compose_batch_impl(programs, mains, auxs, next_steps, comps, count, natural_order, dev_h_out);
+}
 
 int lvm_eval_composition(const lvm_program *program, const lvm_lde *main, const lvm_lde *aux, size_t next_step,
                          const lvm_composition *comp, int natural_order, uint64_t *dev_h_out) {

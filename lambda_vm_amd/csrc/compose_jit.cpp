@@ -48,10 +48,12 @@ struct Operand {
 
 constexpr size_t JIT_LOAD_WINDOW = 48;
 
-// Straight-line source for the program; empty on a program this generator
-// does not take (then the interpreter runs it).
-std::string generate(const lvm_program *pg, const std::vector<uint32_t> &root_node, const uint32_t *root_group,
-                     int mode) {
+// Straight-line per-row code of the program for natural LDE index r (the
+// statements after `r` is set, with `a` the CompArgs and `N` the LDE size in
+// scope); empty on a program this generator does not take (then the
+// interpreter runs it).
+std::string generate_row(const lvm_program *pg, const std::vector<uint32_t> &root_node, const uint32_t *root_group,
+                         int mode) {
     const size_t n = pg->num_nodes;
     std::vector<std::vector<uint32_t>> emits(n);
     for (size_t c = 0; c < pg->num_roots; c++) {
@@ -210,14 +212,7 @@ std::string generate(const lvm_program *pg, const std::vector<uint32_t> &root_no
         for (size_t k = w0; k < w1; k++)
             if (!is_load[k]) body += node_text[k];
     }
-    std::string src = "typedef unsigned long long uint64_t;\ntypedef unsigned int uint32_t;\n";
-    src += lvm_jit_field_src;
-    src += "\nnamespace lvm {\n__device__ __forceinline__ void store3(uint64_t *o, const fp3 &v) { o[0] = v.c0; o[1] = v.c1; "
-           "o[2] = v.c2; }\n}\n";
-    src += "extern \"C\" __global__ __launch_bounds__(128) void lvm_constraint_kernel(lvm::CompArgs a) {\n";
-    src += "    using namespace lvm;\n    const uint64_t N = 1ull << a.log_N;\n";
-    src += "    for (uint64_t q = (uint64_t)blockIdx.x * 128 + threadIdx.x; q < N; q += a.total_threads) {\n";
-    src += "        const uint32_t r = (uint32_t)q;\n";
+    std::string src;
     for (uint32_t off = 0; off <= max_off; off++)
         src += "        const uint64_t f" + std::to_string(off) + " = frame_row(a, r, " + std::to_string(off) + ");\n";
     if (mode == 0) {
@@ -239,7 +234,71 @@ std::string generate(const lvm_program *pg, const std::vector<uint32_t> &root_no
         src += "        const fp3 h = composition_tail(a, r, gs);\n";
         src += "        store3(a.out + 3 * (a.natural_out ? (uint64_t)r : (uint64_t)bitrev(r, a.log_N)), h);\n";
     }
+    return src;
+}
+
+std::string source_prelude() {
+    std::string src = "typedef unsigned long long uint64_t;\ntypedef unsigned int uint32_t;\n";
+    src += lvm_jit_field_src;
+    src += "\nnamespace lvm {\n__device__ __forceinline__ void store3(uint64_t *o, const fp3 &v) { o[0] = v.c0; o[1] = v.c1; "
+           "o[2] = v.c2; }\n}\n";
+    return src;
+}
+
+// The single-table kernel: a grid-stride loop over the natural rows.
+std::string generate(const lvm_program *pg, const std::vector<uint32_t> &root_node, const uint32_t *root_group,
+                     int mode) {
+    const std::string row = generate_row(pg, root_node, root_group, mode);
+    if (row.empty()) return "";
+    std::string src = source_prelude();
+    src += "extern \"C\" __global__ __launch_bounds__(128) void lvm_constraint_kernel(lvm::CompArgs a) {\n";
+    src += "    using namespace lvm;\n    const uint64_t N = 1ull << a.log_N;\n";
+    src += "    for (uint64_t q = (uint64_t)blockIdx.x * 128 + threadIdx.x; q < N; q += a.total_threads) {\n";
+    src += "        const uint32_t r = (uint32_t)q;\n";
+    src += row;
     src += "    }\n}\n";
+    return src;
+}
+
+// Several tables' programs in one kernel: blocks [prefix[t], prefix[t+1])
+// take table t's natural rows, 128 per block, one per thread; the body of
+// each distinct program is a device function, the block's table picks it.
+std::string generate_batch(const std::vector<const lvm_program *> &pgs,
+                           const std::vector<std::vector<uint32_t>> &root_nodes,
+                           const std::vector<const uint32_t *> &root_groups) {
+    std::string src = source_prelude();
+    std::vector<std::string> bodies;
+    std::vector<size_t> fn_of(pgs.size());
+    for (size_t t = 0; t < pgs.size(); t++) {
+        const std::string row = generate_row(pgs[t], root_nodes[t], root_groups[t], 0);
+        if (row.empty()) return "";
+        size_t f = 0;
+        while (f < bodies.size() && bodies[f] != row) f++;
+        if (f == bodies.size()) bodies.push_back(row);
+        fn_of[t] = f;
+    }
+    for (size_t f = 0; f < bodies.size(); f++) {
+        src += "__device__ __forceinline__ void lvm_program_" + std::to_string(f) +
+               "(const lvm::CompArgs &a, const uint32_t r) {\n    using namespace lvm;\n"
+               "    const uint64_t N = 1ull << a.log_N;\n    (void)N;\n    {\n";
+        src += bodies[f];
+        src += "    }\n}\n";
+    }
+    src += "extern \"C\" __global__ __launch_bounds__(128) void lvm_constraint_batch_kernel(\n"
+           "        const lvm::CompArgs *__restrict__ args, const uint32_t *__restrict__ prefix, uint32_t count) {\n"
+           "    const uint32_t b = blockIdx.x;\n"
+           "    uint32_t lo = 0, hi = count - 1;\n"
+           "    while (lo < hi) {\n"
+           "        const uint32_t mid = (lo + hi + 1) / 2;\n"
+           "        if (prefix[mid] <= b) lo = mid; else hi = mid - 1;\n"
+           "    }\n"
+           "    const lvm::CompArgs &a = args[lo];\n"
+           "    const uint64_t q = (uint64_t)(b - prefix[lo]) * 128 + threadIdx.x;\n"
+           "    if (q >> a.log_N) return;\n"
+           "    switch (lo) {\n";
+    for (size_t t = 0; t < pgs.size(); t++)
+        src += "    case " + std::to_string(t) + ": lvm_program_" + std::to_string(fn_of[t]) + "(a, (uint32_t)q); break;\n";
+    src += "    default: break;\n    }\n}\n";
     return src;
 }
 
@@ -284,6 +343,24 @@ struct JitEntry {
     std::vector<char> code;
     std::unordered_map<int, hipFunction_t> fn;
 };
+
+// The kernel `name` of the entry's module on the current device (its module
+// is loaded on first use and stays loaded). Called with g_jit_mu held.
+hipFunction_t function_of(JitEntry *e, const char *name) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    auto it = e->fn.find(dev);
+    if (it != e->fn.end()) return it->second;
+    hipModule_t mod = nullptr;
+    if (hipModuleLoadData(&mod, e->code.data()) != hipSuccess) return nullptr;
+    hipFunction_t fn = nullptr;
+    if (hipModuleGetFunction(&fn, mod, name) != hipSuccess) {
+        hipModuleUnload(mod);
+        return nullptr;
+    }
+    e->fn.emplace(dev, fn);
+    return fn;
+}
 
 std::mutex g_jit_mu;
 std::unordered_map<std::string, std::unique_ptr<JitEntry>> g_jit;  // keyed by generated source
@@ -373,27 +450,36 @@ bool jit_launch(const lvm_program *pg, const std::vector<uint32_t> &root_node, c
                 CompArgs &args, uint64_t blocks, hipStream_t st) {
     const std::string src = generate(pg, root_node, root_group, mode);
     if (src.empty()) return false;
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return false;
     hipFunction_t fn = nullptr;
     {
         std::lock_guard<std::mutex> lk(g_jit_mu);
         JitEntry *e = entry_for(src);
-        if (!e) return false;
-        auto it = e->fn.find(dev);
-        if (it != e->fn.end()) {
-            fn = it->second;
-        } else {
-            hipModule_t mod = nullptr;
-            if (hipModuleLoadData(&mod, e->code.data()) != hipSuccess) return false;
-            if (hipModuleGetFunction(&fn, mod, "lvm_constraint_kernel") != hipSuccess) {
-                hipModuleUnload(mod);
-                return false;
-            }
-            e->fn.emplace(dev, fn);  // the module stays loaded for the life of the process
-        }
+        if (!e || !(fn = function_of(e, "lvm_constraint_kernel"))) return false;
     }
     void *params[] = {&args};
+    return hipModuleLaunchKernel(fn, (unsigned)blocks, 1, 1, COMP_BLOCK, 1, 1, 0, st, params, nullptr) == hipSuccess;
+}
+
+bool jit_prepare_batch(const std::vector<const lvm_program *> &pgs, const std::vector<std::vector<uint32_t>> &root_nodes,
+                       const std::vector<const uint32_t *> &root_groups) {
+    const std::string src = generate_batch(pgs, root_nodes, root_groups);
+    if (src.empty()) return false;
+    std::lock_guard<std::mutex> lk(g_jit_mu);
+    return entry_for(src) != nullptr;
+}
+
+bool jit_launch_batch(const std::vector<const lvm_program *> &pgs, const std::vector<std::vector<uint32_t>> &root_nodes,
+                      const std::vector<const uint32_t *> &root_groups, const CompArgs *d_args,
+                      const uint32_t *d_prefix, uint32_t count, uint64_t blocks, hipStream_t st) {
+    const std::string src = generate_batch(pgs, root_nodes, root_groups);
+    if (src.empty()) return false;
+    hipFunction_t fn = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_jit_mu);
+        JitEntry *e = entry_for(src);
+        if (!e || !(fn = function_of(e, "lvm_constraint_batch_kernel"))) return false;
+    }
+    void *params[] = {&d_args, &d_prefix, &count};
     return hipModuleLaunchKernel(fn, (unsigned)blocks, 1, 1, COMP_BLOCK, 1, 1, 0, st, params, nullptr) == hipSuccess;
 }
 

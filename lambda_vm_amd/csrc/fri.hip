@@ -28,6 +28,19 @@ __device__ __forceinline__ void st3(uint64_t *p, fp3 v) {
     p[2] = v.c2;
 }
 
+// One fold of a codeword of 2*half Fp3 values into `out` (half values);
+// `leaves` (nullable) receives the half/2 layer leaves.
+struct FoldJob {
+    const uint64_t *in;
+    uint64_t *out;
+    uint64_t *leaves;
+    uint64_t half;
+    uint64_t z0, z1, z2;
+    uint64_t ginv;  // inverse of the codeword's coset offset
+    uint32_t log_half;
+    uint32_t pad;
+};
+
 __device__ __forceinline__ fp3 fold_one(const uint64_t *in, uint64_t j, fp3 zeta, uint64_t ginv, uint32_t log_half,
                                         const uint64_t *ilo, const uint64_t *ihi) {
     fp3 lo = ld3(in + 6 * j), hi = ld3(in + 6 * j + 3);
@@ -39,13 +52,16 @@ __device__ __forceinline__ fp3 fold_one(const uint64_t *in, uint64_t j, fp3 zeta
     return f3_add(sum, f3_mul_base(f3_mul(zeta, diff), tw));
 }
 
-// out has len/2 elements; leaves (nullable) receives len/4 digests
-__global__ __launch_bounds__(256) void fri_fold_kernel(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
-                                                       uint64_t half, uint32_t log_half, uint64_t z0, uint64_t z1,
-                                                       uint64_t z2, uint64_t ginv, const uint64_t *ilo,
-                                                       const uint64_t *ihi, uint64_t *__restrict__ leaves) {
-    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const fp3 zeta{z0, z1, z2};
+// Thread t of one fold: out has half elements; leaves (nullable) receives
+// half/2 digests.
+__device__ __forceinline__ void fri_fold_body(const FoldJob &j, uint64_t t, const uint64_t *ilo,
+                                              const uint64_t *ihi) {
+    const uint64_t *__restrict__ in = j.in;
+    uint64_t *__restrict__ out = j.out;
+    uint64_t *__restrict__ leaves = j.leaves;
+    const uint64_t half = j.half, ginv = j.ginv;
+    const uint32_t log_half = j.log_half;
+    const fp3 zeta{j.z0, j.z1, j.z2};
     if (leaves) {
         // thread t: folded elements 2t, 2t+1 and leaf t
         if (2 * t >= half) return;
@@ -73,6 +89,21 @@ __global__ __launch_bounds__(256) void fri_fold_kernel(const uint64_t *__restric
         if (t >= half) return;
         st3(out + 3 * t, fold_one(in, t, zeta, ginv, log_half, ilo, ihi));
     }
+}
+
+__global__ __launch_bounds__(256) void fri_fold_kernel(FoldJob j, const uint64_t *ilo, const uint64_t *ihi) {
+    fri_fold_body(j, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x, ilo, ihi);
+}
+
+// The same fold for several codewords in one launch: blocks [prefix[t],
+// prefix[t+1]) run jobs[t].
+__global__ __launch_bounds__(256) void fri_fold_batched_kernel(const FoldJob *__restrict__ jobs,
+                                                               const uint32_t *__restrict__ prefix, uint32_t count,
+                                                               const uint64_t *ilo, const uint64_t *ihi) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t t = find_slot(prefix, count, b);
+    const FoldJob j = jobs[t];
+    fri_fold_body(j, (uint64_t)(b - prefix[t]) * blockDim.x + threadIdx.x, ilo, ihi);
 }
 
 // Symmetric evaluations and authentication paths of one layer for every
@@ -189,9 +220,10 @@ static int fri_fold_impl(lvm_fri *f, const uint64_t zeta[3], bool commit, uint8_
     {
         ProfScope ps(commit ? "fri_fold_leaves" : "fri_fold", st);
         uint64_t threads = commit ? half / 2 : half;
-        hipLaunchKernelGGL(fri_fold_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, f->cur, out,
-                           half, log_half, zeta[0], zeta[1], zeta[2], ginv, d->tb.inv_lo, d->tb.inv_hi,
-                           commit ? (uint64_t *)(layer.nodes + (layer.L - 1) * 32) : (uint64_t *)nullptr);
+        const FoldJob j{f->cur, out, commit ? (uint64_t *)(layer.nodes + (layer.L - 1) * 32) : (uint64_t *)nullptr,
+                        half, zeta[0], zeta[1], zeta[2], ginv, log_half, 0};
+        hipLaunchKernelGGL(fri_fold_kernel, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0, st, j,
+                           d->tb.inv_lo, d->tb.inv_hi);
     }
     if (commit) {
         ProfScope ps("fri_merkle", st);
@@ -221,6 +253,152 @@ static int fri_fold_impl(lvm_fri *f, const uint64_t zeta[3], bool commit, uint8_
     }
     return LVM_OK;
 }
+
+}  // extern "C"
+
+namespace lvm {
+
+// One fold of every codeword fs[0..count) (zeta t at zetas[3t]) on the
+// calling thread's stream: one work-list launch for the folds (and their
+// leaves), one launch per tree step over all layer trees, one copy of all
+// roots. Each state then holds what fri_fold_impl would leave. On a device
+// error no state changes.
+int fri_fold_batch(lvm_fri *const *fs, size_t count, const uint64_t *zetas, bool commit, uint8_t *roots_out) {
+    if (!fs || !zetas || (commit && !roots_out)) return fail(LVM_ERR_ARG, "null argument");
+    if (count == 0) return LVM_OK;
+    Device *d = device_state();
+    if (!d) return LVM_ERR_DEVICE;
+    for (size_t t = 0; t < count; t++) {
+        if (!fs[t]) return fail(LVM_ERR_ARG, "null FRI state");
+        if (fs[t]->device != d->id) return fail(LVM_ERR_ARG, "FRI states of another device");
+        if (fs[t]->len < (commit ? 4u : 2u)) return fail(LVM_ERR_STATE, "codeword too short for this fold");
+    }
+    if (take_injected_fault()) return fail(LVM_ERR_INJECTED, "injected fault");
+    hipStream_t st = thread_stream(d);
+    std::vector<uint64_t *> outs(count, nullptr);
+    std::vector<FriLayer> layers(count);
+    uint8_t *arena = nullptr;
+    auto release = [&]() {
+        for (size_t t = 0; t < count; t++) {
+            lvm_free(outs[t]);
+            lvm_free(layers[t].nodes);
+        }
+        lvm_free(arena);
+    };
+    // descriptors: fold jobs + block prefix, then per tree step the steps + prefix, then the tree pointers
+    std::vector<FoldJob> jobs(count);
+    std::vector<uint32_t> prefix(1, 0);
+    uint64_t total = 0;
+    std::vector<std::vector<std::pair<TreeStep, uint32_t>>> tsteps(count);
+    size_t max_steps = 0;
+    for (size_t t = 0; t < count; t++) {
+        lvm_fri *f = fs[t];
+        const uint64_t half = f->len / 2;
+        if (lvm_malloc(&outs[t], half * 24) != hipSuccess) {
+            release();
+            return fail(LVM_ERR_NOMEM, "FRI layer allocation failed");
+        }
+        if (commit) {
+            layers[t].L = padded_leaves(half / 2);
+            if (lvm_malloc(&layers[t].nodes, (2 * layers[t].L - 1) * 32) != hipSuccess) {
+                release();
+                return fail(LVM_ERR_NOMEM, "FRI tree allocation failed");
+            }
+            tsteps[t] = tree_steps(layers[t].nodes, layers[t].L);
+            max_steps = std::max(max_steps, tsteps[t].size());
+        }
+        jobs[t] = FoldJob{f->cur, outs[t],
+                          commit ? (uint64_t *)(layers[t].nodes + (layers[t].L - 1) * 32) : (uint64_t *)nullptr,
+                          half, zetas[3 * t], zetas[3 * t + 1], zetas[3 * t + 2], inv(f->offset), log2u(half), 0};
+        total += ((commit ? half / 2 : half) + 255) / 256;
+        if (total >= (1ull << 31)) {
+            release();
+            return fail(LVM_ERR_ARG, "batched fold exceeds 2^31 blocks");
+        }
+        prefix.push_back((uint32_t)total);
+    }
+    std::vector<uint8_t> blob;
+    auto put = [&](const void *p, size_t bytes) {
+        const size_t off = (blob.size() + 15) & ~(size_t)15;
+        blob.resize(off + bytes);
+        std::memcpy(blob.data() + off, p, bytes);
+        return off;
+    };
+    const size_t o_jobs = put(jobs.data(), count * sizeof(FoldJob));
+    const size_t o_prefix = put(prefix.data(), prefix.size() * 4);
+    struct StepLaunch {
+        size_t steps_off, prefix_off;
+        uint32_t count, blocks;
+    };
+    std::vector<StepLaunch> steps_l;
+    for (size_t s = 0; s < max_steps; s++) {
+        std::vector<TreeStep> steps;
+        std::vector<uint32_t> sp(1, 0);
+        uint64_t sb = 0;
+        for (size_t t = 0; t < count; t++) {
+            if (s >= tsteps[t].size()) continue;
+            steps.push_back(tsteps[t][s].first);
+            sb += tsteps[t][s].second;
+            sp.push_back((uint32_t)sb);
+        }
+        steps_l.push_back({put(steps.data(), steps.size() * sizeof(TreeStep)), put(sp.data(), sp.size() * 4),
+                           (uint32_t)steps.size(), (uint32_t)sb});
+    }
+    std::vector<uint64_t *> trees(count);
+    for (size_t t = 0; t < count; t++) trees[t] = (uint64_t *)layers[t].nodes;
+    const size_t o_trees = put(trees.data(), count * sizeof(uint64_t *));
+    const size_t o_roots = (blob.size() + 15) & ~(size_t)15;  // the gathered roots
+    blob.resize(o_roots + count * 32);
+    if (lvm_malloc(&arena, blob.size()) != hipSuccess) {
+        release();
+        return fail(LVM_ERR_NOMEM, "FRI descriptor allocation failed");
+    }
+    StageScope ss(st);
+    hipError_t e = stage_upload(arena, blob.data(), blob.size(), st);
+    if (e == hipSuccess) {
+        ProfScope ps(commit ? "fri_fold_leaves" : "fri_fold", st);
+        hipLaunchKernelGGL(fri_fold_batched_kernel, dim3((unsigned)total), dim3(256), 0, st,
+                           (const FoldJob *)(arena + o_jobs), (const uint32_t *)(arena + o_prefix), (uint32_t)count,
+                           d->tb.inv_lo, d->tb.inv_hi);
+    }
+    if (e == hipSuccess && commit) {
+        ProfScope ps("fri_merkle", st);
+        for (const StepLaunch &l : steps_l)
+            launch_tree_steps_batched((const TreeStep *)(arena + l.steps_off), (const uint32_t *)(arena + l.prefix_off),
+                                      l.count, l.blocks, st);
+        launch_gather_roots((uint64_t *const *)(arena + o_trees), (uint32_t)count, (uint64_t *)(arena + o_roots), st);
+    }
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess && commit) e = stage_download(roots_out, arena + o_roots, count * 32, st);
+    if (e == hipSuccess) e = stage_sync(st);
+    if (e != hipSuccess) {
+        release();
+        return fail(LVM_ERR_DEVICE, std::string("batched FRI fold: ") + hipGetErrorString(e));
+    }
+    lvm_free(arena);
+    for (size_t t = 0; t < count; t++) {
+        lvm_fri *f = fs[t];
+        if (f->cur_owned) lvm_free(f->cur);
+        f->cur = outs[t];
+        f->len /= 2;
+        f->offset = sqr(f->offset);
+        if (commit) {
+            FriLayer &l = layers[t];
+            std::memcpy(l.root, roots_out + 32 * t, 32);
+            l.evals = outs[t];
+            l.len = f->len;
+            f->cur_owned = false;
+            f->layers.push_back(l);
+        } else {
+            f->cur_owned = true;
+        }
+    }
+    return LVM_OK;
+}
+
+}  // namespace lvm
+
+extern "C" {
 
 int lvm_fri_fold_and_commit_layer(lvm_fri *f, const uint64_t zeta[3], uint8_t root_out[32]) {
     return fri_fold_impl(f, zeta, true, root_out);

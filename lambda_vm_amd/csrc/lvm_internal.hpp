@@ -196,6 +196,11 @@ void launch_leaves_batched(const LeafJob *d_jobs, const uint32_t *d_prefix, uint
 void launch_tree_steps_batched(const TreeStep *d_steps, const uint32_t *d_prefix, uint32_t count, uint32_t blocks,
                                hipStream_t st);
 void launch_gather_roots(uint64_t *const *d_trees, uint32_t count, uint64_t *d_roots, hipStream_t st);
+// One FRI fold of each of count codewords in shared launches (fri.hip);
+// zetas: count x 3; roots_out (count x 32) when committing.
+int fri_fold_batch(lvm_fri *const *fs, size_t count, const uint64_t *zetas, bool commit, uint8_t *roots_out);
+// the launches of one tree of L leaves (a power of two) at `nodes`: (step, blocks)
+std::vector<std::pair<TreeStep, uint32_t>> tree_steps(uint8_t *nodes, uint64_t L);
 
 // Constraint programs (compose.hip interpreter, compose_jit.cpp compiled form).
 struct CompArgs;
@@ -219,4 +224,12 @@ bool jit_prepare(const lvm_program *pg, const std::vector<uint32_t> &root_node, 
 // Launches the compiled kernel on the current device; false on failure.
 bool jit_launch(const lvm_program *pg, const std::vector<uint32_t> &root_node, const uint32_t *root_group, int mode,
                 CompArgs &args, uint64_t blocks, hipStream_t st);
+// The composition (mode 0) of several tables' programs as one kernel: compile
+// (cached per program set), then launch with table t's arguments at d_args[t]
+// and its blocks [d_prefix[t], d_prefix[t+1]) (128 natural rows each).
+bool jit_prepare_batch(const std::vector<const lvm_program *> &pgs, const std::vector<std::vector<uint32_t>> &root_nodes,
+                       const std::vector<const uint32_t *> &root_groups);
+bool jit_launch_batch(const std::vector<const lvm_program *> &pgs, const std::vector<std::vector<uint32_t>> &root_nodes,
+                      const std::vector<const uint32_t *> &root_groups, const CompArgs *d_args,
+                      const uint32_t *d_prefix, uint32_t count, uint64_t blocks, hipStream_t st);
 }  // namespace lvm

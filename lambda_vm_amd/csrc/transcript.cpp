@@ -349,4 +349,85 @@ int lvm_fri_commit_phase(lvm_fri *f, lvm_transcript *t, unsigned blowup_log, uns
     return LVM_OK;
 }
 
+int lvm_fri_commit_phase_batch(lvm_fri *const *fs, lvm_transcript *const *ts, size_t count,
+                               const unsigned *blowup_log, unsigned k, uint8_t *const *roots_out,
+                               size_t *num_committed_out, uint64_t *const *coeffs_out, size_t *num_coeffs_out) {
+    if (count && (!fs || !ts || !blowup_log)) return fail(LVM_ERR_ARG, "null argument");
+    struct Plan {
+        size_t committed;
+        unsigned folds, eff_k;
+    };
+    std::vector<Plan> plan(count);
+    size_t max_committed = 0;
+    for (size_t t = 0; t < count; t++) {
+        if (!fs[t] || !ts[t]) return fail(LVM_ERR_ARG, "null FRI state or transcript");
+        size_t len = 0;
+        uint64_t offset = 0;
+        int rc = lvm_fri_current(fs[t], nullptr, &len, &offset);
+        if (rc) return rc;
+        const unsigned lde_log = (unsigned)__builtin_ctzll(len);
+        // FriFoldLayout (crypto/stark/src/fri/terminal.rs:45-55)
+        const unsigned terminal_log = blowup_log[t] + k < lde_log ? blowup_log[t] + k : lde_log;
+        if (terminal_log < blowup_log[t]) return fail(LVM_ERR_ARG, "codeword shorter than the blowup");
+        plan[t].folds = lde_log - terminal_log;
+        plan[t].committed = plan[t].folds ? plan[t].folds - 1 : 0;
+        plan[t].eff_k = terminal_log - blowup_log[t];
+        max_committed = plan[t].committed > max_committed ? plan[t].committed : max_committed;
+    }
+    // every transcript is restored if the device fails mid-loop (gpu_lde.rs:3196-3229)
+    std::vector<lvm_transcript> snapshot;
+    snapshot.reserve(count);
+    for (size_t t = 0; t < count; t++) snapshot.push_back(*ts[t]);
+    auto restore = [&](int rc) {
+        for (size_t t = 0; t < count; t++) *ts[t] = snapshot[t];
+        return rc;
+    };
+    std::vector<lvm_fri *> act;
+    std::vector<size_t> idx;
+    std::vector<uint64_t> zetas;
+    std::vector<uint8_t> roots;
+    for (size_t l = 0; l <= max_committed; l++) {
+        // depth l < committed: a committed fold; l == committed: the final fold
+        for (int final_fold = 0; final_fold < 2; final_fold++) {
+            act.clear();
+            idx.clear();
+            zetas.clear();
+            for (size_t t = 0; t < count; t++) {
+                const bool take = final_fold ? (plan[t].folds > 0 && l == plan[t].committed) : l < plan[t].committed;
+                if (!take) continue;
+                uint64_t z[3];
+                lvm_transcript_sample_fp3(ts[t], z);
+                zetas.insert(zetas.end(), z, z + 3);
+                act.push_back(fs[t]);
+                idx.push_back(t);
+            }
+            if (act.empty()) continue;
+            roots.assign(act.size() * 32, 0);
+            int rc = fri_fold_batch(act.data(), act.size(), zetas.data(), !final_fold, roots.data());
+            if (rc) return restore(rc);
+            if (final_fold) continue;
+            for (size_t i = 0; i < act.size(); i++) {
+                const size_t t = idx[i];
+                if (roots_out && roots_out[t]) std::memcpy(roots_out[t] + 32 * l, roots.data() + 32 * i, 32);
+                lvm_transcript_append_bytes(ts[t], roots.data() + 32 * i, 32);
+            }
+        }
+    }
+    for (size_t t = 0; t < count; t++) {
+        size_t len = 0;
+        uint64_t offset = 0;
+        int rc = lvm_fri_current(fs[t], nullptr, &len, &offset);
+        std::vector<F3> cw(len);
+        if (!rc) rc = lvm_fri_current(fs[t], (uint64_t *)cw.data(), &len, &offset);
+        if (rc) return restore(rc);
+        std::vector<F3> coeffs;
+        terminal_coeffs(cw, offset, plan[t].eff_k, coeffs);
+        for (auto &c : coeffs) lvm_transcript_append_fp3(ts[t], c.c);
+        if (coeffs_out && coeffs_out[t]) std::memcpy(coeffs_out[t], coeffs.data(), coeffs.size() * 24);
+        if (num_committed_out) num_committed_out[t] = plan[t].committed;
+        if (num_coeffs_out) num_coeffs_out[t] = coeffs.size();
+    }
+    return LVM_OK;
+}
+
 }  // extern "C"

@@ -15,7 +15,7 @@ bytes.
 
 Field order of one proof:
   trace_length; lde_trace_main_merkle_root; lde_trace_aux_merkle_root
-  (Option); lde_trace_precomputed_merkle_root (Option, always none here);
+  (Option); lde_trace_precomputed_merkle_root (Option: preprocessed tables);
   trace_ood_evaluations (Table: 1 row x W); trace_ood_next_evaluations
   (Table: K-1 rows x C, the C columns the AIR opens at the next rows,
   crypto/stark/src/ood.rs:94-126);
@@ -24,7 +24,8 @@ Field order of one proof:
   (Vec of FriDecommitment: layers_auth_paths (Vec of Vec<Commitment>),
   layers_evaluations_sym (Vec<E>), fri/fri_decommit.rs:17-20);
   deep_poly_openings (Vec of DeepPolynomialOpening: composition_poly,
-  main_trace_polys, precomputed_trace_polys (Option, none), aux_trace_polys
+  main_trace_polys (the multiplicity columns of a preprocessed table),
+  precomputed_trace_polys (Option: its fixed columns), aux_trace_polys
   (Option); each PolynomialOpenings = proof (Vec<Commitment>), evaluations,
   evaluations_sym, proof/stark.rs:46-69); nonce (Option<u64>);
   bus_public_inputs (Option<E>: the table contribution L); the query indices
@@ -167,7 +168,9 @@ def serialize_table_proof(p: AirTableProof) -> bytes:
     w.u8(p.aux_root is not None)
     if p.aux_root is not None:
         w.raw(p.aux_root)
-    w.u8(0)  # precomputed root: none
+    w.u8(p.precomputed_root is not None)
+    if p.precomputed_root is not None:
+        w.raw(p.precomputed_root)
     b0 = np.asarray(p.ood_block0, dtype=np.uint64).reshape(-1, 3)
     W = b0.shape[0]
     w.u64(W)
@@ -202,7 +205,9 @@ def serialize_table_proof(p: AirTableProof) -> bytes:
     for q in range(Q):
         _write_opening(w, p.parts_paths[q], p.parts_rows[q], ext=True)
         _write_opening(w, p.main_paths[q], p.main_rows[q], ext=False)
-        w.u8(0)  # precomputed trace openings: none
+        w.u8(p.precomputed_rows is not None)
+        if p.precomputed_rows is not None:
+            _write_opening(w, p.precomputed_paths[q], p.precomputed_rows[q], ext=False)
         w.u8(p.aux_rows is not None)
         if p.aux_rows is not None:
             _write_opening(w, p.aux_paths[q], p.aux_rows[q], ext=True)
@@ -221,8 +226,7 @@ def deserialize_table_proof(data: bytes, reader: _R | None = None) -> AirTablePr
     n = r.u64()
     main_root = r.raw(32)
     aux_root = r.raw(32) if r.u8() else None
-    if r.u8():
-        raise ValueError("precomputed commitments are not produced by this prover")
+    pre_root = r.raw(32) if r.u8() else None
     W = r.count(24, "current-row block width")
     if r.u64() != 1:
         raise ValueError("current-row block must have one row")
@@ -256,12 +260,17 @@ def deserialize_table_proof(data: bytes, reader: _R | None = None) -> AirTablePr
     if r.u64() != Q:
         raise ValueError("opening count")
     parts_paths, parts_rows, main_paths, main_rows, aux_paths, aux_rows = [], [], [], [], [], []
+    pre_paths, pre_rows = [], []
     has_aux = None
     for q in range(Q):
         pp, pr = _read_opening(r, ext=True)
         mp, mr = _read_opening(r, ext=False)
-        if r.u8():
-            raise ValueError("precomputed openings are not produced by this prover")
+        if bool(r.u8()) != (pre_root is not None):
+            raise ValueError("precomputed openings without a precomputed commitment, or the reverse")
+        if pre_root is not None:
+            pp_, pr_ = _read_opening(r, ext=False)
+            pre_paths.append(pp_)
+            pre_rows.append(pr_)
         a = bool(r.u8())
         has_aux = a if has_aux is None else has_aux
         if a != has_aux:
@@ -287,7 +296,9 @@ def deserialize_table_proof(data: bytes, reader: _R | None = None) -> AirTablePr
         main_paths=np.stack(main_paths) if Q else main_paths,
         aux_rows=np.stack(aux_rows) if has_aux else None, aux_paths=np.stack(aux_paths) if has_aux else None,
         parts_rows=np.stack(parts_rows) if Q else np.zeros((0, 2, 0), np.uint64),
-        parts_paths=np.stack(parts_paths) if Q else parts_paths)
+        parts_paths=np.stack(parts_paths) if Q else parts_paths, precomputed_root=pre_root,
+        precomputed_rows=(np.stack(pre_rows) if Q else np.zeros((0, 2, 0), np.uint64)) if pre_root else None,
+        precomputed_paths=(np.stack(pre_paths) if Q else pre_paths) if pre_root else None)
 
 
 def serialize_multi_proof(proofs) -> bytes:

@@ -55,6 +55,9 @@ class ProofOptions:
     num_queries: int = 16
     grinding_factor: int = 0
     vram_budget: int | None = None  # device bytes for tables in flight (None: no limit)
+    # FRI commit phase of all tables in shared launches (lvm_fri_commit_phase_batch)
+    # when the budget admits every table at once; else each table folds on its own thread
+    batch_fri: bool = True
 
 
 @dataclass
@@ -82,6 +85,12 @@ class AirTableProof:
     parts_paths: np.ndarray
     transcript_state: bytes = b""
     challenges: dict = field(default_factory=dict)
+    # preprocessed tables (StarkProof::lde_trace_precomputed_merkle_root and
+    # DeepPolynomialOpening::precomputed_trace_polys, proof/stark.rs:65-90):
+    # main_root / main_rows / main_paths then cover the multiplicity columns only
+    precomputed_root: bytes | None = None
+    precomputed_rows: np.ndarray | None = None   # (Q, 2, k)
+    precomputed_paths: np.ndarray | None = None
 
 
 def frame_offsets(air: AirProgram) -> int:
@@ -130,14 +139,28 @@ def prove_air_tables(tables, opts: ProofOptions = ProofOptions(), label: bytes =
     threads, prover.rs:3590-3600); workers=1 proves them one after another."""
     t = B.Transcript(label)
     Bf, g = opts.blowup, opts.coset_offset
-    # ---- R1: main commitments of every table, one batched plan ----
+    # ---- R1: main commitments of every table, one batched plan; a
+    # preprocessed table commits its fixed and its multiplicity columns to two
+    # trees over one LDE, and its fixed-columns root goes first (prover.rs:3457-3466) ----
     weights = [B.coset_weights_dev(n, g) for _, _, n in tables]
-    plan = B.BatchPlan([(tr, w, n, air.num_main, Bf) for (air, tr, n), w in zip(tables, weights)], use_graph=False)
-    roots = plan.run()
-    mains = [plan.take(i) for i in range(len(tables))]
-    plan.close()
-    for r in roots:
-        t.append_bytes(r)
+    plain = [i for i, (air, _, _) in enumerate(tables) if not air.num_precomputed]
+    mains, pres = [None] * len(tables), [None] * len(tables)
+    if plain:
+        plan = B.BatchPlan([(tables[i][1], weights[i], tables[i][2], tables[i][0].num_main, Bf) for i in plain],
+                           use_graph=False)
+        plan.run()
+        for k, i in enumerate(plain):
+            mains[i] = plan.take(k)
+        plan.close()
+    for i, (air, tr, n) in enumerate(tables):
+        if air.num_precomputed:
+            h, pre_root, pre_nodes = B.coset_lde_split_trees_dev(tr, n, air.num_main, Bf, weights[i],
+                                                                 air.num_precomputed)
+            mains[i], pres[i] = h, (pre_root, pre_nodes, air.num_precomputed)
+    for i in range(len(tables)):
+        if pres[i] is not None:
+            t.append_bytes(pres[i][0])
+        t.append_bytes(mains[i].root)
     needs_lookup = any(air.interactions for air, _, _ in tables)
     z_logup = alpha = None
     if needs_lookup:
@@ -170,33 +193,97 @@ def prove_air_tables(tables, opts: ProofOptions = ProofOptions(), label: bytes =
         for _, aux_dev, _, _, _ in built:
             aux_dev.free()
 
-    def one(idx):
-        air, tr, n = tables[idx]
-        return _prove_table(air, tr, n, mains[idx], weights[idx], forks[idx], opts, z_logup, alpha, auxs[idx])
-
     # rounds 2-4 per table: driver threads take tables heaviest first, each
     # admitted through the device-byte gate (prover.rs:3590-3600 with
     # run_admitted / VramGate / heaviest_first, prover.rs:677-789)
     from . import _native
     from .scheduler import VramGate, estimate_table_vram_bytes, heaviest_first, run_admitted
     est = [estimate_table_vram_bytes(air.num_main, air.num_aux, n * Bf) for air, _, n in tables]
-    gate = VramGate(opts.vram_budget if opts.vram_budget is not None else 1 << 62)
+    budget = opts.vram_budget if opts.vram_budget is not None else 1 << 62
+    gate = VramGate(budget)
     w = len(tables) if workers is None else workers
-    proofs = run_admitted(heaviest_first(est), est, gate, w, one,
-                          initializer=lambda: _native.ensure_device_this_thread(0))
+    init = lambda: _native.ensure_device_this_thread(0)  # noqa: E731
+    # all tables' FRI layers fold together when every table can be resident at once
+    batched = opts.batch_fri and len(tables) > 1 and sum(est) <= budget
+
+    def upto_deep(idx):
+        air, tr, n = tables[idx]
+        return _rounds_to_deep(air, tr, n, mains[idx], weights[idx], forks[idx], opts, z_logup, alpha, auxs[idx],
+                               pres[idx])
+
+    if batched:
+        # R2: every table's composition in shared launches, then per table its
+        # parts commitment, R3 and DEEP on the driver threads
+        inputs = [_composition_inputs(air, tr, n, mains[i], weights[i], forks[i], opts, z_logup, alpha, auxs[i])
+                  for i, (air, tr, n) in enumerate(tables)]
+        Hs = B.eval_composition_batch([c for c, _ in inputs])
+
+        def after(idx):
+            air, _, n = tables[idx]
+            return _rounds_after_composition(air, n, mains[idx], forks[idx], opts, inputs[idx][1], Hs[idx], pres[idx])
+
+        states = run_admitted(heaviest_first(est), est, gate, w, after, initializer=init)
+        fris = [B.GpuFri(None, g, on_device=st.deep, length=st.n * Bf, adopt=True) for st in states]
+        fri_out = B.fri_commit_phase_batch(fris, [st.t for st in states], [Bf.bit_length() - 1] * len(states),
+                                           opts.fri_final_log_degree)
+        proofs = [_finish_table(st, opts, fri, roots, coeffs) for st, fri, (roots, coeffs) in zip(states, fris, fri_out)]
+    else:
+        def one(idx):
+            st = upto_deep(idx)
+            fri = B.GpuFri(None, g, on_device=st.deep, length=st.n * Bf, adopt=True)
+            roots, coeffs = fri.commit_phase(st.t, Bf.bit_length() - 1, opts.fri_final_log_degree)
+            return _finish_table(st, opts, fri, roots, coeffs)
+        proofs = run_admitted(heaviest_first(est), est, gate, w, one, initializer=init)
     for h, wt in zip(mains, weights):
         h.close()
         wt.free()
     return proofs
 
 
+@dataclass
+class _TableState:
+    """One table between its DEEP codeword and the FRI commit phase."""
+    n: int
+    t: B.Transcript
+    main: B.GpuLde
+    aux: B.GpuLde | None
+    parts: B.GpuLde
+    deep: B.DeviceBuffer
+    pre: tuple | None  # (fixed-columns root, its host node array, k) of a preprocessed table
+    L: tuple | None
+    ood: np.ndarray
+    block1: np.ndarray
+    parts_ood: np.ndarray
+    challenges: dict
+
+
 def _prove_table(air: AirProgram, tr: B.DeviceBuffer, n: int, main: B.GpuLde, weights: B.DeviceBuffer,
-                 t: B.Transcript, opts: ProofOptions, z_logup, alpha, committed_aux=None) -> AirTableProof:
-    """Rounds of one table after R1. committed_aux: (aux LDE handle, LogUp
-    descriptor, alpha powers, L) when the table's aux trace was committed with
-    the other tables' (prove_air_tables), else it is built and committed here."""
+                 t: B.Transcript, opts: ProofOptions, z_logup, alpha, committed_aux=None, pre=None) -> AirTableProof:
+    """Rounds of one table after R1, its FRI commit phase on its own."""
+    st = _rounds_to_deep(air, tr, n, main, weights, t, opts, z_logup, alpha, committed_aux, pre)
+    fri = B.GpuFri(None, opts.coset_offset, on_device=st.deep, length=n * opts.blowup, adopt=True)
+    roots, coeffs = fri.commit_phase(st.t, opts.blowup.bit_length() - 1, opts.fri_final_log_degree)
+    return _finish_table(st, opts, fri, roots, coeffs)
+
+
+def _rounds_to_deep(air: AirProgram, tr: B.DeviceBuffer, n: int, main: B.GpuLde, weights: B.DeviceBuffer,
+                    t: B.Transcript, opts: ProofOptions, z_logup, alpha, committed_aux=None, pre=None) -> _TableState:
+    """Rounds of one table after R1 up to its DEEP codeword, its composition
+    evaluated on its own."""
+    comp, ctx = _composition_inputs(air, tr, n, main, weights, t, opts, z_logup, alpha, committed_aux)
+    H = B.eval_composition(**comp)
+    return _rounds_after_composition(air, n, main, t, opts, ctx, H, pre)
+
+
+def _composition_inputs(air: AirProgram, tr: B.DeviceBuffer, n: int, main: B.GpuLde, weights: B.DeviceBuffer,
+                        t: B.Transcript, opts: ProofOptions, z_logup, alpha, committed_aux=None):
+    """R1's per-table part (aux trace, its root and contribution absorbed) and
+    R2's challenge: the arguments of the table's composition (a dict for
+    eval_composition / eval_composition_batch) and what the later rounds need.
+    committed_aux: (aux LDE handle, LogUp descriptor, alpha powers, L) when the
+    table's aux trace was committed with the other tables' (prove_air_tables),
+    else it is built and committed here."""
     Bf, g = opts.blowup, opts.coset_offset
-    N = n * Bf
     m = air.num_main
     rap = ap = toff = None
     aux = None
@@ -223,8 +310,20 @@ def _prove_table(air: AirProgram, tr: B.DeviceBuffer, n: int, main: B.GpuLde, we
     coeffs = B.fp3_powers(beta, T + nb)
     groups, ex = air.groups()
     bnd = [(b.col, b.is_aux, b.step, b.value) for b in air.boundary]
+    comp = dict(dev=air.device, main=main, aux=aux, next_step=Bf, trace_rows=n, coset_offset=g, betas=coeffs[:T],
+                root_group=groups, group_exemptions=ex, boundary=bnd, b_betas=coeffs[T:], rap=rap, alpha=ap,
+                table_offset=toff)
+    return comp, dict(aux=aux, L=L, beta=beta, z_logup=z_logup, alpha=alpha)
+
+
+def _rounds_after_composition(air: AirProgram, n: int, main: B.GpuLde, t: B.Transcript, opts: ProofOptions, ctx,
+                              H: B.DeviceBuffer, pre=None) -> _TableState:
+    """R2's commitment of the composition parts, R3 and R4 up to the DEEP
+    codeword."""
+    Bf, g = opts.blowup, opts.coset_offset
+    N = n * Bf
+    aux, L, beta, z_logup, alpha = ctx["aux"], ctx["L"], ctx["beta"], ctx["z_logup"], ctx["alpha"]
     Pn = num_parts(air)
-    H = B.eval_composition(air.device, main, aux, Bf, n, g, coeffs[:T], groups, ex, bnd, coeffs[T:], rap, ap, toff)
     parts = B.composition_commit_parts(H, N, n, Pn, g)
     H.free()
     t.append_bytes(parts.root)
@@ -264,8 +363,16 @@ def _prove_table(air: AirProgram, tr: B.DeviceBuffer, n: int, main: B.GpuLde, we
             trace_ood[j * K + k] = ood[k, j]
     deep = B.deep_composition(main, g, pts, trace_ood, gammas_tr, aux=aux, parts=parts, z_parts=zP,
                               parts_ood=parts_ood, gammas_h=gp[S:])
-    fri = B.GpuFri(None, g, on_device=deep, length=N, adopt=True)
-    fri_roots, fri_coeffs = fri.commit_phase(t, Bf.bit_length() - 1, opts.fri_final_log_degree)
+    return _TableState(n=n, t=t, main=main, aux=aux, parts=parts, deep=deep, pre=pre, L=L, ood=ood, block1=block1,
+                       parts_ood=parts_ood,
+                       challenges=dict(z_logup=z_logup, alpha=alpha, beta=beta, z=z, gamma=gamma))
+
+
+def _finish_table(st: _TableState, opts: ProofOptions, fri, fri_roots, fri_coeffs) -> AirTableProof:
+    """After the FRI commit phase: grinding, queries, FRI decommitment and
+    the row-pair openings of the main, aux and parts trees."""
+    t, n, main, aux, parts = st.t, st.n, st.main, st.aux, st.parts
+    N = n * opts.blowup
     nonce = None
     if opts.grinding_factor > 0:
         nonce = B.generate_nonce(t.state(), opts.grinding_factor)
@@ -274,6 +381,12 @@ def _prove_table(air: AirProgram, tr: B.DeviceBuffer, n: int, main: B.GpuLde, we
     fri_sym, fri_paths = fri.query(iotas)
     fri.close()
     main_paths, main_rows = main.open(iotas)
+    pre_root = pre_rows = pre_paths = None
+    if st.pre is not None:
+        # whole rows come back: the fixed columns open against their own tree
+        pre_root, pre_nodes, k = st.pre
+        pre_rows, main_rows = main_rows[:, :, :k], main_rows[:, :, k:]
+        pre_paths = B.merkle_paths_from_nodes(pre_nodes, iotas)
     aux_paths = aux_rows = None
     if aux is not None:
         aux_paths, aux_rows = aux.open(iotas)
@@ -283,13 +396,13 @@ def _prove_table(air: AirProgram, tr: B.DeviceBuffer, n: int, main: B.GpuLde, we
     canon = lambda a: None if a is None else np.where(a >= np.uint64(P), a - np.uint64(P), a)  # noqa: E731
     return AirTableProof(
         trace_rows=n, main_root=main.root, aux_root=None if aux is None else bytes(aux.root),
-        table_contribution=L, composition_root=parts.root,
-        ood_block0=canon(ood[0].copy()), ood_block1=canon(block1), parts_ood=canon(parts_ood),
+        table_contribution=st.L, composition_root=parts.root,
+        ood_block0=canon(st.ood[0].copy()), ood_block1=canon(st.block1), parts_ood=canon(st.parts_ood),
         fri_roots=fri_roots, fri_coeffs=canon(fri_coeffs), nonce=nonce, iotas=iotas, fri_sym=canon(fri_sym),
         fri_paths=fri_paths, main_rows=canon(main_rows), main_paths=main_paths, aux_rows=canon(aux_rows),
         aux_paths=aux_paths, parts_rows=canon(parts_rows), parts_paths=parts_paths,
-        transcript_state=t.state(),
-        challenges=dict(z_logup=z_logup, alpha=alpha, beta=beta, z=z, gamma=gamma))
+        transcript_state=t.state(), challenges=st.challenges, precomputed_root=pre_root,
+        precomputed_rows=canon(pre_rows), precomputed_paths=pre_paths)
 
 
 def air_descriptor(air: AirProgram):

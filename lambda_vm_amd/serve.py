@@ -302,12 +302,15 @@ class ProvingService:
 
     def run_async(self, arrivals: list, time_scale: float = 1.0) -> dict:
         """Work-conserving form of run(): no tick barrier. The executor's worker
-        pool (`ex.pool`, `ex.workers` threads) receives the commit of each
-        admitted batch as soon as it is admitted and a table's remaining rounds
-        as soon as a worker is free, oldest job's tables first; at most
-        workers - 1 table tasks are in flight so that a commit never queues
-        behind them. A job's tables start when its commit has finished (they
-        fork its transcript). Same admission rule and accounting as run()."""
+        pool (`ex.pool`, `ex.workers` threads) receives a commit as soon as no
+        other commit is in flight: every job admitted while one runs waits for
+        it and then commits with the others admitted meanwhile in one batched
+        plan (continuous batching: a lone job commits at once, under load the
+        commits grow). A table's remaining rounds go to the pool as soon as a
+        worker is free, oldest job's tables first; at most workers - 1 table
+        tasks are in flight so that a commit never queues behind them. A job's
+        tables start when its commit has finished (they fork its transcript).
+        Same admission rule and accounting as run()."""
         pool, workers = getattr(self.ex, "pool", None), getattr(self.ex, "workers", 1)
         if pool is None:
             raise ValueError("run_async needs an executor with a worker pool")
@@ -318,16 +321,19 @@ class ProvingService:
         inflight = {}
         tables_in_flight = 0
         table_slots = max(1, workers - 1)
+        to_commit: list = []
+        commit_busy = False
         while len(self.done) < total:
             now = self.clock() - t0
             while pending and pending[0][0] * time_scale <= now:
                 t, spec = pending.popleft()
                 self.submit(spec, t0 + t * time_scale)
-            admitted = self._admit(self.clock())
-            if admitted:
+            to_commit.extend(self._admit(self.clock()))
+            if to_commit and not commit_busy:
                 self.ticks += 1
-                self.batch_sizes.append(len(admitted))
-                inflight[pool.submit(self.ex.commit, admitted)] = ("commit", admitted)
+                self.batch_sizes.append(len(to_commit))
+                inflight[pool.submit(self.ex.commit, to_commit)] = ("commit", to_commit)
+                to_commit, commit_busy = [], True
             while ready and tables_in_flight < table_slots:
                 js, ti = ready.popleft()
                 inflight[pool.submit(self.ex.prove_table, js, ti)] = ("table", (js, ti))
@@ -344,6 +350,7 @@ class ProvingService:
                 kind, payload = inflight.pop(f)
                 res = f.result()
                 if kind == "commit":
+                    commit_busy = False
                     for js in payload:
                         self.running.append(js)
                         ready.extend((js, ti) for ti in range(len(js.spec.tables)))

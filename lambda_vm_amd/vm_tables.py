@@ -781,3 +781,171 @@ def load_table_air() -> AirProgram:
     return AirProgram("load", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
                       boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
                       max_degree=max(3, 1 + absorbed))
+
+
+# ---------------------------------------------------------------------------
+# Preprocessed tables: the leading columns are fixed per table (a function of
+# the row index only) and commit to a tree of their own; the multiplicity
+# columns after them commit to the main tree (prover.rs:1194-1290).
+
+BUS_HWSL, BUS_KECCAK_RC = 9, 23
+
+# Keccak-f[1600] round constants (executor/src/vm/instruction/execution.rs:1001, FIPS 202)
+KECCAK_ROUND_CONSTANTS = (
+    0x0000000000000001, 0x0000000000008082, 0x800000000000808A, 0x8000000080008000,
+    0x000000000000808B, 0x0000000080000001, 0x8000000080008081, 0x8000000000008009,
+    0x000000000000008A, 0x0000000000000088, 0x0000000080008009, 0x000000008000000A,
+    0x000000008000808B, 0x800000000000008B, 0x8000000000008089, 0x8000000000008003,
+    0x8000000000008002, 0x8000000000000080, 0x000000000000800A, 0x800000008000000A,
+    0x8000000080008081, 0x8000000000008080, 0x0000000080000001, 0x8000000080008008)
+
+
+class KeccakRcCols:
+    """prover/src/tables/keccak_rc.rs:27-35: the round index, the 8 bytes of its
+    round constant (little-endian), the multiplicity."""
+    ROUND = 0
+    RC = tuple(range(1, 9))
+    MU = 9
+    NUM_COLUMNS = 10
+    NUM_PRECOMPUTED = 9
+    NUM_ROWS = 32  # 24 rounds padded to a power of two (keccak_rc.rs:41-45)
+
+
+def keccak_rc_precomputed() -> np.ndarray:
+    """The (32, 9) precomputed columns: [round, rc byte 0..7] per row, zero
+    constants past round 23 (keccak_rc.rs:52-67)."""
+    c = KeccakRcCols
+    t = np.zeros((c.NUM_ROWS, c.NUM_PRECOMPUTED), dtype=np.uint64)
+    for r in range(c.NUM_ROWS):
+        rc = KECCAK_ROUND_CONSTANTS[r] if r < 24 else 0
+        t[r, 0] = r
+        for i in range(8):
+            t[r, 1 + i] = (rc >> (8 * i)) & 0xFF
+    return t
+
+
+def generate_keccak_rc_trace(num_keccak_ops: int) -> np.ndarray:
+    """The precomputed columns and MU = number of permutations on each real
+    round row (keccak_rc.rs:191-227)."""
+    c = KeccakRcCols
+    t = np.zeros((c.NUM_ROWS, c.NUM_COLUMNS), dtype=np.uint64)
+    t[:, :c.NUM_PRECOMPUTED] = keccak_rc_precomputed()
+    t[:24, c.MU] = num_keccak_ops
+    return t
+
+
+def keccak_rc_bus_interactions():
+    """One receiver on the KeccakRc bus: [round, rc[0..8]] with MU (keccak_rc.rs:236-250)."""
+    c = KeccakRcCols
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    return [BusInteraction.receiver(BUS_KECCAK_RC, Multiplicity("Column", (c.MU,)),
+                                    [d(c.ROUND)] + [d(col) for col in c.RC])]
+
+
+def keccak_rc_table_air() -> AirProgram:
+    """KECCAK_RC as an AIR: only the LogUp constraints of its receiver; the
+    first nine columns are preprocessed."""
+    c = KeccakRcCols
+    b = ir.IrBuilder()
+    its = keccak_rc_bus_interactions()
+    n_total = emit_logup_constraints(b, its, 0)
+    prog = b.finish(c.NUM_COLUMNS)
+    pairs, absorbed = split_interactions(len(its))
+    return AirProgram("keccak_rc", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
+                      boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
+                      max_degree=max(3 if pairs else 0, 1 + absorbed),
+                      num_precomputed=c.NUM_PRECOMPUTED, precomputed=keccak_rc_precomputed)
+
+
+class BitwiseCols:
+    """prover/src/tables/bitwise.rs:46-95: X, Y (bytes), Z (4 bits), AND, OR,
+    XOR, MSB8, MSB16, ZERO, SLL, SLLC (precomputed), then one multiplicity
+    column per lookup type."""
+    X, Y, Z, AND, OR, XOR, MSB8, MSB16, ZERO, SLL, SLLC = range(11)
+    MU_MSB8, MU_MSB16, MU_ZERO, MU_ARE_BYTES, MU_IS_HALF, MU_IS_B20, MU_HWSL = range(11, 18)
+    MU_BYTE_ALU_AND, MU_BYTE_ALU_OR, MU_BYTE_ALU_XOR = 18, 19, 20
+    NUM_COLUMNS = 21
+    NUM_PRECOMPUTED = 11
+    NUM_ROWS = 1 << 20
+
+
+# lookup type -> multiplicity column (bitwise.rs:441-471)
+BITWISE_MU_COLUMN = {"msb8": BitwiseCols.MU_MSB8, "msb16": BitwiseCols.MU_MSB16, "zero": BitwiseCols.MU_ZERO,
+                     "are_bytes": BitwiseCols.MU_ARE_BYTES, "is_half": BitwiseCols.MU_IS_HALF,
+                     "is_b20": BitwiseCols.MU_IS_B20, "hwsl": BitwiseCols.MU_HWSL,
+                     "and": BitwiseCols.MU_BYTE_ALU_AND, "or": BitwiseCols.MU_BYTE_ALU_OR,
+                     "xor": BitwiseCols.MU_BYTE_ALU_XOR}
+
+
+def bitwise_row_index(x: int, y: int, z: int) -> int:
+    """Row of (X, Y, Z) = x + 256 y + 65536 z (bitwise.rs:396-400)."""
+    return x + 256 * y + 65536 * z
+
+
+def bitwise_precomputed() -> np.ndarray:
+    """The (2^20, 11) precomputed columns of every (X, Y, Z) (bitwise.rs:117-160):
+    row x + 256 y + 65536 z holds X, Y, Z, X&Y, X|Y, X^Y, MSB of X, MSB of the
+    halfword X + 256 Y, [X = Y = Z = 0], SLL = (halfword << Z) mod 2^16 and
+    SLLC = halfword >> (16 - Z) (0 when Z = 0)."""
+    idx = np.arange(BitwiseCols.NUM_ROWS, dtype=np.uint64)
+    x, y, z = idx & np.uint64(0xFF), (idx >> np.uint64(8)) & np.uint64(0xFF), (idx >> np.uint64(16)) & np.uint64(0xF)
+    hw = x + y * np.uint64(256)
+    t = np.empty((BitwiseCols.NUM_ROWS, BitwiseCols.NUM_PRECOMPUTED), dtype=np.uint64)
+    t[:, 0], t[:, 1], t[:, 2] = x, y, z
+    t[:, 3], t[:, 4], t[:, 5] = x & y, x | y, x ^ y
+    t[:, 6] = (x >> np.uint64(7)) & np.uint64(1)
+    t[:, 7] = (hw >> np.uint64(15)) & np.uint64(1)
+    t[:, 8] = (idx == 0).astype(np.uint64)
+    t[:, 9] = (hw << z) & np.uint64(0xFFFF)
+    zz = np.where(z == 0, np.uint64(16), z)  # z = 0: halfword >> 16 = 0
+    t[:, 10] = np.where(z == 0, np.uint64(0), hw >> (np.uint64(16) - zz))
+    return t
+
+
+def generate_bitwise_trace(lookups) -> np.ndarray:
+    """lookups: (type, x, y, z) tuples (type a BITWISE_MU_COLUMN key); the
+    precomputed columns and, per lookup, one more count in its type's
+    multiplicity column at row (x, y, z) (bitwise.rs:402-427)."""
+    c = BitwiseCols
+    t = np.zeros((c.NUM_ROWS, c.NUM_COLUMNS), dtype=np.uint64)
+    t[:, :c.NUM_PRECOMPUTED] = bitwise_precomputed()
+    for kind, x, y, z in lookups:
+        t[bitwise_row_index(x, y, z), BITWISE_MU_COLUMN[kind]] += 1
+    return t
+
+
+def bitwise_bus_interactions():
+    """The ten receivers of bitwise.rs:728-936, in that order."""
+    c = BitwiseCols
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    mu = lambda col: Multiplicity("Column", (col,))  # noqa: E731
+    hw = BusValue.of_linear([LinearTerm(1, c.X), LinearTerm(256, c.Y)])
+    b20 = BusValue.of_linear([LinearTerm(1, c.X), LinearTerm(256, c.Y), LinearTerm(65536, c.Z)])
+    const = lambda v: BusValue.of_linear([LinearTerm(v)])  # noqa: E731
+    return [
+        BusInteraction.receiver(BUS_MSB8, mu(c.MU_MSB8), [d(c.X), d(c.MSB8)]),
+        BusInteraction.receiver(BUS_MSB16, mu(c.MU_MSB16), [hw, d(c.MSB16)]),
+        BusInteraction.receiver(BUS_ZERO, mu(c.MU_ZERO), [b20, d(c.ZERO)]),
+        BusInteraction.receiver(BUS_ARE_BYTES, mu(c.MU_ARE_BYTES), [d(c.X), d(c.Y)]),
+        BusInteraction.receiver(BUS_IS_HALFWORD, mu(c.MU_IS_HALF), [hw]),
+        BusInteraction.receiver(BUS_IS_B20, mu(c.MU_IS_B20), [b20]),
+        BusInteraction.receiver(BUS_HWSL, mu(c.MU_HWSL), [hw, d(c.Z), d(c.SLL), d(c.SLLC)]),
+        BusInteraction.receiver(BUS_BYTE_ALU, mu(c.MU_BYTE_ALU_AND), [const(ALU_OP_AND), d(c.X), d(c.Y), d(c.AND)]),
+        BusInteraction.receiver(BUS_BYTE_ALU, mu(c.MU_BYTE_ALU_OR), [const(ALU_OP_OR), d(c.X), d(c.Y), d(c.OR)]),
+        BusInteraction.receiver(BUS_BYTE_ALU, mu(c.MU_BYTE_ALU_XOR), [const(ALU_OP_XOR), d(c.X), d(c.Y), d(c.XOR)]),
+    ]
+
+
+def bitwise_table_air() -> AirProgram:
+    """BITWISE as an AIR: only the LogUp constraints of its ten receivers; the
+    first eleven columns are preprocessed."""
+    c = BitwiseCols
+    b = ir.IrBuilder()
+    its = bitwise_bus_interactions()
+    n_total = emit_logup_constraints(b, its, 0)
+    prog = b.finish(c.NUM_COLUMNS)
+    pairs, absorbed = split_interactions(len(its))
+    return AirProgram("bitwise", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
+                      boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
+                      max_degree=max(3 if pairs else 0, 1 + absorbed),
+                      num_precomputed=c.NUM_PRECOMPUTED, precomputed=bitwise_precomputed)

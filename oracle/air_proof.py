@@ -265,9 +265,11 @@ def replay_air_tables(tables, blowup, offset, fri_k, num_queries, grinding, labe
     field for field what lambda_vm_amd.prover returns."""
     Bf, g = blowup, offset
     t = Transcript(label)
-    commits = [lde_commit(tr, Bf, g) for _, tr in tables]
-    for _, nodes in commits:
-        t.append_bytes(bytes(nodes[0]))
+    commits = [_commit_main(air, tr, Bf, g) for air, tr in tables]
+    for c in commits:
+        if len(c) == 3:  # a preprocessed table's fixed-columns root first (prover.rs:3457-3466)
+            t.append_bytes(bytes(c[2][0]))
+        t.append_bytes(bytes(c[1][0]))
     z_logup = alpha = None
     if any(air.interactions for air, _ in tables):
         z_logup, alpha = f3(t.sample_fp3()), f3(t.sample_fp3())
@@ -284,11 +286,34 @@ def replay_air_tables(tables, blowup, offset, fri_k, num_queries, grinding, labe
     return out
 
 
+def _commit_main(air, tr, Bf, g):
+    """(LDE, main tree) of a table; for a preprocessed table (first k columns
+    fixed) the main tree covers the multiplicity columns [k, m) and a third
+    entry is the fixed columns' tree (prover.rs:1194-1290, lde.rs:726)."""
+    k = getattr(air, "num_precomputed", 0)
+    if not k:
+        return lde_commit(tr, Bf, g)
+    lde, _ = lde_commit(tr, Bf, g)
+    m = tr.shape[1]
+    return lde, commit_rows_bit_reversed(lde, 1, k, m), commit_rows_bit_reversed(lde, 1, 0, k)
+
+
+def precomputed_commitment(air, Bf, g) -> bytes:
+    """The verifier's own commitment of a preprocessed table's fixed columns:
+    coset LDE of air.precomputed() and its row-pair Keccak tree (the static
+    commitments of prover/src/tables/keccak_rc.rs:97-116, bitwise.rs:199-218
+    are this value for coset offset 3)."""
+    _, nodes = lde_commit(np.ascontiguousarray(air.precomputed(), dtype=np.uint64), Bf, g, want_lde=False)
+    return bytes(nodes[0])
+
+
 def _replay_table(air, tr, commit, t, Bf, g, fri_k, num_queries, grinding, z_logup, alpha):
     n, m = tr.shape
     N = n * Bf
     bits = N.bit_length() - 1
-    main_lde, main_nodes = commit
+    main_lde, main_nodes = commit[0], commit[1]
+    pre_nodes = commit[2] if len(commit) == 3 else None
+    kpre = air.num_precomputed if pre_nodes is not None else 0
     rap = ap = toff = None
     aux_lde = aux_nodes = L = None
     if air.interactions:
@@ -367,6 +392,10 @@ def _replay_table(air, tr, commit, t, Bf, g, fri_k, num_queries, grinding, z_log
         paths = [merkle_path(nodes, i) for i in iotas]
         return rows, paths
     main_rows, main_paths = open_rows(main_lde, main_nodes)
+    pre_rows = pre_paths = None
+    if pre_nodes is not None:
+        pre_rows, main_rows = main_rows[:, :, :kpre].copy(), main_rows[:, :, kpre:].copy()
+        pre_paths = [merkle_path(pre_nodes, i) for i in iotas]
     aux_rows = aux_paths = None
     if aux_lde is not None:
         aux_rows, aux_paths = open_rows(aux_lde, aux_nodes)
@@ -387,7 +416,9 @@ def _replay_table(air, tr, commit, t, Bf, g, fri_k, num_queries, grinding, z_log
                 fri_roots=[bytes(r) for r in roots], fri_coeffs=[f3(c) for c in fcoeffs], nonce=nonce, iotas=iotas,
                 fri_sym=fri_sym, fri_paths=fri_paths, main_rows=main_rows, main_paths=main_paths, aux_rows=aux_rows,
                 aux_paths=aux_paths, parts_rows=parts_rows, parts_paths=parts_paths, transcript_state=t.state(),
-                challenges=dict(z_logup=z_logup, alpha=alpha, beta=beta, z=z, gamma=gamma))
+                challenges=dict(z_logup=z_logup, alpha=alpha, beta=beta, z=z, gamma=gamma),
+                precomputed_root=None if pre_nodes is None else bytes(pre_nodes[0]), precomputed_rows=pre_rows,
+                precomputed_paths=pre_paths)
 
 
 # ---------------- the verifier ----------------
@@ -418,7 +449,15 @@ def verify_air_tables(airs, proofs, blowup, offset, fri_k, num_queries, grinding
     if len(airs) != len(proofs):
         return False, "table count"
     t = Transcript(label)
-    for pr in proofs:
+    for air, pr in zip(airs, proofs):
+        if getattr(air, "num_precomputed", 0):
+            # the fixed columns' root must be the verifier's own commitment of them
+            pre = _get(pr, "precomputed_root")
+            if pre is None or bytes(pre) != precomputed_commitment(air, blowup, offset):
+                return False, "precomputed commitment"
+            t.append_bytes(bytes(pre))
+        elif _get(pr, "precomputed_root") is not None:
+            return False, "unexpected precomputed commitment"
         t.append_bytes(bytes(_get(pr, "main_root")))
     z_logup = alpha = None
     if any(air.interactions for air in airs):
@@ -532,9 +571,17 @@ def _verify_table(air, pr, t, Bf, g, fri_k, num_queries, grinding, z_logup, alph
     parts_rows = np.asarray(G("parts_rows"))
     aux_rows = G("aux_rows")
     main_root, comp_root = bytes(G("main_root")), bytes(G("composition_root"))
+    kpre = getattr(air, "num_precomputed", 0)
+    if kpre:
+        pre_rows = np.asarray(G("precomputed_rows"))
+        if pre_rows.shape != (len(iotas), 2, kpre) or main_rows.shape[2] != m - kpre:
+            return False, "preprocessed opening shape"
     for q, i in enumerate(iotas):
         if not merkle_verify(main_root, row_pair_leaf(main_rows[q, 0], main_rows[q, 1]), i, G("main_paths")[q]):
             return False, "main opening"
+        if kpre and not merkle_verify(bytes(G("precomputed_root")), row_pair_leaf(pre_rows[q, 0], pre_rows[q, 1]),
+                                      i, G("precomputed_paths")[q]):
+            return False, "precomputed opening"
         if not merkle_verify(comp_root, row_pair_leaf(parts_rows[q, 0], parts_rows[q, 1]), i,
                              G("parts_paths")[q]):
             return False, "composition opening"
@@ -570,7 +617,8 @@ def _verify_table(air, pr, t, Bf, g, fri_k, num_queries, grinding, z_logup, alph
         deep_pair = []
         for s in (0, 1):
             x = g * pow(wN, bitrev(2 * i + s, bits), P) % P
-            tvals = [(int(main_rows[q, s, j]), 0, 0) for j in range(m)]
+            base = list(pre_rows[q, s]) + list(main_rows[q, s]) if kpre else list(main_rows[q, s])
+            tvals = [(int(base[j]), 0, 0) for j in range(m)]
             if ma:
                 aw = [int(v) for v in np.asarray(aux_rows[q][s]).reshape(-1)]
                 tvals += [(aw[3 * c] % P, aw[3 * c + 1] % P, aw[3 * c + 2] % P) for c in range(ma)]

@@ -6,6 +6,10 @@ Sources (read as text from a checkout of the reference at REF):
   crypto/crypto/src/tests/default_transcript_tests.rs:10-46  transcript challenges
   crypto/stark/src/tests/grinding_tests.rs               grinding (seed, nonce, g, valid)
   crypto/math/src/field/goldilocks.rs:536-540            two-adicity and 2^32-th root
+  prover/src/tables/keccak_rc.rs:97-116, bitwise.rs:199-218
+                                                         static commitments of the
+                                                         preprocessed columns (coset 3,
+                                                         blowups 2/4/8)
 
 Usage: python tests/golden/extract_kats.py [REF]   (REF defaults to /root/reference)
 """
@@ -79,12 +83,30 @@ def field_constants():
     return {"two_adicity": two_adicity, "root_of_unity_2_32": root}
 
 
+def static_commitments(rel):
+    """blowup factor -> 32-byte root from a table's `fn static_commitment`."""
+    src = read(rel)
+    body = src[src.index("fn static_commitment"):]
+    body = body[: body.index("\n}\n")]
+    out = {}
+    for m in re.finditer(r"(\d+) => Some\(\[([^\]]*)\]\)", body):
+        out[m.group(1)] = byte_list(m.group(2))
+    return out
+
+
+def preprocessed_commitments():
+    return {"coset_offset": 3,
+            "keccak_rc": static_commitments("prover/src/tables/keccak_rc.rs"),
+            "bitwise": static_commitments("prover/src/tables/bitwise.rs")}
+
+
 def main():
     data = {
         "keccak256": keccak_kats(),
         "transcript_basic_challenge": transcript_kats(),
         "grinding": grinding_kats(),
         "goldilocks": field_constants(),
+        "preprocessed_commitments": preprocessed_commitments(),
     }
     with open(OUT, "w") as f:
         json.dump(data, f, indent=1)

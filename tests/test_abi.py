@@ -61,3 +61,12 @@ def test_lde_shapes_refused_before_device_work(n, m, blowup):
                    ctypes.byref(h))
         assert rc == 1, (fn, rc)  # LVM_ERR_ARG
         assert h.value is None
+
+
+def test_batched_fri_arguments_refused_before_device_work():
+    """lvm_fri_commit_phase_batch refuses missing arrays with LVM_ERR_ARG
+    (no device touched); zero tables is a no-op."""
+    lib = _native.lib()
+    f = lib.lvm_fri_commit_phase_batch
+    assert f(None, None, ctypes.c_size_t(0), None, ctypes.c_uint(2), None, None, None, None) == 0
+    assert f(None, None, ctypes.c_size_t(1), None, ctypes.c_uint(2), None, None, None, None) == 1

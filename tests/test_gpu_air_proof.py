@@ -103,6 +103,19 @@ def _tables(n, which, seed):
             a = A.logup_air(24, A.four_bus_interactions()[:1])
             a.bus_descriptor = A.descriptor(a)
             tr = workload.trace_table(8, 3, n, 24)
+        elif kind == "keccak_rc":
+            from lambda_vm_amd import vm_tables as V
+            a = V.keccak_rc_table_air()
+            a.bus_descriptor = A.descriptor(a)
+            tr = V.generate_keccak_rc_trace(num_keccak_ops=int(rng.integers(1, 1000)))
+        elif kind == "bitwise":
+            from lambda_vm_amd import vm_tables as V
+            a = V.bitwise_table_air()
+            a.bus_descriptor = A.descriptor(a)
+            kinds = list(V.BITWISE_MU_COLUMN)
+            tr = V.generate_bitwise_trace([(kinds[int(k)], int(x), int(y), int(z)) for k, x, y, z in zip(
+                rng.integers(0, len(kinds), 5000), rng.integers(0, 256, 5000), rng.integers(0, 256, 5000),
+                rng.integers(0, 16, 5000))])
         elif kind in ("product", "product_nobus"):
             a = A.product_bus_air(A.four_bus_interactions() if kind == "product" else [])
             a.bus_descriptor = A.descriptor(a) if a.interactions else None
@@ -134,13 +147,15 @@ def _f3rows(x):
                                                      (("load", "memw_register"), 256, 4, 2, 8, 2),
                                                      (("fib", "quad", "logup"), 512, 4, 3, 10, 4),
                                                      (("bus_deg2", "product"), 256, 4, 2, 8, 0),
-                                                     (("product_nobus", "fib"), 512, 8, 2, 8, 2)])
+                                                     (("product_nobus", "fib"), 512, 8, 2, 8, 2),
+                                                     (("keccak_rc",), 32, 4, 2, 8, 0),
+                                                     (("keccak_rc", "eq", "fib"), 64, 8, 2, 8, 2)])
 def test_gpu_proofs_match_oracle_and_verify(gpu, which, n, blowup, k, q, grind):
     from lambda_vm_amd import prover
     tables = _tables(n, which, seed=n + len(which))
     opts = prover.ProofOptions(blowup=blowup, coset_offset=3, fri_final_log_degree=k, num_queries=q,
                                grinding_factor=grind)
-    dev = [(a, gpu.DeviceBuffer.from_array(tr), n) for a, tr in tables]
+    dev = [(a, gpu.DeviceBuffer.from_array(tr), tr.shape[0]) for a, tr in tables]
     got = prover.prove_air_tables(dev, opts, label=b"gpu-air")
     for _, d, _ in dev:
         d.free()
@@ -169,6 +184,11 @@ def test_gpu_proofs_match_oracle_and_verify(gpu, which, n, blowup, k, q, grind):
                 assert np.array_equal(g.fri_paths[li][qi], r["fri_paths"][li][qi]), (t, li, qi)
         for qi in range(q):
             assert np.array_equal(g.main_paths[qi], r["main_paths"][qi])
+        assert g.precomputed_root == r["precomputed_root"], t
+        if r["precomputed_root"] is not None:
+            assert np.array_equal(canon(g.precomputed_rows), canon(r["precomputed_rows"])), t
+            for qi in range(q):
+                assert np.array_equal(g.precomputed_paths[qi], r["precomputed_paths"][qi]), (t, qi)
         assert g.transcript_state == r["transcript_state"], t
     ok, why = verify_air_tables([a for a, _ in tables], got, blowup, 3, k, q, grind, b"gpu-air")
     assert ok, why
@@ -187,12 +207,65 @@ def test_table_scheduling_does_not_change_proofs(gpu):
     from lambda_vm_amd import proof_io, prover
     tables = _tables(256, ("fib", "quad", "eq", "bytewise"), seed=41)
     out = []
-    for workers, budget in ((None, None), (2, 1), (1, None)):
+    # (unlimited: the FRI layers of all tables fold in shared launches; a tiny
+    # budget: one table at a time, each folding on its own; per-table FRI forced)
+    for workers, budget, batch_fri in ((None, None, True), (2, 1, True), (1, None, True), (None, None, False)):
         opts = prover.ProofOptions(blowup=4, coset_offset=3, fri_final_log_degree=2, num_queries=8,
-                                   vram_budget=budget)
+                                   vram_budget=budget, batch_fri=batch_fri)
         dev = [(a, gpu.DeviceBuffer.from_array(tr), 256) for a, tr in tables]
         out.append(proof_io.serialize_multi_proof(prover.prove_air_tables(dev, opts, label=b"sched",
                                                                           workers=workers)))
         for _, d, _ in dev:
             d.free()
-    assert out[0] == out[1] == out[2]
+    assert out[0] == out[1] == out[2] == out[3]
+
+
+@pytest.mark.parametrize("heights", [(1 << 8, 1 << 8), (1 << 10, 1 << 6, 1 << 8), (1 << 12, 1 << 4)])
+def test_batched_fri_commit_equals_per_table(gpu, heights):
+    """lvm_fri_commit_phase_batch over codewords of different lengths (tables
+    that stop committing at different depths) gives every table the roots,
+    terminal coefficients and transcript of its own commit phase."""
+    rng = np.random.default_rng(len(heights) * 7 + heights[0])
+    k = 2
+    cws = [rng.integers(0, P, size=(4 * h, 3), dtype=np.uint64) for h in heights]
+    labels = [b"fri-%d" % i for i in range(len(heights))]
+    ref = []
+    for cw, lab in zip(cws, labels):
+        t = gpu.Transcript(lab)
+        f = gpu.GpuFri(cw, 3)
+        roots, coeffs = f.commit_phase(t, 2, k)
+        ref.append((roots, coeffs, t.state()))
+        f.close()
+    ts = [gpu.Transcript(lab) for lab in labels]
+    fs = [gpu.GpuFri(cw, 3) for cw in cws]
+    got = gpu.fri_commit_phase_batch(fs, ts, [2] * len(fs), k)
+    for (r_roots, r_coeffs, r_state), (g_roots, g_coeffs), t, f in zip(ref, got, ts, fs):
+        assert np.array_equal(g_roots, r_roots)
+        assert np.array_equal(canon(g_coeffs), canon(r_coeffs))
+        assert t.state() == r_state
+        f.close()
+
+
+@pytest.mark.slow
+def test_gpu_bitwise_table_proof(gpu):
+    """The 2^20-row BITWISE table (eleven fixed columns, ten receivers) beside
+    EQ: the GPU proof equals the oracle replay, its precomputed root is the
+    reference's static commitment for blowup 2, and it verifies."""
+    import json
+    import os
+
+    from lambda_vm_amd import prover
+    print("[gpu-air] BITWISE 2^20 x 21 + EQ 256, blowup 2", flush=True)
+    tables = _tables(256, ("bitwise", "eq"), seed=9)
+    opts = prover.ProofOptions(blowup=2, coset_offset=3, fri_final_log_degree=4, num_queries=6)
+    dev = [(a, gpu.DeviceBuffer.from_array(tr), tr.shape[0]) for a, tr in tables]
+    got = prover.prove_air_tables(dev, opts, label=b"gpu-bitwise")
+    for _, d, _ in dev:
+        d.free()
+    ref = replay_air_tables(tables, 2, 3, 4, 6, 0, b"gpu-bitwise")
+    kats = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "kats.json")))
+    assert got[0].precomputed_root == bytes(kats["preprocessed_commitments"]["bitwise"]["2"])
+    from lambda_vm_amd import proof_io
+    from lambda_vm_amd.prover import AirTableProof
+    assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
+    assert verify_air_tables([a for a, _ in tables], got, 2, 3, 4, 6, 0, b"gpu-bitwise")[0]

@@ -118,7 +118,10 @@ def test_ntt_plan_override_keeps_results(gpu):
         plan.close()
         with pytest.raises(gpu.LvmError):
             gpu.set_ntt_plan(12, [8, 8])  # does not sum to 12
+        gpu.set_ntt_plan(12, [])  # an empty split restores the built-in default
+        assert sum(gpu.get_ntt_plan(12)) == 12
     finally:
-        gpu.set_ntt_plan(12, [])
-        gpu.set_ntt_plan(13, [])
+        # back to whatever was installed before (the tuned splits, if any)
+        gpu.set_ntt_plan(12, keep12)
+        gpu.set_ntt_plan(13, keep13)
     assert gpu.get_ntt_plan(12) == keep12 and gpu.get_ntt_plan(13) == keep13

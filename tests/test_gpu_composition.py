@@ -229,3 +229,36 @@ def test_composition_commit_parts_rejects_bad_shapes(gpu):
         gpu.composition_commit_parts(hb, 64, 16, 0, G)
     hb.free()
 
+
+
+@pytest.mark.parametrize("natural", [False, True])
+def test_batched_composition_equals_per_table(gpu, ora, natural):
+    """lvm_eval_composition_batch over tables of different programs, sizes and
+    blowups (one LDE under 256 rows, two tables sharing one program) equals
+    lvm_eval_composition per table, and the oracle, and runs as one compiled
+    launch."""
+    rng = random.Random(404 + natural)
+    specs = [("fib", 256, 4, 9), ("quad", 128, 8, 3), ("logup", 512, 4, 0), ("fib", 16, 4, 9), ("fib", 1024, 2, 9)]
+    cases, comps, refs = [], [], []
+    for which, n, B, m in specs:
+        c = _case(gpu, ora, which, n, B, m, rng)
+        air = c["air"]
+        args = (B, n, G, c["betas"], c["groups"], c["ex"], c["bnd"], c["b_betas"], c["rap"], c["alpha"], c["toff"])
+        ref = canon(ora.eval_composition(air.device, c["main_lde"], c["aux_lde"], *args))
+        refs.append(ref if natural else ref[_bitrev_perm(n * B)])
+        single = gpu.eval_composition(air.device, c["main_h"], c["aux_h"], *args, natural_order=natural)
+        assert np.array_equal(canon(single.to_array(np.uint64, (n * B, 3))), refs[-1])
+        single.free()
+        cases.append(c)
+        comps.append(dict(dev=air.device, main=c["main_h"], aux=c["aux_h"], next_step=B, trace_rows=n, coset_offset=G,
+                          betas=c["betas"], root_group=c["groups"], group_exemptions=c["ex"], boundary=c["bnd"],
+                          b_betas=c["b_betas"], rap=c["rap"], alpha=c["alpha"], table_offset=c["toff"]))
+    gpu.prof_enable(True)
+    gpu.prof_reset()
+    outs = gpu.eval_composition_batch(comps, natural_order=natural)
+    launches = gpu.prof_query("composition_compiled")[0]
+    gpu.prof_enable(False)
+    assert launches == 1
+    for (which, n, B, m), o, ref in zip(specs, outs, refs):
+        assert np.array_equal(canon(o.to_array(np.uint64, (n * B, 3))), ref), (which, n, B)
+        o.free()

@@ -160,3 +160,36 @@ def test_async_dispatch_needs_a_pool():
     svc, _ = _service(1 << 40)
     with pytest.raises(ValueError):
         svc.run_async([(0.0, serve.JobSpec(0, [(256, 8)]))])
+
+
+def test_async_dispatch_batches_commits_under_load():
+    """Jobs admitted while a commit runs join the next one: at most one
+    commit is in flight, so a burst of arrivals shares commits, while a job
+    arriving at an idle service commits alone at once."""
+    class SlowCommit(ThreadedFakeExecutor):
+        def __init__(self):
+            super().__init__(workers=4)
+            self.active = 0
+            self.max_active = 0
+
+        def commit(self, jobs):
+            import time
+            self.active += 1
+            self.max_active = max(self.max_active, self.active)
+            time.sleep(0.01)
+            super().commit(jobs)
+            self.active -= 1
+
+    arrivals = serve.synth_jobs(rate=3000.0, count=30, seed=8, max_log_cells=22)
+    ex = SlowCommit()
+    svc = serve.ProvingService(1 << 50, ex)
+    st = svc.run_async(arrivals)
+    ex.pool.shutdown()
+    assert st["jobs"] == 30 and ex.max_active == 1
+    assert st["mean_jobs_per_commit_batch"] > 1.5
+    assert [j for b in ex.commits for j in b] == list(range(30))
+    lone = SlowCommit()
+    svc = serve.ProvingService(1 << 50, lone)
+    svc.run_async([(0.0, serve.JobSpec(0, [(256, 8)] * 4)), (0.5, serve.JobSpec(1, [(256, 8)] * 4))])
+    lone.pool.shutdown()
+    assert lone.commits == [[0], [1]]
