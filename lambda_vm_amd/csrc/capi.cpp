@@ -108,8 +108,8 @@ int lde_core(Device *d, const uint64_t *d_trace, size_t n, size_t w, size_t blow
     {
         ProfScope ps("ntt_lde", st);
         ntt_inverse_rows(d_trace, (uint32_t)w, coef.as<uint64_t>(), (uint32_t)w, (uint32_t)w, log2u(n), d->tb, st);
-        ntt_coset_expand(coef.as<uint64_t>(), (uint32_t)w, lde.as<uint64_t>(), (uint32_t)w, (uint32_t)w, log2u(n),
-                         log2u(blowup), d_weights, d->tb, st);
+        TRY(ntt_coset_expand(coef.as<uint64_t>(), (uint32_t)w, lde.as<uint64_t>(), (uint32_t)w, (uint32_t)w,
+                             log2u(n), log2u(blowup), d_weights, d->tb, st));
     }
     {
         ProfScope ps("keccak_leaves", st);

@@ -128,7 +128,7 @@ void ntt_inverse_rows(const uint64_t *in, uint32_t in_pitch, uint64_t *out, uint
                       unsigned log_n, const Tables &tb, hipStream_t st);
 void ntt_forward_rows(const uint64_t *in, uint32_t in_pitch, uint64_t *out, uint32_t out_pitch, uint32_t m,
                       unsigned log_n, const Tables &tb, hipStream_t st);
-void ntt_coset_expand(const uint64_t *coef, uint32_t coef_pitch, uint64_t *lde, uint32_t lde_pitch, uint32_t m,
+int ntt_coset_expand(const uint64_t *coef, uint32_t coef_pitch, uint64_t *lde, uint32_t lde_pitch, uint32_t m,
                       unsigned log_n, unsigned log_blowup, const uint64_t *weights, const Tables &tb,
                       hipStream_t st);
 

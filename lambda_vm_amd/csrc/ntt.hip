@@ -522,13 +522,14 @@ void ntt_forward_rows(const uint64_t *in, uint32_t in_pitch, uint64_t *out, uint
 // Coset LDE expansion: coefficient table `coef` (n rows, bit-reversed order,
 // unnormalised iDFT of the trace) -> `lde` (n*B rows, bit-reversed order over
 // log2(n*B) bits) of the evaluations on offset*<w_N> (offset folded into weights).
-// log_n >= 1 (every entry point rejects tables of fewer than two rows).
-void ntt_coset_expand(const uint64_t *coef, uint32_t coef_pitch, uint64_t *lde, uint32_t lde_pitch, uint32_t m,
+// log_n >= 1 (every entry point rejects tables of fewer than two rows); returns
+// LVM_ERR_ARG without writing `lde` for a size that has no pass plan.
+int ntt_coset_expand(const uint64_t *coef, uint32_t coef_pitch, uint64_t *lde, uint32_t lde_pitch, uint32_t m,
                       unsigned log_n, unsigned log_blowup, const uint64_t *weights, const Tables &tb,
                       hipStream_t st) {
     int r[8];
     int np = plan_passes(log_n, r);
-    if (np == 0) return;  // unreachable: n >= 2
+    if (np == 0) return LVM_ERR_ARG;
     const uint32_t B = 1u << log_blowup;
     PassArgs a{};
     a.in = coef;
@@ -551,6 +552,7 @@ void ntt_coset_expand(const uint64_t *coef, uint32_t coef_pitch, uint64_t *lde, 
     a.tw_r = tb.fwd_r;
     launch_pass(r[0], a, B, st, "ntt_coset_pass0");
     run_passes(lde, lde_pitch, m, (size_t)B << log_n, log_n, 1, r, np, tb, false, st);
+    return LVM_OK;
 }
 
 }  // namespace lvm

@@ -949,3 +949,104 @@ def bitwise_table_air() -> AirProgram:
                       boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
                       max_degree=max(3 if pairs else 0, 1 + absorbed),
                       num_precomputed=c.NUM_PRECOMPUTED, precomputed=bitwise_precomputed)
+
+
+# REGISTER (prover/src/tables/register.rs): one row per register word address
+# (x0..x31 at 0..63, x254 at 508/509, x255 = pc at 510/511), padded to 128.
+STACK_TOP = 0xFFFFFFFFFFFFFFF0  # executor/src/vm/registers.rs:3
+CPU_PADDING_PC = 1              # prover/src/tables/cpu.rs:45
+
+
+class RegisterCols:
+    """register.rs:81-103: OFFSET (word address), INIT, FINI (words), IS_PC
+    (fixed), then the final timestamp as two words."""
+    OFFSET, INIT, FINI, IS_PC, TIMESTAMP_LO, TIMESTAMP_HI = range(6)
+    NUM_COLUMNS = 6
+    NUM_PRECOMPUTED = 4
+    NUM_ADDRESSES = 68
+    NUM_ROWS = 128
+
+
+def register_word_address_list() -> list:
+    """register.rs:124-139."""
+    return list(range(64)) + [508, 509, 510, 511]
+
+
+def register_init_from_entry_point(entry_point: int) -> list:
+    """The program-start register image (register.rs:160-184): the stack
+    pointer's words hold STACK_TOP, the pc's the entry point, the rest 0."""
+    vals = {4: STACK_TOP & 0xFFFFFFFF, 5: STACK_TOP >> 32, 510: entry_point & 0xFFFFFFFF, 511: entry_point >> 32}
+    return [vals.get(a, 0) for a in register_word_address_list()]
+
+
+def halted_fini(committed_bytes: int) -> list:
+    """The final registers of every run that halts (register.rs:201-206): zero
+    but x254 (the committed byte count) and the pc's low word (the padding pc)."""
+    fini = [0] * RegisterCols.NUM_ADDRESSES
+    fini[64] = committed_bytes
+    fini[66] = CPU_PADDING_PC
+    return fini
+
+
+def fini_from_final_state(final_state: dict, init: list) -> list:
+    """FINI in address order: the final value of an accessed word, else its
+    init (register.rs:323-335). final_state: word address -> (timestamp, value)."""
+    return [final_state[a][1] if a in final_state else init[i] for i, a in enumerate(register_word_address_list())]
+
+
+def register_precomputed(init: list, fini: list) -> np.ndarray:
+    """The (128, 4) fixed columns OFFSET, INIT, FINI, IS_PC (register.rs:340-362)."""
+    c = RegisterCols
+    t = np.zeros((c.NUM_ROWS, c.NUM_PRECOMPUTED), dtype=np.uint64)
+    for i, a in enumerate(register_word_address_list()):
+        t[i] = (a, init[i], fini[i], int(a in (510, 511)))
+    return t
+
+
+def generate_register_trace(final_state: dict, init: list) -> np.ndarray:
+    """register.rs:254-293: per address its fixed columns, FINI = its final
+    value and the final timestamp (1 and FINI = INIT when never accessed);
+    padding rows carry timestamp 1 so their tokens cancel."""
+    c = RegisterCols
+    t = np.zeros((c.NUM_ROWS, c.NUM_COLUMNS), dtype=np.uint64)
+    fini = fini_from_final_state(final_state, init)
+    t[:, :c.NUM_PRECOMPUTED] = register_precomputed(init, fini)
+    for i, a in enumerate(register_word_address_list()):
+        ts = final_state[a][0] if a in final_state else 1
+        t[i, c.TIMESTAMP_LO], t[i, c.TIMESTAMP_HI] = ts & 0xFFFFFFFF, ts >> 32
+    t[c.NUM_ADDRESSES:, c.TIMESTAMP_LO] = 1
+    return t
+
+
+def register_bus_interactions(final_pc_ts: int):
+    """register.rs:399-462: the initial Memory token (timestamp 1) received
+    once per row; the final token sent, at the fixed final pc timestamp on
+    the pc rows (multiplicity IS_PC) and at the row's timestamp elsewhere
+    (multiplicity 1 - IS_PC)."""
+    c = RegisterCols
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    k = lambda v: BusValue.of_linear([LinearTerm(v)])  # noqa: E731
+    return [
+        BusInteraction.receiver(BUS_MEMORY, ONE, [k(1), d(c.OFFSET), k(0), k(1), k(0), d(c.INIT)]),
+        BusInteraction.sender(BUS_MEMORY, Multiplicity("Column", (c.IS_PC,)),
+                              [k(1), d(c.OFFSET), k(0), k(final_pc_ts), k(0), d(c.FINI)]),
+        BusInteraction.sender(BUS_MEMORY, Multiplicity("Negated", (c.IS_PC,)),
+                              [k(1), d(c.OFFSET), k(0), d(c.TIMESTAMP_LO), d(c.TIMESTAMP_HI), d(c.FINI)]),
+    ]
+
+
+def register_table_air(init: list, fini: list, final_pc_ts: int) -> AirProgram:
+    """REGISTER as an AIR: only the LogUp constraints of its three Memory
+    interactions; OFFSET, INIT, FINI and IS_PC are fixed (a monolithic proof
+    fixes FINI = halted_fini, an epoch's FINI is the next epoch's INIT)."""
+    c = RegisterCols
+    b = ir.IrBuilder()
+    its = register_bus_interactions(final_pc_ts)
+    n_total = emit_logup_constraints(b, its, 0)
+    prog = b.finish(c.NUM_COLUMNS)
+    pairs, absorbed = split_interactions(len(its))
+    init, fini = list(init), list(fini)
+    return AirProgram("register", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
+                      boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
+                      max_degree=max(3 if pairs else 0, 1 + absorbed),
+                      num_precomputed=c.NUM_PRECOMPUTED, precomputed=lambda: register_precomputed(init, fini))

@@ -6,7 +6,8 @@ verifier restatement accepts the GPU proofs (also after a round trip through
 the bytes).
 Tables: FibonacciMultiColumn (one composition part), Quadratic, a 4-bus
 LogUp table and the VM's EQ, BYTEWISE, IS_WH, LT, STORE, BRANCH, MEMW_REGISTER
-and LOAD tables (two parts), a one-interaction bus table (degree 2: one
+and LOAD tables (two parts), the preprocessed KECCAK_RC, BITWISE and REGISTER
+tables (fixed columns under their own tree), a one-interaction bus table (degree 2: one
 part) and a degree-4 bus table with and without interactions (three parts),
 alone and together in one transcript. Bus tables open only their
 accumulated column at the next row."""
@@ -269,3 +270,34 @@ def test_gpu_bitwise_table_proof(gpu):
     from lambda_vm_amd.prover import AirTableProof
     assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
     assert verify_air_tables([a for a, _ in tables], got, 2, 3, 4, 6, 0, b"gpu-bitwise")[0]
+
+
+def test_gpu_register_table_proof(gpu):
+    """REGISTER (four fixed columns, its final-state FINI column and the
+    Memory-bus sender pair) with the MEMW_REGISTER chain whose tokens it
+    closes, in one transcript: the GPU proofs equal the oracle replay byte
+    for byte, REGISTER's precomputed root is the verifier's own commitment and
+    both verify."""
+    from lambda_vm_amd import proof_io, prover
+    from lambda_vm_amd import vm_tables as V
+    from lambda_vm_amd.prover import AirTableProof
+    from oracle.air_proof import precomputed_commitment
+    from tests.test_vm_table_register import register_chain
+
+    print("[gpu-air] REGISTER 128 x 6 + MEMW_REGISTER, blowup 4", flush=True)
+    acc, final, init = register_chain(200, 11)
+    reg = V.register_table_air(init, V.fini_from_final_state(final, init), final_pc_ts=1)
+    reg.bus_descriptor = A.descriptor(reg)
+    mem = V.memw_register_table_air()
+    mem.bus_descriptor = A.descriptor(mem)
+    tables = [(reg, V.generate_register_trace(final, init)), (mem, V.generate_memw_register_trace(acc))]
+    tables = [(a, np.ascontiguousarray(tr, dtype=np.uint64)) for a, tr in tables]
+    opts = prover.ProofOptions(blowup=4, coset_offset=3, fri_final_log_degree=2, num_queries=8)
+    dev = [(a, gpu.DeviceBuffer.from_array(tr), tr.shape[0]) for a, tr in tables]
+    got = prover.prove_air_tables(dev, opts, label=b"gpu-register")
+    for _, d, _ in dev:
+        d.free()
+    ref = replay_air_tables(tables, 4, 3, 2, 8, 0, b"gpu-register")
+    assert got[0].precomputed_root == precomputed_commitment(reg, 4, 3)
+    assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
+    assert verify_air_tables([reg, mem], got, 4, 3, 2, 8, 0, b"gpu-register")[0]
