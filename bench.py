@@ -515,6 +515,7 @@ def run_constraints(args) -> dict:
     ap = np.array(ap, dtype=np.uint64)
     betas = [[r3() for _ in range(a.num_constraints)] for a in airs]
     cells = sum(n * m * B for _, n, m in specs)
+    copy_gbs = copy_rate_gbs(backend)
 
     def step():
         plan = backend.BatchPlan([(traces[i], weights[i], n, m, B) for i, (_, n, m) in enumerate(specs)],
@@ -601,8 +602,11 @@ def run_constraints(args) -> dict:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4) if achieved else None,
-            "traffic": None,
+            "traffic": measured_traffic(comp_kernel, "constraints3"),
+            "valu_issue_frac": measured_valu_issue(comp_kernel, "constraints3"),
+            "compulsory_bytes": comp_bytes,
             "stage_ms_per_step": stage,
+            "copy_kernel_GBps": round(copy_gbs, 1),
         },
     }
 
