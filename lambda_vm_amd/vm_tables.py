@@ -1050,3 +1050,151 @@ def register_table_air(init: list, fini: list, final_pc_ts: int) -> AirProgram:
                       boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
                       max_degree=max(3 if pairs else 0, 1 + absorbed),
                       num_precomputed=c.NUM_PRECOMPUTED, precomputed=lambda: register_precomputed(init, fini))
+
+
+# PAGE (prover/src/tables/page.rs) and GLOBAL_MEMORY (global_memory.rs): one
+# row per byte of a page. Both fix OFFSET and INIT (a private-input page fixes
+# OFFSET only, page.rs:86-95); the address is the constant page base plus OFFSET.
+PAGE_SIZE = 1 << 18            # page.rs:48 DEFAULT_PAGE_SIZE
+GENESIS_EPOCH = 0              # local_to_global.rs:74
+BUS_GLOBAL_MEMORY = 31         # types.rs:368
+
+
+class PageCols:
+    """page.rs:61-80."""
+    OFFSET, INIT, FINI, TIMESTAMP_LO, TIMESTAMP_HI = range(5)
+    NUM_COLUMNS = 5
+
+
+class GlobalMemoryCols:
+    """global_memory.rs:55-80."""
+    OFFSET, INIT, FINI, FINI_EPOCH = range(4)
+    NUM_COLUMNS = 4
+
+
+def page_precomputed(init_values, num_fixed: int = 2, page_size: int = PAGE_SIZE) -> np.ndarray:
+    """The fixed columns of a page (page.rs:466-493): OFFSET = 0..page_size-1 and,
+    unless the page holds private input (num_fixed = 1), INIT = the initial
+    bytes, zero past their end."""
+    t = np.zeros((page_size, num_fixed), dtype=np.uint64)
+    t[:, 0] = np.arange(page_size, dtype=np.uint64)
+    if num_fixed > 1 and init_values is not None:
+        b = np.frombuffer(bytes(init_values), dtype=np.uint8)
+        if len(b) > page_size:
+            raise ValueError("initial data exceeds the page")
+        t[: len(b), 1] = b
+    return t
+
+
+def _page_base_check(page_base: int, page_size: int):
+    if page_base % page_size:
+        raise ValueError("page base must be page-aligned")
+
+
+def generate_page_trace(page_base: int, init_values, final_state: dict, page_size: int = PAGE_SIZE) -> np.ndarray:
+    """page.rs:253-300: per offset its INIT byte and, if the address was accessed,
+    its final (timestamp, value); else timestamp 0 and FINI = INIT.
+    final_state: byte address -> (timestamp, value)."""
+    _page_base_check(page_base, page_size)
+    c = PageCols
+    t = np.zeros((page_size, c.NUM_COLUMNS), dtype=np.uint64)
+    t[:, :2] = page_precomputed(init_values, 2, page_size)
+    t[:, c.FINI] = t[:, c.INIT]
+    for addr, (ts, val) in final_state.items():
+        off = addr - page_base
+        if not 0 <= off < page_size:
+            raise ValueError("final state holds an address outside the page")
+        if not 0 <= val < 256:
+            raise ValueError("page values are bytes")
+        t[off, c.FINI] = val
+        t[off, c.TIMESTAMP_LO], t[off, c.TIMESTAMP_HI] = ts & 0xFFFFFFFF, ts >> 32
+    return t
+
+
+def _page_address(page_base: int, offset_col: int):
+    """address_lo = page_base_lo + OFFSET, address_hi = page_base_hi (page.rs:588-598)."""
+    lo = BusValue.of_linear([LinearTerm(page_base & 0xFFFFFFFF), LinearTerm(1, offset_col)])
+    hi = BusValue.of_linear([LinearTerm(page_base >> 32)])
+    return lo, hi
+
+
+def page_bus_interactions(page_base: int):
+    """page.rs:600-667: ARE_BYTES[init, fini] sent once per row; the initial
+    Memory token [0, address, 0, 0, init] received; the final token
+    [0, address, timestamp, fini] sent."""
+    c = PageCols
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    k = lambda v: BusValue.of_linear([LinearTerm(v)])  # noqa: E731
+    lo, hi = _page_address(page_base, c.OFFSET)
+    return [
+        BusInteraction.sender(BUS_ARE_BYTES, ONE, [d(c.INIT), d(c.FINI)]),
+        BusInteraction.receiver(BUS_MEMORY, ONE, [k(0), lo, hi, k(0), k(0), d(c.INIT)]),
+        BusInteraction.sender(BUS_MEMORY, ONE, [k(0), lo, hi, d(c.TIMESTAMP_LO), d(c.TIMESTAMP_HI), d(c.FINI)]),
+    ]
+
+
+def _bus_only_air(name, num_columns, its, num_fixed, precomputed) -> AirProgram:
+    b = ir.IrBuilder()
+    n_total = emit_logup_constraints(b, its, 0)
+    prog = b.finish(num_columns)
+    pairs, absorbed = split_interactions(len(its))
+    return AirProgram(name, num_columns, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
+                      boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
+                      max_degree=max(3 if pairs else 0, 1 + absorbed),
+                      num_precomputed=num_fixed, precomputed=precomputed)
+
+
+def page_table_air(page_base: int, init_values=None, private: bool = False, page_size: int = PAGE_SIZE) -> AirProgram:
+    """PAGE as an AIR: only the LogUp constraints of its three interactions,
+    OFFSET and INIT fixed (OFFSET alone for a private-input page, whose INIT is
+    a committed trace column)."""
+    _page_base_check(page_base, page_size)
+    k = 1 if private else 2
+    data = None if init_values is None else bytes(init_values)
+    return _bus_only_air("page", PageCols.NUM_COLUMNS, page_bus_interactions(page_base), k,
+                         lambda: page_precomputed(data, k, page_size))
+
+
+def generate_global_memory_trace(page_base: int, init_values, final_state: dict,
+                                 page_size: int = PAGE_SIZE) -> np.ndarray:
+    """global_memory.rs:107-149: per offset its genesis byte and, if a later
+    epoch touched the address, its final (value, epoch); else FINI = INIT at
+    GENESIS_EPOCH. final_state: byte address -> (value, epoch)."""
+    _page_base_check(page_base, page_size)
+    c = GlobalMemoryCols
+    t = np.zeros((page_size, c.NUM_COLUMNS), dtype=np.uint64)
+    t[:, :2] = page_precomputed(init_values, 2, page_size)
+    t[:, c.FINI] = t[:, c.INIT]
+    t[:, c.FINI_EPOCH] = GENESIS_EPOCH
+    for addr, (val, epoch) in final_state.items():
+        off = addr - page_base
+        if not 0 <= off < page_size:
+            raise ValueError("final state holds an address outside the page")
+        if not 0 <= val < 256:
+            raise ValueError("page values are bytes")
+        t[off, c.FINI], t[off, c.FINI_EPOCH] = val, epoch
+    return t
+
+
+def global_memory_bus_interactions(page_base: int):
+    """global_memory.rs:165-213: the genesis token [address, init, GENESIS]
+    sent, the finalization token [address, fini, fini_epoch] received, on the
+    GlobalMemory bus."""
+    c = GlobalMemoryCols
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    lo, hi = _page_address(page_base, c.OFFSET)
+    return [
+        BusInteraction.sender(BUS_GLOBAL_MEMORY, ONE, [lo, hi, d(c.INIT), BusValue.of_linear([LinearTerm(GENESIS_EPOCH)])]),
+        BusInteraction.receiver(BUS_GLOBAL_MEMORY, ONE, [lo, hi, d(c.FINI), d(c.FINI_EPOCH)]),
+    ]
+
+
+def global_memory_table_air(page_base: int, init_values=None, private: bool = False,
+                            page_size: int = PAGE_SIZE) -> AirProgram:
+    """GLOBAL_MEMORY as an AIR: the LogUp constraints of its two interactions;
+    the fixed columns are PAGE's (so is their commitment)."""
+    _page_base_check(page_base, page_size)
+    k = 1 if private else 2
+    data = None if init_values is None else bytes(init_values)
+    return _bus_only_air("global_memory", GlobalMemoryCols.NUM_COLUMNS, global_memory_bus_interactions(page_base), k,
+                         lambda: page_precomputed(data, k, page_size))

@@ -6,8 +6,8 @@ Sources (read as text from a checkout of the reference at REF):
   crypto/crypto/src/tests/default_transcript_tests.rs:10-46  transcript challenges
   crypto/stark/src/tests/grinding_tests.rs               grinding (seed, nonce, g, valid)
   crypto/math/src/field/goldilocks.rs:536-540            two-adicity and 2^32-th root
-  prover/src/tables/keccak_rc.rs:97-116, bitwise.rs:199-218
-                                                         static commitments of the
+  prover/src/tables/keccak_rc.rs:97-116, bitwise.rs:199-218,
+  page.rs:416-455                                        static commitments of the
                                                          preprocessed columns (coset 3,
                                                          blowups 2/4/8)
 
@@ -83,10 +83,11 @@ def field_constants():
     return {"two_adicity": two_adicity, "root_of_unity_2_32": root}
 
 
-def static_commitments(rel):
-    """blowup factor -> 32-byte root from a table's `fn static_commitment`."""
+def static_commitments(rel, fn="fn static_commitment"):
+    """blowup factor -> 32-byte root from a table's static-commitment match
+    (`fn static_commitment` unless another function is named)."""
     src = read(rel)
-    body = src[src.index("fn static_commitment"):]
+    body = src[src.index(fn):]
     body = body[: body.index("\n}\n")]
     out = {}
     for m in re.finditer(r"(\d+) => Some\(\[([^\]]*)\]\)", body):
@@ -97,7 +98,11 @@ def static_commitments(rel):
 def preprocessed_commitments():
     return {"coset_offset": 3,
             "keccak_rc": static_commitments("prover/src/tables/keccak_rc.rs"),
-            "bitwise": static_commitments("prover/src/tables/bitwise.rs")}
+            "bitwise": static_commitments("prover/src/tables/bitwise.rs"),
+            # page.rs:416-455: OFFSET + zero INIT of a zero-initialised page, and
+            # OFFSET alone (private-input pages; also GLOBAL_MEMORY's)
+            "page_zero": static_commitments("prover/src/tables/page.rs", "fn static_zero_page_commitment"),
+            "page_private": static_commitments("prover/src/tables/page.rs", "fn static_private_page_commitment")}
 
 
 def main():

@@ -301,3 +301,44 @@ def test_gpu_register_table_proof(gpu):
     assert got[0].precomputed_root == precomputed_commitment(reg, 4, 3)
     assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
     assert verify_air_tables([reg, mem], got, 4, 3, 2, 8, 0, b"gpu-register")[0]
+
+
+def test_gpu_page_tables_proof(gpu):
+    """A zero-initialised 2^18-byte PAGE with 300 touched bytes, the table of
+    accesses that closes its Memory tokens, and a private-input
+    GLOBAL_MEMORY page (OFFSET fixed alone) in one transcript at blowup 2:
+    the GPU proofs equal the oracle replay byte for byte, the fixed-columns
+    roots are the reference's static zero-page and private-page commitments,
+    and the proofs verify."""
+    import json
+    import os
+
+    from lambda_vm_amd import proof_io, prover
+    from lambda_vm_amd import vm_tables as V
+    from lambda_vm_amd.prover import AirTableProof
+    from tests.test_vm_table_page import access_chain_table
+
+    print("[gpu-air] PAGE 2^18 x 5 + accesses + GLOBAL_MEMORY 2^18 x 4, blowup 2", flush=True)
+    rng = np.random.default_rng(12)
+    base = 0x3_0004_0000
+    final = {base + int(o): (int(t), int(v)) for o, t, v in
+             zip(rng.choice(V.PAGE_SIZE, 300, replace=False), rng.integers(4, 2**40, 300), rng.integers(0, 256, 300))}
+    page = V.page_table_air(base)
+    page.bus_descriptor = A.descriptor(page)
+    chain_air, chain_tr = access_chain_table(base, b"", final)
+    gm = V.global_memory_table_air(base, b"\x01\x02\x03", private=True)
+    gm.bus_descriptor = A.descriptor(gm)
+    gm_tr = V.generate_global_memory_trace(base, b"\x01\x02\x03", {base + 2: (7, 5)})
+    tables = [(page, V.generate_page_trace(base, None, final)), (chain_air, chain_tr), (gm, gm_tr)]
+    tables = [(a, np.ascontiguousarray(tr, dtype=np.uint64)) for a, tr in tables]
+    opts = prover.ProofOptions(blowup=2, coset_offset=3, fri_final_log_degree=4, num_queries=6)
+    dev = [(a, gpu.DeviceBuffer.from_array(tr), tr.shape[0]) for a, tr in tables]
+    got = prover.prove_air_tables(dev, opts, label=b"gpu-page")
+    for _, d, _ in dev:
+        d.free()
+    kats = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "kats.json")))["preprocessed_commitments"]
+    assert got[0].precomputed_root == bytes(kats["page_zero"]["2"])
+    assert got[2].precomputed_root == bytes(kats["page_private"]["2"])
+    ref = replay_air_tables(tables, 2, 3, 4, 6, 0, b"gpu-page")
+    assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
+    assert verify_air_tables([page, chain_air, gm], got, 2, 3, 4, 6, 0, b"gpu-page")[0]
