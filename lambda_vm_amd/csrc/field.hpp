@@ -15,6 +15,14 @@
 
 #define LVM_HD __host__ __device__ __forceinline__
 
+// Device-memory views in the global address space: loads and stores through
+// them are global_* instructions, which count only against vmcnt, where
+// generic (flat) ones also count against lgkmcnt and so make every LDS wait
+// wait for the memory loads in flight as well.
+typedef __attribute__((address_space(1))) uint64_t gmem_u64;
+#define LVM_GLOBAL(p) ((gmem_u64 *)(p))
+#define LVM_GLOBAL_CONST(p) ((const gmem_u64 *)(p))
+
 namespace lvm {
 
 constexpr uint64_t P = 0xFFFFFFFF00000001ULL;
@@ -179,6 +187,51 @@ LVM_HD uint64_t cmul_pow2(uint64_t x) {
     } else {
         return cmul(x, pow2_mod(S));
     }
+}
+
+// x * 2^-K for canonical x and 0 < K <= 32. p = 1 mod 2^32, so m = -x mod 2^K
+// makes x + m*p a multiple of 2^K, and
+//   (x + m*p) / 2^K = ((x + m) >> K) + (m * 2^(32-K)) * (2^32 - 1)
+// where x + m <= p - 1 + 2^32 - 1 < 2^64. The first term is at most
+// 2^(64-K) - 2^(32-K) and the second at most 2^64 - 2^32 - 2^(64-K) + 2^(32-K),
+// so the result is at most p - 1: canonical with no correction (a shift, an
+// add and one v_mad_u64_u32 where a general product needs four and a reduction).
+template <int K>
+LVM_HD uint64_t cmul_inv_pow2_32(uint64_t x) {
+    static_assert(K > 0 && K <= 32, "divisor out of range");
+    constexpr uint32_t MASK = K == 32 ? 0xFFFFFFFFu : ((1u << (K & 31)) - 1u);
+    const uint32_t m = (0u - (uint32_t)x) & MASK;
+    const uint64_t q = (x + m) >> K;
+    return q + (uint64_t)(m << (32 - K)) * EPS;
+}
+
+// x * 2^-K for canonical x and 0 < K < 96, as divisions by at most 2^32
+template <int K>
+LVM_HD uint64_t cmul_inv_pow2(uint64_t x) {
+    static_assert(K > 0 && K < 96, "divisor out of range");
+    if constexpr (K <= 32)
+        return cmul_inv_pow2_32<K>(x);
+    else
+        return cmul_inv_pow2<K - 32>(cmul_inv_pow2_32<32>(x));
+}
+
+// A multiplication by 2^T (any T >= 0; 2^192 = 1, 2^96 = -1) as the cheapest
+// of the two forms: with S = T mod 96, 2^T = +-2^S; S <= 32 is a left shift
+// and reduction (cmul_pow2), S > 32 the division 2^S = -2^-(96 - S).
+// pow2_negates<T>() says whether the form computes -x * 2^T, so that a
+// caller folds the sign into an operand order.
+template <int T>
+constexpr bool pow2_negates() {
+    constexpr int S = (T % 192) % 96;
+    return ((T % 192) >= 96) != (S > 32);
+}
+template <int T>
+LVM_HD uint64_t cmul_pow2_form(uint64_t x) {
+    constexpr int S = (T % 192) % 96;
+    if constexpr (S <= 32)
+        return cmul_pow2<S>(x);
+    else
+        return cmul_inv_pow2<96 - S>(x);
 }
 
 // ---------------- Fp3 ----------------

@@ -122,6 +122,7 @@ struct PassArgs {
     const uint64_t *tw_r;          // w_256^(+-e), e < 256
     uint32_t cosets_per_block;     // cosets produced per block (grid.z = cosets / this)
     uint32_t inverse;              // 1: inverse transform (tw_* hold inverse powers)
+    uint32_t in_canonical;         // 1: input values are below p (an earlier pass wrote them)
 };
 
 void ntt_inverse_rows(const uint64_t *in, uint32_t in_pitch, uint64_t *out, uint32_t out_pitch, uint32_t m,

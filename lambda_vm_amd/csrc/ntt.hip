@@ -41,7 +41,14 @@ namespace lvm {
 
 // w_{2^32}^e via the two-level table (canonical)
 __device__ __forceinline__ uint64_t tw32(const uint64_t *lo, const uint64_t *hi, uint32_t e) {
-    return cmul(lo[e & 0xFFFFu], hi[e >> 16]);
+    return cmul(LVM_GLOBAL_CONST(lo)[e & 0xFFFFu], LVM_GLOBAL_CONST(hi)[e >> 16]);
+}
+
+// bit reversal of i over `bits` bits, for compile-time indices
+constexpr uint32_t bitrev_small(uint32_t i, int bits) {
+    uint32_t r = 0;
+    for (int b = 0; b < bits; b++) r |= ((i >> b) & 1u) << (bits - 1 - b);
+    return r;
 }
 
 // log2 of w_R as a power of two (w_R = w_16^(16/R)), forward and inverse
@@ -61,15 +68,18 @@ constexpr uint64_t cmod_pow(uint64_t a, uint64_t e) {
 static_assert(cmod_pow(1753635133440165772ull, 1ull << 28) == cmod_pow(2, 156), "forward w_16 is 2^156");
 static_assert(cmod_mul(cmod_pow(2, 156), cmod_pow(2, 36)) == 1, "inverse w_16 is 2^36");
 
-// DIF butterfly (u, v) -> (u + v, (u - v) * 2^T); 2^T = -2^(T-96) for T >= 96
+// DIF butterfly (u, v) -> (u + v, (u - v) * 2^T); where the cheaper form of
+// the power of two computes -2^T, the subtraction is swapped instead
 template <int T>
 __device__ __forceinline__ void bfly_pow2(uint64_t &u, uint64_t &v) {
     const uint64_t a = u, b = v;
     u = cadd(a, b);
-    if constexpr (T >= 96)
-        v = cmul_pow2<T - 96>(csub(b, a));
+    if constexpr (T % 192 == 0)
+        v = csub(a, b);
+    else if constexpr (pow2_negates<T>())
+        v = cmul_pow2_form<T>(csub(b, a));
     else
-        v = cmul_pow2<T>(csub(a, b));
+        v = cmul_pow2_form<T>(csub(a, b));
 }
 
 template <int R, int LOGW, int H, int I>
@@ -130,7 +140,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t bx, ui
     const uint64_t row0 = (blk << a.log_msub) + jlo;  // logical row of t = 0
 
     // inner twiddles w_R^e = w_256^(e*256/R), e < R
-    for (int e = tid; e < R; e += 256) s_tw[e] = a.tw_r[e << (8 - LOG_R)];
+    for (int e = tid; e < R; e += 256) s_tw[e] = LVM_GLOBAL_CONST(a.tw_r)[e << (8 - LOG_R)];
     // per-tile post-twiddle factors, shared by the C columns
     const int tin = tid % (C * TPC);
     for (int t = tin; t < R; t += C * TPC) {
@@ -147,19 +157,26 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t bx, ui
     }
 
     uint64_t x[TPC > R1 ? TPC : R1];
-    uint64_t xin[MULTI ? R1 : 1];
+    uint64_t xin[R1];
     const bool pre_needed = a.weights != nullptr || a.shift_log != 0;
-    auto load_in = [&](int i) -> uint64_t {
-        if (!c_ok) return 0;
-        uint64_t lrow = row0 + ((uint64_t)(tc + R2 * i) << log_s);
-        uint64_t prow = a.in_bitrev_bits ? (uint64_t)bitrev((uint32_t)lrow, a.in_bitrev_bits) : lrow;
-        return a.in[prow * a.in_pitch + col];
-    };
-    // MULTI: the tile's input is read once and serves every coset this block produces
-    if constexpr (MULTI) {
-        if (tc < R2) {
+    // Input element i of a loading thread (tc < R2) is logical row
+    // rowA + (i << (log_s + LOG_R2)) with disjoint bit fields, so with
+    // bit-reversed input rows (over in_bitrev_bits = log_msub bits) its
+    // physical row is bitrev(rowA) + bitrev_R1(i) << (bits - log_s - LOG_R):
+    // either way a per-thread base plus a uniform stride per element, and the
+    // R1 loads are issued back to back. A lane outside the table reads row 0,
+    // column 0 (its results are never stored).
+    const uint32_t bits = a.in_bitrev_bits;
+    const bool loads = tc < R2;
+    const uint64_t rowA = row0 + ((uint64_t)tc << log_s);
+    const uint64_t physA = bits ? (uint64_t)bitrev((uint32_t)rowA, bits) : rowA;
+    const gmem_u64 *src = LVM_GLOBAL_CONST(a.in) + (c_ok && loads ? physA * a.in_pitch + col : 0);
+    const uint64_t in_step = (uint64_t)a.in_pitch << (bits ? bits - log_s - LOG_R : log_s + LOG_R2);
+    if (loads) {
 #pragma unroll
-            for (int i = 0; i < R1; i++) xin[i] = load_in(i);
+        for (int i = 0; i < R1; i++) {
+            const uint64_t sel = bits ? bitrev_small(i, LOG_R1) : (uint64_t)i;
+            xin[i] = src[sel * in_step];
         }
     }
 
@@ -172,7 +189,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t bx, ui
             if (g_ok) {
                 uint64_t pre = 1;
                 uint64_t lrow = row0 + ((uint64_t)t << log_s);
-                if (a.weights) pre = a.weights[lrow];
+                if (a.weights) pre = LVM_GLOBAL_CONST(a.weights)[lrow];
                 if (a.shift_log) {
                     uint32_t e = (uint32_t)(((uint64_t)coset * lrow) << (32 - a.shift_log));
                     pre = cmul(pre, tw32(a.shift_lo, a.shift_hi, e));
@@ -187,14 +204,8 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t bx, ui
     if (tc < R2) {
         const int t0 = tc;
 #pragma unroll
-        for (int i = 0; i < R1; i++) {
-            uint64_t v;
-            if constexpr (MULTI)
-                v = xin[i];
-            else
-                v = load_in(i);
-            x[i] = pre_needed ? cmul(v, s_pre[tl][t0 + R2 * i]) : ccanon(v);
-        }
+        for (int i = 0; i < R1; i++)
+            x[i] = pre_needed ? cmul(xin[i], s_pre[tl][t0 + R2 * i]) : a.in_canonical ? xin[i] : ccanon(xin[i]);
         dft_small<R1>(x, a.inverse);
         // position i holds frequency k_a = bitrev_R1(i)
         if constexpr (R2 > 1) {
@@ -215,13 +226,11 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t bx, ui
         if (c_ok) {
             uint64_t obase = 0;
             if (a.out_coset_rows) obase = (uint64_t)bitrev(coset, a.coset_bits) * a.out_coset_rows;
+            // output q = R2*t1 + i is row obase + row0 + (q << log_s): a base and a uniform stride
+            gmem_u64 *dst = LVM_GLOBAL(a.out) + (obase + row0 + ((uint64_t)(R2 * t1) << log_s)) * a.out_pitch + col;
+            const uint64_t out_step = (uint64_t)a.out_pitch << log_s;
 #pragma unroll
-            for (int i = 0; i < R2; i++) {
-                const int q = R2 * t1 + i;
-                uint64_t v = log_s ? cmul(x[i], s_post[tl][q]) : x[i];
-                uint64_t orow = obase + row0 + ((uint64_t)q << log_s);
-                a.out[orow * a.out_pitch + col] = v;
-            }
+            for (int i = 0; i < R2; i++) dst[i * out_step] = log_s ? cmul(x[i], s_post[tl][R2 * t1 + i]) : x[i];
         }
     }
     __syncthreads();  // the LDS tile and factors are rewritten by the next coset
@@ -398,6 +407,7 @@ int lde_pass_list(const uint64_t *trace, uint32_t m, unsigned log_n, unsigned lo
         a.tw_hi = tb.inv_hi;
         a.tw_r = tb.inv_r;
         a.inverse = 1;
+        a.in_canonical = p > 0;
         a.cosets_per_block = 1;
         r_out[k] = r[p];
         cos_out[k] = 1;
@@ -417,6 +427,7 @@ int lde_pass_list(const uint64_t *trace, uint32_t m, unsigned log_n, unsigned lo
         a.tw_lo = tb.fwd_lo;
         a.tw_hi = tb.fwd_hi;
         a.tw_r = tb.fwd_r;
+        a.in_canonical = 1;
         if (p == 0) {
             a.in_bitrev_bits = log_n;
             a.total_groups = (1ull << log_n) >> r[0];
@@ -462,6 +473,7 @@ static void run_passes(uint64_t *buf, uint32_t pitch, uint32_t m, size_t rows, u
         a.tw_hi = inverse ? tb.inv_hi : tb.fwd_hi;
         a.tw_r = inverse ? tb.inv_r : tb.fwd_r;
         a.inverse = inverse ? 1u : 0u;
+        a.in_canonical = 1;
         launch_pass(r[p], a, 1, st, PASS_NAMES[inverse][p < 3 ? p : 3]);
         done += r[p];
     }
@@ -474,7 +486,7 @@ void ntt_inverse_rows(const uint64_t *in, uint32_t in_pitch, uint64_t *out, uint
     int r[8];
     int np = plan_passes(log_n, r);
     if (np == 0) {
-        hipMemcpy2DAsync(out, out_pitch * 8, in, in_pitch * 8, m * 8, 1, hipMemcpyDeviceToDevice, st);
+        (void)hipMemcpy2DAsync(out, out_pitch * 8, in, in_pitch * 8, m * 8, 1, hipMemcpyDeviceToDevice, st);
         return;
     }
     PassArgs a{};
@@ -500,7 +512,7 @@ void ntt_forward_rows(const uint64_t *in, uint32_t in_pitch, uint64_t *out, uint
     int r[8];
     int np = plan_passes(log_n, r);
     if (np == 0) {
-        hipMemcpy2DAsync(out, out_pitch * 8, in, in_pitch * 8, m * 8, 1, hipMemcpyDeviceToDevice, st);
+        (void)hipMemcpy2DAsync(out, out_pitch * 8, in, in_pitch * 8, m * 8, 1, hipMemcpyDeviceToDevice, st);
         return;
     }
     PassArgs a{};
@@ -545,6 +557,7 @@ int ntt_coset_expand(const uint64_t *coef, uint32_t coef_pitch, uint64_t *lde, u
     a.coset_bits = log_blowup;
     a.shift_log = log_n + log_blowup;
     a.weights = weights;
+    a.in_canonical = 1;
     a.tw_lo = tb.fwd_lo;
     a.tw_hi = tb.fwd_hi;
     a.shift_lo = tb.fwd_lo;
