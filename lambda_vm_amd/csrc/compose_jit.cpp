@@ -46,7 +46,16 @@ struct Operand {
     bool ext;
 };
 
-constexpr size_t JIT_LOAD_WINDOW = 48;
+// Nodes per load window (trace reads of a window are issued together); the
+// environment variable LVM_JIT_LOAD_WINDOW overrides it for measurement.
+size_t jit_load_window() {
+    static const size_t w = [] {
+        const char *e = std::getenv("LVM_JIT_LOAD_WINDOW");
+        const long v = e ? std::strtol(e, nullptr, 10) : 0;
+        return v > 0 ? (size_t)v : (size_t)48;
+    }();
+    return w;
+}
 
 // Straight-line per-row code of the program for natural LDE index r (the
 // statements after `r` is set, with `a` the CompArgs and `N` the LDE size in
@@ -101,7 +110,7 @@ std::string generate_row(const lvm_program *pg, const std::vector<uint32_t> &roo
     };
     auto as_ext = [](const Operand &o) { return o.ext ? o.expr : "embed(" + o.expr + ")"; };
     // per-node source; trace reads are then moved to the front of their
-    // window of JIT_LOAD_WINDOW nodes so that a window's loads are in flight
+    // window of jit_load_window() nodes so that a window's loads are in flight
     // together (they depend on nothing, so the order of values is unchanged)
     std::vector<std::string> node_text(n);
     std::vector<bool> is_load(n, false);
@@ -205,8 +214,9 @@ std::string generate_row(const lvm_program *pg, const std::vector<uint32_t> &roo
             }
         }
     }
-    for (size_t w0 = 0; w0 < n; w0 += JIT_LOAD_WINDOW) {
-        const size_t w1 = std::min(n, w0 + JIT_LOAD_WINDOW);
+    const size_t window = jit_load_window();
+    for (size_t w0 = 0; w0 < n; w0 += window) {
+        const size_t w1 = std::min(n, w0 + window);
         for (size_t k = w0; k < w1; k++)
             if (is_load[k]) body += node_text[k];
         for (size_t k = w0; k < w1; k++)

@@ -148,11 +148,13 @@ LVM_HD uint64_t csub(uint64_t a, uint64_t b) {
 // t0 + t1 wraps at most once and its correction cannot wrap again.
 LVM_HD uint64_t cred(uint64_t lo, uint64_t hi) {
     const uint64_t hi_hi = hi >> 32, hi_lo = hi & EPS;
-    uint64_t t0 = lo - hi_hi;
-    t0 -= (lo < hi_hi) ? EPS : 0;
-    const uint64_t t1 = (hi_lo << 32) - hi_lo;
-    uint64_t r = t0 + t1;
-    r += (r < t0) ? EPS : 0;
+    uint64_t t0, r;
+    // the borrow and carry as the operations' own flags (no 64-bit compare
+    // against a zero-extended word)
+    const bool borrow = __builtin_sub_overflow(lo, hi_hi, &t0);
+    t0 -= borrow ? EPS : 0;
+    const bool carry = __builtin_add_overflow(t0, (hi_lo << 32) - hi_lo, &r);
+    r += carry ? EPS : 0;
     return ccanon(r);
 }
 

@@ -1198,3 +1198,48 @@ def global_memory_table_air(page_base: int, init_values=None, private: bool = Fa
     data = None if init_values is None else bytes(init_values)
     return _bus_only_air("global_memory", GlobalMemoryCols.NUM_COLUMNS, global_memory_bus_interactions(page_base), k,
                          lambda: page_precomputed(data, k, page_size))
+
+
+# CPU (prover/src/tables/cpu.rs): its one fixed column and the height rules
+# the fixed column depends on (the rest of the CPU AIR is not captured here).
+CPU_TIMESTAMP_STRIDE = 4        # cpu.rs:58
+CPU_MAX_ROWS = (1 << 30) - 1    # cpu.rs:64
+
+
+def cpu_timestamp_precomputed(height: int) -> np.ndarray:
+    """The fixed TIMESTAMP column of a CPU table of `height` rows: row i holds
+    4·(i + 1) (cpu.rs:161-165, :619-622)."""
+    return (CPU_TIMESTAMP_STRIDE * np.arange(1, height + 1, dtype=np.uint64)).reshape(height, 1)
+
+
+def cpu_final_pc_timestamp(total_rows: int) -> int:
+    """cpu.rs:72-74: the last row 4·N writes next_pc at 4·N + 1."""
+    return CPU_TIMESTAMP_STRIDE * total_rows + 1
+
+
+def cpu_timestamp_bases(heights) -> list:
+    """cpu.rs:78-87: table k adds 4·(h_0 + ... + h_{k-1}) to its timestamps."""
+    out, rows = [], 0
+    for h in heights:
+        out.append(CPU_TIMESTAMP_STRIDE * rows)
+        rows += h
+    return out
+
+
+def cpu_table_heights(rows: int, max_rows_cpu: int) -> list:
+    """cpu.rs:116-121: full chunks of max_rows_cpu, the last padded to a power
+    of two (at least 4)."""
+    step = max(1, max_rows_cpu)
+    return [max(4, 1 << (min(rows - s, max_rows_cpu) - 1).bit_length()) for s in range(0, rows, step)]
+
+
+def cpu_validate_heights(heights, max_rows_cpu: int) -> None:
+    """cpu.rs:92-111: each height a power of two in [4, max_rows_cpu], at most
+    CPU_MAX_ROWS rows in total; ValueError otherwise."""
+    total = 0
+    for h in heights:
+        if h <= 0 or h & (h - 1) or not 4 <= h <= max_rows_cpu:
+            raise ValueError(f"CPU table height must be a power of two in [4, {max_rows_cpu}], got {h}")
+        total += h
+    if total > CPU_MAX_ROWS:
+        raise ValueError(f"{total} CPU rows exceed the per-epoch maximum {CPU_MAX_ROWS}")

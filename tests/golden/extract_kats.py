@@ -7,7 +7,7 @@ Sources (read as text from a checkout of the reference at REF):
   crypto/stark/src/tests/grinding_tests.rs               grinding (seed, nonce, g, valid)
   crypto/math/src/field/goldilocks.rs:536-540            two-adicity and 2^32-th root
   prover/src/tables/keccak_rc.rs:97-116, bitwise.rs:199-218,
-  page.rs:416-455                                        static commitments of the
+  page.rs:416-455, cpu_timestamp_commitments.rs         static commitments of the
                                                          preprocessed columns (coset 3,
                                                          blowups 2/4/8)
 
@@ -95,6 +95,17 @@ def static_commitments(rel, fn="fn static_commitment"):
     return out
 
 
+def cpu_timestamp_commitments():
+    """blowup factor -> {log2 height: root} of the CPU table's fixed TIMESTAMP
+    column (4·(row + 1)), from the generated table cpu_timestamp_commitments.rs."""
+    src = read("prover/src/tables/cpu_timestamp_commitments.rs")
+    out = {}
+    for blk in re.finditer(r"\(\s*(\d+),\s*&\[(.*?)\],\s*\),", src, re.S):
+        out[blk.group(1)] = {m.group(1): byte_list(m.group(2))
+                             for m in re.finditer(r"// 2\^(\d+)\s*\[([^\]]*)\]", blk.group(2))}
+    return out
+
+
 def preprocessed_commitments():
     return {"coset_offset": 3,
             "keccak_rc": static_commitments("prover/src/tables/keccak_rc.rs"),
@@ -102,7 +113,9 @@ def preprocessed_commitments():
             # page.rs:416-455: OFFSET + zero INIT of a zero-initialised page, and
             # OFFSET alone (private-input pages; also GLOBAL_MEMORY's)
             "page_zero": static_commitments("prover/src/tables/page.rs", "fn static_zero_page_commitment"),
-            "page_private": static_commitments("prover/src/tables/page.rs", "fn static_private_page_commitment")}
+            "page_private": static_commitments("prover/src/tables/page.rs", "fn static_private_page_commitment"),
+            # cpu.rs:161-165 (TIMESTAMP = 4·(row + 1)), table cpu_timestamp_commitments.rs
+            "cpu_timestamp": cpu_timestamp_commitments()}
 
 
 def main():

@@ -1,7 +1,8 @@
 """Preprocessed tables on the GPU (lvm_coset_lde_split_trees_dev): one coset
 LDE of the whole table and two subset trees. The fixed-columns root equals
 the reference's static commitment (prover/src/tables/keccak_rc.rs:97-116,
-bitwise.rs:199-218; tests/golden/kats.json) and the oracle's subset tree, node
+bitwise.rs:199-218, the CPU table's TIMESTAMP column at every height 2^2..2^19
+from cpu_timestamp_commitments.rs; tests/golden/kats.json) and the oracle's subset tree, node
 for node; the multiplicity-columns root equals the oracle's subset tree over
 the remaining columns; opened rows are the whole LDE rows."""
 import json
@@ -68,3 +69,23 @@ def test_bitwise_split_commitment(gpu, ora, blowup):
                    rng.integers(0, 16, 3000))]
     trace = V.generate_bitwise_trace(lookups)
     _check(gpu, ora, trace, V.BitwiseCols.NUM_PRECOMPUTED, blowup, bytes(KATS["bitwise"][str(blowup)]))
+
+
+@pytest.mark.parametrize("blowup", [2, 4, 8])
+def test_cpu_timestamp_split_commitments(gpu, blowup):
+    """The CPU table's fixed TIMESTAMP column beside a multiplicity-like column,
+    every height 2^2..2^19: the fixed-columns root is the reference's static
+    commitment (cpu_timestamp_commitments.rs)."""
+    want = KATS["cpu_timestamp"][str(blowup)]
+    rng = np.random.default_rng(blowup)
+    print(f"[preprocessed] CPU TIMESTAMP 2^2..2^19, blowup {blowup}", flush=True)
+    for log_h in range(2, 20):
+        n = 1 << log_h
+        trace = np.concatenate([V.cpu_timestamp_precomputed(n), rng.integers(0, 2**63, (n, 1), dtype=np.uint64)], 1)
+        d = gpu.DeviceBuffer.from_array(np.ascontiguousarray(trace))
+        w = gpu.coset_weights_dev(n, OFFSET)
+        h, pre_root, _ = gpu.coset_lde_split_trees_dev(d, n, 2, blowup, w, 1, want_precomputed_nodes=False)
+        d.free()
+        w.free()
+        h.close()
+        assert pre_root == bytes(want[str(log_h)]), (blowup, log_h)
