@@ -1243,3 +1243,100 @@ def cpu_validate_heights(heights, max_rows_cpu: int) -> None:
         total += h
     if total > CPU_MAX_ROWS:
         raise ValueError(f"{total} CPU rows exceed the per-epoch maximum {CPU_MAX_ROWS}")
+
+
+# HINT (prover/src/tables/hint.rs): the receiver of the `hint` ecall, one row
+# per call. The hinted bytes are unconstrained; where they land is not: x12
+# (the output address), x10 (the selector) and x11 (the input address) are
+# read through the memory argument at the ecall timestamp, the selector and
+# both address low limbs are range-checked through ALU LT lookups, the 32
+# bytes go to memory as four 8-byte MEMW writes and are range-checked in
+# pairs, and mu is a bit.
+BUS_ECALL = 19                              # types.rs:310
+HINT_SYSCALL_NUMBER = 2**64 - 1 - 31        # executor/src/vm/instruction/execution.rs:171
+HINT_SELECTOR_BOUND = 3                     # execution.rs:182
+HINT_ADDR_LIMB_BOUND = (1 << 32) - 31       # hint.rs:85
+
+
+class HintCols:
+    """hint.rs:87-117."""
+    TIMESTAMP_0, TIMESTAMP_1, ADDR_OUT_0, ADDR_OUT_1 = 0, 1, 2, 3
+    OUT = tuple(range(4, 36))
+    MU, SEL_0, SEL_1, ADDR_IN_0, ADDR_IN_1 = 36, 37, 38, 39, 40
+    NUM_COLUMNS = 41
+
+
+@dataclass(frozen=True)
+class HintOperation:
+    """hint.rs:120-131."""
+    timestamp: int
+    out_addr: int
+    out_bytes: bytes
+    hint_id: int
+    in_addr: int
+
+
+def generate_hint_trace(ops) -> np.ndarray:
+    """hint.rs:136-162: one row per call in program order (mu = 1), zero rows
+    of padding to a power of two, at least 4."""
+    c = HintCols
+    n = max(4, 1 << max(0, (len(ops) - 1).bit_length()))
+    t = np.zeros((n, c.NUM_COLUMNS), dtype=np.uint64)
+    for r, op in enumerate(ops):
+        if not 0 <= op.timestamp < 2**32:
+            raise ValueError("HINT timestamps fit 32 bits")
+        if len(op.out_bytes) != 32:
+            raise ValueError("a hint writes 32 bytes")
+        for col, v in ((c.TIMESTAMP_0, op.timestamp), (c.ADDR_OUT_0, op.out_addr), (c.SEL_0, op.hint_id),
+                       (c.ADDR_IN_0, op.in_addr)):
+            v &= 2**64 - 1
+            t[r, col], t[r, col + 1] = v & 0xFFFFFFFF, v >> 32
+        t[r, c.OUT[0]:c.OUT[-1] + 1] = np.frombuffer(bytes(op.out_bytes), dtype=np.uint8)
+        t[r, c.MU] = 1
+    return t
+
+
+def hint_bus_interactions():
+    """hint.rs:215-317, in its order: the Ecall receiver, the MEMW register reads
+    of x12, x10, x11, the three ALU LT range checks, the four MEMW writes of the
+    output at out_addr + 8i, the sixteen ARE_BYTES pairs; all with multiplicity mu."""
+    c = HintCols
+    mu = Multiplicity("Column", (c.MU,))
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    k = lambda v: BusValue.of_linear([LinearTerm(v)])  # noqa: E731
+
+    def register_read(reg, lo, hi):
+        # hint.rs:188-205: [old[8], 1, 2*reg, 0, value[8], ts, w2=1, w4=0, w8=0], old == value
+        value = [d(lo), d(hi)] + [k(0)] * 6
+        return value + [k(1), k(2 * reg), k(0)] + value + [d(c.TIMESTAMP_0), d(c.TIMESTAMP_1), k(1), k(0), k(0)]
+
+    its = [BusInteraction.receiver(BUS_ECALL, mu, [d(c.TIMESTAMP_0), d(c.TIMESTAMP_1),
+                                                   k(HINT_SYSCALL_NUMBER & 0xFFFFFFFF), k(HINT_SYSCALL_NUMBER >> 32)])]
+    its += [BusInteraction.sender(BUS_MEMW, mu, register_read(reg, lo, hi))
+            for reg, lo, hi in ((12, c.ADDR_OUT_0, c.ADDR_OUT_1), (10, c.SEL_0, c.SEL_1), (11, c.ADDR_IN_0, c.ADDR_IN_1))]
+    its.append(BusInteraction.sender(BUS_ALU, mu, [BusValue.packed(Packing.DWORD_WL, c.SEL_0), k(HINT_SELECTOR_BOUND),
+                                                   k(0), k(ALU_OP_LT), k(1), k(0)]))
+    its += [BusInteraction.sender(BUS_ALU, mu, [d(lo), k(0), k(HINT_ADDR_LIMB_BOUND), k(0), k(ALU_OP_LT), k(1), k(0)])
+            for lo in (c.ADDR_IN_0, c.ADDR_OUT_0)]
+    for i in range(4):
+        base_lo = BusValue.of_linear([LinearTerm(1, c.ADDR_OUT_0), LinearTerm(8 * i)])
+        its.append(BusInteraction.sender(BUS_MEMW, mu, [k(0), base_lo, d(c.ADDR_OUT_1)]
+                                         + [d(c.OUT[8 * i + b]) for b in range(8)]
+                                         + [d(c.TIMESTAMP_0), d(c.TIMESTAMP_1), k(0), k(0), k(1)]))
+    its += [BusInteraction.sender(BUS_ARE_BYTES, mu, [d(c.OUT[2 * i]), d(c.OUT[2 * i + 1])]) for i in range(16)]
+    return its
+
+
+def hint_table_air() -> AirProgram:
+    """HINT as an AIR: IS_BIT(mu) (hint.rs:355-372) and the LogUp constraints of
+    its 27 interactions."""
+    c = HintCols
+    b = ir.IrBuilder()
+    _emit_is_bit(b, 0, c.MU)
+    its = hint_bus_interactions()
+    n_total = emit_logup_constraints(b, its, 1)
+    prog = b.finish(c.NUM_COLUMNS)
+    pairs, absorbed = split_interactions(len(its))
+    return AirProgram("hint", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
+                      boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
+                      max_degree=max(3 if pairs else 0, 1 + absorbed, 2))

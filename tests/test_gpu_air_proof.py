@@ -6,8 +6,9 @@ verifier restatement accepts the GPU proofs (also after a round trip through
 the bytes).
 Tables: FibonacciMultiColumn (one composition part), Quadratic, a 4-bus
 LogUp table and the VM's EQ, BYTEWISE, IS_WH, LT, STORE, BRANCH, MEMW_REGISTER
-and LOAD tables (two parts), the preprocessed KECCAK_RC, BITWISE and REGISTER
-tables (fixed columns under their own tree), a one-interaction bus table (degree 2: one
+and LOAD tables (two parts), the preprocessed KECCAK_RC, BITWISE, REGISTER,
+PAGE and GLOBAL_MEMORY tables (fixed columns under their own tree), HINT (27
+interactions), a one-interaction bus table (degree 2: one
 part) and a degree-4 bus table with and without interactions (three parts),
 alone and together in one transcript. Bus tables open only their
 accumulated column at the next row."""
@@ -342,3 +343,29 @@ def test_gpu_page_tables_proof(gpu):
     ref = replay_air_tables(tables, 2, 3, 4, 6, 0, b"gpu-page")
     assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
     assert verify_air_tables([page, chain_air, gm], got, 2, 3, 4, 6, 0, b"gpu-page")[0]
+
+
+def test_gpu_hint_table_proof(gpu):
+    """HINT (41 columns, 27 interactions: 14 aux columns) beside the LT table
+    that answers its range checks, with grinding: the GPU proofs equal the
+    oracle replay byte for byte and verify."""
+    from lambda_vm_amd import proof_io, prover
+    from lambda_vm_amd import vm_tables as V
+    from lambda_vm_amd.prover import AirTableProof
+    from tests.test_vm_table_hint import _lt_ops, hint_ops
+
+    print("[gpu-air] HINT 128 x 41 + LT, blowup 4, grinding 4", flush=True)
+    ops = hint_ops(100, 21)
+    hint, lt = V.hint_table_air(), V.lt_table_air()
+    for a in (hint, lt):
+        a.bus_descriptor = A.descriptor(a)
+    tables = [(hint, V.generate_hint_trace(ops)), (lt, V.generate_lt_trace(_lt_ops(ops)))]
+    tables = [(a, np.ascontiguousarray(tr, dtype=np.uint64)) for a, tr in tables]
+    opts = prover.ProofOptions(blowup=4, coset_offset=3, fri_final_log_degree=2, num_queries=8, grinding_factor=4)
+    dev = [(a, gpu.DeviceBuffer.from_array(tr), tr.shape[0]) for a, tr in tables]
+    got = prover.prove_air_tables(dev, opts, label=b"gpu-hint")
+    for _, d, _ in dev:
+        d.free()
+    ref = replay_air_tables(tables, 4, 3, 2, 8, 4, b"gpu-hint")
+    assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
+    assert verify_air_tables([hint, lt], got, 4, 3, 2, 8, 4, b"gpu-hint")[0]
