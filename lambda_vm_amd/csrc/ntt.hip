@@ -115,9 +115,13 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t bx, ui
     constexpr int R1 = 1 << LOG_R1, R2 = 1 << LOG_R2, R = R1 * R2, LOG_R = LOG_R1 + LOG_R2;
     // C columns per tile: 16 (128-byte row segments), or 8 for narrow tables
     // whose last 16-column group would leave most lanes idle
-    constexpr int TPC = R1 > R2 ? R1 : R2;  // threads per column
+    // TPC = min(R1, R2) threads per column; each holds R / TPC elements in
+    // both rounds: D1 = R2 / TPC size-R1 DFTs in round 1, D2 = R1 / TPC
+    // size-R2 DFTs in round 2 (for R1 != R2 no thread idles in either round)
+    constexpr int TPC = R1 < R2 ? R1 : R2;
+    constexpr int D1 = R2 / TPC, D2 = R1 / TPC;
     constexpr int TILES = 256 / (C * TPC);
-    static_assert(TILES >= 1, "tile too large");
+    static_assert(TILES >= 1 && D1 * R1 == D2 * R2, "tile shape");
 
     __shared__ uint64_t s_data[TILES][R * C];
     __shared__ uint64_t s_pre[TILES][R];
@@ -156,27 +160,28 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t bx, ui
         }
     }
 
-    uint64_t x[TPC > R1 ? TPC : R1];
-    uint64_t xin[R1];
+    uint64_t x[R1 > R2 ? R1 : R2];
+    uint64_t xin[D1][R1];
     const bool pre_needed = a.weights != nullptr || a.shift_log != 0;
-    // Input element i of a loading thread (tc < R2) is logical row
-    // rowA + (i << (log_s + LOG_R2)) with disjoint bit fields, so with
-    // bit-reversed input rows (over in_bitrev_bits = log_msub bits) its
-    // physical row is bitrev(rowA) + bitrev_R1(i) << (bits - log_s - LOG_R):
-    // either way a per-thread base plus a uniform stride per element, and the
-    // R1 loads are issued back to back. A lane outside the table reads row 0,
-    // column 0 (its results are never stored).
+    // Input element (d, i) of a thread is t = t0 + R2*i with t0 = tc + TPC*d,
+    // logical row rowA + ((TPC*d + R2*i) << log_s) with rowA = row0 + (tc << log_s):
+    // disjoint bit fields, so with bit-reversed input rows (over in_bitrev_bits
+    // = log_msub bits) the physical row is bitrev(rowA) + bitrev_R(TPC*d + R2*i)
+    // << (bits - log_s - LOG_R). Either way a per-thread base plus a uniform
+    // stride per element, and the loads are issued back to back. A lane outside
+    // the table reads row 0, column 0 (its results are never stored).
     const uint32_t bits = a.in_bitrev_bits;
-    const bool loads = tc < R2;
     const uint64_t rowA = row0 + ((uint64_t)tc << log_s);
     const uint64_t physA = bits ? (uint64_t)bitrev((uint32_t)rowA, bits) : rowA;
-    const gmem_u64 *src = LVM_GLOBAL_CONST(a.in) + (c_ok && loads ? physA * a.in_pitch + col : 0);
-    const uint64_t in_step = (uint64_t)a.in_pitch << (bits ? bits - log_s - LOG_R : log_s + LOG_R2);
-    if (loads) {
+    const gmem_u64 *src = LVM_GLOBAL_CONST(a.in) + (c_ok ? physA * a.in_pitch + col : 0);
+    const uint64_t in_step = (uint64_t)a.in_pitch << (bits ? bits - log_s - LOG_R : log_s);
+#pragma unroll
+    for (int d = 0; d < D1; d++) {
 #pragma unroll
         for (int i = 0; i < R1; i++) {
-            const uint64_t sel = bits ? bitrev_small(i, LOG_R1) : (uint64_t)i;
-            xin[i] = src[sel * in_step];
+            const uint32_t u = (uint32_t)(TPC * d + R2 * i);
+            const uint64_t sel = bits ? bitrev_small(u, LOG_R) : (uint64_t)u;
+            xin[d][i] = src[sel * in_step];
         }
     }
 
@@ -200,37 +205,41 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t bx, ui
     }
     __syncthreads();
 
-    // ---- round 1: size-R1 DFT over t = t0 + R2*i, then w_R^(t0*k_a) ----
-    if (tc < R2) {
-        const int t0 = tc;
+    // ---- round 1: size-R1 DFTs over t = t0 + R2*i, then w_R^(t0*k_a) ----
+#pragma unroll
+    for (int d = 0; d < D1; d++) {
+        const int t0 = tc + TPC * d;
 #pragma unroll
         for (int i = 0; i < R1; i++)
-            x[i] = pre_needed ? cmul(xin[i], s_pre[tl][t0 + R2 * i]) : a.in_canonical ? xin[i] : ccanon(xin[i]);
+            x[i] = pre_needed ? cmul(xin[d][i], s_pre[tl][t0 + R2 * i])
+                              : a.in_canonical ? xin[d][i] : ccanon(xin[d][i]);
         dft_small<R1>(x, a.inverse);
         // position i holds frequency k_a = bitrev_R1(i)
         if constexpr (R2 > 1) {
 #pragma unroll
-            for (int i = 1; i < R1; i++) x[i] = cmul(x[i], s_tw[t0 * (int)bitrev((uint32_t)i, LOG_R1)]);
+            for (int i = 1; i < R1; i++) x[i] = cmul(x[i], s_tw[t0 * (int)bitrev_small(i, LOG_R1)]);
         }
 #pragma unroll
         for (int i = 0; i < R1; i++) s_data[tl][(t0 + R2 * i) * C + c] = x[i];
     }
     __syncthreads();
 
-    // ---- round 2: size-R2 DFT over t0 at t = R2*t1 + t0 ----
-    if (tc < R1) {
-        const int t1 = tc;
+    // ---- round 2: size-R2 DFTs over t0 at t = R2*t1 + t0 ----
+    uint64_t obase = 0;
+    if (a.out_coset_rows) obase = (uint64_t)bitrev(coset, a.coset_bits) * a.out_coset_rows;
+    // output q = R2*t1 + i is row obase + row0 + (q << log_s): a base and a uniform stride
+    gmem_u64 *dst = LVM_GLOBAL(a.out) + (obase + row0 + ((uint64_t)(R2 * tc) << log_s)) * a.out_pitch + col;
+    const uint64_t out_step = (uint64_t)a.out_pitch << log_s;
+#pragma unroll
+    for (int d = 0; d < D2; d++) {
+        const int t1 = tc + TPC * d;
 #pragma unroll
         for (int i = 0; i < R2; i++) x[i] = s_data[tl][(R2 * t1 + i) * C + c];
         dft_small<R2>(x, a.inverse);
         if (c_ok) {
-            uint64_t obase = 0;
-            if (a.out_coset_rows) obase = (uint64_t)bitrev(coset, a.coset_bits) * a.out_coset_rows;
-            // output q = R2*t1 + i is row obase + row0 + (q << log_s): a base and a uniform stride
-            gmem_u64 *dst = LVM_GLOBAL(a.out) + (obase + row0 + ((uint64_t)(R2 * t1) << log_s)) * a.out_pitch + col;
-            const uint64_t out_step = (uint64_t)a.out_pitch << log_s;
 #pragma unroll
-            for (int i = 0; i < R2; i++) dst[i * out_step] = log_s ? cmul(x[i], s_post[tl][R2 * t1 + i]) : x[i];
+            for (int i = 0; i < R2; i++)
+                dst[(R2 * TPC * d + i) * out_step] = log_s ? cmul(x[i], s_post[tl][R2 * t1 + i]) : x[i];
         }
     }
     __syncthreads();  // the LDS tile and factors are rewritten by the next coset
@@ -261,14 +270,14 @@ __global__ __launch_bounds__(256) void ntt_pass_batched_kernel(const PassArgs *_
 
 template <int A, int B>
 static constexpr uint32_t pass_tiles() {
-    constexpr int TPC = (1 << A) > (1 << B) ? (1 << A) : (1 << B);
+    constexpr int TPC = (1 << A) < (1 << B) ? (1 << A) : (1 << B);
     return 256 / (BATCH_C * TPC);
 }
 
 // pass shapes: (LOG_R1, LOG_R2) for r = 1..8
 template <int A, int B, int C>
 static void launch_one_c(const PassArgs &a0, uint32_t cosets, hipStream_t st) {
-    constexpr int TPC = (1 << A) > (1 << B) ? (1 << A) : (1 << B);
+    constexpr int TPC = (1 << A) < (1 << B) ? (1 << A) : (1 << B);
     constexpr int TILES = 256 / (C * TPC);
     PassArgs a = a0;
     // all cosets of a tile come from one block, which reads the tile once
