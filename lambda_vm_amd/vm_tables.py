@@ -1340,3 +1340,139 @@ def hint_table_air() -> AirProgram:
     return AirProgram("hint", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
                       boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
                       max_degree=max(3 if pairs else 0, 1 + absorbed, 2))
+
+
+# KECCAK core (prover/src/tables/keccak.rs): one row per keccak-f call. It
+# receives the ecall, reads x10 (the state address), hands the input state to
+# the round chip and takes its output back on the Keccak bus, reads and writes
+# the 25 lanes through MEMW at state_addr + 8·lane, and range-checks the lane
+# addresses (IS_WH + IS_HALFWORD), the alignment (BYTE_ALU AND 7 = 0) and the
+# address bytes (ARE_BYTES).
+BUS_KECCAK = 22                        # types.rs:318
+KECCAK_SYSCALL_NUMBER = 2**64 - 1 - 1  # executor/src/vm/instruction/execution.rs:53
+
+
+class KeccakCols:
+    """keccak.rs:31-76: timestamp (2), addr (8 bytes), input and output state
+    (200 bytes each, lane x + 5y at 8·(x + 5y)), state_ptr (25 × DWordWHH), mu."""
+    TIMESTAMP_0, TIMESTAMP_1, ADDR = 0, 1, 2
+    INPUT_STATE = 10
+    OUTPUT_STATE = 210
+    STATE_PTR = 410
+    MU = 485
+    NUM_COLUMNS = 486
+
+    @staticmethod
+    def state_ptr(lane: int, limb: int) -> int:
+        return KeccakCols.STATE_PTR + 3 * lane + limb
+
+
+@dataclass(frozen=True)
+class KeccakOperation:
+    """keccak.rs:82-88: lanes indexed x + 5y."""
+    timestamp: int
+    state_addr: int
+    input: tuple
+    output: tuple
+
+
+def keccak_is_wh_keys(ops) -> list:
+    """keccak.rs:92-104: the IS_WH key of every lane address of every call."""
+    return [is_wh_key(op.state_addr + 8 * lane) for op in ops for lane in range(25)]
+
+
+def generate_keccak_trace(ops) -> np.ndarray:
+    """keccak.rs:110-167: one row per call (mu = 1); padding rows hold only
+    state_ptr[lane] half_lo = 8·lane."""
+    c = KeccakCols
+    n = max(4, 1 << max(0, (len(ops) - 1).bit_length()))
+    t = np.zeros((n, c.NUM_COLUMNS), dtype=np.uint64)
+
+    def le_bytes(v):
+        return np.frombuffer(int(v).to_bytes(8, "little"), dtype=np.uint8)
+
+    for r, op in enumerate(ops):
+        if op.state_addr + 8 * 24 >= 2**64:
+            raise ValueError("keccak state address range overflows")
+        t[r, c.TIMESTAMP_0], t[r, c.TIMESTAMP_1] = op.timestamp & 0xFFFFFFFF, op.timestamp >> 32
+        t[r, c.ADDR:c.ADDR + 8] = le_bytes(op.state_addr)
+        for lane in range(25):
+            t[r, c.INPUT_STATE + 8 * lane:c.INPUT_STATE + 8 * lane + 8] = le_bytes(op.input[lane])
+            t[r, c.OUTPUT_STATE + 8 * lane:c.OUTPUT_STATE + 8 * lane + 8] = le_bytes(op.output[lane])
+            ptr = op.state_addr + 8 * lane
+            t[r, c.state_ptr(lane, 0)], t[r, c.state_ptr(lane, 1)] = ptr & 0xFFFF, (ptr >> 16) & 0xFFFF
+            t[r, c.state_ptr(lane, 2)] = ptr >> 32
+        t[r, c.MU] = 1
+    for r in range(len(ops), n):
+        for lane in range(25):
+            t[r, c.state_ptr(lane, 0)] = 8 * lane
+    return t
+
+
+def keccak_bus_interactions():
+    """keccak.rs:175-440, in its order (84 interactions, all with multiplicity
+    mu): Ecall receiver; MEMW read of x10 with the address from its bytes;
+    Keccak send [ts, 0, input bytes] and receive [ts, 24, output bytes]; per
+    lane IS_WH[word_hi, half_mid] and IS_HALFWORD[half_lo]; BYTE_ALU[AND,
+    addr[0], 7, 0]; four ARE_BYTES pairs of the address bytes; per lane the
+    MEMW read-write [input bytes, 0, address, output bytes, ts, 0, 0, 1]."""
+    c = KeccakCols
+    mu = Multiplicity("Column", (c.MU,))
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    k = lambda v: BusValue.of_linear([LinearTerm(v)])  # noqa: E731
+    ts = [d(c.TIMESTAMP_0), d(c.TIMESTAMP_1)]
+    its = [BusInteraction.receiver(BUS_ECALL, mu, ts + [k(KECCAK_SYSCALL_NUMBER & 0xFFFFFFFF),
+                                                         k(KECCAK_SYSCALL_NUMBER >> 32)])]
+    addr_lo = BusValue.of_linear([LinearTerm(256 ** i, c.ADDR + i) for i in range(4)])
+    addr_hi = BusValue.of_linear([LinearTerm(256 ** i, c.ADDR + 4 + i) for i in range(4)])
+    word = [addr_lo, addr_hi] + [k(0)] * 6
+    its.append(BusInteraction.sender(BUS_MEMW, mu, word + [k(1), k(20), k(0)] + word + ts + [k(1), k(0), k(0)]))
+    its.append(BusInteraction.sender(BUS_KECCAK, mu, ts + [k(0)] + [d(c.INPUT_STATE + i) for i in range(200)]))
+    its.append(BusInteraction.receiver(BUS_KECCAK, mu, ts + [k(24)] + [d(c.OUTPUT_STATE + i) for i in range(200)]))
+    for lane in range(25):
+        its.append(BusInteraction.sender(BUS_IS_WH, mu, [d(c.state_ptr(lane, 2)), d(c.state_ptr(lane, 1))]))
+        its.append(BusInteraction.sender(BUS_IS_HALFWORD, mu, [d(c.state_ptr(lane, 0))]))
+    its.append(BusInteraction.sender(BUS_BYTE_ALU, mu, [k(ALU_OP_AND), d(c.ADDR), k(7), k(0)]))
+    its += [BusInteraction.sender(BUS_ARE_BYTES, mu, [d(c.ADDR + 2 * i), d(c.ADDR + 2 * i + 1)]) for i in range(4)]
+    for lane in range(25):
+        lo = BusValue.of_linear([LinearTerm(1, c.state_ptr(lane, 0)), LinearTerm(1 << 16, c.state_ptr(lane, 1))])
+        its.append(BusInteraction.sender(BUS_MEMW, mu, [d(c.INPUT_STATE + 8 * lane + b) for b in range(8)]
+                                         + [k(0), lo, d(c.state_ptr(lane, 2))]
+                                         + [d(c.OUTPUT_STATE + 8 * lane + b) for b in range(8)]
+                                         + ts + [k(0), k(0), k(1)]))
+    return its
+
+
+def keccak_table_air() -> AirProgram:
+    """KECCAK as an AIR (keccak.rs:454-524): 0-49 the gated ADD carry pair of
+    addr (DWordBL) + 8·lane = state_ptr[lane] (DWordWHH) per lane, each
+    mu·carry·(1 - carry); 50 mu·carry_1 of addr + 192 = state_ptr[24] (no
+    overflow); 51 mu a bit; then the LogUp constraints of its 84 interactions."""
+    c = KeccakCols
+    b = ir.IrBuilder()
+    inv = b.const_base(INV_SHIFT_32)
+    one = b.const_base(1)
+    m = lambda col: b.main(0, col)  # noqa: E731
+    mu = m(c.MU)
+    addr_lo = m(c.ADDR) + m(c.ADDR + 1) * b.const_base(256) + m(c.ADDR + 2) * b.const_base(65536) \
+        + m(c.ADDR + 3) * b.const_base(1 << 24)
+    addr_hi = m(c.ADDR + 4) + m(c.ADDR + 5) * b.const_base(256) + m(c.ADDR + 6) * b.const_base(65536) \
+        + m(c.ADDR + 7) * b.const_base(1 << 24)
+    carry1_top = None
+    for lane in range(25):
+        ptr_lo = m(c.state_ptr(lane, 0)) + m(c.state_ptr(lane, 1)) * b.const_base(1 << 16)
+        ptr_hi = m(c.state_ptr(lane, 2))
+        carry0 = (addr_lo + b.const_base(8 * lane) - ptr_lo) * inv
+        carry1 = (addr_hi + carry0 - ptr_hi) * inv
+        b.emit(2 * lane, mu * carry0 * (one - carry0))
+        b.emit(2 * lane + 1, mu * carry1 * (one - carry1))
+        carry1_top = carry1
+    b.emit(50, mu * carry1_top)
+    b.emit(51, mu * (one - mu))
+    its = keccak_bus_interactions()
+    n_total = emit_logup_constraints(b, its, 52)
+    prog = b.finish(c.NUM_COLUMNS)
+    pairs, absorbed = split_interactions(len(its))
+    return AirProgram("keccak", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
+                      boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
+                      max_degree=max(3 if pairs else 0, 1 + absorbed, 3))

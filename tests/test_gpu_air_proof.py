@@ -369,3 +369,30 @@ def test_gpu_hint_table_proof(gpu):
     ref = replay_air_tables(tables, 4, 3, 2, 8, 4, b"gpu-hint")
     assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
     assert verify_air_tables([hint, lt], got, 4, 3, 2, 8, 4, b"gpu-hint")[0]
+
+
+def test_gpu_keccak_table_proof(gpu):
+    """KECCAK core (486 columns, 52 constraints, 84 interactions: 42 aux Fp3
+    columns) beside the IS_WH table that answers its lane-address range
+    checks: the GPU proofs equal the oracle replay byte for byte and verify."""
+    from lambda_vm_amd import proof_io, prover
+    from lambda_vm_amd import vm_tables as V
+    from lambda_vm_amd.prover import AirTableProof
+    from tests.test_vm_table_keccak import keccak_ops
+
+    print("[gpu-air] KECCAK 64 x 486 + IS_WH, blowup 4", flush=True)
+    ops = keccak_ops(50, 31)
+    kec, iswh = V.keccak_table_air(), V.is_wh_table_air()
+    for a in (kec, iswh):
+        a.bus_descriptor = A.descriptor(a)
+    tables = [(kec, V.generate_keccak_trace(ops)),
+              (iswh, V.generate_is_wh_trace(V.dedup_is_wh_rows(V.keccak_is_wh_keys(ops))))]
+    tables = [(a, np.ascontiguousarray(tr, dtype=np.uint64)) for a, tr in tables]
+    opts = prover.ProofOptions(blowup=4, coset_offset=3, fri_final_log_degree=2, num_queries=8)
+    dev = [(a, gpu.DeviceBuffer.from_array(tr), tr.shape[0]) for a, tr in tables]
+    got = prover.prove_air_tables(dev, opts, label=b"gpu-keccak")
+    for _, d, _ in dev:
+        d.free()
+    ref = replay_air_tables(tables, 4, 3, 2, 8, 0, b"gpu-keccak")
+    assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
+    assert verify_air_tables([kec, iswh], got, 4, 3, 2, 8, 0, b"gpu-keccak")[0]
