@@ -1476,3 +1476,120 @@ def keccak_table_air() -> AirProgram:
     return AirProgram("keccak", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
                       boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
                       max_degree=max(3 if pairs else 0, 1 + absorbed, 3))
+
+
+# MEMW_A (prover/src/tables/memw_aligned.rs): aligned memory and register
+# accesses whose cells share one old timestamp. Per access it range-checks the
+# address span (IS_HALFWORD), asserts old_ts < ts (ALU LT), consumes the old
+# token and produces the new one on the Memory bus for each cell the width
+# covers, and receives the MEMW read or write.
+@dataclass(frozen=True)
+class MemwOperation:
+    """memw.rs:105-126 (aligned: one old timestamp). value/old: eight cells —
+    bytes for memory, 32-bit register halves (two cells, width 2) for registers."""
+    is_register: bool
+    base_address: int
+    value: tuple
+    timestamp: int
+    width: int
+    is_read: bool
+    old: tuple
+    old_timestamp: int
+
+    def write_flags(self):
+        """memw.rs:165-173."""
+        return {1: (0, 0, 0), 2: (1, 0, 0), 4: (0, 1, 0), 8: (0, 0, 1)}.get(self.width, (0, 0, 0))
+
+
+class MemwAlignedCols:
+    """memw_aligned.rs:53-83."""
+    IS_REGISTER = 0
+    BASE_ADDRESS = (1, 2, 3)
+    VALUE = tuple(range(4, 12))
+    TIMESTAMP_0, TIMESTAMP_1, WRITE2, WRITE4, WRITE8 = 12, 13, 14, 15, 16
+    OLD = tuple(range(17, 25))
+    OLD_TIMESTAMP_0, OLD_TIMESTAMP_1, MU_READ, MU_WRITE = 25, 26, 27, 28
+    NUM_COLUMNS = 29
+
+
+def generate_memw_aligned_trace(ops) -> np.ndarray:
+    """memw_aligned.rs:91-132: one row per access, padded with zero rows to a
+    power of two (at least 4)."""
+    c = MemwAlignedCols
+    n = max(4, 1 << max(0, (len(ops) - 1).bit_length()))
+    t = np.zeros((n, c.NUM_COLUMNS), dtype=np.uint64)
+    for r, op in enumerate(ops):
+        ba = op.base_address & (2**64 - 1)
+        t[r, c.IS_REGISTER] = int(op.is_register)
+        t[r, c.BASE_ADDRESS[0]], t[r, c.BASE_ADDRESS[1]], t[r, c.BASE_ADDRESS[2]] = \
+            ba & 0xFFFF, (ba >> 16) & 0xFFFF, ba >> 32
+        t[r, c.VALUE[0]:c.VALUE[-1] + 1] = list(op.value)
+        t[r, c.TIMESTAMP_0], t[r, c.TIMESTAMP_1] = op.timestamp & 0xFFFFFFFF, op.timestamp >> 32
+        t[r, c.WRITE2], t[r, c.WRITE4], t[r, c.WRITE8] = op.write_flags()
+        t[r, c.OLD[0]:c.OLD[-1] + 1] = list(op.old)
+        t[r, c.OLD_TIMESTAMP_0], t[r, c.OLD_TIMESTAMP_1] = op.old_timestamp & 0xFFFFFFFF, op.old_timestamp >> 32
+        t[r, c.MU_READ], t[r, c.MU_WRITE] = int(op.is_read), int(not op.is_read)
+    return t
+
+
+def memw_aligned_bus_interactions():
+    """memw_aligned.rs:138-661, in its order (20): IS_HALFWORD[ba0 + w2 + 3 w4 +
+    7 w8] and ALU[old_ts, ts, LT, 1, 0] with mu_read + mu_write; the Memory
+    tokens of cell 0 (mu_read + mu_write), cell 1 (w2 + w4 + w8), cells 2-3
+    (w4 + w8) and 4-7 (w8), each the old token [is_register, address + i,
+    old_ts, old_i] sent and the new one [.., ts, value_i] received; the MEMW read
+    receiver [old, is_register, address, value, ts, w2, w4, w8] (mu_read) and
+    write receiver [is_register, address, value, ts, w2, w4, w8] (mu_write)."""
+    c = MemwAlignedCols
+    mu_sum = Multiplicity("Sum", (c.MU_READ, c.MU_WRITE))
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    k = lambda v: BusValue.of_linear([LinearTerm(v)])  # noqa: E731
+    its = [BusInteraction.sender(BUS_IS_HALFWORD, mu_sum, [BusValue.of_linear(
+               [LinearTerm(1, c.BASE_ADDRESS[0]), LinearTerm(1, c.WRITE2), LinearTerm(3, c.WRITE4),
+                LinearTerm(7, c.WRITE8)])]),
+           BusInteraction.sender(BUS_ALU, mu_sum, [BusValue.packed(Packing.DWORD_WL, c.OLD_TIMESTAMP_0),
+                                                   BusValue.packed(Packing.DWORD_WL, c.TIMESTAMP_0),
+                                                   k(ALU_OP_LT), k(1), k(0)])]
+    hi = d(c.BASE_ADDRESS[2])
+
+    def lo(i):
+        terms = [LinearTerm(1, c.BASE_ADDRESS[0]), LinearTerm(1 << 16, c.BASE_ADDRESS[1])]
+        return BusValue.of_linear(terms + ([LinearTerm(i)] if i else []))
+
+    cover = [mu_sum, Multiplicity("Sum3", (c.WRITE2, c.WRITE4, c.WRITE8)),
+             Multiplicity("Sum", (c.WRITE4, c.WRITE8)), Multiplicity("Sum", (c.WRITE4, c.WRITE8))] + \
+        [Multiplicity("Column", (c.WRITE8,))] * 4
+    for i in range(8):
+        its.append(BusInteraction.sender(BUS_MEMORY, cover[i], [d(c.IS_REGISTER), lo(i), hi, d(c.OLD_TIMESTAMP_0),
+                                                                d(c.OLD_TIMESTAMP_1), d(c.OLD[i])]))
+        its.append(BusInteraction.receiver(BUS_MEMORY, cover[i], [d(c.IS_REGISTER), lo(i), hi, d(c.TIMESTAMP_0),
+                                                                  d(c.TIMESTAMP_1), d(c.VALUE[i])]))
+    tail = [d(c.TIMESTAMP_0), d(c.TIMESTAMP_1), d(c.WRITE2), d(c.WRITE4), d(c.WRITE8)]
+    access = [d(c.IS_REGISTER), lo(0), hi] + [d(v) for v in c.VALUE] + tail
+    its.append(BusInteraction.receiver(BUS_MEMW, Multiplicity("Column", (c.MU_READ,)),
+                                       [d(o) for o in c.OLD] + access))
+    its.append(BusInteraction.receiver(BUS_MEMW, Multiplicity("Column", (c.MU_WRITE,)), list(access)))
+    return its
+
+
+def memw_aligned_table_air() -> AirProgram:
+    """MEMW_A as an AIR (memw_aligned.rs:671-699): 0 mu_sum a bit, 1 w2·(1 -
+    mu_sum), 2-3 mu_read and mu_write bits, 4-6 the width flags bits, 7 their
+    sum a bit; then the LogUp constraints of its 20 interactions."""
+    c = MemwAlignedCols
+    b = ir.IrBuilder()
+    one = b.const_base(1)
+    mu_sum = b.main(0, c.MU_READ) + b.main(0, c.MU_WRITE)
+    w2 = b.main(0, c.WRITE2) + b.main(0, c.WRITE4) + b.main(0, c.WRITE8)
+    b.emit(0, mu_sum * (one - mu_sum))
+    b.emit(1, w2 * (one - mu_sum))
+    for idx, col in enumerate((c.MU_READ, c.MU_WRITE, c.WRITE2, c.WRITE4, c.WRITE8)):
+        _emit_is_bit(b, 2 + idx, col)
+    b.emit(7, w2 * (one - w2))
+    its = memw_aligned_bus_interactions()
+    n_total = emit_logup_constraints(b, its, 8)
+    prog = b.finish(c.NUM_COLUMNS)
+    pairs, absorbed = split_interactions(len(its))
+    return AirProgram("memw_aligned", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False),
+                      [0] * n_total, boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
+                      max_degree=max(3 if pairs else 0, 1 + absorbed, 2))

@@ -396,3 +396,29 @@ def test_gpu_keccak_table_proof(gpu):
     ref = replay_air_tables(tables, 4, 3, 2, 8, 0, b"gpu-keccak")
     assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
     assert verify_air_tables([kec, iswh], got, 4, 3, 2, 8, 0, b"gpu-keccak")[0]
+
+
+def test_gpu_memw_aligned_page_proof(gpu):
+    """MEMW_A (aligned accesses replayed by a memory model) with the PAGE whose
+    tokens close its memory chains: the GPU proofs equal the oracle replay byte
+    for byte and verify."""
+    from lambda_vm_amd import proof_io, prover
+    from lambda_vm_amd import vm_tables as V
+    from lambda_vm_amd.prover import AirTableProof
+    from tests.test_vm_table_memw_aligned import BASE, INIT, PAGE, access_chain
+
+    print("[gpu-air] MEMW_A 256 x 29 + PAGE 1024 x 5, blowup 4", flush=True)
+    ops, _, cells = access_chain(200, 41, INIT)
+    memw, page = V.memw_aligned_table_air(), V.page_table_air(BASE, INIT, page_size=PAGE)
+    for a in (memw, page):
+        a.bus_descriptor = A.descriptor(a)
+    tables = [(memw, V.generate_memw_aligned_trace(ops)), (page, V.generate_page_trace(BASE, INIT, cells, PAGE))]
+    tables = [(a, np.ascontiguousarray(tr, dtype=np.uint64)) for a, tr in tables]
+    opts = prover.ProofOptions(blowup=4, coset_offset=3, fri_final_log_degree=2, num_queries=8)
+    dev = [(a, gpu.DeviceBuffer.from_array(tr), tr.shape[0]) for a, tr in tables]
+    got = prover.prove_air_tables(dev, opts, label=b"gpu-memw-a")
+    for _, d, _ in dev:
+        d.free()
+    ref = replay_air_tables(tables, 4, 3, 2, 8, 0, b"gpu-memw-a")
+    assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
+    assert verify_air_tables([memw, page], got, 4, 3, 2, 8, 0, b"gpu-memw-a")[0]
