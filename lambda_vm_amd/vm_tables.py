@@ -1593,3 +1593,170 @@ def memw_aligned_table_air() -> AirProgram:
     return AirProgram("memw_aligned", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False),
                       [0] * n_total, boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
                       max_degree=max(3 if pairs else 0, 1 + absorbed, 2))
+
+
+# MUL (prover/src/tables/mul.rs): 64 x 64 -> 128-bit products, signed or
+# unsigned per operand, deduplicated with lo/hi multiplicities. The product
+# is pinned by four raw convolution columns (sums of 16-bit limb products of
+# the sign-extended operands) whose carries into the 32-bit result words are
+# virtual and range-checked through IS_B20.
+ALU_OP_MUL = 7        # types.rs:569
+MUL_SIGN_FILL = 0xFFFF
+
+
+def _inv2(k: int) -> int:
+    return pow(2, P - 1 - k, P)
+
+
+class MulCols:
+    """mul.rs:57-125."""
+    LHS = (0, 1, 2, 3)
+    LHS_SIGNED = 4
+    RHS = (5, 6, 7, 8)
+    RHS_SIGNED = 9
+    LO = (10, 11, 12, 13)
+    HI = (14, 15, 16, 17)
+    LHS_IS_NEGATIVE, RHS_IS_NEGATIVE = 18, 19
+    RAW_PRODUCT = (20, 21, 22, 23)
+    MU_LO, MU_HI = 24, 25
+    NUM_COLUMNS = 26
+
+
+@dataclass(frozen=True)
+class MulOperation:
+    """mul.rs:149-246."""
+    lhs: int
+    lhs_signed: bool
+    rhs: int
+    rhs_signed: bool
+
+    def _value(self, v, signed):
+        v &= 2**64 - 1
+        return v - 2**64 if signed and v >> 63 else v
+
+    def product(self):
+        p = (self._value(self.lhs, self.lhs_signed) * self._value(self.rhs, self.rhs_signed)) % 2**128
+        return p & (2**64 - 1), p >> 64
+
+    def lhs_is_negative(self):
+        return self.lhs_signed and (self.lhs & (2**64 - 1)) >> 63 == 1
+
+    def rhs_is_negative(self):
+        return self.rhs_signed and (self.rhs & (2**64 - 1)) >> 63 == 1
+
+    @staticmethod
+    def _extended(v, neg):
+        fill = MUL_SIGN_FILL if neg else 0
+        return [(v >> (16 * i)) & 0xFFFF for i in range(4)] + [fill] * 4
+
+    def raw_products(self):
+        """mul.rs:204-231: raw[i] = sum_{k<2} 2^(16k) sum_j lhs_ext[j] rhs_ext[2i+k-j], mod 2^64."""
+        a = self._extended(self.lhs & (2**64 - 1), self.lhs_is_negative())
+        b = self._extended(self.rhs & (2**64 - 1), self.rhs_is_negative())
+        out = []
+        for i in range(4):
+            s = sum((a[j] * b[2 * i + k - j]) << (16 * k) for k in range(2) for j in range(2 * i + k + 1))
+            out.append(s % 2**64)
+        return out
+
+
+def dedup_mul_rows(ops):
+    """mul.rs:260-295: (operation, wants_hi) pairs merged per operation with lo/hi
+    counts, split at MU_MAX; rows follow first occurrence (the reference's map
+    order is arbitrary)."""
+    counts = {}
+    for op, wants_hi in ops:
+        lo, hi = counts.get(op, (0, 0))
+        counts[op] = (lo, hi + 1) if wants_hi else (lo + 1, hi)
+    rows = []
+    for op, (lo, hi) in counts.items():
+        while lo > 0 or hi > 0:
+            rl, rh = min(lo, MU_MAX), min(hi, MU_MAX)
+            lo, hi = lo - rl, hi - rh
+            rows.append((op, rl, rh))
+    return rows
+
+
+def generate_mul_trace(ops) -> np.ndarray:
+    """mul.rs:330-383."""
+    c = MulCols
+    rows = dedup_mul_rows(ops)
+    n = max(4, 1 << max(0, (len(rows) - 1).bit_length()))
+    t = np.zeros((n, c.NUM_COLUMNS), dtype=np.uint64)
+    for r, (op, mu_lo, mu_hi) in enumerate(rows):
+        lo, hi = op.product()
+        for cols, v in ((c.LHS, op.lhs & (2**64 - 1)), (c.RHS, op.rhs & (2**64 - 1)), (c.LO, lo), (c.HI, hi)):
+            for i, col in enumerate(cols):
+                t[r, col] = (v >> (16 * i)) & 0xFFFF
+        t[r, c.LHS_SIGNED], t[r, c.RHS_SIGNED] = int(op.lhs_signed), int(op.rhs_signed)
+        t[r, c.LHS_IS_NEGATIVE], t[r, c.RHS_IS_NEGATIVE] = int(op.lhs_is_negative()), int(op.rhs_is_negative())
+        t[r, c.RAW_PRODUCT[0]:c.RAW_PRODUCT[-1] + 1] = op.raw_products()
+        t[r, c.MU_LO], t[r, c.MU_HI] = mu_lo, mu_hi
+    return t
+
+
+def mul_bus_interactions():
+    """mul.rs:390-728, in its order (25): MSB16 of each operand's top half (with
+    its signed flag), ARE_BYTES[mu_lo, mu_hi] once, IS_HALFWORD of the eight
+    operand halves and the eight result halves and IS_B20 of the four virtual
+    carries (all with mu_lo + mu_hi), then the ALU receivers of the lo result
+    (mu_lo) and the hi result (mu_hi, flags + 128)."""
+    c = MulCols
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    row = Multiplicity("Sum", (c.MU_LO, c.MU_HI))
+    its = [BusInteraction.sender(BUS_MSB16, Multiplicity("Column", (c.LHS_SIGNED,)), [d(c.LHS[3]), d(c.LHS_IS_NEGATIVE)]),
+           BusInteraction.sender(BUS_MSB16, Multiplicity("Column", (c.RHS_SIGNED,)), [d(c.RHS[3]), d(c.RHS_IS_NEGATIVE)]),
+           BusInteraction.sender(BUS_ARE_BYTES, ONE, [d(c.MU_LO), d(c.MU_HI)])]
+    its += [BusInteraction.sender(BUS_IS_HALFWORD, row, [d(col)]) for col in c.LHS + c.RHS + c.LO + c.HI]
+    # carry[i] = sum_{j<=i} 2^(-32(i-j+1)) raw[j] - sum over result halves 2^(16 h - 32(i+1)) res_h (h < 2i+2)
+    res = c.LO + c.HI
+    for i in range(4):
+        terms = [LinearTerm(_inv2(32 * (i - j + 1)), c.RAW_PRODUCT[j], True) for j in range(i, -1, -1)]
+        terms += [LinearTerm(P - _inv2(32 * (i + 1) - 16 * h), res[h], True) for h in range(2 * i + 2)]
+        its.append(BusInteraction.sender(BUS_IS_B20, row, [BusValue.of_linear(terms)]))
+
+    def flags(hi):
+        return BusValue.of_linear([LinearTerm(ALU_OP_MUL + hi), LinearTerm(32, c.LHS_SIGNED),
+                                   LinearTerm(64, c.RHS_SIGNED)])
+
+    for mu, hi, out in ((c.MU_LO, 0, c.LO[0]), (c.MU_HI, 128, c.HI[0])):
+        its.append(BusInteraction.receiver(BUS_ALU, Multiplicity("Column", (mu,)), [
+            BusValue.packed(Packing.DWORD_HL, c.LHS[0]), BusValue.packed(Packing.DWORD_HL, c.RHS[0]), flags(hi),
+            BusValue.packed(Packing.DWORD_HL, out)]))
+    return its
+
+
+def mul_table_air() -> AirProgram:
+    """MUL as an AIR (mul.rs:790-818): 0-1 the signed flags are bits, 2-3 an
+    unsigned operand is not negative, 4-7 raw_product[i] equals the convolution
+    of the sign-extended halves; then the LogUp constraints of its 25
+    interactions."""
+    c = MulCols
+    b = ir.IrBuilder()
+    one = b.const_base(1)
+    m = lambda col: b.main(0, col)  # noqa: E731
+    _emit_is_bit(b, 0, c.LHS_SIGNED)
+    _emit_is_bit(b, 1, c.RHS_SIGNED)
+    b.emit(2, (one - m(c.LHS_SIGNED)) * m(c.LHS_IS_NEGATIVE))
+    b.emit(3, (one - m(c.RHS_SIGNED)) * m(c.RHS_IS_NEGATIVE))
+    fill = b.const_base(MUL_SIGN_FILL)
+    a = [m(col) for col in c.LHS] + [fill * m(c.LHS_IS_NEGATIVE)] * 4
+    bb = [m(col) for col in c.RHS] + [fill * m(c.RHS_IS_NEGATIVE)] * 4
+    for i in range(4):
+        s = None
+        for k in range(2):
+            idx = 2 * i + k
+            inner = None
+            for j in range(idx + 1):
+                t = a[j] * bb[idx - j]
+                inner = t if inner is None else inner + t
+            inner = inner if k == 0 else inner * b.const_base(1 << 16)
+            s = inner if s is None else s + inner
+        b.emit(4 + i, m(c.RAW_PRODUCT[i]) - s)
+    its = mul_bus_interactions()
+    n_total = emit_logup_constraints(b, its, 8)
+    prog = b.finish(c.NUM_COLUMNS)
+    pairs, absorbed = split_interactions(len(its))
+    return AirProgram("mul", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
+                      boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
+                      max_degree=max(3 if pairs else 0, 1 + absorbed, 2))

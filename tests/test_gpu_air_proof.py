@@ -422,3 +422,28 @@ def test_gpu_memw_aligned_page_proof(gpu):
     ref = replay_air_tables(tables, 4, 3, 2, 8, 0, b"gpu-memw-a")
     assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
     assert verify_air_tables([memw, page], got, 4, 3, 2, 8, 0, b"gpu-memw-a")[0]
+
+
+def test_gpu_mul_table_proof(gpu):
+    """MUL (26 columns, degree-2 convolution constraints, 25 interactions) beside
+    HINT: the GPU proofs equal the oracle replay byte for byte and verify."""
+    from lambda_vm_amd import proof_io, prover
+    from lambda_vm_amd import vm_tables as V
+    from lambda_vm_amd.prover import AirTableProof
+    from tests.test_vm_table_hint import hint_ops
+    from tests.test_vm_table_mul import mul_ops
+
+    print("[gpu-air] MUL 512 x 26 + HINT, blowup 2", flush=True)
+    mul, hint = V.mul_table_air(), V.hint_table_air()
+    for a in (mul, hint):
+        a.bus_descriptor = A.descriptor(a)
+    tables = [(mul, V.generate_mul_trace(mul_ops(400, 51))), (hint, V.generate_hint_trace(hint_ops(9, 52)))]
+    tables = [(a, np.ascontiguousarray(tr, dtype=np.uint64)) for a, tr in tables]
+    opts = prover.ProofOptions(blowup=2, coset_offset=3, fri_final_log_degree=3, num_queries=10)
+    dev = [(a, gpu.DeviceBuffer.from_array(tr), tr.shape[0]) for a, tr in tables]
+    got = prover.prove_air_tables(dev, opts, label=b"gpu-mul")
+    for _, d, _ in dev:
+        d.free()
+    ref = replay_air_tables(tables, 2, 3, 3, 10, 0, b"gpu-mul")
+    assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
+    assert verify_air_tables([mul, hint], got, 2, 3, 3, 10, 0, b"gpu-mul")[0]
