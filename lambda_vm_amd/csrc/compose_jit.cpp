@@ -61,9 +61,16 @@ size_t jit_load_window() {
 // statements after `r` is set, with `a` the CompArgs and `N` the LDE size in
 // scope); empty on a program this generator does not take (then the
 // interpreter runs it).
+// Largest program (in nodes) that is compiled: the run-time compiler's time
+// grows faster than linearly with the straight-line code (a 1687-node program,
+// the VM's 486-column KECCAK table with 84 interactions, takes 267 s at -O1 and
+// over 15 min at -O3), so larger programs run on the interpreter kernel.
+constexpr size_t JIT_MAX_NODES = 1024;
+
 std::string generate_row(const lvm_program *pg, const std::vector<uint32_t> &root_node, const uint32_t *root_group,
                          int mode) {
     const size_t n = pg->num_nodes;
+    if (n > JIT_MAX_NODES) return "";
     std::vector<std::vector<uint32_t>> emits(n);
     for (size_t c = 0; c < pg->num_roots; c++) {
         if (root_node[c] >= n || (mode == 0 && root_group[c] >= (uint32_t)COMP_MAX_GROUPS)) return "";

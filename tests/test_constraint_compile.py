@@ -60,3 +60,20 @@ def test_unsupported_programs_reported(lib):
     # a valid one-node program compiles
     dev = _raw_program([(ir.OP_CONST_BASE, 0, 0, 0)], [0], 1, 0)
     B.prepare_constraints(dev, [0])
+
+
+def test_large_program_runs_on_the_interpreter(lib):
+    """Programs above the compile size limit (the VM's KECCAK table: 1687
+    nodes) are declined quickly instead of compiled; they run on the
+    interpreter kernel."""
+    import time
+
+    from lambda_vm_amd import vm_tables as V
+
+    a = V.keccak_table_air()
+    groups, _ = a.groups()
+    t0 = time.time()
+    with pytest.raises(B.LvmError):
+        B.prepare_constraints(a.device, groups, num_rap=2, num_alpha=A.descriptor(a).alpha_powers_len,
+                              has_table_offset=True)
+    assert time.time() - t0 < 5
