@@ -1760,3 +1760,123 @@ def mul_table_air() -> AirProgram:
     return AirProgram("mul", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
                       boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
                       max_degree=max(3 if pairs else 0, 1 + absorbed, 2))
+
+
+# MEMW (prover/src/tables/memw.rs): the general memory access — unaligned, or
+# with cells of different old timestamps. The base address is a DWordWL, cell
+# i > 0 lives at base + i with a carry bit into the high word, and every cell
+# carries its own old timestamp (and its own old_ts < ts check).
+class MemwCols:
+    """memw.rs:51-98."""
+    IS_REGISTER, BASE_ADDRESS_0, BASE_ADDRESS_1 = 0, 1, 2
+    VALUE = tuple(range(3, 11))
+    TIMESTAMP_0, TIMESTAMP_1, WRITE2, WRITE4, WRITE8 = 11, 12, 13, 14, 15
+    OLD = tuple(range(16, 24))
+    CARRY = tuple(range(24, 31))
+    OLD_TIMESTAMP_START = 31
+    MU_READ, MU_WRITE = 47, 48
+    NUM_COLUMNS = 49
+
+    @staticmethod
+    def old_timestamp(i: int) -> int:
+        return MemwCols.OLD_TIMESTAMP_START + 2 * i
+
+
+@dataclass(frozen=True)
+class MemwGeneralOperation:
+    """memw.rs:105-126: one old timestamp per cell."""
+    is_register: bool
+    base_address: int
+    value: tuple
+    timestamp: int
+    width: int
+    is_read: bool
+    old: tuple
+    old_timestamp: tuple
+
+    def write_flags(self):
+        return {1: (0, 0, 0), 2: (1, 0, 0), 4: (0, 1, 0), 8: (0, 0, 1)}.get(self.width, (0, 0, 0))
+
+
+def generate_memw_trace(ops) -> np.ndarray:
+    """memw.rs:177-233: carry[i] = 1 when the low word of base + i + 1 wraps."""
+    c = MemwCols
+    n = max(4, 1 << max(0, (len(ops) - 1).bit_length()))
+    t = np.zeros((n, c.NUM_COLUMNS), dtype=np.uint64)
+    for r, op in enumerate(ops):
+        ba = op.base_address & (2**64 - 1)
+        lo = ba & 0xFFFFFFFF
+        t[r, c.IS_REGISTER] = int(op.is_register)
+        t[r, c.BASE_ADDRESS_0], t[r, c.BASE_ADDRESS_1] = lo, ba >> 32
+        t[r, c.VALUE[0]:c.VALUE[-1] + 1] = list(op.value)
+        t[r, c.TIMESTAMP_0], t[r, c.TIMESTAMP_1] = op.timestamp & 0xFFFFFFFF, op.timestamp >> 32
+        t[r, c.WRITE2], t[r, c.WRITE4], t[r, c.WRITE8] = op.write_flags()
+        t[r, c.OLD[0]:c.OLD[-1] + 1] = list(op.old)
+        for i in range(7):
+            t[r, c.CARRY[i]] = int(lo + i + 1 >= 1 << 32)
+        for i in range(8):
+            ots = op.old_timestamp[i]
+            t[r, c.old_timestamp(i)], t[r, c.old_timestamp(i) + 1] = ots & 0xFFFFFFFF, ots >> 32
+        t[r, c.MU_READ], t[r, c.MU_WRITE] = int(op.is_read), int(not op.is_read)
+    return t
+
+
+def memw_bus_interactions():
+    """memw.rs:242-828, in its order (26): per cell the old Memory token sent
+    and the new one received (cell 0 with mu_read + mu_write, cell 1 with w2 +
+    w4 + w8, cells 2-3 with w4 + w8, cells 4-7 with w8; cell i > 0 at
+    [base_lo + i - 2^32 carry[i-1], base_hi + carry[i-1]]); the MEMW read and
+    write receivers; per cell ALU[old_ts_i, ts, LT, 1, 0] with the cell's
+    multiplicity."""
+    c = MemwCols
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    k = lambda v: BusValue.of_linear([LinearTerm(v)])  # noqa: E731
+    cover = [Multiplicity("Sum", (c.MU_READ, c.MU_WRITE)), Multiplicity("Sum3", (c.WRITE2, c.WRITE4, c.WRITE8)),
+             Multiplicity("Sum", (c.WRITE4, c.WRITE8)), Multiplicity("Sum", (c.WRITE4, c.WRITE8))] + \
+        [Multiplicity("Column", (c.WRITE8,))] * 4
+
+    def address(i):
+        if i == 0:
+            return [d(c.BASE_ADDRESS_0), d(c.BASE_ADDRESS_1)]
+        return [BusValue.of_linear([LinearTerm(1, c.BASE_ADDRESS_0), LinearTerm(i), LinearTerm(-(1 << 32), c.CARRY[i - 1])]),
+                BusValue.of_linear([LinearTerm(1, c.BASE_ADDRESS_1), LinearTerm(1, c.CARRY[i - 1])])]
+
+    its = []
+    for i in range(8):
+        ots = c.old_timestamp(i)
+        its.append(BusInteraction.sender(BUS_MEMORY, cover[i], [d(c.IS_REGISTER)] + address(i) +
+                                         [d(ots), d(ots + 1), d(c.OLD[i])]))
+        its.append(BusInteraction.receiver(BUS_MEMORY, cover[i], [d(c.IS_REGISTER)] + address(i) +
+                                           [d(c.TIMESTAMP_0), d(c.TIMESTAMP_1), d(c.VALUE[i])]))
+    access = [d(c.IS_REGISTER), d(c.BASE_ADDRESS_0), d(c.BASE_ADDRESS_1)] + [d(v) for v in c.VALUE] + \
+        [d(c.TIMESTAMP_0), d(c.TIMESTAMP_1), d(c.WRITE2), d(c.WRITE4), d(c.WRITE8)]
+    its.append(BusInteraction.receiver(BUS_MEMW, Multiplicity("Column", (c.MU_READ,)), [d(o) for o in c.OLD] + access))
+    its.append(BusInteraction.receiver(BUS_MEMW, Multiplicity("Column", (c.MU_WRITE,)), list(access)))
+    for i in range(8):
+        its.append(BusInteraction.sender(BUS_ALU, cover[i], [BusValue.packed(Packing.DWORD_WL, c.old_timestamp(i)),
+                                                             BusValue.packed(Packing.DWORD_WL, c.TIMESTAMP_0),
+                                                             k(ALU_OP_LT), k(1), k(0)]))
+    return its
+
+
+def memw_table_air() -> AirProgram:
+    """MEMW as an AIR (memw.rs:852-902): 0 mu_sum a bit, 1 w2·(1 - mu_sum),
+    2-3 mu_read and mu_write bits, 4-10 the carries bits, 11-13 the width flags
+    bits, 14 their sum a bit; then the LogUp constraints of its 26 interactions."""
+    c = MemwCols
+    b = ir.IrBuilder()
+    one = b.const_base(1)
+    mu_sum = b.main(0, c.MU_READ) + b.main(0, c.MU_WRITE)
+    w2 = b.main(0, c.WRITE2) + b.main(0, c.WRITE4) + b.main(0, c.WRITE8)
+    b.emit(0, mu_sum * (one - mu_sum))
+    b.emit(1, w2 * (one - mu_sum))
+    for idx, col in enumerate((c.MU_READ, c.MU_WRITE) + c.CARRY + (c.WRITE2, c.WRITE4, c.WRITE8)):
+        _emit_is_bit(b, 2 + idx, col)
+    b.emit(14, w2 * (one - w2))
+    its = memw_bus_interactions()
+    n_total = emit_logup_constraints(b, its, 15)
+    prog = b.finish(c.NUM_COLUMNS)
+    pairs, absorbed = split_interactions(len(its))
+    return AirProgram("memw", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
+                      boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
+                      max_degree=max(3 if pairs else 0, 1 + absorbed, 2))

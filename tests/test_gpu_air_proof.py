@@ -447,3 +447,28 @@ def test_gpu_mul_table_proof(gpu):
     ref = replay_air_tables(tables, 2, 3, 3, 10, 0, b"gpu-mul")
     assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
     assert verify_air_tables([mul, hint], got, 2, 3, 3, 10, 0, b"gpu-mul")[0]
+
+
+def test_gpu_memw_page_proof(gpu):
+    """MEMW (unaligned accesses across a 2^32 low-word boundary) with the two
+    PAGE tables that close its memory chains: the GPU proofs equal the oracle
+    replay byte for byte and verify."""
+    from lambda_vm_amd import proof_io, prover
+    from lambda_vm_amd import vm_tables as V
+    from lambda_vm_amd.prover import AirTableProof
+    from tests.test_vm_table_memw import _pages, unaligned_chain
+
+    print("[gpu-air] MEMW 256 x 49 + 2 PAGE 1024 x 5, blowup 4", flush=True)
+    ops, finals = unaligned_chain(200, 61)
+    memw = V.memw_table_air()
+    memw.bus_descriptor = A.descriptor(memw)
+    tables = [(memw, V.generate_memw_trace(ops))] + _pages(finals, memory_only=False)
+    tables = [(a, np.ascontiguousarray(tr, dtype=np.uint64)) for a, tr in tables]
+    opts = prover.ProofOptions(blowup=4, coset_offset=3, fri_final_log_degree=2, num_queries=8)
+    dev = [(a, gpu.DeviceBuffer.from_array(tr), tr.shape[0]) for a, tr in tables]
+    got = prover.prove_air_tables(dev, opts, label=b"gpu-memw")
+    for _, d, _ in dev:
+        d.free()
+    ref = replay_air_tables(tables, 4, 3, 2, 8, 0, b"gpu-memw")
+    assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
+    assert verify_air_tables([a for a, _ in tables], got, 4, 3, 2, 8, 0, b"gpu-memw")[0]
