@@ -159,6 +159,16 @@ void launch_parts_scatter(const uint64_t *coef, unsigned log_N, uint32_t P, cons
 // ---- work lists over several tables (batched stages) ----
 // Work-list slot of block b: the t with prefix[t] <= b < prefix[t+1]
 // (prefix ascending, prefix[0] = 0), by binary search over `count` slots.
+// Workgroups are dispatched to the 8 XCDs round-robin by block index, and
+// each XCD has its own L2. This bijection on [0, nb) gives XCD x the
+// contiguous logical range [x*q, (x+1)*q) (q = nb / 8; the nb % 8 tail blocks
+// keep their index), so neighbouring logical tiles that share cache lines
+// (the column groups of one row tile) are served by one L2.
+__device__ __forceinline__ uint32_t xcd_contiguous(uint32_t b, uint32_t nb) {
+    const uint32_t q = nb >> 3;
+    return b < (q << 3) ? (b & 7u) * q + (b >> 3) : b;
+}
+
 __device__ __forceinline__ uint32_t find_slot(const uint32_t *__restrict__ prefix, uint32_t count, uint32_t b) {
     uint32_t lo = 0, hi = count - 1;
     while (lo < hi) {

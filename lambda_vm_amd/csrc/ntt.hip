@@ -247,8 +247,11 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t bx, ui
 }
 
 template <int LOG_R1, int LOG_R2, bool MULTI, int C>
-__global__ __launch_bounds__(256) void ntt_pass_kernel(PassArgs a) {
-    ntt_pass_body<LOG_R1, LOG_R2, MULTI, C>(a, blockIdx.x, blockIdx.y, blockIdx.z);
+__global__ __launch_bounds__(256) void ntt_pass_kernel(PassArgs a, uint32_t groups) {
+    // one-dimensional grid, column groups fastest, XCD-contiguous (the column
+    // groups of a row tile share cache lines)
+    const uint32_t b = xcd_contiguous(blockIdx.x, gridDim.x);
+    ntt_pass_body<LOG_R1, LOG_R2, MULTI, C>(a, b / groups, b % groups, 0);
 }
 
 // One launch for the same pass shape over several tables: `args[t]` is
@@ -260,7 +263,7 @@ constexpr int BATCH_C = 16;  // 8-column tiles measured slower here (28.4 vs 30.
 template <int LOG_R1, int LOG_R2, bool MULTI>
 __global__ __launch_bounds__(256) void ntt_pass_batched_kernel(const PassArgs *__restrict__ args,
                                                                const uint32_t *__restrict__ prefix, uint32_t count) {
-    const uint32_t b = blockIdx.x;
+    const uint32_t b = xcd_contiguous(blockIdx.x, gridDim.x);
     const uint32_t t = find_slot(prefix, count, b);
     const PassArgs a = args[t];
     const uint32_t local = b - prefix[t];
@@ -282,11 +285,12 @@ static void launch_one_c(const PassArgs &a0, uint32_t cosets, hipStream_t st) {
     PassArgs a = a0;
     // all cosets of a tile come from one block, which reads the tile once
     a.cosets_per_block = cosets;
-    dim3 grid((unsigned)((a.total_groups + TILES - 1) / TILES), (a.m + C - 1) / C, 1);
+    const uint32_t groups = (a.m + C - 1) / C;
+    const dim3 grid((unsigned)((a.total_groups + TILES - 1) / TILES * groups));
     if (cosets > 1)
-        hipLaunchKernelGGL((ntt_pass_kernel<A, B, true, C>), grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((ntt_pass_kernel<A, B, true, C>), grid, dim3(256), 0, st, a, groups);
     else
-        hipLaunchKernelGGL((ntt_pass_kernel<A, B, false, C>), grid, dim3(256), 0, st, a);
+        hipLaunchKernelGGL((ntt_pass_kernel<A, B, false, C>), grid, dim3(256), 0, st, a, groups);
 }
 
 // 8-column tiles for tables of at most 8 columns (a 16-column tile would

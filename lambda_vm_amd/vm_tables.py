@@ -1880,3 +1880,232 @@ def memw_table_air() -> AirProgram:
     return AirProgram("memw", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
                       boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
                       max_degree=max(3 if pairs else 0, 1 + absorbed, 2))
+
+
+# SHIFT (prover/src/tables/shift.rs): SLL/SRL/SRA and their word forms. An
+# intra-limb shift by bit_shift = shift mod 16 through paired HWSL lookups
+# ([SLL, SLLC] of each input half), then a whole-limb shift by the one-hot
+# limb_shift (its last entry virtual).
+ALU_OP_SHIFT = 5   # types.rs:567
+
+
+class ShiftCols:
+    """shift.rs:31-99."""
+    IN = (0, 1, 2, 3)
+    SHIFT_AMOUNT, DIRECTION, SIGNED, WORD_INSTR = 4, 5, 6, 7
+    OUT_0, OUT_1 = 8, 9
+    IS_NEGATIVE, BIT_SHIFT, ZBS = 10, 11, 12
+    X = (13, 14, 15, 16, 17)
+    Y = (18, 19, 20, 21)
+    LIMB_SHIFT_RAW = (22, 23, 24)
+    MU = 25
+    SHIFT_B1, SHIFT_H1, SHIFT_HIGH = 26, 27, 28
+    NUM_COLUMNS = 29
+
+
+@dataclass(frozen=True)
+class ShiftOperation:
+    """shift.rs:105-145: the input value, the full shift operand (only its low
+    byte shifts), left (direction 0) or right, arithmetic, word instruction."""
+    value: int
+    shift_amount: int
+    direction: bool
+    signed: bool
+    word_instr: bool
+
+    def halves(self):
+        v = self.value & (2**64 - 1)
+        return [(v >> (16 * i)) & 0xFFFF for i in range(4)]
+
+    def result(self) -> int:
+        """shift.rs:170-205 (the architectural result, word forms sign-extended)."""
+        v, s = self.value & (2**64 - 1), self.shift_amount & 0xFF
+        sh = s % (32 if self.word_instr else 64)
+        if self.word_instr:
+            lo = v & 0xFFFFFFFF
+            if not self.direction:
+                r = (lo << sh) & 0xFFFFFFFF
+            elif not self.signed:
+                r = lo >> sh
+            else:
+                r = ((lo - (1 << 32) if lo >> 31 else lo) >> sh) & 0xFFFFFFFF
+            return (r - (1 << 32) if r >> 31 else r) & (2**64 - 1)
+        if not self.direction:
+            return (v << sh) & (2**64 - 1)
+        if not self.signed:
+            return v >> sh
+        return ((v - 2**64 if v >> 63 else v) >> sh) & (2**64 - 1)
+
+    def aux(self):
+        """shift.rs:218-339: is_negative, bit_shift, zbs, X, Y, limb_shift, out words."""
+        h = self.halves()
+        s = self.shift_amount & 0xFF
+        left = not self.direction
+        is_neg = self.signed and (h[3] >> 15) & 1 == 1
+        ext = 0xFFFF if is_neg else 0
+        bit_shift = (s & 15) if left else ((256 - s) & 15)
+        zbs = bit_shift == 0
+        x, y = [0] * 5, [0] * 4
+        if zbs:
+            for i in range(4):
+                if left:
+                    x[i] = h[i]
+                else:
+                    y[i] = h[i]
+        else:
+            for i in range(4):
+                x[i] = (h[i] << bit_shift) & 0xFFFF
+                y[i] = h[i] >> (16 - bit_shift)
+            x[4] = (ext << bit_shift) & 0xFFFF
+        limb = ((s >> 4) & 1) if self.word_instr else ((s >> 4) & 3)
+        ls = [int(j == limb) for j in range(4)]
+        il = lambda i: x[0] if i == 0 else (x[i] + y[i - 1]) & 0xFFFF  # noqa: E731
+        ir_ = lambda i: (y[i] + x[i + 1]) & 0xFFFF  # noqa: E731
+        shifted = []
+        for i in range(4):
+            val = 0
+            if left:
+                for j in range(i + 1):
+                    if ls[j]:
+                        val = (val + il(i - j)) & 0xFFFF
+            else:
+                for j in range(4 - i):
+                    if ls[j]:
+                        val = (val + ir_(i + j)) & 0xFFFF
+                for j in range(4 - i, 4):
+                    if ls[j]:
+                        val = (val + ext) & 0xFFFF
+            shifted.append(val)
+        out = (shifted[0] | shifted[1] << 16, shifted[2] | shifted[3] << 16)
+        return is_neg, bit_shift, zbs, x, y, ls, out
+
+
+def generate_shift_trace(ops) -> np.ndarray:
+    """shift.rs:353-412: one row per operation (mu = 1, no deduplication);
+    padding rows hold only zbs = 1."""
+    c = ShiftCols
+    n = max(4, 1 << max(0, (len(ops) - 1).bit_length()))
+    t = np.zeros((n, c.NUM_COLUMNS), dtype=np.uint64)
+    for r, op in enumerate(ops):
+        is_neg, bit_shift, zbs, x, y, ls, out = op.aux()
+        sa = op.shift_amount & (2**64 - 1)
+        t[r, c.IN[0]:c.IN[-1] + 1] = op.halves()
+        t[r, c.SHIFT_AMOUNT], t[r, c.SHIFT_B1] = sa & 0xFF, (sa >> 8) & 0xFF
+        t[r, c.SHIFT_H1], t[r, c.SHIFT_HIGH] = (sa >> 16) & 0xFFFF, sa >> 32
+        t[r, c.DIRECTION], t[r, c.SIGNED], t[r, c.WORD_INSTR] = int(op.direction), int(op.signed), int(op.word_instr)
+        t[r, c.OUT_0], t[r, c.OUT_1] = out
+        t[r, c.IS_NEGATIVE], t[r, c.BIT_SHIFT], t[r, c.ZBS] = int(is_neg), bit_shift, int(zbs)
+        t[r, c.X[0]:c.X[-1] + 1] = x
+        t[r, c.Y[0]:c.Y[-1] + 1] = y
+        t[r, c.LIMB_SHIFT_RAW[0]:c.LIMB_SHIFT_RAW[-1] + 1] = ls[:3]
+        t[r, c.MU] = 1
+    t[len(ops):, c.ZBS] = 1
+    return t
+
+
+def shift_bus_interactions():
+    """shift.rs:417-716, in its order (17): MSB16[in[3], is_negative] (signed);
+    BYTE_ALU[AND, shift, 15, bit_shift] (mu - direction) and [AND, 256 - 16 zbs -
+    shift, 15, bit_shift] (direction); ZERO[bit_shift, zbs] (mu); HWSL[in[i],
+    bit_shift, X[i], Y[i]] and HWSL[65535 is_negative, bit_shift, X[4],
+    65535 is_negative - X[4]] (1 - zbs); BYTE_ALU[AND, shift, 48 - 32 word_instr,
+    48 - 48 ls0 - 32 ls1 - 16 ls2] (mu); the ALU receiver [in, shift amount,
+    SHIFT + word_instr + 32 signed + 64 direction, out] (mu); ARE_BYTES[shift
+    bytes 0-1] once; IS_HALFWORD of shift bits 16-31 and of the four input
+    halves (mu)."""
+    c = ShiftCols
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    k = lambda v: BusValue.of_linear([LinearTerm(v)])  # noqa: E731
+    mu = Multiplicity("Column", (c.MU,))
+    not_zbs = Multiplicity("Negated", (c.ZBS,))
+    its = [BusInteraction.sender(BUS_MSB16, Multiplicity("Column", (c.SIGNED,)), [d(c.IN[3]), d(c.IS_NEGATIVE)]),
+           BusInteraction.sender(BUS_BYTE_ALU, Multiplicity("Diff", (c.MU, c.DIRECTION)),
+                                 [k(ALU_OP_AND), d(c.SHIFT_AMOUNT), k(15), d(c.BIT_SHIFT)]),
+           BusInteraction.sender(BUS_BYTE_ALU, Multiplicity("Column", (c.DIRECTION,)), [
+               k(ALU_OP_AND), BusValue.of_linear([LinearTerm(256), LinearTerm(-16, c.ZBS), LinearTerm(-1, c.SHIFT_AMOUNT)]),
+               k(15), d(c.BIT_SHIFT)]),
+           BusInteraction.sender(BUS_ZERO, mu, [d(c.BIT_SHIFT), d(c.ZBS)])]
+    its += [BusInteraction.sender(BUS_HWSL, not_zbs, [d(c.IN[i]), d(c.BIT_SHIFT), d(c.X[i]), d(c.Y[i])])
+            for i in range(4)]
+    its.append(BusInteraction.sender(BUS_HWSL, not_zbs, [
+        BusValue.of_linear([LinearTerm(65535, c.IS_NEGATIVE)]), d(c.BIT_SHIFT), d(c.X[4]),
+        BusValue.of_linear([LinearTerm(65535, c.IS_NEGATIVE), LinearTerm(-1, c.X[4])])]))
+    its.append(BusInteraction.sender(BUS_BYTE_ALU, mu, [
+        k(ALU_OP_AND), d(c.SHIFT_AMOUNT), BusValue.of_linear([LinearTerm(48), LinearTerm(-32, c.WORD_INSTR)]),
+        BusValue.of_linear([LinearTerm(48), LinearTerm(-48, c.LIMB_SHIFT_RAW[0]), LinearTerm(-32, c.LIMB_SHIFT_RAW[1]),
+                            LinearTerm(-16, c.LIMB_SHIFT_RAW[2])])]))
+    its.append(BusInteraction.receiver(BUS_ALU, mu, [
+        BusValue.packed(Packing.DWORD_HL, c.IN[0]),
+        BusValue.of_linear([LinearTerm(1, c.SHIFT_AMOUNT), LinearTerm(1 << 8, c.SHIFT_B1), LinearTerm(1 << 16, c.SHIFT_H1)]),
+        d(c.SHIFT_HIGH),
+        BusValue.of_linear([LinearTerm(ALU_OP_SHIFT), LinearTerm(1, c.WORD_INSTR), LinearTerm(32, c.SIGNED),
+                            LinearTerm(64, c.DIRECTION)]),
+        BusValue.packed(Packing.DWORD_WL, c.OUT_0)]))
+    its.append(BusInteraction.sender(BUS_ARE_BYTES, ONE, [d(c.SHIFT_AMOUNT), d(c.SHIFT_B1)]))
+    its.append(BusInteraction.sender(BUS_IS_HALFWORD, mu, [d(c.SHIFT_H1)]))
+    its += [BusInteraction.sender(BUS_IS_HALFWORD, mu, [d(col)]) for col in c.IN]
+    return its
+
+
+def shift_table_air() -> AirProgram:
+    """SHIFT as an AIR (shift.rs:823-940): 0 direction implies mu; 1-4 zbs
+    pins X[i] to in[i]·(mu - direction), 5 zbs·X[4], 6-9 zbs pins Y[i] to
+    in[i]·direction; 10-13 limb_shift bits; 14-15 out words equal the shifted
+    halves; 16-18 direction, signed, word_instr bits; 19 zbs a bit; 20 mu a
+    bit; 21 an unsigned input is not negative; then the LogUp constraints of
+    its 17 interactions."""
+    c = ShiftCols
+    b = ir.IrBuilder()
+    m = lambda col: b.main(0, col)  # noqa: E731
+    one = b.const_base(1)
+
+    def ls(i):
+        return m(c.LIMB_SHIFT_RAW[i]) if i < 3 else \
+            one - m(c.LIMB_SHIFT_RAW[0]) - m(c.LIMB_SHIFT_RAW[1]) - m(c.LIMB_SHIFT_RAW[2])
+
+    def intra_left(i):
+        return m(c.X[0]) if i == 0 else m(c.X[i]) + m(c.Y[i - 1])
+
+    def intra_right(i):
+        return m(c.Y[i]) + m(c.X[i + 1])
+
+    def shifted_half(i):
+        left = m(c.MU) - m(c.DIRECTION)
+        right = m(c.DIRECTION)
+        extension = m(c.IS_NEGATIVE) * b.const_base(65535)
+        lp = None
+        for j in range(i + 1):
+            t = ls(j) * intra_left(i - j)
+            lp = t if lp is None else lp + t
+        rp = None
+        for j in range(4 - i):
+            t = ls(j) * intra_right(i + j)
+            rp = t if rp is None else rp + t
+        es = None
+        for j in range(4 - i, 4):
+            es = ls(j) if es is None else es + ls(j)
+        right_part = rp + extension * es if es is not None else rp
+        return left * lp + right * right_part
+
+    b.emit(0, m(c.DIRECTION) * (one - m(c.MU)))
+    for i in range(4):
+        b.emit(1 + i, m(c.ZBS) * (m(c.X[i]) - m(c.IN[i]) * (m(c.MU) - m(c.DIRECTION))))
+    b.emit(5, m(c.ZBS) * m(c.X[4]))
+    for i in range(4):
+        b.emit(6 + i, m(c.ZBS) * (m(c.Y[i]) - m(c.IN[i]) * m(c.DIRECTION)))
+    for i in range(4):
+        b.emit(10 + i, ls(i) * (one - ls(i)))
+    for i in range(2):
+        b.emit(14 + i, m(c.OUT_0 + i) - shifted_half(2 * i) - shifted_half(2 * i + 1) * b.const_base(1 << 16))
+    for off, col in enumerate((c.DIRECTION, c.SIGNED, c.WORD_INSTR)):
+        _emit_is_bit(b, 16 + off, col)
+    _emit_is_bit(b, 19, c.ZBS)
+    _emit_is_bit(b, 20, c.MU)
+    b.emit(21, (one - m(c.SIGNED)) * m(c.IS_NEGATIVE))
+    its = shift_bus_interactions()
+    n_total = emit_logup_constraints(b, its, 22)
+    prog = b.finish(c.NUM_COLUMNS)
+    pairs, absorbed = split_interactions(len(its))
+    return AirProgram("shift", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
+                      boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
+                      max_degree=max(3 if pairs else 0, 1 + absorbed, 3))
