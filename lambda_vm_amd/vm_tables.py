@@ -2109,3 +2109,243 @@ def shift_table_air() -> AirProgram:
     return AirProgram("shift", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
                       boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
                       max_degree=max(3 if pairs else 0, 1 + absorbed, 3))
+
+
+# DVRM (prover/src/tables/dvrm.rs): 64-bit DIV/DIVU/REM/REMU. n = q·d + r is
+# proven through lookups: MUL[d, q] lo = n - r and hi = its sign extension,
+# LT[|r|, |d|] = 1 unless d = 0, NEG templates tie abs_r/abs_d to r/d, and the
+# ZERO checks pin div_by_zero and overflow (MIN / -1). Deduplicated by
+# (n, d, signed) with separate quotient and remainder multiplicities.
+ALU_OP_DIVREM = 8   # types.rs:570
+DVRM_SIGN_FILL = 0xFFFF
+
+
+class DvrmCols:
+    """dvrm.rs:48-140."""
+    N = (0, 1, 2, 3)
+    D = (4, 5, 6, 7)
+    SIGNED = 8
+    Q = (9, 10, 11, 12)
+    R = (13, 14, 15, 16)
+    DIV_BY_ZERO, OVERFLOW = 17, 18
+    ABS_R = (19, 20)
+    ABS_D = (21, 22)
+    N_SUB_R = (23, 24, 25, 26)
+    SIGN_N_SUB_R, SIGN_N, SIGN_D, SIGN_Q, SIGN_R = 27, 28, 29, 30, 31
+    MU_Q, MU_R = 32, 33
+    NUM_COLUMNS = 34
+
+
+@dataclass(frozen=True)
+class DvrmOperation:
+    """dvrm.rs:157-272: quotient and remainder as RISC-V defines them (division
+    by zero: q = 2^64 - 1, r = n; signed MIN / -1: q = MIN, r = 0; signed
+    division truncates toward zero)."""
+    n: int
+    d: int
+    signed: bool
+
+    def is_div_by_zero(self) -> bool:
+        return self.d == 0
+
+    def is_overflow(self) -> bool:
+        return self.signed and self.n == 1 << 63 and self.d == 2**64 - 1
+
+    @staticmethod
+    def _s(v):
+        return v - 2**64 if v >> 63 else v
+
+    def quotient(self) -> int:
+        if self.is_div_by_zero():
+            return 2**64 - 1
+        if self.is_overflow():
+            return self.n
+        if self.signed:
+            a, b = self._s(self.n), self._s(self.d)
+            q = abs(a) // abs(b)
+            return (-q if (a < 0) != (b < 0) else q) & (2**64 - 1)
+        return self.n // self.d
+
+    def remainder(self) -> int:
+        if self.is_div_by_zero():
+            return self.n
+        if self.is_overflow():
+            return 0
+        if self.signed:
+            a, b = self._s(self.n), self._s(self.d)
+            r = abs(a) % abs(b)
+            return (-r if a < 0 else r) & (2**64 - 1)
+        return self.n % self.d
+
+    def sign_n(self) -> bool:
+        return self.signed and self.n >> 63 == 1
+
+    def sign_d(self) -> bool:
+        return self.signed and self.d >> 63 == 1
+
+    def sign_q(self) -> bool:
+        return self.signed and not self.is_overflow()
+
+    def sign_r(self) -> bool:
+        return self.signed and self.remainder() >> 63 == 1
+
+    def abs_r(self) -> int:
+        r = self.remainder()
+        return (2**64 - r) & (2**64 - 1) if self.sign_r() else r
+
+    def abs_d(self) -> int:
+        return (2**64 - self.d) & (2**64 - 1) if self.sign_d() else self.d
+
+    def n_sub_r(self) -> int:
+        return (self.n - self.remainder()) & (2**64 - 1)
+
+    def sign_n_sub_r(self) -> bool:
+        return self.signed and self.n_sub_r() >> 63 == 1
+
+
+def dedup_dvrm_rows(ops):
+    """dvrm.rs:287-320: (operation, wants_remainder) pairs merged per (n, d,
+    signed) into (mu_q, mu_r) counts, split at MU_MAX; rows follow first
+    occurrence (the reference's map order is arbitrary)."""
+    counts = {}
+    for op, wants_r in ops:
+        q, r = counts.get(op, (0, 0))
+        counts[op] = (q, r + 1) if wants_r else (q + 1, r)
+    rows = []
+    for op, (q, r) in counts.items():
+        while q > 0 or r > 0:
+            rq, rr = min(q, MU_MAX), min(r, MU_MAX)
+            q, r = q - rq, r - rr
+            rows.append((op, rq, rr))
+    return rows
+
+
+def generate_dvrm_trace(ops) -> np.ndarray:
+    """dvrm.rs:322-385: one row per deduplicated operation, zero rows to a power
+    of two (at least 4)."""
+    c = DvrmCols
+    rows = dedup_dvrm_rows(ops)
+    n = max(4, 1 << max(0, (len(rows) - 1).bit_length()))
+    t = np.zeros((n, c.NUM_COLUMNS), dtype=np.uint64)
+    for i, (op, mu_q, mu_r) in enumerate(rows):
+        for cols, v in ((c.N, op.n), (c.D, op.d), (c.Q, op.quotient()), (c.R, op.remainder()),
+                        (c.N_SUB_R, op.n_sub_r())):
+            for k, col in enumerate(cols):
+                t[i, col] = (v >> (16 * k)) & 0xFFFF
+        for cols, v in ((c.ABS_R, op.abs_r()), (c.ABS_D, op.abs_d())):
+            t[i, cols[0]], t[i, cols[1]] = v & 0xFFFFFFFF, v >> 32
+        t[i, c.SIGNED] = int(op.signed)
+        t[i, c.DIV_BY_ZERO], t[i, c.OVERFLOW] = int(op.is_div_by_zero()), int(op.is_overflow())
+        t[i, c.SIGN_N_SUB_R], t[i, c.SIGN_N] = int(op.sign_n_sub_r()), int(op.sign_n())
+        t[i, c.SIGN_D], t[i, c.SIGN_Q], t[i, c.SIGN_R] = int(op.sign_d()), int(op.sign_q()), int(op.sign_r())
+        t[i, c.MU_Q], t[i, c.MU_R] = mu_q, mu_r
+    return t
+
+
+def _neg_carry_zero(x, absx, sign_col):
+    """The two ZERO sends of the NEG template x + abs_x = 2^64 (dvrm.rs:673-
+    865): ZERO[x0 + x1; 1 - carry0] and ZERO[x0 + .. + x3; 1 - carry1] with
+    carry0 = 2^-32 (x.lo + abs.lo), carry1 = 2^-32 (x.hi + abs.hi + carry0)."""
+    neg = lambda k: P - _inv2(k)  # noqa: E731
+    mu = Multiplicity("Column", (sign_col,))
+    c0 = [LinearTerm(1), LinearTerm(neg(32), absx[0], True), LinearTerm(neg(32), x[0], True),
+          LinearTerm(neg(16), x[1], True)]
+    c1 = [LinearTerm(1), LinearTerm(neg(32), absx[1], True), LinearTerm(neg(32), x[2], True),
+          LinearTerm(neg(16), x[3], True), LinearTerm(neg(64), absx[0], True), LinearTerm(neg(64), x[0], True),
+          LinearTerm(neg(48), x[1], True)]
+    return [BusInteraction.sender(BUS_ZERO, mu, [BusValue.of_linear([LinearTerm(1, x[0]), LinearTerm(1, x[1])]),
+                                                 BusValue.of_linear(c0)]),
+            BusInteraction.sender(BUS_ZERO, mu, [BusValue.of_linear([LinearTerm(1, col) for col in x]),
+                                                 BusValue.of_linear(c1)])]
+
+
+def dvrm_bus_interactions():
+    """dvrm.rs:393-1023, in its order (35): ARE_BYTES[mu_q, mu_r] once;
+    IS_HALFWORD of n, d, r, n - r and q halves (mu_q + mu_r); MSB16 of n[3],
+    r[3], d[3] (signed); ALU LT[abs_r, abs_d] = 1 - div_by_zero; ALU MUL lo
+    [d, q] = n - r and hi = its sign extension (flags MUL + 32 signed + 64
+    sign_q, + 128 for hi); the NEG ZERO pairs of r (sign_r) and d (sign_d);
+    ZERO[overflow sum; overflow] and ZERO[d halves; div_by_zero]; the ALU
+    receivers of q (mu_q) and r (mu_r, flags + 128)."""
+    c = DvrmCols
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    hl = lambda col: BusValue.packed(Packing.DWORD_HL, col)  # noqa: E731
+    k = lambda v: BusValue.of_linear([LinearTerm(v)])  # noqa: E731
+    row = Multiplicity("Sum", (c.MU_Q, c.MU_R))
+    signed = Multiplicity("Column", (c.SIGNED,))
+    its = [BusInteraction.sender(BUS_ARE_BYTES, ONE, [d(c.MU_Q), d(c.MU_R)])]
+    its += [BusInteraction.sender(BUS_IS_HALFWORD, row, [d(col)]) for col in c.N + c.D + c.R + c.N_SUB_R + c.Q]
+    its += [BusInteraction.sender(BUS_MSB16, signed, [d(top), d(sign)])
+            for top, sign in ((c.N[3], c.SIGN_N), (c.R[3], c.SIGN_R), (c.D[3], c.SIGN_D))]
+    its.append(BusInteraction.sender(BUS_ALU, row, [
+        BusValue.packed(Packing.DWORD_WL, c.ABS_R[0]), BusValue.packed(Packing.DWORD_WL, c.ABS_D[0]), k(ALU_OP_LT),
+        BusValue.of_linear([LinearTerm(1), LinearTerm(-1, c.DIV_BY_ZERO)]), k(0)]))
+
+    def mul_flags(hi):
+        return BusValue.of_linear([LinearTerm(ALU_OP_MUL + hi), LinearTerm(32, c.SIGNED), LinearTerm(64, c.SIGN_Q)])
+
+    ext = BusValue.of_linear([LinearTerm(DVRM_SIGN_FILL + (DVRM_SIGN_FILL << 16), c.SIGN_N_SUB_R)])
+    its.append(BusInteraction.sender(BUS_ALU, row, [hl(c.D[0]), hl(c.Q[0]), mul_flags(0), hl(c.N_SUB_R[0])]))
+    its.append(BusInteraction.sender(BUS_ALU, row, [hl(c.D[0]), hl(c.Q[0]), mul_flags(128), ext, ext]))
+    its += _neg_carry_zero(c.R, c.ABS_R, c.SIGN_R)
+    its += _neg_carry_zero(c.D, c.ABS_D, c.SIGN_D)
+    overflow_sum = [LinearTerm(1, col) for col in c.N] + [LinearTerm(-32769, c.SIGN_N), LinearTerm(1 + 4 * 65535)]
+    overflow_sum += [LinearTerm(-1, col) for col in c.D]
+    its.append(BusInteraction.sender(BUS_ZERO, row, [BusValue.of_linear(overflow_sum), d(c.OVERFLOW)]))
+    its.append(BusInteraction.sender(BUS_ZERO, row, [BusValue.of_linear([LinearTerm(1, col) for col in c.D]),
+                                                     d(c.DIV_BY_ZERO)]))
+    for mu, extra, out in ((c.MU_Q, 0, c.Q[0]), (c.MU_R, 128, c.R[0])):
+        its.append(BusInteraction.receiver(BUS_ALU, Multiplicity("Column", (mu,)), [
+            hl(c.N[0]), hl(c.D[0]), BusValue.of_linear([LinearTerm(ALU_OP_DIVREM + extra), LinearTerm(32, c.SIGNED)]),
+            hl(out)]))
+    return its
+
+
+def dvrm_table_air() -> AirProgram:
+    """DVRM as an AIR (dvrm.rs:1116-1209): 0 signed a bit; 1 a nonzero remainder
+    has the numerator's sign; 2-3 abs_r = r and 4-5 abs_d = d when not
+    negative; 6 sign_q = signed (1 - overflow); 7-10 the carries of the
+    sign-extended n - r + r = n are bits; 11 sign_n_sub_r a bit; 12-14 no sign
+    bits on unsigned rows; 15-18 division by zero forces q halves to 65535;
+    then the LogUp constraints of its 35 interactions."""
+    c = DvrmCols
+    b = ir.IrBuilder()
+    m = lambda col: b.main(0, col)  # noqa: E731
+    one, two16 = b.const_base(1), b.const_base(1 << 16)
+    inv32 = b.const_base(INV_SHIFT_32)
+
+    def ext_word(halves, sign, i):
+        if i < 2:
+            return m(halves[2 * i]) + m(halves[2 * i + 1]) * two16
+        return m(sign) * b.const_base(DVRM_SIGN_FILL + (DVRM_SIGN_FILL << 16))
+
+    def carry(i):
+        s = ext_word(c.N_SUB_R, c.SIGN_N_SUB_R, i) + ext_word(c.R, c.SIGN_R, i)
+        if i > 0:
+            s = s + carry(i - 1)
+        return (s - ext_word(c.N, c.SIGN_N, i)) * inv32
+
+    _emit_is_bit(b, 0, c.SIGNED)
+    r_sum = m(c.R[0]) + m(c.R[1]) + m(c.R[2]) + m(c.R[3])
+    b.emit(1, r_sum * (m(c.SIGN_R) - m(c.SIGN_N)))
+    for off, (absc, lo, hi) in enumerate(((c.ABS_R[0], c.R[0], c.R[1]), (c.ABS_R[1], c.R[2], c.R[3]))):
+        b.emit(2 + off, (one - m(c.SIGN_R)) * (m(absc) - (m(lo) + m(hi) * two16)))
+    for off, (absc, lo, hi) in enumerate(((c.ABS_D[0], c.D[0], c.D[1]), (c.ABS_D[1], c.D[2], c.D[3]))):
+        b.emit(4 + off, (one - m(c.SIGN_D)) * (m(absc) - (m(lo) + m(hi) * two16)))
+    b.emit(6, m(c.SIGNED) * (one - m(c.OVERFLOW)) - m(c.SIGN_Q))
+    for i in range(4):
+        ci = carry(i)
+        b.emit(7 + i, ci * (one - ci))
+    _emit_is_bit(b, 11, c.SIGN_N_SUB_R)
+    for off, col in enumerate((c.SIGN_N, c.SIGN_R, c.SIGN_D)):
+        b.emit(12 + off, (one - m(c.SIGNED)) * m(col))
+    fill = b.const_base(DVRM_SIGN_FILL)
+    for i, col in enumerate(c.Q):
+        b.emit(15 + i, m(c.DIV_BY_ZERO) * (m(col) - fill))
+    its = dvrm_bus_interactions()
+    n_total = emit_logup_constraints(b, its, 19)
+    prog = b.finish(c.NUM_COLUMNS)
+    pairs, absorbed = split_interactions(len(its))
+    return AirProgram("dvrm", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
+                      boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
+                      max_degree=max(3 if pairs else 0, 1 + absorbed, 2))

@@ -497,3 +497,28 @@ def test_gpu_shift_table_proof(gpu):
     ref = replay_air_tables(tables, 4, 3, 2, 8, 0, b"gpu-shift")
     assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
     assert verify_air_tables([shift, mul], got, 4, 3, 2, 8, 0, b"gpu-shift")[0]
+
+
+def test_gpu_dvrm_table_proof(gpu):
+    """DVRM (34 columns, 19 constraints, 35 interactions) beside MUL: the GPU
+    proofs equal the oracle replay byte for byte and verify."""
+    from lambda_vm_amd import proof_io, prover
+    from lambda_vm_amd import vm_tables as V
+    from lambda_vm_amd.prover import AirTableProof
+    from tests.test_vm_table_dvrm import dvrm_ops
+    from tests.test_vm_table_mul import mul_ops
+
+    print("[gpu-air] DVRM 512 x 34 + MUL, blowup 4", flush=True)
+    dvrm, mul = V.dvrm_table_air(), V.mul_table_air()
+    for a in (dvrm, mul):
+        a.bus_descriptor = A.descriptor(a)
+    tables = [(dvrm, V.generate_dvrm_trace(dvrm_ops(480, 81))), (mul, V.generate_mul_trace(mul_ops(60, 82)))]
+    tables = [(a, np.ascontiguousarray(tr, dtype=np.uint64)) for a, tr in tables]
+    opts = prover.ProofOptions(blowup=4, coset_offset=3, fri_final_log_degree=2, num_queries=8)
+    dev = [(a, gpu.DeviceBuffer.from_array(tr), tr.shape[0]) for a, tr in tables]
+    got = prover.prove_air_tables(dev, opts, label=b"gpu-dvrm")
+    for _, d, _ in dev:
+        d.free()
+    ref = replay_air_tables(tables, 4, 3, 2, 8, 0, b"gpu-dvrm")
+    assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
+    assert verify_air_tables([dvrm, mul], got, 4, 3, 2, 8, 0, b"gpu-dvrm")[0]
