@@ -1412,7 +1412,8 @@ def generate_keccak_trace(ops) -> np.ndarray:
 def keccak_bus_interactions():
     """keccak.rs:175-440, in its order (84 interactions, all with multiplicity
     mu): Ecall receiver; MEMW read of x10 with the address from its bytes;
-    Keccak send [ts, 0, input bytes] and receive [ts, 24, output bytes]; per
+    Keccak send [ts, 0, input bytes] and receive [ts, 24, output bytes] (the
+    state bytes x-major: lane x + 5y for x, then y); per
     lane IS_WH[word_hi, half_mid] and IS_HALFWORD[half_lo]; BYTE_ALU[AND,
     addr[0], 7, 0]; four ARE_BYTES pairs of the address bytes; per lane the
     MEMW read-write [input bytes, 0, address, output bytes, ts, 0, 0, 1]."""
@@ -1427,8 +1428,10 @@ def keccak_bus_interactions():
     addr_hi = BusValue.of_linear([LinearTerm(256 ** i, c.ADDR + 4 + i) for i in range(4)])
     word = [addr_lo, addr_hi] + [k(0)] * 6
     its.append(BusInteraction.sender(BUS_MEMW, mu, word + [k(1), k(20), k(0)] + word + ts + [k(1), k(0), k(0)]))
-    its.append(BusInteraction.sender(BUS_KECCAK, mu, ts + [k(0)] + [d(c.INPUT_STATE + i) for i in range(200)]))
-    its.append(BusInteraction.receiver(BUS_KECCAK, mu, ts + [k(24)] + [d(c.OUTPUT_STATE + i) for i in range(200)]))
+    # state bytes in x-major order: lane x + 5y for x, then y, then byte (keccak.rs:290-334)
+    order = [8 * (x + 5 * y) + byte for x in range(5) for y in range(5) for byte in range(8)]
+    its.append(BusInteraction.sender(BUS_KECCAK, mu, ts + [k(0)] + [d(c.INPUT_STATE + i) for i in order]))
+    its.append(BusInteraction.receiver(BUS_KECCAK, mu, ts + [k(24)] + [d(c.OUTPUT_STATE + i) for i in order]))
     for lane in range(25):
         its.append(BusInteraction.sender(BUS_IS_WH, mu, [d(c.state_ptr(lane, 2)), d(c.state_ptr(lane, 1))]))
         its.append(BusInteraction.sender(BUS_IS_HALFWORD, mu, [d(c.state_ptr(lane, 0))]))
@@ -2347,5 +2350,232 @@ def dvrm_table_air() -> AirProgram:
     prog = b.finish(c.NUM_COLUMNS)
     pairs, absorbed = split_interactions(len(its))
     return AirProgram("dvrm", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
+                      boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
+                      max_degree=max(3 if pairs else 0, 1 + absorbed, 2))
+
+
+# KECCAK_RND (prover/src/tables/keccak_rnd.rs): the Keccak-f[1600] round chip,
+# one row per round (24 per permutation), every intermediate state as bytes.
+# XOR/AND go to BYTE_ALU lookups; the theta rotate-by-1 and the rho rotations
+# are halfword shifts pinned by mu-gated linear identities over (left, right)
+# byte pairs, the carries of theta's single-bit shifts as bit columns.
+KECCAK_RHO = ((0, 36, 3, 41, 18), (1, 44, 10, 45, 2), (62, 6, 43, 15, 61), (28, 55, 25, 21, 56),
+              (27, 20, 39, 8, 14))  # [x][y], executor/src/vm/instruction/execution.rs:1029-1035
+
+
+class KeccakRndCols:
+    """keccak_rnd.rs:28-90: timestamp (2), round, start (200), Cxz chain (160),
+    Cxz_left (40), Cxz_right carries (20), Dxz (40), theta (200), rot_left and
+    rot_right (200 each), chi_ands (200), chi (200), rc (8), iota (8), mu."""
+    TIMESTAMP_0, TIMESTAMP_1, ROUND = 0, 1, 2
+    START, CXZ, CXZ_LEFT, CXZ_RIGHT, DXZ = 3, 203, 363, 403, 423
+    THETA, ROT_LEFT, ROT_RIGHT, CHI_ANDS, CHI = 463, 663, 863, 1063, 1263
+    RC, IOTA, MU = 1463, 1471, 1479
+    NUM_COLUMNS = 1480
+
+    @staticmethod
+    def lane(base, x, y, byte):
+        """Byte `byte` of lane (x, y) of a 200-byte state group."""
+        return base + (x + 5 * y) * 8 + byte
+
+    @staticmethod
+    def cxz(x, stage, byte):
+        return KeccakRndCols.CXZ + (x * 4 + stage) * 8 + byte
+
+    @staticmethod
+    def cxz_left(x, byte):
+        return KeccakRndCols.CXZ_LEFT + x * 8 + byte
+
+    @staticmethod
+    def cxz_right(x, hw):
+        return KeccakRndCols.CXZ_RIGHT + x * 4 + hw
+
+    @staticmethod
+    def dxz(x, byte):
+        return KeccakRndCols.DXZ + x * 8 + byte
+
+    @staticmethod
+    def pi_src(x, y, z):
+        """keccak_rnd.rs:165-179: the (rot_left, rot_right) cells whose sum is
+        byte z of pi lane (x, y) = rho lane ((x + 3y) mod 5, x): a rotation by
+        16·rbc + rnc takes byte z from the left part of byte z - 2 rbc and the
+        right part of byte z - 2 rbc - 2."""
+        c = KeccakRndCols
+        sx, sy = (x + 3 * y) % 5, x
+        rbc = KECCAK_RHO[sx][sy] // 16
+        return (c.lane(c.ROT_LEFT, sx, sy, (z - 2 * rbc) % 8), c.lane(c.ROT_RIGHT, sx, sy, (z - 2 * rbc - 2) % 8))
+
+
+@dataclass(frozen=True)
+class KeccakRoundOperation:
+    """keccak_rnd.rs:233-238: one permutation (24 rows); lanes indexed x + 5y."""
+    timestamp: int
+    input: tuple
+    output: tuple
+
+
+def _hwsl(h: int, shift: int):
+    """(h << shift) mod 2^16 and h >> (16 - shift), (h, 0) for shift 0."""
+    return (h, 0) if shift == 0 else ((h << shift) & 0xFFFF, h >> (16 - shift))
+
+
+def _rotl64(v: int, r: int) -> int:
+    r %= 64
+    return ((v << r) | (v >> (64 - r))) & (2**64 - 1) if r else v
+
+
+def generate_keccak_rnd_trace(ops) -> np.ndarray:
+    """keccak_rnd.rs:262-448: per operation 24 rows, each round's values at
+    byte granularity; zero rows to a power of two (at least 4)."""
+    c = KeccakRndCols
+    n = max(4, 1 << max(0, (24 * len(ops) - 1).bit_length()))
+    t = np.zeros((n, c.NUM_COLUMNS), dtype=np.uint64)
+    by = lambda v: [(v >> (8 * i)) & 0xFF for i in range(8)]  # noqa: E731
+    lane_of = lambda bs: sum(b << (8 * i) for i, b in enumerate(bs))  # noqa: E731
+    for k, op in enumerate(ops):
+        state = list(op.input)
+        for rnd in range(24):
+            row = t[24 * k + rnd]
+            row[c.TIMESTAMP_0], row[c.TIMESTAMP_1], row[c.ROUND] = op.timestamp & 0xFFFFFFFF, op.timestamp >> 32, rnd
+            for x in range(5):
+                for y in range(5):
+                    row[c.lane(c.START, x, y, 0):c.lane(c.START, x, y, 8)] = by(state[x + 5 * y])
+            # theta: parities as a chain of XORs, then C rotated by one bit
+            cpar = []
+            for x in range(5):
+                acc = by(state[x])
+                for stage in range(4):
+                    acc = [a ^ b for a, b in zip(acc, by(state[x + 5 * (stage + 1)]))]
+                    row[c.cxz(x, stage, 0):c.cxz(x, stage, 8)] = acc
+                cpar.append(acc)
+            rot_c = []
+            for x in range(5):
+                left, carries = [0] * 8, [0] * 4
+                for hw in range(4):
+                    sh, carry = _hwsl(cpar[x][2 * hw] | cpar[x][2 * hw + 1] << 8, 1)
+                    left[2 * hw], left[2 * hw + 1], carries[hw] = sh & 0xFF, sh >> 8, carry
+                row[c.cxz_left(x, 0):c.cxz_left(x, 8)] = left
+                row[c.cxz_right(x, 0):c.cxz_right(x, 4)] = carries
+                # the carry of halfword hw - 1 lands in the low byte of halfword hw
+                rot_c.append([left[b] + (carries[(b // 2 + 3) % 4] if b % 2 == 0 else 0) for b in range(8)])
+            dd = []
+            for x in range(5):
+                d = [a ^ b for a, b in zip(cpar[(x + 4) % 5], rot_c[(x + 1) % 5])]
+                row[c.dxz(x, 0):c.dxz(x, 8)] = d
+                dd.append(lane_of(d))
+            theta = [state[i] ^ dd[i % 5] for i in range(25)]
+            for x in range(5):
+                for y in range(5):
+                    row[c.lane(c.THETA, x, y, 0):c.lane(c.THETA, x, y, 8)] = by(theta[x + 5 * y])
+                    left, right = [0] * 8, [0] * 8
+                    for hw in range(4):
+                        sh, carry = _hwsl((theta[x + 5 * y] >> (16 * hw)) & 0xFFFF, KECCAK_RHO[x][y] % 16)
+                        left[2 * hw], left[2 * hw + 1] = sh & 0xFF, sh >> 8
+                        right[2 * hw], right[2 * hw + 1] = carry & 0xFF, carry >> 8
+                    row[c.lane(c.ROT_LEFT, x, y, 0):c.lane(c.ROT_LEFT, x, y, 8)] = left
+                    row[c.lane(c.ROT_RIGHT, x, y, 0):c.lane(c.ROT_RIGHT, x, y, 8)] = right
+            pi = [0] * 25
+            for x in range(5):
+                for y in range(5):
+                    pi[y + 5 * ((2 * x + 3 * y) % 5)] = _rotl64(theta[x + 5 * y], KECCAK_RHO[x][y])
+            chi = [0] * 25
+            for x in range(5):
+                for y in range(5):
+                    a = ~pi[(x + 1) % 5 + 5 * y] & pi[(x + 2) % 5 + 5 * y] & (2**64 - 1)
+                    chi[x + 5 * y] = pi[x + 5 * y] ^ a
+                    row[c.lane(c.CHI_ANDS, x, y, 0):c.lane(c.CHI_ANDS, x, y, 8)] = by(a)
+                    row[c.lane(c.CHI, x, y, 0):c.lane(c.CHI, x, y, 8)] = by(chi[x + 5 * y])
+            rc = KECCAK_ROUND_CONSTANTS[rnd]
+            row[c.RC:c.RC + 8] = by(rc)
+            chi[0] ^= rc
+            row[c.IOTA:c.IOTA + 8] = by(chi[0])
+            row[c.MU] = 1
+            state = chi
+    return t
+
+
+def keccak_rnd_bus_interactions():
+    """keccak_rnd.rs:461-860, in its order (1031): the Keccak receiver [ts,
+    round, start bytes] and sender [ts, round + 1, round output bytes] (x-major,
+    iota in lane (0, 0)), the KeccakRc lookup [round, rc bytes] (all mu); the
+    BYTE_ALU XORs of the parity chain (160), ARE_BYTES of the Cxz_left pairs
+    (20, multiplicity one), the XORs of D (40) — the rotated parity byte as
+    Cxz_left plus, on even bytes, a carry bit — and of theta (200), ARE_BYTES of
+    the (rot_left, rot_right) pairs (200, one), chi's ANDs of 255 - pi and pi
+    (200) and XORs (200) with pi as rot_left + rot_right cells, iota's XORs
+    (8)."""
+    c = KeccakRndCols
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    k = lambda v: BusValue.of_linear([LinearTerm(v)])  # noqa: E731
+    lin = lambda *cols: BusValue.of_linear([LinearTerm(1, col) for col in cols])  # noqa: E731
+    mu = Multiplicity("Column", (c.MU,))
+    xyz = [(x, y, b) for x in range(5) for y in range(5) for b in range(8)]
+    ts = [d(c.TIMESTAMP_0), d(c.TIMESTAMP_1)]
+    out_col = lambda x, y, b: c.IOTA + b if (x, y) == (0, 0) else c.lane(c.CHI, x, y, b)  # noqa: E731
+    its = [BusInteraction.receiver(BUS_KECCAK, mu, ts + [d(c.ROUND)] + [d(c.lane(c.START, *p)) for p in xyz]),
+           BusInteraction.sender(BUS_KECCAK, mu, ts + [BusValue.of_linear([LinearTerm(1, c.ROUND), LinearTerm(1)])]
+                                 + [d(out_col(*p)) for p in xyz]),
+           BusInteraction.sender(BUS_KECCAK_RC, mu, [d(c.ROUND)] + [d(c.RC + b) for b in range(8)])]
+    xor = lambda a, b_, out: BusInteraction.sender(BUS_BYTE_ALU, mu, [k(ALU_OP_XOR), a, b_, d(out)])  # noqa: E731
+    its += [xor(d(c.lane(c.START, x, 0, b)), d(c.lane(c.START, x, 1, b)), c.cxz(x, 0, b))
+            for x in range(5) for b in range(8)]
+    its += [xor(d(c.cxz(x, s - 1, b)), d(c.lane(c.START, x, s + 1, b)), c.cxz(x, s, b))
+            for x in range(5) for s in range(1, 4) for b in range(8)]
+    its += [BusInteraction.sender(BUS_ARE_BYTES, ONE, [d(c.cxz_left(x, 2 * i)), d(c.cxz_left(x, 2 * i + 1))])
+            for x in range(5) for i in range(4)]
+    for x in range(5):
+        for b in range(8):
+            xr = (x + 1) % 5
+            rot = lin(c.cxz_left(xr, b), c.cxz_right(xr, (b // 2 + 3) % 4)) if b % 2 == 0 else lin(c.cxz_left(xr, b))
+            its.append(xor(d(c.cxz((x + 4) % 5, 3, b)), rot, c.dxz(x, b)))
+    its += [xor(d(c.lane(c.START, x, y, b)), d(c.dxz(x, b)), c.lane(c.THETA, x, y, b)) for x, y, b in xyz]
+    its += [BusInteraction.sender(BUS_ARE_BYTES, ONE, [d(c.lane(c.ROT_LEFT, x, y, b)), d(c.lane(c.ROT_RIGHT, x, y, b))])
+            for x, y, b in xyz]
+    for x, y, b in xyz:
+        p1, p2 = c.pi_src((x + 1) % 5, y, b), c.pi_src((x + 2) % 5, y, b)
+        its.append(BusInteraction.sender(BUS_BYTE_ALU, mu, [
+            k(ALU_OP_AND), BusValue.of_linear([LinearTerm(255), LinearTerm(-1, p1[0]), LinearTerm(-1, p1[1])]),
+            lin(*p2), d(c.lane(c.CHI_ANDS, x, y, b))]))
+    its += [xor(lin(*c.pi_src(x, y, b)), d(c.lane(c.CHI_ANDS, x, y, b)), c.lane(c.CHI, x, y, b)) for x, y, b in xyz]
+    its += [xor(d(c.lane(c.CHI, 0, 0, b)), d(c.RC + b), c.IOTA + b) for b in range(8)]
+    return its
+
+
+def keccak_rnd_table_air() -> AirProgram:
+    """KECCAK_RND as an AIR (keccak_rnd.rs:862-951, 141 constraints of degree
+    2): 0-19 the theta carries are bits; 20-39 mu·(C_hw·2 - carry·2^16 -
+    left_hw) per parity halfword; 40-139 mu·(theta_hw·2^rnc - right_hw·2^16 -
+    left_hw) per lane and halfword (rnc = rho mod 16); 140 mu a bit; then the
+    LogUp constraints of its 1031 interactions."""
+    c = KeccakRndCols
+    b = ir.IrBuilder()
+    m = lambda col: b.main(0, col)  # noqa: E731
+    two16, c256, one = b.const_base(1 << 16), b.const_base(256), b.const_base(1)
+    half = lambda lo: m(lo) + m(lo + 1) * c256  # noqa: E731
+    idx = 0
+    for x in range(5):
+        for hw in range(4):
+            _emit_is_bit(b, idx, c.cxz_right(x, hw))
+            idx += 1
+    for x in range(5):
+        for hw in range(4):
+            ident = half(c.cxz(x, 3, 2 * hw)) * b.const_base(2) - m(c.cxz_right(x, hw)) * two16 - half(c.cxz_left(x, 2 * hw))
+            b.emit(idx, m(c.MU) * ident)
+            idx += 1
+    for x in range(5):
+        for y in range(5):
+            pw = b.const_base(1 << (KECCAK_RHO[x][y] % 16))
+            for hw in range(4):
+                ident = (half(c.lane(c.THETA, x, y, 2 * hw)) * pw - half(c.lane(c.ROT_RIGHT, x, y, 2 * hw)) * two16
+                         - half(c.lane(c.ROT_LEFT, x, y, 2 * hw)))
+                b.emit(idx, m(c.MU) * ident)
+                idx += 1
+    b.emit(idx, m(c.MU) * (one - m(c.MU)))
+    idx += 1
+    its = keccak_rnd_bus_interactions()
+    n_total = emit_logup_constraints(b, its, idx)
+    prog = b.finish(c.NUM_COLUMNS)
+    pairs, absorbed = split_interactions(len(its))
+    return AirProgram("keccak_rnd", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
                       boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
                       max_degree=max(3 if pairs else 0, 1 + absorbed, 2))

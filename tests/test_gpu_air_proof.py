@@ -522,3 +522,27 @@ def test_gpu_dvrm_table_proof(gpu):
     ref = replay_air_tables(tables, 4, 3, 2, 8, 0, b"gpu-dvrm")
     assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
     assert verify_air_tables([dvrm, mul], got, 4, 3, 2, 8, 0, b"gpu-dvrm")[0]
+
+
+def test_gpu_keccak_rnd_table_proof(gpu):
+    """KECCAK_RND (1480 columns, 1031 interactions, 516 aux Fp3 columns; its
+    constraint program exceeds the run-time compiler's node cap and runs on
+    the interpreter): the GPU proof of two permutations equals the oracle
+    replay byte for byte and verifies."""
+    from lambda_vm_amd import proof_io, prover
+    from lambda_vm_amd import vm_tables as V
+    from lambda_vm_amd.prover import AirTableProof
+    from tests.test_vm_table_keccak_rnd import rnd_ops
+
+    print("[gpu-air] KECCAK_RND 64 x 1480, blowup 2", flush=True)
+    rnd = V.keccak_rnd_table_air()
+    rnd.bus_descriptor = A.descriptor(rnd)
+    tables = [(rnd, np.ascontiguousarray(V.generate_keccak_rnd_trace(rnd_ops(2, 91)), dtype=np.uint64))]
+    opts = prover.ProofOptions(blowup=2, coset_offset=3, fri_final_log_degree=2, num_queries=6)
+    dev = [(a, gpu.DeviceBuffer.from_array(tr), tr.shape[0]) for a, tr in tables]
+    got = prover.prove_air_tables(dev, opts, label=b"gpu-keccak-rnd")
+    for _, d, _ in dev:
+        d.free()
+    ref = replay_air_tables(tables, 2, 3, 2, 6, 0, b"gpu-keccak-rnd")
+    assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
+    assert verify_air_tables([rnd], got, 2, 3, 2, 6, 0, b"gpu-keccak-rnd")[0]
