@@ -2579,3 +2579,228 @@ def keccak_rnd_table_air() -> AirProgram:
     return AirProgram("keccak_rnd", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
                       boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
                       max_degree=max(3 if pairs else 0, 1 + absorbed, 2))
+
+
+# CPU32 (prover/src/tables/cpu32.rs): the 32-bit word instructions (*W) the
+# CPU delegates; decodes through DECODE, reads rs1/rs2, extends the low words
+# (sign or zero by the signed bit of alu_flags), runs the ALU or the ADD/SUB
+# fast path and writes rd with the sign-extended low word of the result.
+BUS_DECODE, BUS_CPU32 = 18, 27      # types.rs:309, :343
+ALU_OP_SHIFTW = 6                   # types.rs:568
+# packed_decode bit positions (types.rs:586-604)
+PD_READ_REG1, PD_READ_REG2, PD_WRITE_REG, PD_WORD_INSTR, PD_ALU, PD_ADD, PD_SUB = 0, 1, 2, 3, 4, 5, 6
+PD_MEMORY, PD_BRANCH, PD_ECALL, PD_RS1, PD_RS2, PD_RD = 7, 8, 9, 10, 18, 26
+PD_HALF_INSTRUCTION_LENGTH, PD_ALU_FLAGS, PD_MEM_FLAGS = 34, 42, 50
+ALU_FLAGS_SIGNED = 5
+CPU32_HI_FILL = 0xFFFFFFFF
+
+# (alu op, signed, signed2 | invert, muldiv, add, sub) of each RISC-V arithmetic
+# op (types.rs:870-895); "shift" stands for SHIFT, or SHIFTW on word forms
+ARITH_OP_FLAGS = {
+    "add": (0, False, False, False, True, False), "sub": (0, False, False, False, False, True),
+    "and": (ALU_OP_AND, False, False, False, False, False), "or": (ALU_OP_OR, False, False, False, False, False),
+    "xor": (ALU_OP_XOR, False, False, False, False, False), "sll": ("shift", False, False, False, False, False),
+    "srl": ("shift", False, True, False, False, False), "sra": ("shift", True, True, False, False, False),
+    "slt": (ALU_OP_LT, True, False, False, False, False), "sltu": (ALU_OP_LT, False, False, False, False, False),
+    "mul": (ALU_OP_MUL, True, True, False, False, False), "mulh": (ALU_OP_MUL, True, True, True, False, False),
+    "mulhsu": (ALU_OP_MUL, True, False, True, False, False), "mulhu": (ALU_OP_MUL, False, False, True, False, False),
+    "div": (ALU_OP_DIVREM, True, False, False, False, False), "divu": (ALU_OP_DIVREM, False, False, False, False, False),
+    "rem": (ALU_OP_DIVREM, True, False, True, False, False), "remu": (ALU_OP_DIVREM, False, False, True, False, False),
+}
+
+
+def arith_decode(op: str, word_instr: bool):
+    """alu_flags byte and the alu/add/sub flags of an arithmetic op
+    (types.rs:864-900, build_alu_flags: op + 32 signed + 64 signed2|invert +
+    128 muldiv)."""
+    alu, signed, s2, muldiv, add, sub = ARITH_OP_FLAGS[op]
+    if alu == "shift":
+        alu = ALU_OP_SHIFTW if word_instr else ALU_OP_SHIFT
+    return alu + 32 * signed + 64 * s2 + 128 * muldiv, not (add or sub), add, sub
+
+
+class Cpu32Cols:
+    """cpu32.rs:37-98."""
+    TIMESTAMP_0, TIMESTAMP_1, PC_0, PC_1 = 0, 1, 2, 3
+    RS1, READ_REGISTER1, RV1 = 4, 5, (6, 7, 8)
+    RV1_SIGN, ARG1_0, ARG1_1 = 9, 10, 11
+    RS2, READ_REGISTER2, RV2 = 12, 13, (14, 15, 16)
+    RV2_SIGN, IMM_0, IMM_1, ARG2_0, ARG2_1 = 17, 18, 19, 20, 21
+    RES = (22, 23, 24, 25)
+    RES_SIGN, RD, WRITE_REGISTER, RVD_0, RVD_1 = 26, 27, 28, 29, 30
+    ALU, ALU_FLAGS, ADD, SUB, HALF_INSTRUCTION_LENGTH, SIGNED, MU = 31, 32, 33, 34, 35, 36, 37
+    NUM_COLUMNS = 38
+
+
+@dataclass(frozen=True)
+class Cpu32Operation:
+    """cpu32.rs:108-190: res is the raw 64-bit ALU (or ADD/SUB) result."""
+    timestamp: int
+    pc: int
+    rs1: int
+    read_register1: bool
+    rv1: int
+    rs2: int
+    read_register2: bool
+    rv2: int
+    imm: int
+    res: int
+    rd: int
+    write_register: bool
+    alu: bool
+    alu_flags: int
+    add: bool
+    sub: bool
+    half_instruction_length: int
+
+    def signed(self) -> bool:
+        return bool(self.alu_flags >> ALU_FLAGS_SIGNED & 1)
+
+    def aux(self):
+        """cpu32.rs:150-188: rv1/rv2 signs (gated by signed), arg1 = ext(rv1
+        low word), arg2 = ext(rv2 low word) + imm per word, res sign, rvd =
+        the sign-extended low word of res."""
+        s = self.signed()
+        rv1_sign = s and (self.rv1 >> 31) & 1 == 1
+        rv2_sign = s and (self.rv2 >> 31) & 1 == 1
+        res_sign = (self.res >> 31) & 1 == 1
+        arg1 = (self.rv1 & 0xFFFFFFFF) | (CPU32_HI_FILL << 32 if rv1_sign else 0)
+        lo = ((self.rv2 & 0xFFFFFFFF) + (self.imm & 0xFFFFFFFF)) & 0xFFFFFFFF
+        hi = ((CPU32_HI_FILL if rv2_sign else 0) + (self.imm >> 32)) & 0xFFFFFFFF
+        rvd = (self.res & 0xFFFFFFFF) | (CPU32_HI_FILL << 32 if res_sign else 0)
+        return s, rv1_sign, arg1, rv2_sign, lo | hi << 32, res_sign, rvd
+
+
+def generate_cpu32_trace(ops) -> np.ndarray:
+    """cpu32.rs:192-247: one row per operation (mu = 1), zero rows to a power
+    of two (at least 4)."""
+    c = Cpu32Cols
+    n = max(4, 1 << max(0, (len(ops) - 1).bit_length()))
+    t = np.zeros((n, c.NUM_COLUMNS), dtype=np.uint64)
+    whh = lambda v: (v & 0xFFFF, (v >> 16) & 0xFFFF, v >> 32)  # noqa: E731
+    for i, op in enumerate(ops):
+        signed, rv1_sign, arg1, rv2_sign, arg2, res_sign, rvd = op.aux()
+        r = t[i]
+        r[c.TIMESTAMP_0], r[c.TIMESTAMP_1] = op.timestamp & 0xFFFFFFFF, op.timestamp >> 32
+        r[c.PC_0], r[c.PC_1] = op.pc & 0xFFFFFFFF, op.pc >> 32
+        r[c.RS1], r[c.READ_REGISTER1], r[c.RV1_SIGN] = op.rs1, int(op.read_register1), int(rv1_sign)
+        r[c.RV1[0]:c.RV1[2] + 1] = whh(op.rv1)
+        r[c.ARG1_0], r[c.ARG1_1] = arg1 & 0xFFFFFFFF, arg1 >> 32
+        r[c.RS2], r[c.READ_REGISTER2], r[c.RV2_SIGN] = op.rs2, int(op.read_register2), int(rv2_sign)
+        r[c.RV2[0]:c.RV2[2] + 1] = whh(op.rv2)
+        r[c.IMM_0], r[c.IMM_1] = op.imm & 0xFFFFFFFF, op.imm >> 32
+        r[c.ARG2_0], r[c.ARG2_1] = arg2 & 0xFFFFFFFF, arg2 >> 32
+        r[c.RES[0]:c.RES[3] + 1] = [(op.res >> (16 * k)) & 0xFFFF for k in range(4)]
+        r[c.RES_SIGN], r[c.RD], r[c.WRITE_REGISTER] = int(res_sign), op.rd, int(op.write_register)
+        r[c.RVD_0], r[c.RVD_1] = rvd & 0xFFFFFFFF, rvd >> 32
+        r[c.ALU], r[c.ALU_FLAGS], r[c.ADD], r[c.SUB] = int(op.alu), op.alu_flags, int(op.add), int(op.sub)
+        r[c.HALF_INSTRUCTION_LENGTH], r[c.SIGNED], r[c.MU] = op.half_instruction_length, int(signed), 1
+    return t
+
+
+def _register_dword(lo0, lo1, hi):
+    """A register value as the 8-element MEMW value: [lo0 + 2^16 lo1, hi, 0 x 6]
+    (cpu32.rs:255-274)."""
+    return [BusValue.of_linear([LinearTerm(1, lo0), LinearTerm(1 << 16, lo1)]), BusValue.packed(Packing.DIRECT, hi)] \
+        + [BusValue.of_linear([LinearTerm(0)])] * 6
+
+
+def _cpu32_ts(offset):
+    return [BusValue.of_linear([LinearTerm(1, Cpu32Cols.TIMESTAMP_0), LinearTerm(offset)]),
+            BusValue.packed(Packing.DIRECT, Cpu32Cols.TIMESTAMP_1)]
+
+
+def cpu32_bus_interactions():
+    """cpu32.rs:359-560, in its order (23): DECODE[pc, imm, packed_decode] (mu;
+    word_instr the constant bit); ARE_BYTES[x, 0] of half_instruction_length,
+    alu_flags, rs1, rs2, rd (mu); IS_HALFWORD of the rv1/rv2 low halves and the
+    four res halves (mu); the MEMW register reads of rs1 at ts and rs2 at ts + 1
+    (24 elements, with read_register1/2) and the write of rd at ts + 2 (16, with
+    write_register); ALU[arg1, arg2, alu_flags, res] (alu); BYTE_ALU[AND, 32,
+    alu_flags, 32 signed] (mu); MSB16 of rv1[1], rv2[1] (signed) and res[1]
+    (mu); the CPU32 receiver [ts, pc, half_instruction_length] (mu)."""
+    c = Cpu32Cols
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    k = lambda v: BusValue.of_linear([LinearTerm(v)])  # noqa: E731
+    col = lambda cc: Multiplicity("Column", (cc,))  # noqa: E731
+    mu = col(c.MU)
+    packed = [LinearTerm(1 << PD_READ_REG1, c.READ_REGISTER1), LinearTerm(1 << PD_READ_REG2, c.READ_REGISTER2),
+              LinearTerm(1 << PD_WRITE_REG, c.WRITE_REGISTER), LinearTerm(1 << PD_WORD_INSTR),
+              LinearTerm(1 << PD_ALU, c.ALU), LinearTerm(1 << PD_ADD, c.ADD), LinearTerm(1 << PD_SUB, c.SUB),
+              LinearTerm(1 << PD_RS1, c.RS1), LinearTerm(1 << PD_RS2, c.RS2), LinearTerm(1 << PD_RD, c.RD),
+              LinearTerm(1 << PD_HALF_INSTRUCTION_LENGTH, c.HALF_INSTRUCTION_LENGTH),
+              LinearTerm(1 << PD_ALU_FLAGS, c.ALU_FLAGS)]
+    its = [BusInteraction.sender(BUS_DECODE, mu, [BusValue.packed(Packing.DWORD_WL, c.PC_0),
+                                                  BusValue.packed(Packing.DWORD_WL, c.IMM_0),
+                                                  BusValue.of_linear(packed)])]
+    its += [BusInteraction.sender(BUS_ARE_BYTES, mu, [d(cc), k(0)])
+            for cc in (c.HALF_INSTRUCTION_LENGTH, c.ALU_FLAGS, c.RS1, c.RS2, c.RD)]
+    its += [BusInteraction.sender(BUS_IS_HALFWORD, mu, [d(cc)])
+            for cc in (c.RV1[0], c.RV1[1], c.RV2[0], c.RV2[1]) + c.RES]
+    for rs, rv, off, flag in ((c.RS1, c.RV1, 0, c.READ_REGISTER1), (c.RS2, c.RV2, 1, c.READ_REGISTER2)):
+        its.append(BusInteraction.sender(BUS_MEMW, col(flag), _register_dword(*rv) + [
+            k(1), BusValue.of_linear([LinearTerm(2, rs)]), k(0)] + _register_dword(*rv) + _cpu32_ts(off)
+            + [k(1), k(0), k(0)]))
+    its.append(BusInteraction.sender(BUS_MEMW, col(c.WRITE_REGISTER), [
+        k(1), BusValue.of_linear([LinearTerm(2, c.RD)]), k(0), d(c.RVD_0), d(c.RVD_1)] + [k(0)] * 6
+        + _cpu32_ts(2) + [k(1), k(0), k(0)]))
+    its.append(BusInteraction.sender(BUS_ALU, col(c.ALU), [
+        BusValue.packed(Packing.DWORD_WL, c.ARG1_0), BusValue.packed(Packing.DWORD_WL, c.ARG2_0), d(c.ALU_FLAGS),
+        BusValue.packed(Packing.DWORD_HL, c.RES[0])]))
+    its.append(BusInteraction.sender(BUS_BYTE_ALU, mu, [
+        k(ALU_OP_AND), k(1 << ALU_FLAGS_SIGNED), d(c.ALU_FLAGS),
+        BusValue.of_linear([LinearTerm(1 << ALU_FLAGS_SIGNED, c.SIGNED)])]))
+    its += [BusInteraction.sender(BUS_MSB16, col(gate), [d(half), d(sign)])
+            for half, sign, gate in ((c.RV1[1], c.RV1_SIGN, c.SIGNED), (c.RV2[1], c.RV2_SIGN, c.SIGNED),
+                                     (c.RES[1], c.RES_SIGN, c.MU))]
+    its.append(BusInteraction.receiver(BUS_CPU32, mu, [
+        BusValue.packed(Packing.DWORD_WL, c.TIMESTAMP_0), BusValue.packed(Packing.DWORD_WL, c.PC_0),
+        d(c.HALF_INSTRUCTION_LENGTH)]))
+    return its
+
+
+def cpu32_table_air() -> AirProgram:
+    """CPU32 as an AIR (cpu32.rs:600-724, 32 constraints, degree 3): 0-6 the
+    flags and mu are bits; 7-8 the ADD carry pair of arg1 + arg2 = res (gated
+    by add) and 9-10 of arg2 + res = arg1 (gated by sub); 11-16 unread register
+    limbs are zero; 17-22 arg1, arg2 and rvd are the extensions of rv1, rv2
+    (+ imm) and res; 23-24 no sign on unsigned rows; 25-26 no immediate when
+    rs2 is read; 27-31 no register flag, signed or res sign on padding rows;
+    then the LogUp constraints of its 23 interactions."""
+    c = Cpu32Cols
+    b = ir.IrBuilder()
+    m = lambda col: b.main(0, col)  # noqa: E731
+    one, inv, s16, fill = b.const_base(1), b.const_base(INV_SHIFT_32), b.const_base(1 << 16), b.const_base(CPU32_HI_FILL)
+    for i, col in enumerate((c.READ_REGISTER1, c.READ_REGISTER2, c.WRITE_REGISTER, c.ALU, c.ADD, c.SUB, c.MU)):
+        _emit_is_bit(b, i, col)
+    res = (m(c.RES[0]) + m(c.RES[1]) * s16, m(c.RES[2]) + m(c.RES[3]) * s16)
+    arg1, arg2 = (m(c.ARG1_0), m(c.ARG1_1)), (m(c.ARG2_0), m(c.ARG2_1))
+    for idx, cond, lhs, rhs, out in ((7, c.ADD, arg1, arg2, res), (9, c.SUB, arg2, res, arg1)):
+        carry0 = (lhs[0] + rhs[0] - out[0]) * inv
+        carry1 = (lhs[1] + rhs[1] + carry0 - out[1]) * inv
+        b.emit(idx, m(cond) * carry0 * (one - carry0))
+        b.emit(idx + 1, m(cond) * carry1 * (one - carry1))
+    idx = 11
+    for read, limbs in ((c.READ_REGISTER1, c.RV1), (c.READ_REGISTER2, c.RV2)):
+        for limb in limbs:
+            b.emit(idx, (one - m(read)) * m(limb))
+            idx += 1
+    b.emit(17, m(c.ARG1_0) - m(c.RV1[0]) - s16 * m(c.RV1[1]))
+    b.emit(18, m(c.ARG1_1) - fill * m(c.RV1_SIGN))
+    b.emit(19, m(c.ARG2_0) - m(c.RV2[0]) - s16 * m(c.RV2[1]) - m(c.IMM_0))
+    b.emit(20, m(c.ARG2_1) - fill * m(c.RV2_SIGN) - m(c.IMM_1))
+    b.emit(21, m(c.RVD_0) - m(c.RES[0]) - s16 * m(c.RES[1]))
+    b.emit(22, m(c.RVD_1) - fill * m(c.RES_SIGN))
+    for i, sign in enumerate((c.RV1_SIGN, c.RV2_SIGN)):
+        b.emit(23 + i, (one - m(c.SIGNED)) * m(sign))
+    for i, imm in enumerate((c.IMM_0, c.IMM_1)):
+        b.emit(25 + i, m(c.READ_REGISTER2) * m(imm))
+    for i, flag in enumerate((c.READ_REGISTER1, c.READ_REGISTER2, c.WRITE_REGISTER, c.SIGNED, c.RES_SIGN)):
+        b.emit(27 + i, (one - m(c.MU)) * m(flag))
+    its = cpu32_bus_interactions()
+    n_total = emit_logup_constraints(b, its, 32)
+    prog = b.finish(c.NUM_COLUMNS)
+    pairs, absorbed = split_interactions(len(its))
+    return AirProgram("cpu32", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
+                      boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
+                      max_degree=max(3, 1 + absorbed))

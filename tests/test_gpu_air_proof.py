@@ -546,3 +546,29 @@ def test_gpu_keccak_rnd_table_proof(gpu):
     ref = replay_air_tables(tables, 2, 3, 2, 6, 0, b"gpu-keccak-rnd")
     assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
     assert verify_air_tables([rnd], got, 2, 3, 2, 6, 0, b"gpu-keccak-rnd")[0]
+
+
+def test_gpu_cpu32_table_proof(gpu):
+    """CPU32 (38 columns, 32 degree-3 constraints, 23 interactions) beside the
+    SHIFT chip its word shifts use: the GPU proofs equal the oracle replay byte
+    for byte and verify."""
+    from lambda_vm_amd import proof_io, prover
+    from lambda_vm_amd import vm_tables as V
+    from lambda_vm_amd.prover import AirTableProof
+    from tests.test_vm_table_cpu32 import chip_ops, cpu32_ops
+
+    print("[gpu-air] CPU32 512 x 38 + SHIFT, blowup 4", flush=True)
+    ops, _ = cpu32_ops(500, 101)
+    cpu, sh = V.cpu32_table_air(), V.shift_table_air()
+    for a in (cpu, sh):
+        a.bus_descriptor = A.descriptor(a)
+    tables = [(cpu, V.generate_cpu32_trace(ops)), (sh, V.generate_shift_trace(chip_ops(ops)[0]))]
+    tables = [(a, np.ascontiguousarray(tr, dtype=np.uint64)) for a, tr in tables]
+    opts = prover.ProofOptions(blowup=4, coset_offset=3, fri_final_log_degree=2, num_queries=8)
+    dev = [(a, gpu.DeviceBuffer.from_array(tr), tr.shape[0]) for a, tr in tables]
+    got = prover.prove_air_tables(dev, opts, label=b"gpu-cpu32")
+    for _, d, _ in dev:
+        d.free()
+    ref = replay_air_tables(tables, 4, 3, 2, 8, 0, b"gpu-cpu32")
+    assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
+    assert verify_air_tables([cpu, sh], got, 4, 3, 2, 8, 0, b"gpu-cpu32")[0]
