@@ -2804,3 +2804,279 @@ def cpu32_table_air() -> AirProgram:
     return AirProgram("cpu32", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
                       boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
                       max_degree=max(3, 1 + absorbed))
+
+
+# DECODE (prover/src/tables/decode.rs): the program's decoded instructions as
+# fixed columns (pc, packed_decode, imm) with a multiplicity, one receiver of
+# the CPU/CPU32 DECODE lookups. The RV64IM decoder (executor/src/vm/
+# instruction/decoding.rs) and the decode-word layout (types.rs:580-1100) are
+# restated with it.
+@dataclass(frozen=True)
+class Instruction:
+    """One decoded instruction (decoding.rs:172-240). kind: arith, arith_imm,
+    arith_w, arith_imm_w, jal, jalr, store, load, branch, lui, auipc, csr,
+    ecall, fence; op an ARITH_OP_FLAGS key, width b/h/w/d/bu/hu/wu, cond
+    eq/ne/lt/ge/ltu/geu."""
+    kind: str
+    dst: int = 0
+    src1: int = 0
+    src2: int = 0
+    imm: int = 0
+    op: str = ""
+    width: str = ""
+    cond: str = ""
+    csr: int = 0
+
+
+_R_OPS = {(0, 0x00): "add", (0, 0x20): "sub", (4, 0): "xor", (6, 0): "or", (7, 0): "and", (1, 0): "sll",
+          (5, 0): "srl", (5, 0x20): "sra", (2, 0): "slt", (3, 0): "sltu", (0, 1): "mul", (1, 1): "mulh",
+          (2, 1): "mulhsu", (3, 1): "mulhu", (4, 1): "div", (5, 1): "divu", (6, 1): "rem", (7, 1): "remu"}
+_W_OPS = {"add", "sub", "sll", "srl", "sra", "mul", "div", "divu", "rem", "remu"}
+_LOAD_WIDTH = {0: "b", 1: "h", 2: "w", 3: "d", 4: "bu", 5: "hu", 6: "wu"}
+_BRANCH_COND = {0: "eq", 1: "ne", 4: "lt", 5: "ge", 6: "ltu", 7: "geu"}
+
+
+def decode_instruction(word: int) -> Instruction:
+    """decoding.rs:255-600: the RV64IM decoder (I-type immediates sign-extended
+    from 12 bits, shift immediates 6 bits — 5 for the word forms — with their
+    function bits checked; ECALL/EBREAK one instruction; CSRRW/CSRRS/CSRRCI
+    accepted, the other CSR forms rejected). Raises ValueError on an encoding
+    the reference rejects."""
+    word &= 0xFFFFFFFF
+    opc, rd, f3 = word & 0x7F, (word >> 7) & 0x1F, (word >> 12) & 7
+    rs1, rs2, f7 = (word >> 15) & 0x1F, (word >> 20) & 0x1F, word >> 25
+    neg = word >> 31 == 1
+    if opc in (0b0110011, 0b0111011):  # R: OP, OP-32
+        op = _R_OPS.get((f3, f7))
+        if op is None:
+            raise ValueError(f"unknown func3/func7 {f3:#x}/{f7:#x}")
+        if opc == 0b0110011:
+            return Instruction("arith", rd, rs1, rs2, op=op)
+        if op not in _W_OPS:
+            raise ValueError("invalid word instruction")
+        return Instruction("arith_w", rd, rs1, rs2, op=op)
+    if opc in (0b0010011, 0b0011011, 0b0000011, 0b1100111, 0b1110011, 0b0001111):  # I
+        csr = word >> 20
+        imm = csr - (1 << 12) if neg else csr
+        if opc == 0b0010011:
+            ops = {0: "add", 4: "xor", 6: "or", 7: "and", 2: "slt", 3: "sltu"}
+            if f3 == 1:
+                if imm >> 6 != 0:
+                    raise ValueError("unknown shift-left variant")
+                op, imm = "sll", imm & 0x3F
+            elif f3 == 5:
+                fid, imm = imm >> 6, imm & 0x3F
+                op = {0: "srl", 0x10: "sra"}.get(fid)
+                if op is None:
+                    raise ValueError("unknown shift-right variant")
+            else:
+                op = ops[f3]
+            return Instruction("arith_imm", rd, rs1, imm=imm, op=op)
+        if opc == 0b0011011:
+            if f3 == 0:
+                op = "add"
+            elif f3 == 1:
+                if imm >> 5 != 0:
+                    raise ValueError("unknown shift-left variant")
+                op, imm = "sll", imm & 0x1F
+            elif f3 == 5:
+                fid, imm = imm >> 5, imm & 0x1F
+                op = {0: "srl", 0x20: "sra"}.get(fid)
+                if op is None:
+                    raise ValueError("unknown shift-right variant")
+            else:
+                raise ValueError("invalid word instruction")
+            return Instruction("arith_imm_w", rd, rs1, imm=imm, op=op)
+        if opc == 0b1100111:
+            if f3 != 0:
+                raise ValueError("invalid JALR")
+            return Instruction("jalr", rd, rs1, imm=imm)
+        if opc == 0b0000011:
+            if f3 not in _LOAD_WIDTH:
+                raise ValueError("invalid load width")
+            return Instruction("load", rd, rs1, imm=imm, width=_LOAD_WIDTH[f3])
+        if opc == 0b1110011:
+            if f3 == 0:
+                return Instruction("ecall")
+            if f3 in (1, 2, 7):
+                return Instruction("csr", rd, rs1, op={1: "csrrw", 2: "csrrs", 7: "csrrci"}[f3], csr=csr)
+            raise ValueError("invalid system instruction")
+        return Instruction("fence")
+    if opc == 0b0100011:  # S
+        imm = ((word >> 25) << 5) | rd
+        imm = imm - (1 << 12) if neg else imm
+        if f3 > 3:
+            raise ValueError("invalid store width")
+        return Instruction("store", 0, rs1, rs2, imm=imm, width=_LOAD_WIDTH[f3])
+    if opc == 0b1100011:  # B
+        imm = ((word >> 20) & 0x7E0) | ((word >> 7) & 0x1E) | ((word & 0x80) << 4)
+        imm = imm - (1 << 12) if neg else imm
+        if f3 not in _BRANCH_COND:
+            raise ValueError("unknown branch condition")
+        return Instruction("branch", 0, rs1, rs2, imm=imm, cond=_BRANCH_COND[f3])
+    if opc == 0b1101111:  # J
+        imm = (word & 0xFF000) | ((word & 0x100000) >> 9) | ((word >> 20) & 0x7FE)
+        return Instruction("jal", rd, imm=imm - (1 << 20) if neg else imm)
+    if opc in (0b0110111, 0b0010111):  # U
+        return Instruction("lui" if opc == 0b0110111 else "auipc", rd, imm=word & 0xFFFFF000)
+    raise ValueError(f"unknown opcode {opc:#x}")
+
+
+@dataclass(frozen=True)
+class ShrunkDecode:
+    """types.rs:655-700: the decode fields packed into one field element."""
+    read_register1: bool = False
+    read_register2: bool = False
+    write_register: bool = False
+    word_instr: bool = False
+    alu: bool = False
+    add: bool = False
+    sub: bool = False
+    memory: bool = False
+    branch: bool = False
+    ecall: bool = False
+    rs1: int = 0
+    rs2: int = 0
+    rd: int = 0
+    half_instruction_length: int = 0
+    alu_flags: int = 0
+    mem_flags: int = 0
+
+    def pack(self) -> int:
+        """types.rs:702-720 (58 bits)."""
+        bits = ((self.read_register1, PD_READ_REG1), (self.read_register2, PD_READ_REG2),
+                (self.write_register, PD_WRITE_REG), (self.word_instr, PD_WORD_INSTR), (self.alu, PD_ALU),
+                (self.add, PD_ADD), (self.sub, PD_SUB), (self.memory, PD_MEMORY), (self.branch, PD_BRANCH),
+                (self.ecall, PD_ECALL), (self.rs1, PD_RS1), (self.rs2, PD_RS2), (self.rd, PD_RD),
+                (self.half_instruction_length, PD_HALF_INSTRUCTION_LENGTH), (self.alu_flags, PD_ALU_FLAGS),
+                (self.mem_flags, PD_MEM_FLAGS))
+        return sum(int(v) << pos for v, pos in bits)
+
+
+def build_mem_flags(jalr_or_op, signed, m2, m4, m8) -> int:
+    """types.rs:635-648: JALR|memory_op, signed, 2-, 4-, 8-byte bits."""
+    return int(jalr_or_op) | int(signed) << 1 | int(m2) << 2 | int(m4) << 3 | int(m8) << 4
+
+
+_STORE_WIDTH_BITS = {"b": (0, 0, 0), "bu": (0, 0, 0), "h": (1, 0, 0), "hu": (1, 0, 0), "w": (0, 1, 0),
+                     "wu": (0, 1, 0), "d": (0, 0, 1)}
+_LOAD_WIDTH_BITS = {"b": (0, 0, 0, 1), "bu": (0, 0, 0, 0), "h": (1, 0, 0, 1), "hu": (1, 0, 0, 0),
+                    "w": (0, 1, 0, 1), "wu": (0, 1, 0, 0), "d": (0, 0, 1, 0)}
+# branch condition -> (alu op, signed, invert) (types.rs:1025-1034)
+_BRANCH_FLAGS = {"eq": (ALU_OP_EQ, 0, 0), "ne": (ALU_OP_EQ, 0, 1), "lt": (ALU_OP_LT, 1, 0), "ltu": (ALU_OP_LT, 0, 0),
+                 "ge": (ALU_OP_LT, 1, 1), "geu": (ALU_OP_LT, 0, 1)}
+
+
+def shrunk_decode(ins: Instruction, instruction_length: int = 4) -> ShrunkDecode:
+    """types.rs:734-860: the decode fields of an instruction (JAL as JALR rd,
+    x255; AUIPC as ADDI rd, x255; LUI as ADDI rd, x0; FENCE and CSR as ADDI x0,
+    x0, 0; ECALL reads a7)."""
+    k, f = ins.kind, dict(half_instruction_length=instruction_length // 2)
+    if k in ("arith", "arith_w", "arith_imm", "arith_imm_w"):
+        word = k in ("arith_w", "arith_imm_w")
+        flags, alu, add, sub = arith_decode(ins.op, word)
+        f.update(rd=ins.dst, rs1=ins.src1, read_register1=ins.src1 != 0, write_register=ins.dst != 0,
+                 word_instr=word, alu=alu, add=add, sub=sub, alu_flags=flags)
+        if k in ("arith", "arith_w"):
+            f.update(rs2=ins.src2, read_register2=ins.src2 != 0)
+    elif k == "jal":
+        f.update(rd=ins.dst, rs1=255, read_register1=True, write_register=ins.dst != 0, branch=True,
+                 mem_flags=build_mem_flags(1, 0, 0, 0, 0))
+    elif k == "jalr":
+        f.update(rd=ins.dst, rs1=ins.src1, read_register1=ins.src1 != 0, write_register=ins.dst != 0, branch=True,
+                 mem_flags=build_mem_flags(1, 0, 0, 0, 0))
+    elif k == "store":
+        f.update(rs1=ins.src1, rs2=ins.src2, read_register1=ins.src1 != 0, read_register2=ins.src2 != 0, add=True,
+                 memory=True, mem_flags=build_mem_flags(1, 0, *_STORE_WIDTH_BITS[ins.width]))
+    elif k == "load":
+        m2, m4, m8, s = _LOAD_WIDTH_BITS[ins.width]
+        f.update(rd=ins.dst, rs1=ins.src1, read_register1=ins.src1 != 0, write_register=ins.dst != 0, add=True,
+                 memory=True, mem_flags=build_mem_flags(0, s, m2, m4, m8))
+    elif k == "branch":
+        op, s, inv = _BRANCH_FLAGS[ins.cond]
+        f.update(rs1=ins.src1, rs2=ins.src2, read_register1=ins.src1 != 0, read_register2=ins.src2 != 0,
+                 branch=True, alu=True, alu_flags=op + 32 * s + 64 * inv)
+    elif k == "lui":
+        f.update(rd=ins.dst, write_register=ins.dst != 0, add=True)
+    elif k == "auipc":
+        f.update(rd=ins.dst, rs1=255, read_register1=True, write_register=ins.dst != 0, add=True)
+    elif k == "ecall":
+        f.update(rs1=17, read_register1=True, ecall=True)
+    else:  # fence, csr
+        f.update(add=True)
+    return ShrunkDecode(**f)
+
+
+def instruction_imm(ins: Instruction) -> int:
+    """types.rs:1096-1110: the immediate sign-extended to 64 bits (0 for R-type,
+    ECALL, FENCE, CSR)."""
+    if ins.kind in ("arith_imm", "arith_imm_w", "jal", "jalr", "store", "load", "branch"):
+        return ins.imm & (2**64 - 1)
+    if ins.kind in ("lui", "auipc"):
+        v = ins.imm & 0xFFFFFFFF
+        return (v - (1 << 32) if v >> 31 else v) & (2**64 - 1)
+    return 0
+
+
+class DecodeCols:
+    """decode.rs:48-78: pc (DWordWL), packed_decode, imm (DWordWL) fixed; mu."""
+    PC_0, PC_1, PACKED_DECODE, IMM_0, IMM_1, MU = range(6)
+    NUM_COLUMNS = 6
+    NUM_PRECOMPUTED = 5
+
+
+def decode_entries_from_words(segments) -> list:
+    """decode.rs:356-377 tables_from_elf: (pc, imm, packed) of every 4-byte
+    word of the executable segments [(base_address, words)], in segment order."""
+    out = []
+    for base, words in segments:
+        for i, w in enumerate(words):
+            ins = decode_instruction(w)
+            out.append((base + 4 * i, instruction_imm(ins), shrunk_decode(ins, 4).pack()))
+    return out
+
+
+def decode_precomputed(entries) -> np.ndarray:
+    """decode.rs:379-429: the entries, then the CPU padding entry (pc =
+    CPU_PADDING_PC, zero decode), then padding entries (pc 1, zero decode) to a
+    power of two rows (at least 2)."""
+    rows = max(2, 1 << len(entries).bit_length())  # next power of two of len + 1
+    t = np.zeros((rows, DecodeCols.NUM_PRECOMPUTED), dtype=np.uint64)
+    t[:, DecodeCols.PC_0] = 1  # padding_entry pc (types.rs:1073-1079)
+    t[len(entries), DecodeCols.PC_0] = CPU_PADDING_PC
+    for r, (pc, imm, packed) in enumerate(entries):
+        t[r] = (pc & 0xFFFFFFFF, pc >> 32, packed, imm & 0xFFFFFFFF, imm >> 32)
+    return t
+
+
+def generate_decode_trace(entries, lookups) -> np.ndarray:
+    """The fixed columns and the multiplicity: one count per looked-up pc
+    (decode.rs:140-155 update_multiplicities; CPU_PADDING_PC counts on the CPU
+    padding row, pcs outside the program are not counted)."""
+    pre = decode_precomputed(entries)
+    t = np.zeros((pre.shape[0], DecodeCols.NUM_COLUMNS), dtype=np.uint64)
+    t[:, :DecodeCols.NUM_PRECOMPUTED] = pre
+    row_of = {pc: r for r, (pc, _, _) in enumerate(entries)}
+    row_of[CPU_PADDING_PC] = len(entries)
+    for pc in lookups:
+        if pc in row_of:
+            t[row_of[pc], DecodeCols.MU] += 1
+    return t
+
+
+def decode_bus_interactions():
+    """decode.rs:157-180: DECODE[pc, imm, packed_decode] received with mu."""
+    c = DecodeCols
+    return [BusInteraction.receiver(BUS_DECODE, Multiplicity("Column", (c.MU,)), [
+        BusValue.packed(Packing.DWORD_WL, c.PC_0), BusValue.packed(Packing.DWORD_WL, c.IMM_0),
+        BusValue.packed(Packing.DIRECT, c.PACKED_DECODE)])]
+
+
+def decode_table_air(entries) -> AirProgram:
+    """DECODE as an AIR: the LogUp constraints of its one receiver; the first
+    five columns fixed (they depend on the program, so no static commitment of
+    the reference pins them)."""
+    c = DecodeCols
+    its = decode_bus_interactions()
+    pre = decode_precomputed(entries)
+    return _bus_only_air("decode", c.NUM_COLUMNS, its, c.NUM_PRECOMPUTED, lambda: pre)

@@ -572,3 +572,33 @@ def test_gpu_cpu32_table_proof(gpu):
     ref = replay_air_tables(tables, 4, 3, 2, 8, 0, b"gpu-cpu32")
     assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
     assert verify_air_tables([cpu, sh], got, 4, 3, 2, 8, 0, b"gpu-cpu32")[0]
+
+
+def test_gpu_decode_table_proof(gpu):
+    """DECODE (five fixed columns of a decoded RV64IM program, one receiver)
+    beside the CPU32 rows of its word instructions: the GPU proofs equal the
+    oracle replay byte for byte, DECODE's precomputed root is the verifier's own
+    commitment and both verify."""
+    from lambda_vm_amd import proof_io, prover
+    from lambda_vm_amd import vm_tables as V
+    from lambda_vm_amd.prover import AirTableProof
+    from oracle.air_proof import precomputed_commitment
+    from tests.test_vm_table_decode import word_program
+
+    print("[gpu-air] DECODE 512 x 6 + CPU32, blowup 4", flush=True)
+    words, ops = word_program(400, 111)
+    entries = V.decode_entries_from_words([(0x2000, words)])
+    dec, cpu = V.decode_table_air(entries), V.cpu32_table_air()
+    for a in (dec, cpu):
+        a.bus_descriptor = A.descriptor(a)
+    tables = [(dec, V.generate_decode_trace(entries, [o.pc for o in ops])), (cpu, V.generate_cpu32_trace(ops))]
+    tables = [(a, np.ascontiguousarray(tr, dtype=np.uint64)) for a, tr in tables]
+    opts = prover.ProofOptions(blowup=4, coset_offset=3, fri_final_log_degree=2, num_queries=8)
+    dev = [(a, gpu.DeviceBuffer.from_array(tr), tr.shape[0]) for a, tr in tables]
+    got = prover.prove_air_tables(dev, opts, label=b"gpu-decode")
+    for _, d, _ in dev:
+        d.free()
+    ref = replay_air_tables(tables, 4, 3, 2, 8, 0, b"gpu-decode")
+    assert got[0].precomputed_root == precomputed_commitment(dec, 4, 3)
+    assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
+    assert verify_air_tables([dec, cpu], got, 4, 3, 2, 8, 0, b"gpu-decode")[0]
