@@ -3080,3 +3080,239 @@ def decode_table_air(entries) -> AirProgram:
     its = decode_bus_interactions()
     pre = decode_precomputed(entries)
     return _bus_only_air("decode", c.NUM_COLUMNS, its, c.NUM_PRECOMPUTED, lambda: pre)
+
+
+# CPU (prover/src/tables/cpu.rs, constraints/cpu.rs): the central execution
+# table, one row per executed instruction. It fetches through DECODE, reads and
+# writes registers through MEMW, dispatches to the ALU chips (ALU bus), to
+# LOAD/STORE (MEMORY op bus), to BRANCH and ECALL, delegates word instructions
+# to CPU32, and carries the pc as inline Memory-bus tokens at address 510/511.
+BUS_ECALL_CPU = BUS_ECALL
+
+
+class CpuCols:
+    """cpu.rs:218-307 (TIMESTAMP fixed)."""
+    TIMESTAMP, PC_0, PC_1, RS1, RS2, RD = 0, 1, 2, 3, 4, 5
+    READ_REGISTER1, READ_REGISTER2, WRITE_REGISTER, IMM_0, IMM_1 = 6, 7, 8, 9, 10
+    HALF_INSTRUCTION_LENGTH, WORD_INSTR, ALU, ALU_FLAGS, ADD, SUB = 11, 12, 13, 14, 15, 16
+    MEMORY, MEM_FLAGS, BRANCH, ECALL, NEXT_PC_0, NEXT_PC_1, RVD_0, RVD_1 = 17, 18, 19, 20, 21, 22, 23, 24
+    PC_DOUBLE_READ, RV1_0, RV1_1, RV2_0, RV2_1, ARG2_0, ARG2_1 = 25, 26, 27, 28, 29, 30, 31
+    RES = (32, 33, 34, 35)
+    BRANCH_COND = 36
+    NUM_COLUMNS = 37
+    NUM_PRECOMPUTED = 1
+
+
+@dataclass(frozen=True)
+class CpuOperation:
+    """cpu.rs:318-350: the decoded instruction (pc, imm, fields) and the row's
+    run-time values."""
+    pc: int
+    imm: int
+    fields: ShrunkDecode
+    timestamp: int = 0
+    next_pc: int = 0
+    rvd: int = 0
+    rv1: int = 0
+    rv2: int = 0
+    arg2: int = 0
+    res: int = 0
+    branch_cond: bool = False
+
+
+def cpu_operation(pc, ins: Instruction, timestamp, src1_val, src2_val, dst_val, next_pc) -> CpuOperation:
+    """cpu.rs:381-500 from_log: the executor's log of one step (register values
+    read, the value written, the next pc) and the decoded instruction. rv1 is
+    the pc for the x255 reads (JAL, AUIPC); arg2 = imm (memory), rv2 (branch)
+    or rv2 + imm; res = rv1 ± arg2 on the ADD/SUB paths, the branch decision
+    on branches, the logged result otherwise; rvd = the loaded value (0 for
+    stores), pc + length on branches, res otherwise; word rows only carry
+    their register values for CPU32."""
+    f = shrunk_decode(ins, 4)
+    imm = instruction_imm(ins)
+    M = 2**64 - 1
+    length = 2 * f.half_instruction_length
+    if f.word_instr:
+        return CpuOperation(pc, imm, f, timestamp, (pc + length) & M, dst_val, src1_val,
+                            src2_val if f.read_register2 else 0)
+    rv1 = pc if f.rs1 == 255 else (src1_val if f.read_register1 else 0)
+    rv2 = src2_val if f.read_register2 else 0
+    jalr = f.mem_flags & 1 == 1
+    arg2 = imm if f.memory else rv2 if f.branch else (rv2 + imm) & M
+    cond = False
+    if f.branch:
+        if jalr:
+            cond = True
+        else:
+            op, signed, invert = f.alu_flags & 0x1F, f.alu_flags >> 5 & 1, f.alu_flags >> 6 & 1
+            s = lambda v: v - 2**64 if v >> 63 else v  # noqa: E731
+            cmp = rv1 == rv2 if op == ALU_OP_EQ else (s(rv1) < s(rv2) if signed else rv1 < rv2) if op == ALU_OP_LT \
+                else False
+            cond = cmp != bool(invert)
+    if f.add:
+        res = (rv1 + arg2) & M
+    elif f.sub:
+        res = (rv1 - arg2) & M
+    elif f.alu:
+        res = int(cond) if f.branch else dst_val
+    else:
+        res = 0
+    if f.memory:
+        rvd = 0 if jalr else dst_val
+    elif f.branch:
+        rvd = (pc + length) & M
+    else:
+        rvd = res
+    nxt = next_pc if (cond and not f.ecall) else (pc + length) & M
+    return CpuOperation(pc, imm, f, timestamp, nxt, rvd, rv1, rv2, arg2, res, cond)
+
+
+def generate_cpu_trace(ops) -> np.ndarray:
+    """cpu.rs:624-700: one row per operation (TIMESTAMP = 4 (row + 1)); a word
+    row keeps only pc, next pc, the half length and WORD_INSTR; padding rows
+    hold pc = next pc = CPU_PADDING_PC; zero rows to a power of two (at least
+    4)."""
+    c = CpuCols
+    n = max(4, 1 << max(0, (len(ops) - 1).bit_length()))
+    t = np.zeros((n, c.NUM_COLUMNS), dtype=np.uint64)
+    t[:, c.TIMESTAMP] = CPU_TIMESTAMP_STRIDE * np.arange(1, n + 1, dtype=np.uint64)
+    t[len(ops):, c.PC_0] = CPU_PADDING_PC
+    t[len(ops):, c.NEXT_PC_0] = CPU_PADDING_PC
+    w = lambda r, col, v: (r.__setitem__(col, v & 0xFFFFFFFF), r.__setitem__(col + 1, (v >> 32) & 0xFFFFFFFF))  # noqa
+    for i, op in enumerate(ops):
+        f, r = op.fields, t[i]
+        word = f.word_instr
+        eff = lambda flag: int(flag and not word)  # noqa: E731
+        w(r, c.PC_0, op.pc)
+        if not word:
+            r[c.RS1], r[c.RS2], r[c.RD] = f.rs1, f.rs2, f.rd
+            w(r, c.IMM_0, op.imm)
+            w(r, c.RVD_0, op.rvd)
+            w(r, c.RV1_0, op.rv1)
+            w(r, c.RV2_0, op.rv2)
+            w(r, c.ARG2_0, op.arg2)
+            r[c.RES[0]:c.RES[3] + 1] = [(op.res >> (16 * k)) & 0xFFFF for k in range(4)]
+            r[c.ALU_FLAGS], r[c.MEM_FLAGS] = f.alu_flags, f.mem_flags
+        r[c.READ_REGISTER1] = eff(f.read_register1 and f.rs1 != 0)
+        r[c.READ_REGISTER2] = eff(f.read_register2 and f.rs2 != 0)
+        r[c.WRITE_REGISTER] = eff(f.write_register and f.rd != 0)
+        r[c.HALF_INSTRUCTION_LENGTH], r[c.WORD_INSTR] = f.half_instruction_length, int(word)
+        r[c.ALU], r[c.ADD], r[c.SUB] = eff(f.alu), eff(f.add), eff(f.sub)
+        r[c.MEMORY], r[c.BRANCH], r[c.ECALL] = eff(f.memory), eff(f.branch), eff(f.ecall)
+        w(r, c.NEXT_PC_0, op.next_pc)
+        r[c.BRANCH_COND] = int(op.branch_cond)
+        r[c.PC_DOUBLE_READ] = int(not word and f.read_register1 and f.rs1 == 255)
+    return t
+
+
+def cpu_bus_interactions(ts_base: int = 0):
+    """cpu.rs:832-1160, in its order (20): DECODE[pc, imm, packed decode] (1 -
+    word_instr); ALU[rv1, arg2, alu_flags, res] (ALU); CPU32[ts, 0, pc,
+    half length] (word_instr); MEMW reads of rs1 at ts and rs2 at ts + 1 and
+    the write of rd at ts + 2 (24/24/16 elements, their flags); the MEMORY
+    op [ts, 0, res, rv2, mem_flags, rvd] (MEMORY); per pc word the Memory
+    token [1, 510 + i, 0, ts - 3 + 3 pc_double_read, 0, pc_i] sent and [1, 510 +
+    i, 0, ts + 1, 0, next_pc_i] received (multiplicity one); BRANCH[next pc,
+    pc, imm, rv1, mem_flags] (branch_cond); ARE_BYTES of (rs1, rs2), (rd, half
+    length), (alu_flags, mem_flags) and IS_HALFWORD of the res halves (one);
+    ECALL[ts, 0, rv1] (ECALL). Timestamps are the fixed column plus ts_base."""
+    c = CpuCols
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    wl = lambda col: BusValue.packed(Packing.DWORD_WL, col)  # noqa: E731
+    k = lambda v: BusValue.of_linear([LinearTerm(v)])  # noqa: E731
+    col = lambda cc: Multiplicity("Column", (cc,))  # noqa: E731
+    ts = lambda off: BusValue.of_linear([LinearTerm(1, c.TIMESTAMP), LinearTerm(ts_base + off)])  # noqa: E731
+    res_wl = BusValue.packed(Packing.DWORD_HL, c.RES[0])
+    packed = [LinearTerm(1 << pos, cc) for pos, cc in (
+        (PD_READ_REG1, c.READ_REGISTER1), (PD_READ_REG2, c.READ_REGISTER2), (PD_WRITE_REG, c.WRITE_REGISTER),
+        (PD_WORD_INSTR, c.WORD_INSTR), (PD_ALU, c.ALU), (PD_ADD, c.ADD), (PD_SUB, c.SUB), (PD_MEMORY, c.MEMORY),
+        (PD_BRANCH, c.BRANCH), (PD_ECALL, c.ECALL), (PD_RS1, c.RS1), (PD_RS2, c.RS2), (PD_RD, c.RD),
+        (PD_HALF_INSTRUCTION_LENGTH, c.HALF_INSTRUCTION_LENGTH), (PD_ALU_FLAGS, c.ALU_FLAGS),
+        (PD_MEM_FLAGS, c.MEM_FLAGS))]
+    zeros6 = [k(0)] * 6
+    its = [BusInteraction.sender(BUS_DECODE, Multiplicity("Negated", (c.WORD_INSTR,)),
+                                 [wl(c.PC_0), wl(c.IMM_0), BusValue.of_linear(packed)]),
+           BusInteraction.sender(BUS_ALU, col(c.ALU), [wl(c.RV1_0), wl(c.ARG2_0), d(c.ALU_FLAGS), res_wl]),
+           BusInteraction.sender(BUS_CPU32, col(c.WORD_INSTR), [ts(0), k(0), wl(c.PC_0), d(c.HALF_INSTRUCTION_LENGTH)])]
+    for flag, rs, lo, off in ((c.READ_REGISTER1, c.RS1, c.RV1_0, 0), (c.READ_REGISTER2, c.RS2, c.RV2_0, 1)):
+        val = [d(lo), d(lo + 1)] + zeros6
+        its.append(BusInteraction.sender(BUS_MEMW, col(flag), val + [k(1), BusValue.of_linear([LinearTerm(2, rs)]),
+                                                                      k(0)] + val + [ts(off), k(0), k(1), k(0), k(0)]))
+    its.append(BusInteraction.sender(BUS_MEMW, col(c.WRITE_REGISTER), [
+        k(1), BusValue.of_linear([LinearTerm(2, c.RD)]), k(0), d(c.RVD_0), d(c.RVD_1)] + zeros6
+        + [ts(2), k(0), k(1), k(0), k(0)]))
+    its.append(BusInteraction.sender(BUS_MEMORY_OP, col(c.MEMORY), [ts(0), k(0), res_wl, wl(c.RV2_0), d(c.MEM_FLAGS),
+                                                                    wl(c.RVD_0)]))
+    prev = BusValue.of_linear([LinearTerm(1, c.TIMESTAMP), LinearTerm(ts_base - 3), LinearTerm(3, c.PC_DOUBLE_READ)])
+    for i in range(2):
+        its.append(BusInteraction.sender(BUS_MEMORY, ONE, [k(1), k(510 + i), k(0), prev, k(0), d(c.PC_0 + i)]))
+        its.append(BusInteraction.receiver(BUS_MEMORY, ONE, [k(1), k(510 + i), k(0), ts(1), k(0), d(c.NEXT_PC_0 + i)]))
+    its.append(BusInteraction.sender(BUS_BRANCH, col(c.BRANCH_COND), [d(cc) for cc in (
+        c.NEXT_PC_0, c.NEXT_PC_1, c.PC_0, c.PC_1, c.IMM_0, c.IMM_1, c.RV1_0, c.RV1_1, c.MEM_FLAGS)]))
+    its += [BusInteraction.sender(BUS_ARE_BYTES, ONE, [d(a), d(b)]) for a, b in (
+        (c.RS1, c.RS2), (c.RD, c.HALF_INSTRUCTION_LENGTH), (c.ALU_FLAGS, c.MEM_FLAGS))]
+    its += [BusInteraction.sender(BUS_IS_HALFWORD, ONE, [d(cc)]) for cc in c.RES]
+    its.append(BusInteraction.sender(BUS_ECALL, col(c.ECALL), [ts(0), k(0), wl(c.RV1_0)]))
+    return its
+
+
+def cpu_table_air(height: int, ts_base: int = 0) -> AirProgram:
+    """CPU as an AIR (constraints/cpu.rs:278-379, 38 constraints): 0-10 the
+    flags and pc_double_read are bits; 11-12 the ADD carry pair of rv1 + arg2 =
+    res (gated by ADD), 13-14 of arg2 + res = rv1 (gated by SUB); 15-20 a word
+    row sets no MEMORY, BRANCH, ECALL, register write or reads; 21-22 arg2 is
+    imm (MEMORY), rv2 (BRANCH) or rv2 + imm; 23-26 unread registers are zero;
+    27-28 rvd = res off MEMORY and BRANCH; 29-30 BRANCH: the carry pair of
+    rvd = pc + 2 half length; 31 branch_cond = BRANCH (JALR + (1 - JALR)
+    res[0]); 32-33 without a taken branch the carry pair of next pc = pc +
+    length; 34 MEMORY·BRANCH = 0; 35-36 no immediate beside rs2 on ALU rows; 37
+    mem_flags a bit off MEMORY; then the LogUp constraints of its 20
+    interactions. The TIMESTAMP column is fixed."""
+    c = CpuCols
+    b = ir.IrBuilder()
+    m = lambda col: b.main(0, col)  # noqa: E731
+    one, inv, s16 = b.const_base(1), b.const_base(INV_SHIFT_32), b.const_base(1 << 16)
+    flags = (c.READ_REGISTER1, c.READ_REGISTER2, c.WRITE_REGISTER, c.WORD_INSTR, c.ALU, c.ADD, c.SUB, c.MEMORY,
+             c.BRANCH, c.ECALL, c.PC_DOUBLE_READ)
+    for i, col in enumerate(flags):
+        _emit_is_bit(b, i, col)
+    res = (m(c.RES[0]) + m(c.RES[1]) * s16, m(c.RES[2]) + m(c.RES[3]) * s16)
+    rv1, arg2 = (m(c.RV1_0), m(c.RV1_1)), (m(c.ARG2_0), m(c.ARG2_1))
+
+    def carry_pair(idx, gate, lhs, rhs, out):
+        c0 = (lhs[0] + rhs[0] - out[0]) * inv
+        c1 = (lhs[1] + rhs[1] + c0 - out[1]) * inv
+        b.emit(idx, gate * c0 * (one - c0))
+        b.emit(idx + 1, gate * c1 * (one - c1))
+
+    carry_pair(11, m(c.ADD), rv1, arg2, res)
+    carry_pair(13, m(c.SUB), arg2, res, rv1)
+    for i, col in enumerate((c.MEMORY, c.BRANCH, c.ECALL, c.WRITE_REGISTER, c.READ_REGISTER1, c.READ_REGISTER2)):
+        b.emit(15 + i, m(c.WORD_INSTR) * m(col))
+    mem, br = m(c.MEMORY), m(c.BRANCH)
+    for i, (a2, imm, rv2) in enumerate(((c.ARG2_0, c.IMM_0, c.RV2_0), (c.ARG2_1, c.IMM_1, c.RV2_1))):
+        b.emit(21 + i, m(a2) - (mem * m(imm) + br * m(rv2) + (one - mem - br) * (m(rv2) + m(imm))))
+    for i, (flag, v) in enumerate(((c.READ_REGISTER1, c.RV1_0), (c.READ_REGISTER1, c.RV1_1),
+                                   (c.READ_REGISTER2, c.RV2_0), (c.READ_REGISTER2, c.RV2_1))):
+        b.emit(23 + i, (one - m(flag)) * m(v))
+    for i in range(2):
+        b.emit(27 + i, (one - mem - br) * (m(c.RVD_0 + i) - res[i]))
+    half = m(c.HALF_INSTRUCTION_LENGTH)
+    pc_len = (m(c.PC_0) + half + half, m(c.PC_1))
+    zero = b.const_base(0)
+    carry_pair(29, br, pc_len, (zero, zero), (m(c.RVD_0), m(c.RVD_1)))
+    jalr = m(c.MEM_FLAGS)
+    b.emit(31, m(c.BRANCH_COND) - (br * jalr + br * (one - jalr) * m(c.RES[0])))
+    carry_pair(32, one - m(c.BRANCH_COND), pc_len, (zero, zero), (m(c.NEXT_PC_0), m(c.NEXT_PC_1)))
+    b.emit(34, mem * br)
+    for i, imm in enumerate((c.IMM_0, c.IMM_1)):
+        b.emit(35 + i, (one - mem - br) * m(c.READ_REGISTER2) * m(imm))
+    b.emit(37, (one - mem) * jalr * (one - jalr))
+    its = cpu_bus_interactions(ts_base)
+    n_total = emit_logup_constraints(b, its, 38)
+    prog = b.finish(c.NUM_COLUMNS)
+    pairs, absorbed = split_interactions(len(its))
+    return AirProgram("cpu", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
+                      boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
+                      max_degree=max(3, 1 + absorbed), num_precomputed=c.NUM_PRECOMPUTED,
+                      precomputed=lambda: cpu_timestamp_precomputed(height))

@@ -602,3 +602,38 @@ def test_gpu_decode_table_proof(gpu):
     assert got[0].precomputed_root == precomputed_commitment(dec, 4, 3)
     assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
     assert verify_air_tables([dec, cpu], got, 4, 3, 2, 8, 0, b"gpu-decode")[0]
+
+
+def test_gpu_cpu_table_proof(gpu):
+    """CPU (its TIMESTAMP column fixed, 38 constraints, 20 interactions) over
+    an executed RV64IM program, beside the program's DECODE table and the
+    CPU32 rows of its word instructions: the GPU proofs equal the oracle
+    replay byte for byte, the fixed-column roots are the verifier's own and
+    all verify."""
+    from lambda_vm_amd import proof_io, prover
+    from lambda_vm_amd import vm_tables as V
+    from lambda_vm_amd.prover import AirTableProof
+    from oracle.air_proof import precomputed_commitment
+    from tests.test_vm_table_cpu import ENTRY, cpu32_rows, cpu_ops, program, run
+
+    print("[gpu-air] CPU 512 x 37 + DECODE + CPU32, blowup 4", flush=True)
+    _, logs = run(121, steps=512)
+    ops = cpu_ops(logs)
+    entries = V.decode_entries_from_words([(ENTRY, program(121, length=1024))])
+    cpu_tr = V.generate_cpu_trace(ops)
+    cpu, dec, c32 = V.cpu_table_air(cpu_tr.shape[0]), V.decode_table_air(entries), V.cpu32_table_air()
+    for a in (cpu, dec, c32):
+        a.bus_descriptor = A.descriptor(a)
+    tables = [(cpu, cpu_tr), (dec, V.generate_decode_trace(entries, [o.pc for o in ops])),
+              (c32, V.generate_cpu32_trace(cpu32_rows(ops)))]
+    tables = [(a, np.ascontiguousarray(tr, dtype=np.uint64)) for a, tr in tables]
+    opts = prover.ProofOptions(blowup=4, coset_offset=3, fri_final_log_degree=2, num_queries=8)
+    dev = [(a, gpu.DeviceBuffer.from_array(tr), tr.shape[0]) for a, tr in tables]
+    got = prover.prove_air_tables(dev, opts, label=b"gpu-cpu")
+    for _, d, _ in dev:
+        d.free()
+    ref = replay_air_tables(tables, 4, 3, 2, 8, 0, b"gpu-cpu")
+    assert got[0].precomputed_root == precomputed_commitment(cpu, 4, 3)
+    assert got[1].precomputed_root == precomputed_commitment(dec, 4, 3)
+    assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
+    assert verify_air_tables([cpu, dec, c32], got, 4, 3, 2, 8, 0, b"gpu-cpu")[0]
