@@ -3316,3 +3316,280 @@ def cpu_table_air(height: int, ts_base: int = 0) -> AirProgram:
                       boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
                       max_degree=max(3, 1 + absorbed), num_precomputed=c.NUM_PRECOMPUTED,
                       precomputed=lambda: cpu_timestamp_precomputed(height))
+
+
+# ECDAS (prover/src/tables/ecdas.rs) with the secp256k1 double-and-add witness
+# of crypto/ecsm (curve.rs, witness.rs): one row per double (op 0) or add (op
+# 1) step of k·G. λ, xR and yR are pinned by three byte-limb convolution
+# relations with 33-byte quotients, the offset r = 3p and 64 signed carries
+# each (range-checked through IS_HALFWORD with per-relation offsets); the
+# accumulator travels on the self-referential Ecdas bus, the scalar bits of
+# the adds on the Bit bus.
+BUS_ECDAS, BUS_BIT = 28, 30     # types.rs:350, :354
+SECP_P = 2**256 - 2**32 - 977   # crypto/ecsm/src/lib.rs P_BYTES
+SECP_N = 0xFFFFFFFFFFFFFFFFFFFFFFFFFFFFFFFEBAAEDCE6AF48A03BBFD25E8CD0364141  # N_BYTES
+SECP_B = 7
+SECP_R = 3 * SECP_P             # R_BYTES (33 bytes)
+ECDAS_CARRY_OFFSETS = (32636, 8161, 16320)  # λ, xR, yR (ecdas.rs:24-26)
+
+
+def secp_recover_y(x: int):
+    """The even square root of x^3 + 7 mod p (curve.rs:26-33), None for a
+    non-residue; p = 3 mod 4, so the root is a power."""
+    rhs = (pow(x, 3, SECP_P) + SECP_B) % SECP_P
+    y = pow(rhs, (SECP_P + 1) // 4, SECP_P)
+    if y * y % SECP_P != rhs:
+        return None
+    return y if y % 2 == 0 else SECP_P - y
+
+
+def ecsm_schedule(k: int):
+    """curve.rs:105-125: (round, op, next_op) per step; from bit msb - 1 down,
+    a double at every round and an add at the rounds whose bit is set."""
+    sched, rnd, op = [], k.bit_length() - 2, 0
+    while rnd >= 0:
+        nxt = ((k >> rnd) & 1) if op == 0 else 0
+        sched.append((rnd, op, nxt))
+        sent = rnd - (1 - nxt)
+        if sent < 0:
+            break
+        rnd, op = sent, nxt
+    return sched
+
+
+def ecsm_steps(k: int, gx: int, gy: int):
+    """curve.rs:223-300: the affine steps (A, round, op, next_op, R, λ) of k·G
+    (λ = 3xA²/2yA for a double, (yG - yA)/(xG - xA) for an add)."""
+    out, ax, ay = [], gx, gy
+    for rnd, op, nxt in ecsm_schedule(k):
+        if op == 0:
+            lam = 3 * ax * ax * pow(2 * ay, -1, SECP_P) % SECP_P
+            rx = (lam * lam - 2 * ax) % SECP_P
+        else:
+            lam = (gy - ay) * pow(gx - ax, -1, SECP_P) % SECP_P
+            rx = (lam * lam - ax - gx) % SECP_P
+        ry = (lam * (ax - rx) - ay) % SECP_P
+        out.append(((ax, ay), rnd, op, nxt, (rx, ry), lam))
+        ax, ay = rx, ry
+    return out, (ax, ay)
+
+
+def _le(v: int, n: int) -> list:
+    return [(v >> (8 * i)) & 0xFF for i in range(n)]
+
+
+def _limb_carries(terms) -> list:
+    """witness.rs:88-106: 2^8 c_i = c_{i-1} + terms_i, exact at every limb,
+    closing carry 0."""
+    c, carry = [], 0
+    for t in terms:
+        s = carry + t
+        if s & 0xFF:
+            raise ValueError("convolution limb not divisible by 256")
+        carry = s >> 8
+        c.append(carry)
+    if c[63] != 0:
+        raise ValueError("closing carry must be 0")
+    return c
+
+
+def _conv(a, b, i):
+    return sum(a[j] * b[i - j] for j in range(i + 1))
+
+
+@dataclass(frozen=True)
+class EcdasStep:
+    """witness.rs:60-85 and build_step (:381-470): one step's coordinates and λ
+    as 32 little-endian bytes, quotients as 33 bytes, 64 carries per
+    relation."""
+    xa: int
+    ya: int
+    xg: int
+    yg: int
+    round: int
+    op: int
+    next_op: int
+    lam: int
+    xr: int
+    yr: int
+
+    def witness(self):
+        p = SECP_P
+        L, XA, YA, XG, YG, XR, YR = (_le(v, 32) + [0] * 32 for v in (self.lam, self.xa, self.ya, self.xg, self.yg,
+                                                                     self.xr, self.yr))
+        PP, RR = _le(p, 32) + [0] * 32, _le(SECP_R, 33) + [0] * 31
+        num0 = (self.xg - self.xa) * self.lam - self.yg + self.ya if self.op \
+            else 2 * self.lam * self.ya - 3 * self.xa * self.xa
+        num1 = self.lam * self.lam - self.xa - self.xg - self.xr + (0 if self.op else self.xg - self.xa)
+        num2 = self.lam * (self.xa - self.xr) - self.ya - self.yr
+        qs = []
+        for num in (num0, num1, num2):
+            if num % p:
+                raise ValueError("relation numerator not divisible by p")
+            q = SECP_R + num // p
+            if q < 0 or q >> 264:
+                raise ValueError("quotient out of range")
+            qs.append(q)
+        Q = [_le(q, 33) + [0] * 31 for q in qs]
+        rp = [_conv(RR, PP, i) for i in range(64)]
+        t0 = [(YA[i] - YG[i] + sum(L[j] * (XG[i - j] - XA[i - j]) for j in range(i + 1))) if self.op else
+              sum(2 * L[j] * YA[i - j] - 3 * XA[j] * XA[i - j] for j in range(i + 1)) for i in range(64)]
+        t1 = [_conv(L, L, i) - XA[i] - XG[i] - XR[i] - (0 if self.op else XA[i] - XG[i]) for i in range(64)]
+        t2 = [sum(L[j] * (XA[i - j] - XR[i - j]) for j in range(i + 1)) - YA[i] - YR[i] for i in range(64)]
+        cs = [_limb_carries([t[i] + rp[i] - _conv(q, PP, i) for i in range(64)]) for t, q in zip((t0, t1, t2), Q)]
+        return [_le(q, 33) for q in qs], cs
+
+
+def ecdas_steps(k: int, gx: int):
+    """The ECDAS steps of k·G for the base point with x-coordinate gx (its
+    even y), k in [1, N) (witness.rs:281-330)."""
+    if not 0 < k < SECP_N:
+        raise ValueError("scalar out of range")
+    if not 0 <= gx < SECP_P:
+        raise ValueError("coordinate out of range")
+    gy = secp_recover_y(gx)
+    if gy is None:
+        raise ValueError("not on the curve")
+    steps, res = ecsm_steps(k, gx, gy)
+    return [EcdasStep(a[0], a[1], gx, gy, rnd, op, nxt, lam, r[0], r[1]) for a, rnd, op, nxt, r, lam in steps], res
+
+
+class EcdasCols:
+    """ecdas.rs:30-58."""
+    TIMESTAMP_0, TIMESTAMP_1 = 0, 1
+    XG, YG, XA, YA = 2, 34, 66, 98
+    ROUND, OP = 130, 131
+    XR, YR, LAMBDA = 132, 164, 196
+    Q0, C0, Q1, C1, Q2, C2 = 228, 261, 325, 358, 422, 455
+    NEXT_OP, MU = 519, 520
+    NUM_COLUMNS = 521
+
+
+def generate_ecdas_trace(ops) -> np.ndarray:
+    """ecdas.rs:80-130: ops are (timestamp, EcdasStep); one row each (mu = 1),
+    carries as field elements; padding rows carry only op = 1; zero rows to a
+    power of two (at least 4)."""
+    c = EcdasCols
+    n = max(4, 1 << max(0, (len(ops) - 1).bit_length()))
+    t = np.zeros((n, c.NUM_COLUMNS), dtype=np.uint64)
+    for i, (ts, s) in enumerate(ops):
+        qs, cs = s.witness()
+        r = t[i]
+        r[c.TIMESTAMP_0], r[c.TIMESTAMP_1] = ts & 0xFFFFFFFF, ts >> 32
+        for base, v in ((c.XG, s.xg), (c.YG, s.yg), (c.XA, s.xa), (c.YA, s.ya), (c.XR, s.xr), (c.YR, s.yr),
+                        (c.LAMBDA, s.lam)):
+            r[base:base + 32] = _le(v, 32)
+        r[c.ROUND], r[c.OP], r[c.NEXT_OP], r[c.MU] = s.round, s.op, s.next_op, 1
+        for qb, cb, q, cc in ((c.Q0, c.C0, qs[0], cs[0]), (c.Q1, c.C1, qs[1], cs[1]), (c.Q2, c.C2, qs[2], cs[2])):
+            r[qb:qb + 33] = q
+            r[cb:cb + 64] = [v % P for v in cc]
+    t[len(ops):, c.OP] = 1
+    return t
+
+
+def ecdas_tuple(acc_x, acc_y, gen_x, gen_y, rnd, op, ts_lo, ts_hi):
+    """ecsm.rs:585-606: [0 (the curve id), ts_lo, ts_hi, the 4 × 32 coordinate
+    bytes, round, op]."""
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    v = [BusValue.of_linear([LinearTerm(0)]), ts_lo, ts_hi]
+    for base in (acc_x, acc_y, gen_x, gen_y):
+        v += [d(base + i) for i in range(32)]
+    return v + [rnd, op]
+
+
+def ecdas_bus_interactions():
+    """ecdas.rs:152-245, in its order (388): the Ecdas receiver of the
+    accumulator [A, G, round, op] (mu); ARE_BYTES[x, 0] of round and of the λ,
+    q0, xR, q1, yR, q2 bytes (mu); IS_HALFWORD[c_i + offset] of carries 0..62
+    of each relation (mu); Bit[ts, round] (next_op); the Ecdas sender of [R,
+    G, round + next_op - 1, next_op] (mu)."""
+    c = EcdasCols
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    mu = Multiplicity("Column", (c.MU,))
+    ts = (d(c.TIMESTAMP_0), d(c.TIMESTAMP_1))
+    its = [BusInteraction.receiver(BUS_ECDAS, mu, ecdas_tuple(c.XA, c.YA, c.XG, c.YG, d(c.ROUND), d(c.OP), *ts))]
+    for base, n in ((c.ROUND, 1), (c.LAMBDA, 32), (c.Q0, 33), (c.XR, 32), (c.Q1, 33), (c.YR, 32), (c.Q2, 33)):
+        its += [BusInteraction.sender(BUS_ARE_BYTES, mu, [d(base + i), BusValue.of_linear([LinearTerm(0)])])
+                for i in range(n)]
+    for base, off in zip((c.C0, c.C1, c.C2), ECDAS_CARRY_OFFSETS):
+        its += [BusInteraction.sender(BUS_IS_HALFWORD, mu, [BusValue.of_linear([LinearTerm(1, base + i),
+                                                                                LinearTerm(off)])])
+                for i in range(63)]
+    its.append(BusInteraction.sender(BUS_BIT, Multiplicity("Column", (c.NEXT_OP,)), list(ts) + [d(c.ROUND)]))
+    its.append(BusInteraction.sender(BUS_ECDAS, mu, ecdas_tuple(
+        c.XR, c.YR, c.XG, c.YG, BusValue.of_linear([LinearTerm(1, c.ROUND), LinearTerm(1, c.NEXT_OP), LinearTerm(-1)]),
+        d(c.NEXT_OP), *ts)))
+    return its
+
+
+def ecdas_table_air() -> AirProgram:
+    """ECDAS as an AIR (ecdas.rs:270-453, 200 constraints, degree 3): 0-2 mu,
+    op, next_op bits; 3 op·next_op = 0; 4 next_op implies mu; then per
+    relation (λ, xR, yR) 64 carry identities 256 c_i - c_{i-1} - S_i and c_63 =
+    0, where S_i is limb i of the relation (op-selected for λ and xR) plus
+    mu·(r·p)_i - (q·p)_i; then the LogUp constraints of its 388
+    interactions."""
+    c = EcdasCols
+    b = ir.IrBuilder()
+    m = lambda col: b.main(0, col)  # noqa: E731
+    one, zero = b.const_base(1), b.const_base(0)
+    pb, rb = _le(SECP_P, 32), _le(SECP_R, 33)
+    at = lambda base, n, j: m(base + j) if j < n else zero  # noqa: E731
+    for i, col in enumerate((c.MU, c.OP, c.NEXT_OP)):
+        _emit_is_bit(b, i, col)
+    b.emit(3, m(c.OP) * m(c.NEXT_OP))
+    b.emit(4, m(c.NEXT_OP) * (one - m(c.MU)))
+
+    def rq(i, qbase):
+        rp = sum(rb[j] * pb[i - j] for j in range(i + 1) if j < 33 and i - j < 32)
+        qp = None
+        for j in range(i + 1):
+            if j < 33 and i - j < 32 and pb[i - j]:
+                t = m(qbase + j) * b.const_base(pb[i - j])
+                qp = t if qp is None else qp + t
+        out = m(c.MU) * b.const_base(rp)
+        return out - qp if qp is not None else out
+
+    lam = lambda j: at(c.LAMBDA, 32, j)  # noqa: E731
+    xg, xa, ya = (lambda j: at(c.XG, 32, j)), (lambda j: at(c.XA, 32, j)), (lambda j: at(c.YA, 32, j))
+    yg, xr, yr = (lambda j: at(c.YG, 32, j)), (lambda j: at(c.XR, 32, j)), (lambda j: at(c.YR, 32, j))
+    op = m(c.OP)
+
+    def s_i(rel, i):
+        if rel == 0:
+            ob = ya(i) - yg(i)
+            nb = None
+            for j in range(i + 1):
+                if j < 32 and i - j < 32:
+                    ob = ob + lam(j) * (xg(i - j) - xa(i - j))
+                    t = b.const_base(2) * lam(j) * ya(i - j) - b.const_base(3) * xa(j) * xa(i - j)
+                    nb = t if nb is None else nb + t
+            nb = nb if nb is not None else zero
+            return op * ob + (one - op) * nb + rq(i, c.Q0)
+        if rel == 1:
+            s = zero
+            for j in range(i + 1):
+                if j < 32 and i - j < 32:
+                    s = s + lam(j) * lam(i - j)
+            return s - xa(i) - xg(i) - xr(i) - (one - op) * (xa(i) - xg(i)) + rq(i, c.Q1)
+        s = zero
+        for j in range(i + 1):
+            if j < 32 and i - j < 32:
+                s = s + lam(j) * (xa(i - j) - xr(i - j))
+        return s - ya(i) - yr(i) + rq(i, c.Q2)
+
+    idx = 5
+    for rel, cb in enumerate((c.C0, c.C1, c.C2)):
+        for i in range(64):
+            prev = m(cb + i - 1) if i else zero
+            b.emit(idx, b.const_base(256) * m(cb + i) - prev - s_i(rel, i))
+            idx += 1
+        b.emit(idx, m(cb + 63))
+        idx += 1
+    its = ecdas_bus_interactions()
+    n_total = emit_logup_constraints(b, its, idx)
+    prog = b.finish(c.NUM_COLUMNS)
+    pairs, absorbed = split_interactions(len(its))
+    return AirProgram("ecdas", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
+                      boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
+                      max_degree=max(3, 1 + absorbed))

@@ -637,3 +637,28 @@ def test_gpu_cpu_table_proof(gpu):
     assert got[1].precomputed_root == precomputed_commitment(dec, 4, 3)
     assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
     assert verify_air_tables([cpu, dec, c32], got, 4, 3, 2, 8, 0, b"gpu-cpu")[0]
+
+
+def test_gpu_ecdas_table_proof(gpu):
+    """ECDAS (521 columns, 200 degree-3 convolution-carry constraints, 388
+    interactions) over the double-and-add steps of two secp256k1 scalar
+    multiplications: the GPU proof equals the oracle replay byte for byte and
+    verifies."""
+    from lambda_vm_amd import proof_io, prover
+    from lambda_vm_amd import vm_tables as V
+    from lambda_vm_amd.prover import AirTableProof
+    from tests.test_vm_table_ecdas import ecdas_ops
+
+    print("[gpu-air] ECDAS 1024 x 521, blowup 4", flush=True)
+    air = V.ecdas_table_air()
+    air.bus_descriptor = A.descriptor(air)
+    k1, k2 = V.SECP_N - 12345, 0x7FFF_FFFF_FFFF_FFFF_0123_4567_89AB_CDEF_0011_2233_4455_6677_8899_AABB_CCDD_EEFF
+    tables = [(air, np.ascontiguousarray(V.generate_ecdas_trace(ecdas_ops([k1, k2])), dtype=np.uint64))]
+    opts = prover.ProofOptions(blowup=4, coset_offset=3, fri_final_log_degree=2, num_queries=8)
+    dev = [(a, gpu.DeviceBuffer.from_array(tr), tr.shape[0]) for a, tr in tables]
+    got = prover.prove_air_tables(dev, opts, label=b"gpu-ecdas")
+    for _, d, _ in dev:
+        d.free()
+    ref = replay_air_tables(tables, 4, 3, 2, 8, 0, b"gpu-ecdas")
+    assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
+    assert verify_air_tables([air], got, 4, 3, 2, 8, 0, b"gpu-ecdas")[0]
