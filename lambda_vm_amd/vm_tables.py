@@ -3593,3 +3593,228 @@ def ecdas_table_air() -> AirProgram:
     return AirProgram("ecdas", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
                       boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
                       max_degree=max(3, 1 + absorbed))
+
+
+# ECSM (prover/src/tables/ecsm.rs): one row per ECALL(-11). Reads xG and k
+# from memory through the x11/x12 pointers, witnesses yG and proves yG^2 =
+# xG^3 + 7 through two convolution relations (x2 = xG^2 mod p with quotient q0,
+# then yG^2 + p^2 - xG·x2 - 7 - q1·p = 0), bounds 0 < k < N, xG < p and xR < p
+# by overflowing additions, serves the scalar's bits on the Bit bus, seeds and
+# drains the ECDAS chain and writes xR back through the x10 pointer.
+ECSM_SYSCALL_NUMBER = 2**64 - 1 - 10    # executor/src/vm/instruction/execution.rs:61
+ECSM_CARRY_OFFSETS = (8160, 16319)      # x2, yG (ecsm.rs:27-28)
+
+
+class EcsmCols:
+    """ecsm.rs:33-98."""
+    TIMESTAMP_0, TIMESTAMP_1 = 0, 1
+    ADDR_XG_0, ADDR_XG_1, ADDR_K_0, ADDR_K_1, ADDR_XR_0, ADDR_XR_1 = 2, 3, 4, 5, 6, 7
+    XR, YR, K, LEN_K = 8, 40, 72, 328
+    XG, YG, X2, Q0, C0, Q1, C1 = 329, 361, 393, 425, 457, 521, 554
+    XG_SUB_P, K_SUB_N, XR_SUB_P = 618, 634, 650
+    MU = 666
+    NUM_COLUMNS = 667
+
+
+@dataclass(frozen=True)
+class EcsmOperation:
+    """ecsm.rs:119-130: the ecall's timestamp and the three memory addresses
+    (xG, k, xR); the scalar and xG as integers."""
+    timestamp: int
+    addr_xg: int
+    addr_k: int
+    addr_xr: int
+    k: int
+    xg: int
+
+    def witness(self):
+        """witness.rs:281-345: yG, x2 = xG^2 mod p with q0, q1 = p + (yG^2 - xG·x2
+        - 7)/p (33 bytes), their carries, the three (v - bound) mod 2^256
+        addends, the top bit of k and the result R = k·G."""
+        p = SECP_P
+        steps, (rx, ry) = ecdas_steps(self.k, self.xg)
+        yg = secp_recover_y(self.xg)
+        x2 = self.xg * self.xg % p
+        q0 = (self.xg * self.xg - x2) // p
+        num = yg * yg - self.xg * x2 - SECP_B
+        if num % p:
+            raise ValueError("yG relation not divisible by p")
+        q1 = p + num // p
+        XG, YG, X2, Q0 = (_le(v, 32) + [0] * 32 for v in (self.xg, yg, x2, q0))
+        Q1 = _le(q1, 33) + [0] * 31
+        PP, BB = _le(p, 32) + [0] * 32, [SECP_B] + [0] * 63
+        c0 = _limb_carries([_conv(XG, XG, i) - X2[i] - _conv(Q0, PP, i) for i in range(64)])
+        c1 = _limb_carries([_conv(YG, YG, i) + _conv(PP, PP, i) - _conv(X2, XG, i) - _conv(Q1, PP, i) - BB[i]
+                            for i in range(64)])
+        sub = [(1 << 256) + v - bound for v, bound in ((self.xg, p), (self.k, SECP_N), (rx, p))]
+        return dict(yg=yg, x2=x2, q0=q0, q1=q1, c0=c0, c1=c1, sub=sub, len_k=self.k.bit_length() - 1, xr=rx, yr=ry,
+                    steps=steps)
+
+
+def generate_ecsm_trace(ops) -> np.ndarray:
+    """ecsm.rs:149-195: one row per ecall (mu = 1), all-zero padding rows to a
+    power of two (at least 4)."""
+    c = EcsmCols
+    n = max(4, 1 << max(0, (len(ops) - 1).bit_length()))
+    t = np.zeros((n, c.NUM_COLUMNS), dtype=np.uint64)
+    for i, op in enumerate(ops):
+        w = op.witness()
+        r = t[i]
+        for col, v in ((c.TIMESTAMP_0, op.timestamp), (c.ADDR_XG_0, op.addr_xg), (c.ADDR_K_0, op.addr_k),
+                       (c.ADDR_XR_0, op.addr_xr)):
+            r[col], r[col + 1] = v & 0xFFFFFFFF, v >> 32
+        for base, v, nb in ((c.XR, w["xr"], 32), (c.YR, w["yr"], 32), (c.XG, op.xg, 32), (c.YG, w["yg"], 32),
+                            (c.X2, w["x2"], 32), (c.Q0, w["q0"], 32), (c.Q1, w["q1"], 33)):
+            r[base:base + nb] = _le(v, nb)
+        r[c.K:c.K + 256] = [(op.k >> b) & 1 for b in range(256)]
+        r[c.LEN_K] = w["len_k"]
+        for base, v in zip((c.XG_SUB_P, c.K_SUB_N, c.XR_SUB_P), w["sub"]):
+            r[base:base + 16] = [(v >> (16 * j)) & 0xFFFF for j in range(16)]
+        r[c.C0:c.C0 + 64] = [v % P for v in w["c0"]]
+        r[c.C1:c.C1 + 64] = [v % P for v in w["c1"]]
+        r[c.MU] = 1
+    return t
+
+
+def _memw_read_tuple(value, is_register, base_lo, base_hi, ts_lo, ts_hi, w2, w8):
+    k = lambda v: BusValue.of_linear([LinearTerm(v)])  # noqa: E731
+    return list(value) + [k(is_register), base_lo, base_hi] + list(value) + [ts_lo, ts_hi, k(w2), k(0), k(w8)]
+
+
+def ecsm_bus_interactions():
+    """ecsm.rs:300-583, in its order (579): the Ecall receiver [ts, -11]; per
+    operand the register read of its pointer (x11 xG at ts, x12 k at ts + 1,
+    x10 xR at ts + 2) and four 8-byte MEMW accesses (reads of xG and of k —
+    whose bytes are sums of its bit columns — and writes of xR); ARE_BYTES[x,
+    0] of the x2, q0, yG, q1 bytes; IS_HALFWORD of carries 0..62 (offset) of
+    both relations and of the 48 halfwords of the three overflow addends;
+    ZERO[sum of k's bytes, 0]; Bit[ts, i] received with multiplicity k_i for
+    every bit, Bit[ts, len_k] sent; the Ecdas seed [G, G, len_k - 1, 0] sent and
+    result [R, G, -1, 0] received (all mu unless noted)."""
+    c = EcsmCols
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    k = lambda v: BusValue.of_linear([LinearTerm(v)])  # noqa: E731
+    mu = Multiplicity("Column", (c.MU,))
+    ts_lo, ts_hi = d(c.TIMESTAMP_0), d(c.TIMESTAMP_1)
+    ts_plus = lambda off: BusValue.of_linear([LinearTerm(1, c.TIMESTAMP_0), LinearTerm(off)])  # noqa: E731
+    reg = lambda lo: [d(lo), d(lo + 1)] + [k(0)] * 6  # noqa: E731
+    its = [BusInteraction.receiver(BUS_ECALL, mu, [ts_lo, ts_hi, k(ECSM_SYSCALL_NUMBER & 0xFFFFFFFF),
+                                                  k(ECSM_SYSCALL_NUMBER >> 32)])]
+    kbyte = lambda i: BusValue.of_linear([LinearTerm(1 << j, c.K + 8 * i + j) for j in range(8)])  # noqa: E731
+    for addr, regno, off, value_of, write in ((c.ADDR_XG_0, 11, 0, lambda i: [d(c.XG + 8 * i + b) for b in range(8)],
+                                               False),
+                                              (c.ADDR_K_0, 12, 1, lambda i: [kbyte(8 * i + b) for b in range(8)], False),
+                                              (c.ADDR_XR_0, 10, 2, lambda i: [d(c.XR + 8 * i + b) for b in range(8)],
+                                               True)):
+        ts = ts_plus(off) if off else ts_lo
+        its.append(BusInteraction.sender(BUS_MEMW, mu, _memw_read_tuple(reg(addr), 1, k(2 * regno), k(0), ts, ts_hi,
+                                                                         1, 0)))
+        for i in range(4):
+            base = BusValue.of_linear([LinearTerm(1, addr), LinearTerm(8 * i)])
+            if write:
+                its.append(BusInteraction.sender(BUS_MEMW, mu, [k(0), base, d(addr + 1)] + value_of(i)
+                                                 + [ts, ts_hi, k(0), k(0), k(1)]))
+            else:
+                its.append(BusInteraction.sender(BUS_MEMW, mu, _memw_read_tuple(value_of(i), 0, base, d(addr + 1), ts,
+                                                                                 ts_hi, 0, 1)))
+    for base, n in ((c.X2, 32), (c.Q0, 32), (c.YG, 32), (c.Q1, 33)):
+        its += [BusInteraction.sender(BUS_ARE_BYTES, mu, [d(base + i), k(0)]) for i in range(n)]
+    for base, off in zip((c.C0, c.C1), ECSM_CARRY_OFFSETS):
+        its += [BusInteraction.sender(BUS_IS_HALFWORD, mu, [BusValue.of_linear([LinearTerm(1, base + i),
+                                                                                LinearTerm(off)])])
+                for i in range(63)]
+    for base in (c.XG_SUB_P, c.K_SUB_N, c.XR_SUB_P):
+        its += [BusInteraction.sender(BUS_IS_HALFWORD, mu, [d(base + i)]) for i in range(16)]
+    its.append(BusInteraction.sender(BUS_ZERO, mu, [
+        BusValue.of_linear([LinearTerm(1 << (b % 8), c.K + b) for b in range(256)]), k(0)]))
+    its += [BusInteraction.receiver(BUS_BIT, Multiplicity("Column", (c.K + i,)), [ts_lo, ts_hi, k(i)])
+            for i in range(256)]
+    its.append(BusInteraction.sender(BUS_BIT, mu, [ts_lo, ts_hi, d(c.LEN_K)]))
+    its.append(BusInteraction.sender(BUS_ECDAS, mu, ecdas_tuple(
+        c.XG, c.YG, c.XG, c.YG, BusValue.of_linear([LinearTerm(1, c.LEN_K), LinearTerm(-1)]), k(0), ts_lo, ts_hi)))
+    its.append(BusInteraction.receiver(BUS_ECDAS, mu, ecdas_tuple(
+        c.XR, c.YR, c.XG, c.YG, BusValue.of_linear([LinearTerm(-1)]), k(0), ts_lo, ts_hi)))
+    return its
+
+
+def ecsm_table_air() -> AirProgram:
+    """ECSM as an AIR (ecsm.rs:700-902, 413 constraints): 0 mu a bit; 1-256 the
+    scalar bits; 257 no bits on padding rows; 258-322 the x2 relation's carry
+    identities and closing carry, 323-387 the yG relation's (with mu·p^2 and
+    mu·7); 388 q1's top byte a bit; then per overflow addition (p + (xG - p)
+    mod 2^256 = xG + 2^256, likewise k against N and xR against p) its seven
+    inner word carries mu-gated bits and the carry-out mu-gated 1; then the
+    LogUp constraints of its 579 interactions."""
+    c = EcsmCols
+    b = ir.IrBuilder()
+    m = lambda col: b.main(0, col)  # noqa: E731
+    one, zero = b.const_base(1), b.const_base(0)
+    pb = _le(SECP_P, 32)
+    at = lambda base, n, j: m(base + j) if j < n else zero  # noqa: E731
+    _emit_is_bit(b, 0, c.MU)
+    for i in range(256):
+        _emit_is_bit(b, 1 + i, c.K + i)
+    ksum = m(c.K)
+    for i in range(1, 256):
+        ksum = ksum + m(c.K + i)
+    b.emit(257, ksum * (one - m(c.MU)))
+    idx = 258
+    for rel, cb in ((0, c.C0), (1, c.C1)):
+        for i in range(64):
+            s = zero
+            p2 = 0
+            for j in range(i + 1):
+                if rel == 0:
+                    if j < 32 and i - j < 32:
+                        s = s + at(c.XG, 32, j) * at(c.XG, 32, i - j)
+                    if j < 32 and i - j < 32 and pb[i - j]:
+                        s = s - at(c.Q0, 32, j) * b.const_base(pb[i - j])
+                else:
+                    if j < 32 and i - j < 32:
+                        s = s + at(c.YG, 32, j) * at(c.YG, 32, i - j) - at(c.X2, 32, j) * at(c.XG, 32, i - j)
+                        p2 += pb[j] * pb[i - j]
+                    if j < 33 and i - j < 32 and pb[i - j]:
+                        s = s - at(c.Q1, 33, j) * b.const_base(pb[i - j])
+            if rel == 0:
+                s = s - at(c.X2, 32, i)
+            else:
+                s = s + m(c.MU) * b.const_base(p2)
+                if i == 0:
+                    s = s - m(c.MU) * b.const_base(SECP_B)
+            prev = m(cb + i - 1) if i else zero
+            b.emit(idx, b.const_base(256) * m(cb + i) - prev - s)
+            idx += 1
+        b.emit(idx, m(cb + 63))
+        idx += 1
+    _emit_is_bit(b, idx, c.Q1 + 32)
+    idx += 1
+    inv = b.const_base(INV_SHIFT_32)
+    for bound, hl, base, bits in ((SECP_P, c.XG_SUB_P, c.XG, False), (SECP_N, c.K_SUB_N, c.K, True),
+                                  (SECP_P, c.XR_SUB_P, c.XR, False)):
+        prev = zero
+        carries = []
+        for i in range(8):
+            addend1 = m(hl + 2 * i) + m(hl + 2 * i + 1) * b.const_base(1 << 16)
+            if bits:
+                s = m(base + 32 * i)
+                for bit in range(1, 32):
+                    s = s + m(base + 32 * i + bit) * b.const_base(1 << bit)
+            else:
+                s = m(base + 4 * i)
+                for byte in range(1, 4):
+                    s = s + m(base + 4 * i + byte) * b.const_base(1 << (8 * byte))
+            ci = (b.const_base((bound >> (32 * i)) & 0xFFFFFFFF) + addend1 + prev - s) * inv
+            carries.append(ci)
+            prev = ci
+        for ci in carries[:7]:
+            b.emit(idx, m(c.MU) * ci * (one - ci))
+            idx += 1
+        b.emit(idx, m(c.MU) * (one - carries[7]))
+        idx += 1
+    its = ecsm_bus_interactions()
+    n_total = emit_logup_constraints(b, its, idx)
+    prog = b.finish(c.NUM_COLUMNS)
+    pairs, absorbed = split_interactions(len(its))
+    return AirProgram("ecsm", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
+                      boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
+                      max_degree=max(3, 1 + absorbed))

@@ -662,3 +662,29 @@ def test_gpu_ecdas_table_proof(gpu):
     ref = replay_air_tables(tables, 4, 3, 2, 8, 0, b"gpu-ecdas")
     assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
     assert verify_air_tables([air], got, 4, 3, 2, 8, 0, b"gpu-ecdas")[0]
+
+
+def test_gpu_ecsm_ecdas_proof(gpu):
+    """ECSM (667 columns, 413 constraints, 579 interactions) with the ECDAS
+    chain of the same three scalar multiplications: the GPU proofs equal the
+    oracle replay byte for byte and verify."""
+    from lambda_vm_amd import proof_io, prover
+    from lambda_vm_amd import vm_tables as V
+    from lambda_vm_amd.prover import AirTableProof
+    from tests.test_vm_table_ecsm import KS, ecdas_rows, ecsm_ops
+
+    print("[gpu-air] ECSM 4 x 667 + ECDAS 1024 x 521, blowup 4", flush=True)
+    ops = ecsm_ops(KS)
+    ecsm, ecdas = V.ecsm_table_air(), V.ecdas_table_air()
+    for a in (ecsm, ecdas):
+        a.bus_descriptor = A.descriptor(a)
+    tables = [(ecsm, V.generate_ecsm_trace(ops)), (ecdas, V.generate_ecdas_trace(ecdas_rows(ops)))]
+    tables = [(a, np.ascontiguousarray(tr, dtype=np.uint64)) for a, tr in tables]
+    opts = prover.ProofOptions(blowup=4, coset_offset=3, fri_final_log_degree=1, num_queries=8)
+    dev = [(a, gpu.DeviceBuffer.from_array(tr), tr.shape[0]) for a, tr in tables]
+    got = prover.prove_air_tables(dev, opts, label=b"gpu-ecsm")
+    for _, d, _ in dev:
+        d.free()
+    ref = replay_air_tables(tables, 4, 3, 1, 8, 0, b"gpu-ecsm")
+    assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
+    assert verify_air_tables([ecsm, ecdas], got, 4, 3, 1, 8, 0, b"gpu-ecsm")[0]
