@@ -3818,3 +3818,124 @@ def ecsm_table_air() -> AirProgram:
     return AirProgram("ecsm", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
                       boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
                       max_degree=max(3, 1 + absorbed))
+
+
+# LOCAL_TO_GLOBAL (prover/src/tables/local_to_global.rs, continuation.rs:
+# 150-200): per epoch, one row per memory cell the epoch touched, with its
+# init claim (the value when first touched, the epoch that last wrote it) and
+# fini claim (the value at the epoch's end, its last access timestamp). Inside
+# the epoch proof the table bookends the cell on the Memory bus like PAGE; in
+# the global proof its init/fini claims chain the epochs on the GlobalMemory
+# bus between GLOBAL_MEMORY's genesis and final tokens. Epoch labels are 1-based
+# (genesis 0); init_epoch < fini_epoch is an IS_B20 check of their gap.
+MAX_EPOCHS = 1 << 20
+
+
+class L2gCols:
+    """local_to_global.rs:210-250."""
+    ADDRESS_LO, ADDRESS_HI, INIT_VALUE, INIT_EPOCH_0, INIT_EPOCH_1 = 0, 1, 2, 3, 4
+    FINI_VALUE, FINI_TIMESTAMP_LO, FINI_TIMESTAMP_HI, MU = 5, 6, 7, 8
+    NUM_COLUMNS = 9
+
+
+@dataclass(frozen=True)
+class CellBoundary:
+    """local_to_global.rs:90-130: a cell's init claim (value, originating epoch,
+    its timestamp) and fini claim (value, this epoch's label, last access
+    timestamp)."""
+    address: int
+    init_value: int
+    init_epoch: int
+    init_timestamp: int
+    fini_value: int
+    fini_epoch: int
+    fini_timestamp: int
+
+
+def epoch_boundaries(initial_memory: dict, epochs) -> list:
+    """local_to_global.rs:140-205: epochs[e] lists (address, end value, end
+    timestamp) of the cells epoch e touched; each cell's init comes from its
+    last writer (genesis: epoch 0, timestamp 0, the program-start value or 0),
+    its fini names epoch label e + 1."""
+    prov = {a: (GENESIS_EPOCH, v, 0) for a, v in initial_memory.items()}
+    out = []
+    for e, touched in enumerate(epochs):
+        label = e + 1
+        rows = []
+        for addr, val, ts in touched:
+            oe, ov, ot = prov.get(addr, (GENESIS_EPOCH, 0, 0))
+            rows.append(CellBoundary(addr, ov, oe, ot, val, label, ts))
+            prov[addr] = (label, val, ts)
+        out.append(rows)
+    return out
+
+
+def generate_l2g_trace(boundaries) -> np.ndarray:
+    """local_to_global.rs:267-292: one row per boundary (mu = 1), zero rows to
+    a power of two (at least 1)."""
+    c = L2gCols
+    n = 1 << max(0, (len(boundaries) - 1).bit_length()) if boundaries else 1
+    t = np.zeros((n, c.NUM_COLUMNS), dtype=np.uint64)
+    for i, bd in enumerate(boundaries):
+        if bd.init_epoch >= 1 << 32:
+            raise ValueError("epoch label exceeds 32 bits")
+        t[i] = (bd.address & 0xFFFFFFFF, bd.address >> 32, bd.init_value & 0xFF, bd.init_epoch & 0xFFFF,
+                (bd.init_epoch >> 16) & 0xFFFF, bd.fini_value & 0xFF, bd.fini_timestamp & 0xFFFFFFFF,
+                bd.fini_timestamp >> 32, 1)
+    return t
+
+
+def l2g_global_bus_interactions(epoch_label: int):
+    """local_to_global.rs:339-365: GlobalMemory[address, init value, init epoch]
+    received and [address, fini value, epoch_label] sent (mu)."""
+    c = L2gCols
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    mu = Multiplicity("Column", (c.MU,))
+    word = BusValue.of_linear([LinearTerm(1, c.INIT_EPOCH_0), LinearTerm(1 << 16, c.INIT_EPOCH_1)])
+    return [BusInteraction.receiver(BUS_GLOBAL_MEMORY, mu, [d(c.ADDRESS_LO), d(c.ADDRESS_HI), d(c.INIT_VALUE), word]),
+            BusInteraction.sender(BUS_GLOBAL_MEMORY, mu, [d(c.ADDRESS_LO), d(c.ADDRESS_HI), d(c.FINI_VALUE),
+                                                          BusValue.of_linear([LinearTerm(epoch_label)])])]
+
+
+def l2g_memory_bus_interactions(epoch_label: int):
+    """local_to_global.rs:367-440 (continuation.rs:186-200 order): the Memory
+    token [0, address, 0, 0, init value] received and [0, address, fini
+    timestamp, fini value] sent; ARE_BYTES[init value, fini value], IS_HALFWORD
+    of both init-epoch halves and IS_B20[epoch_label - 1 - init_epoch] (all mu)."""
+    if epoch_label < 1:
+        raise ValueError("epoch_label must be a 1-based fini epoch")
+    c = L2gCols
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    k = lambda v: BusValue.of_linear([LinearTerm(v)])  # noqa: E731
+    mu = Multiplicity("Column", (c.MU,))
+    return [
+        BusInteraction.receiver(BUS_MEMORY, mu, [k(0), d(c.ADDRESS_LO), d(c.ADDRESS_HI), k(0), k(0), d(c.INIT_VALUE)]),
+        BusInteraction.sender(BUS_MEMORY, mu, [k(0), d(c.ADDRESS_LO), d(c.ADDRESS_HI), d(c.FINI_TIMESTAMP_LO),
+                                               d(c.FINI_TIMESTAMP_HI), d(c.FINI_VALUE)]),
+        BusInteraction.sender(BUS_ARE_BYTES, mu, [d(c.INIT_VALUE), d(c.FINI_VALUE)]),
+        BusInteraction.sender(BUS_IS_HALFWORD, mu, [d(c.INIT_EPOCH_0)]),
+        BusInteraction.sender(BUS_IS_HALFWORD, mu, [d(c.INIT_EPOCH_1)]),
+        BusInteraction.sender(BUS_IS_B20, mu, [BusValue.of_linear([
+            LinearTerm(epoch_label - 1), LinearTerm(-1, c.INIT_EPOCH_0), LinearTerm(-(1 << 16), c.INIT_EPOCH_1)])]),
+    ]
+
+
+def l2g_memory_air(epoch_label: int) -> AirProgram:
+    """The epoch-local L2G AIR (continuation.rs:150-200): mu a bit, then the
+    LogUp constraints of its six interactions."""
+    c = L2gCols
+    b = ir.IrBuilder()
+    _emit_is_bit(b, 0, c.MU)
+    its = l2g_memory_bus_interactions(epoch_label)
+    n_total = emit_logup_constraints(b, its, 1)
+    prog = b.finish(c.NUM_COLUMNS)
+    pairs, absorbed = split_interactions(len(its))
+    return AirProgram("l2g_memory", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
+                      boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
+                      max_degree=max(3 if pairs else 0, 1 + absorbed, 2))
+
+
+def l2g_global_air(epoch_label: int) -> AirProgram:
+    """The global-proof L2G AIR (continuation.rs:159-183): no constraints of
+    its own, the LogUp constraints of its two GlobalMemory interactions."""
+    return _bus_only_air("l2g_global", L2gCols.NUM_COLUMNS, l2g_global_bus_interactions(epoch_label), 0, None)

@@ -688,3 +688,30 @@ def test_gpu_ecsm_ecdas_proof(gpu):
     ref = replay_air_tables(tables, 4, 3, 1, 8, 0, b"gpu-ecsm")
     assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
     assert verify_air_tables([ecsm, ecdas], got, 4, 3, 1, 8, 0, b"gpu-ecsm")[0]
+
+
+def test_gpu_l2g_proofs(gpu):
+    """LOCAL_TO_GLOBAL: the epoch-local AIR (Memory bus, range and ordering
+    checks) and the global AIR (GlobalMemory bus) over one epoch's boundary
+    trace: the GPU proofs equal the oracle replay byte for byte and verify."""
+    from lambda_vm_amd import proof_io, prover
+    from lambda_vm_amd import vm_tables as V
+    from lambda_vm_amd.prover import AirTableProof
+    from tests.test_vm_table_l2g import PAGE, scenario
+
+    print("[gpu-air] L2G memory + global over 32 rows, blowup 4", flush=True)
+    init, touched = scenario(131, epochs=3)
+    rows = V.epoch_boundaries({PAGE + i: v for i, v in enumerate(init)}, touched)[2]
+    tr = np.ascontiguousarray(V.generate_l2g_trace(rows), dtype=np.uint64)
+    mem, glob = V.l2g_memory_air(3), V.l2g_global_air(3)
+    for a in (mem, glob):
+        a.bus_descriptor = A.descriptor(a)
+    tables = [(mem, tr), (glob, tr)]
+    opts = prover.ProofOptions(blowup=4, coset_offset=3, fri_final_log_degree=2, num_queries=8)
+    dev = [(a, gpu.DeviceBuffer.from_array(t), t.shape[0]) for a, t in tables]
+    got = prover.prove_air_tables(dev, opts, label=b"gpu-l2g")
+    for _, d, _ in dev:
+        d.free()
+    ref = replay_air_tables(tables, 4, 3, 2, 8, 0, b"gpu-l2g")
+    assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
+    assert verify_air_tables([mem, glob], got, 4, 3, 2, 8, 0, b"gpu-l2g")[0]
