@@ -3939,3 +3939,263 @@ def l2g_global_air(epoch_label: int) -> AirProgram:
     """The global-proof L2G AIR (continuation.rs:159-183): no constraints of
     its own, the LogUp constraints of its two GlobalMemory interactions."""
     return _bus_only_air("l2g_global", L2gCols.NUM_COLUMNS, l2g_global_bus_interactions(epoch_label), 0, None)
+
+
+# COPY (prover/src/tables/copy.rs): the streaming copy behind memcpy/memmove,
+# memset and write(2). A call is a chain of rows of 8 bytes (then 1-byte rows
+# for the last seven) linked on the CopyNext bus; every row of a call shares
+# the call's timestamp T, reads at T + is_set and writes at T + 1 - is_set (so
+# copies are snapshots and memset's self-overlapping copy propagates its
+# seed), and the read/write multiplicity mu - borrow is a linear form of the
+# count columns (the borrow out of count - step marks the zero-length call).
+BUS_COMMIT, BUS_COPY_NEXT = 21, 33          # types.rs:316, :388
+WRITE_SYSCALL_NUMBER = 64
+DMA_MEMCPY_SYSCALL_NUMBER = 2**64 - 1 - 29  # execution.rs:72
+DMA_MEMSET_SYSCALL_NUMBER = 2**64 - 1 - 30  # execution.rs:143
+DMA_MEMSET_GAP = 8                          # execution.rs:159
+
+
+class CopyCols:
+    """copy.rs:150-213."""
+    TIMESTAMP_0, TIMESTAMP_1 = 0, 1
+    VALUE = tuple(range(2, 10))
+    SRC_0, SRC_1, SRC_INCR = 10, 11, 12
+    DST_0, DST_1, DST_INCR = 16, 17, 18
+    COUNT_0, COUNT_1, COUNT_DECR = 22, 23, 24
+    DST_RES = 28
+    FIRST, END, SINGLE, IS_SET, IS_WRITE, FIRST_WRITE, MU, MU_COM, MU_COM_WIDE = range(32, 41)
+    NUM_COLUMNS = 41
+
+
+@dataclass(frozen=True)
+class CopyOperation:
+    """copy.rs:226-260: one row (functionality: copy, set or write)."""
+    timestamp: int
+    src: int
+    dst: int
+    count: int
+    width: int
+    first: bool
+    end: bool
+    functionality: str
+    value: tuple
+
+    def dst_res(self) -> int:
+        return (self.dst + self.count) & (2**64 - 1) if self.functionality == "write" else self.dst
+
+
+def copy_call_rows(functionality: str, timestamp: int, src: int, dst: int, count: int, memory: dict) -> list:
+    """trace_builder.rs:1155-1255 collect_copy_ops over a byte memory (address
+    -> byte, updated): 8-byte rows while 8 or more bytes remain, then 1-byte
+    rows (execution.rs:101-107); a copy reads every row before its writes land
+    (snapshot), a set writes each row before the next reads (propagation), a
+    write leaves memory unchanged; a zero-length call is one row with count 0,
+    width 1. The last row ends the chain."""
+    M = 2**64 - 1
+    rows, writes, off, rem = [], [], 0, count
+    while rem:
+        width = 1 if rem < 8 else 8
+        s, d = (src + off) & M, (dst + off) & M
+        vals = [memory.get((s + i) & M, 0) for i in range(width)]
+        if functionality == "set":
+            for i, v in enumerate(vals):
+                memory[(d + i) & M] = v
+        elif functionality == "copy":
+            writes.append((d, vals))
+        rows.append(CopyOperation(timestamp, s, d, rem, width, off == 0, False, functionality,
+                                  tuple(vals + [0] * (8 - width))))
+        off += width
+        rem -= width
+    for d, vals in writes:
+        for i, v in enumerate(vals):
+            memory[(d + i) & M] = v
+    if rows:
+        last = rows[-1]
+        rows[-1] = CopyOperation(last.timestamp, last.src, last.dst, last.count, last.width, last.first, True,
+                                 last.functionality, last.value)
+    else:
+        rows.append(CopyOperation(timestamp, src, dst, 0, 1, True, True, functionality, (0,) * 8))
+    return rows
+
+
+def generate_copy_trace(ops) -> np.ndarray:
+    """copy.rs:262-320: one row per operation; padding rows carry count =
+    src_incr = dst_incr = 8 (every relation and borrow at zero); zero rows to a
+    power of two (at least 4)."""
+    c = CopyCols
+    M = 2**64 - 1
+    n = max(4, 1 << max(0, (len(ops) - 1).bit_length()))
+    t = np.zeros((n, c.NUM_COLUMNS), dtype=np.uint64)
+    wl = lambda r, col, v: r.__setitem__(slice(col, col + 2), [v & 0xFFFFFFFF, (v >> 32) & 0xFFFFFFFF])  # noqa
+    hl = lambda r, col, v: r.__setitem__(slice(col, col + 4), [(v >> (16 * k)) & 0xFFFF for k in range(4)])  # noqa
+    for i, op in enumerate(ops):
+        r = t[i]
+        wl(r, c.TIMESTAMP_0, op.timestamp)
+        r[c.VALUE[0]:c.VALUE[-1] + 1] = op.value
+        wl(r, c.SRC_0, op.src)
+        hl(r, c.SRC_INCR, (op.src + op.width) & M)
+        wl(r, c.DST_0, op.dst)
+        hl(r, c.DST_INCR, (op.dst + op.width) & M)
+        wl(r, c.COUNT_0, op.count)
+        hl(r, c.COUNT_DECR, (op.count - op.width) & M)
+        hl(r, c.DST_RES, op.dst_res())
+        is_set, is_write = op.functionality == "set", op.functionality == "write"
+        r[c.FIRST], r[c.END], r[c.SINGLE], r[c.MU] = int(op.first), int(op.end), int(op.width == 1), 1
+        r[c.IS_SET], r[c.IS_WRITE], r[c.FIRST_WRITE] = int(is_set), int(is_write), int(op.first and is_write)
+        r[c.MU_COM] = int(op.count != 0 and is_write)
+        r[c.MU_COM_WIDE] = int(op.count != 0 and is_write and op.width == 8)
+    t[len(ops):, c.COUNT_0] = 8
+    t[len(ops):, c.SRC_INCR] = 8
+    t[len(ops):, c.DST_INCR] = 8
+    return t
+
+
+def _copy_borrow_terms():
+    """copy.rs:355-400: borrow_1 of count - step as a linear form:
+    2^-32 decr2 + 2^-16 decr3 - 2^-32 count1 + 2^-64 decr0 + 2^-48 decr1 -
+    2^-64 count0 - 7·2^-64 single + 8·2^-64."""
+    c = CopyCols
+    inv = lambda k: pow(2, P - 1 - k, P)  # noqa: E731
+    return [LinearTerm(inv(32), c.COUNT_DECR + 2, True), LinearTerm(inv(16), c.COUNT_DECR + 3, True),
+            LinearTerm(P - inv(32), c.COUNT_1, True), LinearTerm(inv(64), c.COUNT_DECR, True),
+            LinearTerm(inv(48), c.COUNT_DECR + 1, True), LinearTerm(P - inv(64), c.COUNT_0, True),
+            LinearTerm(P - 7 * inv(64) % P, c.SINGLE, True), LinearTerm(-(1 << 35))]
+
+
+def _neg_terms(terms):
+    return [LinearTerm(-t.coefficient, t.column) if t.column is None or not t.unsigned
+            else LinearTerm((P - t.coefficient) % P, t.column, True) for t in terms]
+
+
+def copy_bus_interactions():
+    """copy.rs:505-775, in its order (33): Ecall [ts, number] with the number a
+    linear function of is_set/is_write (first); the register reads of x11
+    (src) and x12 (count), the read-write of x10 (x254 for write) from dst to
+    dst_res (first), and for write the x10 read-write from the fd 1 to count
+    (first_write); CopyNext [ts, src + step, dst + step, count - step, is_set,
+    is_write] sent (mu - end) and [ts, src, dst, count, is_set, is_write]
+    received (mu - first); IS_HALFWORD of the increments (mu) and of dst_res
+    (is_write); the 8-byte MEMW read of src (at ts + is_set) and write of dst
+    (at ts + 1 - is_set) with multiplicities mu - borrow and mu - borrow -
+    mu_com; the Commit pairs [dst + lane, value_lane] (mu_com for lane 0,
+    mu_com_wide for lanes 1-7)."""
+    c = CopyCols
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    k = lambda v: BusValue.of_linear([LinearTerm(v)])  # noqa: E731
+    col = lambda cc: Multiplicity("Column", (cc,))  # noqa: E731
+    ts = [d(c.TIMESTAMP_0), d(c.TIMESTAMP_1)]
+    lo = lambda v: v & 0xFFFFFFFF  # noqa: E731
+    hi = lambda v: v >> 32  # noqa: E731
+    num = lambda f: BusValue.of_linear([  # noqa: E731
+        LinearTerm(f(DMA_MEMCPY_SYSCALL_NUMBER)),
+        LinearTerm(f(DMA_MEMSET_SYSCALL_NUMBER) - f(DMA_MEMCPY_SYSCALL_NUMBER), c.IS_SET),
+        LinearTerm(f(WRITE_SYSCALL_NUMBER) - f(DMA_MEMCPY_SYSCALL_NUMBER), c.IS_WRITE)])
+
+    def reg(address, old, value):
+        return old + [k(0)] * 6 + [k(1), address, k(0)] + value + [k(0)] * 6 + ts + [k(1), k(0), k(0)]
+
+    w8 = BusValue.of_linear([LinearTerm(1), LinearTerm(-1, c.SINGLE)])
+    w2l = lambda s: [BusValue.packed(Packing.WORD2L, s), BusValue.packed(Packing.WORD2L, s + 2)]  # noqa: E731
+    its = [BusInteraction.receiver(BUS_ECALL, col(c.FIRST), ts + [num(lo), num(hi)]),
+           BusInteraction.sender(BUS_MEMW, col(c.FIRST), reg(k(22), [d(c.SRC_0), d(c.SRC_1)], [d(c.SRC_0), d(c.SRC_1)])),
+           BusInteraction.sender(BUS_MEMW, col(c.FIRST), reg(k(24), [d(c.COUNT_0), d(c.COUNT_1)],
+                                                             [d(c.COUNT_0), d(c.COUNT_1)])),
+           BusInteraction.sender(BUS_MEMW, col(c.FIRST), reg(BusValue.of_linear([LinearTerm(20), LinearTerm(488, c.IS_WRITE)]),
+                                                             [d(c.DST_0), d(c.DST_1)], w2l(c.DST_RES))),
+           BusInteraction.sender(BUS_MEMW, col(c.FIRST_WRITE), reg(k(20), [k(1), k(0)], [d(c.COUNT_0), d(c.COUNT_1)])),
+           BusInteraction.sender(BUS_COPY_NEXT, Multiplicity("Diff", (c.MU, c.END)), ts + [
+               BusValue.packed(Packing.DWORD_HL, c.SRC_INCR), BusValue.packed(Packing.DWORD_HL, c.DST_INCR),
+               BusValue.packed(Packing.DWORD_HL, c.COUNT_DECR), d(c.IS_SET), d(c.IS_WRITE)]),
+           BusInteraction.receiver(BUS_COPY_NEXT, Multiplicity("Diff", (c.MU, c.FIRST)), ts + [
+               BusValue.packed(Packing.DWORD_WL, c.SRC_0), BusValue.packed(Packing.DWORD_WL, c.DST_0),
+               BusValue.packed(Packing.DWORD_WL, c.COUNT_0), d(c.IS_SET), d(c.IS_WRITE)])]
+    for start in (c.SRC_INCR, c.DST_INCR, c.COUNT_DECR):
+        its += [BusInteraction.sender(BUS_IS_HALFWORD, col(c.MU), [d(start + j)]) for j in range(4)]
+    its += [BusInteraction.sender(BUS_IS_HALFWORD, col(c.IS_WRITE), [d(c.DST_RES + j)]) for j in range(4)]
+    active = [LinearTerm(1, c.MU)] + _neg_terms(_copy_borrow_terms())
+    vals = [d(v) for v in c.VALUE]
+    tso = lambda off, s: BusValue.of_linear([LinearTerm(1, c.TIMESTAMP_0), LinearTerm(s, c.IS_SET), LinearTerm(off)])  # noqa
+    its.append(BusInteraction.sender(BUS_MEMW, Multiplicity("Linear", linear=tuple(active)),
+                                     vals + [k(0), d(c.SRC_0), d(c.SRC_1)] + vals
+                                     + [tso(0, 1), d(c.TIMESTAMP_1), k(0), k(0), w8]))
+    its.append(BusInteraction.sender(BUS_MEMW, Multiplicity("Linear", linear=tuple(active + [LinearTerm(-1, c.MU_COM)])),
+                                     [k(0), d(c.DST_0), d(c.DST_1)] + vals
+                                     + [tso(1, -1), d(c.TIMESTAMP_1), k(0), k(0), w8]))
+    for lane in range(8):
+        its.append(BusInteraction.sender(BUS_COMMIT, col(c.MU_COM if lane == 0 else c.MU_COM_WIDE), [
+            BusValue.of_linear([LinearTerm(1, c.DST_0), LinearTerm(lane)]), d(c.VALUE[lane])]))
+    return its
+
+
+def copy_table_air() -> AirProgram:
+    """COPY as an AIR (copy.rs:780-1060, 40 constraints, degree 3): 0-5 bits;
+    6 first/end only on real rows; 7 not both set and write; 8 a functionality
+    only on real rows; 9 first_write = first·is_write; 10-15 the carry pairs of
+    src + step = src_incr, dst + step = dst_incr, count_decr + step = count
+    (step = 8 - 7 single); 16-17 of dst + count = dst_res on write rows; 18-19
+    dst_res = dst otherwise; 20-21 of src + 8 = dst on set rows; 22-28 a
+    single-byte row carries only value[0]; 29-34 the borrow of count - step
+    only on a first row with count 0, and end exactly on the row moving the
+    last bytes (count = step) or on the zero-length row; 35-36 mu_com and
+    mu_com_wide; 37-39 no carry out of src + step, dst + step (while moving
+    bytes) and src + 8 (set rows); then the LogUp constraints of its 33
+    interactions."""
+    c = CopyCols
+    b = ir.IrBuilder()
+    m = lambda col: b.main(0, col)  # noqa: E731
+    one, zero, inv, s16 = b.const_base(1), b.const_base(0), b.const_base(INV_SHIFT_32), b.const_base(1 << 16)
+    for i, col in enumerate((c.FIRST, c.END, c.SINGLE, c.MU, c.IS_SET, c.IS_WRITE)):
+        _emit_is_bit(b, i, col)
+    first, end, mu, single, is_set, is_write = (m(x) for x in (c.FIRST, c.END, c.MU, c.SINGLE, c.IS_SET, c.IS_WRITE))
+    b.emit(6, (first + end) * (one - mu))
+    b.emit(7, is_set * is_write)
+    b.emit(8, (is_set + is_write) * (one - mu))
+    b.emit(9, m(c.FIRST_WRITE) - first * is_write)
+    step = (b.const_base(8) - b.const_base(7) * single, zero)
+    wl = lambda col: (m(col), m(col + 1))  # noqa: E731
+    hl = lambda col: (m(col) + m(col + 1) * s16, m(col + 2) + m(col + 3) * s16)  # noqa: E731
+
+    def carries(lhs, rhs, out):
+        c0 = (lhs[0] + rhs[0] - out[0]) * inv
+        return c0, (lhs[1] + rhs[1] + c0 - out[1]) * inv
+
+    def pair(idx, gate, lhs, rhs, out):
+        c0, c1 = carries(lhs, rhs, out)
+        b.emit(idx, c0 * (one - c0) if gate is None else gate * c0 * (one - c0))
+        b.emit(idx + 1, c1 * (one - c1) if gate is None else gate * c1 * (one - c1))
+
+    pair(10, None, wl(c.SRC_0), step, hl(c.SRC_INCR))
+    pair(12, None, wl(c.DST_0), step, hl(c.DST_INCR))
+    pair(14, None, hl(c.COUNT_DECR), step, wl(c.COUNT_0))
+    pair(16, is_write, wl(c.DST_0), wl(c.COUNT_0), hl(c.DST_RES))
+    dres = hl(c.DST_RES)
+    b.emit(18, (one - is_write) * (dres[0] - m(c.DST_0)))
+    b.emit(19, (one - is_write) * (dres[1] - m(c.DST_1)))
+    gap = (b.const_base(DMA_MEMSET_GAP), zero)
+    pair(20, is_set, wl(c.SRC_0), gap, wl(c.DST_0))
+    for i in range(1, 8):
+        b.emit(21 + i, single * m(c.VALUE[i]))
+    bt = _copy_borrow_terms()
+    borrow = b.const_base(bt[-1].coefficient % P)
+    for t in bt[:-1]:
+        borrow = borrow + b.const_base(t.coefficient) * m(t.column)
+    not_borrow = one - borrow
+    b.emit(29, borrow * (one - first))
+    b.emit(30, borrow * m(c.COUNT_0))
+    b.emit(31, borrow * m(c.COUNT_1))
+    b.emit(32, end * not_borrow * m(c.COUNT_1))
+    b.emit(33, end * not_borrow * (m(c.COUNT_0) - step[0]))
+    b.emit(34, borrow * (one - end))
+    b.emit(35, m(c.MU_COM) - (mu - borrow) * is_write)
+    b.emit(36, m(c.MU_COM_WIDE) - m(c.MU_COM) * (one - single))
+    active = mu - borrow
+    b.emit(37, active * carries(wl(c.SRC_0), step, hl(c.SRC_INCR))[1])
+    b.emit(38, active * carries(wl(c.DST_0), step, hl(c.DST_INCR))[1])
+    b.emit(39, is_set * carries(wl(c.SRC_0), gap, wl(c.DST_0))[1])
+    its = copy_bus_interactions()
+    n_total = emit_logup_constraints(b, its, 40)
+    prog = b.finish(c.NUM_COLUMNS)
+    pairs, absorbed = split_interactions(len(its))
+    return AirProgram("copy", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
+                      boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
+                      max_degree=max(3, 1 + absorbed))

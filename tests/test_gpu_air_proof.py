@@ -715,3 +715,28 @@ def test_gpu_l2g_proofs(gpu):
     ref = replay_air_tables(tables, 4, 3, 2, 8, 0, b"gpu-l2g")
     assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
     assert verify_air_tables([mem, glob], got, 4, 3, 2, 8, 0, b"gpu-l2g")[0]
+
+
+def test_gpu_copy_table_proof(gpu):
+    """COPY (41 columns, 40 degree-3 constraints, 33 interactions, two Linear
+    read/write multiplicities) over memcpy/memmove, memset and write calls of
+    0..64 bytes: the GPU proof equals the oracle replay byte for byte and
+    verifies."""
+    from lambda_vm_amd import proof_io, prover
+    from lambda_vm_amd import vm_tables as V
+    from lambda_vm_amd.prover import AirTableProof
+    from tests.test_vm_table_copy import run
+
+    _, rows, _, _, _ = run(141)
+    print(f"[gpu-air] COPY {len(rows)} rows x 41, blowup 4", flush=True)
+    air = V.copy_table_air()
+    air.bus_descriptor = A.descriptor(air)
+    tables = [(air, np.ascontiguousarray(V.generate_copy_trace(rows), dtype=np.uint64))]
+    opts = prover.ProofOptions(blowup=4, coset_offset=3, fri_final_log_degree=2, num_queries=8)
+    dev = [(a, gpu.DeviceBuffer.from_array(tr), tr.shape[0]) for a, tr in tables]
+    got = prover.prove_air_tables(dev, opts, label=b"gpu-copy")
+    for _, d, _ in dev:
+        d.free()
+    ref = replay_air_tables(tables, 4, 3, 2, 8, 0, b"gpu-copy")
+    assert proof_io.serialize_multi_proof(got) == proof_io.serialize_multi_proof([AirTableProof(**r) for r in ref])
+    assert verify_air_tables([air], got, 4, 3, 2, 8, 0, b"gpu-copy")[0]

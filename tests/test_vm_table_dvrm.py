@@ -199,6 +199,9 @@ def _eval_value(bv, row):
 
 
 def _eval_mult(m, row):
+    if m.kind == "Linear":  # a field value read back as a small signed count
+        v = sum(t.coefficient * (1 if t.column is None else int(row[t.column])) for t in m.linear) % P
+        return v - P if v > P // 2 else v
     col = lambda i: int(row[m.cols[i]])  # noqa: E731
     return {"One": lambda: 1, "Column": lambda: col(0), "Sum": lambda: col(0) + col(1),
             "Sum3": lambda: col(0) + col(1) + col(2), "Negated": lambda: 1 - col(0),
