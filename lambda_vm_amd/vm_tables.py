@@ -4199,3 +4199,57 @@ def copy_table_air() -> AirProgram:
     return AirProgram("copy", c.NUM_COLUMNS, pairs + 1, prog, ir.lower(prog, pin_roots=False), [0] * n_total,
                       boundary=[Boundary(pairs, True, 0, (0, 0, 0))], interactions=its,
                       max_degree=max(3, 1 + absorbed))
+
+
+# HALT (prover/src/tables/halt.rs): the one-row table of the sys_exit ecall.
+# It receives the ecall, finalizes the general registers at timestamp 2^32 - 1
+# (x1-x9 and x11-x31 written 0, x10 read with old value 0: a nonzero exit code
+# leaves the MEMW bus unbalanced) and swaps the pc token the halting
+# instruction wrote for the padding pc 1.
+HALT_SYSCALL_NUMBER = 93
+HALT_FINAL_TIMESTAMP = 2**32 - 1
+
+
+class HaltCols:
+    """halt.rs:50-61."""
+    TIMESTAMP_0, TIMESTAMP_1, PC_0, PC_1 = 0, 1, 2, 3
+    NUM_COLUMNS = 4
+
+
+def generate_halt_trace(timestamp: int, next_pc: int) -> np.ndarray:
+    """halt.rs:74-92: the single row [timestamp, next pc] (DWordWL each)."""
+    if timestamp >= 1 << 32:
+        raise ValueError("HALT timestamp exceeds 32 bits")
+    return np.array([[timestamp & 0xFFFFFFFF, timestamp >> 32, next_pc & 0xFFFFFFFF, next_pc >> 32]], dtype=np.uint64)
+
+
+def halt_bus_interactions():
+    """halt.rs:120-273, in its order (36, all multiplicity one): the Ecall
+    receiver [ts, 93, 0]; MEMW register writes of 0 to x1-x9, the read of x10
+    with old value 0, writes of 0 to x11-x31, all at 2^32 - 1 (register-width
+    accesses); per pc word the Memory token [1, 510 + i, 0, ts + 1, next_pc_i]
+    sent and [1, 510 + i, 0, ts + 1, (1, 0)_i] received."""
+    c = HaltCols
+    d = lambda col: BusValue.packed(Packing.DIRECT, col)  # noqa: E731
+    k = lambda v: BusValue.of_linear([LinearTerm(v)])  # noqa: E731
+    ts = [k(HALT_FINAL_TIMESTAMP), k(0)]
+    zero8 = [k(0)] * 8
+    its = [BusInteraction.receiver(BUS_ECALL, ONE, [d(c.TIMESTAMP_0), d(c.TIMESTAMP_1), k(HALT_SYSCALL_NUMBER), k(0)])]
+    for r in range(1, 32):
+        head = [k(1), k(2 * r), k(0)]
+        if r == 10:
+            its.append(BusInteraction.sender(BUS_MEMW, ONE, zero8 + head + zero8 + ts + [k(1), k(0), k(0)]))
+        else:
+            its.append(BusInteraction.sender(BUS_MEMW, ONE, head + zero8 + ts + [k(1), k(0), k(0)]))
+    nxt = BusValue.of_linear([LinearTerm(1, c.TIMESTAMP_0), LinearTerm(1)])
+    for i in range(2):
+        its.append(BusInteraction.sender(BUS_MEMORY, ONE, [k(1), k(510 + i), k(0), nxt, d(c.TIMESTAMP_1), d(c.PC_0 + i)]))
+    for i, v in enumerate((CPU_PADDING_PC, 0)):
+        its.append(BusInteraction.receiver(BUS_MEMORY, ONE, [k(1), k(510 + i), k(0), nxt, d(c.TIMESTAMP_1), k(v)]))
+    return its
+
+
+def halt_table_air() -> AirProgram:
+    """HALT as an AIR: no constraints of its own, the LogUp constraints of its
+    36 interactions."""
+    return _bus_only_air("halt", HaltCols.NUM_COLUMNS, halt_bus_interactions(), 0, None)
