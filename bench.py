@@ -141,6 +141,8 @@ def main() -> int:
                     help="ragged: jobs of the short serving pass that reports p50/p99 latency")
     ap.add_argument("--no-serve", action="store_true", help="ragged: skip the short serving pass")
     ap.add_argument("--budget-gib", type=float, default=48.0, help="serve: HBM budget for jobs in flight")
+    ap.add_argument("--vm-set", choices=["8", "all"], default="8",
+                    help="vm: eight 2^20-row tables, or every table of a 2^20-cycle run")
     ap.add_argument("--workers", type=int, default=8, help="serve: host threads proving tables concurrently")
     ap.add_argument("--tables-per-tick", type=int, default=8, help="serve: tables advanced per scheduler tick")
     ap.add_argument("--serve-mode", choices=["tick", "async"], default="async",
@@ -617,20 +619,55 @@ STAGE_NAMES = ["ntt_lde", "keccak_leaves", "merkle_inner", "logup_terms", "logup
                "deep_denoms", "deep_rows", "fri_fold_leaves", "fri_merkle", "fri_fold"]
 
 
+def vm_all_tables(V) -> list:
+    """(name, AIR, rows) of every VM table the GPU prover takes, at the heights
+    of a 2^20-cycle run with a 16K-instruction program, 2^10 Keccak
+    permutations, 16 scalar multiplications and two touched 2^18-byte pages
+    (heights chosen here as a representative mix; HALT's single row is below
+    the prover's two-row minimum)."""
+    entries = V.decode_entries_from_words([(0x1000, [0x00150513] * ((1 << 14) - 1))])  # addi a0, a0, 1
+    init = V.register_init_from_entry_point(0x1000)
+    page = 0x1_0000_0000
+    return [
+        ("CPU", V.cpu_table_air(1 << 20), 1 << 20), ("DECODE", V.decode_table_air(entries), 1 << 14),
+        ("CPU32", V.cpu32_table_air(), 1 << 17), ("EQ", V.eq_table_air(), 1 << 18),
+        ("BYTEWISE", V.bytewise_table_air(), 1 << 18), ("IS_WH", V.is_wh_table_air(), 1 << 18),
+        ("LT", V.lt_table_air(), 1 << 19), ("STORE", V.store_table_air(), 1 << 18),
+        ("BRANCH", V.branch_table_air(), 1 << 18), ("MEMW_REGISTER", V.memw_register_table_air(), 1 << 21),
+        ("LOAD", V.load_table_air(), 1 << 18), ("MUL", V.mul_table_air(), 1 << 16),
+        ("SHIFT", V.shift_table_air(), 1 << 17), ("DVRM", V.dvrm_table_air(), 1 << 14),
+        ("MEMW_A", V.memw_aligned_table_air(), 1 << 19), ("MEMW", V.memw_table_air(), 1 << 16),
+        ("HINT", V.hint_table_air(), 1 << 10), ("KECCAK", V.keccak_table_air(), 1 << 10),
+        ("KECCAK_RND", V.keccak_rnd_table_air(), 1 << 15), ("ECDAS", V.ecdas_table_air(), 1 << 12),
+        ("ECSM", V.ecsm_table_air(), 1 << 4), ("COPY", V.copy_table_air(), 1 << 14),
+        ("L2G_MEMORY", V.l2g_memory_air(1), 1 << 14), ("L2G_GLOBAL", V.l2g_global_air(1), 1 << 14),
+        ("KECCAK_RC", V.keccak_rc_table_air(), 32), ("BITWISE", V.bitwise_table_air(), 1 << 20),
+        ("REGISTER", V.register_table_air(init, init, V.cpu_final_pc_timestamp(1 << 20)), 128),
+        ("PAGE", V.page_table_air(page), V.PAGE_SIZE), ("PAGE", V.page_table_air(page + V.PAGE_SIZE), V.PAGE_SIZE),
+        ("GLOBAL_MEMORY", V.global_memory_table_air(page), V.PAGE_SIZE),
+    ]
+
+
 def run_vm(args) -> dict:
-    """Full proof of eight of the VM's own tables in one transcript, at 2^20 rows
-    (IS_WH 2^18): EQ (12 columns), BYTEWISE (26), IS_WH (4), LT (17), STORE
-    (16), BRANCH (14), MEMW_REGISTER (10), LOAD (18), blowup 4,
-    their constraints and bus interactions as captured from the VM's table
-    definitions. Traces are uniform draws of the tables' shapes (the prover's
-    work does not depend on whether the constraints hold; valid traces are
-    proved and verified in the tests)."""
+    """Full proof of VM tables in one transcript, blowup 4, their constraints
+    and bus interactions as captured from the VM's table definitions. The
+    default set is eight tables at 2^20 rows (IS_WH 2^18): EQ (12 columns),
+    BYTEWISE (26), IS_WH (4), LT (17), STORE (16), BRANCH (14), MEMW_REGISTER
+    (10), LOAD (18); --vm-set all proves every table of a 2^20-cycle run
+    (vm_all_tables). Traces are uniform draws of the tables' shapes (the
+    prover's work does not depend on whether the constraints hold; valid
+    traces are proved and verified in the tests)."""
     from lambda_vm_amd import _native, air as A, backend, prover, vm_tables as V
 
     _native.ensure_device(0)
-    airs = [V.eq_table_air(), V.bytewise_table_air(), V.is_wh_table_air(), V.lt_table_air(), V.store_table_air(),
-            V.branch_table_air(), V.memw_register_table_air(), V.load_table_air()]
-    rows = [1 << 20, 1 << 20, 1 << 18, 1 << 20, 1 << 20, 1 << 20, 1 << 20, 1 << 20]
+    if args.vm_set == "all":
+        named = vm_all_tables(V)
+        airs, rows = [a for _, a, _ in named], [n for _, _, n in named]
+    else:
+        named = None
+        airs = [V.eq_table_air(), V.bytewise_table_air(), V.is_wh_table_air(), V.lt_table_air(),
+                V.store_table_air(), V.branch_table_air(), V.memw_register_table_air(), V.load_table_air()]
+        rows = [1 << 20, 1 << 20, 1 << 18, 1 << 20, 1 << 20, 1 << 20, 1 << 20, 1 << 20]
     for a in airs:
         a.bus_descriptor = A.descriptor(a)
     traces = [backend.splitmix_table_dev(n, a.num_main, 2, 40 + t) for t, (a, n) in enumerate(zip(airs, rows))]
@@ -670,9 +707,13 @@ def run_vm(args) -> dict:
         "dtype": "u64 (Goldilocks field)",
         "data": "synthetic: device SplitMix64 traces of the VM tables' shapes (draws mod p)",
         "config": {
-            "workload": ("full proof of eight VM tables in one transcript: EQ 2^20x12, BYTEWISE 2^20x26, "
-                         "IS_WH 2^18x4, LT 2^20x17, STORE 2^20x16, BRANCH 2^20x14, MEMW_REGISTER 2^20x10, "
-                         "LOAD 2^20x18 (their constraints and bus interactions), blowup 4, "
+            "workload": (("full proof of every VM table of a 2^20-cycle run in one transcript: "
+                          + ", ".join("%s 2^%dx%d" % (nm, n.bit_length() - 1, a.num_main) for nm, a, n in named)
+                          if named else
+                          "full proof of eight VM tables in one transcript: EQ 2^20x12, BYTEWISE 2^20x26, "
+                          "IS_WH 2^18x4, LT 2^20x17, STORE 2^20x16, BRANCH 2^20x14, MEMW_REGISTER 2^20x10, "
+                          "LOAD 2^20x18")
+                         + " (their constraints and bus interactions), blowup 4, "
                          "FRI terminal 2^(2+%d), %d queries" % (args.fri_k, args.queries)),
             "lde_cells": cells,
             "proof_ms": [round(x * 1e3, 3) for x in times],

@@ -20,6 +20,7 @@
 // the running products and the term accumulators live in registers. Row sums
 // go to LDS, the block scans them, and a single-block pass scans the block
 // totals; a last pass writes the accumulated column.
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -76,24 +77,14 @@ __device__ __forceinline__ uint64_t multiplicity(const LogupDev &d, uint32_t k, 
     return m;
 }
 
-// Pass 1: term columns, committed columns into aux, block-local inclusive
-// prefix of the row sums into `scan`, block totals into `totals`.
-__global__ __launch_bounds__(LOGUP_BLOCK) void logup_rows_kernel(LogupDev d, MainView mv, uint64_t n,
-                                                                 uint64_t *__restrict__ aux,
-                                                                 uint64_t *__restrict__ terms_out,
-                                                                 uint64_t *__restrict__ scan,
-                                                                 uint64_t *__restrict__ totals) {
-    __shared__ fp3 s_tot[LOGUP_BLOCK];
-    const uint32_t t = threadIdx.x;
-    const uint64_t r0 = ((uint64_t)blockIdx.x * LOGUP_BLOCK + t) * LOGUP_G;
-    const int cnt = r0 >= n ? 0 : (int)((n - r0) < (uint64_t)LOGUP_G ? (n - r0) : (uint64_t)LOGUP_G);
+// The terms of output columns [c0, c1) at the thread's rows r0 .. r0+cnt-1
+// (one batch inversion per interaction over those rows): committed columns
+// into aux, every column into terms_out, their sum added to rowsum.
+__device__ __forceinline__ void logup_terms(const LogupDev &d, const MainView &mv, uint64_t n, uint64_t r0, int cnt,
+                                            uint32_t c0, uint32_t c1, uint64_t *__restrict__ aux,
+                                            uint64_t *__restrict__ terms_out, fp3 rowsum[LOGUP_G]) {
     const uint32_t C = d.num_out_cols;
-
-    fp3 rowsum[LOGUP_G];
-#pragma unroll
-    for (int j = 0; j < LOGUP_G; j++) rowsum[j] = fp3{0, 0, 0};
-
-    for (uint32_t c = 0; c < C; c++) {
+    for (uint32_t c = c0; c < c1; c++) {
         fp3 term[LOGUP_G];
 #pragma unroll
         for (int j = 0; j < LOGUP_G; j++) term[j] = fp3{0, 0, 0};
@@ -140,6 +131,14 @@ __global__ __launch_bounds__(LOGUP_BLOCK) void logup_rows_kernel(LogupDev d, Mai
             }
         }
     }
+}
+
+// Block-local inclusive prefix of the row sums into `scan`, the block total
+// into `totals` (every thread of the block calls it).
+__device__ __forceinline__ void logup_block_scan(fp3 rowsum[LOGUP_G], uint64_t r0, int cnt, uint64_t *__restrict__ scan,
+                                                 uint64_t *__restrict__ totals) {
+    __shared__ fp3 s_tot[LOGUP_BLOCK];
+    const uint32_t t = threadIdx.x;
     // inclusive prefix over the thread's rows
 #pragma unroll
     for (int j = 1; j < LOGUP_G; j++) rowsum[j] = f3_add(rowsum[j - 1], rowsum[j]);
@@ -170,6 +169,68 @@ __global__ __launch_bounds__(LOGUP_BLOCK) void logup_rows_kernel(LogupDev d, Mai
         o[1] = s_tot[t].c1;
         o[2] = s_tot[t].c2;
     }
+}
+
+// Pass 1: every output column of the thread's rows, then the block scan.
+__global__ __launch_bounds__(LOGUP_BLOCK) void logup_rows_kernel(LogupDev d, MainView mv, uint64_t n,
+                                                                 uint64_t *__restrict__ aux,
+                                                                 uint64_t *__restrict__ terms_out,
+                                                                 uint64_t *__restrict__ scan,
+                                                                 uint64_t *__restrict__ totals) {
+    const uint64_t r0 = ((uint64_t)blockIdx.x * LOGUP_BLOCK + threadIdx.x) * LOGUP_G;
+    const int cnt = r0 >= n ? 0 : (int)((n - r0) < (uint64_t)LOGUP_G ? (n - r0) : (uint64_t)LOGUP_G);
+    fp3 rowsum[LOGUP_G];
+#pragma unroll
+    for (int j = 0; j < LOGUP_G; j++) rowsum[j] = fp3{0, 0, 0};
+    logup_terms(d, mv, n, r0, cnt, 0, d.num_out_cols, aux, terms_out, rowsum);
+    logup_block_scan(rowsum, r0, cnt, scan, totals);
+}
+
+// Pass 1 of a table with few rows and many output columns, split in two:
+// (a) block (x, y) computes columns [y*cps, (y+1)*cps) of its rows and
+// stores their row sums in partial[y][row]; (b) the row sums over the
+// slices, then the block scan of pass 1.
+__global__ __launch_bounds__(LOGUP_BLOCK) void logup_slice_kernel(LogupDev d, MainView mv, uint64_t n, uint32_t cps,
+                                                                  uint64_t *__restrict__ aux,
+                                                                  uint64_t *__restrict__ terms_out,
+                                                                  uint64_t *__restrict__ partial) {
+    const uint64_t r0 = ((uint64_t)blockIdx.x * LOGUP_BLOCK + threadIdx.x) * LOGUP_G;
+    const int cnt = r0 >= n ? 0 : (int)((n - r0) < (uint64_t)LOGUP_G ? (n - r0) : (uint64_t)LOGUP_G);
+    const uint32_t c0 = blockIdx.y * cps;
+    const uint32_t c1 = c0 + cps < d.num_out_cols ? c0 + cps : d.num_out_cols;
+    fp3 rowsum[LOGUP_G];
+#pragma unroll
+    for (int j = 0; j < LOGUP_G; j++) rowsum[j] = fp3{0, 0, 0};
+    logup_terms(d, mv, n, r0, cnt, c0, c1, aux, terms_out, rowsum);
+#pragma unroll
+    for (int j = 0; j < LOGUP_G; j++) {
+        if (j < cnt) {
+            uint64_t *o = partial + 3 * ((uint64_t)blockIdx.y * n + r0 + j);
+            o[0] = rowsum[j].c0;
+            o[1] = rowsum[j].c1;
+            o[2] = rowsum[j].c2;
+        }
+    }
+}
+
+__global__ __launch_bounds__(LOGUP_BLOCK) void logup_slice_scan_kernel(const uint64_t *__restrict__ partial,
+                                                                       uint32_t slices, uint64_t n,
+                                                                       uint64_t *__restrict__ scan,
+                                                                       uint64_t *__restrict__ totals) {
+    const uint64_t r0 = ((uint64_t)blockIdx.x * LOGUP_BLOCK + threadIdx.x) * LOGUP_G;
+    const int cnt = r0 >= n ? 0 : (int)((n - r0) < (uint64_t)LOGUP_G ? (n - r0) : (uint64_t)LOGUP_G);
+    fp3 rowsum[LOGUP_G];
+#pragma unroll
+    for (int j = 0; j < LOGUP_G; j++) {
+        rowsum[j] = fp3{0, 0, 0};
+        if (j < cnt) {
+            for (uint32_t y = 0; y < slices; y++) {
+                const uint64_t *q = partial + 3 * ((uint64_t)y * n + r0 + j);
+                rowsum[j] = f3_add(rowsum[j], fp3{q[0], q[1], q[2]});
+            }
+        }
+    }
+    logup_block_scan(rowsum, r0, cnt, scan, totals);
 }
 
 // Pass 2 (one block): exclusive scan of the block totals in place; out[0..3)
@@ -289,16 +350,29 @@ int logup_aux_impl(const uint64_t *dev_main, size_t row_stride, size_t col_strid
                  o_moff = put(blob, d->mult_term_offsets, K + 1), o_mcoef = put(blob, d->mult_term_coef, Tm),
                  o_mcol = put(blob, d->mult_term_col, Tm), o_alpha = put(blob, alpha, 3 * num_alpha);
     const uint64_t nblocks = (n + LOGUP_ROWS_PER_BLOCK - 1) / LOGUP_ROWS_PER_BLOCK;
+    // Few rows and many output columns (a wide chip of a short run): the
+    // columns are split over grid.y so that some 2^17 threads work, each slice
+    // a contiguous run of columns; every column is still computed once.
+    const uint64_t row_threads = (n + LOGUP_G - 1) / LOGUP_G;
+    uint64_t slices = 1, cps = C;
+    if (row_threads < (1u << 15) && C > 1) {
+        slices = std::min<uint64_t>(C, ((1u << 17) + row_threads - 1) / row_threads);
+        slices = std::min<uint64_t>(slices, 65535);
+        cps = (C + slices - 1) / slices;
+        slices = (C + cps - 1) / cps;  // no empty slice
+    }
     uint8_t *db = nullptr;
-    uint64_t *scan = nullptr, *totals = nullptr, *lo = nullptr;
+    uint64_t *scan = nullptr, *totals = nullptr, *lo = nullptr, *partial = nullptr;
     auto cleanup = [&]() {
         lvm_free(db);
         lvm_free(scan);
         lvm_free(totals);
         lvm_free(lo);
+        lvm_free(partial);
     };
     if (lvm_malloc(&db, blob.size()) != hipSuccess || lvm_malloc(&scan, n * 24) != hipSuccess ||
-        lvm_malloc(&totals, nblocks * 24) != hipSuccess || lvm_malloc(&lo, 6 * 8) != hipSuccess) {
+        lvm_malloc(&totals, nblocks * 24) != hipSuccess || lvm_malloc(&lo, 6 * 8) != hipSuccess ||
+        (slices > 1 && lvm_malloc(&partial, slices * n * 24) != hipSuccess)) {
         cleanup();
         return fail(LVM_ERR_NOMEM, "device allocation for the LogUp build failed");
     }
@@ -328,8 +402,15 @@ int logup_aux_impl(const uint64_t *dev_main, size_t row_stride, size_t col_strid
     const uint64_t inv_n = inv((uint64_t)n);
     {
         ProfScope ps("logup_terms", st);
-        hipLaunchKernelGGL(logup_rows_kernel, dim3((unsigned)nblocks), dim3(LOGUP_BLOCK), 0, st, ld, mv, (uint64_t)n,
-                           dev_aux, dev_terms, scan, totals);
+        if (slices == 1) {
+            hipLaunchKernelGGL(logup_rows_kernel, dim3((unsigned)nblocks), dim3(LOGUP_BLOCK), 0, st, ld, mv,
+                               (uint64_t)n, dev_aux, dev_terms, scan, totals);
+        } else {
+            hipLaunchKernelGGL(logup_slice_kernel, dim3((unsigned)nblocks, (unsigned)slices), dim3(LOGUP_BLOCK), 0, st,
+                               ld, mv, (uint64_t)n, (uint32_t)cps, dev_aux, dev_terms, partial);
+            hipLaunchKernelGGL(logup_slice_scan_kernel, dim3((unsigned)nblocks), dim3(LOGUP_BLOCK), 0, st, partial,
+                               (uint32_t)slices, (uint64_t)n, scan, totals);
+        }
     }
     {
         ProfScope ps("logup_accumulate", st);
