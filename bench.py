@@ -615,7 +615,7 @@ def run_constraints(args) -> dict:
 
 # stage names the library's event profiler records in a proof
 STAGE_NAMES = ["ntt_lde", "keccak_leaves", "merkle_inner", "logup_terms", "logup_accumulate", "composition_transpose",
-               "composition_step_inverse", "composition_compiled", "d2_split", "bary_denoms", "bary_sum",
+               "composition_step_inverse", "composition_compiled", "composition", "d2_split", "bary_denoms", "bary_sum",
                "deep_denoms", "deep_rows", "fri_fold_leaves", "fri_merkle", "fri_fold"]
 
 

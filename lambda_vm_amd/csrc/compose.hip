@@ -106,21 +106,23 @@ constexpr int VAR_BATCH = 8;
 // before their slots are written. Every constraint value is handed to
 // emit_value when its node has run.
 template <int MODE>
-__device__ __forceinline__ void walk(const CompArgs &a, uint32_t r, const Slots &sl, uint64_t *gsum) {
-    for (uint32_t k = 0; k < a.num_nodes; k++) {
-        const uint64_t w0 = a.nodes[2 * k], w1 = a.nodes[2 * k + 1];
+__device__ __forceinline__ void walk(const CompArgs &a, const uint64_t *__restrict__ nodes, uint32_t num_nodes,
+                                     const uint32_t *__restrict__ emit_off, const uint32_t *__restrict__ emit_idx,
+                                     uint32_t r, const Slots &sl, uint64_t *gsum) {
+    for (uint32_t k = 0; k < num_nodes; k++) {
+        const uint64_t w0 = nodes[2 * k], w1 = nodes[2 * k + 1];
         const uint32_t op = (uint32_t)w0;
-        if (op == OP_VAR && ((uint32_t)a.nodes[2 * k + 1] >> 16 & 1)) {
+        if (op == OP_VAR && ((uint32_t)nodes[2 * k + 1] >> 16 & 1)) {
             // a run of consecutive main-trace reads, loads issued together
             uint32_t cnt = 1;
-            while (cnt < VAR_BATCH && k + cnt < a.num_nodes && (uint32_t)a.nodes[2 * (k + cnt)] == OP_VAR &&
-                   ((uint32_t)a.nodes[2 * (k + cnt) + 1] >> 16 & 1))
+            while (cnt < VAR_BATCH && k + cnt < num_nodes && (uint32_t)nodes[2 * (k + cnt)] == OP_VAR &&
+                   ((uint32_t)nodes[2 * (k + cnt) + 1] >> 16 & 1))
                 cnt++;
             uint64_t lv[VAR_BATCH];
 #pragma unroll
             for (int j = 0; j < VAR_BATCH; j++) {
                 if ((uint32_t)j < cnt) {
-                    const uint64_t u0 = a.nodes[2 * (k + j)], u1 = a.nodes[2 * (k + j) + 1];
+                    const uint64_t u0 = nodes[2 * (k + j)], u1 = nodes[2 * (k + j) + 1];
                     const uint32_t col = (uint32_t)(u0 >> 32) & 0xFFFF, off = ((uint32_t)u1 >> 8) & 0xFF;
                     lv[j] = a.main[((uint64_t)col << a.log_N) + frame_row(a, r, off)];
                 }
@@ -128,14 +130,14 @@ __device__ __forceinline__ void walk(const CompArgs &a, uint32_t r, const Slots 
 #pragma unroll
             for (int j = 0; j < VAR_BATCH; j++) {
                 if ((uint32_t)j < cnt) {
-                    const uint32_t res = (uint32_t)(a.nodes[2 * (k + j) + 1] >> 32);
+                    const uint32_t res = (uint32_t)(nodes[2 * (k + j) + 1] >> 32);
                     const uint32_t slot = res & 0x7FFFFFFFu;
                     if (res >> 31)
                         sl.set_e(slot, embed(lv[j]));
                     else
                         sl.b(slot) = lv[j];
-                    for (uint32_t q = a.emit_off[k + j]; q < a.emit_off[k + j + 1]; q++)
-                        emit_value<MODE>(a, a.emit_idx[q], false, lv[j], embed(lv[j]), r, gsum);
+                    for (uint32_t q = emit_off[k + j]; q < emit_off[k + j + 1]; q++)
+                        emit_value<MODE>(a, emit_idx[q], false, lv[j], embed(lv[j]), r, gsum);
                 }
             }
             k += cnt - 1;
@@ -200,33 +202,55 @@ __device__ __forceinline__ void walk(const CompArgs &a, uint32_t r, const Slots 
             sl.set_e(slot, v.ext ? v.e : embed(v.b));
         else
             sl.b(slot) = v.b;
-        for (uint32_t q = a.emit_off[k]; q < a.emit_off[k + 1]; q++)
-            emit_value<MODE>(a, a.emit_idx[q], v.ext, v.b, v.ext ? v.e : embed(v.b), r, gsum);
+        for (uint32_t q = emit_off[k]; q < emit_off[k + 1]; q++)
+            emit_value<MODE>(a, emit_idx[q], v.ext, v.b, v.ext ? v.e : embed(v.b), r, gsum);
     }
 }
 
+// ptab (optional): the program split into parts, block row y taking part y
+// (its node list at nodes + 2*ptab[4y], ptab[4y+1] nodes, its emission CSR at
+// emit_off + ptab[4y+2] and emit_idx + ptab[4y+3]); each part's zerofier-group
+// sums go to gpart[(y*3G + k) * N + r] for compose_parts_tail_kernel.
 template <bool IN_LDS, int MODE>
-__global__ __launch_bounds__(COMP_BLOCK) void compose_kernel(CompArgs a) {
+__global__ __launch_bounds__(COMP_BLOCK) void compose_kernel(CompArgs a, const uint32_t *__restrict__ ptab,
+                                                             uint64_t *__restrict__ gpart) {
     extern __shared__ uint64_t s_slots[];
     __shared__ uint64_t s_gsum[3 * COMP_MAX_GROUPS][COMP_BLOCK];  // per-lane zerofier-group sums
+    const uint64_t *nodes = a.nodes;
+    uint32_t num_nodes = a.num_nodes;
+    const uint32_t *emit_off = a.emit_off, *emit_idx = a.emit_idx;
+    if (ptab) {
+        const uint32_t *pt = ptab + 4 * blockIdx.y;
+        nodes = a.nodes + 2 * (uint64_t)pt[0];
+        num_nodes = pt[1];
+        emit_off = a.emit_off + pt[2];
+        emit_idx = a.emit_idx + pt[3];
+    }
     Slots sl;
     if (IN_LDS) {
         sl.base = s_slots + threadIdx.x;
         sl.ext = s_slots + (uint64_t)a.nbs * COMP_BLOCK + threadIdx.x;
         sl.stride = COMP_BLOCK;
     } else {
-        const uint64_t gt = (uint64_t)blockIdx.x * COMP_BLOCK + threadIdx.x;
+        // every (part, lane) has its own slots
+        const uint64_t stride = a.total_threads * gridDim.y;
+        const uint64_t gt = (uint64_t)blockIdx.y * a.total_threads + (uint64_t)blockIdx.x * COMP_BLOCK + threadIdx.x;
         sl.base = a.scratch + gt;
-        sl.ext = a.scratch + (uint64_t)a.nbs * a.total_threads + gt;
-        sl.stride = a.total_threads;
+        sl.ext = a.scratch + (uint64_t)a.nbs * stride + gt;
+        sl.stride = stride;
     }
     const uint64_t N = 1ull << a.log_N;
     for (uint64_t q = (uint64_t)blockIdx.x * COMP_BLOCK + threadIdx.x; q < N; q += a.total_threads) {
         const uint32_t r = (uint32_t)q;  // natural index; its LDE row is bitrev(r)
         uint64_t *gsum = &s_gsum[0][threadIdx.x];
         for (uint32_t k = 0; k < 3 * a.num_groups; k++) gsum[k * COMP_BLOCK] = 0;
-        walk<MODE>(a, r, sl, gsum);
+        walk<MODE>(a, nodes, num_nodes, emit_off, emit_idx, r, sl, gsum);
         if (MODE == 1) continue;
+        if (ptab) {
+            for (uint32_t k = 0; k < 3 * a.num_groups; k++)
+                gpart[((uint64_t)blockIdx.y * 3 * a.num_groups + k) * N + r] = gsum[k * COMP_BLOCK];
+            continue;
+        }
         fp3 gs[COMP_MAX_GROUPS];
 #pragma unroll
         for (int g = 0; g < COMP_MAX_GROUPS; g++)
@@ -240,6 +264,33 @@ __global__ __launch_bounds__(COMP_BLOCK) void compose_kernel(CompArgs a) {
         o[1] = h.c1;
         o[2] = h.c2;
     }
+}
+
+// The group sums of the parts added up, then the composition tail.
+__global__ __launch_bounds__(256) void compose_parts_tail_kernel(CompArgs a, const uint64_t *__restrict__ gpart,
+                                                                 uint32_t parts) {
+    const uint64_t N = 1ull << a.log_N;
+    const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= N) return;
+    const uint32_t r = (uint32_t)q;
+    const uint32_t G3 = 3 * a.num_groups;
+    fp3 gs[COMP_MAX_GROUPS];
+#pragma unroll
+    for (int g = 0; g < COMP_MAX_GROUPS; g++) {
+        gs[g] = fp3{0, 0, 0};
+        if ((uint32_t)g < a.num_groups) {
+            for (uint32_t p = 0; p < parts; p++) {
+                const uint64_t *b = gpart + ((uint64_t)p * G3 + 3 * g) * N + r;
+                gs[g] = f3_add(gs[g], fp3{b[0], b[N], b[2 * N]});
+            }
+        }
+    }
+    const fp3 h = composition_tail(a, r, gs);
+    const uint64_t orow = a.natural_out ? r : bitrev(r, a.log_N);
+    uint64_t *o = a.out + 3 * orow;
+    o[0] = h.c0;
+    o[1] = h.c1;
+    o[2] = h.c2;
 }
 
 // binv[s * N + r] = 1 / (x_r - w^step_s) at every natural LDE index r
@@ -807,6 +858,86 @@ void compose_bind(ComposeJob &j, const uint8_t *db, const uint64_t *cm_main, con
     a.binv = const_cast<uint64_t *>(binv);
 }
 
+// A program's constraints split into `parts` runs, in the order of the nodes
+// computing them. A part's node list is the subsequence of nodes its
+// constraints depend on, each slot read traced to the last node writing that
+// slot before it: the subsequence keeps every write a kept read depends on
+// and drops only writes, so the program's slot assignment stays valid. Its
+// emission CSR over that list holds only its own constraints. ptab: per part
+// its node offset, node count, emit_off offset and emit_idx offset.
+static void split_program(const lvm_program *pg, const std::vector<uint32_t> &root_node, uint32_t parts,
+                          std::vector<uint64_t> &nodes, std::vector<uint32_t> &eoff, std::vector<uint32_t> &eidx,
+                          std::vector<uint32_t> &ptab) {
+    const uint32_t K = (uint32_t)pg->num_nodes, R = (uint32_t)pg->num_roots;
+    constexpr uint32_t NONE = 0xFFFFFFFFu;
+    std::vector<uint32_t> dep(2 * (size_t)K, NONE);
+    std::vector<uint32_t> wb(pg->num_base_slots, NONE), we(pg->num_ext_slots, NONE);
+    auto src = [&](uint32_t enc) -> uint32_t {
+        const uint32_t kind = enc >> 29, x = enc & 0x1FFFFFFFu;
+        if (kind == K_BASE_SLOT) return x < wb.size() ? wb[x] : NONE;
+        if (kind == K_EXT_SLOT) return x < we.size() ? we[x] : NONE;
+        return NONE;
+    };
+    for (uint32_t k = 0; k < K; k++) {
+        const uint64_t w0 = pg->nodes[2 * k], w1 = pg->nodes[2 * k + 1];
+        const uint32_t op = (uint32_t)w0, x = (uint32_t)(w0 >> 32), y = (uint32_t)w1;
+        if (op == OP_ADD || op == OP_SUB || op == OP_MUL) {
+            dep[2 * k] = src(x);
+            dep[2 * k + 1] = src(y);
+        } else if (op > OP_MUL) {  // OP_NEG, OP_EMBED: one operand
+            dep[2 * k] = src(x);
+        }
+        const uint32_t res = (uint32_t)(w1 >> 32), slot = res & 0x7FFFFFFFu;
+        if (res >> 31) {
+            if (slot < we.size()) we[slot] = k;
+        } else if (slot < wb.size()) {
+            wb[slot] = k;
+        }
+    }
+    std::vector<uint32_t> order(R);
+    for (uint32_t c = 0; c < R; c++) order[c] = c;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t u, uint32_t v) { return root_node[u] < root_node[v]; });
+    std::vector<uint32_t> part_of(R), by_node_off(K + 1, 0), by_node(R);
+    for (uint32_t i = 0; i < R; i++) part_of[order[i]] = (uint32_t)((uint64_t)i * parts / R);
+    for (uint32_t c = 0; c < R; c++) by_node_off[root_node[c] + 1]++;
+    for (uint32_t k = 0; k < K; k++) by_node_off[k + 1] += by_node_off[k];
+    {
+        std::vector<uint32_t> fill(by_node_off.begin(), by_node_off.end() - 1);
+        for (uint32_t i = 0; i < R; i++) by_node[fill[root_node[order[i]]]++] = order[i];
+    }
+    std::vector<uint32_t> stamp(K, NONE), stack;
+    for (uint32_t p = 0; p < parts; p++) {
+        for (uint32_t c = 0; c < R; c++)
+            if (part_of[c] == p) stack.push_back(root_node[c]);
+        while (!stack.empty()) {
+            const uint32_t k = stack.back();
+            stack.pop_back();
+            if (stamp[k] == p) continue;
+            stamp[k] = p;
+            for (int i = 0; i < 2; i++)
+                if (dep[2 * k + i] != NONE && stamp[dep[2 * k + i]] != p) stack.push_back(dep[2 * k + i]);
+        }
+        ptab.push_back((uint32_t)(nodes.size() / 2));
+        const size_t count_at = ptab.size();
+        ptab.push_back(0);
+        ptab.push_back((uint32_t)eoff.size());
+        ptab.push_back((uint32_t)eidx.size());
+        const size_t e0 = eidx.size();
+        uint32_t count = 0;
+        for (uint32_t k = 0; k < K; k++) {
+            if (stamp[k] != p) continue;
+            nodes.push_back(pg->nodes[2 * k]);
+            nodes.push_back(pg->nodes[2 * k + 1]);
+            eoff.push_back((uint32_t)(eidx.size() - e0));
+            for (uint32_t q = by_node_off[k]; q < by_node_off[k + 1]; q++)
+                if (part_of[by_node[q]] == p) eidx.push_back(by_node[q]);
+            count++;
+        }
+        eoff.push_back((uint32_t)(eidx.size() - e0));
+        ptab[count_at] = count;
+    }
+}
+
 // Shared by the composition and the per-constraint entry points.
 int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux, size_t next_step,
                  const lvm_composition *cp, int mode, int natural_out, uint64_t *out) {
@@ -815,7 +946,7 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
     if (rc != LVM_OK) return rc;
     const size_t N = j.N;
     CompArgs &a = j.a;
-    const std::vector<uint8_t> &blob = j.blob;
+    std::vector<uint8_t> &blob = j.blob;
     const std::vector<uint32_t> &root_node = j.root_node;
 
     Device *dv = device_state();
@@ -827,17 +958,39 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
     const uint64_t max_blocks = 256 * 16;
     const uint64_t blocks = std::min<uint64_t>((N + COMP_BLOCK - 1) / COMP_BLOCK, max_blocks);
     a.total_threads = blocks * COMP_BLOCK;
+    // the compiled form of the program when there is one (compiled on first use)
+    const bool jit = jit_enabled() && jit_prepare(pg, root_node, mode == 0 ? cp->root_group : nullptr, mode);
+    // An interpreted composition over few points (a short table of a wide
+    // chip) walks its program in parts side by side, about 2^17 lanes in all.
+    uint32_t parts = 1;
+    if (mode == 0 && !jit && pg->num_roots >= 2 && pg->num_nodes >= 256) {
+        const uint64_t want = (1ull << 17) / a.total_threads;
+        parts = (uint32_t)std::min<uint64_t>(std::min<uint64_t>(want, pg->num_roots), 64);
+        if (parts < 2) parts = 1;
+    }
+    size_t o_pnodes = 0, o_peoff = 0, o_peidx = 0, o_ptab = 0;
+    if (parts > 1) {
+        std::vector<uint64_t> pnodes;
+        std::vector<uint32_t> peoff, peidx, ptab;
+        split_program(pg, root_node, parts, pnodes, peoff, peidx, ptab);
+        o_pnodes = put(blob, pnodes.data(), pnodes.size());
+        o_peoff = put(blob, peoff.data(), peoff.size());
+        o_peidx = put(blob, peidx.data(), peidx.size());
+        o_ptab = put(blob, ptab.data(), ptab.size());
+    }
     uint8_t *db = nullptr;
-    uint64_t *scratch = nullptr, *cm_main = nullptr, *cm_aux = nullptr, *binv = nullptr;
+    uint64_t *scratch = nullptr, *cm_main = nullptr, *cm_aux = nullptr, *binv = nullptr, *gpart = nullptr;
     auto release = [&]() {
         lvm_free(db);
         lvm_free(scratch);
         lvm_free(cm_main);
         lvm_free(cm_aux);
         lvm_free(binv);
+        lvm_free(gpart);
     };
     if (lvm_malloc(&db, blob.size()) != hipSuccess ||
-        (!in_lds && lvm_malloc(&scratch, slot_bytes * a.total_threads) != hipSuccess) ||
+        (!in_lds && lvm_malloc(&scratch, slot_bytes * a.total_threads * parts) != hipSuccess) ||
+        (parts > 1 && lvm_malloc(&gpart, (size_t)parts * 3 * a.num_groups * N * 8) != hipSuccess) ||
         lvm_malloc(&cm_main, N * main->w * 8) != hipSuccess ||
         (aux && lvm_malloc(&cm_aux, N * aux->w * 8) != hipSuccess) ||
         (mode == 0 && a.num_steps && lvm_malloc(&binv, N * a.num_steps * 8) != hipSuccess)) {
@@ -859,25 +1012,34 @@ int compose_impl(const lvm_program *pg, const lvm_lde *main, const lvm_lde *aux,
     }
     a.scratch = scratch;
     const size_t lds = in_lds ? slot_bytes * COMP_BLOCK : 0;
-    // the compiled form of the program when there is one (compiled on first use)
-    const bool jit = jit_enabled() && jit_prepare(pg, root_node, mode == 0 ? cp->root_group : nullptr, mode);
     if (jit) {
         ProfScope ps(mode == 0 ? "composition_compiled" : "constraint_evals_compiled", st);
         if (e == hipSuccess && !jit_launch(pg, root_node, mode == 0 ? cp->root_group : nullptr, mode, a, blocks, st))
             e = hipErrorLaunchFailure;
     } else {
         ProfScope ps(mode == 0 ? "composition" : "constraint_evals", st);
+        const uint32_t *ptab = nullptr;
+        if (parts > 1) {
+            a.nodes = (const uint64_t *)(db + o_pnodes);
+            a.emit_off = (const uint32_t *)(db + o_peoff);
+            a.emit_idx = (const uint32_t *)(db + o_peidx);
+            ptab = (const uint32_t *)(db + o_ptab);
+        }
+        const dim3 grid((unsigned)blocks, parts);
         if (in_lds) {
             if (mode == 0)
-                hipLaunchKernelGGL((compose_kernel<true, 0>), dim3((unsigned)blocks), dim3(COMP_BLOCK), lds, st, a);
+                hipLaunchKernelGGL((compose_kernel<true, 0>), grid, dim3(COMP_BLOCK), lds, st, a, ptab, gpart);
             else
-                hipLaunchKernelGGL((compose_kernel<true, 1>), dim3((unsigned)blocks), dim3(COMP_BLOCK), lds, st, a);
+                hipLaunchKernelGGL((compose_kernel<true, 1>), grid, dim3(COMP_BLOCK), lds, st, a, ptab, gpart);
         } else {
             if (mode == 0)
-                hipLaunchKernelGGL((compose_kernel<false, 0>), dim3((unsigned)blocks), dim3(COMP_BLOCK), 0, st, a);
+                hipLaunchKernelGGL((compose_kernel<false, 0>), grid, dim3(COMP_BLOCK), 0, st, a, ptab, gpart);
             else
-                hipLaunchKernelGGL((compose_kernel<false, 1>), dim3((unsigned)blocks), dim3(COMP_BLOCK), 0, st, a);
+                hipLaunchKernelGGL((compose_kernel<false, 1>), grid, dim3(COMP_BLOCK), 0, st, a, ptab, gpart);
         }
+        if (parts > 1)
+            hipLaunchKernelGGL(compose_parts_tail_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, a,
+                               (const uint64_t *)gpart, parts);
     }
     if (e == hipSuccess) e = hipGetLastError();
     if (e == hipSuccess) e = stage_sync(st);
