@@ -351,12 +351,12 @@ int logup_aux_impl(const uint64_t *dev_main, size_t row_stride, size_t col_strid
                  o_mcol = put(blob, d->mult_term_col, Tm), o_alpha = put(blob, alpha, 3 * num_alpha);
     const uint64_t nblocks = (n + LOGUP_ROWS_PER_BLOCK - 1) / LOGUP_ROWS_PER_BLOCK;
     // Few rows and many output columns (a wide chip of a short run): the
-    // columns are split over grid.y so that some 2^17 threads work, each slice
+    // columns are split over grid.y so that some 2^18 threads work, each slice
     // a contiguous run of columns; every column is still computed once.
     const uint64_t row_threads = (n + LOGUP_G - 1) / LOGUP_G;
     uint64_t slices = 1, cps = C;
-    if (row_threads < (1u << 15) && C > 1) {
-        slices = std::min<uint64_t>(C, ((1u << 17) + row_threads - 1) / row_threads);
+    if (row_threads < (1u << 17) && C > 1) {
+        slices = std::min<uint64_t>(C, ((1u << 18) + row_threads - 1) / row_threads);
         slices = std::min<uint64_t>(slices, 65535);
         cps = (C + slices - 1) / slices;
         slices = (C + cps - 1) / cps;  // no empty slice
